@@ -1,0 +1,231 @@
+"""cuacd_amd - MI355X (gfx950) implementation of cuacd's GPU convex-decomposition path.
+
+Drop-in for the reference's ``cuacd.Context`` API (reference cuacd/__init__.py:55-454):
+numpy in, numpy out, through the CPython extension ``cuacd_amd._gpu`` and the C ABI
+declared in include/cuacd_amd.h. No PyTorch is involved; every computation runs in
+hand-written HIP kernels for gfx950. If the native library is missing the import
+fails loudly - there is no CPU fallback.
+"""
+import os
+
+import numpy as np
+
+try:
+    from cuacd_amd import _gpu
+except ImportError as e:  # pragma: no cover - exercised only on a broken install
+    raise ImportError(
+        "cuacd_amd native extension is missing: build it with "
+        "`python cuacd_amd/build.py` (needs hipcc for gfx950)") from e
+
+# The reference installs a kernel blob on import (cuacd/__init__.py:17-29); the
+# kernels here are linked into libcuacd_hip.so, so the hook only keeps the API.
+_gpu.set_fatbin(b"")
+
+# mesh-audit verdict bits (same meaning as the reference's MAV_* flags,
+# cuacd/__init__.py:32-36)
+MAV_DEGEN_TRI = 1 << 0
+MAV_BAD_INDEX = 1 << 1
+MAV_OPEN_EDGE = 1 << 2
+MAV_NONMANIFOLD = 1 << 3
+MAV_FLIPPED = 1 << 4
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _mesh_volume_cpu(verts, tris):
+    """|signed-tetrahedra sum| / 6 (reference cuacd/__init__.py:46-52)."""
+    v0, v1, v2 = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    return abs(float(np.sum(v0 * np.cross(v1, v2)))) / 6.0
+
+
+def _pack(pts_list):
+    arrs = [_f32(p).reshape(-1, 3) for p in pts_list]
+    counts = np.array([len(a) for a in arrs], dtype=np.int64)
+    offsets = np.zeros(len(arrs) + 1, dtype=np.int32)
+    offsets[1:] = np.cumsum(counts)
+    packed = np.ascontiguousarray(np.concatenate(arrs, axis=0)) if arrs else np.zeros((0, 3), np.float32)
+    return packed, offsets, counts
+
+
+class Context:
+    """GPU context: one device, one device heap, the kernels of this library.
+
+    Parameters
+    ----------
+    device : int
+        HIP device ordinal; -1 keeps the current device.
+    pool_bytes : int
+        Bytes reserved for the device heap pool; 0 picks 80% of free HBM capped
+        at 64 GiB.
+    """
+
+    def __init__(self, device=-1, pool_bytes=0):
+        _gpu.init(device, pool_bytes)
+        self._alive = True
+
+    def close(self):
+        if getattr(self, "_alive", False):
+            _gpu.destroy()
+            self._alive = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- device heap --------------------------------------------------------
+    def heap_compact(self):
+        """Size-class heaps do not fragment across classes; kept for API parity."""
+        _gpu.heap_compact()
+
+    def pool_usage(self):
+        """Bytes handed out by the device pool's bump pointer (high-water mark)."""
+        return _gpu.pool_usage()
+
+    def heap_stats(self):
+        """(heap_outstanding, heap_allocs, heap_frees,
+        scratch_outstanding, scratch_allocs, scratch_frees)."""
+        return _gpu.heap_stats()
+
+    def device_info(self):
+        """(device name, compute units, HBM bytes)."""
+        return _gpu.device_info()
+
+    # ---- topology audit -------------------------------------------------------
+    def check_mesh(self, verts, tris):
+        """Audit mesh topology on the GPU (reference cuacd/__init__.py:124-152)."""
+        v = _f32(verts).reshape(-1, 3)
+        t = _i32(tris).reshape(-1, 3)
+        flags = _gpu.mesh_audit(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0])
+        return {
+            "watertight": not (flags & MAV_OPEN_EDGE),
+            "manifold": not (flags & MAV_NONMANIFOLD),
+            "oriented": not (flags & (MAV_FLIPPED | MAV_DEGEN_TRI)),
+            "needs_remesh": bool(flags),
+            "degenerate": bool(flags & MAV_DEGEN_TRI),
+            "bad_index": bool(flags & MAV_BAD_INDEX),
+            "flags": int(flags),
+        }
+
+    def preprocess(self, verts, tris, resolution=64):
+        """The PaMO remesh is not part of this library (raises RuntimeError)."""
+        v = _f32(verts).reshape(-1, 3)
+        t = _i32(tris).reshape(-1, 3)
+        r = int(resolution)
+        if r & (r - 1) or r < 8:
+            raise ValueError("resolution must be a power of two >= 8")
+        vb, tb, nv, nt = _gpu.preprocess(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0], r)
+        return (np.frombuffer(vb, np.float32).reshape(nv, 3).copy(),
+                np.frombuffer(tb, np.int32).reshape(nt, 3).copy())
+
+    # ---- hulls ----------------------------------------------------------------
+    def _hull_batch(self, pts_list):
+        packed, offsets, counts = _pack(pts_list)
+        n = len(counts)
+        max_pts = int(counts.max())
+        max_hv = max_pts
+        max_ht = max(int((2 * counts - 4).max()), 4)
+        out_v = np.empty(n * max_hv * 3, np.float32)
+        out_t = np.empty(n * max_ht * 3, np.int32)
+        out_nv = np.empty(n, np.int32)
+        out_nt = np.empty(n, np.int32)
+        errs = np.empty(n, np.int32)
+        vols = np.empty(n, np.float64)
+        _gpu.hull_dandc_vol(packed.ctypes.data, packed.shape[0], offsets.ctypes.data, n,
+                            max_pts, max_hv, max_ht,
+                            out_v.ctypes.data, out_t.ctypes.data, out_nv.ctypes.data,
+                            out_nt.ctypes.data, errs.ctypes.data, vols.ctypes.data)
+        return out_v, out_t, out_nv, out_nt, errs, vols, max_hv, max_ht
+
+    def batch_hull_volume(self, pts_list, algo=2):
+        """Hull volume per point cloud (reference cuacd/__init__.py:185-233).
+
+        Returns ``(volumes float32, errors int32)``; volumes are computed on the
+        GPU in FP64 over the exact hull. ``algo`` is accepted and ignored.
+        """
+        vols, errs = self.batch_hull_volume_f64(pts_list)
+        return vols.astype(np.float32), errs
+
+    def batch_hull_volume_f64(self, pts_list):
+        """Like batch_hull_volume but returns the FP64 volumes."""
+        if len(pts_list) == 0:
+            return np.empty(0, np.float64), np.empty(0, np.int32)
+        _, _, _, _, errs, vols, _, _ = self._hull_batch(pts_list)
+        return vols, errs
+
+    def batch_hull_dandc_mesh(self, pts_list):
+        """Exact hull mesh per cloud: list of (verts, tris, volume)
+        (reference cuacd/__init__.py:318-361). Vertices are the input points,
+        in ascending input order; triangles wind counter-clockwise seen from
+        outside."""
+        if len(pts_list) == 0:
+            return []
+        out_v, out_t, out_nv, out_nt, errs, vols, mhv, mht = self._hull_batch(pts_list)
+        res = []
+        for i in range(len(pts_list)):
+            nv, nt = int(out_nv[i]), int(out_nt[i])
+            if nv > mhv or nt > mht:
+                raise RuntimeError(f"hull {i} exceeds its output capacity ({nv} verts, {nt} tris)")
+            hv = out_v[i * mhv * 3: i * mhv * 3 + nv * 3].reshape(nv, 3).copy()
+            ht = out_t[i * mht * 3: i * mht * 3 + nt * 3].reshape(nt, 3).copy()
+            res.append((hv, ht, float(vols[i]) if nt > 0 else 0.0))
+        return res
+
+    def batch_kdop_hull_mesh(self, pts_list):
+        """k-DOP prefiltered exact hull per cloud (reference cuacd/__init__.py:267-316):
+        list of (verts, tris, volume)."""
+        if len(pts_list) == 0:
+            return []
+        packed, offsets, counts = _pack(pts_list)
+        n = len(counts)
+        max_hv, max_ht = 4096, 8192
+        out_v = np.empty(n * max_hv * 3, np.float32)
+        out_t = np.empty(n * max_ht * 3, np.int32)
+        out_nv = np.empty(n, np.int32)
+        out_nt = np.empty(n, np.int32)
+        vols = np.empty(n, np.float32)
+        errs = np.empty(n, np.int32)
+        _gpu.kdop_hull(packed.ctypes.data, packed.shape[0], offsets.ctypes.data, n, max_hv, max_ht,
+                       out_v.ctypes.data, out_t.ctypes.data, out_nv.ctypes.data, out_nt.ctypes.data,
+                       vols.ctypes.data, errs.ctypes.data)
+        res = []
+        for i in range(n):
+            nv, nt = int(out_nv[i]), int(out_nt[i])
+            hv = out_v[i * max_hv * 3: i * max_hv * 3 + nv * 3].reshape(nv, 3).copy()
+            ht = out_t[i * max_ht * 3: i * max_ht * 3 + nt * 3].reshape(nt, 3).copy()
+            res.append((hv, ht, float(vols[i]) if nt > 0 else 0.0))
+        return res
+
+    # ---- volumes ----------------------------------------------------------------
+    def batch_mesh_volume(self, verts_list, tris_list):
+        """Volume of each watertight mesh (reference cuacd/__init__.py:239-262)."""
+        n = len(verts_list)
+        if n == 0:
+            return np.empty(0, np.float32)
+        vs = [_f32(v).reshape(-1, 3) for v in verts_list]
+        ts = [_i32(t).reshape(-1, 3) for t in tris_list]
+        allv = np.ascontiguousarray(np.concatenate(vs, 0))
+        allt = np.ascontiguousarray(np.concatenate(ts, 0))
+        voff = np.zeros(n + 1, np.int32)
+        toff = np.zeros(n + 1, np.int32)
+        voff[1:] = np.cumsum([len(v) for v in vs])
+        toff[1:] = np.cumsum([len(t) for t in ts])
+        out = np.empty(n, np.float32)
+        _gpu.batch_mesh_volume(allv.ctypes.data, allv.shape[0], allt.ctypes.data, allt.shape[0],
+                               toff.ctypes.data, voff.ctypes.data, n, out.ctypes.data)
+        return out

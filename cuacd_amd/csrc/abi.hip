@@ -1,0 +1,247 @@
+// abi.hip - context, device pool and plumbing of the C ABI.
+//
+// Mirrors the reference's context lifecycle (csrc/heap.c:29-294): init picks
+// the device, reserves the pool and initialises the heaps; pool_usage and
+// heap_stats read the device bookkeeping back. There is no module loading:
+// the kernels are linked into this library.
+#include <stdlib.h>
+#include "ctx.hpp"
+#include "kerr.h"
+
+using namespace cuacd;
+
+static char g_init_error[512] = "";
+
+void cuacd::set_error(gpu_ctx_t ctx, const char* fmt, ...) {
+    char* dst = ctx ? ctx->last_error : g_init_error;
+    size_t cap = ctx ? sizeof(ctx->last_error) : sizeof(g_init_error);
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(dst, cap, fmt, ap);
+    va_end(ap);
+}
+
+void* cuacd::stage_buf(gpu_ctx_t ctx, int slot, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (ctx->stage_cap[slot] >= bytes) return ctx->stage[slot];
+    if (ctx->stage[slot]) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->stage[slot]);
+        ctx->stage[slot] = nullptr;
+        ctx->stage_cap[slot] = 0;
+    }
+    size_t cap = bytes + bytes / 4;
+    if (hipMalloc(&ctx->stage[slot], cap) != hipSuccess) {
+        ctx->stage[slot] = nullptr;
+        return nullptr;
+    }
+    ctx->stage_cap[slot] = cap;
+    return ctx->stage[slot];
+}
+
+int cuacd::read_and_clear_err(gpu_ctx_t ctx) {
+    int h = 0;
+    if (hipMemcpyAsync(&h, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    if (h) (void)hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream);
+    if (ctx->inject_err) {
+        h |= ctx->inject_err;
+        ctx->inject_err = 0;
+    }
+    return h;
+}
+
+__global__ void dpool_init_kernel(DPool* p, char* base, unsigned long long cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        p->base = base;
+        p->capacity = cap;
+        p->bump = 0;
+        p->heap.outstanding = p->heap.allocs = p->heap.frees = 0;
+        p->scratch.outstanding = p->scratch.allocs = p->scratch.frees = 0;
+    }
+    for (int c = threadIdx.x; c < DH_NCLASS; c += blockDim.x) {
+        p->heap.heads[c] = (unsigned long long)DH_EMPTY;
+        p->scratch.heads[c] = (unsigned long long)DH_EMPTY;
+    }
+}
+
+extern "C" {
+
+void gpu_set_fatbin(const void* data, size_t len) {
+    (void)data;
+    (void)len;
+}
+
+const char* gpu_last_error(gpu_ctx_t ctx) { return ctx ? ctx->last_error : g_init_error; }
+
+void gpu_destroy(gpu_ctx_t ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 8; ++i)
+        if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
+    if (ctx->d_pool_mem) (void)hipFree(ctx->d_pool_mem);
+    if (ctx->d_pool) (void)hipFree(ctx->d_pool);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    free(ctx);
+}
+
+int gpu_init(gpu_ctx_t* out, int device, size_t pool_bytes) {
+    *out = nullptr;
+    gpu_ctx_t ctx = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_error(nullptr, "no HIP device available");
+        return -1;
+    }
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+    }
+    if (device >= ndev) {
+        set_error(nullptr, "device %d out of range (%d devices)", device, ndev);
+        return -1;
+    }
+    ctx = (gpu_ctx_t)calloc(1, sizeof(struct gpu_ctx));
+    if (!ctx) {
+        set_error(nullptr, "out of host memory");
+        return -1;
+    }
+    ctx->device = device;
+    *out = ctx;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    ctx->n_cu = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ctx->ev_a));
+    HIP_TRY(hipEventCreate(&ctx->ev_b));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (pool_bytes == 0) {
+        pool_bytes = (size_t)((double)free_b * 0.8);
+        const size_t cap = (size_t)64 << 30;
+        if (pool_bytes > cap) pool_bytes = cap;
+        if (pool_bytes < ((size_t)64 << 20)) pool_bytes = (size_t)64 << 20;
+    }
+    pool_bytes = (pool_bytes + 255) & ~(size_t)255;
+    ctx->pool_bytes = pool_bytes;
+    HIP_TRY(hipMalloc(&ctx->d_pool_mem, pool_bytes));
+    HIP_TRY(hipMalloc(&ctx->d_pool, sizeof(DPool)));
+    HIP_TRY(hipMalloc(&ctx->d_err, sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream));
+    dpool_init_kernel<<<1, 64, 0, ctx->stream>>>(ctx->d_pool, ctx->d_pool_mem,
+                                                 (unsigned long long)pool_bytes);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int gpu_heap_compact(gpu_ctx_t ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+size_t gpu_pool_usage(gpu_ctx_t ctx) {
+    unsigned long long bump = 0;
+    if (hipMemcpyAsync(&bump, &ctx->d_pool->bump, sizeof(bump), hipMemcpyDeviceToHost, ctx->stream) !=
+        hipSuccess)
+        return 0;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return 0;
+    return (size_t)(bump < ctx->pool_bytes ? bump : ctx->pool_bytes);
+}
+
+int gpu_heap_stats(gpu_ctx_t ctx, unsigned long long out[6]) {
+    DPool h;
+    HIP_TRY(hipMemcpyAsync(&h, ctx->d_pool, sizeof(DPool), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out[0] = h.heap.outstanding;
+    out[1] = h.heap.allocs;
+    out[2] = h.heap.frees;
+    out[3] = h.scratch.outstanding;
+    out[4] = h.scratch.allocs;
+    out[5] = h.scratch.frees;
+    return 0;
+}
+
+void gpu_result_free(struct gpu_result* r) {
+    if (!r) return;
+    if (r->parts) {
+        for (int i = 0; i < r->nparts; ++i) {
+            free(r->parts[i].verts);
+            free(r->parts[i].tris);
+            free(r->parts[i].hull_verts);
+            free(r->parts[i].hull_tris);
+        }
+        free(r->parts);
+    }
+    r->parts = nullptr;
+    r->nparts = 0;
+}
+
+int gpu_preprocess(gpu_ctx_t ctx, const float*, int, const int*, int, int, float** out_verts, int* out_nv,
+                   int** out_tris, int* out_nt) {
+    if (out_verts) *out_verts = nullptr;
+    if (out_tris) *out_tris = nullptr;
+    if (out_nv) *out_nv = 0;
+    if (out_nt) *out_nt = 0;
+    set_error(ctx, "preprocess (PaMO remesh) is not part of this library");
+    return -1;
+}
+
+int gpu_dev_alloc(gpu_ctx_t ctx, size_t bytes, void** out_ptr) {
+    *out_ptr = nullptr;
+    HIP_TRY(hipMalloc(out_ptr, bytes ? bytes : 16));
+    return 0;
+}
+int gpu_dev_free(gpu_ctx_t ctx, void* ptr) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(ptr));
+    return 0;
+}
+int gpu_memcpy_h2d(gpu_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int gpu_memcpy_d2h(gpu_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int gpu_memset_dev(gpu_ctx_t ctx, void* dst, int value, size_t bytes) {
+    HIP_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return 0;
+}
+int gpu_sync(gpu_ctx_t ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int gpu_timer_start(gpu_ctx_t ctx) {
+    HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
+    return 0;
+}
+int gpu_timer_stop(gpu_ctx_t ctx, float* ms) {
+    HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev_b));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev_a, ctx->ev_b));
+    return 0;
+}
+int gpu_device_info(gpu_ctx_t ctx, char* name, int name_len, int* n_cu, size_t* hbm_bytes) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    if (name && name_len > 0) {
+        snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    return 0;
+}
+void gpu_debug_inject_error(gpu_ctx_t ctx, int kerr_bits) {
+    if (ctx) ctx->inject_err |= kerr_bits;
+}
+
+}  // extern "C"

@@ -1,0 +1,476 @@
+// hull_kernels.hip - batch exact hulls: one wave64 workgroup per cloud (or chunk).
+//
+// Backs gpu_hull_dandc / batch_hull_volume / batch_hull_dandc_mesh
+// (reference: csrc/test.c:95-160, cuda/test_hull_dandc.cu:24-77).
+//
+// Clouds of up to 2048 points are hulled directly by one wave. Larger clouds
+// use divide and conquer on the same quantization frame: chunks of <= 1024
+// points are hulled independently, the union of their hull vertices (in
+// ascending input order, so lowest-index duplicate resolution is preserved)
+// replaces the cloud, and the step repeats until one wave can finish it.
+#include <stdlib.h>
+#include <vector>
+#include "ctx.hpp"
+#include "hull.hpp"
+
+using namespace cuacd;
+
+namespace {
+
+constexpr int HULL_VC_MAX = 1024;
+constexpr int CHUNK = 1024;
+
+struct HullItemArgs {
+    const float* pts;        // all clouds, xyz
+    const int* offsets;      // [n_clouds+1]
+    const int* sel;          // optional subset (indices relative to the cloud start)
+    const int* sel_off;      // [n_items+1] ranges into sel
+    const int* item_cloud;   // optional item -> cloud map (identity if null)
+    const QFrame* frames;    // optional frame per cloud (else from the item's points)
+    int n_items;
+    int VC;
+    // mesh output (mode 0)
+    int max_hv, max_ht;
+    float* out_verts;
+    int* out_tris;
+    int* out_nv;
+    int* out_nt;
+    int* out_err;
+    double* out_vol;
+    // survivor output (mode 1): hull vertex indices, same ranges as sel
+    int* out_sel;
+    int* out_cnt;
+};
+
+template <int PPL>
+__global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode) {
+    extern __shared__ __align__(16) char lds[];
+    const int item = blockIdx.x;
+    if (item >= a.n_items) return;
+    const int lane = lane_id();
+    const int cloud = a.item_cloud ? a.item_cloud[item] : item;
+    const float* cpts = a.pts + 3ll * a.offsets[cloud];
+    int n;
+    const int* sel = nullptr;
+    if (a.sel) {
+        sel = a.sel + a.sel_off[item];
+        n = a.sel_off[item + 1] - a.sel_off[item];
+    } else {
+        n = a.offsets[cloud + 1] - a.offsets[cloud];
+    }
+    auto orig = [&](int i) { return sel ? sel[i] : i; };
+
+    if (mode == 0 && lane == 0) {
+        a.out_nv[item] = 0;
+        a.out_nt[item] = 0;
+        a.out_err[item] = 0;
+        if (a.out_vol) a.out_vol[item] = 0.0;
+    }
+    if (mode == 1 && lane == 0) a.out_cnt[item] = 0;
+    if (n < 4 || n > 64 * PPL) {
+        if (n > 64 * PPL && lane == 0) {
+            if (mode == 0) a.out_err[item] = KERR_HULL_CAPACITY;
+        }
+        // fewer than 4 points: empty hull (reference: hull_dandc_warp_mesh n<4);
+        // survivor mode keeps them all
+        if (mode == 1 && n <= 64 * PPL) {
+            for (int i = lane; i < n; i += 64) a.out_sel[a.sel_off[item] + i] = orig(i);
+            if (lane == 0) a.out_cnt[item] = n;
+        }
+        return;
+    }
+
+    float xs[PPL], ys[PPL], zs[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int i = k * 64 + lane;
+        if (i < n) {
+            const float* p = cpts + 3ll * orig(i);
+            xs[k] = p[0];
+            ys[k] = p[1];
+            zs[k] = p[2];
+        }
+    }
+    QFrame fr;
+    if (a.frames) {
+        fr = a.frames[cloud];
+    } else {
+        float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            if (k * 64 + lane < n) {
+                mn[0] = fminf(mn[0], xs[k]); mx[0] = fmaxf(mx[0], xs[k]);
+                mn[1] = fminf(mn[1], ys[k]); mx[1] = fmaxf(mx[1], ys[k]);
+                mn[2] = fminf(mn[2], zs[k]); mx[2] = fmaxf(mx[2], zs[k]);
+            }
+        }
+        for (int j = 0; j < 3; ++j) {
+            mn[j] = wave_min_f(mn[j]);
+            mx[j] = wave_max_f(mx[j]);
+        }
+        fr = qframe_make(mn, mx);
+    }
+    int px[PPL], py[PPL], pz[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        px[k] = py[k] = pz[k] = 0;
+        if (k * 64 + lane < n) {
+            const float v[3] = {xs[k], ys[k], zs[k]};
+            px[k] = qcoord(fr, 0, v[fr.ax[0]]);
+            py[k] = qcoord(fr, 1, v[fr.ax[1]]);
+            pz[k] = qcoord(fr, 2, v[fr.ax[2]]);
+        }
+    }
+
+    HullWS w = hull_ws_carve(lds, a.VC);
+    HullResult r = wave_quickhull<PPL>(w, n, a.VC, px, py, pz);
+    if (r.status == HULL_OVERFLOW) {
+        if (lane == 0) {
+            if (mode == 0) a.out_err[item] = KERR_HULL_CAPACITY;
+            else a.out_cnt[item] = -1;
+        }
+        return;
+    }
+    if (r.status == HULL_DEGENERATE) {
+        // flat or collinear input: no solid hull. Survivor mode keeps all points.
+        if (mode == 1) {
+            for (int i = lane; i < n; i += 64) a.out_sel[a.sel_off[item] + i] = orig(i);
+            if (lane == 0) a.out_cnt[item] = n;
+        }
+        return;
+    }
+    __syncthreads();
+    const int nv = hull_rank_vertices(w, r);
+    if (mode == 1) {
+        int* dst = a.out_sel + a.sel_off[item];
+        for (int v = lane; v < r.nvert; v += 64)
+            if (w.vref[v]) dst[w.vout[v]] = orig(w.vpid[v]);
+        if (lane == 0) a.out_cnt[item] = nv;
+        return;
+    }
+    const int nt = hull_count_faces(w, r);
+    if (lane == 0) {
+        a.out_nv[item] = nv;
+        a.out_nt[item] = nt;
+    }
+    if (nv <= a.max_hv) {
+        float* dv = a.out_verts + 3ll * a.max_hv * item;
+        for (int v = lane; v < r.nvert; v += 64)
+            if (w.vref[v]) {
+                const float* p = cpts + 3ll * orig(w.vpid[v]);
+                const int o = w.vout[v];
+                dv[3 * o + 0] = p[0];
+                dv[3 * o + 1] = p[1];
+                dv[3 * o + 2] = p[2];
+            }
+    }
+    if (nt <= a.max_ht) hull_write_tris(w, r, a.out_tris + 3ll * a.max_ht * item);
+    if (a.out_vol) {
+        auto pos = [&](int p, float* o3) {
+            const float* q = cpts + 3ll * orig(p);
+            o3[0] = q[0];
+            o3[1] = q[1];
+            o3[2] = q[2];
+        };
+        const double vol = hull_volume_f64(w, r, pos);
+        if (lane == 0) a.out_vol[item] = vol;
+    }
+}
+
+// Per-cloud AABB -> quantization frame (for the chunked path).
+__global__ __launch_bounds__(256) void cloud_frame_kernel(const float* pts, const int* offsets,
+                                                          const int* clouds, int n, QFrame* frames) {
+    __shared__ float smn[3][4], smx[3][4];
+    if ((int)blockIdx.x >= n) return;
+    const int c = clouds[blockIdx.x];
+    const int s = offsets[c], e = offsets[c + 1];
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = s + threadIdx.x; i < e; i += 256) {
+        for (int j = 0; j < 3; ++j) {
+            const float v = pts[3ll * i + j];
+            mn[j] = fminf(mn[j], v);
+            mx[j] = fmaxf(mx[j], v);
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    for (int j = 0; j < 3; ++j) {
+        mn[j] = wave_min_f(mn[j]);
+        mx[j] = wave_max_f(mx[j]);
+        if (lane_id() == 0) { smn[j][wv] = mn[j]; smx[j][wv] = mx[j]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a[3], b[3];
+        for (int j = 0; j < 3; ++j) {
+            a[j] = fminf(fminf(smn[j][0], smn[j][1]), fminf(smn[j][2], smn[j][3]));
+            b[j] = fmaxf(fmaxf(smx[j][0], smx[j][1]), fmaxf(smx[j][2], smx[j][3]));
+        }
+        frames[c] = qframe_make(a, b);
+    }
+}
+
+using KernelFn = void (*)(HullItemArgs, int);
+
+KernelFn pick_kernel(int maxn, int* ppl) {
+    if (maxn <= 64) { *ppl = 1; return hull_wave_kernel<1>; }
+    if (maxn <= 128) { *ppl = 2; return hull_wave_kernel<2>; }
+    if (maxn <= 256) { *ppl = 4; return hull_wave_kernel<4>; }
+    if (maxn <= 512) { *ppl = 8; return hull_wave_kernel<8>; }
+    if (maxn <= 1024) { *ppl = 16; return hull_wave_kernel<16>; }
+    *ppl = 32;
+    return hull_wave_kernel<32>;
+}
+
+int launch_items(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
+    int ppl = 1;
+    KernelFn k = pick_kernel(maxn, &ppl);
+    a.VC = 64 * ppl < HULL_VC_MAX ? 64 * ppl : HULL_VC_MAX;
+    const unsigned lds = hull_ws_bytes(a.VC);
+    if (a.n_items <= 0) return 0;
+    if (lds > 65536u)
+        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3(a.n_items), dim3(64), lds, ctx->stream, a, mode);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+// Device-pointer entry: all buffers on the device; stream-ordered.
+// Clouds larger than 2048 points take the chunked divide-and-conquer path,
+// which needs host-side sizing (one synchronisation per merge round).
+static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, const int* h_offsets,
+                             int n_hulls, int max_pts, int max_hv, int max_ht, float* d_verts, int* d_tris,
+                             int* d_nv, int* d_nt, int* d_err, double* d_vol) {
+    HullItemArgs a{};
+    a.pts = d_pts;
+    a.offsets = d_offsets;
+    a.n_items = n_hulls;
+    a.max_hv = max_hv;
+    a.max_ht = max_ht;
+    a.out_verts = d_verts;
+    a.out_tris = d_tris;
+    a.out_nv = d_nv;
+    a.out_nt = d_nt;
+    a.out_err = d_err;
+    a.out_vol = d_vol;
+    if (max_pts <= 2048) return launch_items(ctx, a, max_pts, 0);
+
+    REQUIRE(h_offsets != nullptr, "clouds above 2048 points need host offsets");
+    // ---- chunked divide and conquer for the large clouds ------------------
+    // small clouds first, directly
+    std::vector<int> big;
+    int max_small = 0;
+    for (int c = 0; c < n_hulls; ++c) {
+        int n = h_offsets[c + 1] - h_offsets[c];
+        if (n > 2048) big.push_back(c);
+        else max_small = n > max_small ? n : max_small;
+    }
+    {
+        // run the direct kernel over all clouds; big ones report capacity and are redone below
+        int rc = launch_items(ctx, a, max_small > 4 ? max_small : 4, 0);
+        if (rc) return rc;
+    }
+    const int nb = (int)big.size();
+    int* d_big = (int*)stage_buf(ctx, 4, sizeof(int) * nb);
+    QFrame* d_frames = (QFrame*)stage_buf(ctx, 5, sizeof(QFrame) * (size_t)n_hulls);
+    REQUIRE(d_big && d_frames, "staging allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_big, big.data(), sizeof(int) * nb, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(cloud_frame_kernel, dim3(nb), dim3(256), 0, ctx->stream, d_pts, d_offsets, d_big, nb,
+                       d_frames);
+    HIP_TRY(hipGetLastError());
+
+    // current candidate lists per big cloud (indices relative to the cloud)
+    std::vector<std::vector<int>> cand(nb);
+    for (int b = 0; b < nb; ++b) {
+        int n = h_offsets[big[b] + 1] - h_offsets[big[b]];
+        cand[b].resize(n);
+        for (int i = 0; i < n; ++i) cand[b][i] = i;
+    }
+    for (int round = 0; round < 16; ++round) {
+        // chunk every list that is still above one wave's reach
+        std::vector<int> item_cloud, sel, sel_off{0};
+        std::vector<int> owner;  // which big cloud each chunk belongs to
+        for (int b = 0; b < nb; ++b) {
+            const int n = (int)cand[b].size();
+            if (n <= 2048) continue;
+            for (int s = 0; s < n; s += CHUNK) {
+                const int e = s + CHUNK < n ? s + CHUNK : n;
+                item_cloud.push_back(big[b]);
+                owner.push_back(b);
+                sel.insert(sel.end(), cand[b].begin() + s, cand[b].begin() + e);
+                sel_off.push_back((int)sel.size());
+            }
+        }
+        if (item_cloud.empty()) break;
+        const int ni = (int)item_cloud.size();
+        int* d_ic = (int*)stage_buf(ctx, 0, sizeof(int) * ni);
+        int* d_sel = (int*)stage_buf(ctx, 1, sizeof(int) * sel.size());
+        int* d_off = (int*)stage_buf(ctx, 2, sizeof(int) * (ni + 1));
+        int* d_out = (int*)stage_buf(ctx, 3, sizeof(int) * (sel.size() + ni));
+        REQUIRE(d_ic && d_sel && d_off && d_out, "staging allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ic, item_cloud.data(), sizeof(int) * ni, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_sel, sel.data(), sizeof(int) * sel.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_off, sel_off.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice, ctx->stream));
+        HullItemArgs c{};
+        c.pts = d_pts;
+        c.offsets = d_offsets;
+        c.sel = d_sel;
+        c.sel_off = d_off;
+        c.item_cloud = d_ic;
+        c.frames = d_frames;
+        c.n_items = ni;
+        c.out_sel = d_out;
+        c.out_cnt = d_out + sel.size();
+        int rc = launch_items(ctx, c, CHUNK, 1);
+        if (rc) return rc;
+        std::vector<int> out(sel.size() + ni);
+        HIP_TRY(hipMemcpyAsync(out.data(), d_out, sizeof(int) * out.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int b = 0; b < nb; ++b)
+            if ((int)cand[b].size() > 2048) cand[b].clear();
+        for (int i = 0; i < ni; ++i) {
+            const int cnt = out[sel.size() + i];
+            REQUIRE(cnt >= 0, "hull chunk exceeded capacity");
+            std::vector<int>& dst = cand[owner[i]];
+            dst.insert(dst.end(), out.begin() + sel_off[i], out.begin() + sel_off[i] + cnt);
+        }
+    }
+    // final hull of each big cloud over its survivors
+    std::vector<int> sel, sel_off{0};
+    int maxn = 4;
+    for (int b = 0; b < nb; ++b) {
+        REQUIRE((int)cand[b].size() <= 2048, "hull divide-and-conquer did not converge");
+        sel.insert(sel.end(), cand[b].begin(), cand[b].end());
+        sel_off.push_back((int)sel.size());
+        maxn = (int)cand[b].size() > maxn ? (int)cand[b].size() : maxn;
+    }
+    int* d_sel = (int*)stage_buf(ctx, 1, sizeof(int) * (sel.size() + 1));
+    int* d_off = (int*)stage_buf(ctx, 2, sizeof(int) * (nb + 1));
+    REQUIRE(d_sel && d_off, "staging allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_sel, sel.data(), sizeof(int) * sel.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_off, sel_off.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
+    // outputs are indexed by item; write into per-cloud slots through a temporary
+    // contiguous output then scatter on the host side of this function
+    float* t_verts = (float*)stage_buf(ctx, 6, sizeof(float) * 3ull * max_hv * nb);
+    int* t_misc = (int*)stage_buf(ctx, 7, sizeof(int) * (3ull * max_ht * nb + 3ull * nb) + sizeof(double) * nb + 16);
+    REQUIRE(t_verts && t_misc, "staging allocation failed");
+    int* t_tris = t_misc;
+    int* t_nv = t_tris + 3ll * max_ht * nb;
+    int* t_nt = t_nv + nb;
+    int* t_err = t_nt + nb;
+    double* t_vol = (double*)(((uintptr_t)(t_err + nb) + 15) & ~(uintptr_t)15);
+    HullItemArgs f{};
+    f.pts = d_pts;
+    f.offsets = d_offsets;
+    f.sel = d_sel;
+    f.sel_off = d_off;
+    f.item_cloud = d_big;
+    f.frames = d_frames;
+    f.n_items = nb;
+    f.max_hv = max_hv;
+    f.max_ht = max_ht;
+    f.out_verts = t_verts;
+    f.out_tris = t_tris;
+    f.out_nv = t_nv;
+    f.out_nt = t_nt;
+    f.out_err = t_err;
+    f.out_vol = d_vol ? t_vol : nullptr;
+    int rc = launch_items(ctx, f, maxn, 0);
+    if (rc) return rc;
+    for (int b = 0; b < nb; ++b) {
+        const int c = big[b];
+        HIP_TRY(hipMemcpyAsync(d_verts + 3ll * max_hv * c, t_verts + 3ll * max_hv * b, sizeof(float) * 3ull * max_hv,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_tris + 3ll * max_ht * c, t_tris + 3ll * max_ht * b, sizeof(int) * 3ull * max_ht,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_nv + c, t_nv + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_nt + c, t_nt + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_err + c, t_err + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+        if (d_vol)
+            HIP_TRY(hipMemcpyAsync(d_vol + c, t_vol + b, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return 0;
+}
+
+extern "C" {
+
+int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, int n_hulls, int max_pts,
+                       int max_hv, int max_ht, float* d_verts, int* d_tris, int* d_nv, int* d_nt, int* d_err,
+                       double* d_vol) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    REQUIRE(max_pts <= 2048, "gpu_hull_dandc_dev: clouds above 2048 points need the host entry point");
+    return hull_batch_device(ctx, d_pts, d_offsets, nullptr, n_hulls, max_pts, max_hv, max_ht, d_verts, d_tris,
+                             d_nv, d_nt, d_err, d_vol);
+}
+
+int gpu_hull_dandc_vol(gpu_ctx_t ctx, const float* pts, int total_pts, const int* offsets, int n_hulls,
+                       int max_pts, int max_hv, int max_ht, float* out_verts, int* out_tris, int* out_nv,
+                       int* out_nt, int* out_errors, double* out_volumes) {
+    if (n_hulls <= 0) return 0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    REQUIRE(total_pts >= 0 && max_hv >= 0 && max_ht >= 0, "invalid sizes");
+    // true maximum cloud size from the offsets (max_pts is advisory)
+    int mx = 0;
+    for (int c = 0; c < n_hulls; ++c) {
+        const int n = offsets[c + 1] - offsets[c];
+        REQUIRE(n >= 0 && offsets[c + 1] <= total_pts, "offsets out of range");
+        mx = n > mx ? n : mx;
+    }
+    (void)max_pts;
+    const size_t b_pts = sizeof(float) * 3ull * (size_t)total_pts;
+    const size_t b_off = sizeof(int) * (size_t)(n_hulls + 1);
+    const size_t b_v = sizeof(float) * 3ull * (size_t)max_hv * n_hulls;
+    const size_t b_t = sizeof(int) * 3ull * (size_t)max_ht * n_hulls;
+    const size_t b_i = sizeof(int) * (size_t)n_hulls;
+    // one allocation for inputs, one for outputs (grow-only, reused across calls)
+    static const size_t A = 256;
+    auto al = [](size_t x) { return (x + A - 1) / A * A; };
+    const size_t in_bytes = al(b_pts) + al(b_off);
+    const size_t out_bytes = al(b_v) + al(b_t) + 3 * al(b_i) + al(sizeof(double) * n_hulls);
+    // slots 0-3 are used by the chunked path; inputs/outputs of this call use
+    // dedicated allocations so the chunked path can still stage its lists
+    char* d_inbuf = nullptr;
+    char* d_outbuf = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_inbuf, in_bytes, ctx->stream));
+    HIP_TRY(hipMallocAsync((void**)&d_outbuf, out_bytes, ctx->stream));
+    float* d_pts = (float*)d_inbuf;
+    int* d_off = (int*)(d_inbuf + al(b_pts));
+    float* d_v = (float*)d_outbuf;
+    int* d_t = (int*)(d_outbuf + al(b_v));
+    int* d_nv = (int*)(d_outbuf + al(b_v) + al(b_t));
+    int* d_nt = d_nv + al(b_i) / sizeof(int);
+    int* d_e = d_nt + al(b_i) / sizeof(int);
+    double* d_vol = (double*)((char*)d_e + al(b_i));
+    HIP_TRY(hipMemcpyAsync(d_pts, pts, b_pts, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_off, offsets, b_off, hipMemcpyHostToDevice, ctx->stream));
+    int rc = hull_batch_device(ctx, d_pts, d_off, offsets, n_hulls, mx, max_hv, max_ht, d_v, d_t, d_nv, d_nt, d_e,
+                               out_volumes ? d_vol : nullptr);
+    if (rc == 0) {
+        if (b_v) HIP_TRY(hipMemcpyAsync(out_verts, d_v, b_v, hipMemcpyDeviceToHost, ctx->stream));
+        if (b_t) HIP_TRY(hipMemcpyAsync(out_tris, d_t, b_t, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(out_nv, d_nv, b_i, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(out_nt, d_nt, b_i, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(out_errors, d_e, b_i, hipMemcpyDeviceToHost, ctx->stream));
+        if (out_volumes)
+            HIP_TRY(hipMemcpyAsync(out_volumes, d_vol, sizeof(double) * n_hulls, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    }
+    HIP_TRY(hipFreeAsync(d_inbuf, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_outbuf, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (rc == 0 && ctx->inject_err) {
+        out_errors[0] |= ctx->inject_err;
+        ctx->inject_err = 0;
+    }
+    return rc;
+}
+
+int gpu_hull_dandc(gpu_ctx_t ctx, const float* pts, int total_pts, const int* offsets, int n_hulls, int max_pts,
+                   int max_hv, int max_ht, float* out_verts, int* out_tris, int* out_nv, int* out_nt,
+                   int* out_errors) {
+    return gpu_hull_dandc_vol(ctx, pts, total_pts, offsets, n_hulls, max_pts, max_hv, max_ht, out_verts, out_tris,
+                              out_nv, out_nt, out_errors, nullptr);
+}
+
+}  // extern "C"

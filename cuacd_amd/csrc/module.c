@@ -1,0 +1,439 @@
+/* module.c - CPython extension `cuacd_amd._gpu` over the C ABI (include/cuacd_amd.h).
+ *
+ * Exposes the same functions, argument order and errors as the reference's
+ * `cuacd._gpu` (csrc/module.c:532-562): numpy data crosses as integer
+ * addresses ("K"), one process-global context, RuntimeError("cuacd error %d:
+ * %s") on failure. Built against the CPython 3.10 limited API (abi3).
+ * Additional functions (no reference counterpart) expose the FP64 hull
+ * volume, the device-resident entry points and the timers used by bench.py.
+ */
+#define PY_SSIZE_T_CLEAN
+#define Py_LIMITED_API 0x030A0000
+#include <Python.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "cuacd_amd.h"
+
+static gpu_ctx_t g_ctx = NULL;
+
+static PyObject* raise_error(int rc) {
+    const char* msg = gpu_last_error(g_ctx);
+    PyErr_Format(PyExc_RuntimeError, "cuacd error %d: %s", rc, msg && msg[0] ? msg : "unknown");
+    return NULL;
+}
+
+#define NEED_CTX()                                                                          \
+    do {                                                                                    \
+        if (!g_ctx) {                                                                       \
+            PyErr_SetString(PyExc_RuntimeError, "context not initialized; call init() first"); \
+            return NULL;                                                                    \
+        }                                                                                   \
+    } while (0)
+
+#define P(T, v) ((T)(uintptr_t)(v))
+
+/* init(device=-1, pool_bytes=0) */
+static PyObject* m_init(PyObject* self, PyObject* args, PyObject* kw) {
+    static char* kwlist[] = {"device", "pool_bytes", NULL};
+    int device = -1;
+    unsigned long long pool = 0;
+    if (!PyArg_ParseTupleAndKeywords(args, kw, "|iK", kwlist, &device, &pool)) return NULL;
+    if (g_ctx) {
+        gpu_destroy(g_ctx);
+        g_ctx = NULL;
+    }
+    int rc = gpu_init(&g_ctx, device, (size_t)pool);
+    if (rc) {
+        PyObject* r = raise_error(rc);
+        if (g_ctx) {
+            gpu_destroy(g_ctx);
+            g_ctx = NULL;
+        }
+        return r;
+    }
+    Py_RETURN_NONE;
+}
+
+static PyObject* g_blob = NULL;
+static PyObject* m_set_fatbin(PyObject* self, PyObject* arg) {
+    if (!PyBytes_Check(arg)) {
+        PyErr_SetString(PyExc_TypeError, "set_fatbin() expects a bytes object");
+        return NULL;
+    }
+    Py_INCREF(arg);
+    Py_XDECREF(g_blob);
+    g_blob = arg;
+    gpu_set_fatbin(PyBytes_AsString(arg), (size_t)PyBytes_Size(arg));
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_destroy(PyObject* self, PyObject* args) {
+    if (g_ctx) {
+        gpu_destroy(g_ctx);
+        g_ctx = NULL;
+    }
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_heap_compact(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    int rc = gpu_heap_compact(g_ctx);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_pool_usage(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    return PyLong_FromSize_t(gpu_pool_usage(g_ctx));
+}
+
+static PyObject* m_heap_stats(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    unsigned long long s[6] = {0, 0, 0, 0, 0, 0};
+    if (gpu_heap_stats(g_ctx, s)) {
+        PyErr_SetString(PyExc_RuntimeError, "gpu_heap_stats failed");
+        return NULL;
+    }
+    return Py_BuildValue("(KKKKKK)", s[0], s[1], s[2], s[3], s[4], s[5]);
+}
+
+static PyObject* m_test_warp_sort(PyObject* self, PyObject* args) {
+    unsigned long long pp, op;
+    int total, n;
+    if (!PyArg_ParseTuple(args, "KiKi", &pp, &total, &op, &n)) return NULL;
+    NEED_CTX();
+    int rc = gpu_test_warp_sort(g_ctx, P(int*, pp), total, P(const int*, op), n);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+/* hull_dandc(pts, total, offsets, n_hulls, max_pts, max_hv, max_ht, verts, tris, nv, nt, errors) */
+static PyObject* m_hull_dandc(PyObject* self, PyObject* args) {
+    unsigned long long pp, op, vp, tp, nvp, ntp, ep;
+    int total, nh, mp, mhv, mht;
+    if (!PyArg_ParseTuple(args, "KiKiiiiKKKKK", &pp, &total, &op, &nh, &mp, &mhv, &mht, &vp, &tp, &nvp, &ntp,
+                          &ep))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_hull_dandc(g_ctx, P(const float*, pp), total, P(const int*, op), nh, mp, mhv, mht,
+                            P(float*, vp), P(int*, tp), P(int*, nvp), P(int*, ntp), P(int*, ep));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+/* hull_dandc_vol(..., errors, volumes_f64)  (no reference counterpart) */
+static PyObject* m_hull_dandc_vol(PyObject* self, PyObject* args) {
+    unsigned long long pp, op, vp, tp, nvp, ntp, ep, volp;
+    int total, nh, mp, mhv, mht;
+    if (!PyArg_ParseTuple(args, "KiKiiiiKKKKKK", &pp, &total, &op, &nh, &mp, &mhv, &mht, &vp, &tp, &nvp, &ntp,
+                          &ep, &volp))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_hull_dandc_vol(g_ctx, P(const float*, pp), total, P(const int*, op), nh, mp, mhv, mht,
+                                P(float*, vp), P(int*, tp), P(int*, nvp), P(int*, ntp), P(int*, ep),
+                                P(double*, volp));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_test_mesh_volume(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp, op;
+    int nv, nt;
+    if (!PyArg_ParseTuple(args, "KiKiK", &vp, &nv, &tp, &nt, &op)) return NULL;
+    NEED_CTX();
+    int rc = gpu_test_mesh_volume(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, P(float*, op));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_batch_mesh_volume(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp, top, vop, volp;
+    int tv, tt, n;
+    if (!PyArg_ParseTuple(args, "KiKiKKiK", &vp, &tv, &tp, &tt, &top, &vop, &n, &volp)) return NULL;
+    NEED_CTX();
+    int rc = gpu_batch_mesh_volume(g_ctx, P(const float*, vp), tv, P(const int*, tp), tt, P(const int*, top),
+                                   vop ? P(const int*, vop) : NULL, n, P(float*, volp));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_test_plane_cut(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp, a, b, c, d;
+    int nv, nt, ac, bc, cc, dc;
+    float pa, pb, pc, pd;
+    if (!PyArg_ParseTuple(args, "KiKiffffKiKiKiKi", &vp, &nv, &tp, &nt, &pa, &pb, &pc, &pd, &a, &ac, &b, &bc, &c,
+                          &cc, &d, &dc))
+        return NULL;
+    NEED_CTX();
+    int npv = 0, npt = 0, nnv = 0, nnt = 0;
+    int rc = gpu_test_plane_cut(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, pa, pb, pc, pd,
+                                P(float*, a), ac, P(int*, b), bc, P(float*, c), cc, P(int*, d), dc, &npv, &npt,
+                                &nnv, &nnt);
+    if (rc) {
+        PyErr_Format(PyExc_RuntimeError, "plane_cut failed: %s", gpu_last_error(g_ctx));
+        return NULL;
+    }
+    return Py_BuildValue("iiii", npv, npt, nnv, nnt);
+}
+
+static PyObject* m_test_postprocess_dc(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp, ovp, otp, onvp, ontp;
+    int nv, nt, mc, mv, mt;
+    if (!PyArg_ParseTuple(args, "KiKiiiiKKKK", &vp, &nv, &tp, &nt, &mc, &mv, &mt, &ovp, &otp, &onvp, &ontp))
+        return NULL;
+    NEED_CTX();
+    int ncomp = 0;
+    int rc = gpu_test_postprocess_dc(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, mc, mv, mt,
+                                     P(float*, ovp), P(int*, otp), P(int*, onvp), P(int*, ontp), &ncomp);
+    if (rc) {
+        PyErr_Format(PyExc_RuntimeError, "postprocess_dc failed: %s", gpu_last_error(g_ctx));
+        return NULL;
+    }
+    return Py_BuildValue("i", ncomp);
+}
+
+static PyObject* m_lookahead_decompose(PyObject* self, PyObject* args, PyObject* kw) {
+    static char* kwlist[] = {"verts_ptr", "nv", "tris_ptr", "nt", "hull_verts_ptr", "hull_nv", "hull_tris_ptr",
+                             "hull_nt", "max_iters", "width", "width2", "threshold", "depth", "quick_depth",
+                             "max_n_cutting", "verbose", "debug", "decompose_components",
+                             "no_decompose_components_per_iter", "n_concave_edges", "concave_eps",
+                             "concave_threshold", "concave_iters", "merge_hulls", NULL};
+    unsigned long long vp, tp, hvp, htp;
+    int nv, nt, hnv, hnt, max_iters, width, width2;
+    float threshold;
+    int depth = 2, quick_depth = 1, max_n_cutting = 16, verbose = 0, debug = 0, dcomp = 0, no_dc_per_iter = 0;
+    int n_concave = 32, concave_iters = 10, merge = 0;
+    float concave_eps = 0.005f, concave_thr = 3.49f;
+    if (!PyArg_ParseTupleAndKeywords(args, kw, "KiKiKiKiiiifiii|iiiiiffii", kwlist, &vp, &nv, &tp, &nt, &hvp, &hnv,
+                                     &htp, &hnt, &max_iters, &width, &width2, &threshold, &depth, &quick_depth,
+                                     &max_n_cutting, &verbose, &debug, &dcomp, &no_dc_per_iter, &n_concave,
+                                     &concave_eps, &concave_thr, &concave_iters, &merge))
+        return NULL;
+    NEED_CTX();
+    struct gpu_result res;
+    memset(&res, 0, sizeof(res));
+    int rc = lookahead_decompose(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, P(const float*, hvp), hnv,
+                                 P(const int*, htp), hnt, max_iters, width, width2, threshold, depth, quick_depth,
+                                 max_n_cutting, verbose, debug, dcomp, !no_dc_per_iter, n_concave, concave_eps,
+                                 concave_thr, concave_iters, merge, &res);
+    if (rc) {
+        gpu_result_free(&res);
+        return raise_error(rc);
+    }
+    PyObject* list = PyList_New(res.nparts);
+    if (!list) {
+        gpu_result_free(&res);
+        return NULL;
+    }
+    for (int i = 0; i < res.nparts; ++i) {
+        struct gpu_part_result* p = &res.parts[i];
+        PyObject* vb = PyBytes_FromStringAndSize((const char*)p->verts, (Py_ssize_t)p->nv * 12);
+        PyObject* tb = PyBytes_FromStringAndSize((const char*)p->tris, (Py_ssize_t)p->nt * 12);
+        PyObject* hvb = PyBytes_FromStringAndSize((const char*)p->hull_verts, (Py_ssize_t)p->hull_nv * 12);
+        PyObject* htb = PyBytes_FromStringAndSize((const char*)p->hull_tris, (Py_ssize_t)p->hull_nt * 12);
+        PyObject* tup = NULL;
+        if (vb && tb && hvb && htb)
+            tup = Py_BuildValue("(OOiiOOiiff)", vb, tb, p->nv, p->nt, hvb, htb, p->hull_nv, p->hull_nt,
+                                (double)p->mesh_vol, (double)p->hull_vol);
+        Py_XDECREF(vb);
+        Py_XDECREF(tb);
+        Py_XDECREF(hvb);
+        Py_XDECREF(htb);
+        if (!tup) {
+            Py_DECREF(list);
+            gpu_result_free(&res);
+            return NULL;
+        }
+        PyList_SetItem(list, i, tup);
+    }
+    gpu_result_free(&res);
+    return list;
+}
+
+static PyObject* m_kdop_hull(PyObject* self, PyObject* args) {
+    unsigned long long pp, op, vp, tp, nvp, ntp, volp, ep;
+    int total, nh, mhv, mht;
+    if (!PyArg_ParseTuple(args, "KiKiiiKKKKKK", &pp, &total, &op, &nh, &mhv, &mht, &vp, &tp, &nvp, &ntp, &volp,
+                          &ep))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_kdop_hull(g_ctx, P(const float*, pp), total, P(const int*, op), nh, mhv, mht, P(float*, vp),
+                           P(int*, tp), P(int*, nvp), P(int*, ntp), P(float*, volp), P(int*, ep));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_test_hausdorff(PyObject* self, PyObject* args) {
+    unsigned long long hv, ht, mv, mt, op;
+    int hnv, hnt, mnv, mnt;
+    if (!PyArg_ParseTuple(args, "KiKiKiKiK", &hv, &hnv, &ht, &hnt, &mv, &mnv, &mt, &mnt, &op)) return NULL;
+    NEED_CTX();
+    int rc = gpu_test_hausdorff(g_ctx, P(const float*, hv), hnv, P(const int*, ht), hnt, P(const float*, mv), mnv,
+                                P(const int*, mt), mnt, P(float*, op));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_mesh_audit(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp;
+    int nv, nt;
+    if (!PyArg_ParseTuple(args, "KiKi", &vp, &nv, &tp, &nt)) return NULL;
+    NEED_CTX();
+    unsigned int flags = 0;
+    int rc = gpu_mesh_audit(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, &flags);
+    if (rc) return raise_error(rc);
+    return PyLong_FromUnsignedLong(flags);
+}
+
+static PyObject* m_preprocess(PyObject* self, PyObject* args) {
+    unsigned long long vp, tp;
+    int nv, nt, res;
+    if (!PyArg_ParseTuple(args, "KiKii", &vp, &nv, &tp, &nt, &res)) return NULL;
+    NEED_CTX();
+    float* ov = NULL;
+    int* ot = NULL;
+    int onv = 0, ont = 0;
+    int rc = gpu_preprocess(g_ctx, P(const float*, vp), nv, P(const int*, tp), nt, res, &ov, &onv, &ot, &ont);
+    if (rc) {
+        free(ov);
+        free(ot);
+        return raise_error(rc);
+    }
+    PyObject* vb = PyBytes_FromStringAndSize((const char*)ov, (Py_ssize_t)onv * 12);
+    PyObject* tb = PyBytes_FromStringAndSize((const char*)ot, (Py_ssize_t)ont * 12);
+    free(ov);
+    free(ot);
+    if (!vb || !tb) {
+        Py_XDECREF(vb);
+        Py_XDECREF(tb);
+        return NULL;
+    }
+    PyObject* tup = Py_BuildValue("(OOii)", vb, tb, onv, ont);
+    Py_DECREF(vb);
+    Py_DECREF(tb);
+    return tup;
+}
+
+/* ---- device-resident plumbing (no reference counterpart) ------------------ */
+static PyObject* m_dev_alloc(PyObject* self, PyObject* args) {
+    unsigned long long n;
+    if (!PyArg_ParseTuple(args, "K", &n)) return NULL;
+    NEED_CTX();
+    void* p = NULL;
+    int rc = gpu_dev_alloc(g_ctx, (size_t)n, &p);
+    if (rc) return raise_error(rc);
+    return PyLong_FromUnsignedLongLong((unsigned long long)(uintptr_t)p);
+}
+static PyObject* m_dev_free(PyObject* self, PyObject* args) {
+    unsigned long long p;
+    if (!PyArg_ParseTuple(args, "K", &p)) return NULL;
+    NEED_CTX();
+    int rc = gpu_dev_free(g_ctx, P(void*, p));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_h2d(PyObject* self, PyObject* args) {
+    unsigned long long d, s, n;
+    if (!PyArg_ParseTuple(args, "KKK", &d, &s, &n)) return NULL;
+    NEED_CTX();
+    int rc = gpu_memcpy_h2d(g_ctx, P(void*, d), P(const void*, s), (size_t)n);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_d2h(PyObject* self, PyObject* args) {
+    unsigned long long d, s, n;
+    if (!PyArg_ParseTuple(args, "KKK", &d, &s, &n)) return NULL;
+    NEED_CTX();
+    int rc = gpu_memcpy_d2h(g_ctx, P(void*, d), P(const void*, s), (size_t)n);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_sync(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    int rc = gpu_sync(g_ctx);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_timer_start(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    int rc = gpu_timer_start(g_ctx);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_timer_stop(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    float ms = 0.f;
+    int rc = gpu_timer_stop(g_ctx, &ms);
+    if (rc) return raise_error(rc);
+    return PyFloat_FromDouble((double)ms);
+}
+static PyObject* m_device_info(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    char name[256];
+    int ncu = 0;
+    size_t hbm = 0;
+    int rc = gpu_device_info(g_ctx, name, (int)sizeof(name), &ncu, &hbm);
+    if (rc) return raise_error(rc);
+    return Py_BuildValue("(siK)", name, ncu, (unsigned long long)hbm);
+}
+/* hull_dandc_dev(d_pts, d_offsets, n_hulls, max_pts, max_hv, max_ht, d_verts, d_tris, d_nv, d_nt, d_err, d_vol) */
+static PyObject* m_hull_dandc_dev(PyObject* self, PyObject* args) {
+    unsigned long long pp, op, vp, tp, nvp, ntp, ep, volp;
+    int nh, mp, mhv, mht;
+    if (!PyArg_ParseTuple(args, "KKiiiiKKKKKK", &pp, &op, &nh, &mp, &mhv, &mht, &vp, &tp, &nvp, &ntp, &ep, &volp))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_hull_dandc_dev(g_ctx, P(const float*, pp), P(const int*, op), nh, mp, mhv, mht, P(float*, vp),
+                                P(int*, tp), P(int*, nvp), P(int*, ntp), P(int*, ep), P(double*, volp));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_inject_error(PyObject* self, PyObject* args) {
+    int bits;
+    if (!PyArg_ParseTuple(args, "i", &bits)) return NULL;
+    NEED_CTX();
+    gpu_debug_inject_error(g_ctx, bits);
+    Py_RETURN_NONE;
+}
+
+static PyMethodDef methods[] = {
+    {"set_fatbin", m_set_fatbin, METH_O, "Accepts and ignores a kernel blob (kernels are linked in)."},
+    {"init", (PyCFunction)(void (*)(void))m_init, METH_VARARGS | METH_KEYWORDS, "init(device=-1, pool_bytes=0)"},
+    {"destroy", m_destroy, METH_NOARGS, "Destroy the GPU context."},
+    {"heap_compact", m_heap_compact, METH_NOARGS, "Synchronise the device heaps."},
+    {"pool_usage", m_pool_usage, METH_NOARGS, "Bytes bumped from the device pool (monotonic)."},
+    {"heap_stats", m_heap_stats, METH_NOARGS, "(heap_out, heap_allocs, heap_frees, scr_out, scr_allocs, scr_frees)"},
+    {"test_warp_sort", m_test_warp_sort, METH_VARARGS, "Sort (x,y,z,index) records by (y,x,z,index)."},
+    {"hull_dandc", m_hull_dandc, METH_VARARGS, "Exact hull mesh per point cloud."},
+    {"hull_dandc_vol", m_hull_dandc_vol, METH_VARARGS, "hull_dandc plus FP64 volumes."},
+    {"test_mesh_volume", m_test_mesh_volume, METH_VARARGS, "Volume of one mesh."},
+    {"batch_mesh_volume", m_batch_mesh_volume, METH_VARARGS, "Volumes of a batch of meshes."},
+    {"test_plane_cut", m_test_plane_cut, METH_VARARGS, "Plane cut with cap triangulation."},
+    {"kdop_hull", m_kdop_hull, METH_VARARGS, "k-DOP prefiltered exact hull."},
+    {"test_hausdorff", m_test_hausdorff, METH_VARARGS, "Bidirectional sampled Hausdorff distance."},
+    {"test_postprocess_dc", m_test_postprocess_dc, METH_VARARGS, "Split a mesh into connected components."},
+    {"mesh_audit", m_mesh_audit, METH_VARARGS, "Topology audit -> MAV_* bitmask."},
+    {"preprocess", m_preprocess, METH_VARARGS, "Not part of this library (raises)."},
+    {"lookahead_decompose", (PyCFunction)(void (*)(void))m_lookahead_decompose, METH_VARARGS | METH_KEYWORDS,
+     "Lookahead tree-search convex decomposition."},
+    {"dev_alloc", m_dev_alloc, METH_VARARGS, "Device allocation -> address."},
+    {"dev_free", m_dev_free, METH_VARARGS, "Free a device allocation."},
+    {"memcpy_h2d", m_h2d, METH_VARARGS, "Host to device copy (dst, src, bytes)."},
+    {"memcpy_d2h", m_d2h, METH_VARARGS, "Device to host copy (dst, src, bytes)."},
+    {"sync", m_sync, METH_NOARGS, "Synchronise the context stream."},
+    {"timer_start", m_timer_start, METH_NOARGS, "Record the start event on the context stream."},
+    {"timer_stop", m_timer_stop, METH_NOARGS, "Record the stop event, wait, return elapsed ms."},
+    {"device_info", m_device_info, METH_NOARGS, "(name, n_cu, hbm_bytes)"},
+    {"hull_dandc_dev", m_hull_dandc_dev, METH_VARARGS, "Device-pointer hull batch (stream-ordered)."},
+    {"inject_error", m_inject_error, METH_VARARGS, "OR KERR bits into the next error readback."},
+    {NULL, NULL, 0, NULL}};
+
+static int exec_module(PyObject* m) { return PyModule_AddFunctions(m, methods); }
+
+static PyModuleDef_Slot slots[] = {{Py_mod_exec, (void*)exec_module}, {0, NULL}};
+
+static PyModuleDef moddef = {PyModuleDef_HEAD_INIT, "cuacd_amd._gpu", "MI355X convex decomposition kernels", 0,
+                             NULL, slots, NULL, NULL, NULL};
+
+PyMODINIT_FUNC PyInit__gpu(void) { return PyModuleDef_Init(&moddef); }

@@ -1,0 +1,97 @@
+"""Seeded synthetic inputs for the benchmark workloads and the tests.
+
+* ``ball_clouds`` - point clouds uniform in the unit ball (rejection sampling
+  from [-1, 1]^3, rows kept in draw order);
+* ``icosphere`` / ``noisy_icospheres`` - subdivided icospheres with per-vertex
+  radial noise;
+* ``cube_mesh`` / ``box_mesh`` / ``l_shape`` - closed-form test solids.
+"""
+import numpy as np
+
+
+def ball_clouds(n_clouds=1024, n_pts=256, seed=0):
+    """float32 clouds: for each cloud draw uniform(-1, 1, (512, 3)) chunks from one
+    default_rng(seed) stream, keep rows with |p| <= 1 in order, truncate at n_pts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_clouds):
+        kept = []
+        have = 0
+        while have < n_pts:
+            c = rng.uniform(-1.0, 1.0, (512, 3))
+            c = c[np.einsum("ij,ij->i", c, c) <= 1.0]
+            kept.append(c)
+            have += len(c)
+        out.append(np.concatenate(kept, 0)[:n_pts].astype(np.float32))
+    return out
+
+
+def icosphere(level=3):
+    """Unit icosphere: (verts float64 [V,3], tris int32 [T,3]), outward winding.
+    Level 3 has 642 vertices / 1280 triangles."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts), np.array(faces, np.int32)
+
+
+def noisy_icospheres(n_meshes=512, level=3, sigma=0.15, seed=0):
+    """Icospheres with per-vertex radial scale 1 + sigma*N(0,1), drawn mesh by
+    mesh from one default_rng(seed) stream; float32 verts, int32 tris."""
+    base_v, base_t = icosphere(level)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_meshes):
+        s = 1.0 + sigma * rng.standard_normal(len(base_v))
+        out.append(((base_v * s[:, None]).astype(np.float32), base_t.copy()))
+    return out
+
+
+def box_mesh(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """Closed box with 8 vertices and 12 outward triangles."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    t = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                  [1, 2, 6], [1, 6, 5], [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]], np.int32)
+    return v, t
+
+
+def cube_mesh():
+    return box_mesh()
+
+
+def l_shape():
+    """L-shaped prism (union of two boxes sharing a face), watertight, 12 verts."""
+    v = np.array([[0, 0, 0], [2, 0, 0], [2, 1, 0], [1, 1, 0], [1, 2, 0], [0, 2, 0],
+                  [0, 0, 1], [2, 0, 1], [2, 1, 1], [1, 1, 1], [1, 2, 1], [0, 2, 1]], np.float32)
+    bottom = [(0, 2, 1), (0, 3, 2), (0, 5, 3), (3, 5, 4)]
+    top = [(6, 7, 8), (6, 8, 9), (6, 9, 11), (9, 10, 11)]
+    sides = []
+    ring = [0, 1, 2, 3, 4, 5]
+    for i in range(6):
+        a, b = ring[i], ring[(i + 1) % 6]
+        sides += [(a, b, b + 6), (a, b + 6, a + 6)]
+    return v, np.array(bottom + top + sides, np.int32)

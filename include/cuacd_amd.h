@@ -1,0 +1,168 @@
+/* cuacd_amd.h - C ABI of the MI355X (gfx950) implementation of cuacd's GPU path.
+ *
+ * Plain C: no HIP, CUDA or PyTorch types cross this boundary. Every function
+ * below is one the reference's CPython extension binds (csrc/module.c:532-562)
+ * with the same name, argument meaning, ownership and error behaviour, unless
+ * it is marked "no reference counterpart".
+ *
+ * Conventions (reference csrc/heap.h, csrc/test.h, csrc/lookahead.h):
+ *   - host pointers in, caller-allocated host output buffers out;
+ *   - return 0 on success, a HIP error code, a KERR_* bitmask, or -1;
+ *     the text is available from gpu_last_error(ctx);
+ *   - calls are synchronous unless named *_dev (device-pointer, stream-ordered
+ *     variants for callers whose data already lives in HBM).
+ */
+#ifndef CUACD_AMD_H
+#define CUACD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpu_ctx* gpu_ctx_t;
+
+/* ---- context and device heap (replaces csrc/heap.h:24-64) ---------------- */
+
+/* replaces gpu_set_fatbin (csrc/heap.h:30): kernels are linked into this
+ * library, so the blob is accepted and ignored. */
+void gpu_set_fatbin(const void* data, size_t len);
+
+/* replaces gpu_init (csrc/heap.h:32, csrc/heap.c:59-220). device < 0 keeps the
+ * current device; pool_bytes = 0 reserves 80% of free HBM capped at 64 GiB. */
+int gpu_init(gpu_ctx_t* ctx, int device_ordinal, size_t pool_bytes);
+/* replaces gpu_destroy (csrc/heap.h:33) */
+void gpu_destroy(gpu_ctx_t ctx);
+/* replaces gpu_last_error (csrc/heap.h:35); ctx may be NULL after a failed init */
+const char* gpu_last_error(gpu_ctx_t ctx);
+/* replaces gpu_heap_compact (csrc/heap.h:40): size-class heaps never fragment
+ * across classes, so this only synchronises. */
+int gpu_heap_compact(gpu_ctx_t ctx);
+/* replaces gpu_pool_usage (csrc/heap.h:46): bytes bumped from the pool (monotonic) */
+size_t gpu_pool_usage(gpu_ctx_t ctx);
+/* replaces gpu_heap_stats (csrc/heap.h:51): heap_outstanding, heap_allocs,
+ * heap_frees, scratch_outstanding, scratch_allocs, scratch_frees */
+int gpu_heap_stats(gpu_ctx_t ctx, unsigned long long out[6]);
+
+/* result of lookahead_decompose (csrc/heap.h:66-83) */
+struct gpu_part_result {
+    float* verts;
+    int* tris;
+    int nv;
+    int nt;
+    float* hull_verts;
+    int* hull_tris;
+    int hull_nv;
+    int hull_nt;
+    float mesh_vol;
+    float hull_vol;
+    float hausdorff;
+};
+struct gpu_result {
+    struct gpu_part_result* parts;
+    int nparts;
+};
+/* replaces gpu_result_free (csrc/heap.h:85) */
+void gpu_result_free(struct gpu_result* result);
+
+/* ---- batch kernels (replaces csrc/test.h:18-132) --------------------------- */
+
+/* replaces gpu_test_warp_sort (csrc/test.h:23): sorts int4 (x,y,z,index)
+ * records of each sub-array by (y, x, z, index), in place. */
+int gpu_test_warp_sort(gpu_ctx_t ctx, int* points, int total_pts, const int* offsets, int n_arrays);
+
+/* replaces gpu_hull_dandc (csrc/test.h:35-48): exact hull per cloud. */
+int gpu_hull_dandc(gpu_ctx_t ctx, const float* pts, int total_pts, const int* offsets, int n_hulls,
+                   int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* out_verts,
+                   int* out_tris, int* out_nv, int* out_nt, int* out_errors);
+
+/* no reference counterpart: gpu_hull_dandc plus the FP64 divergence-theorem
+ * volume of every hull (the reference computes it on the host in float32,
+ * cuacd/__init__.py:222-231). out_volumes may be NULL. */
+int gpu_hull_dandc_vol(gpu_ctx_t ctx, const float* pts, int total_pts, const int* offsets, int n_hulls,
+                       int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* out_verts,
+                       int* out_tris, int* out_nv, int* out_nt, int* out_errors, double* out_volumes);
+
+/* replaces gpu_test_mesh_volume (csrc/test.h:53-59) */
+int gpu_test_mesh_volume(gpu_ctx_t ctx, const float* verts, int n_verts, const int* tris, int n_tris,
+                         float* out_volume);
+
+/* replaces gpu_batch_mesh_volume (csrc/test.h:70-79); vert_offsets may be NULL */
+int gpu_batch_mesh_volume(gpu_ctx_t ctx, const float* verts, int total_verts, const int* tris,
+                          int total_tris, const int* tri_offsets, const int* vert_offsets, int n_meshes,
+                          float* out_volumes);
+
+/* replaces gpu_kdop_hull (csrc/test.h:84-97) */
+int gpu_kdop_hull(gpu_ctx_t ctx, const float* pts, int total_pts, const int* offsets, int n_hulls,
+                  int max_hull_verts, int max_hull_tris, float* out_verts, int* out_tris, int* out_nv,
+                  int* out_nt, float* out_volumes, int* out_errors);
+
+/* replaces gpu_test_hausdorff (csrc/test.h:102-108): bidirectional sampled
+ * Hausdorff distance between a hull and a mesh. */
+int gpu_test_hausdorff(gpu_ctx_t ctx, const float* hull_verts, int hull_nv, const int* hull_tris,
+                       int hull_nt, const float* mesh_verts, int mesh_nv, const int* mesh_tris,
+                       int mesh_nt, float* out_hausdorff);
+
+/* replaces gpu_test_plane_cut (csrc/test.h:118-127) */
+int gpu_test_plane_cut(gpu_ctx_t ctx, const float* vertices, int n_verts, const int* triangles,
+                       int n_tris, float pa, float pb, float pc_n, float pd, float* out_pos_verts,
+                       int out_pos_verts_cap, int* out_pos_tris, int out_pos_tris_cap,
+                       float* out_neg_verts, int out_neg_verts_cap, int* out_neg_tris,
+                       int out_neg_tris_cap, int* out_n_pv, int* out_n_pt, int* out_n_nv,
+                       int* out_n_nt);
+
+/* replaces gpu_mesh_audit (csrc/test.h:132-136): MAV_* verdict bitmask */
+int gpu_mesh_audit(gpu_ctx_t ctx, const float* verts, int n_verts, const int* tris, int n_tris,
+                   unsigned int* out_flags);
+
+/* replaces gpu_test_postprocess_dc (csrc/postprocess.h:20-27) */
+int gpu_test_postprocess_dc(gpu_ctx_t ctx, const float* verts, int nv, const int* tris, int nt,
+                            int max_components, int max_verts_per, int max_tris_per,
+                            float* out_verts, int* out_tris, int* out_nv, int* out_nt,
+                            int* out_n_components);
+
+/* replaces lookahead_decompose (csrc/lookahead.h:28-43) */
+int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tris, int nt,
+                        const float* hull_verts, int hull_nv, const int* hull_tris, int hull_nt,
+                        int max_iters, int width, int width2, float threshold, int depth,
+                        int quick_depth, int max_n_cutting, int verbose, int debug,
+                        int decompose_components, int decompose_components_per_iter,
+                        int n_concave_edges, float concave_eps, float concave_threshold,
+                        int concave_iters, int merge_hulls, struct gpu_result* out);
+
+/* replaces gpu_preprocess (csrc/preprocess.h:18-24). The PaMO remesh is not
+ * part of this library: always returns -1 with an explanatory error. */
+int gpu_preprocess(gpu_ctx_t ctx, const float* verts, int nv, const int* tris, int nt, int resolution,
+                   float** out_verts, int* out_nv, int** out_tris, int* out_nt);
+
+/* ---- device-resident variants and plumbing (no reference counterpart) ------
+ * For callers (benchmarks, pipelines) whose inputs already live in HBM. All
+ * run on the context's stream and return without synchronising. */
+int gpu_dev_alloc(gpu_ctx_t ctx, size_t bytes, void** out_ptr);
+int gpu_dev_free(gpu_ctx_t ctx, void* ptr);
+int gpu_memcpy_h2d(gpu_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int gpu_memcpy_d2h(gpu_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int gpu_memset_dev(gpu_ctx_t ctx, void* dst, int value, size_t bytes);
+int gpu_sync(gpu_ctx_t ctx);
+/* HIP events on the context's stream around a timed region */
+int gpu_timer_start(gpu_ctx_t ctx);
+int gpu_timer_stop(gpu_ctx_t ctx, float* elapsed_ms);
+/* device name / CU count for reports */
+int gpu_device_info(gpu_ctx_t ctx, char* name, int name_len, int* n_cu, size_t* hbm_bytes);
+
+/* device-pointer form of gpu_hull_dandc_vol */
+int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, int n_hulls,
+                       int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* d_verts,
+                       int* d_tris, int* d_nv, int* d_nt, int* d_errors, double* d_volumes);
+
+/* sets KERR_INJECTED in the next kernel error word read back by the library
+ * (tests of error propagation; no GPU fault is involved) */
+void gpu_debug_inject_error(gpu_ctx_t ctx, int kerr_bits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUACD_AMD_H */
