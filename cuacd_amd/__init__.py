@@ -211,6 +211,58 @@ class Context:
             res.append((hv, ht, float(vols[i]) if nt > 0 else 0.0))
         return res
 
+    # ---- concavity metric ---------------------------------------------------------
+    def hausdorff(self, hull_verts, hull_tris, mesh_verts, mesh_tris):
+        """Bidirectional sampled Hausdorff distance between a hull and a mesh
+        (reference _gpu.test_hausdorff, csrc/module.c:449-466)."""
+        hv, ht = _f32(hull_verts).reshape(-1, 3), _i32(hull_tris).reshape(-1, 3)
+        mv, mt = _f32(mesh_verts).reshape(-1, 3), _i32(mesh_tris).reshape(-1, 3)
+        out = np.zeros(1, np.float32)
+        _gpu.test_hausdorff(hv.ctypes.data, len(hv), ht.ctypes.data, len(ht),
+                            mv.ctypes.data, len(mv), mt.ctypes.data, len(mt), out.ctypes.data)
+        return float(out[0])
+
+    def batch_hausdorff(self, meshes_a, meshes_b, n_samples=4096, sampler="stratified", return_samples=False):
+        """Directed sampled Hausdorff distances for pairs of meshes.
+
+        meshes_a, meshes_b: lists of (verts, tris). Returns (d_ab, d_ba) float32
+        arrays: d_ab[i] = max over samples of A_i of the distance to B_i.
+        sampler "stratified" draws n_samples area-uniform samples per mesh;
+        "reference" samples like the reference metric (max(2000*area, 1000)).
+        With return_samples (stratified only) also returns the sample arrays.
+        """
+        n = len(meshes_a)
+        if n != len(meshes_b):
+            raise ValueError("meshes_a and meshes_b must pair up")
+        if n == 0:
+            return np.empty(0, np.float32), np.empty(0, np.float32)
+        mode = {"reference": 0, "stratified": 1}[sampler]
+
+        def pack(ms):
+            vs = [_f32(v).reshape(-1, 3) for v, _ in ms]
+            ts = [_i32(t).reshape(-1, 3) for _, t in ms]
+            voff = np.zeros(n + 1, np.int32)
+            toff = np.zeros(n + 1, np.int32)
+            voff[1:] = np.cumsum([len(v) for v in vs])
+            toff[1:] = np.cumsum([len(t) for t in ts])
+            return np.ascontiguousarray(np.concatenate(vs)), voff, np.ascontiguousarray(np.concatenate(ts)), toff
+
+        av, avo, at, ato = pack(meshes_a)
+        bv, bvo, bt, bto = pack(meshes_b)
+        dab = np.empty(n, np.float32)
+        dba = np.empty(n, np.float32)
+        sa = sb = None
+        if return_samples and mode == 1:
+            sa = np.empty((n, n_samples, 3), np.float32)
+            sb = np.empty((n, n_samples, 3), np.float32)
+        _gpu.hausdorff_batch(av.ctypes.data, avo.ctypes.data, at.ctypes.data, ato.ctypes.data,
+                             bv.ctypes.data, bvo.ctypes.data, bt.ctypes.data, bto.ctypes.data,
+                             n, mode, int(n_samples), dab.ctypes.data, dba.ctypes.data,
+                             sa.ctypes.data if sa is not None else 0, sb.ctypes.data if sb is not None else 0)
+        if sa is not None:
+            return dab, dba, sa, sb
+        return dab, dba
+
     # ---- volumes ----------------------------------------------------------------
     def batch_mesh_volume(self, verts_list, tris_list):
         """Volume of each watertight mesh (reference cuacd/__init__.py:239-262)."""

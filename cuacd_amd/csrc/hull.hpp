@@ -1,4 +1,4 @@
-// hull.hpp - exact convex hull of a point cloud on one wave64.
+// hull.hpp - exact convex hull of a point cloud, cooperative over one workgroup.
 //
 // Semantics follow the reference D&C hull (cuda/hull_dandc.cuh): the fp32
 // input is quantized onto an integer grid (span/10216 per unit, axes permuted
@@ -7,13 +7,16 @@
 // input index (cuda/hull_dandc.cuh:1101-1104, 1494-1509), and the output mesh
 // uses the ORIGINAL fp32 points looked up by index (1268-1274).
 //
-// The algorithm is this library's own and is built for a 64-lane wave:
-//   * points live in registers, PPL per lane (point i sits in lane i%64,
-//     slot i/64), so "farthest conflict point" is one 64-bit wave max;
-//   * faces (plane, vertices, neighbours) live in LDS; the region visible from
-//     a new point is found by a wave-parallel BFS over face adjacency, the
-//     horizon is extracted lane-parallel, and the cone of new faces is stitched
-//     with per-vertex start/end maps (no serial ring walk);
+// The algorithm is this library's own and is built for 64-lane waves:
+//   * points live in registers, PPL per thread (local point i sits in thread
+//     i % T, slot i / T, T = threads of the group), so "farthest conflict
+//     point" is one 64-bit max reduction;
+//   * faces (plane, vertices, neighbours) live in a workspace that is LDS for
+//     small hulls (one wave, AS = 3) and global memory for big ones (several
+//     waves, AS = 1); the region visible from a new apex is found by a
+//     cooperative BFS over face adjacency, the horizon is extracted in
+//     parallel and the cone of new faces is stitched with per-vertex start/end
+//     maps (no serial ring walk);
 //   * only conflict points of deleted faces are re-tested, against the new
 //     faces (quickhull outside-set reassignment).
 // Orientation predicate: fp32 plane filter with a proven error bound, and the
@@ -24,6 +27,16 @@
 #include "kerr.h"
 
 namespace cuacd {
+
+template <int AS, class T> using asptr = __attribute__((address_space(AS))) T*;
+
+// clang vector types (not HIP's wrapper classes) so they load and store
+// through address-space-qualified pointers as single ds/global ops
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned short u4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+constexpr int AS_GLOBAL = 1;
+constexpr int AS_LDS = 3;
 
 constexpr float QUANT_STEPS = 10216.0f;  // grid span (cuda/hull_dandc.cuh:1374-1376)
 
@@ -49,9 +62,12 @@ __device__ __forceinline__ QFrame qframe_make(const float mn[3], const float mx[
     QFrame f;
     f.ax[0] = medAx; f.ax[1] = maxAx; f.ax[2] = minAx;
     for (int j = 0; j < 3; ++j) {
-        int a = f.ax[j];
-        f.cen[j] = (mn[a] + mx[a]) * 0.5f;
-        f.inv[j] = (sc[a] != 0.0f) ? 1.0f / sc[a] : 0.0f;
+        const int a = f.ax[j];
+        const float lo = a == 0 ? mn[0] : (a == 1 ? mn[1] : mn[2]);
+        const float hi = a == 0 ? mx[0] : (a == 1 ? mx[1] : mx[2]);
+        const float s = a == 0 ? sc[0] : (a == 1 ? sc[1] : sc[2]);
+        f.cen[j] = (lo + hi) * 0.5f;
+        f.inv[j] = (s != 0.0f) ? 1.0f / s : 0.0f;
     }
     return f;
 }
@@ -61,83 +77,140 @@ __device__ __forceinline__ int qcoord(const QFrame& f, int j, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS workspace
+// Workspace
 // ---------------------------------------------------------------------------
-struct HullWS {
-    float4* fpl;     // [FC] unit outward normal + offset, grid space
-    ushort4* fv;     // [FC] v0,v1,v2 (vertex ids, CCW seen from outside), .w = birth/death stamp
-    ushort4* fn;     // [FC] neighbour across edge (v0v1, v1v2, v2v0), .w = alive flag
-    int* fstamp;     // [FC] BFS visit stamp
-    uint8_t* fvis;   // [FC] visibility from the current apex
-    uint16_t* ffree; // [FC] free face ids
-    uint16_t* vis;   // [FC] visible faces of the current insertion
-    short4* vq;      // [VC] grid coordinates of hull vertices
-    int* vpid;       // [VC] local point id of each hull vertex
-    uint16_t* smap;  // [VC] horizon edge starting at vertex
-    uint16_t* emap;  // [VC] horizon edge ending at vertex
-    uint16_t* hface; // [VC] face beyond horizon edge i
-    uint8_t* hedge;  // [VC] edge slot of edge i inside hface
-    uint16_t* ha;    // [VC] horizon edge i = ha -> hb
-    uint16_t* hb;    // [VC]
-    uint16_t* hnew;  // [VC] new face built on horizon edge i
-    uint16_t* vout;  // [VC] output index of a hull vertex
-    uint8_t* vref;   // [VC] vertex referenced by a live face
-    uint32_t* bm;    // [64] bitmap over local point ids (for output ordering)
-    int* wpre;       // [64] prefix popcounts of bm
-    int* ctr;        // [8] shared counters
+template <int AS> struct HullWS {
+    asptr<AS, f4v> fpl;     // [FC] unit outward normal + offset, grid space
+    asptr<AS, u4v> fv;     // [FC] v0,v1,v2 (CCW seen from outside), .w = birth/death stamp
+    asptr<AS, u4v> fn;     // [FC] neighbour across edge (v0v1, v1v2, v2v0), .w = alive flag
+    asptr<AS, int> fstamp;     // [FC] BFS visit stamp
+    asptr<AS, uint8_t> fvis;   // [FC] visibility from the current apex
+    asptr<AS, uint16_t> ffree; // [FC] free face ids
+    asptr<AS, uint16_t> vis;   // [FC] visible faces of the current insertion
+    asptr<AS, s4v> vq;      // [VC] grid coordinates of hull vertices
+    asptr<AS, int> vpid;       // [VC] local point id of each hull vertex
+    asptr<AS, uint16_t> smap;  // [VC] horizon edge starting at vertex
+    asptr<AS, uint16_t> emap;  // [VC] horizon edge ending at vertex
+    asptr<AS, uint16_t> hface; // [VC] face beyond horizon edge i
+    asptr<AS, uint8_t> hedge;  // [VC] edge slot of edge i inside hface
+    asptr<AS, uint16_t> ha;    // [VC] horizon edge i = ha -> hb
+    asptr<AS, uint16_t> hb;    // [VC]
+    asptr<AS, uint16_t> hnew;  // [VC] new face built on horizon edge i
+    asptr<AS, uint16_t> vout;  // [VC] output index of a hull vertex
+    asptr<AS, uint8_t> vref;   // [VC] vertex referenced by a live face
+    // Always in LDS, whatever AS is: these are updated with atomics and then
+    // read back; global atomics execute in L2 while plain loads may hit a
+    // stale line of the CU's L1, so atomic-then-load state stays in LDS.
+    asptr<AS_LDS, uint32_t> bm; // [NP/32] bitmap over local point ids (output order)
+    asptr<AS_LDS, int> wpre;    // [NP/32] prefix popcounts of bm
+    asptr<AS_LDS, int> ctr;     // [8] counters: 0 nvis, 1 nhz
+    int VC, FC, NP;
 };
 
 // FC = 2*VC + 8 faces bound the live faces of a hull with VC vertices plus the
-// transient pair created by one insertion (nhz <= nvis + 2).
+// transient pair created by one insertion (nhz <= nvis + 2). Face and vertex
+// ids are 16-bit, so VC <= 32760.
 __host__ __device__ constexpr int hull_fc(int vc) { return 2 * vc + 8; }
 
-__host__ __device__ constexpr unsigned hull_ws_bytes(int vc) {
-    // keep every sub-array 16-byte aligned
-    return (unsigned)(((unsigned)hull_fc(vc) * (16u + 8u + 8u + 4u) + 15u) / 16u * 16u
-                    + ((unsigned)hull_fc(vc) * (1u + 2u + 2u) + 15u) / 16u * 16u
-                    + (unsigned)vc * 8u + (unsigned)vc * 4u
-                    + ((unsigned)vc * (2u * 6u + 1u + 1u) + 15u) / 16u * 16u
-                    + 64u * 4u + 64u * 4u + 8u * 4u + 64u);
+// Carves the workspace from `p` (16-byte aligned sub-arrays) and returns its
+// size; with p == nullptr it only sizes it. One routine for both, so the size
+// requested and the carve can never disagree. np = point capacity.
+template <int AS>
+__host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int np, HullWS<AS>* out) {
+    HullWS<AS> w{};
+    const unsigned fc = (unsigned)hull_fc(vc), v = (unsigned)vc, nw = (unsigned)(np + 31) / 32u;
+    unsigned off = 0;
+    auto take = [&](unsigned bytes) {
+        char* r = p ? p + off : nullptr;
+        off += (bytes + 15u) / 16u * 16u;
+        return r;
+    };
+    w.fpl = (asptr<AS, f4v>)take(fc * 16u);
+    w.fv = (asptr<AS, u4v>)take(fc * 8u);
+    w.fn = (asptr<AS, u4v>)take(fc * 8u);
+    w.fstamp = (asptr<AS, int>)take(fc * 4u);
+    w.fvis = (asptr<AS, uint8_t>)take(fc);
+    w.ffree = (asptr<AS, uint16_t>)take(fc * 2u);
+    w.vis = (asptr<AS, uint16_t>)take(fc * 2u);
+    w.vq = (asptr<AS, s4v>)take(v * 8u);
+    w.vpid = (asptr<AS, int>)take(v * 4u);
+    w.smap = (asptr<AS, uint16_t>)take(v * 2u);
+    w.emap = (asptr<AS, uint16_t>)take(v * 2u);
+    w.hface = (asptr<AS, uint16_t>)take(v * 2u);
+    w.hedge = (asptr<AS, uint8_t>)take(v);
+    w.ha = (asptr<AS, uint16_t>)take(v * 2u);
+    w.hb = (asptr<AS, uint16_t>)take(v * 2u);
+    w.hnew = (asptr<AS, uint16_t>)take(v * 2u);
+    w.vout = (asptr<AS, uint16_t>)take(v * 2u);
+    w.vref = (asptr<AS, uint8_t>)take(v);
+    (void)nw;  // bm / wpre / ctr live in HullBlockLDS (hull_ws_attach)
+    w.VC = vc;
+    w.FC = (int)fc;
+    w.NP = np;
+    if (out) *out = w;
+    return off;
 }
 
-__device__ __forceinline__ HullWS hull_ws_carve(char* p, int vc) {
-    HullWS w;
-    const int fc = hull_fc(vc);
-    auto take = [&](unsigned bytes) { char* r = p; p += (bytes + 15u) / 16u * 16u; return r; };
-    w.fpl = (float4*)take(fc * 16u);
-    w.fv = (ushort4*)take(fc * 8u);
-    w.fn = (ushort4*)take(fc * 8u);
-    w.fstamp = (int*)take(fc * 4u);
-    w.fvis = (uint8_t*)take(fc);
-    w.ffree = (uint16_t*)take(fc * 2u);
-    w.vis = (uint16_t*)take(fc * 2u);
-    w.vq = (short4*)take(vc * 8u);
-    w.vpid = (int*)take(vc * 4u);
-    w.smap = (uint16_t*)take(vc * 2u);
-    w.emap = (uint16_t*)take(vc * 2u);
-    w.hface = (uint16_t*)take(vc * 2u);
-    w.hedge = (uint8_t*)take(vc);
-    w.ha = (uint16_t*)take(vc * 2u);
-    w.hb = (uint16_t*)take(vc * 2u);
-    w.hnew = (uint16_t*)take(vc * 2u);
-    w.vout = (uint16_t*)take(vc * 2u);
-    w.vref = (uint8_t*)take(vc);
-    w.bm = (uint32_t*)take(64u * 4u);
-    w.wpre = (int*)take(64u * 4u);
-    w.ctr = (int*)take(8u * 4u);
+template <int AS> __host__ __device__ inline unsigned hull_ws_bytes(int vc, int np) {
+    return hull_ws_layout<AS>(nullptr, vc, np, nullptr);
+}
+
+template <int AS> __device__ __forceinline__ HullWS<AS> hull_ws_carve(char* p, int vc, int np) {
+    HullWS<AS> w;
+    hull_ws_layout<AS>(p, vc, np, &w);
+    return w;
+}
+
+__device__ __forceinline__ s4v mk_s4(int x, int y, int z, int w) {
+    s4v r = {(short)x, (short)y, (short)z, (short)w};
+    return r;
+}
+__device__ __forceinline__ u4v mk_u4(int x, int y, int z, int w) {
+    u4v r = {(unsigned short)x, (unsigned short)y, (unsigned short)z, (unsigned short)w};
+    return r;
+}
+__device__ __forceinline__ f4v mk_f4(float x, float y, float z, float w) {
+    f4v r = {x, y, z, w};
+    return r;
+}
+__device__ __forceinline__ float sel3(const float* v, int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
+
+constexpr int HULL_NP_MAX = 16384;  // largest point set one workgroup hulls
+
+// Small block-wide scratch that always lives in LDS (reductions, broadcasts,
+// counters and the output-order bitmap).
+struct HullBlockLDS {
+    unsigned long long red[16];
+    double dred[16];
+    int bc[8];
+    int ctr[8];
+    float bmn[16][3];  // per-wave AABB partials
+    float bmx[16][3];
+    uint32_t bm[HULL_NP_MAX / 32];
+    int wpre[HULL_NP_MAX / 32];
+};
+
+template <int AS> __device__ __forceinline__ HullWS<AS> hull_ws_attach(HullWS<AS> w, HullBlockLDS* sb) {
+    w.bm = (asptr<AS_LDS, uint32_t>)sb->bm;
+    w.wpre = (asptr<AS_LDS, int>)sb->wpre;
+    w.ctr = (asptr<AS_LDS, int>)sb->ctr;
     return w;
 }
 
 // ---------------------------------------------------------------------------
 // Predicates
 // ---------------------------------------------------------------------------
-// Filter bound (grid units). With |q_i| <= 5109 and a unit normal rounded once
-// from an FP64 normalisation, |t_f - t| <= eps32*(|q|_inf*sum|n_i| + |w|)
-// + 3 fma roundings of magnitude <= 2*8850 -> < 4.5e-3. 1/64 leaves 3.4x margin.
+// Filter bound (grid units). The stored plane is n' = s*n + e, w = -fl(n'.a),
+// with n the exact integer normal, s ~ 1/|n| and |e_i| <= 2 eps s|n_i|. For
+// |q_i| <= 5109 and |q - a|_inf <= 10218 the computed value satisfies
+// |t_f - s*T| <= eps*|n'|_1*(2*10218 + 9*5109) < 7e-3 (T = exact orientation,
+// eps = 2^-24, |n'|_1 <= sqrt(3)). 1/64 leaves a 2.2x margin, so |t_f| > 1/64
+// decides the sign of T.
 constexpr float HULL_FILTER_EPS = 1.0f / 64.0f;
 
-__device__ __forceinline__ double orient_exact(short4 a, short4 b, short4 c, int qx, int qy, int qz) {
-    // ((b-a) x (c-a)) . (q-a); every product < 2^29 * 2^14, exact in FP64.
+__device__ __forceinline__ double orient_exact(s4v a, s4v b, s4v c, int qx, int qy, int qz) {
+    // ((b-a) x (c-a)) . (q-a); every product < 2^29 * 2^14, exact in FP64
+    // (also under fma contraction: each fma adds exact integers below 2^53).
     double bx = b.x - a.x, by = b.y - a.y, bz = b.z - a.z;
     double cx = c.x - a.x, cy = c.y - a.y, cz = c.z - a.z;
     double dx = qx - a.x, dy = qy - a.y, dz = qz - a.z;
@@ -147,34 +220,95 @@ __device__ __forceinline__ double orient_exact(short4 a, short4 b, short4 c, int
     return nx * dx + ny * dy + nz * dz;
 }
 
-// >0 iff q strictly above face f; *tf receives the fp32 signed distance.
-__device__ __forceinline__ bool face_above(const HullWS& w, int f, int qx, int qy, int qz, float* tf) {
-    float4 pl = w.fpl[f];
-    float t = fmaf(pl.x, (float)qx, fmaf(pl.y, (float)qy, fmaf(pl.z, (float)qz, pl.w)));
+// true iff q is strictly above face f; *tf receives the fp32 signed distance.
+template <int AS>
+__device__ __forceinline__ bool face_above(const HullWS<AS> w, int f, int qx, int qy, int qz, float* tf) {
+    const f4v pl = w.fpl[f];
+    const float t = fmaf(pl.x, (float)qx, fmaf(pl.y, (float)qy, fmaf(pl.z, (float)qz, pl.w)));
     *tf = t;
     if (t > HULL_FILTER_EPS) return true;
     if (t < -HULL_FILTER_EPS) return false;
-    ushort4 v = w.fv[f];
+    const u4v v = w.fv[f];
     return orient_exact(w.vq[v.x], w.vq[v.y], w.vq[v.z], qx, qy, qz) > 0.0;
 }
 
-__device__ __forceinline__ void face_set_plane(const HullWS& w, int f, short4 a, short4 b, short4 c) {
-    int bx = b.x - a.x, by = b.y - a.y, bz = b.z - a.z;
-    int cx = c.x - a.x, cy = c.y - a.y, cz = c.z - a.z;
+template <int AS>
+__device__ __forceinline__ void face_set_plane(const HullWS<AS> w, int f, s4v a, s4v b, s4v c) {
+    const int bx = b.x - a.x, by = b.y - a.y, bz = b.z - a.z;
+    const int cx = c.x - a.x, cy = c.y - a.y, cz = c.z - a.z;
     // exact in int32: |components| < 2^29
-    int nx = by * cz - bz * cy;
-    int ny = bz * cx - bx * cz;
-    int nz = bx * cy - by * cx;
-    double dnx = nx, dny = ny, dnz = nz;
-    double len = sqrt(dnx * dnx + dny * dny + dnz * dnz);
-    double il = len > 0.0 ? 1.0 / len : 0.0;
-    double ux = dnx * il, uy = dny * il, uz = dnz * il;
-    double ww = -(ux * a.x + uy * a.y + uz * a.z);
-    w.fpl[f] = make_float4((float)ux, (float)uy, (float)uz, (float)ww);
+    const float nx = (float)(by * cz - bz * cy);
+    const float ny = (float)(bz * cx - bx * cz);
+    const float nz = (float)(bx * cy - by * cx);
+    const float l2 = fmaf(nx, nx, fmaf(ny, ny, nz * nz));
+    const float il = l2 > 0.0f ? rsqrtf(l2) : 0.0f;
+    const float ux = nx * il, uy = ny * il, uz = nz * il;
+    const float ww = -fmaf(ux, (float)a.x, fmaf(uy, (float)a.y, uz * (float)a.z));
+    w.fpl[f] = mk_f4(ux, uy, uz, ww);
+}
+
+template <class P> __device__ __forceinline__ int atom_exch(P p, int v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class P> __device__ __forceinline__ int atom_add(P p, int v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class P> __device__ __forceinline__ void atom_or(P p, uint32_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Block-wide max of a 64-bit key (every thread gets the result).
+template <int NW> __device__ __forceinline__ uint64_t block_max_u64(uint64_t v, HullBlockLDS* s) {
+    v = wave_max_u64(v);
+    if (NW == 1) return v;
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 0) s->red[wv] = v;
+    __syncthreads();
+    uint64_t r = s->red[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r = s->red[i] > r ? s->red[i] : r;
+    __syncthreads();
+    return r;
+}
+template <int NW> __device__ __forceinline__ uint64_t block_min_u64(uint64_t v, HullBlockLDS* s) {
+    v = wave_min_u64(v);
+    if (NW == 1) return v;
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 0) s->red[wv] = v;
+    __syncthreads();
+    uint64_t r = s->red[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r = s->red[i] < r ? s->red[i] : r;
+    __syncthreads();
+    return r;
+}
+template <int NW> __device__ __forceinline__ int block_sum_i(int v, HullBlockLDS* s) {
+    v = wave_sum_i(v);
+    if (NW == 1) return v;
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 0) s->red[wv] = (unsigned long long)(unsigned)v;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) r += (int)(unsigned)s->red[i];
+    __syncthreads();
+    return r;
+}
+template <int NW> __device__ __forceinline__ double block_sum_d(double v, HullBlockLDS* s) {
+    v = wave_sum_d(v);
+    if (NW == 1) return v;
+    const int wv = threadIdx.x >> 6;
+    if (lane_id() == 0) s->dred[wv] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) r += s->dred[i];
+    __syncthreads();
+    return r;
 }
 
 // ---------------------------------------------------------------------------
-// Quickhull on one wave.
+// Quickhull
 // ---------------------------------------------------------------------------
 enum HullStatus { HULL_OK = 0, HULL_DEGENERATE = 1, HULL_OVERFLOW = 2 };
 
@@ -184,40 +318,53 @@ struct HullResult {
     int ftop;   // face ids in use are [0, ftop)
 };
 
-// px/py/pz: grid coords of local points i = k*64 + lane (valid iff i < n).
-// VC: vertex capacity of the workspace (>= hull vertex count).
-template <int PPL>
-__device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (&px)[PPL],
-                                     const int (&py)[PPL], const int (&pz)[PPL]) {
-    const int lane = lane_id();
-    const int FC = hull_fc(VC);
+// Cooperative exact quickhull over NW waves (blockDim.x == 64*NW).
+// px/py/pz: grid coords of local points i = k*T + tid (valid iff i < n), T = 64*NW.
+template <int PPL, int AS, int NW>
+__device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBlockLDS* sb, int n, const int (&px)[PPL],
+                                      const int (&py)[PPL], const int (&pz)[PPL]) {
+    constexpr int T = 64 * NW;
+    const int tid = threadIdx.x;
+    const int VC = w.VC, FC = w.FC;
     int conf[PPL];
     float dist[PPL];
     HullResult res{HULL_OK, 0, 0};
 
+    // coordinates of local point ip, broadcast to the whole block
     auto fetch = [&](int ip, int& x, int& y, int& z) {
-        const int own = ip & 63, slot = ip >> 6;
+        const int own = ip % T, slot = ip / T;
+        if (NW == 1) {
 #pragma unroll
-        for (int k = 0; k < PPL; ++k)
-            if (k == slot) {
-                x = read_lane(px[k], own);
-                y = read_lane(py[k], own);
-                z = read_lane(pz[k], own);
+            for (int k = 0; k < PPL; ++k)
+                if (k == slot) {
+                    x = read_lane(px[k], own);
+                    y = read_lane(py[k], own);
+                    z = read_lane(pz[k], own);
+                }
+        } else {
+            if (tid == own) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    if (k == slot) { sb->bc[0] = px[k]; sb->bc[1] = py[k]; sb->bc[2] = pz[k]; }
             }
+            __syncthreads();
+            x = sb->bc[0]; y = sb->bc[1]; z = sb->bc[2];
+            __syncthreads();
+        }
     };
 
     // ---- initial simplex --------------------------------------------------
     uint64_t key = ~0ull;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        const int i = k * T + tid;
         if (i < n) {
             uint64_t kk = ((uint64_t)(py[k] + 8192) << 49) | ((uint64_t)(px[k] + 8192) << 35) |
                           ((uint64_t)(pz[k] + 8192) << 21) | (uint64_t)i;
             key = kk < key ? kk : key;
         }
     }
-    key = wave_min_u64(key);
+    key = block_min_u64<NW>(key, sb);
     const int i0 = (int)(key & 0x1FFFFF);
     int x0, y0, z0;
     fetch(i0, x0, y0, z0);
@@ -225,7 +372,7 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
     key = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        const int i = k * T + tid;
         if (i < n) {
             int dx = px[k] - x0, dy = py[k] - y0, dz = pz[k] - z0;
             uint64_t d2 = (uint64_t)(dx * dx) + (uint64_t)(dy * dy) + (uint64_t)(dz * dz);
@@ -233,7 +380,7 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
             key = kk > key ? kk : key;
         }
     }
-    key = wave_max_u64(key);
+    key = block_max_u64<NW>(key, sb);
     if ((key >> 32) == 0) { res.status = HULL_DEGENERATE; return res; }
     int i1 = (int)(0xFFFFFFFFu - (uint32_t)key);
     int x1, y1, z1;
@@ -243,7 +390,7 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
     key = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        const int i = k * T + tid;
         if (i < n) {
             int dx = px[k] - x0, dy = py[k] - y0, dz = pz[k] - z0;
             double cx = (double)(ey * dz - ez * dy), cy = (double)(ez * dx - ex * dz),
@@ -253,76 +400,63 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
             key = kk > key ? kk : key;
         }
     }
-    key = wave_max_u64(key);
+    key = block_max_u64<NW>(key, sb);
     if ((key >> 32) == 0) { res.status = HULL_DEGENERATE; return res; }
     int i2 = (int)(0xFFFFFFFFu - (uint32_t)key);
     int x2, y2, z2;
     fetch(i2, x2, y2, z2);
 
-    const short4 s0 = make_short4(x0, y0, z0, 0);
-    const short4 s1 = make_short4(x1, y1, z1, 0);
-    const short4 s2 = make_short4(x2, y2, z2, 0);
+    const s4v s0 = mk_s4(x0, y0, z0, 0);
+    const s4v s1 = mk_s4(x1, y1, z1, 0);
+    const s4v s2 = mk_s4(x2, y2, z2, 0);
     key = 0;
-    double o_lane = 0.0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        const int i = k * T + tid;
         if (i < n) {
             double o = orient_exact(s0, s1, s2, px[k], py[k], pz[k]);
             float ao = (float)fabs(o);
             uint64_t kk = ((uint64_t)__float_as_uint(ao) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)i);
-            if (kk > key) { key = kk; o_lane = o; }
+            key = kk > key ? kk : key;
         }
     }
-    {
-        uint64_t km = wave_max_u64(key);
-        // the winning lane holds the exact signed orientation of the apex
-        uint64_t win = ballot(key == km && km != 0);
-        if ((km >> 32) == 0) { res.status = HULL_DEGENERATE; return res; }
-        int wl = __builtin_ffsll((long long)win) - 1;
-        double o3 = __shfl(o_lane, wl);
-        key = km;
-        if (o3 < 0.0) {  // make (p0,p1,p2,p3) positively oriented
-            int t = i1; i1 = i2; i2 = t;
-        }
-    }
+    key = block_max_u64<NW>(key, sb);
+    if ((key >> 32) == 0) { res.status = HULL_DEGENERATE; return res; }
     const int i3 = (int)(0xFFFFFFFFu - (uint32_t)key);
-    {
-        const int ids[4] = {i0, i1, i2, i3};
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int x, y, z;
-            fetch(ids[v], x, y, z);
-            if (lane == 0) {
-                w.vq[v] = make_short4(x, y, z, 0);
-                w.vpid[v] = ids[v];
-            }
-        }
+    int x3, y3, z3;
+    fetch(i3, x3, y3, z3);
+    if (orient_exact(s0, s1, s2, x3, y3, z3) < 0.0) {  // make (p0,p1,p2,p3) positively oriented
+        int t = i1; i1 = i2; i2 = t;
+        t = x1; x1 = x2; x2 = t;
+        t = y1; y1 = y2; y2 = t;
+        t = z1; z1 = z2; z2 = t;
+    }
+    if (tid == 0) {
+        w.vq[0] = mk_s4(x0, y0, z0, 0); w.vpid[0] = i0;
+        w.vq[1] = mk_s4(x1, y1, z1, 0); w.vpid[1] = i1;
+        w.vq[2] = mk_s4(x2, y2, z2, 0); w.vpid[2] = i2;
+        w.vq[3] = mk_s4(x3, y3, z3, 0); w.vpid[3] = i3;
     }
     __syncthreads();
-    // faces (0,2,1) (0,1,3) (1,2,3) (0,3,2); neighbours derived from shared edges
-    if (lane < 4) {
-        const ushort4 fvv[4] = {make_ushort4(0, 2, 1, 0), make_ushort4(0, 1, 3, 0),
-                                make_ushort4(1, 2, 3, 0), make_ushort4(0, 3, 2, 0)};
-        const ushort4 fnn[4] = {make_ushort4(3, 2, 1, 1), make_ushort4(0, 2, 3, 1),
-                                make_ushort4(0, 3, 1, 1), make_ushort4(1, 2, 0, 1)};
-        ushort4 fvl = fvv[0], fnl = fnn[0];
-        if (lane == 1) { fvl = fvv[1]; fnl = fnn[1]; }
-        if (lane == 2) { fvl = fvv[2]; fnl = fnn[2]; }
-        if (lane == 3) { fvl = fvv[3]; fnl = fnn[3]; }
-        w.fv[lane] = fvl;
-        w.fn[lane] = fnl;
-        w.fstamp[lane] = 0;
-        w.fvis[lane] = 0;
-        face_set_plane(w, lane, w.vq[fvl.x], w.vq[fvl.y], w.vq[fvl.z]);
+    // faces (0,2,1) (0,1,3) (1,2,3) (0,3,2); neighbours from the shared edges
+    if (tid < 4) {
+        u4v fvl, fnl;
+        if (tid == 0) { fvl = mk_u4(0, 2, 1, 0); fnl = mk_u4(3, 2, 1, 1); }
+        else if (tid == 1) { fvl = mk_u4(0, 1, 3, 0); fnl = mk_u4(0, 2, 3, 1); }
+        else if (tid == 2) { fvl = mk_u4(1, 2, 3, 0); fnl = mk_u4(0, 3, 1, 1); }
+        else { fvl = mk_u4(0, 3, 2, 0); fnl = mk_u4(1, 2, 0, 1); }
+        w.fv[tid] = fvl;
+        w.fn[tid] = fnl;
+        w.fstamp[tid] = 0;
+        w.fvis[tid] = 0;
+        face_set_plane(w, tid, w.vq[fvl.x], w.vq[fvl.y], w.vq[fvl.z]);
     }
-    if (lane == 0) { w.ctr[0] = 0; w.ctr[1] = 0; }
     __syncthreads();
 
     // ---- assign every other point to the face it is farthest above ----------
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        const int i = k * T + tid;
         conf[k] = -1;
         dist[k] = 0.0f;
         if (i < n) {
@@ -348,30 +482,43 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
         for (int k = 0; k < PPL; ++k) {
             if (conf[k] >= 0) {
                 uint64_t kk = ((uint64_t)fkey(dist[k]) << 32) |
-                              (uint64_t)(0xFFFFFFFFu - (uint32_t)(k * 64 + lane));
+                              (uint64_t)(0xFFFFFFFFu - (uint32_t)(k * T + tid));
                 sk = kk > sk ? kk : sk;
             }
         }
-        sk = wave_max_u64(sk);
+        sk = block_max_u64<NW>(sk, sb);
         if (sk == 0) break;
         const int ip = (int)(0xFFFFFFFFu - (uint32_t)sk);
         if (nvert >= VC) { res.status = HULL_OVERFLOW; break; }
+        const int vid = nvert++;
         int qx, qy, qz, f0 = 0;
         {
-            const int own = ip & 63, slot = ip >> 6;
+            const int own = ip % T, slot = ip / T;
+            if (NW == 1) {
 #pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if (k == slot) {
-                    qx = read_lane(px[k], own);
-                    qy = read_lane(py[k], own);
-                    qz = read_lane(pz[k], own);
-                    f0 = read_lane(conf[k], own);
-                    if (lane == own) conf[k] = -2;
+                for (int k = 0; k < PPL; ++k)
+                    if (k == slot) {
+                        qx = read_lane(px[k], own);
+                        qy = read_lane(py[k], own);
+                        qz = read_lane(pz[k], own);
+                        f0 = read_lane(conf[k], own);
+                        if (tid == own) conf[k] = -2;
+                    }
+            } else {
+                if (tid == own) {
+#pragma unroll
+                    for (int k = 0; k < PPL; ++k)
+                        if (k == slot) {
+                            sb->bc[0] = px[k]; sb->bc[1] = py[k]; sb->bc[2] = pz[k]; sb->bc[3] = conf[k];
+                            conf[k] = -2;
+                        }
                 }
+                __syncthreads();
+                qx = sb->bc[0]; qy = sb->bc[1]; qz = sb->bc[2]; f0 = sb->bc[3];
+            }
         }
-        const int vid = nvert++;
-        if (lane == 0) {
-            w.vq[vid] = make_short4(qx, qy, qz, 0);
+        if (tid == 0) {
+            w.vq[vid] = mk_s4(qx, qy, qz, 0);
             w.vpid[vid] = ip;
             w.fstamp[f0] = it;
             w.fvis[f0] = 1;
@@ -384,20 +531,20 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
         // ---- BFS over the faces visible from the apex --------------------
         int head = 0, nvis = 1;
         while (head < nvis) {
-            const int cnt = min(nvis - head, 64);
-            if (lane < cnt) {
-                const int f = w.vis[head + lane];
-                const ushort4 nb = w.fn[f];
+            const int cnt = min(nvis - head, T);
+            if (tid < cnt) {
+                const int f = w.vis[head + tid];
+                const u4v nb = w.fn[f];
                 const int g3[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
                     const int g = g3[e];
-                    if (atomicExch(&w.fstamp[g], it) != it) {
+                    if (atom_exch(&w.fstamp[g], it) != it) {
                         float t;
                         const bool v = face_above(w, g, qx, qy, qz, &t);
                         w.fvis[g] = v ? 1 : 0;
                         if (v) {
-                            int s = atomicAdd(&w.ctr[0], 1);
+                            const int s = atom_add(&w.ctr[0], 1);
                             w.vis[s] = (uint16_t)g;
                         }
                     }
@@ -405,37 +552,39 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
             }
             head += cnt;
             __syncthreads();
-            nvis = uniform(w.ctr[0]);
+            nvis = w.ctr[0];
+            if (NW == 1) nvis = uniform(nvis);
+            // waves run independently: nobody may append to ctr[0] in the next
+            // round before every wave has read this round's count
+            if (NW > 1) __syncthreads();
         }
 
         // ---- horizon edges (visible face -> non-visible neighbour) ---------
-        for (int b = 0; b < nvis; b += 64) {
-            const int j = b + lane;
-            if (j < nvis) {
-                const int f = w.vis[j];
-                const ushort4 fvl = w.fv[f];
-                const ushort4 nb = w.fn[f];
-                const int vv[3] = {fvl.x, fvl.y, fvl.z};
-                const int gg[3] = {nb.x, nb.y, nb.z};
+        for (int j = tid; j < nvis; j += T) {
+            const int f = w.vis[j];
+            const u4v fvl = w.fv[f];
+            const u4v nb = w.fn[f];
+            const int vv[3] = {fvl.x, fvl.y, fvl.z};
+            const int gg[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const int g = gg[e];
-                    if (!w.fvis[g]) {
-                        const ushort4 gn = w.fn[g];
-                        const int ge = (gn.x == f) ? 0 : ((gn.y == f) ? 1 : 2);
-                        int s = atomicAdd(&w.ctr[1], 1);
-                        if (s < VC) {
-                            w.ha[s] = (uint16_t)vv[e];
-                            w.hb[s] = (uint16_t)vv[(e + 1) % 3];
-                            w.hface[s] = (uint16_t)g;
-                            w.hedge[s] = (uint8_t)ge;
-                        }
+            for (int e = 0; e < 3; ++e) {
+                const int g = gg[e];
+                if (!w.fvis[g]) {
+                    const u4v gn = w.fn[g];
+                    const int ge = (gn.x == f) ? 0 : ((gn.y == f) ? 1 : 2);
+                    const int s = atom_add(&w.ctr[1], 1);
+                    if (s < VC) {
+                        w.ha[s] = (uint16_t)vv[e];
+                        w.hb[s] = (uint16_t)vv[e == 2 ? 0 : e + 1];
+                        w.hface[s] = (uint16_t)g;
+                        w.hedge[s] = (uint8_t)ge;
                     }
                 }
             }
         }
         __syncthreads();
-        const int nhz = uniform(w.ctr[1]);
+        int nhz = w.ctr[1];
+        if (NW == 1) nhz = uniform(nhz);
         if (nhz > VC || nhz < 3) { res.status = HULL_OVERFLOW; break; }
 
         // ---- face ids for the cone: reuse visible ids, then free list, then top
@@ -443,8 +592,8 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
         const int from_free = min(extra, nfree);
         const int from_top = extra - from_free;
         if (ftop + from_top > FC) { res.status = HULL_OVERFLOW; break; }
-        for (int b = 0; b < max(nhz, nvis); b += 64) {
-            const int j = b + lane;
+        const int mx = max(nhz, nvis);
+        for (int j = tid; j < mx; j += T) {
             if (j < nhz) {
                 int id;
                 if (j < nvis) id = w.vis[j];
@@ -453,14 +602,14 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
                 w.hnew[j] = (uint16_t)id;
                 w.smap[w.ha[j]] = (uint16_t)j;
                 w.emap[w.hb[j]] = (uint16_t)j;
-            } else if (j < nvis) {
+            } else {
                 // surplus visible faces die and go back to the free list
                 const int f = w.vis[j];
                 w.ffree[nfree - from_free + (j - nhz)] = (uint16_t)f;
-                ushort4 t = w.fn[f];
+                u4v t = w.fn[f];
                 t.w = 0;
                 w.fn[f] = t;
-                ushort4 u = w.fv[f];
+                u4v u = w.fv[f];
                 u.w = (uint16_t)it;
                 w.fv[f] = u;
             }
@@ -470,30 +619,23 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
         __syncthreads();
 
         // ---- stitch the cone ---------------------------------------------
-        const short4 sp = make_short4(qx, qy, qz, 0);
-        for (int b = 0; b < nhz; b += 64) {
-            const int j = b + lane;
-            if (j < nhz) {
-                const int id = w.hnew[j];
-                const int a = w.ha[j], c = w.hb[j];
-                const int g = w.hface[j];
-                w.fv[id] = make_ushort4(a, c, vid, (uint16_t)it);
-                w.fn[id] = make_ushort4(g, w.hnew[w.smap[c]], w.hnew[w.emap[a]], 1);
-                w.fstamp[id] = it;
-                w.fvis[id] = 0;
-                face_set_plane(w, id, w.vq[a], w.vq[c], sp);
-            }
+        const s4v sp = mk_s4(qx, qy, qz, 0);
+        for (int j = tid; j < nhz; j += T) {
+            const int id = w.hnew[j];
+            const int a = w.ha[j], c = w.hb[j];
+            const int g = w.hface[j];
+            w.fv[id] = mk_u4(a, c, vid, (uint16_t)it);
+            w.fn[id] = mk_u4(g, w.hnew[w.smap[c]], w.hnew[w.emap[a]], 1);
+            w.fstamp[id] = it;
+            w.fvis[id] = 0;
+            face_set_plane(w, id, w.vq[a], w.vq[c], sp);
         }
         __syncthreads();
-        for (int b = 0; b < nhz; b += 64) {
-            const int j = b + lane;
-            if (j < nhz) {
-                const int g = w.hface[j];
-                const int ge = w.hedge[j];
-                const uint16_t id = w.hnew[j];
-                uint16_t* slotp = ((uint16_t*)&w.fn[g]) + ge;
-                *slotp = id;
-            }
+        for (int j = tid; j < nhz; j += T) {
+            const int g = w.hface[j];
+            const int ge = w.hedge[j];
+            asptr<AS, uint16_t> slotp = ((asptr<AS, uint16_t>)&w.fn[g]) + ge;
+            *slotp = w.hnew[j];
         }
         __syncthreads();
 
@@ -519,77 +661,110 @@ __device__ HullResult wave_quickhull(const HullWS& w, int n, int VC, const int (
     }
     res.nvert = nvert;
     res.ftop = ftop;
+    __syncthreads();
     return res;
 }
 
-// After wave_quickhull: mark referenced vertices and give them output indices in
-// ascending local-point order. Returns the output vertex count (wave-uniform).
-__device__ inline int hull_rank_vertices(const HullWS& w, const HullResult& r) {
-    const int lane = lane_id();
-    for (int v = lane; v < r.nvert; v += 64) w.vref[v] = 0;
-    w.bm[lane] = 0;
+// After block_quickhull: mark referenced vertices and give them output
+// indices in ascending local-point order. Returns the output vertex count.
+template <int AS, int NW>
+__device__ __forceinline__ int hull_rank_vertices(const HullWS<AS> w, HullBlockLDS* sb, const HullResult& r, int n) {
+    constexpr int T = 64 * NW;
+    const int tid = threadIdx.x;
+    const int nw = (n + 31) >> 5;
+    for (int v = tid; v < r.nvert; v += T) w.vref[v] = 0;
+    for (int i = tid; i < nw; i += T) w.bm[i] = 0;
     __syncthreads();
-    for (int f = lane; f < r.ftop; f += 64) {
-        const ushort4 nb = w.fn[f];
+    for (int f = tid; f < r.ftop; f += T) {
+        const u4v nb = w.fn[f];
         if (nb.w) {
-            const ushort4 v = w.fv[f];
+            const u4v v = w.fv[f];
             w.vref[v.x] = 1;
             w.vref[v.y] = 1;
             w.vref[v.z] = 1;
         }
     }
     __syncthreads();
-    for (int v = lane; v < r.nvert; v += 64)
+    for (int v = tid; v < r.nvert; v += T)
         if (w.vref[v]) {
             const int p = w.vpid[v];
-            atomicOr(&w.bm[p >> 5], 1u << (p & 31));
+            atom_or(&w.bm[p >> 5], 1u << (p & 31));
         }
     __syncthreads();
-    const int c = __popc(w.bm[lane]);
-    const int incl = wave_incl_scan(c);
-    w.wpre[lane] = incl - c;
-    __syncthreads();
-    for (int v = lane; v < r.nvert; v += 64)
+    // exclusive prefix popcount over the bitmap words, 64 words per wave step
+    int carry = 0;
+    for (int b = 0; b < nw; b += 64) {
+        int c = 0;
+        if (tid < 64 && b + tid < nw) c = __popc(w.bm[b + tid]);
+        const int incl = wave_incl_scan(c);
+        if (tid < 64 && b + tid < nw) w.wpre[b + tid] = carry + incl - c;
+        carry += __shfl(incl, 63);
+    }
+    if (NW > 1) {
+        if (tid == 0) sb->bc[7] = carry;
+        __syncthreads();
+        carry = sb->bc[7];
+        __syncthreads();
+    } else {
+        __syncthreads();
+    }
+    for (int v = tid; v < r.nvert; v += T)
         if (w.vref[v]) {
             const int p = w.vpid[v];
             w.vout[v] = (uint16_t)(w.wpre[p >> 5] + __popc(w.bm[p >> 5] & ((1u << (p & 31)) - 1u)));
         }
     __syncthreads();
-    return uniform(__shfl(incl, 63));
+    return carry;
 }
 
-__device__ inline int hull_count_faces(const HullWS& w, const HullResult& r) {
+template <int AS, int NW>
+__device__ __forceinline__ int hull_count_faces(const HullWS<AS> w, HullBlockLDS* sb, const HullResult& r) {
     int c = 0;
-    for (int f = lane_id(); f < r.ftop; f += 64) c += w.fn[f].w ? 1 : 0;
-    return wave_sum_i(c);
+    for (int f = threadIdx.x; f < r.ftop; f += 64 * NW) c += w.fn[f].w ? 1 : 0;
+    return block_sum_i<NW>(c, sb);
 }
 
 // Write triangles (output vertex indices) in face-id order; returns count.
-__device__ inline int hull_write_tris(const HullWS& w, const HullResult& r, int* __restrict__ out_tris) {
-    const int lane = lane_id();
+template <int AS, int NW>
+__device__ __forceinline__ int hull_write_tris(const HullWS<AS> w, HullBlockLDS* sb, const HullResult& r,
+                                      int* __restrict__ out_tris) {
+    const int tid = threadIdx.x;
     int base = 0;
-    for (int b = 0; b < r.ftop; b += 64) {
-        const int f = b + lane;
+    for (int b = 0; b < r.ftop; b += 64 * NW) {
+        const int f = b + tid;
         const bool alive = f < r.ftop && w.fn[f].w;
         const uint64_t m = ballot(alive);
+        int wbase = 0;
+        if (NW > 1) {
+            const int wv = tid >> 6;
+            if (lane_id() == 0) sb->bc[wv] = __popcll(m);
+            __syncthreads();
+            for (int i = 0; i < wv; ++i) wbase += sb->bc[i];
+        }
         if (alive) {
-            const int t = base + mask_rank(m);
-            const ushort4 v = w.fv[f];
+            const int t = base + wbase + mask_rank(m);
+            const u4v v = w.fv[f];
             out_tris[3 * t + 0] = w.vout[v.x];
             out_tris[3 * t + 1] = w.vout[v.y];
             out_tris[3 * t + 2] = w.vout[v.z];
         }
-        base += __popcll(m);
+        if (NW > 1) {
+            int tot = 0;
+            for (int i = 0; i < NW; ++i) tot += sb->bc[i];
+            base += tot;
+            __syncthreads();
+        } else {
+            base += __popcll(m);
+        }
     }
     return base;
 }
 
 // FP64 divergence-theorem volume over live faces, original fp32 coordinates
 // of the point with local id p fetched by `pos(p, xyz)`. Anchored at the first
-// hull vertex to avoid cancellation. Wave-uniform result.
-template <class PosFn>
-__device__ inline double hull_volume_f64(const HullWS& w, const HullResult& r, PosFn pos) {
-    // anchor: first vertex of the lowest live face id (wave-uniform)
+// vertex of the lowest live face to avoid cancellation. Block-uniform result.
+template <int AS, int NW, class PosFn>
+__device__ __forceinline__ double hull_volume_f64(const HullWS<AS> w, HullBlockLDS* sb, const HullResult& r, PosFn pos) {
     int f0 = 0;
     for (int b = 0; b < r.ftop; b += 64) {
         const uint64_t m = ballot(b + lane_id() < r.ftop && w.fn[b + lane_id()].w);
@@ -599,9 +774,9 @@ __device__ inline double hull_volume_f64(const HullWS& w, const HullResult& r, P
     pos(w.vpid[w.fv[f0].x], a);
     const double ax = a[0], ay = a[1], az = a[2];
     double acc = 0.0;
-    for (int f = lane_id(); f < r.ftop; f += 64) {
+    for (int f = threadIdx.x; f < r.ftop; f += 64 * NW) {
         if (!w.fn[f].w) continue;
-        const ushort4 v = w.fv[f];
+        const u4v v = w.fv[f];
         float p0[3], p1[3], p2[3];
         pos(w.vpid[v.x], p0);
         pos(w.vpid[v.y], p1);
@@ -611,7 +786,7 @@ __device__ inline double hull_volume_f64(const HullWS& w, const HullResult& r, P
         const double x2 = p2[0] - ax, y2 = p2[1] - ay, z2 = p2[2] - az;
         acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
     }
-    return wave_sum_d(acc) / 6.0;
+    return block_sum_d<NW>(acc, sb) / 6.0;
 }
 
 }  // namespace cuacd

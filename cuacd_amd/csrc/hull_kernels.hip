@@ -1,13 +1,17 @@
-// hull_kernels.hip - batch exact hulls: one wave64 workgroup per cloud (or chunk).
+// hull_kernels.hip - batch exact hulls.
 //
 // Backs gpu_hull_dandc / batch_hull_volume / batch_hull_dandc_mesh
 // (reference: csrc/test.c:95-160, cuda/test_hull_dandc.cu:24-77).
 //
-// Clouds of up to 2048 points are hulled directly by one wave. Larger clouds
-// use divide and conquer on the same quantization frame: chunks of <= 1024
-// points are hulled independently, the union of their hull vertices (in
-// ascending input order, so lowest-index duplicate resolution is preserved)
-// replaces the cloud, and the step repeats until one wave can finish it.
+// Three tiers, all exact on the same quantization grid:
+//   * clouds of <= 1024 points: one wave64 workgroup, points in registers,
+//     workspace in LDS (hull_wave_kernel);
+//   * clouds of <= 16384 points: eight waves, points in registers, workspace
+//     in global memory (hull_block_kernel);
+//   * larger clouds: divide and conquer - chunks of 1024 points are hulled
+//     independently on the cloud's grid, the union of their hull vertices
+//     (kept in ascending input order, so lowest-index duplicate resolution is
+//     preserved) replaces the cloud, until the block kernel can finish it.
 #include <stdlib.h>
 #include <vector>
 #include "ctx.hpp"
@@ -17,7 +21,9 @@ using namespace cuacd;
 
 namespace {
 
-constexpr int HULL_VC_MAX = 1024;
+constexpr int WAVE_MAX = 1024;    // points per cloud for the one-wave tier
+constexpr int BLOCK_NW = 8;       // waves of the block tier
+constexpr int BLOCK_MAX = 16384;  // points per cloud for the block tier
 constexpr int CHUNK = 1024;
 
 struct HullItemArgs {
@@ -28,8 +34,12 @@ struct HullItemArgs {
     const int* item_cloud;   // optional item -> cloud map (identity if null)
     const QFrame* frames;    // optional frame per cloud (else from the item's points)
     int n_items;
-    int VC;
-    // mesh output (mode 0)
+    int VC;                  // vertex capacity of the workspace
+    int NP;                  // point capacity of the workspace
+    int min_n, max_n;        // items outside [min_n, max_n] are left to another tier
+    char* gws;               // global workspaces (block tier), gws_stride bytes each
+    unsigned long long gws_stride;
+    // mesh output (mode 0), indexed by cloud
     int max_hv, max_ht;
     float* out_verts;
     int* out_tris;
@@ -42,12 +52,11 @@ struct HullItemArgs {
     int* out_cnt;
 };
 
-template <int PPL>
-__global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode) {
-    extern __shared__ __align__(16) char lds[];
-    const int item = blockIdx.x;
-    if (item >= a.n_items) return;
-    const int lane = lane_id();
+template <int PPL, int AS, int NW>
+__device__ __forceinline__ void hull_item(const HullItemArgs& a, int mode, int item, const HullWS<AS> w,
+                                          HullBlockLDS* sb) {
+    constexpr int T = 64 * NW;
+    const int tid = threadIdx.x;
     const int cloud = a.item_cloud ? a.item_cloud[item] : item;
     const float* cpts = a.pts + 3ll * a.offsets[cloud];
     int n;
@@ -58,24 +67,28 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
     } else {
         n = a.offsets[cloud + 1] - a.offsets[cloud];
     }
+    if (n < a.min_n || n > a.max_n) return;  // another tier owns this item
     auto orig = [&](int i) { return sel ? sel[i] : i; };
-
-    if (mode == 0 && lane == 0) {
-        a.out_nv[item] = 0;
-        a.out_nt[item] = 0;
-        a.out_err[item] = 0;
-        if (a.out_vol) a.out_vol[item] = 0.0;
-    }
-    if (mode == 1 && lane == 0) a.out_cnt[item] = 0;
-    if (n < 4 || n > 64 * PPL) {
-        if (n > 64 * PPL && lane == 0) {
-            if (mode == 0) a.out_err[item] = KERR_HULL_CAPACITY;
+    if (tid == 0) {
+        if (mode == 0) {
+            a.out_nv[cloud] = 0;
+            a.out_nt[cloud] = 0;
+            a.out_err[cloud] = 0;
+            if (a.out_vol) a.out_vol[cloud] = 0.0;
+        } else {
+            a.out_cnt[item] = 0;
         }
-        // fewer than 4 points: empty hull (reference: hull_dandc_warp_mesh n<4);
-        // survivor mode keeps them all
-        if (mode == 1 && n <= 64 * PPL) {
-            for (int i = lane; i < n; i += 64) a.out_sel[a.sel_off[item] + i] = orig(i);
-            if (lane == 0) a.out_cnt[item] = n;
+    }
+    if (n < 4 || n > T * PPL) {
+        if (n > T * PPL) {
+            if (tid == 0) {
+                if (mode == 0) a.out_err[cloud] = KERR_HULL_CAPACITY;
+                else a.out_cnt[item] = -1;
+            }
+        } else if (mode == 1) {
+            // fewer than 4 points: no hull (reference hull_dandc_warp_mesh n<4); keep them all
+            for (int i = tid; i < n; i += T) a.out_sel[a.sel_off[item] + i] = orig(i);
+            if (tid == 0) a.out_cnt[item] = n;
         }
         return;
     }
@@ -83,7 +96,8 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
     float xs[PPL], ys[PPL], zs[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int i = k * 64 + lane;
+        xs[k] = ys[k] = zs[k] = 0.0f;
+        const int i = k * T + tid;
         if (i < n) {
             const float* p = cpts + 3ll * orig(i);
             xs[k] = p[0];
@@ -98,7 +112,7 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
         float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
-            if (k * 64 + lane < n) {
+            if (k * T + tid < n) {
                 mn[0] = fminf(mn[0], xs[k]); mx[0] = fmaxf(mx[0], xs[k]);
                 mn[1] = fminf(mn[1], ys[k]); mx[1] = fmaxf(mx[1], ys[k]);
                 mn[2] = fminf(mn[2], zs[k]); mx[2] = fmaxf(mx[2], zs[k]);
@@ -108,25 +122,40 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
             mn[j] = wave_min_f(mn[j]);
             mx[j] = wave_max_f(mx[j]);
         }
+        if (NW > 1) {
+            const int wv = tid >> 6;
+            if (lane_id() == 0)
+                for (int j = 0; j < 3; ++j) {
+                    sb->bmn[wv][j] = mn[j];
+                    sb->bmx[wv][j] = mx[j];
+                }
+            __syncthreads();
+            for (int j = 0; j < 3; ++j) {
+                for (int q = 0; q < NW; ++q) {
+                    mn[j] = fminf(mn[j], sb->bmn[q][j]);
+                    mx[j] = fmaxf(mx[j], sb->bmx[q][j]);
+                }
+            }
+            __syncthreads();
+        }
         fr = qframe_make(mn, mx);
     }
     int px[PPL], py[PPL], pz[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         px[k] = py[k] = pz[k] = 0;
-        if (k * 64 + lane < n) {
+        if (k * T + tid < n) {
             const float v[3] = {xs[k], ys[k], zs[k]};
-            px[k] = qcoord(fr, 0, v[fr.ax[0]]);
-            py[k] = qcoord(fr, 1, v[fr.ax[1]]);
-            pz[k] = qcoord(fr, 2, v[fr.ax[2]]);
+            px[k] = qcoord(fr, 0, sel3(v, fr.ax[0]));
+            py[k] = qcoord(fr, 1, sel3(v, fr.ax[1]));
+            pz[k] = qcoord(fr, 2, sel3(v, fr.ax[2]));
         }
     }
 
-    HullWS w = hull_ws_carve(lds, a.VC);
-    HullResult r = wave_quickhull<PPL>(w, n, a.VC, px, py, pz);
+    HullResult r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
     if (r.status == HULL_OVERFLOW) {
-        if (lane == 0) {
-            if (mode == 0) a.out_err[item] = KERR_HULL_CAPACITY;
+        if (tid == 0) {
+            if (mode == 0) a.out_err[cloud] = KERR_HULL_CAPACITY;
             else a.out_cnt[item] = -1;
         }
         return;
@@ -134,28 +163,27 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
     if (r.status == HULL_DEGENERATE) {
         // flat or collinear input: no solid hull. Survivor mode keeps all points.
         if (mode == 1) {
-            for (int i = lane; i < n; i += 64) a.out_sel[a.sel_off[item] + i] = orig(i);
-            if (lane == 0) a.out_cnt[item] = n;
+            for (int i = tid; i < n; i += T) a.out_sel[a.sel_off[item] + i] = orig(i);
+            if (tid == 0) a.out_cnt[item] = n;
         }
         return;
     }
-    __syncthreads();
-    const int nv = hull_rank_vertices(w, r);
+    const int nv = hull_rank_vertices<AS, NW>(w, sb, r, n);
     if (mode == 1) {
         int* dst = a.out_sel + a.sel_off[item];
-        for (int v = lane; v < r.nvert; v += 64)
+        for (int v = tid; v < r.nvert; v += T)
             if (w.vref[v]) dst[w.vout[v]] = orig(w.vpid[v]);
-        if (lane == 0) a.out_cnt[item] = nv;
+        if (tid == 0) a.out_cnt[item] = nv;
         return;
     }
-    const int nt = hull_count_faces(w, r);
-    if (lane == 0) {
-        a.out_nv[item] = nv;
-        a.out_nt[item] = nt;
+    const int nt = hull_count_faces<AS, NW>(w, sb, r);
+    if (tid == 0) {
+        a.out_nv[cloud] = nv;
+        a.out_nt[cloud] = nt;
     }
     if (nv <= a.max_hv) {
-        float* dv = a.out_verts + 3ll * a.max_hv * item;
-        for (int v = lane; v < r.nvert; v += 64)
+        float* dv = a.out_verts + 3ll * a.max_hv * cloud;
+        for (int v = tid; v < r.nvert; v += T)
             if (w.vref[v]) {
                 const float* p = cpts + 3ll * orig(w.vpid[v]);
                 const int o = w.vout[v];
@@ -164,7 +192,7 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
                 dv[3 * o + 2] = p[2];
             }
     }
-    if (nt <= a.max_ht) hull_write_tris(w, r, a.out_tris + 3ll * a.max_ht * item);
+    if (nt <= a.max_ht) hull_write_tris<AS, NW>(w, sb, r, a.out_tris + 3ll * a.max_ht * cloud);
     if (a.out_vol) {
         auto pos = [&](int p, float* o3) {
             const float* q = cpts + 3ll * orig(p);
@@ -172,9 +200,30 @@ __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode)
             o3[1] = q[1];
             o3[2] = q[2];
         };
-        const double vol = hull_volume_f64(w, r, pos);
-        if (lane == 0) a.out_vol[item] = vol;
+        const double vol = hull_volume_f64<AS, NW>(w, sb, r, pos);
+        if (tid == 0) a.out_vol[cloud] = vol;
     }
+}
+
+template <int PPL>
+__global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ HullBlockLDS sb;
+    const int item = blockIdx.x;
+    if (item >= a.n_items) return;
+    // compile-time capacities: every sub-array address folds into ds offsets
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
+    hull_item<PPL, AS_LDS, 1>(a, mode, item, w, &sb);
+}
+
+template <int PPL>
+__global__ __launch_bounds__(64 * BLOCK_NW) void hull_block_kernel(HullItemArgs a, int mode) {
+    __shared__ HullBlockLDS sb;
+    const int item = blockIdx.x;
+    if (item >= a.n_items) return;
+    const HullWS<AS_GLOBAL> w =
+        hull_ws_attach(hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)item * a.gws_stride, a.VC, a.NP), &sb);
+    hull_item<PPL, AS_GLOBAL, BLOCK_NW>(a, mode, item, w, &sb);
 }
 
 // Per-cloud AABB -> quantization frame (for the chunked path).
@@ -211,22 +260,18 @@ __global__ __launch_bounds__(256) void cloud_frame_kernel(const float* pts, cons
 
 using KernelFn = void (*)(HullItemArgs, int);
 
-KernelFn pick_kernel(int maxn, int* ppl) {
-    if (maxn <= 64) { *ppl = 1; return hull_wave_kernel<1>; }
-    if (maxn <= 128) { *ppl = 2; return hull_wave_kernel<2>; }
-    if (maxn <= 256) { *ppl = 4; return hull_wave_kernel<4>; }
-    if (maxn <= 512) { *ppl = 8; return hull_wave_kernel<8>; }
-    if (maxn <= 1024) { *ppl = 16; return hull_wave_kernel<16>; }
-    *ppl = 32;
-    return hull_wave_kernel<32>;
-}
-
-int launch_items(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
-    int ppl = 1;
-    KernelFn k = pick_kernel(maxn, &ppl);
-    a.VC = 64 * ppl < HULL_VC_MAX ? 64 * ppl : HULL_VC_MAX;
-    const unsigned lds = hull_ws_bytes(a.VC);
+int launch_wave(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
     if (a.n_items <= 0) return 0;
+    KernelFn k;
+    int ppl;
+    if (maxn <= 64) { ppl = 1; k = hull_wave_kernel<1>; }
+    else if (maxn <= 128) { ppl = 2; k = hull_wave_kernel<2>; }
+    else if (maxn <= 256) { ppl = 4; k = hull_wave_kernel<4>; }
+    else if (maxn <= 512) { ppl = 8; k = hull_wave_kernel<8>; }
+    else { ppl = 16; k = hull_wave_kernel<16>; }
+    a.VC = 64 * ppl;
+    a.NP = 64 * ppl;
+    const unsigned lds = hull_ws_bytes<AS_LDS>(a.VC, a.NP);
     if (lds > 65536u)
         HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k, dim3(a.n_items), dim3(64), lds, ctx->stream, a, mode);
@@ -234,11 +279,31 @@ int launch_items(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
     return 0;
 }
 
+int launch_block(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
+    if (a.n_items <= 0) return 0;
+    KernelFn k;
+    int ppl;
+    constexpr int T = 64 * BLOCK_NW;
+    if (maxn <= 4 * T) { ppl = 4; k = hull_block_kernel<4>; }
+    else if (maxn <= 8 * T) { ppl = 8; k = hull_block_kernel<8>; }
+    else if (maxn <= 16 * T) { ppl = 16; k = hull_block_kernel<16>; }
+    else { ppl = 32; k = hull_block_kernel<32>; }
+    int np = maxn;
+    np = (np + 63) / 64 * 64;
+    a.NP = np;
+    a.VC = np < 32704 ? np : 32704;
+    a.gws_stride = (hull_ws_bytes<AS_GLOBAL>(a.VC, a.NP) + 255ull) & ~255ull;
+    a.gws = (char*)stage_buf(ctx, 6, a.gws_stride * (unsigned long long)a.n_items);
+    REQUIRE(a.gws, "hull workspace allocation failed (%llu bytes)", a.gws_stride * (unsigned long long)a.n_items);
+    hipLaunchKernelGGL(k, dim3(a.n_items), dim3(T), 0, ctx->stream, a, mode);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
-// Device-pointer entry: all buffers on the device; stream-ordered.
-// Clouds larger than 2048 points take the chunked divide-and-conquer path,
-// which needs host-side sizing (one synchronisation per merge round).
+// Device-pointer entry: all buffers on the device; stream-ordered. Clouds
+// above BLOCK_MAX points need h_offsets (host sizing of the chunk rounds).
 static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, const int* h_offsets,
                              int n_hulls, int max_pts, int max_hv, int max_ht, float* d_verts, int* d_tris,
                              int* d_nv, int* d_nt, int* d_err, double* d_vol) {
@@ -254,23 +319,20 @@ static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_off
     a.out_nt = d_nt;
     a.out_err = d_err;
     a.out_vol = d_vol;
-    if (max_pts <= 2048) return launch_items(ctx, a, max_pts, 0);
+    a.min_n = 0;
+    a.max_n = WAVE_MAX;
+    int rc = launch_wave(ctx, a, max_pts < WAVE_MAX ? max_pts : WAVE_MAX, 0);
+    if (rc || max_pts <= WAVE_MAX) return rc;
+    a.min_n = WAVE_MAX + 1;
+    a.max_n = BLOCK_MAX;
+    rc = launch_block(ctx, a, max_pts < BLOCK_MAX ? max_pts : BLOCK_MAX, 0);
+    if (rc || max_pts <= BLOCK_MAX) return rc;
 
-    REQUIRE(h_offsets != nullptr, "clouds above 2048 points need host offsets");
-    // ---- chunked divide and conquer for the large clouds ------------------
-    // small clouds first, directly
+    REQUIRE(h_offsets != nullptr, "clouds above %d points need the host entry point", BLOCK_MAX);
+    // ---- divide and conquer for the largest clouds --------------------------
     std::vector<int> big;
-    int max_small = 0;
-    for (int c = 0; c < n_hulls; ++c) {
-        int n = h_offsets[c + 1] - h_offsets[c];
-        if (n > 2048) big.push_back(c);
-        else max_small = n > max_small ? n : max_small;
-    }
-    {
-        // run the direct kernel over all clouds; big ones report capacity and are redone below
-        int rc = launch_items(ctx, a, max_small > 4 ? max_small : 4, 0);
-        if (rc) return rc;
-    }
+    for (int c = 0; c < n_hulls; ++c)
+        if (h_offsets[c + 1] - h_offsets[c] > BLOCK_MAX) big.push_back(c);
     const int nb = (int)big.size();
     int* d_big = (int*)stage_buf(ctx, 4, sizeof(int) * nb);
     QFrame* d_frames = (QFrame*)stage_buf(ctx, 5, sizeof(QFrame) * (size_t)n_hulls);
@@ -280,20 +342,17 @@ static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_off
                        d_frames);
     HIP_TRY(hipGetLastError());
 
-    // current candidate lists per big cloud (indices relative to the cloud)
     std::vector<std::vector<int>> cand(nb);
     for (int b = 0; b < nb; ++b) {
         int n = h_offsets[big[b] + 1] - h_offsets[big[b]];
         cand[b].resize(n);
         for (int i = 0; i < n; ++i) cand[b][i] = i;
     }
-    for (int round = 0; round < 16; ++round) {
-        // chunk every list that is still above one wave's reach
-        std::vector<int> item_cloud, sel, sel_off{0};
-        std::vector<int> owner;  // which big cloud each chunk belongs to
+    for (int round = 0; round < 32; ++round) {
+        std::vector<int> item_cloud, sel, sel_off{0}, owner;
         for (int b = 0; b < nb; ++b) {
             const int n = (int)cand[b].size();
-            if (n <= 2048) continue;
+            if (n <= BLOCK_MAX) continue;
             for (int s = 0; s < n; s += CHUNK) {
                 const int e = s + CHUNK < n ? s + CHUNK : n;
                 item_cloud.push_back(big[b]);
@@ -320,27 +379,34 @@ static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_off
         c.item_cloud = d_ic;
         c.frames = d_frames;
         c.n_items = ni;
+        c.min_n = 0;
+        c.max_n = CHUNK;
         c.out_sel = d_out;
         c.out_cnt = d_out + sel.size();
-        int rc = launch_items(ctx, c, CHUNK, 1);
+        rc = launch_wave(ctx, c, CHUNK, 1);
         if (rc) return rc;
         std::vector<int> out(sel.size() + ni);
         HIP_TRY(hipMemcpyAsync(out.data(), d_out, sizeof(int) * out.size(), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (int b = 0; b < nb; ++b)
-            if ((int)cand[b].size() > 2048) cand[b].clear();
+        std::vector<size_t> before(nb);
+        for (int b = 0; b < nb; ++b) {
+            before[b] = cand[b].size();
+            if ((int)cand[b].size() > BLOCK_MAX) cand[b].clear();
+        }
         for (int i = 0; i < ni; ++i) {
             const int cnt = out[sel.size() + i];
             REQUIRE(cnt >= 0, "hull chunk exceeded capacity");
             std::vector<int>& dst = cand[owner[i]];
             dst.insert(dst.end(), out.begin() + sel_off[i], out.begin() + sel_off[i] + cnt);
         }
+        for (int b = 0; b < nb; ++b)
+            REQUIRE(cand[b].size() <= BLOCK_MAX || cand[b].size() < before[b],
+                    "hull of a %zu-point cloud has more than %d vertices", before[b], BLOCK_MAX);
     }
-    // final hull of each big cloud over its survivors
     std::vector<int> sel, sel_off{0};
     int maxn = 4;
     for (int b = 0; b < nb; ++b) {
-        REQUIRE((int)cand[b].size() <= 2048, "hull divide-and-conquer did not converge");
+        REQUIRE((int)cand[b].size() <= BLOCK_MAX, "hull divide-and-conquer did not converge");
         sel.insert(sel.end(), cand[b].begin(), cand[b].end());
         sel_off.push_back((int)sel.size());
         maxn = (int)cand[b].size() > maxn ? (int)cand[b].size() : maxn;
@@ -350,47 +416,15 @@ static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_off
     REQUIRE(d_sel && d_off, "staging allocation failed");
     HIP_TRY(hipMemcpyAsync(d_sel, sel.data(), sizeof(int) * sel.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(d_off, sel_off.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
-    // outputs are indexed by item; write into per-cloud slots through a temporary
-    // contiguous output then scatter on the host side of this function
-    float* t_verts = (float*)stage_buf(ctx, 6, sizeof(float) * 3ull * max_hv * nb);
-    int* t_misc = (int*)stage_buf(ctx, 7, sizeof(int) * (3ull * max_ht * nb + 3ull * nb) + sizeof(double) * nb + 16);
-    REQUIRE(t_verts && t_misc, "staging allocation failed");
-    int* t_tris = t_misc;
-    int* t_nv = t_tris + 3ll * max_ht * nb;
-    int* t_nt = t_nv + nb;
-    int* t_err = t_nt + nb;
-    double* t_vol = (double*)(((uintptr_t)(t_err + nb) + 15) & ~(uintptr_t)15);
-    HullItemArgs f{};
-    f.pts = d_pts;
-    f.offsets = d_offsets;
+    HullItemArgs f = a;
     f.sel = d_sel;
     f.sel_off = d_off;
     f.item_cloud = d_big;
     f.frames = d_frames;
     f.n_items = nb;
-    f.max_hv = max_hv;
-    f.max_ht = max_ht;
-    f.out_verts = t_verts;
-    f.out_tris = t_tris;
-    f.out_nv = t_nv;
-    f.out_nt = t_nt;
-    f.out_err = t_err;
-    f.out_vol = d_vol ? t_vol : nullptr;
-    int rc = launch_items(ctx, f, maxn, 0);
-    if (rc) return rc;
-    for (int b = 0; b < nb; ++b) {
-        const int c = big[b];
-        HIP_TRY(hipMemcpyAsync(d_verts + 3ll * max_hv * c, t_verts + 3ll * max_hv * b, sizeof(float) * 3ull * max_hv,
-                               hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(d_tris + 3ll * max_ht * c, t_tris + 3ll * max_ht * b, sizeof(int) * 3ull * max_ht,
-                               hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(d_nv + c, t_nv + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(d_nt + c, t_nt + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(d_err + c, t_err + b, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-        if (d_vol)
-            HIP_TRY(hipMemcpyAsync(d_vol + c, t_vol + b, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    return 0;
+    f.min_n = 0;
+    f.max_n = BLOCK_MAX;
+    return launch_block(ctx, f, maxn, 0);
 }
 
 extern "C" {
@@ -399,7 +433,8 @@ int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, 
                        int max_hv, int max_ht, float* d_verts, int* d_tris, int* d_nv, int* d_nt, int* d_err,
                        double* d_vol) {
     HIP_TRY(hipSetDevice(ctx->device));
-    REQUIRE(max_pts <= 2048, "gpu_hull_dandc_dev: clouds above 2048 points need the host entry point");
+    REQUIRE(max_pts <= BLOCK_MAX, "gpu_hull_dandc_dev: clouds above %d points need the host entry point",
+            BLOCK_MAX);
     return hull_batch_device(ctx, d_pts, d_offsets, nullptr, n_hulls, max_pts, max_hv, max_ht, d_verts, d_tris,
                              d_nv, d_nt, d_err, d_vol);
 }
@@ -423,13 +458,9 @@ int gpu_hull_dandc_vol(gpu_ctx_t ctx, const float* pts, int total_pts, const int
     const size_t b_v = sizeof(float) * 3ull * (size_t)max_hv * n_hulls;
     const size_t b_t = sizeof(int) * 3ull * (size_t)max_ht * n_hulls;
     const size_t b_i = sizeof(int) * (size_t)n_hulls;
-    // one allocation for inputs, one for outputs (grow-only, reused across calls)
-    static const size_t A = 256;
-    auto al = [](size_t x) { return (x + A - 1) / A * A; };
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t in_bytes = al(b_pts) + al(b_off);
     const size_t out_bytes = al(b_v) + al(b_t) + 3 * al(b_i) + al(sizeof(double) * n_hulls);
-    // slots 0-3 are used by the chunked path; inputs/outputs of this call use
-    // dedicated allocations so the chunked path can still stage its lists
     char* d_inbuf = nullptr;
     char* d_outbuf = nullptr;
     HIP_TRY(hipMallocAsync((void**)&d_inbuf, in_bytes, ctx->stream));

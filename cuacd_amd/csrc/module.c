@@ -389,6 +389,52 @@ static PyObject* m_hull_dandc_dev(PyObject* self, PyObject* args) {
     if (rc) return raise_error(rc);
     Py_RETURN_NONE;
 }
+/* hausdorff_batch(av, avoff, at, atoff, bv, bvoff, bt, btoff, n_pairs, mode, n_samples,
+ *                 out_ab, out_ba, samples_a, samples_b)  (0 for a NULL pointer) */
+static PyObject* m_hausdorff_batch(PyObject* self, PyObject* args) {
+    unsigned long long av, avo, at, ato, bv, bvo, bt, bto, oab, oba, sa, sb;
+    int n, mode, ns;
+    if (!PyArg_ParseTuple(args, "KKKKKKKKiiiKKKK", &av, &avo, &at, &ato, &bv, &bvo, &bt, &bto, &n, &mode, &ns, &oab,
+                          &oba, &sa, &sb))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_hausdorff_batch(g_ctx, P(const float*, av), P(const int*, avo), P(const int*, at),
+                                 P(const int*, ato), P(const float*, bv), P(const int*, bvo), P(const int*, bt),
+                                 P(const int*, bto), n, mode, ns, P(float*, oab), P(float*, oba), P(float*, sa),
+                                 P(float*, sb));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_hausdorff_plan_create(PyObject* self, PyObject* args) {
+    unsigned long long av, at, bv, bt, avo, ato, bvo, bto, oab, oba, sa, sb;
+    int n, mode, ns;
+    if (!PyArg_ParseTuple(args, "KKKKKKKKiiiKKKK", &av, &at, &bv, &bt, &avo, &ato, &bvo, &bto, &n, &mode, &ns, &oab,
+                          &oba, &sa, &sb))
+        return NULL;
+    NEED_CTX();
+    hd_plan_t plan = NULL;
+    int rc = gpu_hausdorff_plan_create(g_ctx, P(const float*, av), P(const int*, at), P(const float*, bv),
+                                       P(const int*, bt), P(const int*, avo), P(const int*, ato),
+                                       P(const int*, bvo), P(const int*, bto), n, mode, ns, P(float*, oab),
+                                       P(float*, oba), P(float*, sa), P(float*, sb), &plan);
+    if (rc) return raise_error(rc);
+    return PyLong_FromUnsignedLongLong((unsigned long long)(uintptr_t)plan);
+}
+static PyObject* m_hausdorff_plan_run(PyObject* self, PyObject* args) {
+    unsigned long long p;
+    if (!PyArg_ParseTuple(args, "K", &p)) return NULL;
+    NEED_CTX();
+    int rc = gpu_hausdorff_plan_run(g_ctx, P(hd_plan_t, p));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_hausdorff_plan_destroy(PyObject* self, PyObject* args) {
+    unsigned long long p;
+    if (!PyArg_ParseTuple(args, "K", &p)) return NULL;
+    gpu_hausdorff_plan_destroy(g_ctx, P(hd_plan_t, p));
+    Py_RETURN_NONE;
+}
+
 static PyObject* m_inject_error(PyObject* self, PyObject* args) {
     int bits;
     if (!PyArg_ParseTuple(args, "i", &bits)) return NULL;
@@ -427,6 +473,10 @@ static PyMethodDef methods[] = {
     {"device_info", m_device_info, METH_NOARGS, "(name, n_cu, hbm_bytes)"},
     {"hull_dandc_dev", m_hull_dandc_dev, METH_VARARGS, "Device-pointer hull batch (stream-ordered)."},
     {"inject_error", m_inject_error, METH_VARARGS, "OR KERR bits into the next error readback."},
+    {"hausdorff_batch", m_hausdorff_batch, METH_VARARGS, "Directed sampled Hausdorff distances for mesh pairs."},
+    {"hausdorff_plan_create", m_hausdorff_plan_create, METH_VARARGS, "Plan a device-resident Hausdorff batch."},
+    {"hausdorff_plan_run", m_hausdorff_plan_run, METH_VARARGS, "Launch a Hausdorff plan (stream-ordered)."},
+    {"hausdorff_plan_destroy", m_hausdorff_plan_destroy, METH_VARARGS, "Free a Hausdorff plan."},
     {NULL, NULL, 0, NULL}};
 
 static int exec_module(PyObject* m) { return PyModule_AddFunctions(m, methods); }

@@ -157,6 +157,29 @@ int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, 
                        int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* d_verts,
                        int* d_tris, int* d_nv, int* d_nt, int* d_errors, double* d_volumes);
 
+/* ---- batched directed Hausdorff (no reference counterpart) -----------------
+ * For pairs (A_i, B_i): out_ab[i] = max over samples p of A_i of the distance
+ * from p to B_i's surface, out_ba[i] the other way round. mode 0 samples like
+ * the reference's Hausdorff (max(int(2000*area), 1000) samples,
+ * cuda/hausdorff.cuh:380-558); mode 1 draws n_samples stratified
+ * area-uniform samples. Triangle indices are relative to each mesh's own
+ * vertex range. Optional sample dumps (mode 1): n_pairs*n_samples*3 floats. */
+int gpu_hausdorff_batch(gpu_ctx_t ctx, const float* a_verts, const int* a_vert_off, const int* a_tris,
+                        const int* a_tri_off, const float* b_verts, const int* b_vert_off, const int* b_tris,
+                        const int* b_tri_off, int n_pairs, int mode, int n_samples, float* out_ab, float* out_ba,
+                        float* out_samples_a, float* out_samples_b);
+
+/* Plans: the same computation on device-resident meshes, prepared once and
+ * launched many times (stream-ordered). Outputs are SQUARED distances. */
+typedef struct hd_plan* hd_plan_t;
+int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_a_verts, const int* d_a_tris, const float* d_b_verts,
+                              const int* d_b_tris, const int* h_a_vert_off, const int* h_a_tri_off,
+                              const int* h_b_vert_off, const int* h_b_tri_off, int n_pairs, int mode,
+                              int n_samples, float* d_out_ab2, float* d_out_ba2, float* d_samples_a,
+                              float* d_samples_b, hd_plan_t* plan);
+int gpu_hausdorff_plan_run(gpu_ctx_t ctx, hd_plan_t plan);
+void gpu_hausdorff_plan_destroy(gpu_ctx_t ctx, hd_plan_t plan);
+
 /* sets KERR_INJECTED in the next kernel error word read back by the library
  * (tests of error propagation; no GPU fault is involved) */
 void gpu_debug_inject_error(gpu_ctx_t ctx, int kerr_bits);

@@ -33,6 +33,12 @@ def lib():
         L.orc_hull_batch.restype = C.c_int
         L.orc_hull_batch_serial.argtypes = [P, P, C.c_int, P, P, P]
         L.orc_hull_batch_serial.restype = C.c_int
+        L.orc_pt_tri_d2.argtypes = [P, P, P, P]
+        L.orc_pt_tri_d2.restype = C.c_double
+        L.orc_points_mesh_d2.argtypes = [P, C.c_int, P, P, C.c_int, P]
+        L.orc_directed_hausdorff.argtypes = [P, C.c_int, P, P, C.c_int]
+        L.orc_directed_hausdorff.restype = C.c_double
+        L.orc_directed_batch.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P]
         _LIB = L
     return _LIB
 
@@ -97,3 +103,96 @@ def hull_batch(pts, offsets, check_general=True, serial=False):
     if bad:
         raise RuntimeError(f"oracle hull failed on {bad} clouds")
     return flags[:len(pts)].astype(bool), nf, vol, gen
+
+
+def pt_tri_d2(p, a, b, c):
+    arr = [np.ascontiguousarray(x, np.float64) for x in (p, a, b, c)]
+    return lib().orc_pt_tri_d2(*[_p(x) for x in arr])
+
+
+def points_mesh_d2(pts, verts, tris):
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    out = np.empty(len(pts), np.float64)
+    lib().orc_points_mesh_d2(_p(pts), len(pts), _p(verts), _p(tris), len(tris), _p(out))
+    return out
+
+
+def directed_hausdorff(pts, verts, tris):
+    """max over pts of the exact distance to the mesh surface (FP64 brute force)."""
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    return lib().orc_directed_hausdorff(_p(pts), len(pts), _p(verts), _p(tris), len(tris))
+
+
+def directed_batch(samples, meshes):
+    """samples: (n_pairs, n_samples, 3) float32; meshes: list of (verts, tris)."""
+    samples = np.ascontiguousarray(samples, np.float32)
+    n = len(meshes)
+    vs = [np.ascontiguousarray(v, np.float32).reshape(-1, 3) for v, _ in meshes]
+    ts = [np.ascontiguousarray(t, np.int32).reshape(-1, 3) for _, t in meshes]
+    voff = np.zeros(n + 1, np.int32)
+    toff = np.zeros(n + 1, np.int32)
+    voff[1:] = np.cumsum([len(v) for v in vs])
+    toff[1:] = np.cumsum([len(t) for t in ts])
+    av = np.ascontiguousarray(np.concatenate(vs))
+    at = np.ascontiguousarray(np.concatenate(ts))
+    out = np.empty(n, np.float64)
+    lib().orc_directed_batch(_p(samples), samples.shape[1], _p(av), _p(voff), _p(at), _p(toff), n, _p(out))
+    return out
+
+
+def reference_sampler_counts(verts, tris):
+    """Per-triangle sample counts of the reference Hausdorff sampler
+    (cuda/hausdorff.cuh:380-382, 432-466), evaluated in float32."""
+    v = np.asarray(verts, np.float32)
+    t = np.asarray(tris, np.int64)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    n = np.cross(b - a, c - a).astype(np.float32)
+    area_t = (np.float32(0.5) * np.sqrt((n * n).sum(1, dtype=np.float32))).astype(np.float32)
+    area = np.float32(area_t.sum(dtype=np.float32))
+    res = max(int(np.float32(2000.0) * area), 1000)
+    nt = len(t)
+    step = nt // res if nt > res else 2
+    dens = np.float32(res) / area if area > 1e-20 else np.float32(0)
+    ac = (dens * area_t).astype(np.float32).astype(np.int64)
+    idx = np.arange(nt)
+    return np.where(ac > 0, ac, (idx % step == 0).astype(np.int64)), res
+
+
+def _wang(s):
+    s = np.asarray(s, np.uint64) & np.uint64(0xFFFFFFFF)
+    s = (s ^ np.uint64(61)) ^ (s >> np.uint64(16))
+    s = (s * np.uint64(9)) & np.uint64(0xFFFFFFFF)
+    s = s ^ (s >> np.uint64(4))
+    s = (s * np.uint64(0x27D4EB2D)) & np.uint64(0xFFFFFFFF)
+    s = s ^ (s >> np.uint64(15))
+    return s
+
+
+def reference_samples(verts, tris):
+    """Sample points of the reference Hausdorff sampler (cuda/hausdorff.cuh:516-558):
+    triangle t gets counts[t] points, point k from Wang hashes of t*65537+k,
+    barycentric (1-sqrt(ra), sqrt(ra)(1-rb), sqrt(ra) rb)."""
+    v = np.asarray(verts, np.float32)
+    t = np.asarray(tris, np.int64)
+    counts, _ = reference_sampler_counts(v, t)
+    tri = np.repeat(np.arange(len(t)), counts)
+    k = np.concatenate([np.arange(c) for c in counts]) if len(counts) else np.zeros(0, np.int64)
+    ha = _wang(tri.astype(np.uint64) * np.uint64(65537) + k.astype(np.uint64))
+    hb = _wang(ha + np.uint64(0x9E3779B9))
+    ra = (ha & np.uint64(0xFFFFFF)).astype(np.float32) / np.float32(0xFFFFFF)
+    rb = (hb & np.uint64(0xFFFFFF)).astype(np.float32) / np.float32(0xFFFFFF)
+    sqa = np.sqrt(ra)
+    a, b, c = v[t[tri, 0]], v[t[tri, 1]], v[t[tri, 2]]
+    w = np.stack([1 - sqa, sqa * (1 - rb), sqa * rb], 1).astype(np.float32)
+    return (w[:, :1] * a + w[:, 1:2] * b + w[:, 2:3] * c).astype(np.float32)
+
+
+def bidirectional_hausdorff_reference(va, ta, vb, tb):
+    """sqrt(max(dA->B, dB->A)) with reference-sampler points and exact distances."""
+    sa = reference_samples(va, ta)
+    sb = reference_samples(vb, tb)
+    return max(directed_hausdorff(sa, vb, tb), directed_hausdorff(sb, va, ta))

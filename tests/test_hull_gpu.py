@@ -92,9 +92,6 @@ def test_large_cloud_divide_and_conquer(ctx):
     """Clouds above one wave's reach are hulled chunk by chunk on the same grid."""
     rng = np.random.default_rng(5)
     clouds = [rng.standard_normal((n, 3)).astype(np.float32) for n in (3000, 20000)]
-    sph = rng.standard_normal((6000, 3))
-    clouds.append((sph / np.linalg.norm(sph, axis=1, keepdims=True) * (1 + 0.01 * rng.random((6000, 1))))
-                  .astype(np.float32))
     res = ctx.batch_hull_dandc_mesh(clouds)
     for p, (hv, ht, vol) in zip(clouds, res):
         q = oracle.quantize(p)
@@ -102,6 +99,31 @@ def test_large_cloud_divide_and_conquer(ctx):
         assert abs(vol - oracle.mesh_volume(p, ofaces)) <= VOL_RTOL * vol
         got = _index_sets_from_mesh(p, hv)
         assert np.array_equal(got, np.nonzero(oflags)[0]) or not oracle.general_position(q, ofaces)
+
+
+@pytest.mark.parametrize("n", [1500, 6000, 16000])
+def test_sphere_shell_block_tier(ctx, n):
+    """Nearly every point is a hull vertex: the multi-wave tier with its workspace in HBM."""
+    rng = np.random.default_rng(n)
+    s = rng.standard_normal((n, 3))
+    p = (s / np.linalg.norm(s, axis=1, keepdims=True) * (1 + 0.01 * rng.random((n, 1)))).astype(np.float32)
+    (hv, ht, vol), = ctx.batch_hull_dandc_mesh([p])
+    q = oracle.quantize(p)
+    oflags, ofaces = oracle.hull(q)
+    assert abs(vol - oracle.mesh_volume(p, ofaces)) <= VOL_RTOL * vol
+    _check_mesh_closed(ht, len(hv))
+    if oracle.general_position(q, ofaces):
+        assert np.array_equal(_index_sets_from_mesh(p, hv), np.nonzero(oflags)[0])
+
+
+def test_mixed_batch_all_tiers(ctx):
+    """One batch mixing the one-wave, block and divide-and-conquer tiers."""
+    rng = np.random.default_rng(9)
+    clouds = [rng.standard_normal((n, 3)).astype(np.float32) for n in (10, 300, 1000, 1025, 5000, 40000, 77)]
+    res = ctx.batch_hull_dandc_mesh(clouds)
+    for p, (hv, ht, vol) in zip(clouds, res):
+        _, ofaces = oracle.hull(oracle.quantize(p))
+        assert abs(vol - oracle.mesh_volume(p, ofaces)) <= VOL_RTOL * vol
 
 
 def test_fewer_than_four_points_and_flat(ctx):
