@@ -281,3 +281,63 @@ class Context:
         _gpu.batch_mesh_volume(allv.ctypes.data, allv.shape[0], allt.ctypes.data, allt.shape[0],
                                toff.ctypes.data, voff.ctypes.data, n, out.ctypes.data)
         return out
+
+    # ---- plane cut ----------------------------------------------------------------
+    def plane_cut(self, verts, tris, plane):
+        """Cut a closed mesh by plane (a, b, c, d): a x + b y + c z + d = 0.
+
+        Returns (pos_verts, pos_tris, neg_verts, neg_tris): both halves closed by
+        cap triangles; pos is the side a x + b y + c z + d >= 0 (reference
+        _gpu.test_plane_cut, csrc/module.c:468-520; tests/test_plane_cut.py:35-61).
+        A cut whose cap cannot be built returns the whole mesh on the positive
+        side and an empty negative side, as the reference does.
+        """
+        v, t = _f32(verts).reshape(-1, 3), _i32(tris).reshape(-1, 3)
+        nv, nt = len(v), len(t)
+        # output bound: every triangle splits into at most 3 pieces, plus caps
+        cap_v = nv + 3 * nt + 16
+        cap_t = 3 * nt + 2 * cap_v + 16
+        bufs = [np.empty((cap_v, 3), np.float32), np.empty((cap_t, 3), np.int32),
+                np.empty((cap_v, 3), np.float32), np.empty((cap_t, 3), np.int32)]
+        pa, pb, pc, pd = (float(x) for x in plane)
+        n_pv, n_pt, n_nv, n_nt = _gpu.test_plane_cut(
+            v.ctypes.data, nv, t.ctypes.data, nt, pa, pb, pc, pd,
+            bufs[0].ctypes.data, cap_v, bufs[1].ctypes.data, cap_t,
+            bufs[2].ctypes.data, cap_v, bufs[3].ctypes.data, cap_t)
+        if max(n_pv, n_nv) > cap_v or max(n_pt, n_nt) > cap_t:
+            raise RuntimeError("plane_cut: output exceeds the size bound")
+        return bufs[0][:n_pv].copy(), bufs[1][:n_pt].copy(), bufs[2][:n_nv].copy(), bufs[3][:n_nt].copy()
+
+    def batch_plane_cut(self, meshes, mesh_of, planes):
+        """Cut many (mesh, plane) pairs in one launch (one workgroup per pair).
+
+        meshes: list of (verts, tris); mesh_of[i] names the mesh of cut i and
+        planes[i] = (a, b, c, d) its plane. Returns (vol_pos, vol_neg, sizes,
+        status): FP64 volumes of both closed halves, int32 [n, 4] sizes
+        (pos nv, pos nt, neg nv, neg nt) and int32 status (0 cut, 1 cap failed
+        and the mesh was returned whole on the positive side, 2 one side empty).
+        """
+        mesh_of = _i32(mesh_of).reshape(-1)
+        pl = _f32(planes).reshape(-1, 4)
+        n = len(mesh_of)
+        if len(pl) != n:
+            raise ValueError("mesh_of and planes must have the same length")
+        if n and (mesh_of.min() < 0 or mesh_of.max() >= len(meshes)):
+            raise ValueError("mesh_of index out of range")
+        vs = [_f32(v).reshape(-1, 3) for v, _ in meshes]
+        ts = [_i32(t).reshape(-1, 3) for _, t in meshes]
+        voff = np.zeros(len(meshes) + 1, np.int32)
+        toff = np.zeros(len(meshes) + 1, np.int32)
+        voff[1:] = np.cumsum([len(x) for x in vs])
+        toff[1:] = np.cumsum([len(x) for x in ts])
+        allv = np.ascontiguousarray(np.concatenate(vs)) if vs else np.zeros((0, 3), np.float32)
+        allt = np.ascontiguousarray(np.concatenate(ts)) if ts else np.zeros((0, 3), np.int32)
+        vp = np.zeros(n, np.float64)
+        vn = np.zeros(n, np.float64)
+        sz = np.zeros((n, 4), np.int32)
+        st = np.zeros(n, np.int32)
+        if n:
+            _gpu.plane_cut_batch(allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
+                                 len(meshes), mesh_of.ctypes.data, pl.ctypes.data, n,
+                                 vp.ctypes.data, vn.ctypes.data, sz.ctypes.data, st.ctypes.data)
+        return vp, vn, sz, st

@@ -41,7 +41,7 @@
 #define KERR_COUNT    18
 #define KERR_ALL_BITS ((1 << KERR_COUNT) - 1)
 
-#ifndef __HIP_DEVICE_COMPILE__
+/* host-only helper (no __device__ attribute), visible to both compile passes */
 #include <stdio.h>
 /* Decode an error word into "GPU error 0x.. [name, name]". Returns chars written. */
 static inline int kerr_decode(int err, char* buf, int bufsize) {
@@ -80,6 +80,5 @@ static inline int kerr_decode(int err, char* buf, int bufsize) {
     if (off < bufsize) off += snprintf(buf + off, (size_t)(bufsize - off), "]");
     return off;
 }
-#endif
 
 #endif /* CUACD_KERR_H */

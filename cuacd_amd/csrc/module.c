@@ -177,6 +177,19 @@ static PyObject* m_test_plane_cut(PyObject* self, PyObject* args) {
     return Py_BuildValue("iiii", npv, npt, nnv, nnt);
 }
 
+static PyObject* m_plane_cut_batch(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, mo, pl, ovp, ovn, osz, ost;
+    int nm, nc;
+    if (!PyArg_ParseTuple(args, "KKKKiKKiKKKK", &v, &vo, &t, &to, &nm, &mo, &pl, &nc, &ovp, &ovn, &osz, &ost))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_plane_cut_batch(g_ctx, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
+                                 nm, P(const int*, mo), P(const float*, pl), nc, P(double*, ovp), P(double*, ovn),
+                                 P(int*, osz), P(int*, ost));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
 static PyObject* m_test_postprocess_dc(PyObject* self, PyObject* args) {
     unsigned long long vp, tp, ovp, otp, onvp, ontp;
     int nv, nt, mc, mv, mt;
@@ -456,6 +469,7 @@ static PyMethodDef methods[] = {
     {"test_mesh_volume", m_test_mesh_volume, METH_VARARGS, "Volume of one mesh."},
     {"batch_mesh_volume", m_batch_mesh_volume, METH_VARARGS, "Volumes of a batch of meshes."},
     {"test_plane_cut", m_test_plane_cut, METH_VARARGS, "Plane cut with cap triangulation."},
+    {"plane_cut_batch", m_plane_cut_batch, METH_VARARGS, "Batched plane cuts: volumes, sizes, status."},
     {"kdop_hull", m_kdop_hull, METH_VARARGS, "k-DOP prefiltered exact hull."},
     {"test_hausdorff", m_test_hausdorff, METH_VARARGS, "Bidirectional sampled Hausdorff distance."},
     {"test_postprocess_dc", m_test_postprocess_dc, METH_VARARGS, "Split a mesh into connected components."},

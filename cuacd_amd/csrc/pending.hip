@@ -12,12 +12,6 @@ int gpu_kdop_hull(gpu_ctx_t ctx, const float*, int, const int*, int, int, int, f
     return -1;
 }
 
-int gpu_test_plane_cut(gpu_ctx_t ctx, const float*, int, const int*, int, float, float, float, float, float*, int,
-                       int*, int, float*, int, int*, int, int*, int*, int*, int*) {
-    cuacd::set_error(ctx, "test_plane_cut: not built yet");
-    return -1;
-}
-
 int gpu_mesh_audit(gpu_ctx_t ctx, const float*, int, const int*, int, unsigned int*) {
     cuacd::set_error(ctx, "mesh_audit: not built yet");
     return -1;

@@ -95,3 +95,75 @@ def l_shape():
         a, b = ring[i], ring[(i + 1) % 6]
         sides += [(a, b, b + 6), (a, b + 6, a + 6)]
     return v, np.array(bottom + top + sides, np.int32)
+
+
+def voxel_solid(occ, origin=(0.0, 0.0, 0.0), h=1.0):
+    """Closed surface of a union of voxels: occ is a bool [nx, ny, nz] grid of
+    cells of edge h starting at origin. Every face between a filled and an
+    empty cell becomes two outward triangles on the shared lattice vertices,
+    so the result is watertight (2-manifold when no two filled cells touch
+    only along an edge or a corner)."""
+    occ = np.asarray(occ, bool)
+    nx, ny, nz = occ.shape
+    pad = np.zeros((nx + 2, ny + 2, nz + 2), bool)
+    pad[1:-1, 1:-1, 1:-1] = occ
+    vid = {}
+    verts = []
+    tris = []
+
+    def v(i, j, k):
+        key = (i, j, k)
+        if key not in vid:
+            vid[key] = len(verts)
+            verts.append((origin[0] + h * i, origin[1] + h * j, origin[2] + h * k))
+        return vid[key]
+
+    # (axis, direction): the four corners of the face in CCW order seen from outside
+    for i, j, k in zip(*np.nonzero(occ)):
+        for ax in range(3):
+            for sgn in (-1, 1):
+                nb = [i + 1, j + 1, k + 1]
+                nb[ax] += sgn
+                if pad[tuple(nb)]:
+                    continue
+                u, w = (ax + 1) % 3, (ax + 2) % 3
+                base = [i, j, k]
+                if sgn > 0:
+                    base[ax] += 1
+                c = []
+                for du, dw in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                    p = list(base)
+                    p[u] += du
+                    p[w] += dw
+                    c.append(v(*p))
+                if sgn < 0:
+                    c = c[::-1]
+                tris += [(c[0], c[1], c[2]), (c[0], c[2], c[3])]
+    return np.array(verts, np.float32), np.array(tris, np.int32)
+
+
+def hollow_tube():
+    """2x2x2 box centred at the origin minus a 1x1 square through-hole along z
+    (the ring-cap solid of the reference's tests/test_plane_cut.py:115-126).
+    Volume 6."""
+    occ = np.ones((4, 4, 4), bool)
+    occ[1:3, 1:3, :] = False
+    return voxel_solid(occ, (-1.0, -1.0, -1.0), 0.5)
+
+
+def two_holes():
+    """4x2x2 box centred at the origin minus two 0.5x0.5 through-holes along z
+    at x = -1 and x = +1 (reference tests/test_plane_cut.py:128-141). Volume 15."""
+    occ = np.ones((16, 8, 8), bool)
+    occ[3:5, 3:5, :] = False
+    occ[11:13, 3:5, :] = False
+    return voxel_solid(occ, (-2.0, -1.0, -1.0), 0.25)
+
+
+def l_shape_union():
+    """Union of a 1x2x1 and a 1x1x1 box: the L of the reference's tests
+    (tests/test_plane_cut.py:102-113), spanning [0,2]x[0,2]x[0,1]. Volume 3."""
+    occ = np.zeros((2, 2, 1), bool)
+    occ[0, :, 0] = True
+    occ[1, 0, 0] = True
+    return voxel_solid(occ, (0.0, 0.0, 0.0), 1.0)

@@ -169,6 +169,17 @@ int gpu_hausdorff_batch(gpu_ctx_t ctx, const float* a_verts, const int* a_vert_o
                         const int* b_tri_off, int n_pairs, int mode, int n_samples, float* out_ab, float* out_ba,
                         float* out_samples_a, float* out_samples_b);
 
+/* Batched plane cuts (no reference counterpart; the lookahead search cuts
+ * many (mesh, plane) pairs per step, cuda/lookahead.cu:270-420). Meshes are
+ * packed (vert_off/tri_off prefix arrays, n_meshes + 1 entries); cut i cuts
+ * mesh mesh_of[i] by plane planes[4i..4i+3] = (a, b, c, d). Per cut: FP64
+ * volumes of both closed halves, sizes {pos nv, pos nt, neg nv, neg nt} and a
+ * status (0 cut, 1 cap failed - mesh returned whole on the positive side,
+ * 2 one side empty). Host arrays in and out. */
+int gpu_plane_cut_batch(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris,
+                        const int* tri_off, int n_meshes, const int* mesh_of, const float* planes, int n_cuts,
+                        double* out_vol_pos, double* out_vol_neg, int* out_sizes, int* out_status);
+
 /* Plans: the same computation on device-resident meshes, prepared once and
  * launched many times (stream-ordered). Outputs are SQUARED distances. */
 typedef struct hd_plan* hd_plan_t;

@@ -196,3 +196,30 @@ def bidirectional_hausdorff_reference(va, ta, vb, tb):
     sa = reference_samples(va, ta)
     sb = reference_samples(vb, tb)
     return max(directed_hausdorff(sa, vb, tb), directed_hausdorff(sb, va, ta))
+
+
+def halfspace_volumes(verts, tris, plane):
+    """FP64 volumes of the mesh on each side of plane (a, b, c, d): (pos, neg)."""
+    L = lib()
+    if not getattr(L, "_hs", False):
+        L.orc_halfspace_volumes.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, C.c_void_p]
+        L._hs = True
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.int32)
+    out = np.zeros(2, np.float64)
+    L.orc_halfspace_volumes(_p(v), _p(t), len(t), *[float(x) for x in plane], _p(out))
+    return float(out[0]), float(out[1])
+
+
+def edge_manifold_report(tris):
+    """(open directed edges, duplicated directed edges) - 0, 0 for a closed,
+    consistently oriented 2-manifold (audit semantics of cuda/mesh_audit.cuh)."""
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    key = e[:, 0] * (1 << 32) + e[:, 1]
+    twin = e[:, 1] * (1 << 32) + e[:, 0]
+    uk, cnt = np.unique(key, return_counts=True)
+    dup = int((cnt > 1).sum())
+    open_ = int((~np.isin(twin, uk)).sum())
+    return open_, dup

@@ -1,0 +1,190 @@
+"""GPU tests of the plane cut with cap triangulation.
+
+Reference behaviour: cuda/plane_cut.cuh:167-1562 and its tests
+(tests/test_plane_cut.py:151-355): both halves closed and consistently
+oriented, volume conserved, ring and multi-hole caps, on-plane vertices and
+faces, disjoint pieces. The bar here is stricter than the reference's 2-5%
+volume tolerance: every half's volume matches the FP64 half-space oracle to
+1e-5 relative (float32 intersection vertices), and each half is a closed
+edge-manifold (every directed edge has exactly one twin).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cuacd_amd import datasets
+
+pytestmark = pytest.mark.gpu
+REL = 1e-5
+
+
+def _signed_volume(v, t):
+    v = v.astype(np.float64)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum()) / 6.0
+
+
+def _check(ctx, v, t, plane, expect=None, closed=(True, True)):
+    pv, pt, nv, nt = ctx.plane_cut(v, t, plane)
+    total = abs(oracle.mesh_volume(v, t))
+    ref_p, ref_n = oracle.halfspace_volumes(v, t, plane)
+    for name, (hv, ht), ref, want_closed in (("pos", (pv, pt), ref_p, closed[0]), ("neg", (nv, nt), ref_n, closed[1])):
+        if len(ht) == 0:
+            assert abs(ref) <= REL * total, (name, ref)
+            continue
+        assert ht.min() >= 0 and ht.max() < len(hv), name
+        if want_closed:
+            assert oracle.edge_manifold_report(ht) == (0, 0), (name, oracle.edge_manifold_report(ht))
+        vol = _signed_volume(hv, ht)
+        assert abs(vol - ref) <= REL * total + 1e-7, (name, vol, ref)  # also fixes the orientation: outward
+    if expect is not None:
+        assert abs(ref_p - expect[0]) < 1e-9 and abs(ref_n - expect[1]) < 1e-9
+    return pv, pt, nv, nt
+
+
+def _centered_box(ex):
+    e = np.asarray(ex, np.float64) / 2
+    return datasets.box_mesh(tuple(-e), tuple(e))
+
+
+@pytest.mark.parametrize("plane", [(0, 0, 1, 0), (1, 0, 0, 0), (0, 1, 0, 0)])
+def test_cube_midpoints(ctx, plane):
+    _check(ctx, *_centered_box((1, 1, 1)), plane, expect=(0.5, 0.5))
+
+
+def test_cube_off_center(ctx):
+    _check(ctx, *_centered_box((1, 1, 1)), (0, 0, 1, -0.25), expect=(0.25, 0.75))
+
+
+@pytest.mark.parametrize("plane,pos", [((1, 0, 0, 0), 12.0), ((0, 1, 0, 0.5), 16.0), ((0, 0, 1, -0.3), 10.2)])
+def test_rectangular_box(ctx, plane, pos):
+    _check(ctx, *_centered_box((2, 3, 4)), plane, expect=(pos, 24 - pos))
+
+
+def test_no_cut(ctx):
+    v, t = _centered_box((1, 1, 1))
+    pv, pt, nv, nt = ctx.plane_cut(v, t, (0, 0, 1, -2))
+    assert len(pt) == 0 and len(nt) == len(t)
+    assert abs(_signed_volume(nv, nt) - 1.0) < 1e-6
+
+
+def test_sphere(ctx):
+    v, t = datasets.icosphere(3)
+    _check(ctx, v.astype(np.float32), t, (0, 0, 1, 0))
+
+
+def test_l_shape(ctx):
+    v, t = datasets.l_shape_union()
+    _check(ctx, v, t, (0, 1, 0, -0.5), expect=(2.0, 1.0))
+    _check(ctx, v, t, (0, 0, 1, -0.5), expect=(1.5, 1.5))  # L-shaped (non-convex) cap
+    _check(ctx, v, t, (1, 1, 0, -1.7))                     # diagonal through the notch
+
+
+def test_hollow_tube_ring_cap(ctx):
+    v, t = datasets.hollow_tube()
+    _check(ctx, v, t, (0, 0, 1, 0), expect=(3.0, 3.0))     # through lattice vertices
+    _check(ctx, v, t, (0, 0, 1, -0.1))                     # ring cap, generic crossings
+    _check(ctx, v, t, (0.1, 0.2, 1, 0.05))                 # tilted ring
+    _check(ctx, v, t, (1, 0, 0, 0))                        # two separate caps
+    _check(ctx, v, t, (1, 0, 0, -0.75), expect=(1.0, 5.0))
+
+
+def test_two_holes(ctx):
+    v, t = datasets.two_holes()
+    _check(ctx, v, t, (0, 0, 1, 0), expect=(7.5, 7.5))
+    _check(ctx, v, t, (0.05, -0.1, 1, -0.3))               # one outer loop, two holes
+
+
+def test_nested_islands(ctx):
+    """A hollow box with a cube floating inside it: the cap through all three
+    has an outer loop, a hole and an island inside the hole."""
+    occ = np.ones((5, 5, 5), bool)
+    occ[1:4, 1:4, 1:4] = False
+    occ[2, 2, 2] = True
+    v, t = datasets.voxel_solid(occ, (0, 0, 0), 1.0)
+    _check(ctx, v, t, (0, 0, 1, -2.5), expect=(49.5, 49.5))
+    _check(ctx, v, t, (0.1, 0.05, 1, -2.4))
+
+
+def test_plane_through_vertex(ctx):
+    _check(ctx, *_centered_box((1, 1, 1)), (1, 1, 1, -1.5))
+
+
+def test_plane_through_face(ctx):
+    """The plane holds the y = -0.5 face: the face goes to the side its normal
+    points to and the rest is capped on the positive side."""
+    v, t = _centered_box((1, 1, 1))
+    pv, pt, nv, nt = ctx.plane_cut(v, t, (0, 1, 0, 0.5))
+    assert oracle.edge_manifold_report(pt) == (0, 0)
+    assert abs(_signed_volume(pv, pt) - 1.0) < 1e-6
+    assert abs(_signed_volume(nv, nt)) < 1e-6
+
+
+def test_diagonal_plane(ctx):
+    n = float(np.sqrt(0.5))
+    _check(ctx, *_centered_box((1, 1, 1)), (n, n, 0, 0), expect=(0.5, 0.5))
+
+
+def test_two_disjoint_cubes(ctx):
+    c = _centered_box((0.5, 0.5, 0.5))
+    v = np.vstack([c[0] - [1, 0, 0], c[0] + [1, 0, 0]]).astype(np.float32)
+    t = np.vstack([c[1], c[1] + len(c[0])])
+    _check(ctx, v, t, (0, 0, 1, 0), expect=(0.125, 0.125))
+    pv, pt, nv, nt = ctx.plane_cut(v, t, (1, 0, 0, 0))
+    assert abs(_signed_volume(pv, pt) - 0.125) < 1e-6 and abs(_signed_volume(nv, nt) - 0.125) < 1e-6
+
+
+def test_random_planes_noisy_spheres_batch(ctx):
+    """128 random cuts of 32 star-shaped noisy icospheres in one launch: every
+    half's FP64 volume equals the oracle's; the same cuts one at a time agree."""
+    meshes = datasets.noisy_icospheres(32, 3, 0.15, seed=5)
+    rng = np.random.default_rng(7)
+    n = 128
+    mesh_of = rng.integers(0, len(meshes), n).astype(np.int32)
+    nrm = rng.standard_normal((n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    planes = np.concatenate([nrm, rng.uniform(-0.8, 0.8, (n, 1))], 1).astype(np.float32)
+    vp, vn, sz, st = ctx.batch_plane_cut(meshes, mesh_of, planes)
+    assert (st != 1).all(), np.nonzero(st == 1)
+    for i in range(n):
+        v, t = meshes[mesh_of[i]]
+        rp, rn = oracle.halfspace_volumes(v, t, planes[i])
+        tot = rp + rn
+        assert abs(vp[i] - rp) <= REL * tot and abs(vn[i] - rn) <= REL * tot, (i, vp[i], rp, vn[i], rn)
+    for i in (0, 1, 2, 3):
+        pv, pt, nv, nt = _check(ctx, *meshes[mesh_of[i]], tuple(planes[i]))
+        assert (len(pv), len(pt), len(nv), len(nt)) == tuple(sz[i])
+
+
+def test_random_planes_rotated_solids_batch(ctx):
+    """Random cuts of rotated tube / two-hole / L solids: rings, several holes,
+    several islands."""
+    rng = np.random.default_rng(11)
+    base = [datasets.hollow_tube(), datasets.two_holes(), datasets.l_shape_union()]
+    meshes = []
+    for k, (v, t) in enumerate(base):
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        q = q * np.sign(np.diag(r))
+        if np.linalg.det(q) < 0:
+            q[:, 0] = -q[:, 0]
+        meshes.append(((v.astype(np.float64) @ q.T).astype(np.float32), t))
+    n = 96
+    mesh_of = (np.arange(n) % 3).astype(np.int32)
+    nrm = rng.standard_normal((n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    planes = np.concatenate([nrm, rng.uniform(-0.9, 0.9, (n, 1))], 1).astype(np.float32)
+    vp, vn, sz, st = ctx.batch_plane_cut(meshes, mesh_of, planes)
+    assert (st != 1).all()
+    for i in range(n):
+        rp, rn = oracle.halfspace_volumes(*meshes[mesh_of[i]], planes[i])
+        tot = rp + rn
+        assert abs(vp[i] - rp) <= REL * tot and abs(vn[i] - rn) <= REL * tot, (i, vp[i], rp, vn[i], rn)
+
+
+def test_heap_balanced_after_cuts(ctx):
+    before = ctx.heap_stats()
+    meshes = datasets.noisy_icospheres(4, 2, 0.1, seed=1)
+    ctx.batch_plane_cut(meshes, np.arange(4), np.tile([0, 0, 1, 0], (4, 1)))
+    ctx.plane_cut(*meshes[0], (1, 0, 0, 0.1))
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)
