@@ -23,10 +23,17 @@
 // scans (deterministic triangle order), boundary edges are the on-plane
 // directed edges without a twin (hash set), loops come from pointer jumping,
 // and one wave64 runs nesting, bridging and ear clipping with lane-parallel
-// inner tests. Each side's FP64 volume is computed on the way out.
+// inner tests. Each side's FP64 volume is computed on the way out. Every
+// choice is order-independent of thread timing: new vertices are numbered in
+// (triangle, edge) order of first occurrence and boundary edges are listed in
+// (positive triangle, edge) order, so a cut is reproducible bit for bit.
 #pragma once
 #include "dheap.hpp"
 #include "hausdorff.hpp"
+
+// Geometry below is evaluated exactly as written (no FMA contraction), so
+// that a serial C evaluation of the same expressions gives the same bits.
+#pragma clang fp contract(off)
 
 namespace cuacd {
 
@@ -268,7 +275,7 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
     if (nv <= 0 || nt <= 0) { R.status = 2; return R; }
 
     // ---- stage A: signs, per-triangle crossing and output counts ------------
-    const unsigned long long szA = al16(nv) + 16ull * nt;
+    const unsigned long long szA = al16(nv) + 20ull * nt;
     char* A = (char*)pc_alloc(pool, szA, s, 1, err, 1);
     if (!A) { R.status = -1; return R; }
     int8_t* sg = (int8_t*)A;
@@ -276,6 +283,7 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
     int* toff = tc + nt;               // scratch for scans
     int* toff2 = toff + nt;
     int* toff3 = toff2 + nt;
+    int* tnew = toff3 + nt;            // per tri: first new-vertex slot it owns
     if (tid == 0) { s->cnt[0] = 0; s->cnt[1] = 0; s->fail = 0; }
     for (int i = tid; i < nv; i += PC_T) {
         const float d = pc_side(pa, pb, pc, pd, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
@@ -335,12 +343,16 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
     }
 
     // ---- stage B: intersection vertices via hash, split triangles ---------------
+    // Crossing-edge instances are numbered in (triangle, edge) order; the
+    // lowest instance of an undirected edge owns it, and owners of edges with
+    // both ends off the plane number the new vertices in instance order. The
+    // output is therefore the same on every run and equals the serial order.
     uint32_t H = 16;
     while ((int)H < 2 * n_cross_inst + 2) H <<= 1;
     const int maxnew = n_cross_inst;
     const int nall_cap = nv + maxnew;
-    const unsigned long long szB = 8ull * H + 4ull * H + al16(12ull * maxnew) + al16(4ull * n_cross_inst + 4) +
-                                   12ull * npt + 12ull * nnt + 8ull * nall_cap + 64;
+    const unsigned long long szB = 8ull * H + 8ull * H + al16(12ull * maxnew) + 12ull * npt + 12ull * nnt +
+                                   8ull * nall_cap + 64;
     char* B = (char*)pc_alloc(pool, szB, s, 2, err, 1);
     if (!B) {
         if (tid == 0) heap_free(pool, A, err);
@@ -348,20 +360,20 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
         return R;
     }
     unsigned long long* hk = (unsigned long long*)B;
-    int* hv = (int*)(B + 8ull * H);
-    float* newv = (float*)(B + 12ull * H);
-    int* winners = (int*)((char*)newv + al16(12ull * maxnew));
-    int* ptri = (int*)((char*)winners + al16(4ull * n_cross_inst + 4));
+    int* hmin = (int*)(B + 8ull * H);       // lowest instance of the edge
+    int* hid = hmin + H;                    // vertex id of the edge's crossing point
+    float* newv = (float*)(B + 16ull * H);
+    int* ptri = (int*)((char*)newv + al16(12ull * maxnew));
     int* ntri = ptri + 3 * npt;
     int* remap = ntri + 3 * nnt;  // [nall_cap] per side, reused
     int* remap2 = remap + nall_cap;
-    for (uint32_t i = tid; i < H; i += PC_T) hk[i] = PC_EMPTY;
-    if (tid == 0) { s->cnt[1] = 0; s->cnt[2] = 0; }
+    for (uint32_t i = tid; i < H; i += PC_T) { hk[i] = PC_EMPTY; hmin[i] = 0x7fffffff; }
     __syncthreads();
-    // pass 1: insert undirected crossing edges; winners append their slot
+    // pass 1: insert undirected crossing edges, keep the lowest instance
     for (int k = tid; k < nt; k += PC_T) {
         if (!((tc[k] >> 16) & 0xFF)) continue;
         const int vi[3] = {T[3 * k], T[3 * k + 1], T[3 * k + 2]};
+        int inst = toff[k];
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
             const int a = vi[e], b = vi[e == 2 ? 0 : e + 1];
@@ -369,39 +381,55 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
             const int lo = min(a, b), hi = max(a, b);
             int fresh = 0;
             const int slot = pc_ht_insert(hk, H - 1, ((unsigned long long)lo << 32) | (unsigned)hi, &fresh);
-            if (fresh) winners[atomicAdd(&s->cnt[1], 1)] = slot;
+            atomicMin(&hmin[slot], inst);
+            ++inst;
         }
     }
     __syncthreads();
-    const int nwin = s->cnt[1];
-    // pass 2: each unique crossing edge gets its vertex
-    for (int w = tid; w < nwin; w += PC_T) {
-        const int slot = winners[w];
-        const unsigned long long key = hk[slot];
-        const int a = (int)(key >> 32), b = (int)(key & 0xFFFFFFFFu);
-        int id;
-        if (sg[a] == 0) id = a;
-        else if (sg[b] == 0) id = b;
-        else {
-            const int j = atomicAdd(&s->cnt[2], 1);
-            const float ax = V[3 * a], ay = V[3 * a + 1], az = V[3 * a + 2];
-            const float bx = V[3 * b], by = V[3 * b + 1], bz = V[3 * b + 2];
-            const float d0 = pc_side(pa, pb, pc, pd, ax, ay, az), d1 = pc_side(pa, pb, pc, pd, bx, by, bz);
-            const float t = d0 / (d0 - d1);
-            newv[3 * j] = ax + t * (bx - ax);
-            newv[3 * j + 1] = ay + t * (by - ay);
-            newv[3 * j + 2] = az + t * (bz - az);
-            id = nv + j;
+    // pass 2: owners of off-plane edges get consecutive new-vertex slots
+    auto owned_new = [&](int k, int* first_slot_out) {
+        const int vi[3] = {T[3 * k], T[3 * k + 1], T[3 * k + 2]};
+        int inst = toff[k], c = 0;
+        for (int e = 0; e < 3; ++e) {
+            const int a = vi[e], b = vi[e == 2 ? 0 : e + 1];
+            if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+            const int lo = min(a, b), hi = max(a, b);
+            const int slot = pc_ht_find(hk, H - 1, ((unsigned long long)lo << 32) | (unsigned)hi);
+            if (hmin[slot] == inst) {
+                if (sg[a] != 0 && sg[b] != 0) {
+                    if (first_slot_out) {
+                        const int j = *first_slot_out + c;
+                        const float ax = V[3 * lo], ay = V[3 * lo + 1], az = V[3 * lo + 2];
+                        const float bx = V[3 * hi], by = V[3 * hi + 1], bz = V[3 * hi + 2];
+                        const float d0 = pc_side(pa, pb, pc, pd, ax, ay, az);
+                        const float d1 = pc_side(pa, pb, pc, pd, bx, by, bz);
+                        const float t = d0 / (d0 - d1);
+                        newv[3 * j] = ax + t * (bx - ax);
+                        newv[3 * j + 1] = ay + t * (by - ay);
+                        newv[3 * j + 2] = az + t * (bz - az);
+                        hid[slot] = nv + j;
+                    }
+                    ++c;
+                } else if (first_slot_out) {
+                    hid[slot] = sg[lo] == 0 ? lo : hi;
+                }
+            }
+            ++inst;
         }
-        hv[slot] = id;
-    }
+        return c;
+    };
+    const int n_new = pc_block_scan(nt, tnew, s, [&](int k) { return ((tc[k] >> 16) & 0xFF) ? owned_new(k, nullptr) : 0; });
+    for (int k = tid; k < nt; k += PC_T)
+        if ((tc[k] >> 16) & 0xFF) {
+            int first = tnew[k];
+            owned_new(k, &first);
+        }
     __syncthreads();
-    const int n_new = s->cnt[2];
     const int nall = nv + n_new;
     auto isect = [&](int a, int b) {
         const int lo = min(a, b), hi = max(a, b);
         const int slot = pc_ht_find(hk, H - 1, ((unsigned long long)lo << 32) | (unsigned)hi);
-        return slot >= 0 ? hv[slot] : -1;
+        return slot >= 0 ? hid[slot] : -1;
     };
     // pass 3: split triangles into their precomputed slots
     for (int k = tid; k < nt; k += PC_T) {
@@ -475,7 +503,7 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
         // hole + 2 bridge vertices per hole)
         const unsigned long long E = (unsigned long long)nbe, W = E + 2ull * PC_MAX_LOOPS + 4ull;
         const unsigned long long szC = 8ull * H2 + 8ull * E + 4ull * E * 7 + 4ull * E + 8ull * E +
-                                       W * (4ull + 8ull + 4ull + 4ull + 12ull) + al16(W) + 256ull;
+                                       W * (4ull + 8ull + 4ull + 4ull + 12ull) + al16(W) + 4ull * npt + 256ull;
         Cbuf = (char*)pc_alloc(pool, szC, s, 3, err, 1);
         if (!Cbuf) {
             if (tid == 0) heap_free(pool, B, err);
@@ -499,6 +527,7 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
         int* nxt = prv + W;                    // [W]
         int* capbuf = nxt + W;                 // [3W] cap triangles
         uint8_t* refl = (uint8_t*)(capbuf + 3 * W);  // [W] reflex flags
+        int* boff = (int*)(refl + al16(W));          // [npt] first boundary edge of each pos triangle
         for (uint32_t i = tid; i < H2; i += PC_T) ek[i] = PC_EMPTY;
         if (tid == 0) s->cnt[3] = 0;
         __syncthreads();
@@ -511,18 +540,22 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
                 }
             }
         __syncthreads();
-        // boundary = on-plane directed edge whose twin is absent
-        for (int k = tid; k < npt; k += PC_T)
-            for (int e = 0; e < 3; ++e) {
-                const int a = ptri[3 * k + e], b = ptri[3 * k + (e == 2 ? 0 : e + 1)];
-                if (remap[a] == 3 && remap[b] == 3 &&
-                    pc_ht_find(ek, H2 - 1, ((unsigned long long)b << 32) | (unsigned)a) < 0) {
-                    const int j = atomicAdd(&s->cnt[3], 1);
-                    be[j] = make_int2(a, b);
-                }
-            }
+        // boundary = on-plane directed edge whose twin is absent, listed in
+        // (positive triangle, edge) order
+        auto is_bnd = [&](int k, int e) {
+            const int a = ptri[3 * k + e], b = ptri[3 * k + (e == 2 ? 0 : e + 1)];
+            return remap[a] == 3 && remap[b] == 3 &&
+                   pc_ht_find(ek, H2 - 1, ((unsigned long long)b << 32) | (unsigned)a) < 0;
+        };
+        const int nb = pc_block_scan(npt, boff, s, [&](int k) {
+            return (int)is_bnd(k, 0) + (int)is_bnd(k, 1) + (int)is_bnd(k, 2);
+        });
+        for (int k = tid; k < npt; k += PC_T) {
+            int j = boff[k];
+            for (int e = 0; e < 3; ++e)
+                if (is_bnd(k, e)) be[j++] = make_int2(ptri[3 * k + e], ptri[3 * k + (e == 2 ? 0 : e + 1)]);
+        }
         __syncthreads();
-        const int nb = s->cnt[3];
         // successor of edge j: the boundary edge starting where j ends. Start
         // vertices are unique on a manifold cut; scatter head[start] = edge
         // (atomic min keeps it deterministic), non-manifold touches fixed below.
@@ -649,17 +682,18 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
         // ---- wave 0: orientation, nesting, bridging, ear clipping ------------
         if (tid < 64) {
             const int lane = tid;
-            // signed areas (lane-parallel over vertices of each loop)
-            for (int l = 0; l < nl; ++l) {
-                const int b0 = s->loop_start[l], sz = s->loop_start[l + 1] - b0;
-                float a = 0.0f;
-                for (int i = lane; i < sz; i += 64) {
-                    const P2 p = uv[b0 + i], q = uv[b0 + (i + 1 == sz ? 0 : i + 1)];
-                    a += p.u * q.v - q.u * p.v;
+            // signed areas: one FP64 pass per loop in vertex order on lane 0 (a
+            // fixed summation order keeps orientation and nesting reproducible)
+            if (lane == 0)
+                for (int l = 0; l < nl; ++l) {
+                    const int b0 = s->loop_start[l], sz = s->loop_start[l + 1] - b0;
+                    double a = 0.0;
+                    for (int i = 0; i < sz; ++i) {
+                        const P2 p = uv[b0 + i], q = uv[b0 + (i + 1 == sz ? 0 : i + 1)];
+                        a += (double)p.u * (double)q.v - (double)q.u * (double)p.v;
+                    }
+                    s->loop_area[l] = (float)(a * 0.5);
                 }
-                a = wave_sum_f(a) * 0.5f;
-                if (lane == 0) s->loop_area[l] = a;
-            }
             __threadfence_block();
             // nesting: parent = smallest enclosing loop (ray casting from loop i's first vertex)
             for (int l = 0; l < nl; ++l) {
@@ -784,9 +818,9 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
                                 const bool in = (d1 >= 0.f && d2 >= 0.f && d3 >= 0.f) ||
                                                 (d1 <= 0.f && d2 <= 0.f && d3 <= 0.f);
                                 if (!in) continue;
-                                const float du = X.u - M.u, dv = X.v - M.v;
-                                const float ncos = -du * rsqrtf(du * du + dv * dv);
-                                const uint64_t kk = ((uint64_t)fkey(ncos) << 32) | (uint32_t)i;
+                                // smallest angle to the ray = smallest |dv| / du (du > 0)
+                                const float tang = fabsf(X.v - M.v) / (X.u - M.u);
+                                const uint64_t kk = ((uint64_t)fkey(tang) << 32) | (uint32_t)i;
                                 vk = kk < vk ? kk : vk;
                             }
                             vk = wave_min_u64(vk);

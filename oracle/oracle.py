@@ -223,3 +223,28 @@ def edge_manifold_report(tris):
     dup = int((cnt > 1).sum())
     open_ = int((~np.isin(twin, uk)).sum())
     return open_, dup
+
+
+def plane_cut(verts, tris, plane):
+    """Serial plane cut with caps: (status, pos_v, pos_t, neg_v, neg_t)."""
+    L = lib()
+    if not getattr(L, "_pc", False):
+        L.orc_plane_cut.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int] + [C.c_float] * 4 + \
+            [C.c_void_p] * 4 + [C.c_int, C.c_int, C.c_void_p]
+        L.orc_plane_cut.restype = C.c_int
+        L._pc = True
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    cap_v = len(v) + 3 * len(t) + 16
+    cap_t = 3 * len(t) + 2 * cap_v + 16
+    pv = np.zeros((cap_v, 3), np.float32)
+    pt = np.zeros((cap_t, 3), np.int32)
+    nv_ = np.zeros((cap_v, 3), np.float32)
+    nt_ = np.zeros((cap_t, 3), np.int32)
+    sz = np.zeros(4, np.int32)
+    a, b, c, d = (float(np.float32(x)) for x in plane)
+    st = L.orc_plane_cut(_p(v), len(v), _p(t), len(t), a, b, c, d, _p(pv), _p(pt), _p(nv_), _p(nt_),
+                         cap_v, cap_t, _p(sz))
+    if st < 0:
+        raise RuntimeError("orc_plane_cut: capacity")
+    return st, pv[:sz[0]].copy(), pt[:sz[1]].copy(), nv_[:sz[2]].copy(), nt_[:sz[3]].copy()

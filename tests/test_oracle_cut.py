@@ -91,3 +91,83 @@ def test_manifold_report_detects_defects():
     bad[0] = bad[0][::-1]
     o, d = oracle.edge_manifold_report(bad)
     assert d > 0                                               # flipped triangle
+
+
+# ---- the serial plane cut with caps ------------------------------------------------
+
+def _signed_volume(v, t):
+    v = v.astype(np.float64)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum()) / 6.0
+
+
+def _cut_cases():
+    cube = datasets.box_mesh((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    box = datasets.box_mesh((-1, -1.5, -2), (1, 1.5, 2))
+    occ = np.ones((5, 5, 5), bool)
+    occ[1:4, 1:4, 1:4] = False
+    occ[2, 2, 2] = True
+    nested = datasets.voxel_solid(occ, (0, 0, 0), 1.0)
+    sph = datasets.icosphere(3)
+    return [
+        ("cube_z", cube, (0, 0, 1, 0)), ("cube_off", cube, (0, 0, 1, -0.25)),
+        ("cube_diag", cube, (0.7071, 0.7071, 0, 0)), ("cube_vertex", cube, (1, 1, 1, -1.5)),
+        ("box_y", box, (0, 1, 0, 0.5)), ("box_z", box, (0, 0, 1, -0.3)),
+        ("sphere", (sph[0].astype(np.float32), sph[1]), (0, 0, 1, 0)),
+        ("sphere_tilt", (sph[0].astype(np.float32), sph[1]), (0.3, -0.5, 0.8, 0.1)),
+        ("tube_ring", datasets.hollow_tube(), (0, 0, 1, -0.1)),
+        ("tube_lattice", datasets.hollow_tube(), (0, 0, 1, 0)),
+        ("tube_two_caps", datasets.hollow_tube(), (1, 0, 0, 0)),
+        ("two_holes", datasets.two_holes(), (0.05, -0.1, 1, -0.3)),
+        ("two_holes_lattice", datasets.two_holes(), (0, 0, 1, 0)),
+        ("l_notch", datasets.l_shape_union(), (0, 0, 1, -0.5)),
+        ("l_diag", datasets.l_shape_union(), (1, 1, 0, -1.7)),
+        ("nested", nested, (0.1, 0.05, 1, -2.4)),
+        ("nested_lattice", nested, (0, 0, 1, -2.5)),
+    ]
+
+
+@pytest.mark.parametrize("name,mesh,plane", _cut_cases(), ids=[c[0] for c in _cut_cases()])
+def test_oracle_plane_cut_closed_and_exact_volume(name, mesh, plane):
+    v, t = mesh
+    st, pv, pt, nv, nt = oracle.plane_cut(v, t, plane)
+    assert st in (0, 2), st
+    total = abs(oracle.mesh_volume(v, t))
+    ref_p, ref_n = oracle.halfspace_volumes(v, t, np.float32(plane))
+    for hv, ht, ref in ((pv, pt, ref_p), (nv, nt, ref_n)):
+        if len(ht) == 0:
+            assert abs(ref) < 1e-6 * total
+            continue
+        assert oracle.edge_manifold_report(ht) == (0, 0), name
+        assert abs(_signed_volume(hv, ht) - ref) < 1e-5 * total, (name, _signed_volume(hv, ht), ref)
+
+
+def test_oracle_plane_cut_random_noisy_spheres():
+    meshes = datasets.noisy_icospheres(6, 3, 0.15, seed=21)
+    rng = np.random.default_rng(2)
+    for v, t in meshes:
+        for _ in range(6):
+            n = rng.standard_normal(3)
+            n /= np.linalg.norm(n)
+            plane = np.float32([*n, rng.uniform(-0.7, 0.7)])
+            st, pv, pt, nv, nt = oracle.plane_cut(v, t, plane)
+            assert st in (0, 2)
+            ref_p, ref_n = oracle.halfspace_volumes(v, t, plane)
+            for hv, ht, ref in ((pv, pt, ref_p), (nv, nt, ref_n)):
+                if len(ht):
+                    assert oracle.edge_manifold_report(ht) == (0, 0)
+                    assert abs(_signed_volume(hv, ht) - ref) < 1e-5 * (ref_p + ref_n)
+
+
+def test_oracle_plane_cut_sdf_meshes():
+    from cuacd_amd import sdf
+    for v, t in sdf.sdf_meshes(2, seed=3, res=24):
+        for plane in ((1, 0, 0, 0.013), (0.2, 0.9, -0.3, -0.05)):
+            plane = np.float32(plane)
+            st, pv, pt, nv, nt = oracle.plane_cut(v, t, plane)
+            assert st in (0, 2)
+            ref_p, ref_n = oracle.halfspace_volumes(v, t, plane)
+            for hv, ht, ref in ((pv, pt, ref_p), (nv, nt, ref_n)):
+                if len(ht):
+                    assert oracle.edge_manifold_report(ht) == (0, 0)
+                    assert abs(_signed_volume(hv, ht) - ref) < 1e-5 * (ref_p + ref_n)

@@ -12,6 +12,8 @@ import numpy as np
 import pytest
 
 import oracle
+import sys, os
+sys.path.insert(0, os.path.dirname(__file__))
 from cuacd_amd import datasets
 
 pytestmark = pytest.mark.gpu
@@ -188,3 +190,22 @@ def test_heap_balanced_after_cuts(ctx):
     ctx.plane_cut(*meshes[0], (1, 0, 0, 0.1))
     after = ctx.heap_stats()
     assert after[0] == before[0] and after[3] == before[3], (before, after)
+
+
+def test_bit_exact_against_serial_oracle(ctx):
+    """The GPU cut's output (vertex bits, triangle indices, including the cap
+    triangulation) equals the serial oracle's on every case."""
+    import test_oracle_cut as toc
+    cases = toc._cut_cases()
+    meshes = datasets.noisy_icospheres(4, 3, 0.15, seed=21)
+    rng = np.random.default_rng(2)
+    for k, (v, t) in enumerate(meshes):
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        cases.append((f"noisy{k}", (v, t), (*n, rng.uniform(-0.7, 0.7))))
+    for name, (v, t), plane in cases:
+        plane = np.float32(plane)
+        st, opv, opt, onv, ont = oracle.plane_cut(v, t, plane)
+        pv, pt, nv, nt = ctx.plane_cut(v, t, plane)
+        assert pv.tobytes() == opv.tobytes() and np.array_equal(pt, opt), name
+        assert nv.tobytes() == onv.tobytes() and np.array_equal(nt, ont), name
