@@ -341,3 +341,94 @@ class Context:
                                  len(meshes), mesh_of.ctypes.data, pl.ctypes.data, n,
                                  vp.ctypes.data, vn.ctypes.data, sz.ctypes.data, st.ctypes.data)
         return vp, vn, sz, st
+
+    # ---- lookahead decomposition ------------------------------------------------------
+    def _input_hull(self, verts):
+        (hv, ht, _), = self.batch_hull_dandc_mesh([verts])
+        return _f32(hv).reshape(-1, 3), _i32(ht).reshape(-1, 3)
+
+    def lookahead_decompose(self, verts, tris, *, max_iters=100, width=60, width2=5, threshold=0.05,
+                            depth=2, quick_depth=0, max_n_cutting=16, verbose=0, debug=0,
+                            decompose_components=False, no_decompose_components_per_iter=False,
+                            n_concave_edges=32, concave_eps=0.005, concave_threshold=3.49,
+                            concave_iters=10, merge_hulls=False):
+        """Decompose a mesh into convex parts by lookahead tree search
+        (reference cuacd/__init__.py:367-454). Returns a list of
+        (verts, tris, hull_verts, hull_tris) tuples, one per part.
+
+        The input's convex hull comes from this library's exact GPU hull (the
+        reference computes it with scipy on the host)."""
+        verts = _f32(verts).reshape(-1, 3)
+        tris = _i32(tris).reshape(-1, 3)
+        hv, ht = self._input_hull(verts)
+        if width2 is None:
+            width2 = width
+        total_width = width + 4 * n_concave_edges
+        if total_width >= 512:
+            raise ValueError(f"width + 4*n_concave_edges must be < 512 (got {total_width}); "
+                             f"the evaluation keeps one slot per candidate in a 512-entry table")
+        raw = _gpu.lookahead_decompose(
+            verts.ctypes.data, len(verts), tris.ctypes.data, len(tris),
+            hv.ctypes.data, len(hv), ht.ctypes.data, len(ht),
+            max_iters, width, width2, threshold, depth, quick_depth, max_n_cutting,
+            verbose, debug, decompose_components=int(decompose_components),
+            no_decompose_components_per_iter=int(no_decompose_components_per_iter),
+            n_concave_edges=n_concave_edges, concave_eps=concave_eps,
+            concave_threshold=concave_threshold, concave_iters=concave_iters,
+            merge_hulls=int(merge_hulls))
+        return [_part_tuple(r)[:4] for r in raw]
+
+    def batch_lookahead_decompose(self, meshes, *, max_iters=100, width=60, width2=5, threshold=0.05,
+                                  depth=2, max_n_cutting=16, decompose_components=False,
+                                  no_decompose_components_per_iter=False, n_concave_edges=32,
+                                  concave_eps=0.005, concave_threshold=3.49, concave_iters=10,
+                                  merge_hulls=False, return_info=False):
+        """lookahead_decompose over many meshes at once: one search whose
+        launches span every mesh of the batch; each mesh's result is the same
+        as its own lookahead_decompose call. Returns a list (per mesh) of part
+        lists; with return_info also a list of dicts (iterations, per-part
+        mesh/hull volumes and Hausdorff distance)."""
+        n = len(meshes)
+        if n == 0:
+            return ([], []) if return_info else []
+        vs = [_f32(v).reshape(-1, 3) for v, _ in meshes]
+        ts = [_i32(t).reshape(-1, 3) for _, t in meshes]
+        hulls = self.batch_hull_dandc_mesh(vs)
+        hvs = [_f32(h[0]).reshape(-1, 3) for h in hulls]
+        hts = [_i32(h[1]).reshape(-1, 3) for h in hulls]
+
+        def pack(arrs, dt):
+            off = np.zeros(n + 1, np.int32)
+            off[1:] = np.cumsum([len(a) for a in arrs])
+            return np.ascontiguousarray(np.concatenate(arrs).astype(dt)), off
+
+        v, vo = pack(vs, np.float32)
+        t, to = pack(ts, np.int32)
+        hv, hvo = pack(hvs, np.float32)
+        ht, hto = pack(hts, np.int32)
+        if width + 4 * n_concave_edges >= 512:
+            raise ValueError("width + 4*n_concave_edges must be < 512")
+        iters = np.zeros(n, np.int32)
+        raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
+                                   hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
+                                   max_iters, width, width2, float(threshold), depth, max_n_cutting,
+                                   int(decompose_components), int(not no_decompose_components_per_iter),
+                                   n_concave_edges, float(concave_eps), float(concave_threshold),
+                                   concave_iters, int(merge_hulls), 0, iters.ctypes.data)
+        out = [[_part_tuple(r)[:4] for r in parts] for parts in raw]
+        if not return_info:
+            return out
+        info = [{"iters": int(iters[m]),
+                 "mesh_vol": np.array([r[8] for r in raw[m]], np.float32),
+                 "hull_vol": np.array([r[9] for r in raw[m]], np.float32),
+                 "hausdorff": np.array([r[10] for r in raw[m]], np.float32)} for m in range(n)]
+        return out, info
+
+
+def _part_tuple(r):
+    vb, tb, nv, nt, hvb, htb, hnv, hnt = r[:8]
+    v = np.frombuffer(vb, dtype=np.float32).reshape(nv, 3).copy()
+    t = np.frombuffer(tb, dtype=np.int32).reshape(nt, 3).copy()
+    hv = np.frombuffer(hvb, dtype=np.float32).reshape(hnv, 3).copy()
+    ht = np.frombuffer(htb, dtype=np.int32).reshape(hnt, 3).copy()
+    return (v, t, hv, ht) + tuple(r[8:])

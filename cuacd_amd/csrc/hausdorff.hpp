@@ -337,4 +337,20 @@ __device__ inline float hd_nearest_d2(const HdBvh& b, V3 p) {
     }
 }
 
+// One directed Hausdorff task: samples on the source surface, nearest
+// distances to the target surface.
+struct HdTask {
+    const float* sv;  // source vertices / triangles (samples are drawn here)
+    const int* st;
+    const float* tv;  // target vertices / triangles (distances are measured to it)
+    const int* tt;
+    int snv, snt, tnv, tnt;
+    unsigned long long ws_off;  // byte offset of the task's workspace
+    int mode;                   // 0 reference sampler, 1 stratified K samples
+    int K;
+    float* out_d2;              // max squared nearest distance
+    int* out_ns;                // samples used (may be null)
+    float* out_samples;         // optional sample dump (K*3 floats)
+};
+
 }  // namespace cuacd

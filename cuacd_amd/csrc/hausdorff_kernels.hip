@@ -15,20 +15,6 @@
 
 using namespace cuacd;
 
-struct HdTask {
-    const float* sv;  // source vertices / triangles (samples are drawn here)
-    const int* st;
-    const float* tv;  // target vertices / triangles (distances are measured to it)
-    const int* tt;
-    int snv, snt, tnv, tnt;
-    unsigned long long ws_off;  // byte offset of the task's workspace
-    int mode;                   // 0 reference sampler, 1 stratified K samples
-    int K;
-    float* out_d2;              // max squared nearest distance
-    int* out_ns;                // samples used (may be null)
-    float* out_samples;         // optional sample dump (K*3 floats)
-};
-
 struct hd_plan {
     HdTask* d_tasks;
     char* d_ws;
@@ -211,6 +197,11 @@ int launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
 }
 
 }  // namespace
+
+// Internal entry for other kernels' hosts (lookahead): run a device task table.
+int hd_launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
+    return launch_tasks(ctx, d_tasks, n, d_ws);
+}
 
 extern "C" {
 

@@ -1,0 +1,1155 @@
+// la_kernels.hip - device work of the lookahead search: hulls (with the k-DOP
+// prefilter and bound pruning), level-0 plane cuts, leaf scoring, connected
+// components, concave edges. Each kernel runs over a flat task table; the
+// host (lookahead.hip) builds the tables for all meshes of a batch at once.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+#include "lookahead.hpp"
+#include "plane_cut.hpp"
+
+using namespace cuacd;
+
+namespace {
+
+constexpr int LA_WAVE_MAX = 1024;
+constexpr int LA_BLOCK_NW = 8;
+constexpr int KDOP_AXES_N = 40;
+
+// The 40 k-DOP axes: unit normals of the 80 faces of the level-1 icosphere,
+// one per antipodal pair (reference cuda/kdop_const.cu:9-50 lists the same
+// set). Generated on the host at first use and kept in constant memory.
+__constant__ float c_kdop_axes[KDOP_AXES_N][3];
+
+// ---------------------------------------------------------------------------
+// Hull of one task's point set
+// ---------------------------------------------------------------------------
+template <int PPL, int AS, int NW>
+__device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, HullBlockLDS* sb, DPool* pool,
+                                             int* err) {
+    constexpr int T = 64 * NW;
+    const int tid = threadIdx.x;
+    const float* P = t.filt ? t.filt : t.pts;
+    const int n = t.filt ? t.nf : t.n;
+    if (n < 4) {
+        if (tid == 0) { t.vol = 0.0; t.hnv = t.hnt = 0; t.hv = nullptr; t.ht = nullptr; }
+        return;
+    }
+    float xs[PPL], ys[PPL], zs[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        xs[k] = ys[k] = zs[k] = 0.0f;
+        const int i = k * T + tid;
+        if (i < n) {
+            xs[k] = P[3ll * i];
+            ys[k] = P[3ll * i + 1];
+            zs[k] = P[3ll * i + 2];
+        }
+    }
+    QFrame fr;
+    if (t.use_frame) {
+        fr = t.frame;
+    } else {
+        float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+            if (k * T + tid < n) {
+                mn[0] = fminf(mn[0], xs[k]); mx[0] = fmaxf(mx[0], xs[k]);
+                mn[1] = fminf(mn[1], ys[k]); mx[1] = fmaxf(mx[1], ys[k]);
+                mn[2] = fminf(mn[2], zs[k]); mx[2] = fmaxf(mx[2], zs[k]);
+            }
+        for (int j = 0; j < 3; ++j) {
+            mn[j] = wave_min_f(mn[j]);
+            mx[j] = wave_max_f(mx[j]);
+        }
+        if (NW > 1) {
+            const int wv = tid >> 6;
+            if (lane_id() == 0)
+                for (int j = 0; j < 3; ++j) { sb->bmn[wv][j] = mn[j]; sb->bmx[wv][j] = mx[j]; }
+            __syncthreads();
+            for (int j = 0; j < 3; ++j)
+                for (int q = 0; q < NW; ++q) {
+                    mn[j] = fminf(mn[j], sb->bmn[q][j]);
+                    mx[j] = fmaxf(mx[j], sb->bmx[q][j]);
+                }
+            __syncthreads();
+        }
+        fr = qframe_make(mn, mx);
+    }
+    int px[PPL], py[PPL], pz[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        px[k] = py[k] = pz[k] = 0;
+        if (k * T + tid < n) {
+            const float v[3] = {xs[k], ys[k], zs[k]};
+            px[k] = qcoord(fr, 0, sel3(v, fr.ax[0]));
+            py[k] = qcoord(fr, 1, sel3(v, fr.ax[1]));
+            pz[k] = qcoord(fr, 2, sel3(v, fr.ax[2]));
+        }
+    }
+    HullResult r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
+    if (r.status != HULL_OK) {
+        if (tid == 0) {
+            t.vol = 0.0;
+            t.hnv = t.hnt = 0;
+            t.hv = nullptr;
+            t.ht = nullptr;
+            if (r.status == HULL_OVERFLOW) { t.err = KERR_HULL_CAPACITY; err_or(err, KERR_HULL_CAPACITY); }
+        }
+        return;
+    }
+    auto pos = [&](int p, float* o3) {
+        o3[0] = P[3ll * p];
+        o3[1] = P[3ll * p + 1];
+        o3[2] = P[3ll * p + 2];
+    };
+    const double vol = hull_volume_f64<AS, NW>(w, sb, r, pos);
+    if (tid == 0) t.vol = vol;
+    if (!t.emit) return;
+    const int nv = hull_rank_vertices<AS, NW>(w, sb, r, n);
+    const int nt = hull_count_faces<AS, NW>(w, sb, r);
+    const unsigned long long vb = al16(12ull * nv);
+    if (tid == 0) sb->red[0] = (unsigned long long)heap_alloc(pool, 0, vb + 12ull * nt, err);
+    __syncthreads();
+    float* chunk = (float*)sb->red[0];
+    __syncthreads();
+    if (!chunk) {
+        if (tid == 0) { t.hv = nullptr; t.ht = nullptr; t.hnv = t.hnt = 0; t.err = KERR_HULL_HEAP_OUTPUT; }
+        return;
+    }
+    int* tris = (int*)((char*)chunk + vb);
+    for (int v = tid; v < r.nvert; v += T)
+        if (w.vref[v]) {
+            const int p = w.vpid[v], o = w.vout[v];
+            chunk[3 * o] = P[3ll * p];
+            chunk[3 * o + 1] = P[3ll * p + 1];
+            chunk[3 * o + 2] = P[3ll * p + 2];
+        }
+    hull_write_tris<AS, NW>(w, sb, r, tris);
+    if (tid == 0) { t.hv = chunk; t.ht = tris; t.hnv = nv; t.hnt = nt; }
+}
+
+template <int PPL>
+__global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int* ids, int n, DPool* pool, int* err) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ HullBlockLDS sb;
+    if ((int)blockIdx.x >= n) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
+    la_hull_item<PPL, AS_LDS, 1>(t, w, &sb, pool, err);
+    if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+}
+
+template <int PPL>
+__global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* tasks, const int* ids, int n, char* gws,
+                                                                  unsigned long long stride, int VC, int NP,
+                                                                  DPool* pool, int* err) {
+    __shared__ HullBlockLDS sb;
+    if ((int)blockIdx.x >= n) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const HullWS<AS_GLOBAL> w =
+        hull_ws_attach(hull_ws_carve<AS_GLOBAL>(gws + (unsigned long long)blockIdx.x * stride, VC, NP), &sb);
+    la_hull_item<PPL, AS_GLOBAL, LA_BLOCK_NW>(t, w, &sb, pool, err);
+    if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+}
+
+// k-DOP prefilter (reference cuda/kdop_hull.cuh:116-337): extreme points along
+// the 40 axes -> rough inner hull -> optional bound pruning on its volume ->
+// keep the points strictly outside some rough-hull face, plus the rough hull's
+// vertices. The filtered set keeps the full set's quantization frame, so its
+// exact hull equals the full set's.
+constexpr int KD_T = 256;
+constexpr int KD_NW = KD_T / 64;
+
+__global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const int* ids, int n, int2* res,
+                                                       DPool* pool, int* err) {
+    __shared__ HullBlockLDS sb;
+    __shared__ __align__(16) char lds[20480];
+    __shared__ float ep[2 * KDOP_AXES_N][3];
+    __shared__ float4 planes[2 * 128 + 8];
+    __shared__ unsigned long long kred[KD_NW][16];
+    __shared__ int s_cnt[KD_NW + 2];
+    __shared__ float s_ref[3];
+    if ((int)blockIdx.x >= n) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const float* P = t.pts;
+    const int np = t.n;
+    // full-set bbox -> frame
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = tid; i < np; i += KD_T)
+        for (int j = 0; j < 3; ++j) {
+            const float x = P[3ll * i + j];
+            mn[j] = fminf(mn[j], x);
+            mx[j] = fmaxf(mx[j], x);
+        }
+    for (int j = 0; j < 3; ++j) {
+        mn[j] = wave_min_f(mn[j]);
+        mx[j] = wave_max_f(mx[j]);
+    }
+    if (lane == 0)
+        for (int j = 0; j < 3; ++j) { sb.bmn[wv][j] = mn[j]; sb.bmx[wv][j] = mx[j]; }
+    __syncthreads();
+    for (int j = 0; j < 3; ++j)
+        for (int q = 0; q < KD_NW; ++q) { mn[j] = fminf(mn[j], sb.bmn[q][j]); mx[j] = fmaxf(mx[j], sb.bmx[q][j]); }
+    __syncthreads();
+    const QFrame full = qframe_make(mn, mx);
+    // extremes, 8 axes per pass: max key (value, lowest index), min key (value, lowest index)
+    for (int a0 = 0; a0 < KDOP_AXES_N; a0 += 8) {
+        unsigned long long kmax[8], kmin[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { kmax[q] = 0ull; kmin[q] = ~0ull; }
+        for (int i = tid; i < np; i += KD_T) {
+            const float x = P[3ll * i], y = P[3ll * i + 1], z = P[3ll * i + 2];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float d = c_kdop_axes[a0 + q][0] * x + c_kdop_axes[a0 + q][1] * y + c_kdop_axes[a0 + q][2] * z;
+                const unsigned long long hi = (unsigned long long)fkey(d) << 32;
+                const unsigned long long km = hi | (0xFFFFFFFFu - (uint32_t)i);
+                const unsigned long long kn = hi | (uint32_t)i;
+                kmax[q] = km > kmax[q] ? km : kmax[q];
+                kmin[q] = kn < kmin[q] ? kn : kmin[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            kmax[q] = wave_max_u64(kmax[q]);
+            kmin[q] = wave_min_u64(kmin[q]);
+        }
+        if (lane == 0)
+            for (int q = 0; q < 8; ++q) { kred[wv][q] = kmax[q]; kred[wv][8 + q] = kmin[q]; }
+        __syncthreads();
+        if (tid < 16) {
+            unsigned long long k = kred[0][tid];
+            for (int w2 = 1; w2 < KD_NW; ++w2) {
+                const unsigned long long o = kred[w2][tid];
+                k = tid < 8 ? (o > k ? o : k) : (o < k ? o : k);
+            }
+            const int q = tid & 7, isMin = tid >> 3;
+            const int idx = isMin ? (int)(uint32_t)k : (int)(0xFFFFFFFFu - (uint32_t)k);
+            const int slot = 2 * (a0 + q) + isMin;
+            ep[slot][0] = P[3ll * idx];
+            ep[slot][1] = P[3ll * idx + 1];
+            ep[slot][2] = P[3ll * idx + 2];
+        }
+        __syncthreads();
+    }
+    // rough hull of the 80 extreme points, all waves cooperating (PPL 1 x 256 lanes)
+    constexpr int NE = 2 * KDOP_AXES_N;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 128, 256), &sb);
+    float emn[3] = {1e30f, 1e30f, 1e30f}, emx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < NE; ++i)
+        for (int j = 0; j < 3; ++j) { emn[j] = fminf(emn[j], ep[i][j]); emx[j] = fmaxf(emx[j], ep[i][j]); }
+    const QFrame rf = qframe_make(emn, emx);
+    int px[1] = {0}, py[1] = {0}, pz[1] = {0};
+    if (tid < NE) {
+        const float v[3] = {ep[tid][0], ep[tid][1], ep[tid][2]};
+        px[0] = qcoord(rf, 0, sel3(v, rf.ax[0]));
+        py[0] = qcoord(rf, 1, sel3(v, rf.ax[1]));
+        pz[0] = qcoord(rf, 2, sel3(v, rf.ax[2]));
+    }
+    HullResult r = block_quickhull<1, AS_LDS, KD_NW>(w, &sb, NE, px, py, pz);
+    if (r.status != HULL_OK) {
+        // degenerate extremes (flat set): no prefilter, hull the full set
+        if (tid == 0) { res[blockIdx.x] = make_int2(np, 0); t.use_frame = 1; t.frame = full; t.filt = nullptr; t.nf = 0; }
+        return;
+    }
+    auto epos = [&](int p, float* o3) { o3[0] = ep[p][0]; o3[1] = ep[p][1]; o3[2] = ep[p][2]; };
+    const double rough = hull_volume_f64<AS_LDS, KD_NW>(w, &sb, r, epos);
+    if (t.prune_gap < 1e29) {
+        const double gap = fmax(rough - t.mesh_vol, 0.0);
+        if (gap > t.prune_gap) {
+            if (tid == 0) {
+                t.vol = rough;
+                t.pruned = 1;
+                t.nf = 0;
+                t.filt = nullptr;
+                res[blockIdx.x] = make_int2(0, 1);
+            }
+            return;
+        }
+    }
+    const int nrv = hull_rank_vertices<AS_LDS, KD_NW>(w, &sb, r, NE);
+    // interior reference: centroid of the rough hull's vertices (output order)
+    if (tid == 0) {
+        float rx = 0.f, ry = 0.f, rz = 0.f;
+        for (int p = 0; p < NE; ++p) {
+            // output order = ascending local id among referenced vertices
+            bool used = false;
+            for (int v = 0; v < r.nvert; ++v)
+                if (w.vref[v] && w.vpid[v] == p) { used = true; break; }
+            if (used) { rx += ep[p][0]; ry += ep[p][1]; rz += ep[p][2]; }
+        }
+        const float inv = 1.f / (float)nrv;
+        s_ref[0] = rx * inv; s_ref[1] = ry * inv; s_ref[2] = rz * inv;
+    }
+    __syncthreads();
+    // oriented planes of the live rough faces, in face-id order
+    if (tid == 0) s_cnt[KD_NW] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        int c = 0;
+        for (int f = 0; f < r.ftop; ++f) {
+            if (!w.fn[f].w) continue;
+            const u4v fv = w.fv[f];
+            const int ia = w.vpid[fv.x], ib = w.vpid[fv.y], ic = w.vpid[fv.z];
+            const float ax = ep[ia][0], ay = ep[ia][1], az = ep[ia][2];
+            const float e1x = ep[ib][0] - ax, e1y = ep[ib][1] - ay, e1z = ep[ib][2] - az;
+            const float e2x = ep[ic][0] - ax, e2y = ep[ic][1] - ay, e2z = ep[ic][2] - az;
+            float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+            float d = nx * ax + ny * ay + nz * az;
+            if (nx * s_ref[0] + ny * s_ref[1] + nz * s_ref[2] > d) { nx = -nx; ny = -ny; nz = -nz; d = -d; }
+            planes[c++] = make_float4(nx, ny, nz, d);
+        }
+        s_cnt[KD_NW] = c;
+    }
+    __syncthreads();
+    const int npl = s_cnt[KD_NW];
+    // filtered buffer: the points outside the rough hull, then its vertices
+    if (tid == 0) sb.red[0] = (unsigned long long)heap_alloc(pool, 1, 12ull * (np + NE), err);
+    __syncthreads();
+    float* filt = (float*)sb.red[0];
+    __syncthreads();
+    if (!filt) {
+        if (tid == 0) { t.err = KERR_KDOP_SCRATCH_OOM; res[blockIdx.x] = make_int2(-1, 0); }
+        return;
+    }
+    int base = 0;
+    for (int b = 0; b < np; b += KD_T) {
+        const int i = b + tid;
+        bool keep = false;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (i < np) {
+            x = P[3ll * i]; y = P[3ll * i + 1]; z = P[3ll * i + 2];
+            for (int f = 0; f < npl; ++f) {
+                const float4 pl = planes[f];
+                if (pl.x * x + pl.y * y + pl.z * z > pl.w) { keep = true; break; }
+            }
+        }
+        const uint64_t m = ballot(keep);
+        if (lane == 0) s_cnt[wv] = __popcll(m);
+        __syncthreads();
+        int wbase = base;
+        for (int q = 0; q < wv; ++q) wbase += s_cnt[q];
+        if (keep) {
+            const int o = wbase + mask_rank(m);
+            filt[3 * o] = x; filt[3 * o + 1] = y; filt[3 * o + 2] = z;
+        }
+        for (int q = 0; q < KD_NW; ++q) base += s_cnt[q];
+        __syncthreads();
+    }
+    // rough hull vertices in ascending extreme-slot order
+    if (tid == 0) {
+        int o = base;
+        for (int p = 0; p < NE; ++p) {
+            bool used = false;
+            for (int v = 0; v < r.nvert; ++v)
+                if (w.vref[v] && w.vpid[v] == p) { used = true; break; }
+            if (used) {
+                filt[3 * o] = ep[p][0]; filt[3 * o + 1] = ep[p][1]; filt[3 * o + 2] = ep[p][2];
+                ++o;
+            }
+        }
+        t.filt = filt;
+        t.nf = o;
+        t.frame = full;
+        t.use_frame = 1;
+        res[blockIdx.x] = make_int2(o, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Level-0 cut: both closed halves (kept in heap 0), volumes and boxes
+// ---------------------------------------------------------------------------
+__device__ inline void la_block_bbox(const float* v, int nv, float* lo, float* hi, PcLDS* s, float (*scr)[6]) {
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = threadIdx.x; i < nv; i += PC_T)
+        for (int j = 0; j < 3; ++j) {
+            const float x = v[3 * i + j];
+            mn[j] = fminf(mn[j], x);
+            mx[j] = fmaxf(mx[j], x);
+        }
+    for (int j = 0; j < 3; ++j) { mn[j] = wave_min_f(mn[j]); mx[j] = wave_max_f(mx[j]); }
+    if (lane_id() == 0)
+        for (int j = 0; j < 3; ++j) { scr[threadIdx.x >> 6][j] = mn[j]; scr[threadIdx.x >> 6][3 + j] = mx[j]; }
+    __syncthreads();
+    for (int j = 0; j < 3; ++j) {
+        float a = scr[0][j], b = scr[0][3 + j];
+        for (int q = 1; q < PC_T / 64; ++q) { a = fminf(a, scr[q][j]); b = fmaxf(b, scr[q][3 + j]); }
+        lo[j] = a;
+        hi[j] = b;
+    }
+    __syncthreads();
+    (void)s;
+}
+
+__global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, DPool* pool, int* err) {
+    __shared__ PcLDS s;
+    __shared__ float scr[PC_T / 64][6];
+    if ((int)blockIdx.x >= n) return;
+    LaCutTask& T = tasks[blockIdx.x];
+    PcResult r = plane_cut_block(T.v, T.t, T.nv, T.nt, T.plane[0], T.plane[1], T.plane[2], T.plane[3], pool, err, &s);
+    float plo[3], phi[3], nlo[3], nhi[3];
+    const bool ok = r.status == 0 && r.pos.v && r.neg.v;
+    if (ok) {
+        la_block_bbox(r.pos.v, r.pos.nv, plo, phi, &s, scr);
+        la_block_bbox(r.neg.v, r.neg.nv, nlo, nhi, &s, scr);
+    }
+    if (threadIdx.x == 0) {
+        if (!ok) {
+            heap_free(pool, r.pos.v, err);
+            heap_free(pool, r.neg.v, err);
+            T.pv = T.nvp = nullptr;
+            T.pt = T.ntp = nullptr;
+            T.pnv = T.pnt = T.nnv = T.nnt = 0;
+            T.pvol = T.nvol = 0.0;
+            T.status = r.status == 0 ? 2 : r.status;
+        } else {
+            T.pv = r.pos.v; T.pt = r.pos.t; T.pnv = r.pos.nv; T.pnt = r.pos.nt; T.pvol = r.pos.vol;
+            T.nvp = r.neg.v; T.ntp = r.neg.t; T.nnv = r.neg.nv; T.nnt = r.neg.nt; T.nvol = r.neg.vol;
+            for (int j = 0; j < 3; ++j) { T.plo[j] = plo[j]; T.phi[j] = phi[j]; T.nlo[j] = nlo[j]; T.nhi[j] = nhi[j]; }
+            T.status = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Leaf scoring: half-space volumes and hull point sets of both sides
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int n, DPool* pool, int* err) {
+    __shared__ PcLDS s;
+    __shared__ double dsum[2][PC_T / 64];
+    if ((int)blockIdx.x >= n) return;
+    LaScoreTask& T = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const float* V = T.v;
+    const int* Tr = T.t;
+    const int nv = T.nv, nt = T.nt;
+    const float pa = T.plane[0], pb = T.plane[1], pc = T.plane[2], pd = T.plane[3];
+    // A: signs (int8), side marks (int), crossing-instance offsets, new-point offsets
+    const unsigned long long szA = al16(nv) + 4ull * nv + 8ull * nt + 64;
+    char* A = (char*)pc_alloc(pool, szA, &s, 1, err, 1);
+    if (!A) return;
+    int8_t* sg = (int8_t*)A;
+    int* mark = (int*)(A + al16(nv));
+    int* toff = mark + nv;
+    int* tnew = toff + nt;
+    for (int i = tid; i < nv; i += PC_T) {
+        const float d = pc_side(pa, pb, pc, pd, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+        sg[i] = d > PC_EPS ? 1 : (d < -PC_EPS ? -1 : 0);
+        mark[i] = 0;
+    }
+    __syncthreads();
+    // divergence origin: the point of the plane nearest the coordinate origin
+    const double n2 = (double)pa * pa + (double)pb * pb + (double)pc * pc;
+    const double ox = -(double)pd * pa / n2, oy = -(double)pd * pb / n2, oz = -(double)pd * pc / n2;
+    auto tet = [&](const float* a, const float* b, const float* c) {
+        const double ax = a[0] - ox, ay = a[1] - oy, az = a[2] - oz;
+        const double bx = b[0] - ox, by = b[1] - oy, bz = b[2] - oz;
+        const double cx = c[0] - ox, cy = c[1] - oy, cz = c[2] - oz;
+        return ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    };
+    // crossing point of edge (a, b), computed as the cut computes it
+    auto xpt = [&](int a, int b, float* o) {
+        const int lo = min(a, b), hi = max(a, b);
+        const float ax = V[3 * lo], ay = V[3 * lo + 1], az = V[3 * lo + 2];
+        const float bx = V[3 * hi], by = V[3 * hi + 1], bz = V[3 * hi + 2];
+        const float d0 = pc_side(pa, pb, pc, pd, ax, ay, az), d1 = pc_side(pa, pb, pc, pd, bx, by, bz);
+        const float t = d0 / (d0 - d1);
+        o[0] = ax + t * (bx - ax);
+        o[1] = ay + t * (by - ay);
+        o[2] = az + t * (bz - az);
+    };
+    double vp = 0.0, vn = 0.0;
+    int cntp = 0, cntn = 0;
+    for (int k = tid; k < nt; k += PC_T) {
+        const int i0 = Tr[3 * k], i1 = Tr[3 * k + 1], i2 = Tr[3 * k + 2];
+        const int s0 = sg[i0], s1 = sg[i1], s2 = sg[i2];
+        const float* p0 = V + 3 * i0;
+        const float* p1 = V + 3 * i1;
+        const float* p2 = V + 3 * i2;
+        const int hp = (s0 > 0) | (s1 > 0) | (s2 > 0), hn = (s0 < 0) | (s1 < 0) | (s2 < 0);
+        if (!hp || !hn) {
+            int side;
+            if (hn) side = 2;
+            else if (hp) side = 1;
+            else {
+                const float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
+                const float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
+                const float dt = pa * (e1y * e2z - e1z * e2y) + pb * (e1z * e2x - e1x * e2z) + pc * (e1x * e2y - e1y * e2x);
+                side = dt > 0 ? 1 : 2;
+            }
+            const double v = tet(p0, p1, p2);
+            if (side == 1) { vp += v; ++cntp; } else { vn += v; ++cntn; }
+            atomicOr(&mark[i0], side); atomicOr(&mark[i1], side); atomicOr(&mark[i2], side);
+            continue;
+        }
+        const int vi[3] = {i0, i1, i2}, si[3] = {s0, s1, s2};
+        int on = -1;
+        for (int q = 0; q < 3; ++q) if (si[q] == 0) { on = q; break; }
+        if (on >= 0) {
+            const int ov = vi[on], a = vi[(on + 1) % 3], b = vi[(on + 2) % 3], sa = si[(on + 1) % 3];
+            float x[3];
+            xpt(a, b, x);
+            const float* O = V + 3 * ov;
+            const float* A3 = V + 3 * a;
+            const float* B3 = V + 3 * b;
+            if (sa > 0) { vp += tet(O, A3, x); vn += tet(O, x, B3); }
+            else { vn += tet(O, A3, x); vp += tet(O, x, B3); }
+            ++cntp; ++cntn;
+            atomicOr(&mark[ov], 3);
+            atomicOr(&mark[a], sa > 0 ? 1 : 2);
+            atomicOr(&mark[b], sa > 0 ? 2 : 1);
+        } else {
+            int lone = 0;
+            for (int q = 0; q < 3; ++q)
+                if (si[q] != si[(q + 1) % 3] && si[q] != si[(q + 2) % 3]) { lone = q; break; }
+            const int lv = vi[lone], o1 = vi[(lone + 1) % 3], o2 = vi[(lone + 2) % 3];
+            float x1[3], x2[3];
+            xpt(lv, o1, x1);
+            xpt(lv, o2, x2);
+            const float* L = V + 3 * lv;
+            const float* Q1 = V + 3 * o1;
+            const float* Q2 = V + 3 * o2;
+            const double a1 = tet(L, x1, x2);
+            const double a2 = tet(Q1, x2, x1) + tet(Q1, Q2, x2);
+            const int ls = si[lone] > 0 ? 1 : 2;
+            if (ls == 1) { vp += a1; vn += a2; } else { vn += a1; vp += a2; }
+            ++cntp; ++cntn;
+            atomicOr(&mark[lv], ls);
+            atomicOr(&mark[o1], 3 - ls);
+            atomicOr(&mark[o2], 3 - ls);
+        }
+    }
+    vp = wave_sum_d(vp);
+    vn = wave_sum_d(vn);
+    cntp = wave_sum_i(cntp);
+    cntn = wave_sum_i(cntn);
+    if (lane_id() == 0) {
+        dsum[0][tid >> 6] = vp;
+        dsum[1][tid >> 6] = vn;
+        s.cnt[tid >> 6] = cntp;
+        s.cnt[4 + (tid >> 6)] = cntn;
+    }
+    __syncthreads();
+    double VP = 0.0, VN = 0.0;
+    int totp = 0, totn = 0;
+    for (int q = 0; q < PC_T / 64; ++q) {
+        VP += dsum[0][q];
+        VN += dsum[1][q];
+        totp += s.cnt[q];
+        totn += s.cnt[4 + q];
+    }
+    __syncthreads();
+    // crossing instances and their owners (lowest instance of each undirected edge)
+    auto ncr = [&](int k) {
+        const int a = Tr[3 * k], b = Tr[3 * k + 1], c = Tr[3 * k + 2];
+        return (int)((sg[a] >= 0) != (sg[b] >= 0)) + (int)((sg[b] >= 0) != (sg[c] >= 0)) +
+               (int)((sg[c] >= 0) != (sg[a] >= 0));
+    };
+    const int ncross = pc_block_scan(nt, toff, &s, ncr);
+    uint32_t H = 16;
+    while ((int)H < 2 * ncross + 2) H <<= 1;
+    char* B = (char*)pc_alloc(pool, 12ull * H + 64, &s, 2, err, 1);
+    if (!B) {
+        if (tid == 0) heap_free(pool, A, err);
+        return;
+    }
+    unsigned long long* hk = (unsigned long long*)B;
+    int* hmin = (int*)(B + 8ull * H);
+    for (uint32_t i = tid; i < H; i += PC_T) { hk[i] = PC_EMPTY; hmin[i] = 0x7fffffff; }
+    __syncthreads();
+    for (int k = tid; k < nt; k += PC_T) {
+        int inst = toff[k];
+        for (int e = 0; e < 3; ++e) {
+            const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
+            if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+            int fresh;
+            const int slot = pc_ht_insert(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b), &fresh);
+            atomicMin(&hmin[slot], inst);
+            ++inst;
+        }
+    }
+    __syncthreads();
+    auto owned_new = [&](int k) {
+        int inst = toff[k], c = 0;
+        for (int e = 0; e < 3; ++e) {
+            const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
+            if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+            const int slot = pc_ht_find(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b));
+            if (hmin[slot] == inst && sg[a] != 0 && sg[b] != 0) ++c;
+            ++inst;
+        }
+        return c;
+    };
+    const int n_new = pc_block_scan(nt, tnew, &s, owned_new);
+    // point sets: marked original vertices in index order, then the new points
+    const int nposo = pc_block_scan(nv, nullptr, &s, [&](int i) { return mark[i] & 1; });
+    const int nnego = pc_block_scan(nv, nullptr, &s, [&](int i) { return (mark[i] >> 1) & 1; });
+    float* pp = nullptr;
+    float* npp = nullptr;
+    if (totp > 0 && totn > 0) {
+        pp = (float*)pc_alloc(pool, 12ull * (nposo + n_new) + 16, &s, 3, err, 1);
+        npp = (float*)pc_alloc(pool, 12ull * (nnego + n_new) + 16, &s, 0, err, 1);
+        if (pp && npp) {
+            // originals, stable by index: per-chunk ballot compaction
+            int bp = 0, bn = 0;
+            for (int b0 = 0; b0 < nv; b0 += PC_T) {
+                const int i = b0 + tid;
+                const int m = i < nv ? mark[i] : 0;
+                const uint64_t mp = ballot(m & 1), mn_ = ballot(m & 2);
+                if (lane_id() == 0) { s.cnt[tid >> 6] = __popcll(mp); s.cnt[4 + (tid >> 6)] = __popcll(mn_); }
+                __syncthreads();
+                int op = bp, on_ = bn;
+                for (int q = 0; q < (tid >> 6); ++q) { op += s.cnt[q]; on_ += s.cnt[4 + q]; }
+                if (m & 1) {
+                    const int o = op + mask_rank(mp);
+                    pp[3 * o] = V[3 * i]; pp[3 * o + 1] = V[3 * i + 1]; pp[3 * o + 2] = V[3 * i + 2];
+                }
+                if (m & 2) {
+                    const int o = on_ + mask_rank(mn_);
+                    npp[3 * o] = V[3 * i]; npp[3 * o + 1] = V[3 * i + 1]; npp[3 * o + 2] = V[3 * i + 2];
+                }
+                for (int q = 0; q < PC_T / 64; ++q) { bp += s.cnt[q]; bn += s.cnt[4 + q]; }
+                __syncthreads();
+            }
+            for (int k = tid; k < nt; k += PC_T) {
+                int inst = toff[k], c = 0;
+                for (int e = 0; e < 3; ++e) {
+                    const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
+                    if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+                    const int slot =
+                        pc_ht_find(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b));
+                    if (hmin[slot] == inst && sg[a] != 0 && sg[b] != 0) {
+                        float x[3];
+                        xpt(a, b, x);
+                        const int j = tnew[k] + c;
+                        float* dp = pp + 3ll * (nposo + j);
+                        float* dn = npp + 3ll * (nnego + j);
+                        dp[0] = x[0]; dp[1] = x[1]; dp[2] = x[2];
+                        dn[0] = x[0]; dn[1] = x[1]; dn[2] = x[2];
+                        ++c;
+                    }
+                    ++inst;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        T.pvol = VP / 6.0;
+        T.nvol = VN / 6.0;
+        T.npt = totp;
+        T.nnt = totn;
+        T.ppts = pp;
+        T.npts = npp;
+        T.np_pts = pp ? nposo + n_new : 0;
+        T.nn_pts = npp ? nnego + n_new : 0;
+        heap_free(pool, B, err);
+        heap_free(pool, A, err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Connected components (reference cuda/postprocess.cuh:139-527)
+// ---------------------------------------------------------------------------
+__device__ inline int uf_find(int* par, int x) {
+    for (;;) {
+        const int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+        if (p == x) return x;
+        x = p;
+    }
+}
+__device__ inline void uf_union(int* par, int a, int b) {
+    for (;;) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        const int hi = max(a, b), lo = min(a, b);
+        if (atomicCAS(&par[hi], hi, lo) == hi) return;
+    }
+}
+
+__global__ __launch_bounds__(PC_T) void la_comp_kernel(LaCompTask* tasks, int n, DPool* pool, int* err) {
+    __shared__ PcLDS s;
+    __shared__ double dred[PC_T / 64];
+    __shared__ float scr[PC_T / 64][6];
+    if ((int)blockIdx.x >= n) return;
+    LaCompTask& T = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const float* V = T.v;
+    const int* Tr = T.t;
+    const int nv = T.nv, nt = T.nt;
+    if (nv <= 0 || nt <= 0) {
+        if (tid == 0) T.n_comp = 1;
+        return;
+    }
+    char* A = (char*)pc_alloc(pool, 12ull * nv + 64, &s, 1, err, 1);
+    if (!A) {
+        if (tid == 0) { T.n_comp = 1; err_or(err, KERR_DC_OOM); }
+        return;
+    }
+    int* par = (int*)A;
+    int* cid = par + nv;   // used flag, then component of a root
+    int* lidx = cid + nv;  // local index inside its component
+    for (int i = tid; i < nv; i += PC_T) { par[i] = i; cid[i] = 0; }
+    __syncthreads();
+    for (int k = tid; k < nt; k += PC_T) {
+        const int a = Tr[3 * k], b = Tr[3 * k + 1], c = Tr[3 * k + 2];
+        cid[a] = 1; cid[b] = 1; cid[c] = 1;
+        uf_union(par, a, b);
+        uf_union(par, b, c);
+    }
+    __syncthreads();
+    for (int i = tid; i < nv; i += PC_T) par[i] = uf_find(par, i);
+    __syncthreads();
+    // components numbered by their lowest vertex
+    const int ncomp = pc_block_scan(nv, lidx, &s, [&](int i) { return (cid[i] && par[i] == i) ? 1 : 0; });
+    if (ncomp <= 1) {
+        if (tid == 0) { T.n_comp = 1; heap_free(pool, A, err); }
+        return;
+    }
+    // component id of every used vertex (roots first, then members)
+    for (int i = tid; i < nv; i += PC_T)
+        if (cid[i] && par[i] == i) cid[i] = 1 + min(lidx[i], LA_MAX_COMP - 1);
+    __syncthreads();
+    for (int i = tid; i < nv; i += PC_T)
+        if (cid[i] && par[i] != i) cid[i] = -(1 + min(lidx[par[i]], LA_MAX_COMP - 1));
+    __syncthreads();
+    for (int i = tid; i < nv; i += PC_T) if (cid[i] < 0) cid[i] = -cid[i];
+    __syncthreads();
+    const int nc = min(ncomp, LA_MAX_COMP);
+    for (int c = 0; c < nc; ++c) {
+        const int cv = pc_block_scan(nv, lidx, &s, [&](int i) { return cid[i] == c + 1 ? 1 : 0; });
+        const int ct = pc_block_scan(nt, nullptr, &s, [&](int k) { return cid[Tr[3 * k]] == c + 1 ? 1 : 0; });
+        const unsigned long long vb = al16(12ull * cv);
+        float* chunk = (float*)pc_alloc(pool, vb + 12ull * ct, &s, 2, err, 0);
+        if (!chunk) {
+            if (tid == 0) { T.out[c].v = nullptr; T.out[c].nv = 0; T.out[c].nt = 0; T.out[c].svol = 0.0; }
+            continue;
+        }
+        int* ot = (int*)((char*)chunk + vb);
+        for (int i = tid; i < nv; i += PC_T)
+            if (cid[i] == c + 1) {
+                const int o = lidx[i];
+                chunk[3 * o] = V[3 * i]; chunk[3 * o + 1] = V[3 * i + 1]; chunk[3 * o + 2] = V[3 * i + 2];
+            }
+        __syncthreads();
+        // triangles in input order (ballot compaction per 256-chunk)
+        int base = 0;
+        for (int b0 = 0; b0 < nt; b0 += PC_T) {
+            const int k = b0 + tid;
+            const bool in = k < nt && cid[Tr[3 * k]] == c + 1;
+            const uint64_t m = ballot(in);
+            if (lane_id() == 0) s.cnt[tid >> 6] = __popcll(m);
+            __syncthreads();
+            int o = base;
+            for (int q = 0; q < (tid >> 6); ++q) o += s.cnt[q];
+            if (in) {
+                o += mask_rank(m);
+                ot[3 * o] = lidx[Tr[3 * k]]; ot[3 * o + 1] = lidx[Tr[3 * k + 1]]; ot[3 * o + 2] = lidx[Tr[3 * k + 2]];
+            }
+            for (int q = 0; q < PC_T / 64; ++q) base += s.cnt[q];
+            __syncthreads();
+        }
+        // FP64 signed volume anchored at the component's first vertex
+        double acc = 0.0;
+        const double ax = chunk[0], ay = chunk[1], az = chunk[2];
+        for (int k = tid; k < ct; k += PC_T) {
+            const int a = ot[3 * k], b = ot[3 * k + 1], d = ot[3 * k + 2];
+            const double x0 = chunk[3 * a] - ax, y0 = chunk[3 * a + 1] - ay, z0 = chunk[3 * a + 2] - az;
+            const double x1 = chunk[3 * b] - ax, y1 = chunk[3 * b + 1] - ay, z1 = chunk[3 * b + 2] - az;
+            const double x2 = chunk[3 * d] - ax, y2 = chunk[3 * d + 1] - ay, z2 = chunk[3 * d + 2] - az;
+            acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
+        }
+        acc = wave_sum_d(acc);
+        if (lane_id() == 0) dred[tid >> 6] = acc;
+        float lo[3], hi[3];
+        la_block_bbox(chunk, cv, lo, hi, &s, scr);
+        double sv = 0.0;
+        for (int q = 0; q < PC_T / 64; ++q) sv += dred[q];
+        if (tid == 0) {
+            LaCompOut& o = T.out[c];
+            o.v = chunk; o.t = ot; o.nv = cv; o.nt = ct; o.svol = sv / 6.0;
+            for (int j = 0; j < 3; ++j) { o.lo[j] = lo[j]; o.hi[j] = hi[j]; }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        // drop inner shells (negative signed volume) when inner and outer mix
+        int n_inner = 0;
+        for (int c = 0; c < nc; ++c) n_inner += T.out[c].v && T.out[c].svol < 0.0;
+        int keep = nc;
+        if (n_inner > 0 && n_inner < nc) {
+            keep = 0;
+            for (int c = 0; c < nc; ++c) {
+                if (T.out[c].v && T.out[c].svol < 0.0) {
+                    heap_free(pool, T.out[c].v, err);
+                } else {
+                    if (keep != c) T.out[keep] = T.out[c];
+                    ++keep;
+                }
+            }
+        }
+        T.n_comp = keep;
+        heap_free(pool, A, err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Concave edges (reference cuda/la_concave.cu:48-363)
+// ---------------------------------------------------------------------------
+__device__ inline bool la_unit_normal(const float* V, const int* Tr, int f, float* n) {
+    const int a = Tr[3 * f], b = Tr[3 * f + 1], c = Tr[3 * f + 2];
+    const float e1x = V[3 * b] - V[3 * a], e1y = V[3 * b + 1] - V[3 * a + 1], e1z = V[3 * b + 2] - V[3 * a + 2];
+    const float e2x = V[3 * c] - V[3 * a], e2y = V[3 * c + 1] - V[3 * a + 1], e2z = V[3 * c + 2] - V[3 * a + 2];
+    float x = e1y * e2z - e1z * e2y, y = e1z * e2x - e1x * e2z, z = e1x * e2y - e1y * e2x;
+    const float len = sqrtf(x * x + y * y + z * z);
+    if (len < 1e-12f) return false;
+    n[0] = x / len; n[1] = y / len; n[2] = z / len;
+    return true;
+}
+
+__global__ __launch_bounds__(PC_T) void la_edges_kernel(LaEdgeTask* tasks, int n, float cos_pos, float cos_neg,
+                                                        int pos_mode, int neg_mode, DPool* pool, int* err) {
+    __shared__ PcLDS s;
+    if ((int)blockIdx.x >= n) return;
+    LaEdgeTask& T = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const float* V = T.v;
+    const int* Tr = T.t;
+    const int nt = T.nt;
+    if (T.nv <= 0 || nt < 2) {
+        if (tid == 0) T.count = 0;
+        return;
+    }
+    uint32_t H = 16;
+    while ((int)H < 6 * nt + 2) H <<= 1;
+    char* B = (char*)pc_alloc(pool, 16ull * H + 64, &s, 1, err, 1);
+    if (!B) {
+        if (tid == 0) T.count = 0;
+        return;
+    }
+    unsigned long long* hk = (unsigned long long*)B;
+    int* f0 = (int*)(B + 8ull * H);
+    int* f1 = f0 + H;
+    for (uint32_t i = tid; i < H; i += PC_T) { hk[i] = PC_EMPTY; f0[i] = 0x7fffffff; f1[i] = 0x7fffffff; }
+    if (tid == 0) s.cnt[0] = 0;
+    __syncthreads();
+    auto key = [&](int k, int e) {
+        const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
+        return ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b);
+    };
+    for (int k = tid; k < nt; k += PC_T)
+        for (int e = 0; e < 3; ++e) {
+            int fresh;
+            const int slot = pc_ht_insert(hk, H - 1, key(k, e), &fresh);
+            atomicMin(&f0[slot], k);
+        }
+    __syncthreads();
+    for (int k = tid; k < nt; k += PC_T)
+        for (int e = 0; e < 3; ++e) {
+            const int slot = pc_ht_find(hk, H - 1, key(k, e));
+            if (f0[slot] != k) atomicMin(&f1[slot], k);
+        }
+    __syncthreads();
+    for (int k = tid; k < nt; k += PC_T)
+        for (int e = 0; e < 3; ++e) {
+            const unsigned long long kk = key(k, e);
+            // the edge's first face handles it, once (first matching edge slot of that face)
+            bool first = true;
+            for (int e2 = 0; e2 < e; ++e2) first &= key(k, e2) != kk;
+            if (!first) continue;
+            const int slot = pc_ht_find(hk, H - 1, kk);
+            if (f0[slot] != k || f1[slot] == 0x7fffffff) continue;
+            const int g = f1[slot];
+            float n0[3], n1[3];
+            if (!la_unit_normal(V, Tr, k, n0) || !la_unit_normal(V, Tr, g, n1)) continue;
+            const int v0 = (int)(kk >> 32), v1 = (int)(kk & 0xFFFFFFFFu);
+            float ca = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
+            ca = fmaxf(-1.0f, fminf(1.0f, ca));
+            int vo = -1;
+            for (int q = 0; q < 3; ++q) {
+                const int vv = Tr[3 * k + q];
+                if (vv != v0 && vv != v1) { vo = vv; break; }
+            }
+            if (vo < 0) continue;
+            const float dx = V[3 * vo] - V[3 * v0], dy = V[3 * vo + 1] - V[3 * v0 + 1], dz = V[3 * vo + 2] - V[3 * v0 + 2];
+            const float d = dx * n1[0] + dy * n1[1] + dz * n1[2];
+            // dihedral = pi + acos(ca) if d > 0 else pi - acos(ca); concave iff dihedral > threshold,
+            // decided on the cosine (acos is decreasing) with the host's per-branch constants
+            bool concave;
+            if (d > 0.0f) concave = pos_mode == 1 || (pos_mode == 2 && ca < cos_pos);
+            else concave = neg_mode == 1 || (neg_mode == 2 && ca > cos_neg);
+            if (!concave) continue;
+            const int j = atomicAdd(&s.cnt[0], 1);
+            if (j < T.cap) {
+                LaEdgeRec& r = T.out[j];
+                r.lo = v0; r.hi = v1; r.f0 = k; r.f1 = g;
+                for (int q = 0; q < 3; ++q) {
+                    r.n0[q] = n0[q];
+                    r.n1[q] = n1[q];
+                    r.mid[q] = (V[3 * v0 + q] + V[3 * v1 + q]) * 0.5f;
+                }
+                r.pad = 0;
+            }
+        }
+    __syncthreads();
+    if (tid == 0) {
+        T.count = s.cnt[0];
+        heap_free(pool, B, err);
+    }
+}
+
+__global__ __launch_bounds__(PC_T) void la_bbox_kernel(LaBoxTask* tasks, int n) {
+    __shared__ PcLDS s;
+    __shared__ float scr[PC_T / 64][6];
+    if ((int)blockIdx.x >= n) return;
+    LaBoxTask& T = tasks[blockIdx.x];
+    float lo[3], hi[3];
+    la_block_bbox(T.v, T.nv, lo, hi, &s, scr);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 3; ++j) { T.lo[j] = lo[j]; T.hi[j] = hi[j]; }
+}
+
+__global__ void la_free_kernel(void** ptrs, int n, DPool* pool, int* err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ptrs[i]) heap_free(pool, ptrs[i], err);
+}
+
+// host helpers --------------------------------------------------------------------
+template <class T> T* dev_tmp(gpu_ctx_t ctx, size_t n) {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, n * sizeof(T) + 16, ctx->stream) != hipSuccess) return nullptr;
+    return (T*)p;
+}
+
+int kdop_axes_ready = -1;
+
+int ensure_kdop_axes(gpu_ctx_t ctx) {
+    if (kdop_axes_ready == ctx->device) return 0;
+    // level-1 icosphere (12 vertices + 30 edge midpoints on the unit sphere)
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<std::array<double, 3>> v = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                                            {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    for (auto& p : v) {
+        const double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        for (auto& c : p) c /= l;
+    }
+    const int F[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+                          {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
+                          {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    auto mid = [&](int a, int b) {
+        std::array<double, 3> m{v[a][0] + v[b][0], v[a][1] + v[b][1], v[a][2] + v[b][2]};
+        const double l = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (auto& c : m) c /= l;
+        for (size_t i = 0; i < v.size(); ++i)
+            if (std::fabs(v[i][0] - m[0]) + std::fabs(v[i][1] - m[1]) + std::fabs(v[i][2] - m[2]) < 1e-12) return (int)i;
+        v.push_back(m);
+        return (int)v.size() - 1;
+    };
+    std::vector<std::array<double, 3>> normals;
+    for (auto& f : F) {
+        const int a = f[0], b = f[1], c = f[2];
+        const int ab = mid(a, b), bc = mid(b, c), ca = mid(c, a);
+        const int sub[4][3] = {{a, ab, ca}, {b, bc, ab}, {c, ca, bc}, {ab, bc, ca}};
+        for (auto& s3 : sub) {
+            const auto &p = v[s3[0]], &q = v[s3[1]], &r = v[s3[2]];
+            const double ux = q[0] - p[0], uy = q[1] - p[1], uz = q[2] - p[2];
+            const double wx = r[0] - p[0], wy = r[1] - p[1], wz = r[2] - p[2];
+            std::array<double, 3> nn{uy * wz - uz * wy, uz * wx - ux * wz, ux * wy - uy * wx};
+            const double l = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            for (auto& c2 : nn) c2 /= l;
+            bool dup = false;
+            for (auto& o : normals)
+                if (std::fabs(o[0] + nn[0]) + std::fabs(o[1] + nn[1]) + std::fabs(o[2] + nn[2]) < 1e-9) dup = true;
+            if (!dup) normals.push_back(nn);
+        }
+    }
+    REQUIRE(normals.size() == KDOP_AXES_N, "k-DOP axis construction gave %d axes", (int)normals.size());
+    float ax[KDOP_AXES_N][3];
+    for (int i = 0; i < KDOP_AXES_N; ++i)
+        for (int j = 0; j < 3; ++j) ax[i][j] = (float)normals[i][j];
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_kdop_axes), ax, sizeof(ax), 0, hipMemcpyHostToDevice));
+    kdop_axes_ready = ctx->device;
+    return 0;
+}
+
+using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
+using BlockFn = void (*)(LaHullTask*, const int*, int, char*, unsigned long long, int, int, DPool*, int*);
+
+int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl) {
+    if (ids.empty()) return 0;
+    WaveFn k = ppl == 1 ? la_hull_wave<1> : ppl == 2 ? la_hull_wave<2> : ppl == 4 ? la_hull_wave<4>
+             : ppl == 8 ? la_hull_wave<8> : la_hull_wave<16>;
+    const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
+    if (lds > 65536u)
+        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int* d_ids = dev_tmp<int>(ctx, ids.size());
+    REQUIRE(d_ids, "lookahead: id list allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64), lds, ctx->stream, d_tasks, d_ids, (int)ids.size(),
+                       ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+    return 0;
+}
+
+int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn) {
+    if (ids.empty()) return 0;
+    constexpr int T = 64 * LA_BLOCK_NW;
+    BlockFn k;
+    if (maxn <= 4 * T) k = la_hull_block<4>;
+    else if (maxn <= 8 * T) k = la_hull_block<8>;
+    else if (maxn <= 16 * T) k = la_hull_block<16>;
+    else k = la_hull_block<32>;
+    const int np = (maxn + 63) / 64 * 64;
+    const int vc = np < 32704 ? np : 32704;
+    const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull;
+    // bounded workspace: launch in slices of at most 2048 tasks
+    const size_t slice = 2048;
+    char* gws = nullptr;
+    const size_t nslice = std::min(slice, ids.size());
+    HIP_TRY(hipMallocAsync((void**)&gws, stride * nslice + 256, ctx->stream));
+    int* d_ids = dev_tmp<int>(ctx, ids.size());
+    REQUIRE(d_ids, "lookahead: id list allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+    for (size_t b = 0; b < ids.size(); b += slice) {
+        const int cnt = (int)std::min(slice, ids.size() - b);
+        hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, ctx->stream, d_tasks, d_ids + b, cnt, gws, stride, vc, np,
+                           ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+    HIP_TRY(hipFreeAsync(gws, ctx->stream));
+    return 0;
+}
+
+int ppl_class(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16; }
+
+int launch_by_size(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn) {
+    std::vector<int> cls[5];
+    std::vector<int> big;
+    int bigmax = 0;
+    for (auto& q : idn) {
+        const int n = q.second;
+        if (n <= LA_WAVE_MAX) {
+            const int p = ppl_class(n);
+            cls[p == 1 ? 0 : p == 2 ? 1 : p == 4 ? 2 : p == 8 ? 3 : 4].push_back(q.first);
+        } else {
+            big.push_back(q.first);
+            bigmax = std::max(bigmax, n);
+        }
+    }
+    const int ppls[5] = {1, 2, 4, 8, 16};
+    for (int c = 0; c < 5; ++c)
+        if (int rc = launch_wave_ids(ctx, d_tasks, cls[c], ppls[c])) return rc;
+    if (!big.empty()) {
+        // group the block tier by size class to bound the workspaces
+        std::vector<int> b4, b8, b16, b32;
+        int m4 = 0, m8 = 0, m16 = 0, m32 = 0;
+        std::vector<std::pair<int, int>> bn;
+        for (auto& q : idn) if (q.second > LA_WAVE_MAX) bn.push_back(q);
+        for (auto& q : bn) {
+            const int n = q.second;
+            if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); }
+            else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); }
+            else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); }
+            else { b32.push_back(q.first); m32 = std::max(m32, n); }
+        }
+        if (int rc = launch_block_ids(ctx, d_tasks, b4, m4)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b8, m8)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b16, m16)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b32, m32)) return rc;
+    }
+    return 0;
+}
+
+}  // namespace
+
+namespace cuacd {
+
+// Hulls of n tasks. Host copy h_tasks gives the point counts; sets above the
+// wave tier are prefiltered (k-DOP) first, which needs one readback.
+int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n) {
+    if (n <= 0) return 0;
+    if (int rc = ensure_kdop_axes(ctx)) return rc;
+    std::vector<std::pair<int, int>> small;
+    std::vector<int> big;
+    for (int i = 0; i < n; ++i) {
+        if (h_tasks[i].n <= LA_WAVE_MAX) small.push_back({i, h_tasks[i].n});
+        else big.push_back(i);
+    }
+    if (int rc = launch_by_size(ctx, d_tasks, small)) return rc;
+    if (big.empty()) return 0;
+    REQUIRE(big.size() < (1u << 31), "too many hull tasks");
+    int* d_ids = dev_tmp<int>(ctx, big.size());
+    int2* d_res = dev_tmp<int2>(ctx, big.size());
+    REQUIRE(d_ids && d_res, "lookahead: k-DOP buffers allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_ids, big.data(), 4 * big.size(), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
+                       (int)big.size(), d_res, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    std::vector<int2> res(big.size());
+    HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_res, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<std::pair<int, int>> fin;
+    for (size_t i = 0; i < big.size(); ++i) {
+        if (res[i].y) continue;          // pruned: volume is the rough bound
+        if (res[i].x < 0) continue;      // allocation failure, flagged in the task
+        REQUIRE(res[i].x <= HULL_NP_MAX, "lookahead: filtered hull set of %d points exceeds %d", res[i].x,
+                HULL_NP_MAX);
+        fin.push_back({big[i], res[i].x});
+    }
+    return launch_by_size(ctx, d_tasks, fin);
+}
+
+int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_cut_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_score_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_components(gpu_ctx_t ctx, LaCompTask* d_tasks, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_comp_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_edges(gpu_ctx_t ctx, LaEdgeTask* d_tasks, int n, float cos_pos, float cos_neg, int pos_mode,
+                    int neg_mode) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_edges_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, cos_pos, cos_neg, pos_mode,
+                       neg_mode, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_bbox(gpu_ctx_t ctx, LaBoxTask* d_tasks, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_bbox_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_free_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_ptrs, n, ctx->d_pool,
+                       ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace cuacd

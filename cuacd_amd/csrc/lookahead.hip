@@ -1,0 +1,950 @@
+// lookahead.hip - lookahead_decompose (reference csrc/lookahead.c:144-1124)
+// and its batched form over many meshes.
+//
+// The search is the reference's: each iteration splits the parts into
+// connected components, hulls new parts, measures the Hausdorff distance of
+// the parts whose volume cost is already below the threshold, takes up to
+// max_n_cutting parts whose full cost max(0.3 rv, H) is at or above the
+// threshold, and for each one expands a lookahead tree: level 0 tries `width`
+// axis cuts plus up to 4 planes per sampled concave edge, deeper levels cut
+// the worst part of each item with `width2` axis cuts. A leaf's path cost is
+// the mean of the worst-part rv costs of its levels; the initial cut of the
+// best leaf is applied. Branch-and-bound pruning at the last level, extra
+// leaves for parts too small to cut, and the greedy level-0 fallback follow
+// the reference (cuda/la_expand.cu, cuda/la_refine.cu, cuda/la_lifecycle.cu).
+//
+// What is this library's own: every phase is one launch over all meshes of a
+// batch (a mesh drops out when it converges); the last level scores a cut
+// without building it (exact half-space volumes + the hull point sets of both
+// sides) since its halves are never used again; volumes are FP64 and the
+// costs are rounded to float, the reference's cost precision; new parts are
+// appended in cutting order, so runs are deterministic.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "lookahead.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace cuacd;
+
+namespace {
+
+constexpr int LA_MAX_PARTS = 8;
+constexpr int LA_MAX_LEVELS = 4;
+constexpr int LA_MAX_DECOMP = 16384;
+constexpr float LA_PI_F = 3.14159265358979f;
+
+struct Params {
+    int max_iters, width, width2, depth, quick_depth, max_n_cutting, verbose;
+    float threshold;
+    int dc_final, dc_per_iter, n_concave, concave_iters, merge;
+    float concave_eps, concave_threshold;
+};
+
+struct Part {
+    float* v = nullptr;   // mesh chunk (device)
+    int* t = nullptr;
+    int nv = 0, nt = 0;
+    float* hv = nullptr;  // hull chunk (device) or null
+    int* ht = nullptr;
+    int hnv = 0, hnt = 0;
+    double mvol = 0.0, hvol = 0.0;
+    float haus = 0.0f;
+    int cc = 0;
+    bool own = false;     // mesh chunk belongs to the search (heap-allocated)
+    bool own_hull = false;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
+// rv-only cost (reference cuda/la_common.cuh:30-34), FP64 then rounded to float
+float cost_rv(double hv, double mv) {
+    const double g = hv - mv > 0.0 ? hv - mv : 0.0;
+    return (float)(0.3 * std::cbrt(3.0 / (4.0 * M_PI) * g));
+}
+float cost_full(const Part& p) { return std::max(cost_rv(p.hvol, p.mvol), p.haus); }
+
+struct AxisCut {
+    int axis, slice, cpa;
+};
+// reference cuda/la_axis_cut.cuh:10-19
+AxisCut axis_cut(int width, int k) {
+    const int count = width / 3, rem = width % 3;
+    const int xc = count + (rem > 0), yc = count + (rem > 1);
+    if (k < xc) return {0, k, xc};
+    if (k < xc + yc) return {1, k - xc, yc};
+    return {2, k - xc - yc, count};
+}
+
+// One part inside a search item: a reference to a mesh plus its volumes.
+struct IPart {
+    float* v = nullptr;
+    int* t = nullptr;
+    int nv = 0, nt = 0;
+    double mvol = 0.0, hvol = 0.0;
+    float lo[3], hi[3];
+    int cc = 0;
+    int store = -1;       // index of the owning half in the iteration's store, -1 = decomp part
+};
+struct Item {
+    int mesh = 0, src = 0, cut0 = 0;
+    int np = 0;
+    IPart parts[LA_MAX_PARTS];
+    float lc[LA_MAX_LEVELS] = {0, 0, 0, 0};
+    int nl = 0;
+    bool pruned = false;
+};
+
+float item_path_cost(const Item& it) {
+    float s = 0.0f;
+    for (int l = 0; l < it.nl; ++l) s += it.lc[l];
+    return s / (float)it.nl;
+}
+
+// stable insertion sort by rv cost (reference cuda/la_refine.cu:376-387)
+void sort_item_parts(Item& it) {
+    for (int i = 1; i < it.np; ++i) {
+        IPart key = it.parts[i];
+        const float kc = cost_rv(key.hvol, key.mvol);
+        int j = i - 1;
+        while (j >= 0 && cost_rv(it.parts[j].hvol, it.parts[j].mvol) > kc) {
+            it.parts[j + 1] = it.parts[j];
+            --j;
+        }
+        it.parts[j + 1] = key;
+    }
+}
+
+struct MeshState {
+    std::vector<Part> parts;
+    int cc_counter = 0;
+    int iters = 0;
+    bool done = false;
+    float* in_v = nullptr;  // caller's mesh uploaded (not heap)
+    int* in_t = nullptr;
+    float* in_hv = nullptr;
+    int* in_ht = nullptr;
+};
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    gpu_ctx_t ctx = nullptr;
+    int alloc(gpu_ctx_t c, size_t count) {
+        ctx = c;
+        n = count;
+        if (!count) return 0;
+        return hipMallocAsync((void**)&p, sizeof(T) * count + 16, c->stream) == hipSuccess ? 0 : -1;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFreeAsync(p, ctx->stream);
+    }
+};
+
+#define LA_TRY(x)                                \
+    do {                                         \
+        int _rc = (x);                           \
+        if (_rc) return _rc;                     \
+    } while (0)
+
+struct Engine {
+    gpu_ctx_t ctx;
+    Params P;
+    std::vector<MeshState> M;
+    std::vector<void*> to_free;  // device heap chunks to free at the next flush
+
+    int check_err(const char* where) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        int e = read_and_clear_err(ctx);
+        if (e) {
+            char buf[256];
+            kerr_decode(e, buf, sizeof(buf));
+            set_error(ctx, "lookahead_decompose: %s (%s)", buf, where);
+            return e;
+        }
+        return 0;
+    }
+
+    int flush_frees() {
+        if (to_free.empty()) return 0;
+        DevBuf<void*> d;
+        REQUIRE(d.alloc(ctx, to_free.size()) == 0, "lookahead: free list allocation failed");
+        HIP_TRY(hipMemcpyAsync(d.p, to_free.data(), sizeof(void*) * to_free.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        LA_TRY(la_launch_free(ctx, d.p, (int)to_free.size()));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        to_free.clear();
+        return 0;
+    }
+
+    void drop_part(Part& p) {
+        if (p.own && p.v) to_free.push_back(p.v);
+        if (p.own_hull && p.hv) to_free.push_back(p.hv);
+        p.v = nullptr;
+        p.hv = nullptr;
+    }
+
+    // ---- phase: connected components of every part (cuda/la_lifecycle.cu:310-407)
+    int components(const std::vector<int>& meshes) {
+        std::vector<LaCompTask> h;
+        std::vector<std::pair<int, int>> who;
+        for (int m : meshes)
+            for (int i = 0; i < (int)M[m].parts.size(); ++i) {
+                Part& p = M[m].parts[i];
+                if (p.nv <= 0 || p.nt <= 0) continue;
+                LaCompTask t{};
+                t.v = p.v; t.t = p.t; t.nv = p.nv; t.nt = p.nt;
+                h.push_back(t);
+                who.push_back({m, i});
+            }
+        if (h.empty()) return 0;
+        DevBuf<LaCompOut> out;
+        REQUIRE(out.alloc(ctx, h.size() * LA_MAX_COMP) == 0, "lookahead: component table allocation failed");
+        for (size_t k = 0; k < h.size(); ++k) h[k].out = out.p + k * LA_MAX_COMP;
+        DevBuf<LaCompTask> d;
+        REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+        HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaCompTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(la_launch_components(ctx, d.p, (int)h.size()));
+        HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaCompTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        LA_TRY(check_err("components"));
+        // split parts (ascending mesh, part order: new pieces append in that order)
+        std::vector<LaCompOut> o(LA_MAX_COMP);
+        for (size_t k = 0; k < h.size(); ++k) {
+            const int m = who[k].first, i = who[k].second;
+            MeshState& S = M[m];
+            const int nc = h[k].n_comp;
+            if (nc <= 1 && S.parts[i].cc != 0) continue;
+            if (nc <= 1) {
+                S.parts[i].cc = ++S.cc_counter;  // first pass over this lineage: fresh id
+                continue;
+            }
+            HIP_TRY(hipMemcpy(o.data(), h[k].out, sizeof(LaCompOut) * nc, hipMemcpyDeviceToHost));
+            Part old = S.parts[i];
+            int cc_base = 0;
+            const bool fresh = old.cc == 0;
+            if (fresh) { cc_base = S.cc_counter; S.cc_counter += nc; }
+            drop_part(old);
+            for (int c = 0; c < nc; ++c) {
+                Part q;
+                q.v = o[c].v; q.t = o[c].t; q.nv = o[c].nv; q.nt = o[c].nt;
+                q.mvol = std::fabs(o[c].svol);
+                q.hvol = 0.0;
+                q.haus = 0.0f;
+                q.own = true;
+                q.cc = fresh ? cc_base + c + 1 : old.cc;
+                for (int j = 0; j < 3; ++j) { q.lo[j] = o[c].lo[j]; q.hi[j] = o[c].hi[j]; }
+                if (c == 0) S.parts[i] = q;
+                else {
+                    REQUIRE((int)S.parts.size() < LA_MAX_DECOMP, "lookahead: more than %d parts", LA_MAX_DECOMP);
+                    S.parts.push_back(q);
+                }
+            }
+        }
+        return flush_frees();
+    }
+
+    // ---- phase: hulls of the parts without one (cuda/la_lifecycle.cu:268-300)
+    int hull_parts(const std::vector<int>& meshes) {
+        std::vector<LaHullTask> h;
+        std::vector<std::pair<int, int>> who;
+        for (int m : meshes)
+            for (int i = 0; i < (int)M[m].parts.size(); ++i) {
+                Part& p = M[m].parts[i];
+                if (p.hvol > 0.0 || p.nv == 0) continue;
+                LaHullTask t{};
+                t.pts = p.v; t.n = p.nv; t.emit = 1; t.prune_gap = 1e30;
+                h.push_back(t);
+                who.push_back({m, i});
+            }
+        if (h.empty()) return 0;
+        DevBuf<LaHullTask> d;
+        REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+        HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaHullTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
+        HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        LA_TRY(check_err("part hulls"));
+        for (size_t k = 0; k < h.size(); ++k) {
+            Part& p = M[who[k].first].parts[who[k].second];
+            if (p.own_hull && p.hv) to_free.push_back(p.hv);
+            p.hv = h[k].hv; p.ht = h[k].ht; p.hnv = h[k].hnv; p.hnt = h[k].hnt;
+            p.own_hull = p.hv != nullptr;
+            p.hvol = h[k].vol;
+            if (p.mvol > p.hvol) p.mvol = p.hvol;  // cuda/la_lifecycle.cu:292-296
+        }
+        return flush_frees();
+    }
+
+    // ---- phase: Hausdorff of parts already under the threshold by volume (cuda/la_refine.cu:19-45)
+    int hausdorff_parts(const std::vector<int>& meshes) {
+        std::vector<HdTask> h;
+        std::vector<std::pair<int, int>> who;
+        unsigned long long ws = 0;
+        for (int m : meshes)
+            for (int i = 0; i < (int)M[m].parts.size(); ++i) {
+                Part& p = M[m].parts[i];
+                if (!p.hv || p.haus > 0.0f || cost_rv(p.hvol, p.mvol) >= P.threshold) continue;
+                HdTask a{p.hv, p.ht, p.v, p.t, p.hnv, p.hnt, p.nv, p.nt, ws, 0, 0, nullptr, nullptr, nullptr};
+                ws += (hd_ws_bytes(p.nt, p.hnt) + 255ull) & ~255ull;
+                HdTask b{p.v, p.t, p.hv, p.ht, p.nv, p.nt, p.hnv, p.hnt, ws, 0, 0, nullptr, nullptr, nullptr};
+                ws += (hd_ws_bytes(p.hnt, p.nt) + 255ull) & ~255ull;
+                h.push_back(a);
+                h.push_back(b);
+                who.push_back({m, i});
+            }
+        if (h.empty()) return 0;
+        DevBuf<float> d2;
+        DevBuf<HdTask> d;
+        DevBuf<char> dws;
+        REQUIRE(d2.alloc(ctx, h.size()) == 0 && d.alloc(ctx, h.size()) == 0 && dws.alloc(ctx, ws + 256) == 0,
+                "lookahead: Hausdorff buffers allocation failed");
+        for (size_t k = 0; k < h.size(); ++k) h[k].out_d2 = d2.p + k;
+        HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(HdTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(hd_launch_tasks(ctx, d.p, (int)h.size(), dws.p));
+        std::vector<float> r(h.size());
+        HIP_TRY(hipMemcpyAsync(r.data(), d2.p, 4 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        LA_TRY(check_err("hausdorff"));
+        for (size_t k = 0; k < who.size(); ++k)
+            M[who[k].first].parts[who[k].second].haus = std::sqrt(std::max(r[2 * k], r[2 * k + 1]));
+        return 0;
+    }
+
+    // ---- concave-edge planes for the cutting parts (cuda/la_concave.cu:48-363)
+    int concave_planes(const std::vector<std::pair<int, int>>& cutting, std::vector<std::vector<float>>& planes) {
+        planes.assign(cutting.size(), {});
+        std::vector<LaEdgeTask> h(cutting.size());
+        size_t cap_total = 0;
+        std::vector<size_t> off(cutting.size());
+        for (size_t c = 0; c < cutting.size(); ++c) {
+            const Part& p = M[cutting[c].first].parts[cutting[c].second];
+            off[c] = cap_total;
+            const int cap = (3 * p.nt) / 2 + 16;
+            h[c] = LaEdgeTask{p.v, p.t, p.nv, p.nt, nullptr, cap, 0};
+            cap_total += cap;
+        }
+        DevBuf<LaEdgeRec> recs;
+        DevBuf<LaEdgeTask> d;
+        REQUIRE(recs.alloc(ctx, cap_total) == 0 && d.alloc(ctx, h.size()) == 0,
+                "lookahead: concave-edge buffers allocation failed");
+        for (size_t c = 0; c < h.size(); ++c) h[c].out = recs.p + off[c];
+        // dihedral > thr  <=>  (d > 0 and acos(ca) > thr - pi) or (d <= 0 and acos(ca) < pi - thr)
+        const double thr = P.concave_threshold;
+        const double x1 = thr - M_PI, x2 = M_PI - thr;
+        int pmode, nmode;
+        float cpos = 0.f, cneg = 0.f;
+        if (x1 < 0.0) pmode = 1; else if (x1 >= M_PI) pmode = 0; else { pmode = 2; cpos = (float)std::cos(x1); }
+        if (x2 <= 0.0) nmode = 0; else if (x2 > M_PI) nmode = 1; else { nmode = 2; cneg = (float)std::cos(x2); }
+        HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaEdgeTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(la_launch_edges(ctx, d.p, (int)h.size(), cpos, cneg, pmode, nmode));
+        HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaEdgeTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        LA_TRY(check_err("concave edges"));
+        std::vector<LaEdgeRec> r;
+        const int K = P.n_concave;
+        for (size_t c = 0; c < h.size(); ++c) {
+            const int cnt = std::min(h[c].count, h[c].cap);
+            r.resize(cnt);
+            if (cnt) HIP_TRY(hipMemcpy(r.data(), h[c].out, sizeof(LaEdgeRec) * cnt, hipMemcpyDeviceToHost));
+            // edges in (lo, hi) order, reservoir sampling with the reference's LCG
+            std::sort(r.begin(), r.end(), [](const LaEdgeRec& a, const LaEdgeRec& b) {
+                return a.lo != b.lo ? a.lo < b.lo : (a.hi != b.hi ? a.hi < b.hi : a.f0 < b.f0);
+            });
+            std::vector<int> res;
+            unsigned int rng = 1u;
+            int seen = 0;
+            for (int e = 0; e < cnt; ++e) {
+                ++seen;
+                if ((int)res.size() < K) {
+                    res.push_back(e);
+                } else {
+                    rng = rng * 1664525u + 1013904223u;
+                    const unsigned int thr_r = (unsigned int)((float)K / (float)seen * 4294967295.0f);
+                    if (rng < thr_r) {
+                        rng = rng * 1664525u + 1013904223u;
+                        res[rng % (unsigned)K] = e;
+                    }
+                }
+            }
+            std::vector<float>& pl = planes[c];
+            const float eps = P.concave_eps;
+            for (int e : res) {
+                const LaEdgeRec& x = r[e];
+                const float mx = x.mid[0], my = x.mid[1], mz = x.mid[2];
+                const float* n0 = x.n0;
+                const float* n1 = x.n1;
+                pl.insert(pl.end(), {n1[0], n1[1], n1[2], -(n1[0] * mx + n1[1] * my + n1[2] * mz + eps)});
+                pl.insert(pl.end(), {n0[0], n0[1], n0[2], -(n0[0] * mx + n0[1] * my + n0[2] * mz + eps)});
+                float bx = n0[0] + n1[0], by = n0[1] + n1[1], bz = n0[2] + n1[2];
+                const float bl = std::sqrt(bx * bx + by * by + bz * bz);
+                if (bl >= 1e-4f) {
+                    bx /= bl; by /= bl; bz /= bl;
+                    pl.insert(pl.end(), {bx, by, bz, -(bx * mx + by * my + bz * mz + eps)});
+                    pl.insert(pl.end(), {bx, by, bz, -(bx * mx + by * my + bz * mz - eps)});
+                }
+            }
+        }
+        return 0;
+    }
+
+    // Hull volumes for item halves. prune: per task gap (>=1e29 none).
+    int hull_volumes(std::vector<LaHullTask>& h) {
+        if (h.empty()) return 0;
+        DevBuf<LaHullTask> d;
+        REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+        HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaHullTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
+        HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        return check_err("item hulls");
+    }
+
+    // ---- one search iteration over the meshes in `act`; returns meshes still active
+    int iterate(std::vector<int>& act, int iter) {
+        if (P.dc_per_iter) LA_TRY(components(act));
+        LA_TRY(hull_parts(act));
+        LA_TRY(hausdorff_parts(act));
+        // sort by full cost, pick the cutting parts (cuda/la_lifecycle.cu:90-144)
+        std::vector<std::pair<int, int>> cutting;   // (mesh, part index)
+        std::vector<int> cut_begin(M.size(), 0), cut_count(M.size(), 0);
+        std::vector<int> next;
+        for (int m : act) {
+            MeshState& S = M[m];
+            std::stable_sort(S.parts.begin(), S.parts.end(),
+                             [](const Part& a, const Part& b) { return cost_full(a) < cost_full(b); });
+            const int np = (int)S.parts.size();
+            int first = np;
+            for (int i = 0; i < np; ++i)
+                if (cost_full(S.parts[i]) >= P.threshold) { first = i; break; }
+            int start = first;
+            if (np - first > P.max_n_cutting) start = np - P.max_n_cutting;
+            cut_begin[m] = (int)cutting.size();
+            for (int i = start; i < np && cut_count[m] < P.max_n_cutting; ++i)
+                if (cost_full(S.parts[i]) >= P.threshold) {
+                    cutting.push_back({m, i});
+                    ++cut_count[m];
+                }
+            if (cut_count[m] == 0) S.done = true;
+            else next.push_back(m);
+        }
+        act = next;
+        if (cutting.empty()) return 0;
+        for (int m : act) M[m].iters = iter + 1;
+        float med = P.threshold * 0.25f;
+        if (med > 0.015f) med = 0.015f;
+        // concave-edge planes
+        std::vector<std::vector<float>> eplanes(cutting.size());
+        std::vector<int> n_edge(M.size(), 0);
+        if (P.n_concave > 0 && iter < P.concave_iters) {
+            LA_TRY(concave_planes(cutting, eplanes));
+            for (size_t c = 0; c < cutting.size(); ++c)
+                n_edge[cutting[c].first] = std::max(n_edge[cutting[c].first], (int)eplanes[c].size() / 4);
+        }
+        const int total_levels = P.depth;
+        // the iteration's store of halves built by full cuts
+        struct Half {
+            float* v;
+            bool keep;
+        };
+        std::vector<Half> store;
+        std::vector<Item> extra;       // leaves for parts too small to cut
+        // level-0 items per cutting part and initial cut, -1 = no item
+        std::vector<std::vector<int>> l0(cutting.size());
+        std::vector<Item> cur;         // items entering the next level
+        // seeds
+        std::vector<Item> seeds(cutting.size());
+        for (size_t c = 0; c < cutting.size(); ++c) {
+            const Part& p = M[cutting[c].first].parts[cutting[c].second];
+            Item& it = seeds[c];
+            it.mesh = cutting[c].first;
+            it.src = (int)c - cut_begin[it.mesh];
+            it.cut0 = 0;
+            it.np = 1;
+            IPart& q = it.parts[0];
+            q.v = p.v; q.t = p.t; q.nv = p.nv; q.nt = p.nt; q.mvol = p.mvol; q.hvol = p.hvol; q.cc = p.cc;
+            for (int j = 0; j < 3; ++j) { q.lo[j] = p.lo[j]; q.hi[j] = p.hi[j]; }
+            REQUIRE(p.nv > 0 && p.nt > 0 && p.nv <= 1000000 && p.nt <= 1000000, "lookahead: invalid part mesh");
+            l0[c].assign(P.width + n_edge[it.mesh], -1);
+        }
+        std::vector<Item> level0_items;  // all valid level-0 items (index = l0 entries)
+        // items of a mesh whose next level produced nothing stay its leaves
+        // (reference: the level loop breaks with next_n == 0, csrc/lookahead.c:579-581)
+        std::vector<Item> leaves;
+        std::vector<int> leaves_c;
+        struct Leaf {
+            int c, cut0;
+            float path;
+            bool pruned;
+        };
+        std::vector<Leaf> leaf_recs;
+        cur = seeds;
+        std::vector<int> cur_c(cutting.size());  // cutting slot of each item
+        for (size_t c = 0; c < cutting.size(); ++c) cur_c[c] = (int)c;
+        for (int d = 0; d < P.depth; ++d) {
+            const bool last = d == P.depth - 1;
+            const bool full = !last || P.depth == 1;
+            // candidates
+            struct Cand {
+                int item, k;
+                float pl[4];
+            };
+            std::vector<Cand> cands;
+            for (size_t ii = 0; ii < cur.size(); ++ii) {
+                const Item& it = cur[ii];
+                const int w = d == 0 ? P.width : P.width2;
+                const int ne = d == 0 ? n_edge[it.mesh] : 0;
+                const int tw = w + ne;
+                const IPart& wp = it.parts[it.np - 1];
+                REQUIRE(it.np + 1 <= LA_MAX_PARTS, "lookahead: item part capacity exceeded");
+                for (int k = 0; k < tw; ++k) {
+                    Cand cd;
+                    cd.item = (int)ii;
+                    cd.k = k;
+                    if (k >= w) {
+                        const std::vector<float>& ep = eplanes[cur_c[ii]];
+                        const int e = k - w;
+                        if (4 * e + 3 < (int)ep.size()) {
+                            for (int j = 0; j < 4; ++j) cd.pl[j] = ep[4 * e + j];
+                        } else {
+                            cd.pl[0] = cd.pl[1] = cd.pl[2] = cd.pl[3] = 0.f;
+                        }
+                        if (cd.pl[0] == 0.f && cd.pl[1] == 0.f && cd.pl[2] == 0.f) continue;
+                    }
+                    const float eps = 1e-6f;
+                    const bool small = (wp.hi[0] - wp.lo[0] <= 2.f * med + eps) && (wp.hi[1] - wp.lo[1] <= 2.f * med + eps) &&
+                                       (wp.hi[2] - wp.lo[2] <= 2.f * med + eps);
+                    if (small) {
+                        if (k == 0) {
+                            Item x = it;
+                            for (int l = x.nl; l < total_levels && l < LA_MAX_LEVELS; ++l) x.lc[l] = 0.0f;
+                            x.nl = std::min(total_levels, LA_MAX_LEVELS);
+                            extra.push_back(x);
+                        }
+                        continue;
+                    }
+                    if (k < w) {
+                        const AxisCut a = axis_cut(w, k);
+                        const float lov = wp.lo[a.axis] + med, hiv = wp.hi[a.axis] - med;
+                        if (lov >= hiv) continue;
+                        const float frac = (a.slice + 1.f) / (a.cpa + 1.f);
+                        const float pos = lov + frac * (hiv - lov);
+                        cd.pl[0] = a.axis == 0 ? 1.f : 0.f;
+                        cd.pl[1] = a.axis == 1 ? 1.f : 0.f;
+                        cd.pl[2] = a.axis == 2 ? 1.f : 0.f;
+                        cd.pl[3] = -pos;
+                    }
+                    cands.push_back(cd);
+                }
+            }
+            // build the children: halves + volumes
+            std::vector<Item> nxt;
+            std::vector<int> nxt_c;
+            std::vector<LaHullTask> ht;
+            std::vector<std::pair<int, int>> ht_who;  // (child, which half)
+            std::vector<void*> pts_free;
+            // last-level children are scored, not built: parent, cut, volumes
+            struct SRec {
+                int item, k;
+                double pvol, nvol, phv, nhv;
+                bool pruned;
+            };
+            std::vector<SRec> srec;
+            if (full) {
+                std::vector<LaCutTask> h(cands.size());
+                for (size_t q = 0; q < cands.size(); ++q) {
+                    const IPart& wp = cur[cands[q].item].parts[cur[cands[q].item].np - 1];
+                    LaCutTask t{};
+                    t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
+                    for (int j = 0; j < 4; ++j) t.plane[j] = cands[q].pl[j];
+                    h[q] = t;
+                }
+                if (!h.empty()) {
+                    DevBuf<LaCutTask> dt;
+                    REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+                    HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice,
+                                           ctx->stream));
+                    LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
+                    HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
+                                           ctx->stream));
+                    LA_TRY(check_err("cuts"));
+                }
+                for (size_t q = 0; q < cands.size(); ++q) {
+                    const LaCutTask& t = h[q];
+                    if (t.status != 0 || !t.pv || !t.nvp) continue;
+                    const Item& par = cur[cands[q].item];
+                    Item ch = par;
+                    IPart pp, nn;
+                    pp.v = t.pv; pp.t = t.pt; pp.nv = t.pnv; pp.nt = t.pnt; pp.mvol = t.pvol;
+                    nn.v = t.nvp; nn.t = t.ntp; nn.nv = t.nnv; nn.nt = t.nnt; nn.mvol = t.nvol;
+                    pp.cc = nn.cc = par.parts[par.np - 1].cc;
+                    for (int j = 0; j < 3; ++j) {
+                        pp.lo[j] = t.plo[j]; pp.hi[j] = t.phi[j];
+                        nn.lo[j] = t.nlo[j]; nn.hi[j] = t.nhi[j];
+                    }
+                    pp.store = (int)store.size();
+                    store.push_back({t.pv, false});
+                    nn.store = (int)store.size();
+                    store.push_back({t.nvp, false});
+                    ch.parts[par.np - 1] = pp;
+                    ch.parts[par.np] = nn;
+                    ch.np = par.np + 1;
+                    if (par.nl == 0) ch.cut0 = cands[q].k;
+                    const int ci = (int)nxt.size();
+                    nxt.push_back(ch);
+                    nxt_c.push_back(cur_c[cands[q].item]);
+                    LaHullTask a{};
+                    a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30;
+                    LaHullTask b{};
+                    b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30;
+                    ht.push_back(a);
+                    ht_who.push_back({ci, 0});
+                    ht.push_back(b);
+                    ht_who.push_back({ci, 1});
+                }
+            } else {
+                std::vector<LaScoreTask> h(cands.size());
+                for (size_t q = 0; q < cands.size(); ++q) {
+                    const IPart& wp = cur[cands[q].item].parts[cur[cands[q].item].np - 1];
+                    LaScoreTask t{};
+                    t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
+                    for (int j = 0; j < 4; ++j) t.plane[j] = cands[q].pl[j];
+                    h[q] = t;
+                }
+                if (!h.empty()) {
+                    DevBuf<LaScoreTask> dt;
+                    REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+                    HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaScoreTask) * h.size(), hipMemcpyHostToDevice,
+                                           ctx->stream));
+                    LA_TRY(la_launch_score(ctx, dt.p, (int)h.size()));
+                    HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaScoreTask) * h.size(), hipMemcpyDeviceToHost,
+                                           ctx->stream));
+                    LA_TRY(check_err("leaf scoring"));
+                }
+                // branch-and-bound bound per mesh (cuda/la_expand.cu:434-457): min over the
+                // items entering this level of sum(lc) + worst * (levels left)
+                std::vector<float> best_ub(M.size(), 1e30f);
+                for (const Item& it : cur) {
+                    if (it.nl <= 0) continue;
+                    float ps = 0.0f;
+                    for (int l = 0; l < it.nl; ++l) ps += it.lc[l];
+                    const float ub = ps + it.lc[it.nl - 1] * (float)(total_levels - it.nl);
+                    if (ub < best_ub[it.mesh]) best_ub[it.mesh] = ub;
+                }
+                for (size_t q = 0; q < cands.size(); ++q) {
+                    const LaScoreTask& t = h[q];
+                    if (t.ppts) pts_free.push_back(t.ppts);
+                    if (t.npts) pts_free.push_back(t.npts);
+                    if (t.npt == 0 || t.nnt == 0 || !t.ppts || !t.npts) continue;
+                    const Item& par = cur[cands[q].item];
+                    // pruning threshold (cuda/la_expand.cu:380-399)
+                    double gap = 1e30;
+                    const float ub = best_ub[par.mesh];
+                    const int nl = par.nl;
+                    if (ub < 1e29f && nl > 0 && nl < total_levels) {
+                        float ps = 0.0f;
+                        for (int l = 0; l < nl; ++l) ps += par.lc[l];
+                        const float rv_thr = ub - ps;
+                        if (rv_thr <= 0.0f) gap = -1.0;
+                        else {
+                            const float r = rv_thr / 0.3f;
+                            gap = (double)(r * r * r * (4.0f * LA_PI_F / 3.0f));
+                        }
+                    }
+                    const int ci = (int)srec.size();
+                    srec.push_back({cands[q].item, cands[q].k, t.pvol, t.nvol, 0.0, 0.0, false});
+                    LaHullTask a{};
+                    a.pts = t.ppts; a.n = t.np_pts; a.prune_gap = gap; a.mesh_vol = t.pvol;
+                    LaHullTask b{};
+                    b.pts = t.npts; b.n = t.nn_pts; b.prune_gap = gap; b.mesh_vol = t.nvol;
+                    ht.push_back(a);
+                    ht_who.push_back({ci, 0});
+                    ht.push_back(b);
+                    ht_who.push_back({ci, 1});
+                }
+            }
+            LA_TRY(hull_volumes(ht));
+            to_free.insert(to_free.end(), pts_free.begin(), pts_free.end());
+            if (!full) {
+                // leaf costs: the worst part is the maximum rv cost over the
+                // inherited parts and both halves; path cost = mean level cost
+                for (size_t q = 0; q < ht.size(); ++q) {
+                    SRec& r = srec[ht_who[q].first];
+                    (ht_who[q].second ? r.nhv : r.phv) = ht[q].vol;
+                    if (ht[q].pruned) r.pruned = true;
+                }
+                std::vector<char> has_child(M.size(), 0);
+                for (const SRec& r : srec) {
+                    const Item& par = cur[r.item];
+                    has_child[par.mesh] = 1;
+                    float worst = std::max(cost_rv(r.phv, r.pvol), cost_rv(r.nhv, r.nvol));
+                    for (int i = 0; i < par.np - 1; ++i) worst = std::max(worst, cost_rv(par.parts[i].hvol, par.parts[i].mvol));
+                    float sum = 0.0f;
+                    for (int l = 0; l < par.nl; ++l) sum += par.lc[l];
+                    sum += worst;
+                    Leaf L;
+                    L.c = cur_c[r.item];
+                    L.cut0 = par.nl == 0 ? r.k : par.cut0;
+                    L.path = sum / (float)(par.nl + 1);
+                    L.pruned = r.pruned;
+                    leaf_recs.push_back(L);
+                }
+                for (size_t i = 0; i < cur.size(); ++i)
+                    if (!has_child[cur[i].mesh]) {
+                        leaves.push_back(cur[i]);
+                        leaves_c.push_back(cur_c[i]);
+                    }
+                cur.clear();
+                cur_c.clear();
+                break;
+            }
+            for (Item& ch : nxt) {
+                sort_item_parts(ch);
+                if (ch.nl < LA_MAX_LEVELS) {
+                    ch.lc[ch.nl] = cost_rv(ch.parts[ch.np - 1].hvol, ch.parts[ch.np - 1].mvol);
+                    ch.nl++;
+                }
+            }
+            if (d == 0) {
+                level0_items = nxt;
+                for (size_t i = 0; i < nxt.size(); ++i) l0[nxt_c[i]][nxt[i].cut0] = (int)i;
+            }
+            std::vector<char> has_child(M.size(), 0);
+            for (const Item& ch : nxt) has_child[ch.mesh] = 1;
+            for (size_t i = 0; i < cur.size(); ++i)
+                if (!has_child[cur[i].mesh] && d > 0) {
+                    leaves.push_back(cur[i]);
+                    leaves_c.push_back(cur_c[i]);
+                }
+            cur = nxt;
+            cur_c = nxt_c;
+            if (cur.empty()) break;
+        }
+        for (size_t i = 0; i < cur.size(); ++i) {
+            leaves.push_back(cur[i]);
+            leaves_c.push_back(cur_c[i]);
+        }
+        // evaluate (cuda/la_refine.cu:418-527) and apply (cuda/la_lifecycle.cu:151-261)
+        std::vector<std::vector<float>> best(cutting.size());
+        for (size_t c = 0; c < cutting.size(); ++c) best[c].assign(l0[c].size(), 1e30f);
+        auto consider = [&](const Item& it, int c) {
+            if (it.pruned || it.nl == 0) return;
+            const float pc = item_path_cost(it);
+            if (it.cut0 >= 0 && it.cut0 < (int)best[c].size() && pc < best[c][it.cut0]) best[c][it.cut0] = pc;
+        };
+        for (size_t i = 0; i < leaves.size(); ++i) consider(leaves[i], leaves_c[i]);
+        for (const Leaf& L : leaf_recs)
+            if (!L.pruned && L.cut0 >= 0 && L.cut0 < (int)best[L.c].size() && L.path < best[L.c][L.cut0])
+                best[L.c][L.cut0] = L.path;
+        for (const Item& x : extra) consider(x, cut_begin[x.mesh] + x.src);
+        std::vector<int> adds_part;
+        for (size_t c = 0; c < cutting.size(); ++c) {
+            bool any = false;
+            for (float b : best[c]) any |= b < 1e30f;
+            if (!any)
+                for (size_t k = 0; k < l0[c].size(); ++k)
+                    if (l0[c][k] >= 0 && level0_items[l0[c][k]].nl >= 1) best[c][k] = level0_items[l0[c][k]].lc[0];
+            float bc = 1e30f;
+            int bi = 0;
+            for (size_t k = 0; k < best[c].size(); ++k)
+                if (best[c][k] < bc) { bc = best[c][k]; bi = (int)k; }
+            if (P.verbose)
+                fprintf(stderr, "[la] mesh %d iter %d part %d: best cut %d cost %.6f\n", cutting[c].first, iter,
+                        cutting[c].second, bi, bc);
+            if (bi >= (int)l0[c].size() || l0[c][bi] < 0) continue;
+            const Item& w = level0_items[l0[c][bi]];
+            // adopt the unsorted halves of the level-0 cut: positive half replaces the part
+            const IPart* hp[2] = {nullptr, nullptr};
+            for (int q = 0; q < w.np; ++q) {
+                if (w.parts[q].store < 0) continue;
+                // store indices are consecutive per cut: positive first
+                if (!hp[0] || w.parts[q].store < hp[0]->store) { hp[1] = hp[0]; hp[0] = &w.parts[q]; }
+                else hp[1] = &w.parts[q];
+            }
+            if (!hp[0] || !hp[1]) continue;
+            const int m = cutting[c].first;
+            MeshState& S = M[m];
+            Part& old = S.parts[cutting[c].second];
+            const int cc = old.cc;
+            drop_part(old);
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const IPart& x = *hp[s2];
+                store[x.store].keep = true;
+                Part q;
+                q.v = x.v; q.t = x.t; q.nv = x.nv; q.nt = x.nt; q.mvol = x.mvol; q.hvol = 0.0; q.haus = 0.0f;
+                q.cc = cc; q.own = true;
+                for (int j = 0; j < 3; ++j) { q.lo[j] = x.lo[j]; q.hi[j] = x.hi[j]; }
+                if (s2 == 0) old = q;
+                else {
+                    REQUIRE((int)S.parts.size() < LA_MAX_DECOMP, "lookahead: more than %d parts", LA_MAX_DECOMP);
+                    S.parts.push_back(q);
+                }
+            }
+        }
+        for (const Half& hf : store)
+            if (!hf.keep) to_free.push_back(hf.v);
+        return flush_frees();
+    }
+};
+
+}  // namespace
+
+// Batched lookahead over n meshes (packed), each decomposed independently.
+static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* voff, const int* tris,
+                         const int* toff, const float* hverts, const int* hvoff, const int* htris, const int* htoff,
+                         const Params& P, struct gpu_result* out, int* iters_out) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (int m = 0; m < n_meshes; ++m) { out[m].parts = nullptr; out[m].nparts = 0; }
+    REQUIRE(P.depth >= 1 && P.depth <= LA_MAX_LEVELS, "lookahead: depth must be in [1, %d]", LA_MAX_LEVELS);
+    REQUIRE(P.quick_depth == 0, "lookahead: quick_depth > 0 is not supported");
+    REQUIRE(P.width >= 1 && P.width2 >= 1, "lookahead: width and width2 must be positive");
+    REQUIRE(P.width + 4 * P.n_concave < 512, "lookahead: width + 4*n_concave_edges must be < 512");
+    REQUIRE(P.max_n_cutting >= 1, "lookahead: max_n_cutting must be positive");
+    REQUIRE(!P.merge, "lookahead: merge_hulls is not built yet");
+    Engine E;
+    E.ctx = ctx;
+    E.P = P;
+    E.M.resize(n_meshes);
+    const size_t tv = (size_t)voff[n_meshes], tt = (size_t)toff[n_meshes];
+    const size_t thv = (size_t)hvoff[n_meshes], tht = (size_t)htoff[n_meshes];
+    float* d_v = nullptr;
+    int* d_t = nullptr;
+    float* d_hv = nullptr;
+    int* d_ht = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_v, 12 * tv + 16, ctx->stream));
+    HIP_TRY(hipMallocAsync((void**)&d_t, 12 * tt + 16, ctx->stream));
+    HIP_TRY(hipMallocAsync((void**)&d_hv, 12 * thv + 16, ctx->stream));
+    HIP_TRY(hipMallocAsync((void**)&d_ht, 12 * tht + 16, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_v, verts, 12 * tv, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_t, tris, 12 * tt, hipMemcpyHostToDevice, ctx->stream));
+    if (thv) HIP_TRY(hipMemcpyAsync(d_hv, hverts, 12 * thv, hipMemcpyHostToDevice, ctx->stream));
+    if (tht) HIP_TRY(hipMemcpyAsync(d_ht, htris, 12 * tht, hipMemcpyHostToDevice, ctx->stream));
+    int rc = 0;
+    std::vector<int> act;
+    for (int m = 0; m < n_meshes; ++m) {
+        Part p;
+        p.v = d_v + 3ll * voff[m];
+        p.t = d_t + 3ll * toff[m];
+        p.nv = voff[m + 1] - voff[m];
+        p.nt = toff[m + 1] - toff[m];
+        const float* hv = verts + 3ll * voff[m];
+        // FP64 volumes of the input mesh and of its hull (cuda/la_lifecycle.cu:18-81)
+        auto vol = [](const float* V, const int* T, int nt) {
+            if (nt <= 0) return 0.0;
+            const double ax = V[0], ay = V[1], az = V[2];
+            double acc = 0.0;
+            for (int k = 0; k < nt; ++k) {
+                const float* a = V + 3 * T[3 * k];
+                const float* b = V + 3 * T[3 * k + 1];
+                const float* c = V + 3 * T[3 * k + 2];
+                const double x0 = a[0] - ax, y0 = a[1] - ay, z0 = a[2] - az;
+                const double x1 = b[0] - ax, y1 = b[1] - ay, z1 = b[2] - az;
+                const double x2 = c[0] - ax, y2 = c[1] - ay, z2 = c[2] - az;
+                acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
+            }
+            return std::fabs(acc) / 6.0;
+        };
+        p.mvol = vol(hv, tris + 3ll * toff[m], p.nt);
+        const int hn = htoff[m + 1] - htoff[m];
+        p.hv = d_hv + 3ll * hvoff[m];
+        p.ht = d_ht + 3ll * htoff[m];
+        p.hnv = hvoff[m + 1] - hvoff[m];
+        p.hnt = hn;
+        p.hvol = vol(hverts + 3ll * hvoff[m], htris + 3ll * htoff[m], hn);
+        p.own = false;
+        for (int j = 0; j < 3; ++j) { p.lo[j] = 1e30f; p.hi[j] = -1e30f; }
+        for (int i = 0; i < p.nv; ++i)
+            for (int j = 0; j < 3; ++j) {
+                p.lo[j] = std::min(p.lo[j], hv[3 * i + j]);
+                p.hi[j] = std::max(p.hi[j], hv[3 * i + j]);
+            }
+        E.M[m].parts.push_back(p);
+        act.push_back(m);
+    }
+    for (int iter = 0; iter < P.max_iters && !act.empty() && rc == 0; ++iter) rc = E.iterate(act, iter);
+    // meshes that ran out of iterations: last apply still needs components + hulls
+    if (rc == 0 && !act.empty()) {
+        if (P.dc_per_iter) rc = E.components(act);
+        if (rc == 0) rc = E.hull_parts(act);
+    }
+    if (rc == 0 && P.dc_final) {
+        std::vector<int> all;
+        for (int m = 0; m < n_meshes; ++m) all.push_back(m);
+        rc = E.components(all);
+        if (rc == 0) rc = E.hull_parts(all);
+    }
+    // results
+    if (rc == 0) {
+        for (int m = 0; m < n_meshes && rc == 0; ++m) {
+            MeshState& S = E.M[m];
+            if (iters_out) iters_out[m] = S.iters;
+            const int np = (int)S.parts.size();
+            out[m].nparts = np;
+            out[m].parts = (struct gpu_part_result*)calloc(np > 0 ? np : 1, sizeof(struct gpu_part_result));
+            for (int i = 0; i < np; ++i) {
+                Part& p = S.parts[i];
+                gpu_part_result& r = out[m].parts[i];
+                r.nv = p.nv; r.nt = p.nt; r.hull_nv = p.hnv; r.hull_nt = p.hnt;
+                r.mesh_vol = (float)p.mvol;
+                r.hull_vol = (float)p.hvol;
+                r.hausdorff = p.haus;
+                r.verts = (float*)malloc(12ull * (p.nv ? p.nv : 1));
+                r.tris = (int*)malloc(12ull * (p.nt ? p.nt : 1));
+                r.hull_verts = (float*)malloc(12ull * (p.hnv ? p.hnv : 1));
+                r.hull_tris = (int*)malloc(12ull * (p.hnt ? p.hnt : 1));
+                if (p.nv) HIP_TRY(hipMemcpyAsync(r.verts, p.v, 12ull * p.nv, hipMemcpyDeviceToHost, ctx->stream));
+                if (p.nt) HIP_TRY(hipMemcpyAsync(r.tris, p.t, 12ull * p.nt, hipMemcpyDeviceToHost, ctx->stream));
+                if (p.hnv && p.hv)
+                    HIP_TRY(hipMemcpyAsync(r.hull_verts, p.hv, 12ull * p.hnv, hipMemcpyDeviceToHost, ctx->stream));
+                if (p.hnt && p.ht)
+                    HIP_TRY(hipMemcpyAsync(r.hull_tris, p.ht, 12ull * p.hnt, hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    // free everything the search still holds
+    for (auto& S : E.M)
+        for (auto& p : S.parts) {
+            if (p.own && p.v) E.to_free.push_back(p.v);
+            if (p.own_hull && p.hv) E.to_free.push_back(p.hv);
+            p.v = nullptr;
+            p.hv = nullptr;
+        }
+    const int frc = E.flush_frees();
+    HIP_TRY(hipFreeAsync(d_v, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_t, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_hv, ctx->stream));
+    HIP_TRY(hipFreeAsync(d_ht, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (rc) {
+        for (int m = 0; m < n_meshes; ++m) gpu_result_free(&out[m]);
+    }
+    return rc ? rc : frc;
+}
+
+extern "C" {
+
+int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tris, int nt, const float* hull_verts,
+                        int hull_nv, const int* hull_tris, int hull_nt, int max_iters, int width, int width2,
+                        float threshold, int depth, int quick_depth, int max_n_cutting, int verbose, int debug,
+                        int decompose_components, int decompose_components_per_iter, int n_concave_edges,
+                        float concave_eps, float concave_threshold, int concave_iters, int merge_hulls,
+                        struct gpu_result* out) {
+    (void)debug;
+    Params P{max_iters, width, width2, depth, quick_depth, max_n_cutting, verbose, threshold, decompose_components,
+             decompose_components_per_iter, n_concave_edges, concave_iters, merge_hulls, concave_eps,
+             concave_threshold};
+    const int voff[2] = {0, nv}, toff[2] = {0, nt}, hvoff[2] = {0, hull_nv}, htoff[2] = {0, hull_nt};
+    return lookahead_run(ctx, 1, verts, voff, tris, toff, hull_verts, hvoff, hull_tris, htoff, P, out, nullptr);
+}
+
+int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
+                        const int* tri_off, const float* hull_verts, const int* hull_vert_off, const int* hull_tris,
+                        const int* hull_tri_off, int max_iters, int width, int width2, float threshold, int depth,
+                        int max_n_cutting, int decompose_components, int decompose_components_per_iter,
+                        int n_concave_edges, float concave_eps, float concave_threshold, int concave_iters,
+                        int merge_hulls, struct gpu_result* out, int* out_iters) {
+    Params P{max_iters, width, width2, depth, 0, max_n_cutting, 0, threshold, decompose_components,
+             decompose_components_per_iter, n_concave_edges, concave_iters, merge_hulls, concave_eps,
+             concave_threshold};
+    return lookahead_run(ctx, n_meshes, verts, vert_off, tris, tri_off, hull_verts, hull_vert_off, hull_tris,
+                         hull_tri_off, P, out, out_iters);
+}
+
+}  // extern "C"

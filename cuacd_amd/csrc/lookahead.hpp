@@ -1,0 +1,119 @@
+// lookahead.hpp - task records shared by the lookahead kernels and their host.
+//
+// The lookahead search (reference csrc/lookahead.c:144-1124 and the la_*
+// kernels under cuda/) is driven from the host over all meshes of a batch at
+// once: every phase is one launch over a flat task table that mixes the
+// meshes, so a launch holds thousands of workgroups even when each mesh has
+// only a few parts. Meshes live in device-heap chunks ([verts | tris], freed
+// with heap_free on the chunk base); tasks carry raw device pointers.
+#pragma once
+#include "ctx.hpp"
+#include "hull.hpp"
+#include "hausdorff.hpp"
+
+namespace cuacd {
+
+constexpr int LA_MAX_COMP = 256;  // components kept per part (more merge into the last)
+
+// Hull of a point set: FP64 volume, optionally the hull mesh in a heap chunk.
+// Sets above LA_HULL_WAVE_MAX points go through the k-DOP prefilter first
+// (reference cuda/kdop_hull.cuh:49-398), which may also prune the set.
+struct LaHullTask {
+    const float* pts;    // xyz
+    int n;
+    int emit;            // 1: write the hull mesh into a heap-0 chunk (hv, ht)
+    double mesh_vol;     // for pruning
+    double prune_gap;    // prune when max(rough_vol - mesh_vol, 0) > prune_gap; >= 1e29 disables
+    QFrame frame;        // quantization frame of the full set (k-DOP path)
+    int use_frame;
+    int nf;              // filtered point count (k-DOP path)
+    float* filt;         // filtered points (scratch-heap chunk, freed by the final hull)
+    // results
+    double vol;
+    int pruned;
+    int err;
+    float* hv;           // hull mesh chunk: verts, then tris at ht
+    int* ht;
+    int hnv, hnt;
+};
+
+// Full plane cut of one part (level 0 of the search): both closed halves.
+struct LaCutTask {
+    const float* v;
+    const int* t;
+    int nv, nt;
+    float plane[4];
+    // results
+    float* pv; int* pt; int pnv, pnt;
+    float* nvp; int* ntp; int nnv, nnt;
+    double pvol, nvol;
+    float plo[3], phi[3], nlo[3], nhi[3];
+    int status;          // 0 cut, 1 cap failed, 2 one side empty, <0 error
+};
+
+// Leaf scoring of one cut (last level): exact half-space volumes of both
+// sides and the point sets their hulls are built from (scratch-heap chunks).
+struct LaScoreTask {
+    const float* v;
+    const int* t;
+    int nv, nt;
+    float plane[4];
+    // results
+    double pvol, nvol;   // half-space volumes (divergence origin on the plane)
+    int npt, nnt;        // triangles (whole or split pieces) on each side; 0 = empty side
+    float* ppts; int np_pts;
+    float* npts; int nn_pts;
+};
+
+// Connected components of one part (union-find over triangle vertices).
+struct LaCompOut {
+    float* v; int* t; int nv, nt;
+    double svol;
+    float lo[3], hi[3];
+};
+struct LaCompTask {
+    const float* v;
+    const int* t;
+    int nv, nt;
+    // results
+    int n_comp;          // 1: unchanged; >1: out[0..n_comp-1] hold the pieces
+    LaCompOut* out;      // LA_MAX_COMP slots
+};
+
+// Concave-edge detection of one part: records of shared edges whose dihedral
+// angle exceeds the threshold (reference cuda/la_concave.cu:48-363).
+struct LaEdgeRec {
+    int lo, hi, f0, f1;
+    float n0[3], n1[3], mid[3];
+    int pad;
+};
+struct LaEdgeTask {
+    const float* v;
+    const int* t;
+    int nv, nt;
+    LaEdgeRec* out;      // capacity cap records
+    int cap;
+    int count;           // result (may exceed cap: then records were dropped)
+};
+
+// bbox of a mesh's vertices
+struct LaBoxTask {
+    const float* v;
+    int nv;
+    float lo[3], hi[3];
+};
+
+// ---- launchers (la_kernels.hip); all stream-ordered on ctx->stream --------------
+int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n);
+int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n);
+int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n);
+int la_launch_components(gpu_ctx_t ctx, LaCompTask* d_tasks, int n);
+int la_launch_edges(gpu_ctx_t ctx, LaEdgeTask* d_tasks, int n, float cos_pos, float cos_neg, int pos_mode,
+                    int neg_mode);
+int la_launch_bbox(gpu_ctx_t ctx, LaBoxTask* d_tasks, int n);
+// heap_free every non-null pointer of the list (device array)
+int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n);
+
+}  // namespace cuacd
+
+int hd_launch_tasks(gpu_ctx_t ctx, const cuacd::HdTask* d_tasks, int n, char* d_ws);

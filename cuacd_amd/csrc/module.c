@@ -190,6 +190,67 @@ static PyObject* m_plane_cut_batch(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* part_list(struct gpu_result* res) {
+    PyObject* list = PyList_New(res->nparts);
+    if (!list) return NULL;
+    for (int i = 0; i < res->nparts; ++i) {
+        struct gpu_part_result* p = &res->parts[i];
+        PyObject* vb = PyBytes_FromStringAndSize((const char*)p->verts, (Py_ssize_t)p->nv * 12);
+        PyObject* tb = PyBytes_FromStringAndSize((const char*)p->tris, (Py_ssize_t)p->nt * 12);
+        PyObject* hvb = PyBytes_FromStringAndSize((const char*)p->hull_verts, (Py_ssize_t)p->hull_nv * 12);
+        PyObject* htb = PyBytes_FromStringAndSize((const char*)p->hull_tris, (Py_ssize_t)p->hull_nt * 12);
+        PyObject* tup = NULL;
+        if (vb && tb && hvb && htb)
+            tup = Py_BuildValue("(OOiiOOiifff)", vb, tb, p->nv, p->nt, hvb, htb, p->hull_nv, p->hull_nt,
+                                (double)p->mesh_vol, (double)p->hull_vol, (double)p->hausdorff);
+        Py_XDECREF(vb);
+        Py_XDECREF(tb);
+        Py_XDECREF(hvb);
+        Py_XDECREF(htb);
+        if (!tup) {
+            Py_DECREF(list);
+            return NULL;
+        }
+        PyList_SetItem(list, i, tup);
+    }
+    return list;
+}
+
+static PyObject* m_lookahead_batch(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, hv, hvo, ht, hto, outp, itp;
+    int n, max_iters, width, width2, depth, max_cut, dcomp, dc_iter, n_conc, conc_iters, merge;
+    float thr, conc_eps, conc_thr;
+    if (!PyArg_ParseTuple(args, "iKKKKKKKKiiifiiiiiffiiKK", &n, &v, &vo, &t, &to, &hv, &hvo, &ht, &hto, &max_iters,
+                          &width, &width2, &thr, &depth, &max_cut, &dcomp, &dc_iter, &n_conc, &conc_eps, &conc_thr,
+                          &conc_iters, &merge, &outp, &itp))
+        return NULL;
+    NEED_CTX();
+    struct gpu_result* res = (struct gpu_result*)calloc(n > 0 ? n : 1, sizeof(struct gpu_result));
+    if (!res) return PyErr_NoMemory();
+    int rc = gpu_lookahead_batch(g_ctx, n, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
+                                 P(const float*, hv), P(const int*, hvo), P(const int*, ht), P(const int*, hto),
+                                 max_iters, width, width2, thr, depth, max_cut, dcomp, dc_iter, n_conc, conc_eps,
+                                 conc_thr, conc_iters, merge, res, P(int*, itp));
+    (void)outp;
+    if (rc) {
+        free(res);
+        return raise_error(rc);
+    }
+    PyObject* all = PyList_New(n);
+    for (int m = 0; m < n && all; ++m) {
+        PyObject* l = part_list(&res[m]);
+        if (!l) {
+            Py_DECREF(all);
+            all = NULL;
+            break;
+        }
+        PyList_SetItem(all, m, l);
+    }
+    for (int m = 0; m < n; ++m) gpu_result_free(&res[m]);
+    free(res);
+    return all;
+}
+
 static PyObject* m_test_postprocess_dc(PyObject* self, PyObject* args) {
     unsigned long long vp, tp, ovp, otp, onvp, ontp;
     int nv, nt, mc, mv, mt;
@@ -477,6 +538,7 @@ static PyMethodDef methods[] = {
     {"preprocess", m_preprocess, METH_VARARGS, "Not part of this library (raises)."},
     {"lookahead_decompose", (PyCFunction)(void (*)(void))m_lookahead_decompose, METH_VARARGS | METH_KEYWORDS,
      "Lookahead tree-search convex decomposition."},
+    {"lookahead_batch", m_lookahead_batch, METH_VARARGS, "Batched lookahead decomposition of packed meshes."},
     {"dev_alloc", m_dev_alloc, METH_VARARGS, "Device allocation -> address."},
     {"dev_free", m_dev_free, METH_VARARGS, "Free a device allocation."},
     {"memcpy_h2d", m_h2d, METH_VARARGS, "Host to device copy (dst, src, bytes)."},

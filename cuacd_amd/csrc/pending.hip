@@ -23,13 +23,4 @@ int gpu_test_postprocess_dc(gpu_ctx_t ctx, const float*, int, const int*, int, i
     return -1;
 }
 
-int lookahead_decompose(gpu_ctx_t ctx, const float*, int, const int*, int, const float*, int, const int*, int,
-                        int, int, int, float, int, int, int, int, int, int, int, int, float, float, int, int,
-                        struct gpu_result* out) {
-    out->parts = nullptr;
-    out->nparts = 0;
-    cuacd::set_error(ctx, "lookahead_decompose: not built yet");
-    return -1;
-}
-
 }  // extern "C"

@@ -183,45 +183,71 @@ __device__ inline double pc_volume(const float* v, const int* t, int nt, PcLDS* 
     return fabs(r) / 6.0;
 }
 
-// Ear-clip one simple polygon (indices into all-vertex space, CCW in 2D).
-// Executed by wave 0. Writes up to n-2 triangles to out_tris (vertex ids) and
-// returns the count, or -1 when no ear can be found (cap failure).
-__device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv, int* nxt, uint8_t* reflex,
+// Closed point-in-triangle test, either orientation (reference cuda/plane_cut.cuh:101-109).
+__device__ __forceinline__ bool pc_pt_in_tri(P2 X, P2 A, P2 B, P2 C) {
+    const float d1 = cross2(A, B, X), d2 = cross2(B, C, X), d3 = cross2(C, A, X);
+    return !((d1 < -1e-10f || d2 < -1e-10f || d3 < -1e-10f) && (d1 > 1e-10f || d2 > 1e-10f || d3 > 1e-10f));
+}
+
+// Ear-clip one weakly simple polygon (indices into all-vertex space, CCW in
+// 2D), executed by wave 0 (reference ear test, cuda/plane_cut.cuh:1195-1290):
+// cross < -1e-9 is not an ear; |cross| <= 1e-9 is clipped as a zero-area ear;
+// otherwise an ear has no other live vertex (farther than 1e-6 from its
+// corners) in its closed triangle, tested lane-parallel. The walk moves on to
+// the next vertex after a clip. After a lap without an ear, the least reflex
+// vertex is clipped when it is reflex only by rounding (a sliver: cross above
+// -1e-6 of the squared extent); otherwise the cap fails (-1). Writes n - 2
+// triangles (vertex ids) to out_tris and returns that count.
+__device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv, int* nxt, uint8_t* unused,
                                  int* out_tris) {
+    (void)unused;
     const int lane = lane_id();
     if (n < 3) return 0;
-    auto conv = [&](int p, int c, int q) { return cross2(uv[p], uv[c], uv[q]); };
+    float mnu = 3.4e38f, mxu = -3.4e38f, mnv = 3.4e38f, mxv = -3.4e38f;
     for (int i = lane; i < n; i += 64) {
+        mnu = fminf(mnu, uv[i].u); mxu = fmaxf(mxu, uv[i].u);
+        mnv = fminf(mnv, uv[i].v); mxv = fmaxf(mxv, uv[i].v);
         prv[i] = i == 0 ? n - 1 : i - 1;
         nxt[i] = i == n - 1 ? 0 : i + 1;
     }
-    __builtin_amdgcn_wave_barrier();
+    mnu = wave_min_f(mnu); mxu = wave_max_f(mxu); mnv = wave_min_f(mnv); mxv = wave_max_f(mxv);
+    const float ext = fmaxf(mxu - mnu, mxv - mnv);
+    const float sliver = -1e-6f * ext * ext;
     __threadfence_block();
-    for (int i = lane; i < n; i += 64) reflex[i] = conv(prv[i], i, nxt[i]) < -1e-9f ? 1 : 0;
-    __threadfence_block();
-    int nt = 0, remaining = n, cur = 0, miss = 0;
+    auto conv = [&](int p, int c, int q) { return cross2(uv[p], uv[c], uv[q]); };
+    int nt = 0, remaining = n, cur = 0, lap = 0;
     while (remaining > 3) {
-        if (miss > remaining) return -1;
-        const int p = prv[cur], q = nxt[cur];
+        int p = prv[cur], q = nxt[cur];
         const float cr = conv(p, cur, q);
         bool ear;
-        if (cr < -1e-9f) {
+        if (lap >= remaining) {
+            // a lap without an ear: least reflex vertex, first in ring order from cur
+            int best = cur, c = cur;
+            float bc = cr;
+            for (int k = 0; k < remaining; ++k) {
+                const float x = conv(prv[c], c, nxt[c]);
+                if (x > bc) { bc = x; best = c; }
+                c = nxt[c];
+            }
+            if (bc < sliver) return -1;
+            cur = best;
+            p = prv[cur];
+            q = nxt[cur];
+            ear = true;
+        } else if (cr < -1e-9f) {
             ear = false;
         } else if (cr <= 1e-9f) {
-            ear = true;  // collinear or duplicate: a zero-area ear keeps the cap closed
+            ear = true;
         } else {
-            // no remaining reflex vertex may lie inside (p, cur, q)
             const P2 A = uv[p], B = uv[cur], C = uv[q];
             bool bad = false;
             for (int i = lane; i < n; i += 64) {
-                if (i == p || i == cur || i == q || !reflex[i]) continue;
+                if (i == p || i == cur || i == q) continue;
                 if (nxt[prv[i]] != i) continue;  // removed
                 const P2 X = uv[i];
-                if ((fabsf(X.u - A.u) + fabsf(X.v - A.v) <= 1e-7f) || (fabsf(X.u - B.u) + fabsf(X.v - B.v) <= 1e-7f) ||
-                    (fabsf(X.u - C.u) + fabsf(X.v - C.v) <= 1e-7f))
-                    continue;
-                const float d1 = cross2(A, B, X), d2 = cross2(B, C, X), d3 = cross2(C, A, X);
-                if (d1 >= -1e-10f && d2 >= -1e-10f && d3 >= -1e-10f) bad = true;
+                if (fabsf(X.u - A.u) + fabsf(X.v - A.v) > 1e-6f && fabsf(X.u - B.u) + fabsf(X.v - B.v) > 1e-6f &&
+                    fabsf(X.u - C.u) + fabsf(X.v - C.v) > 1e-6f && pc_pt_in_tri(X, A, B, C))
+                    bad = true;
             }
             ear = !__any(bad);
         }
@@ -236,17 +262,11 @@ __device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv,
             __threadfence_block();
             ++nt;
             --remaining;
-            // neighbours may have stopped being reflex
-            if (lane == 0) {
-                reflex[p] = conv(prv[p], p, nxt[p]) < -1e-9f ? 1 : 0;
-                reflex[q] = conv(prv[q], q, nxt[q]) < -1e-9f ? 1 : 0;
-            }
-            __threadfence_block();
-            cur = p;  // the previous vertex may have become an ear
-            miss = 0;
+            cur = q;
+            lap = 0;
         } else {
             cur = q;
-            ++miss;
+            ++lap;
         }
     }
     if (remaining == 3) {
@@ -810,21 +830,44 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
                             const P2 a = puv[be_], b = puv[bn];
                             const P2 I{a.u + (M.v - a.v) / (b.v - a.v) * (b.u - a.u), M.v};
                             const P2 Pp = puv[pp];
+                            // boundary points count as inside (a vertex on the bridge must be
+                            // bridged to instead), within a tolerance of the triangle's size;
+                            // P itself competes, ties go to the nearest
+                            const float L = fmaxf(fmaxf(fabsf(Pp.u - M.u), fabsf(Pp.v - M.v)), I.u - M.u);
+                            const float tol = 1e-5f * L * L;
+                            uint32_t vt = 0xFFFFFFFFu;
                             uint64_t vk = ~0ull;
                             for (int i = lane; i < n; i += 64) {
                                 const P2 X = puv[i];
-                                if (i == pp || (X.u == Pp.u && X.v == Pp.v) || X.u <= M.u) continue;
+                                if (X.u <= M.u) continue;
                                 const float d1 = cross2(M, I, X), d2 = cross2(I, Pp, X), d3 = cross2(Pp, M, X);
-                                const bool in = (d1 >= 0.f && d2 >= 0.f && d3 >= 0.f) ||
-                                                (d1 <= 0.f && d2 <= 0.f && d3 <= 0.f);
+                                const bool in = (d1 >= -tol && d2 >= -tol && d3 >= -tol) ||
+                                                (d1 <= tol && d2 <= tol && d3 <= tol);
                                 if (!in) continue;
-                                // smallest angle to the ray = smallest |dv| / du (du > 0)
-                                const float tang = fabsf(X.v - M.v) / (X.u - M.u);
-                                const uint64_t kk = ((uint64_t)fkey(tang) << 32) | (uint32_t)i;
-                                vk = kk < vk ? kk : vk;
+                                // smallest angle to the ray = smallest |dv| / du (du > 0), then nearest
+                                const uint32_t k1 = fkey(fabsf(X.v - M.v) / (X.u - M.u));
+                                const uint64_t k2 = ((uint64_t)fkey(X.u - M.u) << 32) | (uint32_t)i;
+                                if (k1 < vt || (k1 == vt && k2 < vk)) { vt = k1; vk = k2; }
                             }
-                            vk = wave_min_u64(vk);
+                            const uint32_t vtm = (uint32_t)wave_min_u64(vt);
+                            vk = wave_min_u64(vt == vtm ? vk : ~0ull);
                             if (vk != ~0ull) pp = (int)(uint32_t)vk;
+                        }
+                        {
+                            // an earlier bridge may have duplicated the chosen vertex: take
+                            // the lowest copy whose interior wedge contains M
+                            const P2 Q = puv[pp];
+                            int best = 0x7fffffff;
+                            for (int i = lane; i < n; i += 64) {
+                                if (puv[i].u != Q.u || puv[i].v != Q.v) continue;
+                                const P2 A = puv[i == 0 ? n - 1 : i - 1], Bn = puv[i + 1 == n ? 0 : i + 1];
+                                bool in;
+                                if (cross2(A, Q, Bn) > 0.0f) in = cross2(Q, Bn, M) >= 0.0f && cross2(Q, M, A) >= 0.0f;
+                                else in = cross2(Q, A, M) <= 0.0f || cross2(Q, M, Bn) <= 0.0f;
+                                if (in) best = min(best, i);
+                            }
+                            best = (int)wave_min_u64((uint64_t)(uint32_t)best);
+                            if (best != 0x7fffffff) pp = best;
                         }
                         // splice: poly[0..pp], hole from mi around back to mi, poly[pp], poly[pp+1..]
                         const int nn = n + hs + 2;

@@ -133,45 +133,69 @@ static int opc_map_find(const opc_map* m, uint64_t key) {
     return -1;
 }
 
-/* ear clipping of one CCW polygon (ids poly[], coords uv[]); -1 on failure */
-static int opc_earclip(const int* poly, int n, const opc_p2* uv, int* prv, int* nxt, unsigned char* reflex,
+/* ear clipping of one CCW polygon (ids poly[], coords uv[]); -1 on failure.
+ * Reference semantics (cuda/plane_cut.cuh:1195-1290): a vertex with cross <
+ * -1e-9 is not an ear; a collinear one (|cross| <= 1e-9) is clipped as a
+ * zero-area ear; otherwise it is an ear when no other live vertex (farther
+ * than 1e-6 from the ear's corners) lies in the closed triangle; after a clip
+ * the walk continues at the next vertex. When a whole lap finds no ear, the
+ * least reflex vertex is clipped if it is only reflex by rounding (cross above
+ * -1e-6 of the squared extent): a sliver, not a failure. */
+static int opc_pt_in_tri(opc_p2 X, opc_p2 A, opc_p2 B, opc_p2 C) {
+    float d1 = opc_cross2(A, B, X), d2 = opc_cross2(B, C, X), d3 = opc_cross2(C, A, X);
+    return !((d1 < -1e-10f || d2 < -1e-10f || d3 < -1e-10f) && (d1 > 1e-10f || d2 > 1e-10f || d3 > 1e-10f));
+}
+static int opc_earclip(const int* poly, int n, const opc_p2* uv, int* prv, int* nxt, unsigned char* unused,
                        int* out) {
+    (void)unused;
     if (n < 3) return 0;
-#define CONV(p, c, q) opc_cross2(uv[p], uv[c], uv[q])
+    float mnu = uv[0].u, mxu = uv[0].u, mnv = uv[0].v, mxv = uv[0].v;
+    for (int i = 1; i < n; ++i) {
+        mnu = fminf(mnu, uv[i].u); mxu = fmaxf(mxu, uv[i].u);
+        mnv = fminf(mnv, uv[i].v); mxv = fmaxf(mxv, uv[i].v);
+    }
+    const float ext = fmaxf(mxu - mnu, mxv - mnv);
+    const float sliver = -1e-6f * ext * ext;
     for (int i = 0; i < n; ++i) { prv[i] = i == 0 ? n - 1 : i - 1; nxt[i] = i == n - 1 ? 0 : i + 1; }
-    for (int i = 0; i < n; ++i) reflex[i] = CONV(prv[i], i, nxt[i]) < -1e-9f;
-    int nt = 0, remaining = n, cur = 0, miss = 0;
+    int nt = 0, remaining = n, cur = 0, lap = 0;
     while (remaining > 3) {
-        if (miss > remaining) return -1;
         int p = prv[cur], q = nxt[cur];
-        float cr = CONV(p, cur, q);
+        float cr = opc_cross2(uv[p], uv[cur], uv[q]);
         int ear;
-        if (cr < -1e-9f) ear = 0;
-        else if (cr <= 1e-9f) ear = 1;
-        else {
-            opc_p2 A = uv[p], B = uv[cur], Cc = uv[q];
-            int bad = 0;
-            for (int i = 0; i < n && !bad; ++i) {
-                if (i == p || i == cur || i == q || !reflex[i]) continue;
-                if (nxt[prv[i]] != i) continue;
-                opc_p2 X = uv[i];
-                if ((fabsf(X.u - A.u) + fabsf(X.v - A.v) <= 1e-7f) || (fabsf(X.u - B.u) + fabsf(X.v - B.v) <= 1e-7f) ||
-                    (fabsf(X.u - Cc.u) + fabsf(X.v - Cc.v) <= 1e-7f))
-                    continue;
-                float d1 = opc_cross2(A, B, X), d2 = opc_cross2(B, Cc, X), d3 = opc_cross2(Cc, A, X);
-                if (d1 >= -1e-10f && d2 >= -1e-10f && d3 >= -1e-10f) bad = 1;
+        if (lap >= remaining) {
+            /* a full lap without an ear: clip the least reflex vertex if it is a sliver */
+            int best = cur, c = cur;
+            float bc = cr;
+            for (int k = 0; k < remaining; ++k) {
+                float x = opc_cross2(uv[prv[c]], uv[c], uv[nxt[c]]);
+                if (x > bc) { bc = x; best = c; }
+                c = nxt[c];
             }
-            ear = !bad;
+            if (bc < sliver) return -1;
+            cur = best; p = prv[cur]; q = nxt[cur];
+            ear = 1;
+        } else if (cr < -1e-9f) {
+            ear = 0;
+        } else if (cr <= 1e-9f) {
+            ear = 1;
+        } else {
+            opc_p2 A = uv[p], B = uv[cur], Cc = uv[q];
+            ear = 1;
+            for (int i = 0; i < n && ear; ++i) {
+                if (i == p || i == cur || i == q || nxt[prv[i]] != i) continue;
+                opc_p2 X = uv[i];
+                if (fabsf(X.u - A.u) + fabsf(X.v - A.v) > 1e-6f && fabsf(X.u - B.u) + fabsf(X.v - B.v) > 1e-6f &&
+                    fabsf(X.u - Cc.u) + fabsf(X.v - Cc.v) > 1e-6f && opc_pt_in_tri(X, A, B, Cc))
+                    ear = 0;
+            }
         }
         if (ear) {
             out[3 * nt] = poly[p]; out[3 * nt + 1] = poly[cur]; out[3 * nt + 2] = poly[q];
             nxt[p] = q; prv[q] = p;
             ++nt; --remaining;
-            reflex[p] = CONV(prv[p], p, nxt[p]) < -1e-9f;
-            reflex[q] = CONV(prv[q], q, nxt[q]) < -1e-9f;
-            cur = p; miss = 0;
+            cur = q; lap = 0;
         } else {
-            cur = q; ++miss;
+            cur = q; ++lap;
         }
     }
     if (remaining == 3) {
@@ -179,7 +203,6 @@ static int opc_earclip(const int* poly, int n, const opc_p2* uv, int* prv, int* 
         out[3 * nt] = poly[p]; out[3 * nt + 1] = poly[cur]; out[3 * nt + 2] = poly[q];
         ++nt;
     }
-#undef CONV
     return nt;
 }
 
@@ -495,18 +518,39 @@ int orc_plane_cut(const float* V, int nv, const int* T, int nt, float pa, float 
                         I.u = a.u + (M.v - a.v) / (b.v - a.v) * (b.u - a.u);
                         I.v = M.v;
                         opc_p2 Pp = wuv[pp];
-                        uint64_t vk = ~0ull;
+                        /* boundary points count as inside (a vertex on the bridge must be
+                           bridged to instead), within a tolerance of the triangle's size */
+                        float L = fmaxf(fmaxf(fabsf(Pp.u - M.u), fabsf(Pp.v - M.v)), I.u - M.u);
+                        float tol = 1e-5f * L * L;
+                        uint64_t vk = ~0ull, vt = ~0ull;
                         for (int i = 0; i < n; ++i) {
                             opc_p2 X = wuv[i];
-                            if (i == pp || (X.u == Pp.u && X.v == Pp.v) || X.u <= M.u) continue;
+                            if (X.u <= M.u) continue;  /* P itself competes: ties go to the nearest */
                             float d1 = opc_cross2(M, I, X), d2 = opc_cross2(I, Pp, X), d3 = opc_cross2(Pp, M, X);
-                            int in = (d1 >= 0.f && d2 >= 0.f && d3 >= 0.f) || (d1 <= 0.f && d2 <= 0.f && d3 <= 0.f);
+                            int in = (d1 >= -tol && d2 >= -tol && d3 >= -tol) || (d1 <= tol && d2 <= tol && d3 <= tol);
                             if (!in) continue;
+                            /* smallest angle to the ray, then nearest, then lowest index */
                             float tang = fabsf(X.v - M.v) / (X.u - M.u);
-                            uint64_t kk = ((uint64_t)opc_fkey(tang) << 32) | (uint32_t)i;
-                            if (kk < vk) vk = kk;
+                            uint64_t k1 = (uint64_t)opc_fkey(tang);
+                            uint64_t k2 = ((uint64_t)opc_fkey(X.u - M.u) << 32) | (uint32_t)i;
+                            if (k1 < vt || (k1 == vt && k2 < vk)) { vt = k1; vk = k2; }
                         }
                         if (vk != ~0ull) pp = (int)(uint32_t)vk;
+                    }
+                    {
+                        /* an earlier bridge may have duplicated the chosen vertex: take
+                           the lowest copy whose interior wedge contains M */
+                        opc_p2 Q = wuv[pp];
+                        int best = -1;
+                        for (int i = 0; i < n && best < 0; ++i) {
+                            if (wuv[i].u != Q.u || wuv[i].v != Q.v) continue;
+                            opc_p2 A = wuv[i == 0 ? n - 1 : i - 1], Bn = wuv[i + 1 == n ? 0 : i + 1];
+                            int in;
+                            if (opc_cross2(A, Q, Bn) > 0.0f) in = opc_cross2(Q, Bn, M) >= 0.0f && opc_cross2(Q, M, A) >= 0.0f;
+                            else in = opc_cross2(Q, A, M) <= 0.0f || opc_cross2(Q, M, Bn) <= 0.0f;
+                            if (in) best = i;
+                        }
+                        if (best >= 0) pp = best;
                     }
                     for (int i = n - 1; i > pp; --i) { wid[i + hs + 2] = wid[i]; wuv[i + hs + 2] = wuv[i]; }
                     for (int j = 0; j <= hs; ++j) {

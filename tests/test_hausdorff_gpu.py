@@ -67,7 +67,7 @@ def test_stratified_parity_noisy_icospheres(ctx):
     o_ba = oracle.directed_batch(sb, meshes)
     assert np.abs(dab - o_ab).max() <= ABS_TOL, np.abs(dab - o_ab).max()
     assert np.abs(dba - o_ba).max() <= ABS_TOL, np.abs(dba - o_ba).max()
-    assert (dba > dab).mean() > 0.9  # hull samples are the far side for concave meshes
+    assert (dab > dba).mean() > 0.9  # samples in the dents of the noisy mesh lie deepest inside the hull
 
 
 def test_stratified_samples_are_area_uniform(ctx):

@@ -1,0 +1,75 @@
+"""GPU tests of lookahead_decompose (reference tests/test_lookahead.py:124-187
+and the search semantics of csrc/lookahead.c / cuda/la_*.cu).
+
+Every run checks what any decomposition must satisfy: the parts are closed
+meshes, their volumes add up to the input's, and (unless the iteration cap
+was hit) every part's cost max(0.3 rv, H) is below the threshold.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cuacd_amd import datasets, sdf
+
+pytestmark = pytest.mark.gpu
+
+
+def _normalize(v):
+    lo, hi = v.min(0), v.max(0)
+    c = (lo + hi) / 2
+    s = (hi - lo).max() / 2
+    return ((v - c) / s).astype(np.float32)
+
+
+def _check_parts(parts, v, t, info=None, tau=0.05, rel=1e-4):
+    total = abs(oracle.mesh_volume(v, t))
+    vol = 0.0
+    for pv, pt, hv, ht in parts:
+        assert len(pt) >= 4 and len(ht) >= 4
+        assert oracle.edge_manifold_report(pt) == (0, 0)
+        vol += abs(oracle.mesh_volume(pv, pt))
+        # the hull encloses the part
+        assert abs(oracle.mesh_volume(hv, ht)) >= abs(oracle.mesh_volume(pv, pt)) * (1 - 1e-5)
+    assert abs(vol - total) <= rel * total, (vol, total)
+    if info is not None:
+        rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(info["hull_vol"] - info["mesh_vol"], 0))
+        cost = np.maximum(rv, info["hausdorff"])
+        return cost
+    return None
+
+
+def test_cube_is_one_part(ctx):
+    v, t = datasets.cube_mesh()
+    parts = ctx.lookahead_decompose(_normalize(v), t, max_iters=5, threshold=0.05)
+    assert len(parts) == 1
+
+
+def test_lshape(ctx):
+    v, t = datasets.l_shape_union()
+    v = _normalize(v)
+    parts = ctx.lookahead_decompose(v, t, max_iters=50, threshold=0.05)
+    assert 2 <= len(parts) <= 10
+    _check_parts(parts, v, t)
+
+
+def test_batch_matches_single_and_converges(ctx):
+    meshes = [(_normalize(v), t) for v, t in sdf.sdf_meshes(3, seed=7, res=24)]
+    meshes.append((_normalize(datasets.l_shape_union()[0]), datasets.l_shape_union()[1]))
+    kw = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, n_concave_edges=16)
+    batch, info = ctx.batch_lookahead_decompose(meshes, return_info=True, **kw)
+    for m, (v, t) in enumerate(meshes):
+        cost = _check_parts(batch[m], v, t, info[m])
+        if info[m]["iters"] < kw["max_iters"]:
+            assert (cost < kw["threshold"]).all(), (m, cost.max())
+        single = ctx.lookahead_decompose(v, t, **kw)
+        assert len(single) == len(batch[m])
+        for (a, b, _, _), (c, d, _, _) in zip(single, batch[m]):
+            assert a.tobytes() == c.tobytes() and np.array_equal(b, d)
+
+
+def test_heap_balanced(ctx):
+    before = ctx.heap_stats()
+    v, t = datasets.hollow_tube()
+    ctx.lookahead_decompose(_normalize(v), t, max_iters=8, width=12, width2=3, n_concave_edges=4)
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)
