@@ -93,6 +93,7 @@ __device__ __forceinline__ float pt_tri_d2(V3 p, V3 a, V3 b, V3 c) {
 }
 
 __device__ __forceinline__ float tri_area(V3 a, V3 b, V3 c) {
+#pragma clang fp contract(off)
     const V3 ab = v3sub(b, a), ac = v3sub(c, a);
     const float nx = ab.y * ac.z - ab.z * ac.y;
     const float ny = ab.z * ac.x - ab.x * ac.z;

@@ -80,7 +80,10 @@ __device__ float block_incl_scan_area(int n, float* out, HdLDS* s, F f) {
     return (float)tot;
 }
 
+// Sample point from two hashes (evaluated as written, no FMA contraction, so
+// that a serial C evaluation reproduces the point bit for bit).
 __device__ __forceinline__ V3 bary(V3 a, V3 b, V3 c, uint32_t ha, uint32_t hb) {
+#pragma clang fp contract(off)
     const float ra = (float)(ha & 0xFFFFFFu) / (float)0xFFFFFFu;
     const float rb = (float)(hb & 0xFFFFFFu) / (float)0xFFFFFFu;
     const float sqa = sqrtf(ra);
@@ -117,14 +120,18 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
     float local = 0.0f;
     int ns = 0;
     if (tk.mode == 0) {
-        // reference sampler: total area, per-triangle counts, exclusive prefix
-        float ta = 0.0f;
-        for (int k = tid; k < src.nt; k += HD_T) ta += area_of(k);
-        ta = wave_sum_f(ta);
-        if (lane_id() == 0) s.red[wv][7] = ta;
+        // reference sampler: total area (FP64 sum of the fp32 triangle areas,
+        // rounded once, so the counts do not depend on the summation order),
+        // per-triangle counts, exclusive prefix
+        double ta = 0.0;
+        for (int k = tid; k < src.nt; k += HD_T) ta += (double)area_of(k);
+        ta = wave_sum_d(ta);
+        if (lane_id() == 0) s.dred[wv] = ta;
         __syncthreads();
-        float area = 0.0f;
-        for (int q = 0; q < HD_T / 64; ++q) area += s.red[q][7];
+        double tsum = 0.0;
+        for (int q = 0; q < HD_T / 64; ++q) tsum += s.dred[q];
+        __syncthreads();
+        const float area = (float)tsum;
         int res = (int)((float)HD_RESOLUTION * area);
         if (res < HD_MIN_SAMPLES) res = HD_MIN_SAMPLES;
         const int nt = src.nt;

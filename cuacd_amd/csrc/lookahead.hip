@@ -797,6 +797,14 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     REQUIRE(P.width + 4 * P.n_concave < 512, "lookahead: width + 4*n_concave_edges must be < 512");
     REQUIRE(P.max_n_cutting >= 1, "lookahead: max_n_cutting must be positive");
     REQUIRE(!P.merge, "lookahead: merge_hulls is not built yet");
+    // every index must name a vertex of its own mesh (the kernels trust them)
+    for (int m = 0; m < n_meshes; ++m) {
+        const int nvm = voff[m + 1] - voff[m], nhm = hvoff[m + 1] - hvoff[m];
+        for (long long k = 3ll * toff[m]; k < 3ll * toff[m + 1]; ++k)
+            REQUIRE(tris[k] >= 0 && tris[k] < nvm, "lookahead: mesh %d has a triangle index out of range", m);
+        for (long long k = 3ll * htoff[m]; k < 3ll * htoff[m + 1]; ++k)
+            REQUIRE(htris[k] >= 0 && htris[k] < nhm, "lookahead: hull %d has a triangle index out of range", m);
+    }
     Engine E;
     E.ctx = ctx;
     E.P = P;

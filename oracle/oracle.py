@@ -146,13 +146,16 @@ def directed_batch(samples, meshes):
 
 def reference_sampler_counts(verts, tris):
     """Per-triangle sample counts of the reference Hausdorff sampler
-    (cuda/hausdorff.cuh:380-382, 432-466), evaluated in float32."""
+    (cuda/hausdorff.cuh:380-382, 432-466), evaluated in float32 except for the
+    total area, an FP64 sum of the float32 triangle areas rounded once (the
+    reference sums in float32 in an order of its own; a rounded FP64 sum is
+    what a GPU reduction and a serial loop can both reproduce)."""
     v = np.asarray(verts, np.float32)
     t = np.asarray(tris, np.int64)
     a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
     n = np.cross(b - a, c - a).astype(np.float32)
     area_t = (np.float32(0.5) * np.sqrt((n * n).sum(1, dtype=np.float32))).astype(np.float32)
-    area = np.float32(area_t.sum(dtype=np.float32))
+    area = np.float32(area_t.astype(np.float64).sum())  # FP64 sum, rounded once
     res = max(int(np.float32(2000.0) * area), 1000)
     nt = len(t)
     step = nt // res if nt > res else 2

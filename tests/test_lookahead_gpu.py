@@ -9,6 +9,8 @@ import numpy as np
 import pytest
 
 import oracle
+import os, sys
+sys.path.insert(0, os.path.dirname(__file__))
 from cuacd_amd import datasets, sdf
 
 pytestmark = pytest.mark.gpu
@@ -73,3 +75,35 @@ def test_heap_balanced(ctx):
     ctx.lookahead_decompose(_normalize(v), t, max_iters=8, width=12, width2=3, n_concave_edges=4)
     after = ctx.heap_stats()
     assert after[0] == before[0] and after[3] == before[3], (before, after)
+
+
+def _rotated(v, seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    return (v.astype(np.float64) @ q.T).astype(np.float32)
+
+
+def test_parity_with_serial_oracle(ctx):
+    """The GPU search makes the oracle's decisions: same iterations, same
+    parts (vertex bits and triangles) on general-position inputs."""
+    import lookahead_oracle as LO
+    cases = [(_rotated(datasets.l_shape_union()[0], 1), datasets.l_shape_union()[1]),
+             (_rotated(datasets.hollow_tube()[0], 2), datasets.hollow_tube()[1])]
+    cases += sdf.sdf_meshes(2, seed=11, res=16)
+    cases.append(datasets.noisy_icospheres(1, 4, 0.1, seed=3)[0])
+    kw = dict(max_iters=12, width=12, width2=3, threshold=0.05, depth=2, n_concave_edges=4, max_n_cutting=4)
+    for ci, (v, t) in enumerate(cases):
+        v = _normalize(v)
+        if ci == len(cases) - 1:
+            kw = dict(kw, max_iters=3)
+        hv, ht = ctx._input_hull(v)
+        gpu, info = ctx.batch_lookahead_decompose([(v, t)], return_info=True, **kw)
+        ref, iters = LO.lookahead(v, t, hv, ht, **kw)
+        assert info[0]["iters"] == iters, (ci, info[0]["iters"], iters)
+        assert len(gpu[0]) == len(ref), (ci, len(gpu[0]), len(ref))
+        key = lambda p: (len(p[1]), p[0].tobytes())
+        g = sorted(gpu[0], key=key)
+        r = sorted(ref, key=key)
+        for (gv, gt, ghv, ght), rp in zip(g, r):
+            assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
+            assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-9 * max(rp[5], 1e-12) + 1e-12, ci

@@ -1,0 +1,80 @@
+"""CPU checks of the lookahead oracle's pieces (leaf scoring, concave edges,
+k-DOP axes, cost) and of the oracle search itself on small solids."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import lookahead_oracle as LO
+from cuacd_amd import datasets
+
+
+def _norm(v):
+    lo, hi = v.min(0), v.max(0)
+    return ((v - (lo + hi) / 2) / ((hi - lo).max() / 2)).astype(np.float32)
+
+
+def test_cost_closed_form():
+    # rv of a volume gap g is the radius of the sphere of volume g
+    g = 4.0 / 3.0 * math.pi * 0.2 ** 3
+    assert abs(float(LO.cost_rv(1.0 + g, 1.0)) - 0.3 * 0.2) < 1e-7
+    assert LO.cost_rv(1.0, 2.0) == 0.0
+
+
+def test_kdop_axes():
+    A = LO.kdop_axes().astype(np.float64)
+    assert A.shape == (40, 3)
+    assert np.allclose(np.linalg.norm(A, axis=1), 1.0, atol=1e-6)
+    # no two axes parallel or antiparallel
+    G = np.abs(A @ A.T) - np.eye(40)
+    assert G.max() < 0.999
+    # the 8 octant directions of the icosahedron's face normals are among them
+    oct_ = np.abs(A) @ np.ones(3) / math.sqrt(3)
+    assert (np.abs(oct_ - 1.0) < 1e-6).sum() == 4
+
+
+@pytest.mark.parametrize("name,plane", [("x", (1, 0, 0, 0.1)), ("tilt", (0.3, -0.2, 0.9, 0.05)),
+                                        ("diag", (0.7071, 0.7071, 0, -0.2))])
+def test_score_cut_matches_capped_cut(name, plane):
+    """Leaf scoring (no caps) gives the capped halves' volumes and the hull
+    point sets whose hulls are the halves' hulls."""
+    for v, t in (datasets.hollow_tube(), datasets.noisy_icospheres(1, 3, 0.15, seed=2)[0]):
+        pl = np.float32(plane)
+        pvol, nvol, npc, nnc, ppts, npts = LO.score_cut(v, t, pl)
+        ref_p, ref_n = oracle.halfspace_volumes(v, t, pl)
+        assert abs(pvol - ref_p) < 1e-6 * (ref_p + ref_n) and abs(nvol - ref_n) < 1e-6 * (ref_p + ref_n)
+        st, pv, pt, nv, nt = oracle.plane_cut(v, t, pl)
+        assert st == 0
+        # same point sets as the capped halves' vertex sets
+        assert sorted(map(tuple, ppts.tolist())) == sorted(map(tuple, pv.tolist()))
+        assert sorted(map(tuple, npts.tolist())) == sorted(map(tuple, nv.tolist()))
+        assert abs(LO.hull_volume(ppts)[0] - LO.hull_volume(pv)[0]) < 1e-12
+
+
+def test_concave_edges_of_l_shape():
+    v, t = datasets.l_shape_union()
+    p = LO.Part(v, t, 3.0)
+    planes = LO.concave_planes(p, 4, 0.005, float(np.float32(3.49)))
+    # one concave edge (the notch, dihedral 3*pi/2): its two face planes and
+    # the two offset bisector planes
+    assert len(planes) == 4
+    normals = sorted(tuple(np.round(q[:3], 4)) for q in planes)
+    assert normals.count((0.7071, 0.7071, 0.0)) + normals.count((-0.7071, -0.7071, 0.0)) == 2
+    box = datasets.box_mesh()
+    assert LO.concave_planes(LO.Part(*box, 1.0), 4, 0.005, 3.49) == []
+
+
+def test_oracle_search_l_shape_and_tube():
+    for v, t in (datasets.l_shape_union(), datasets.hollow_tube()):
+        v = _norm(v)
+        hv, ht, _ = LO.hull_mesh(v)
+        parts, iters = LO.lookahead(v, t, hv, ht, max_iters=20, width=12, width2=3, n_concave_edges=4)
+        assert len(parts) >= 2
+        total = abs(oracle.mesh_volume(v, t))
+        vol = sum(abs(oracle.mesh_volume(p[0], p[1])) for p in parts)
+        assert abs(vol - total) < 1e-5 * total
+        for p in parts:
+            assert oracle.edge_manifold_report(p[1]) == (0, 0)
+            if iters < 20:
+                assert max(LO.cost_rv(p[5], p[4]), p[6]) < np.float32(0.05)
