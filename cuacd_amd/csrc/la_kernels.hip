@@ -913,6 +913,49 @@ __global__ __launch_bounds__(PC_T) void la_bbox_kernel(LaBoxTask* tasks, int n) 
         for (int j = 0; j < 3; ++j) { T.lo[j] = lo[j]; T.hi[j] = hi[j]; }
 }
 
+__global__ __launch_bounds__(PC_T) void la_gather_kernel(LaGatherTask* tasks, int n) {
+    __shared__ int wc[PC_T / 64];
+    if ((int)blockIdx.x >= n) return;
+    LaGatherTask& T = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 3 * T.n1; i += PC_T) T.dv[i] = T.v1[i];
+    for (int i = tid; i < 3 * T.n2; i += PC_T) T.dv[3 * T.n1 + i] = T.v2[i];
+    if (!T.dt) return;
+    const float pa = T.plane[0], pb = T.plane[1], pc = T.plane[2], pd = T.plane[3];
+    auto on_plane = [&](const float* V, int i) {
+        const float s = pa * V[3 * i] + pb * V[3 * i + 1] + pc * V[3 * i + 2] - pd;
+        return fabsf(s) <= 1e-3f;
+    };
+    int base = 0;
+    for (int part = 0; part < 2; ++part) {
+        const float* V = part ? T.v2 : T.v1;
+        const int* Tr = part ? T.t2 : T.t1;
+        const int m = part ? T.m2 : T.m1;
+        const int off = part ? T.n1 : 0;
+        for (int b0 = 0; b0 < m; b0 += PC_T) {
+            const int k = b0 + tid;
+            bool keep = false;
+            int a = 0, b = 0, c = 0;
+            if (k < m) {
+                a = Tr[3 * k]; b = Tr[3 * k + 1]; c = Tr[3 * k + 2];
+                keep = !(T.has_plane && on_plane(V, a) && on_plane(V, b) && on_plane(V, c));
+            }
+            const uint64_t msk = ballot(keep);
+            if (lane_id() == 0) wc[tid >> 6] = __popcll(msk);
+            __syncthreads();
+            int o = base;
+            for (int q = 0; q < (tid >> 6); ++q) o += wc[q];
+            if (keep) {
+                o += mask_rank(msk);
+                T.dt[3 * o] = a + off; T.dt[3 * o + 1] = b + off; T.dt[3 * o + 2] = c + off;
+            }
+            for (int q = 0; q < PC_T / 64; ++q) base += wc[q];
+            __syncthreads();
+        }
+    }
+    if (tid == 0) T.nt_out = base;
+}
+
 __global__ void la_free_kernel(void** ptrs, int n, DPool* pool, int* err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && ptrs[i]) heap_free(pool, ptrs[i], err);
@@ -1140,6 +1183,13 @@ int la_launch_edges(gpu_ctx_t ctx, LaEdgeTask* d_tasks, int n, float cos_pos, fl
 int la_launch_bbox(gpu_ctx_t ctx, LaBoxTask* d_tasks, int n) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(la_bbox_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_gather(gpu_ctx_t ctx, LaGatherTask* d_tasks, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_gather_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -21,6 +21,7 @@
 // appended in cutting order, so runs are deterministic.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 #include "lookahead.hpp"
@@ -394,6 +395,274 @@ struct Engine {
         LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         return check_err("item hulls");
+    }
+
+    std::vector<void*> ext_bufs;  // hipMallocAsync buffers owned by merged parts
+
+    // ---- merge pass (reference cuda/postprocess_merge.cu, csrc/lookahead.c:988-1067)
+    // One round: every pair of parts of a mesh is scored by the rv cost of its
+    // merged hull (after volume and box rejects), then by max(rv, Hausdorff of
+    // the concatenated mesh - minus triangles on a shared plane - against the
+    // merged hull); pairs under the threshold are matched greedily by cost and
+    // merged. Pair scoring runs on the GPU for all meshes at once.
+    int merge(const std::vector<int>& meshes) {
+        const float tau = P.threshold;
+        struct MPair {
+            int m, p1, p2;
+            float cost;
+            int hull_task;  // index into the hull tasks, -1 if rejected before
+            float rv;
+        };
+        std::vector<MPair> pairs;
+        std::vector<LaGatherTask> pg;       // hull-point concatenations
+        size_t pts_total = 0;
+        for (int m : meshes) {
+            MeshState& S = M[m];
+            const int np = (int)S.parts.size();
+            for (int p1 = 1; p1 < np; ++p1)
+                for (int p2 = 0; p2 < p1; ++p2) {
+                    const Part& A = S.parts[p1];
+                    const Part& B = S.parts[p2];
+                    MPair q{m, p1, p2, 1e20f, -1, 0.f};
+                    if (A.nv == 0 || B.nv == 0 || !A.v || !B.v || !A.hv || !B.hv) { pairs.push_back(q); continue; }
+                    if (A.cc != 0 && B.cc != 0 && A.cc != B.cc) { pairs.push_back(q); continue; }
+                    const double msum = A.mvol + B.mvol;
+                    if (cost_rv(A.hvol + B.hvol, msum) >= tau) { q.cost = 1e10f; pairs.push_back(q); continue; }
+                    float g2 = 0.f;
+                    {
+                        const float gx = std::max(0.0f, std::max(A.lo[0] - B.hi[0], B.lo[0] - A.hi[0]));
+                        const float gy = std::max(0.0f, std::max(A.lo[1] - B.hi[1], B.lo[1] - A.hi[1]));
+                        const float gz = std::max(0.0f, std::max(A.lo[2] - B.hi[2], B.lo[2] - A.hi[2]));
+                        g2 = gx * gx + gy * gy + gz * gz;
+                    }
+                    const float half = tau * 0.5f;
+                    if (g2 >= half * half) { q.cost = 1e9f; pairs.push_back(q); continue; }
+                    q.hull_task = (int)pg.size();
+                    LaGatherTask gt{};
+                    gt.v1 = A.hv; gt.n1 = A.hnv; gt.v2 = B.hv; gt.n2 = B.hnv;
+                    gt.dv = (float*)(uintptr_t)pts_total;  // offset, patched below
+                    pts_total += (size_t)(A.hnv + B.hnv);
+                    pg.push_back(gt);
+                    pairs.push_back(q);
+                }
+        }
+        std::vector<LaHullTask> ht(pg.size());
+        DevBuf<float> cat;
+        if (!pg.empty()) {
+            REQUIRE(cat.alloc(ctx, 3 * pts_total + 3) == 0, "lookahead: merge buffers allocation failed");
+            for (size_t i = 0; i < pg.size(); ++i) {
+                pg[i].dv = cat.p + 3 * (size_t)(uintptr_t)pg[i].dv;
+                ht[i] = LaHullTask{};
+                ht[i].pts = pg[i].dv;
+                ht[i].n = pg[i].n1 + pg[i].n2;
+                ht[i].emit = 1;
+                ht[i].prune_gap = 1e30;
+            }
+            DevBuf<LaGatherTask> dg;
+            REQUIRE(dg.alloc(ctx, pg.size()) == 0, "lookahead: merge buffers allocation failed");
+            HIP_TRY(hipMemcpyAsync(dg.p, pg.data(), sizeof(LaGatherTask) * pg.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            LA_TRY(la_launch_gather(ctx, dg.p, (int)pg.size()));
+            LA_TRY(hull_volumes(ht));
+        }
+        // rv cost of the merged hull; the survivors go on to the Hausdorff stage
+        std::vector<int> hd_pairs;
+        for (size_t i = 0; i < pairs.size(); ++i) {
+            MPair& q = pairs[i];
+            if (q.hull_task < 0) continue;
+            const LaHullTask& h = ht[q.hull_task];
+            const Part& A = M[q.m].parts[q.p1];
+            const Part& B = M[q.m].parts[q.p2];
+            if (!h.hv) { q.cost = 1e20f; continue; }
+            q.rv = cost_rv(h.vol, A.mvol + B.mvol);
+            q.cost = q.rv;
+            if (q.rv >= tau) { to_free.push_back(h.hv); ht[q.hull_task].hv = nullptr; continue; }
+            hd_pairs.push_back((int)i);
+        }
+        if (!hd_pairs.empty()) {
+            // host copies of the parts' hulls for the shared-plane search
+            std::vector<LaGatherTask> mg(hd_pairs.size());
+            size_t vtot = 0, ttot = 0;
+            for (size_t k = 0; k < hd_pairs.size(); ++k) {
+                const MPair& q = pairs[hd_pairs[k]];
+                const Part& A = M[q.m].parts[q.p1];
+                const Part& B = M[q.m].parts[q.p2];
+                LaGatherTask g{};
+                g.v1 = A.v; g.n1 = A.nv; g.v2 = B.v; g.n2 = B.nv;
+                g.t1 = A.t; g.m1 = A.nt; g.t2 = B.t; g.m2 = B.nt;
+                float pl[4];
+                g.has_plane = shared_plane(A, B, pl);
+                for (int j = 0; j < 4; ++j) g.plane[j] = pl[j];
+                g.dv = (float*)(uintptr_t)vtot;
+                g.dt = (int*)(uintptr_t)ttot;
+                vtot += (size_t)(A.nv + B.nv);
+                ttot += (size_t)(A.nt + B.nt);
+                mg[k] = g;
+            }
+            DevBuf<float> mv;
+            DevBuf<int> mt;
+            DevBuf<LaGatherTask> dg;
+            REQUIRE(mv.alloc(ctx, 3 * vtot + 3) == 0 && mt.alloc(ctx, 3 * ttot + 3) == 0 &&
+                        dg.alloc(ctx, mg.size()) == 0,
+                    "lookahead: merge buffers allocation failed");
+            for (auto& g : mg) {
+                g.dv = mv.p + 3 * (size_t)(uintptr_t)g.dv;
+                g.dt = mt.p + 3 * (size_t)(uintptr_t)g.dt;
+            }
+            HIP_TRY(hipMemcpyAsync(dg.p, mg.data(), sizeof(LaGatherTask) * mg.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            LA_TRY(la_launch_gather(ctx, dg.p, (int)mg.size()));
+            HIP_TRY(hipMemcpyAsync(mg.data(), dg.p, sizeof(LaGatherTask) * mg.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            LA_TRY(check_err("merge gather"));
+            std::vector<HdTask> hd;
+            unsigned long long ws = 0;
+            for (size_t k = 0; k < hd_pairs.size(); ++k) {
+                const LaHullTask& h = ht[pairs[hd_pairs[k]].hull_task];
+                const LaGatherTask& g = mg[k];
+                const int cnv = g.n1 + g.n2, cnt = g.nt_out;
+                HdTask a{g.dv, g.dt, h.hv, h.ht, cnv, cnt, h.hnv, h.hnt, ws, 0, 0, nullptr, nullptr, nullptr};
+                ws += (hd_ws_bytes(h.hnt, cnt) + 255ull) & ~255ull;
+                HdTask b{h.hv, h.ht, g.dv, g.dt, h.hnv, h.hnt, cnv, cnt, ws, 0, 0, nullptr, nullptr, nullptr};
+                ws += (hd_ws_bytes(cnt, h.hnt) + 255ull) & ~255ull;
+                hd.push_back(a);
+                hd.push_back(b);
+            }
+            DevBuf<float> d2;
+            DevBuf<HdTask> dh;
+            DevBuf<char> dws;
+            REQUIRE(d2.alloc(ctx, hd.size()) == 0 && dh.alloc(ctx, hd.size()) == 0 && dws.alloc(ctx, ws + 256) == 0,
+                    "lookahead: merge Hausdorff buffers allocation failed");
+            for (size_t k = 0; k < hd.size(); ++k) hd[k].out_d2 = d2.p + k;
+            HIP_TRY(hipMemcpyAsync(dh.p, hd.data(), sizeof(HdTask) * hd.size(), hipMemcpyHostToDevice, ctx->stream));
+            LA_TRY(hd_launch_tasks(ctx, dh.p, (int)hd.size(), dws.p));
+            std::vector<float> r(hd.size());
+            HIP_TRY(hipMemcpyAsync(r.data(), d2.p, 4 * hd.size(), hipMemcpyDeviceToHost, ctx->stream));
+            LA_TRY(check_err("merge hausdorff"));
+            for (size_t k = 0; k < hd_pairs.size(); ++k) {
+                MPair& q = pairs[hd_pairs[k]];
+                // an empty concatenation (every triangle on the shared plane) scores 0
+                const float h = mg[k].nt_out > 0 ? std::sqrt(std::max(r[2 * k], r[2 * k + 1])) : 0.0f;
+                q.cost = std::max(q.rv, h);
+                if (q.cost >= tau) {
+                    to_free.push_back(ht[q.hull_task].hv);
+                    ht[q.hull_task].hv = nullptr;
+                }
+            }
+        }
+        // greedy matching per mesh: repeatedly the cheapest pair under the
+        // threshold (lowest pair index on ties), then drop its two parts
+        size_t b = 0;
+        while (b < pairs.size()) {
+            const int m = pairs[b].m;
+            size_t e = b;
+            while (e < pairs.size() && pairs[e].m == m) ++e;
+            MeshState& S = M[m];
+            const int np = (int)S.parts.size();
+            std::vector<char> used(np, 0);
+            std::vector<std::pair<int, int>> match;
+            for (;;) {
+                float bc = tau;
+                size_t bi = SIZE_MAX;
+                for (size_t i = b; i < e; ++i) {
+                    const MPair& q = pairs[i];
+                    if (used[q.p1] || used[q.p2]) continue;
+                    if (q.cost < bc) { bc = q.cost; bi = i; }
+                }
+                if (bi == SIZE_MAX) break;
+                used[pairs[bi].p1] = used[pairs[bi].p2] = 1;
+                match.push_back({(int)bi, 0});
+            }
+            // apply: merged mesh = A's then B's vertices and triangles; hull = merged hull
+            std::vector<LaGatherTask> ag;
+            for (auto& mm : match) {
+                const MPair& q = pairs[mm.first];
+                const Part& A = S.parts[q.p1];
+                const Part& B = S.parts[q.p2];
+                LaGatherTask g{};
+                g.v1 = A.v; g.n1 = A.nv; g.v2 = B.v; g.n2 = B.nv;
+                g.t1 = A.t; g.m1 = A.nt; g.t2 = B.t; g.m2 = B.nt;
+                void* buf = nullptr;
+                HIP_TRY(hipMallocAsync(&buf, 12ull * (A.nv + B.nv) + 16 + 12ull * (A.nt + B.nt) + 16, ctx->stream));
+                ext_bufs.push_back(buf);
+                g.dv = (float*)buf;
+                g.dt = (int*)((char*)buf + ((12ull * (A.nv + B.nv) + 15) & ~15ull));
+                ag.push_back(g);
+            }
+            if (!ag.empty()) {
+                DevBuf<LaGatherTask> dg;
+                REQUIRE(dg.alloc(ctx, ag.size()) == 0, "lookahead: merge buffers allocation failed");
+                HIP_TRY(hipMemcpyAsync(dg.p, ag.data(), sizeof(LaGatherTask) * ag.size(), hipMemcpyHostToDevice,
+                                       ctx->stream));
+                LA_TRY(la_launch_gather(ctx, dg.p, (int)ag.size()));
+                LA_TRY(check_err("merge apply"));
+            }
+            std::vector<char> dead(np, 0);
+            for (size_t k = 0; k < match.size(); ++k) {
+                MPair& q = pairs[match[k].first];
+                Part& A = S.parts[q.p1];
+                Part& B = S.parts[q.p2];
+                LaHullTask& h = ht[q.hull_task];
+                Part mp;
+                mp.v = ag[k].dv; mp.t = ag[k].dt; mp.nv = A.nv + B.nv; mp.nt = A.nt + B.nt;
+                mp.hv = h.hv; mp.ht = h.ht; mp.hnv = h.hnv; mp.hnt = h.hnt; mp.own_hull = true;
+                mp.mvol = A.mvol + B.mvol;
+                mp.hvol = h.vol;
+                mp.haus = 0.0f;
+                mp.cc = A.cc ? A.cc : B.cc;
+                mp.own = false;  // an ext buffer, freed with the engine
+                for (int j = 0; j < 3; ++j) { mp.lo[j] = std::min(A.lo[j], B.lo[j]); mp.hi[j] = std::max(A.hi[j], B.hi[j]); }
+                h.hv = nullptr;  // adopted
+                drop_part(A);
+                drop_part(B);
+                A = mp;
+                dead[q.p2] = 1;
+            }
+            std::vector<Part> keep;
+            for (int i = 0; i < np; ++i)
+                if (!dead[i]) keep.push_back(S.parts[i]);
+            S.parts.swap(keep);
+            b = e;
+        }
+        // merged hulls of pairs not chosen
+        for (auto& h : ht)
+            if (h.hv) to_free.push_back(h.hv);
+        return flush_frees();
+    }
+
+    // Shared plane (reference cuda/merge.cuh:111-165): the first face of A's
+    // hull with A's hull on its inner side and B's hull within 1e-3 of it.
+    bool shared_plane(const Part& A, const Part& B, float* out) {
+        std::vector<float> av(3ull * A.hnv), bv(3ull * B.hnv);
+        std::vector<int> at(3ull * A.hnt);
+        if (A.hnv) (void)hipMemcpy(av.data(), A.hv, 12ull * A.hnv, hipMemcpyDeviceToHost);
+        if (A.hnt) (void)hipMemcpy(at.data(), A.ht, 12ull * A.hnt, hipMemcpyDeviceToHost);
+        if (B.hnv) (void)hipMemcpy(bv.data(), B.hv, 12ull * B.hnv, hipMemcpyDeviceToHost);
+        const float tol = 1e-3f;
+        for (int t = 0; t < A.hnt; ++t) {
+            const float* a = &av[3 * at[3 * t]];
+            const float* bq = &av[3 * at[3 * t + 1]];
+            const float* c = &av[3 * at[3 * t + 2]];
+            const float e1x = bq[0] - a[0], e1y = bq[1] - a[1], e1z = bq[2] - a[2];
+            const float e2x = c[0] - a[0], e2y = c[1] - a[1], e2z = c[2] - a[2];
+            float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+            const float len = std::sqrt(nx * nx + ny * ny + nz * nz);
+            if (len < 1e-20f) continue;
+            const float inv = 1.0f / len;
+            nx *= inv; ny *= inv; nz *= inv;
+            const float d = nx * a[0] + ny * a[1] + nz * a[2];
+            bool ok = true;
+            for (int j = 0; j < A.hnv && ok; ++j)
+                if (nx * av[3 * j] + ny * av[3 * j + 1] + nz * av[3 * j + 2] - d > tol) ok = false;
+            for (int j = 0; j < B.hnv && ok; ++j)
+                if (std::fabs(nx * bv[3 * j] + ny * bv[3 * j + 1] + nz * bv[3 * j + 2] - d) > tol) ok = false;
+            if (ok) {
+                out[0] = nx; out[1] = ny; out[2] = nz; out[3] = d;
+                return true;
+            }
+        }
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return false;
     }
 
     // ---- one search iteration over the meshes in `act`; returns meshes still active
@@ -796,7 +1065,6 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     REQUIRE(P.width >= 1 && P.width2 >= 1, "lookahead: width and width2 must be positive");
     REQUIRE(P.width + 4 * P.n_concave < 512, "lookahead: width + 4*n_concave_edges must be < 512");
     REQUIRE(P.max_n_cutting >= 1, "lookahead: max_n_cutting must be positive");
-    REQUIRE(!P.merge, "lookahead: merge_hulls is not built yet");
     // every index must name a vertex of its own mesh (the kernels trust them)
     for (int m = 0; m < n_meshes; ++m) {
         const int nvm = voff[m + 1] - voff[m], nhm = hvoff[m + 1] - hvoff[m];
@@ -877,6 +1145,11 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         rc = E.components(all);
         if (rc == 0) rc = E.hull_parts(all);
     }
+    if (rc == 0 && P.merge) {
+        std::vector<int> all;
+        for (int m = 0; m < n_meshes; ++m) all.push_back(m);
+        rc = E.merge(all);
+    }
     // results
     if (rc == 0) {
         for (int m = 0; m < n_meshes && rc == 0; ++m) {
@@ -915,6 +1188,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
             p.hv = nullptr;
         }
     const int frc = E.flush_frees();
+    for (void* b : E.ext_bufs) (void)hipFreeAsync(b, ctx->stream);
     HIP_TRY(hipFreeAsync(d_v, ctx->stream));
     HIP_TRY(hipFreeAsync(d_t, ctx->stream));
     HIP_TRY(hipFreeAsync(d_hv, ctx->stream));

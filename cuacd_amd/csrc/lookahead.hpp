@@ -103,7 +103,29 @@ struct LaBoxTask {
     float lo[3], hi[3];
 };
 
+// Concatenation of two meshes (or two point sets when the triangle arrays are
+// null): vertices of the first then the second; triangles of the first, then
+// those of the second offset by n1, each dropped when all three of its
+// vertices lie within 1e-3 of `plane` (has_plane) - the merge pass's
+// shared-plane filter (reference cuda/postprocess_merge.cu:264-330).
+struct LaGatherTask {
+    const float* v1;
+    int n1;
+    const float* v2;
+    int n2;
+    const int* t1;
+    int m1;
+    const int* t2;
+    int m2;
+    float plane[4];
+    int has_plane;
+    float* dv;
+    int* dt;
+    int nt_out;  // result
+};
+
 // ---- launchers (la_kernels.hip); all stream-ordered on ctx->stream --------------
+int la_launch_gather(gpu_ctx_t ctx, LaGatherTask* d_tasks, int n);
 int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n);
 int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n);
 int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n);

@@ -107,3 +107,21 @@ def test_parity_with_serial_oracle(ctx):
         for (gv, gt, ghv, ght), rp in zip(g, r):
             assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
             assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-9 * max(rp[5], 1e-12) + 1e-12, ci
+
+
+def test_merge_hulls_pass(ctx):
+    """merge_hulls=True only merges pairs whose merged cost stays under the
+    threshold: fewer (or as many) parts, same total volume, every merged hull
+    encloses its part."""
+    meshes = [(_normalize(v), t) for v, t in sdf.sdf_meshes(2, seed=21, res=20)]
+    kw = dict(max_iters=40, width=30, width2=5, threshold=0.05, depth=2, n_concave_edges=16)
+    plain = ctx.batch_lookahead_decompose(meshes, **kw)
+    merged, info = ctx.batch_lookahead_decompose(meshes, merge_hulls=True, return_info=True, **kw)
+    for m, (v, t) in enumerate(meshes):
+        assert len(merged[m]) <= len(plain[m])
+        total = abs(oracle.mesh_volume(v, t))
+        vol = sum(abs(oracle.mesh_volume(p[0], p[1])) for p in merged[m])
+        assert abs(vol - total) <= 1e-4 * total
+        rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(info[m]["hull_vol"] - info[m]["mesh_vol"], 0))
+        if info[m]["iters"] < kw["max_iters"]:
+            assert (rv < kw["threshold"]).all()
