@@ -1,10 +1,14 @@
-"""Benchmark of the hot path on one MI355X; prints ONE JSON line.
+"""Benchmark of the lookahead decomposition on one MI355X; prints ONE JSON line.
 
-Workload (current): batch exact hull + FP64 volume of 1024 clouds x 256 points
-uniform in the unit ball (seed 0), inputs resident in HBM. A step is one pass
-of the hull kernel over the whole batch.
+Workload: 64 watertight SDF meshes (smooth union of 3-8 random capsules/boxes
+minus 0-3 spheres, marching tetrahedra on a 48^3 grid, 2k-50k triangles,
+extent 2, seed 4), decomposed with lookahead_decompose at tau = 0.05,
+width/width2 30/5, depth 2, quick depth 0, 16 concave edges, 16 parts cut in
+parallel, at most 100 iterations, hull merge on. A step is one batched search
+over all 64 meshes (the meshes are independent; every launch spans all of
+them). Metric: meshes per second.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--meshes N]
 """
 import argparse
 import json
@@ -18,113 +22,107 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_cutting=16,
+              n_concave_edges=16, merge_hulls=True)
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_hull(clouds, reps):
-    """The CPU oracle (exact hull + FP64 volume, OpenMP over clouds) on the same
-    clouds: the CPU baseline, not the product path."""
+def workload(n):
+    from cuacd_amd import sdf
+    return sdf.sdf_meshes(n, seed=4, res=48, min_tris=2000, max_tris=50000)
+
+
+def cpu_baseline(ctx, meshes, gpu_iters, budget_s=25.0):
+    """The serial lookahead oracle (Python control over the C oracle's cut,
+    hull, scoring and Hausdorff pieces, one core) on a bounded sample: the
+    smallest mesh of the set, run until it converges or the time budget ends;
+    the rate is scaled by the fraction of that mesh's iterations done."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    packed = np.concatenate(clouds, 0)
-    off = np.zeros(len(clouds) + 1, np.int32)
-    off[1:] = np.cumsum([len(c) for c in clouds])
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    oracle.hull_batch(packed, off, check_general=False)  # warm
+    import lookahead_oracle as LO
+    sizes = [len(t) for _, t in meshes]
+    m = int(np.argmin(sizes))
+    v, t = meshes[m]
+    hv, ht = ctx._input_hull(v)
+    kw = dict(PARAMS)
+    kw.pop("merge_hulls")
     t0 = time.perf_counter()
-    for _ in range(reps):
-        oracle.hull_batch(packed, off, check_general=False)
-    dt = time.perf_counter() - t0
-    return {"value": reps * len(clouds) / dt, "unit": "clouds/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} passes over the full {len(clouds)}-cloud batch ({dt:.2f} s)"}
+    _, done = LO.lookahead(v, t, hv, ht, deadline=t0 + budget_s, **kw)
+    last = time.perf_counter() - t0
+    total_iters = max(int(gpu_iters[m]), 1)
+    frac = min(max(done, 1) / total_iters, 1.0)
+    return {"value": frac / last, "unit": "meshes/s", "cores": 1, "kind": "port",
+            "sample": f"mesh {m} ({sizes[m]} tris, the smallest of the set): {max(done, 1)} of its "
+                      f"{total_iters} search iterations in {last:.1f} s without the merge pass, "
+                      f"scaled to one mesh"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--meshes", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this workload runs on one GPU only (asked for {args.gpus}); "
-              "per-mesh work shards across GPUs without communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: one GPU only (asked for {args.gpus}); the meshes of the batch are independent and "
+              "shard across GPUs without communication (DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import cuacd_amd
-    from cuacd_amd import _gpu, datasets
 
-    n_clouds, n_pts = 1024, 256
-    clouds = datasets.ball_clouds(n_clouds, n_pts, seed=0)
-    packed = np.ascontiguousarray(np.concatenate(clouds, 0), np.float32)
-    offsets = (np.arange(n_clouds + 1, dtype=np.int32) * n_pts)
-    max_hv, max_ht = n_pts, 2 * n_pts - 4
+    t0 = time.perf_counter()
+    meshes = workload(args.meshes)
+    tris = np.array([len(t) for _, t in meshes])
+    log(f"workload: {len(meshes)} meshes, {tris.min()}-{tris.max()} tris (mean {tris.mean():.0f}), "
+        f"generated in {time.perf_counter() - t0:.1f} s")
 
     ctx = cuacd_amd.Context(device=0)
     name, n_cu, hbm = ctx.device_info()
     log(f"device {name}, {n_cu} CUs, {hbm / 2**30:.0f} GiB")
-
-    def dalloc(nbytes):
-        return _gpu.dev_alloc(int(nbytes))
-
-    d_pts = dalloc(packed.nbytes)
-    d_off = dalloc(offsets.nbytes)
-    d_v = dalloc(n_clouds * max_hv * 12)
-    d_t = dalloc(n_clouds * max_ht * 12)
-    d_nv = dalloc(n_clouds * 4)
-    d_nt = dalloc(n_clouds * 4)
-    d_e = dalloc(n_clouds * 4)
-    d_vol = dalloc(n_clouds * 8)
-    _gpu.memcpy_h2d(d_pts, packed.ctypes.data, packed.nbytes)
-    _gpu.memcpy_h2d(d_off, offsets.ctypes.data, offsets.nbytes)
-
-    def step():
-        _gpu.hull_dandc_dev(d_pts, d_off, n_clouds, n_pts, max_hv, max_ht, d_v, d_t, d_nv, d_nt, d_e, d_vol)
+    # first touch of every kernel (not a step): one small mesh
+    ctx.batch_lookahead_decompose(meshes[:1], **dict(PARAMS, max_iters=2))
 
     for _ in range(args.warmup):
-        step()
-    _gpu.sync()
-    t0 = time.perf_counter()
-    _gpu.timer_start()
+        ctx.batch_lookahead_decompose(meshes, **PARAMS)
+    times = []
     for _ in range(args.steps):
-        step()
-    gpu_ms = _gpu.timer_stop()
-    _gpu.sync()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    kernel_ms = gpu_ms / args.steps
-
-    nv = np.empty(n_clouds, np.int32)
-    nt = np.empty(n_clouds, np.int32)
-    err = np.empty(n_clouds, np.int32)
-    vol = np.empty(n_clouds, np.float64)
-    _gpu.memcpy_d2h(nv.ctypes.data, d_nv, nv.nbytes)
-    _gpu.memcpy_d2h(nt.ctypes.data, d_nt, nt.nbytes)
-    _gpu.memcpy_d2h(err.ctypes.data, d_e, err.nbytes)
-    _gpu.memcpy_d2h(vol.ctypes.data, d_vol, vol.nbytes)
-    if (err != 0).any():
-        log("kernel reported errors:", np.unique(err))
-        sys.exit(1)
-    # algorithmic bytes per launch: read the clouds + offsets, write hull vertex
-    # coordinates, triangles, counts/error and the FP64 volume
-    alg_bytes = packed.nbytes + offsets.nbytes + int(nv.sum()) * 12 + int(nt.sum()) * 12 + n_clouds * (12 + 8)
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    log(f"kernel {kernel_ms:.4f} ms/launch, wall {ms_per_step:.4f} ms/step, mean hull nv {nv.mean():.1f} "
-        f"nt {nt.mean():.1f}, volume {vol.mean():.4f}")
+        t0 = time.perf_counter()
+        parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, **PARAMS)
+        times.append(time.perf_counter() - t0)
+    stats = ctx.last_lookahead_stats
+    ms_per_step = 1e3 * float(np.mean(times))
+    n_parts = np.array([len(p) for p in parts])
+    iters = np.array([i["iters"] for i in info])
+    max_conc = 0.0
+    for i in info:
+        rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(i["hull_vol"] - i["mesh_vol"], 0))
+        c = np.maximum(rv, i["hausdorff"])
+        max_conc = max(max_conc, float(c.max()) if len(c) else 0.0)
+    log(f"step {ms_per_step / 1e3:.2f} s; parts {n_parts.sum()} (per mesh {n_parts.min()}-{n_parts.max()}), "
+        f"iterations {iters.min()}-{iters.max()}, meshes at max_iters {(iters >= PARAMS['max_iters']).sum()}")
+    gpu_ms = sum(s["ms"] for s in stats.values())
+    for k, s in stats.items():
+        log(f"  {k:14s} {s['ms']:9.1f} ms  {s['launches']:5d} launches  {s['alg_bytes'] / 1e9:8.3f} GB "
+            f"({s['alg_bytes'] / max(s['ms'], 1e-9) / 1e6:7.1f} GB/s)")
+    log(f"  GPU time in phases {gpu_ms / 1e3:.2f} s of {ms_per_step / 1e3:.2f} s")
+    dom = max(stats, key=lambda k: stats[k]["ms"])
+    d = stats[dom]
+    achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_hull(clouds, reps=3)
-        log(f"cpu baseline {cpu['value']:.0f} clouds/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(ctx, meshes, iters)
+        log(f"cpu baseline {cpu['value']:.4f} meshes/s ({cpu['sample']})")
 
-    value = n_clouds / (ms_per_step * 1e-3)
+    value = len(meshes) / (ms_per_step * 1e-3)
     line = {
-        "metric": "batch_hull_volume clouds/sec (exact hull + FP64 volume)",
+        "metric": "lookahead meshes/sec at tau=0.05 (plain)",
         "value": value,
-        "unit": "clouds/s",
+        "unit": "meshes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -132,13 +130,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 grid / fp32 filter / fp64 exact",
+        "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid",
         "data": "synthetic",
-        "config": {"workload": "batch_hull_volume: 1024 clouds x 256 fp32 points uniform in the unit ball, seed 0",
-                   "n_clouds": n_clouds, "n_pts": n_pts},
+        "config": {"workload": "plain lookahead_decompose: 64 SDF meshes (marching tetrahedra 48^3, 2k-50k tris, "
+                               "extent 2, seed 4), tau 0.05, width/width2 30/5, depth 2, quick 0, max_iters 100, "
+                               "merge_hulls on, n_concave_edges 16, max_n_cutting 16",
+                   "meshes": len(meshes), "tris_total": int(tris.sum()), "parts_total": int(n_parts.sum()),
+                   "max_part_concavity": max_conc, "iterations_max": int(iters.max())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "hull_wave_kernel<4>", "kernel_ms": kernel_ms, "alg_bytes": alg_bytes},
+                     "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],
+                     "alg_bytes_total": d["alg_bytes"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -29,6 +29,10 @@ MAV_OPEN_EDGE = 1 << 2
 MAV_NONMANIFOLD = 1 << 3
 MAV_FLIPPED = 1 << 4
 
+# phases reported by batch_lookahead_decompose (Context.last_lookahead_stats)
+LOOKAHEAD_PHASES = ("components", "part_hulls", "hausdorff", "concave_edges", "level0_cuts", "leaf_scoring",
+                    "item_hulls", "merge")
+
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
 
@@ -409,12 +413,16 @@ class Context:
         if width + 4 * n_concave_edges >= 512:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
+        stats = np.zeros((8, 3), np.float64)
         raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
                                    hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
                                    max_iters, width, width2, float(threshold), depth, max_n_cutting,
                                    int(decompose_components), int(not no_decompose_components_per_iter),
                                    n_concave_edges, float(concave_eps), float(concave_threshold),
-                                   concave_iters, int(merge_hulls), 0, iters.ctypes.data)
+                                   concave_iters, int(merge_hulls), stats.ctypes.data, iters.ctypes.data)
+        self.last_lookahead_stats = {name: {"ms": float(stats[i, 0]), "launches": int(stats[i, 1]),
+                                            "alg_bytes": float(stats[i, 2])}
+                                     for i, name in enumerate(LOOKAHEAD_PHASES)}
         out = [[_part_tuple(r)[:4] for r in parts] for parts in raw]
         if not return_info:
             return out

@@ -149,11 +149,41 @@ template <class T> struct DevBuf {
         if (_rc) return _rc;                     \
     } while (0)
 
+// Per-phase accounting (GPU time from events around each phase's launches,
+// launch count, algorithmic bytes: what the phase must read and write).
+enum LaPhase { PH_COMP, PH_PHULL, PH_HD, PH_EDGE, PH_CUT, PH_SCORE, PH_IHULL, PH_MERGE, PH_COUNT };
+
 struct Engine {
     gpu_ctx_t ctx;
     Params P;
     std::vector<MeshState> M;
     std::vector<void*> to_free;  // device heap chunks to free at the next flush
+    double stats[PH_COUNT][3] = {};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int hull_phase = PH_IHULL;
+
+    int tbeg() {
+        if (!ev0) {
+            HIP_TRY(hipEventCreate(&ev0));
+            HIP_TRY(hipEventCreate(&ev1));
+        }
+        HIP_TRY(hipEventRecord(ev0, ctx->stream));
+        return 0;
+    }
+    int tend(int ph, double bytes, int launches = 1) {
+        HIP_TRY(hipEventRecord(ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        stats[ph][0] += ms;
+        stats[ph][1] += launches;
+        stats[ph][2] += bytes;
+        return 0;
+    }
+    ~Engine() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
 
     int check_err(const char* where) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -206,7 +236,13 @@ struct Engine {
         DevBuf<LaCompTask> d;
         REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
         HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaCompTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
         LA_TRY(la_launch_components(ctx, d.p, (int)h.size()));
+        {
+            double by = 0;
+            for (auto& x : h) by += 12.0 * (x.nv + x.nt);
+            LA_TRY(tend(PH_COMP, by));
+        }
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaCompTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         LA_TRY(check_err("components"));
         // split parts (ascending mesh, part order: new pieces append in that order)
@@ -262,8 +298,14 @@ struct Engine {
         DevBuf<LaHullTask> d;
         REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
         HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaHullTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
         LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        {
+            double by = 0;
+            for (auto& x : h) by += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
+            LA_TRY(tend(PH_PHULL, by));
+        }
         LA_TRY(check_err("part hulls"));
         for (size_t k = 0; k < h.size(); ++k) {
             Part& p = M[who[k].first].parts[who[k].second];
@@ -301,7 +343,13 @@ struct Engine {
                 "lookahead: Hausdorff buffers allocation failed");
         for (size_t k = 0; k < h.size(); ++k) h[k].out_d2 = d2.p + k;
         HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(HdTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
         LA_TRY(hd_launch_tasks(ctx, d.p, (int)h.size(), dws.p));
+        {
+            double by = 0;
+            for (auto& x : h) by += 12.0 * (x.snv + x.snt + x.tnv + x.tnt) + 4.0;
+            LA_TRY(tend(PH_HD, by));
+        }
         std::vector<float> r(h.size());
         HIP_TRY(hipMemcpyAsync(r.data(), d2.p, 4 * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         LA_TRY(check_err("hausdorff"));
@@ -336,7 +384,13 @@ struct Engine {
         if (x1 < 0.0) pmode = 1; else if (x1 >= M_PI) pmode = 0; else { pmode = 2; cpos = (float)std::cos(x1); }
         if (x2 <= 0.0) nmode = 0; else if (x2 > M_PI) nmode = 1; else { nmode = 2; cneg = (float)std::cos(x2); }
         HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaEdgeTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
         LA_TRY(la_launch_edges(ctx, d.p, (int)h.size(), cpos, cneg, pmode, nmode));
+        {
+            double by = 0;
+            for (auto& x : h) by += 12.0 * (x.nv + x.nt);
+            LA_TRY(tend(PH_EDGE, by));
+        }
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaEdgeTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         LA_TRY(check_err("concave edges"));
         std::vector<LaEdgeRec> r;
@@ -392,8 +446,14 @@ struct Engine {
         DevBuf<LaHullTask> d;
         REQUIRE(d.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
         HIP_TRY(hipMemcpyAsync(d.p, h.data(), sizeof(LaHullTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
         LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        {
+            double by = 0;
+            for (auto& x : h) by += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
+            LA_TRY(tend(hull_phase, by));
+        }
         return check_err("item hulls");
     }
 
@@ -463,7 +523,9 @@ struct Engine {
             HIP_TRY(hipMemcpyAsync(dg.p, pg.data(), sizeof(LaGatherTask) * pg.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
             LA_TRY(la_launch_gather(ctx, dg.p, (int)pg.size()));
+            hull_phase = PH_MERGE;
             LA_TRY(hull_volumes(ht));
+            hull_phase = PH_IHULL;
         }
         // rv cost of the merged hull; the survivors go on to the Hausdorff stage
         std::vector<int> hd_pairs;
@@ -535,7 +597,13 @@ struct Engine {
                     "lookahead: merge Hausdorff buffers allocation failed");
             for (size_t k = 0; k < hd.size(); ++k) hd[k].out_d2 = d2.p + k;
             HIP_TRY(hipMemcpyAsync(dh.p, hd.data(), sizeof(HdTask) * hd.size(), hipMemcpyHostToDevice, ctx->stream));
+            LA_TRY(tbeg());
             LA_TRY(hd_launch_tasks(ctx, dh.p, (int)hd.size(), dws.p));
+            {
+                double by = 0;
+                for (auto& x : hd) by += 12.0 * (x.snv + x.snt + x.tnv + x.tnt) + 4.0;
+                LA_TRY(tend(PH_MERGE, by));
+            }
             std::vector<float> r(hd.size());
             HIP_TRY(hipMemcpyAsync(r.data(), d2.p, 4 * hd.size(), hipMemcpyDeviceToHost, ctx->stream));
             LA_TRY(check_err("merge hausdorff"));
@@ -829,9 +897,15 @@ struct Engine {
                     REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
                     HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice,
                                            ctx->stream));
+                    LA_TRY(tbeg());
                     LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));
+                    {
+                        double by = 0;
+                        for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
+                        LA_TRY(tend(PH_CUT, by));
+                    }
                     LA_TRY(check_err("cuts"));
                 }
                 for (size_t q = 0; q < cands.size(); ++q) {
@@ -881,9 +955,15 @@ struct Engine {
                     REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
                     HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaScoreTask) * h.size(), hipMemcpyHostToDevice,
                                            ctx->stream));
+                    LA_TRY(tbeg());
                     LA_TRY(la_launch_score(ctx, dt.p, (int)h.size()));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaScoreTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));
+                    {
+                        double by = 0;
+                        for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.np_pts + x.nn_pts) + 16.0;
+                        LA_TRY(tend(PH_SCORE, by));
+                    }
                     LA_TRY(check_err("leaf scoring"));
                 }
                 // branch-and-bound bound per mesh (cuda/la_expand.cu:434-457): min over the
@@ -930,6 +1010,11 @@ struct Engine {
             }
             LA_TRY(hull_volumes(ht));
             to_free.insert(to_free.end(), pts_free.begin(), pts_free.end());
+            if (full)
+                for (size_t q = 0; q < ht.size(); ++q) {
+                    Item& ch = nxt[ht_who[q].first];
+                    ch.parts[ch.np - 2 + ht_who[q].second].hvol = ht[q].vol;
+                }
             if (!full) {
                 // leaf costs: the worst part is the maximum rv cost over the
                 // inherited parts and both halves; path cost = mean level cost
@@ -1057,7 +1142,7 @@ struct Engine {
 // Batched lookahead over n meshes (packed), each decomposed independently.
 static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* voff, const int* tris,
                          const int* toff, const float* hverts, const int* hvoff, const int* htris, const int* htoff,
-                         const Params& P, struct gpu_result* out, int* iters_out) {
+                         const Params& P, struct gpu_result* out, int* iters_out, double* stats_out) {
     HIP_TRY(hipSetDevice(ctx->device));
     for (int m = 0; m < n_meshes; ++m) { out[m].parts = nullptr; out[m].nparts = 0; }
     REQUIRE(P.depth >= 1 && P.depth <= LA_MAX_LEVELS, "lookahead: depth must be in [1, %d]", LA_MAX_LEVELS);
@@ -1187,6 +1272,9 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
             p.v = nullptr;
             p.hv = nullptr;
         }
+    if (stats_out)
+        for (int ph = 0; ph < PH_COUNT; ++ph)
+            for (int j = 0; j < 3; ++j) stats_out[3 * ph + j] = E.stats[ph][j];
     const int frc = E.flush_frees();
     for (void* b : E.ext_bufs) (void)hipFreeAsync(b, ctx->stream);
     HIP_TRY(hipFreeAsync(d_v, ctx->stream));
@@ -1213,7 +1301,8 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
              decompose_components_per_iter, n_concave_edges, concave_iters, merge_hulls, concave_eps,
              concave_threshold};
     const int voff[2] = {0, nv}, toff[2] = {0, nt}, hvoff[2] = {0, hull_nv}, htoff[2] = {0, hull_nt};
-    return lookahead_run(ctx, 1, verts, voff, tris, toff, hull_verts, hvoff, hull_tris, htoff, P, out, nullptr);
+    return lookahead_run(ctx, 1, verts, voff, tris, toff, hull_verts, hvoff, hull_tris, htoff, P, out, nullptr,
+                         nullptr);
 }
 
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
@@ -1221,12 +1310,12 @@ int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const i
                         const int* hull_tri_off, int max_iters, int width, int width2, float threshold, int depth,
                         int max_n_cutting, int decompose_components, int decompose_components_per_iter,
                         int n_concave_edges, float concave_eps, float concave_threshold, int concave_iters,
-                        int merge_hulls, struct gpu_result* out, int* out_iters) {
+                        int merge_hulls, struct gpu_result* out, int* out_iters, double* out_stats) {
     Params P{max_iters, width, width2, depth, 0, max_n_cutting, 0, threshold, decompose_components,
              decompose_components_per_iter, n_concave_edges, concave_iters, merge_hulls, concave_eps,
              concave_threshold};
     return lookahead_run(ctx, n_meshes, verts, vert_off, tris, tri_off, hull_verts, hull_vert_off, hull_tris,
-                         hull_tri_off, P, out, out_iters);
+                         hull_tri_off, P, out, out_iters, out_stats);
 }
 
 }  // extern "C"

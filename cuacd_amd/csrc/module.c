@@ -230,8 +230,7 @@ static PyObject* m_lookahead_batch(PyObject* self, PyObject* args) {
     int rc = gpu_lookahead_batch(g_ctx, n, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
                                  P(const float*, hv), P(const int*, hvo), P(const int*, ht), P(const int*, hto),
                                  max_iters, width, width2, thr, depth, max_cut, dcomp, dc_iter, n_conc, conc_eps,
-                                 conc_thr, conc_iters, merge, res, P(int*, itp));
-    (void)outp;
+                                 conc_thr, conc_iters, merge, res, P(int*, itp), P(double*, outp));
     if (rc) {
         free(res);
         return raise_error(rc);

@@ -137,14 +137,16 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * convex hulls (hull_vert_off/hull_tri_off), decomposed independently with the
  * same parameters in one search whose launches span all meshes. out[m] gets
  * mesh m's parts (free each with gpu_result_free); out_iters[m] (optional) the
- * iterations it ran. quick_depth is 0. */
+ * iterations it ran; out_stats (optional, 24 doubles) per phase - components,
+ * part hulls, Hausdorff, concave edges, level-0 cuts, leaf scoring, item hulls,
+ * merge - the GPU milliseconds, launches and algorithmic bytes. quick_depth is 0. */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off,
                         const int* hull_tris, const int* hull_tri_off, int max_iters, int width, int width2,
                         float threshold, int depth, int max_n_cutting, int decompose_components,
                         int decompose_components_per_iter, int n_concave_edges, float concave_eps,
                         float concave_threshold, int concave_iters, int merge_hulls, struct gpu_result* out,
-                        int* out_iters);
+                        int* out_iters, double* out_stats);
 
 /* replaces gpu_preprocess (csrc/preprocess.h:18-24). The PaMO remesh is not
  * part of this library: always returns -1 with an explanatory error. */

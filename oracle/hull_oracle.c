@@ -74,6 +74,32 @@ typedef struct {
  * live faces (indices into the input, outward CCW) to faces (cap = 2n+8
  * triangles). Returns the face count, 0 for a degenerate (flat) input, -1 on
  * allocation failure. */
+/* directed edge (a, b) -> face id, -1 when none; open addressing, entries are
+ * never removed (a dead edge is set to -1), so lookups stay valid */
+typedef struct { uint64_t* k; int* v; uint64_t mask; } OEdgeMap;
+static uint64_t oem_hash(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+    return k;
+}
+static int* oem_slot(OEdgeMap* m, int a, int b, int create) {
+    uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+    uint64_t h = oem_hash(key) & m->mask;
+    for (;;) {
+        if (m->k[h] == key) return &m->v[h];
+        if (m->k[h] == ~0ull) {
+            if (!create) return NULL;
+            m->k[h] = key;
+            m->v[h] = -1;
+            return &m->v[h];
+        }
+        h = (h + 1) & m->mask;
+    }
+}
+static int oem_get(OEdgeMap* m, int a, int b) {
+    int* s = oem_slot(m, a, b, 0);
+    return s ? *s : -1;
+}
+
 int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face_cap) {
     memset(is_vertex, 0, (size_t)n);
     if (n < 4) return 0;
@@ -96,27 +122,37 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
     }
     int cap = 8 * n + 64;
     OFace* F = (OFace*)malloc(sizeof(OFace) * (size_t)cap);
-    int* emap = (int*)malloc(sizeof(int) * (size_t)n * (size_t)n); /* directed edge a->b -> face */
+    OEdgeMap em;
+    uint64_t H = 64;
+    while (H < 2ull * 3ull * (uint64_t)cap + 2) H <<= 1;
+    em.k = (uint64_t*)malloc(8ull * H);
+    em.v = (int*)malloc(4ull * H);
+    em.mask = H - 1;
     int* hz = (int*)malloc(sizeof(int) * 2 * (size_t)cap);
     int* vis = (int*)malloc(sizeof(int) * (size_t)cap);
-    if (!F || !emap || !hz || !vis) { free(F); free(emap); free(hz); free(vis); return -1; }
-    for (size_t k = 0; k < (size_t)n * (size_t)n; ++k) emap[k] = -1;
+    if (!F || !em.k || !em.v || !hz || !vis) { free(F); free(em.k); free(em.v); free(hz); free(vis); return -1; }
+    for (uint64_t k = 0; k < H; ++k) em.k[k] = ~0ull;
     int nf = 0;
 #define ADDF(a, b, c)                                                                     \
     do {                                                                                  \
         F[nf].v[0] = (a); F[nf].v[1] = (b); F[nf].v[2] = (c); F[nf].alive = 1;           \
-        emap[(size_t)(a) * n + (b)] = nf; emap[(size_t)(b) * n + (c)] = nf;               \
-        emap[(size_t)(c) * n + (a)] = nf; ++nf;                                           \
+        *oem_slot(&em, (a), (b), 1) = nf; *oem_slot(&em, (b), (c), 1) = nf;               \
+        *oem_slot(&em, (c), (a), 1) = nf; ++nf;                                           \
     } while (0)
     ADDF(i0, i2, i1);
     ADDF(i0, i1, i3);
     ADDF(i1, i2, i3);
     ADDF(i0, i3, i2);
     int failed = 0;
+    /* live face ids (the face array keeps dead faces so edge-map ids stay valid) */
+    int* L = (int*)malloc(sizeof(int) * (size_t)cap);
+    int nL = 4;
+    for (int k = 0; k < 4; ++k) L[k] = k;
     for (int p = 0; p < n && !failed; ++p) {
         if (p == i0 || p == i1 || p == i2 || p == i3) continue;
         int nvis = 0;
-        for (int f = 0; f < nf; ++f) {
+        for (int li = 0; li < nL; ++li) {
+            int f = L[li];
             if (!F[f].alive) continue;
             if (orient64(q + 3 * F[f].v[0], q + 3 * F[f].v[1], q + 3 * F[f].v[2], q + 3 * p) > 0) {
                 vis[nvis++] = f;
@@ -129,7 +165,7 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
             const OFace* f = &F[vis[k]];
             for (int e = 0; e < 3; ++e) {
                 int a = f->v[e], b = f->v[(e + 1) % 3];
-                int g = emap[(size_t)b * n + a];
+                int g = oem_get(&em, b, a);
                 if (g < 0 || F[g].alive != 2) { hz[2 * nh] = a; hz[2 * nh + 1] = b; ++nh; }
             }
         }
@@ -138,12 +174,18 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
             f->alive = 0;
             for (int e = 0; e < 3; ++e) {
                 int a = f->v[e], b = f->v[(e + 1) % 3];
-                if (emap[(size_t)a * n + b] == vis[k]) emap[(size_t)a * n + b] = -1;
+                int* sl = oem_slot(&em, a, b, 0);
+                if (sl && *sl == vis[k]) *sl = -1;
             }
         }
         if (nf + nh > cap) { failed = 1; break; }
-        for (int k = 0; k < nh; ++k) ADDF(hz[2 * k], hz[2 * k + 1], p);
+        int w = 0;
+        for (int li = 0; li < nL; ++li)
+            if (F[L[li]].alive) L[w++] = L[li];
+        for (int k = 0; k < nh; ++k) { L[w++] = nf; ADDF(hz[2 * k], hz[2 * k + 1], p); }
+        nL = w;
     }
+    free(L);
 #undef ADDF
     int out = 0;
     if (!failed) {
@@ -158,7 +200,7 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
             ++out;
         }
     }
-    free(F); free(emap); free(hz); free(vis);
+    free(F); free(em.k); free(em.v); free(hz); free(vis);
     return failed ? -1 : out;
 }
 

@@ -311,8 +311,11 @@ def _sort_parts(parts):
 
 def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, threshold=0.05, depth=2,
               max_n_cutting=16, decompose_components=False, dc_per_iter=True, n_concave_edges=32,
-              concave_eps=0.005, concave_threshold=3.49, concave_iters=10):
-    """Returns (parts, iters): parts = [(v, t, hv, ht, mesh_vol, hull_vol, hausdorff)]."""
+              concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None):
+    """Returns (parts, iters): parts = [(v, t, hv, ht, mesh_vol, hull_vol, hausdorff)].
+    deadline (time.perf_counter() value): stop before an iteration that would
+    start after it (bounded CPU-baseline samples)."""
+    import time
     tau = np.float32(threshold)
     v0 = np.asarray(verts, np.float32)
     t0 = np.asarray(tris, np.int32)
@@ -359,6 +362,8 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
 
     iters = 0
     for it in range(max_iters):
+        if deadline is not None and time.perf_counter() > deadline:
+            break
         if dc_per_iter:
             run_components()
         run_hulls()

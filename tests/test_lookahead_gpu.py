@@ -31,7 +31,9 @@ def _check_parts(parts, v, t, info=None, tau=0.05, rel=1e-4):
         assert oracle.edge_manifold_report(pt) == (0, 0)
         vol += abs(oracle.mesh_volume(pv, pt))
         # the hull encloses the part
-        assert abs(oracle.mesh_volume(hv, ht)) >= abs(oracle.mesh_volume(pv, pt)) * (1 - 1e-5)
+        # (up to the exact hull's quantization grid, span / 10216, which can
+        # merge a part's near-coincident extreme vertices)
+        assert abs(oracle.mesh_volume(hv, ht)) >= abs(oracle.mesh_volume(pv, pt)) * (1 - 1e-3)
     assert abs(vol - total) <= rel * total, (vol, total)
     if info is not None:
         rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(info["hull_vol"] - info["mesh_vol"], 0))
