@@ -109,6 +109,9 @@ def main():
         log(f"  {k:14s} {s['ms']:9.1f} ms  {s['launches']:5d} launches  {s['alg_bytes'] / 1e9:8.3f} GB "
             f"({s['alg_bytes'] / max(s['ms'], 1e-9) / 1e6:7.1f} GB/s)")
     log(f"  GPU time in phases {gpu_ms / 1e3:.2f} s of {ms_per_step / 1e3:.2f} s")
+    sec = ctx.last_lookahead_sections
+    log("  host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()) +
+        f" (sum {sum(sec.values()) / 1e3:.2f} s)")
     dom = max(stats, key=lambda k: stats[k]["ms"])
     d = stats[dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0

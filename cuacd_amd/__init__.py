@@ -32,6 +32,9 @@ MAV_FLIPPED = 1 << 4
 # phases reported by batch_lookahead_decompose (Context.last_lookahead_stats)
 LOOKAHEAD_PHASES = ("components", "part_hulls", "hausdorff", "concave_edges", "level0_cuts", "leaf_scoring",
                     "item_hulls", "merge")
+# host wall-time sections of the same run (Context.last_lookahead_sections, ms)
+LOOKAHEAD_SECTIONS = ("setup", "refresh_select", "concave_planes", "level0", "deeper_levels", "evaluate_apply",
+                      "post_merge", "readback")
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
@@ -197,16 +200,23 @@ class Context:
             return []
         packed, offsets, counts = _pack(pts_list)
         n = len(counts)
+        # the reference's fixed output caps; a hull larger than them (the
+        # library reports its true size) is computed again with caps that fit
         max_hv, max_ht = 4096, 8192
-        out_v = np.empty(n * max_hv * 3, np.float32)
-        out_t = np.empty(n * max_ht * 3, np.int32)
-        out_nv = np.empty(n, np.int32)
-        out_nt = np.empty(n, np.int32)
-        vols = np.empty(n, np.float32)
-        errs = np.empty(n, np.int32)
-        _gpu.kdop_hull(packed.ctypes.data, packed.shape[0], offsets.ctypes.data, n, max_hv, max_ht,
-                       out_v.ctypes.data, out_t.ctypes.data, out_nv.ctypes.data, out_nt.ctypes.data,
-                       vols.ctypes.data, errs.ctypes.data)
+        while True:
+            out_v = np.empty(n * max_hv * 3, np.float32)
+            out_t = np.empty(n * max_ht * 3, np.int32)
+            out_nv = np.empty(n, np.int32)
+            out_nt = np.empty(n, np.int32)
+            vols = np.empty(n, np.float32)
+            errs = np.empty(n, np.int32)
+            _gpu.kdop_hull(packed.ctypes.data, packed.shape[0], offsets.ctypes.data, n, max_hv, max_ht,
+                           out_v.ctypes.data, out_t.ctypes.data, out_nv.ctypes.data, out_nt.ctypes.data,
+                           vols.ctypes.data, errs.ctypes.data)
+            need_v, need_t = int(out_nv.max()), int(out_nt.max())
+            if need_v <= max_hv and need_t <= max_ht:
+                break
+            max_hv, max_ht = max(max_hv, need_v), max(max_ht, need_t)
         res = []
         for i in range(n):
             nv, nt = int(out_nv[i]), int(out_nt[i])
@@ -413,16 +423,18 @@ class Context:
         if width + 4 * n_concave_edges >= 512:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
-        stats = np.zeros((8, 3), np.float64)
+        stats = np.zeros(32, np.float64)
         raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
                                    hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
                                    max_iters, width, width2, float(threshold), depth, max_n_cutting,
                                    int(decompose_components), int(not no_decompose_components_per_iter),
                                    n_concave_edges, float(concave_eps), float(concave_threshold),
                                    concave_iters, int(merge_hulls), stats.ctypes.data, iters.ctypes.data)
-        self.last_lookahead_stats = {name: {"ms": float(stats[i, 0]), "launches": int(stats[i, 1]),
-                                            "alg_bytes": float(stats[i, 2])}
+        ph = stats[:24].reshape(8, 3)
+        self.last_lookahead_stats = {name: {"ms": float(ph[i, 0]), "launches": int(ph[i, 1]),
+                                            "alg_bytes": float(ph[i, 2])}
                                      for i, name in enumerate(LOOKAHEAD_PHASES)}
+        self.last_lookahead_sections = {name: float(stats[24 + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
         out = [[_part_tuple(r)[:4] for r in parts] for parts in raw]
         if not return_info:
             return out

@@ -57,12 +57,14 @@ __global__ void dpool_init_kernel(DPool* p, char* base, unsigned long long cap) 
         p->base = base;
         p->capacity = cap;
         p->bump = 0;
-        p->heap.outstanding = p->heap.allocs = p->heap.frees = 0;
-        p->scratch.outstanding = p->scratch.allocs = p->scratch.frees = 0;
     }
-    for (int c = threadIdx.x; c < DH_NCLASS; c += blockDim.x) {
-        p->heap.heads[c] = (unsigned long long)DH_EMPTY;
-        p->scratch.heads[c] = (unsigned long long)DH_EMPTY;
+    for (int i = threadIdx.x; i < DH_NCLASS * DH_STRIPES; i += blockDim.x) {
+        p->heap.heads[i / DH_STRIPES][i % DH_STRIPES].v = (unsigned long long)DH_EMPTY;
+        p->scratch.heads[i / DH_STRIPES][i % DH_STRIPES].v = (unsigned long long)DH_EMPTY;
+    }
+    for (int i = threadIdx.x; i < DH_STRIPES; i += blockDim.x) {
+        p->heap.cnt[i] = DCount{0, 0, 0};
+        p->scratch.cnt[i] = DCount{0, 0, 0};
     }
 }
 
@@ -155,15 +157,17 @@ size_t gpu_pool_usage(gpu_ctx_t ctx) {
 }
 
 int gpu_heap_stats(gpu_ctx_t ctx, unsigned long long out[6]) {
-    DPool h;
-    HIP_TRY(hipMemcpyAsync(&h, ctx->d_pool, sizeof(DPool), hipMemcpyDeviceToHost, ctx->stream));
+    DCount c[2][DH_STRIPES];
+    HIP_TRY(hipMemcpyAsync(c[0], ctx->d_pool->heap.cnt, sizeof(c[0]), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(c[1], ctx->d_pool->scratch.cnt, sizeof(c[1]), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    out[0] = h.heap.outstanding;
-    out[1] = h.heap.allocs;
-    out[2] = h.heap.frees;
-    out[3] = h.scratch.outstanding;
-    out[4] = h.scratch.allocs;
-    out[5] = h.scratch.frees;
+    for (int h = 0; h < 2; ++h) {
+        unsigned long long o = 0, a = 0, f = 0;
+        for (int s = 0; s < DH_STRIPES; ++s) { o += c[h][s].outstanding; a += c[h][s].allocs; f += c[h][s].frees; }
+        out[3 * h] = o;
+        out[3 * h + 1] = a;
+        out[3 * h + 2] = f;
+    }
     return 0;
 }
 

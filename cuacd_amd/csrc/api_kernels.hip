@@ -205,7 +205,7 @@ int gpu_test_postprocess_dc(gpu_ctx_t ctx, const float* verts, int nv, const int
     if (rc == 0) {
         HIP_TRY(hipMemcpyAsync(&t, d_task, sizeof(t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (t.n_comp <= 1) {
+        if (t.n_comp == 0) {
             *out_n_components = (nv > 0 && nt > 0) ? 1 : 0;
             out_nv[0] = nv;
             out_nt[0] = nt;

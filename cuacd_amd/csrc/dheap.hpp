@@ -8,14 +8,20 @@
 //   * size classes are powers of two from 256 B (class c holds blocks of
 //     256 << c bytes, header included); a block never changes class, so there
 //     is no coalescing and no boundary tags;
-//   * each class keeps a Treiber stack whose head packs (ABA tag << 32 | block
-//     offset / 256) in one 64-bit word and is updated with a single agent-scope
-//     CAS; nothing ever spins on a lock held by another lane (a lane spinning
-//     on a lock held by a lane of its own wave would never finish on hardware
-//     without independent thread scheduling);
-//   * an empty class refills by bumping the pool with one atomic add.
-// One lane per wave calls heap_alloc/heap_free. Blocks are 256-byte aligned;
-// user data starts 16 bytes in (the header) and is 16-byte aligned.
+//   * each class keeps DH_STRIPES Treiber stacks; a head packs (ABA tag << 32 |
+//     block offset / 256) in one 64-bit word, sits alone in a 128-byte line
+//     and is updated with a single agent-scope CAS. A workgroup pushes to and
+//     pops from the stripe of its block index (then tries the others before
+//     bumping), so thousands of concurrent workgroups spread their CAS
+//     traffic over 32 lines instead of retrying on one; nothing ever spins on
+//     a lock held by another lane (a lane spinning on a lock held by a lane of
+//     its own wave would never finish without independent thread scheduling);
+//   * an empty class refills by bumping the pool with one atomic add;
+//   * heap_free_batch returns many blocks at once: a workgroup links its
+//     blocks of one (heap, class) into a chain in LDS and splices the chain
+//     with one CAS.
+// One lane per workgroup calls heap_alloc/heap_free. Blocks are 256-byte
+// aligned; user data starts 16 bytes in (the header) and is 16-byte aligned.
 #pragma once
 #include "wave.hpp"
 #include "kerr.h"
@@ -23,22 +29,30 @@
 namespace cuacd {
 
 constexpr int DH_NCLASS = 40;            // 256 B .. 128 TiB classes (only small ones used)
+constexpr int DH_STRIPES = 32;
 constexpr uint32_t DH_EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t DH_MAGIC_LIVE = 0xC0DEA11Cu;
 constexpr uint32_t DH_MAGIC_FREE = 0xF4EEB10Cu;
 constexpr int DH_HDR = 16;
 
+struct alignas(128) DHead {
+    unsigned long long v;  // tagged Treiber-stack head
+};
+struct alignas(128) DCount {
+    unsigned long long outstanding;  // live user bytes
+    unsigned long long allocs;       // cumulative alloc calls
+    unsigned long long frees;        // cumulative free calls
+};
+
 struct DHeap {
-    unsigned long long heads[DH_NCLASS];  // tagged Treiber-stack heads
-    unsigned long long outstanding;       // live user bytes
-    unsigned long long allocs;            // cumulative alloc calls
-    unsigned long long frees;             // cumulative free calls
+    DHead heads[DH_NCLASS][DH_STRIPES];
+    DCount cnt[DH_STRIPES];  // summed by the host
 };
 
 struct DPool {
     char* base;                    // device pointer to the pool memory
     unsigned long long capacity;   // bytes
-    unsigned long long bump;       // bytes handed out by the bump pointer (monotonic)
+    alignas(128) unsigned long long bump;  // bytes handed out by the bump pointer (monotonic)
     DHeap heap;                    // output heap (meshes that outlive a kernel)
     DHeap scratch;                 // scratch heap (workspaces freed inside one call)
 };
@@ -58,34 +72,53 @@ __device__ __forceinline__ int dh_class_of(unsigned long long total) {
     return 64 - __builtin_clzll(total - 1) - 8;
 }
 
+__device__ __forceinline__ int dh_stripe() { return (int)(blockIdx.x & (DH_STRIPES - 1)); }
+
+// Pop one block of class c from stripe s; nullptr if that stripe is empty.
+__device__ inline char* dh_pop(DPool* pool, DHeap* hp, int c, int s) {
+    char* base = pool->base;
+    unsigned long long old = ld_acquire(&hp->heads[c][s].v);
+    while ((uint32_t)old != DH_EMPTY) {
+        uint32_t off = (uint32_t)old;
+        DBlockHdr* b = (DBlockHdr*)(base + ((unsigned long long)off << 8));
+        uint32_t nxt = __hip_atomic_load(&b->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long neu = (((old >> 32) + 1ull) << 32) | nxt;
+        if (cas_acq_rel(&hp->heads[c][s].v, old, neu)) return (char*)b;
+    }
+    return nullptr;
+}
+
+// Push the chain first..last (linked through next, offsets / 256) onto stripe s.
+__device__ inline void dh_push_chain(DHeap* hp, int c, int s, DBlockHdr* last, uint32_t first_off) {
+    unsigned long long old = ld_acquire(&hp->heads[c][s].v);
+    for (;;) {
+        __hip_atomic_store(&last->next, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long neu = (((old >> 32) + 1ull) << 32) | first_off;
+        if (cas_acq_rel(&hp->heads[c][s].v, old, neu)) break;
+    }
+}
+
 // Allocate `bytes` user bytes from heap `h` (0 = output heap, 1 = scratch).
 // Returns nullptr and sets KERR_HEAP_OOM in *err on exhaustion.
 __device__ inline void* heap_alloc(DPool* pool, int h, unsigned long long bytes, int* err) {
     DHeap* hp = h ? &pool->scratch : &pool->heap;
     const int c = dh_class_of(bytes + DH_HDR);
     if (c >= DH_NCLASS) { err_or(err, KERR_HEAP_OOM); return nullptr; }
-    char* base = pool->base;
-    char* blk = nullptr;
-    unsigned long long old = ld_acquire(&hp->heads[c]);
-    while ((uint32_t)old != DH_EMPTY) {
-        uint32_t off = (uint32_t)old;
-        DBlockHdr* b = (DBlockHdr*)(base + ((unsigned long long)off << 8));
-        uint32_t nxt = __hip_atomic_load(&b->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned long long neu = (((old >> 32) + 1ull) << 32) | nxt;
-        if (cas_acq_rel(&hp->heads[c], old, neu)) { blk = (char*)b; break; }
-    }
+    const int s0 = dh_stripe();
+    char* blk = dh_pop(pool, hp, c, s0);
+    for (int k = 1; !blk && k < DH_STRIPES; ++k) blk = dh_pop(pool, hp, c, (s0 + k) & (DH_STRIPES - 1));
     if (!blk) {
         unsigned long long sz = 256ull << c;
         unsigned long long o = fetch_add_relaxed(&pool->bump, sz);
         if (o + sz > pool->capacity) { err_or(err, KERR_HEAP_OOM); return nullptr; }
-        blk = base + o;
+        blk = pool->base + o;
     }
     DBlockHdr* hd = (DBlockHdr*)blk;
     hd->meta = dh_meta(c, h, bytes);
     hd->magic = DH_MAGIC_LIVE;
     hd->next = DH_EMPTY;
-    fetch_add_relaxed(&hp->outstanding, bytes);
-    fetch_add_relaxed(&hp->allocs, 1ull);
+    fetch_add_relaxed(&hp->cnt[s0].outstanding, bytes);
+    fetch_add_relaxed(&hp->cnt[s0].allocs, 1ull);
     return blk + DH_HDR;
 }
 
@@ -106,14 +139,59 @@ __device__ inline void heap_free(DPool* pool, void* p, int* err) {
     DHeap* hp = hid ? &pool->scratch : &pool->heap;
     hd->magic = DH_MAGIC_FREE;
     const uint32_t off = (uint32_t)((unsigned long long)(blk - pool->base) >> 8);
-    unsigned long long old = ld_acquire(&hp->heads[c]);
-    for (;;) {
-        __hip_atomic_store(&hd->next, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned long long neu = (((old >> 32) + 1ull) << 32) | off;
-        if (cas_acq_rel(&hp->heads[c], old, neu)) break;
+    const int s = dh_stripe();
+    dh_push_chain(hp, c, s, hd, off);
+    fetch_add_relaxed(&hp->cnt[s].outstanding, (unsigned long long)0 - bytes);
+    fetch_add_relaxed(&hp->cnt[s].frees, 1ull);
+}
+
+// Free up to blockDim.x blocks, one per thread (p == nullptr: nothing), with
+// one CAS per (heap, class) present in the workgroup. Every thread of the
+// workgroup must call it. `lds` needs DH_BATCH_LDS 32-bit words.
+constexpr int DH_KEYS = 2 * DH_NCLASS;
+constexpr int DH_BATCH_LDS = 6 * DH_KEYS;
+__device__ inline void heap_free_batch(DPool* pool, void* p, int* err, uint32_t* lds) {
+    uint32_t* first = lds;                 // chain head per key (block offset / 256)
+    uint32_t* last = lds + DH_KEYS;        // chain tail per key
+    uint32_t* n = lds + 2 * DH_KEYS;       // blocks per key
+    unsigned long long* by = (unsigned long long*)(lds + 4 * DH_KEYS);  // 8-byte aligned: bytes per key
+    for (int k = threadIdx.x; k < DH_KEYS; k += blockDim.x) {
+        first[k] = DH_EMPTY;
+        last[k] = DH_EMPTY;
+        n[k] = 0;
+        by[k] = 0;
     }
-    fetch_add_relaxed(&hp->outstanding, (unsigned long long)0 - bytes);
-    fetch_add_relaxed(&hp->frees, 1ull);
+    __syncthreads();
+    if (p) {
+        char* blk = (char*)p - DH_HDR;
+        DBlockHdr* hd = (DBlockHdr*)blk;
+        const unsigned long long meta = hd->meta;
+        const int c = (int)(meta >> 56);
+        const int hid = (int)((meta >> 48) & 0xFF);
+        if (hd->magic != DH_MAGIC_LIVE || c >= DH_NCLASS || hid > 1) {
+            err_or(err, KERR_HEAP_CORRUPT);
+        } else {
+            const int key = hid * DH_NCLASS + c;
+            hd->magic = DH_MAGIC_FREE;
+            const uint32_t off = (uint32_t)((unsigned long long)(blk - pool->base) >> 8);
+            const uint32_t prev = atomicExch(&first[key], off);
+            __hip_atomic_store(&hd->next, prev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == DH_EMPTY) last[key] = off;
+            atomicAdd(&n[key], 1u);
+            atomicAdd(&by[key], meta & 0xFFFFFFFFFFFFull);
+        }
+    }
+    __threadfence();  // the links are visible device-wide before a chain is published
+    __syncthreads();
+    const int s = dh_stripe();
+    for (int k = threadIdx.x; k < DH_KEYS; k += blockDim.x) {
+        if (first[k] == DH_EMPTY) continue;
+        DHeap* hp = k >= DH_NCLASS ? &pool->scratch : &pool->heap;
+        DBlockHdr* tail = (DBlockHdr*)(pool->base + ((unsigned long long)last[k] << 8));
+        dh_push_chain(hp, k % DH_NCLASS, s, tail, first[k]);
+        fetch_add_relaxed(&hp->cnt[s].outstanding, (unsigned long long)0 - by[k]);
+        fetch_add_relaxed(&hp->cnt[s].frees, (unsigned long long)n[k]);
+    }
 }
 
 }  // namespace cuacd

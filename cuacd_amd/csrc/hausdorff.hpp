@@ -242,6 +242,7 @@ __device__ inline HdBvh hd_build(const MeshView& m, char* ws, HdLDS* s) {
         b.tri[3 * pos + 1] = mk_f4(bb.x, bb.y, bb.z, 0.0f);
         b.tri[3 * pos + 2] = mk_f4(c.x, c.y, c.z, 0.0f);
     }
+    __syncthreads();  // slot 0 is written by whichever thread scattered it
     // pad the last leaf with copies of the first triangle (never nearer than it)
     for (int k = m.nt + tid; k < ntp; k += HD_T)
         for (int e = 0; e < 3; ++e) b.tri[3 * k + e] = b.tri[e];

@@ -682,12 +682,12 @@ __global__ __launch_bounds__(PC_T) void la_comp_kernel(LaCompTask* tasks, int n,
     const int* Tr = T.t;
     const int nv = T.nv, nt = T.nt;
     if (nv <= 0 || nt <= 0) {
-        if (tid == 0) T.n_comp = 1;
+        if (tid == 0) T.n_comp = 0;
         return;
     }
     char* A = (char*)pc_alloc(pool, 12ull * nv + 64, &s, 1, err, 1);
     if (!A) {
-        if (tid == 0) { T.n_comp = 1; err_or(err, KERR_DC_OOM); }
+        if (tid == 0) { T.n_comp = 0; err_or(err, KERR_DC_OOM); }
         return;
     }
     int* par = (int*)A;
@@ -707,7 +707,7 @@ __global__ __launch_bounds__(PC_T) void la_comp_kernel(LaCompTask* tasks, int n,
     // components numbered by their lowest vertex
     const int ncomp = pc_block_scan(nv, lidx, &s, [&](int i) { return (cid[i] && par[i] == i) ? 1 : 0; });
     if (ncomp <= 1) {
-        if (tid == 0) { T.n_comp = 1; heap_free(pool, A, err); }
+        if (tid == 0) { T.n_comp = 0; heap_free(pool, A, err); }
         return;
     }
     // component id of every used vertex (roots first, then members)
@@ -956,9 +956,10 @@ __global__ __launch_bounds__(PC_T) void la_gather_kernel(LaGatherTask* tasks, in
     if (tid == 0) T.nt_out = base;
 }
 
-__global__ void la_free_kernel(void** ptrs, int n, DPool* pool, int* err) {
+__global__ __launch_bounds__(256) void la_free_kernel(void** ptrs, int n, DPool* pool, int* err) {
+    __shared__ unsigned long long lds[DH_BATCH_LDS / 2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && ptrs[i]) heap_free(pool, ptrs[i], err);
+    heap_free_batch(pool, i < n ? ptrs[i] : nullptr, err, (uint32_t*)lds);
 }
 
 // host helpers --------------------------------------------------------------------

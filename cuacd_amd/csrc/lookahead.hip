@@ -20,6 +20,7 @@
 // costs are rounded to float, the reference's cost precision; new parts are
 // appended in cutting order, so runs are deterministic.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -152,6 +153,10 @@ template <class T> struct DevBuf {
 // Per-phase accounting (GPU time from events around each phase's launches,
 // launch count, algorithmic bytes: what the phase must read and write).
 enum LaPhase { PH_COMP, PH_PHULL, PH_HD, PH_EDGE, PH_CUT, PH_SCORE, PH_IHULL, PH_MERGE, PH_COUNT };
+// Host wall time by section of the search (launches, waits and host work):
+// input set-up, per-iteration refresh + selection, concave planes, level 0,
+// deeper levels, evaluate + apply, post-loop passes and merge, result readback.
+enum LaSect { S_SETUP, S_PRE, S_CONC, S_LEVEL0, S_LEVELN, S_APPLY, S_POST, S_OUT, S_COUNT };
 
 struct Engine {
     gpu_ctx_t ctx;
@@ -159,9 +164,16 @@ struct Engine {
     std::vector<MeshState> M;
     std::vector<void*> to_free;  // device heap chunks to free at the next flush
     double stats[PH_COUNT][3] = {};
+    double sect[S_COUNT] = {};
+    std::chrono::steady_clock::time_point t_mark = std::chrono::steady_clock::now();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int hull_phase = PH_IHULL;
 
+    void lap(int s) {
+        const auto t = std::chrono::steady_clock::now();
+        sect[s] += std::chrono::duration<double, std::milli>(t - t_mark).count();
+        t_mark = t;
+    }
     int tbeg() {
         if (!ev0) {
             HIP_TRY(hipEventCreate(&ev0));
@@ -251,8 +263,8 @@ struct Engine {
             const int m = who[k].first, i = who[k].second;
             MeshState& S = M[m];
             const int nc = h[k].n_comp;
-            if (nc <= 1 && S.parts[i].cc != 0) continue;
-            if (nc <= 1) {
+            if (nc == 0 && S.parts[i].cc != 0) continue;
+            if (nc == 0) {
                 S.parts[i].cc = ++S.cc_counter;  // first pass over this lineage: fresh id
                 continue;
             }
@@ -762,6 +774,7 @@ struct Engine {
             else next.push_back(m);
         }
         act = next;
+        lap(S_PRE);
         if (cutting.empty()) return 0;
         for (int m : act) M[m].iters = iter + 1;
         float med = P.threshold * 0.25f;
@@ -774,6 +787,7 @@ struct Engine {
             for (size_t c = 0; c < cutting.size(); ++c)
                 n_edge[cutting[c].first] = std::max(n_edge[cutting[c].first], (int)eplanes[c].size() / 4);
         }
+        lap(S_CONC);
         const int total_levels = P.depth;
         // the iteration's store of halves built by full cuts
         struct Half {
@@ -1046,6 +1060,7 @@ struct Engine {
                     }
                 cur.clear();
                 cur_c.clear();
+                lap(d == 0 ? S_LEVEL0 : S_LEVELN);
                 break;
             }
             for (Item& ch : nxt) {
@@ -1068,6 +1083,7 @@ struct Engine {
                 }
             cur = nxt;
             cur_c = nxt_c;
+            lap(d == 0 ? S_LEVEL0 : S_LEVELN);
             if (cur.empty()) break;
         }
         for (size_t i = 0; i < cur.size(); ++i) {
@@ -1133,7 +1149,9 @@ struct Engine {
         }
         for (const Half& hf : store)
             if (!hf.keep) to_free.push_back(hf.v);
-        return flush_frees();
+        const int frc = flush_frees();
+        lap(S_APPLY);
+        return frc;
     }
 };
 
@@ -1218,6 +1236,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         E.M[m].parts.push_back(p);
         act.push_back(m);
     }
+    E.lap(S_SETUP);
     for (int iter = 0; iter < P.max_iters && !act.empty() && rc == 0; ++iter) rc = E.iterate(act, iter);
     // meshes that ran out of iterations: last apply still needs components + hulls
     if (rc == 0 && !act.empty()) {
@@ -1235,6 +1254,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         for (int m = 0; m < n_meshes; ++m) all.push_back(m);
         rc = E.merge(all);
     }
+    E.lap(S_POST);
     // results
     if (rc == 0) {
         for (int m = 0; m < n_meshes && rc == 0; ++m) {
@@ -1272,9 +1292,6 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
             p.v = nullptr;
             p.hv = nullptr;
         }
-    if (stats_out)
-        for (int ph = 0; ph < PH_COUNT; ++ph)
-            for (int j = 0; j < 3; ++j) stats_out[3 * ph + j] = E.stats[ph][j];
     const int frc = E.flush_frees();
     for (void* b : E.ext_bufs) (void)hipFreeAsync(b, ctx->stream);
     HIP_TRY(hipFreeAsync(d_v, ctx->stream));
@@ -1282,6 +1299,12 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     HIP_TRY(hipFreeAsync(d_hv, ctx->stream));
     HIP_TRY(hipFreeAsync(d_ht, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    E.lap(S_OUT);
+    if (stats_out) {
+        for (int ph = 0; ph < PH_COUNT; ++ph)
+            for (int j = 0; j < 3; ++j) stats_out[3 * ph + j] = E.stats[ph][j];
+        for (int sc = 0; sc < S_COUNT; ++sc) stats_out[3 * PH_COUNT + sc] = E.sect[sc];
+    }
     if (rc) {
         for (int m = 0; m < n_meshes; ++m) gpu_result_free(&out[m]);
     }

@@ -76,7 +76,9 @@ struct LaCompTask {
     const int* t;
     int nv, nt;
     // results
-    int n_comp;          // 1: unchanged; >1: out[0..n_comp-1] hold the pieces
+    // 0: one piece, part unchanged; else out[0..n_comp-1] hold the pieces
+    // (a single one when inner shells were dropped from two or more)
+    int n_comp;
     LaCompOut* out;      // LA_MAX_COMP slots
 };
 
