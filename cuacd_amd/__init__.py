@@ -177,9 +177,11 @@ class Context:
 
     def batch_hull_dandc_mesh(self, pts_list):
         """Exact hull mesh per cloud: list of (verts, tris, volume)
-        (reference cuacd/__init__.py:318-361). Vertices are the input points,
-        in ascending input order; triangles wind counter-clockwise seen from
-        outside."""
+        (reference cuacd/__init__.py:318-361). Canonical form: the vertices
+        are the hull's corners (extreme points; among quantized duplicates the
+        lowest index), in ascending input order; each planar facet is a fan
+        from its lowest corner, counter-clockwise seen from outside; facets
+        are listed by their first two corners."""
         if len(pts_list) == 0:
             return []
         out_v, out_t, out_nv, out_nt, errs, vols, mhv, mht = self._hull_batch(pts_list)

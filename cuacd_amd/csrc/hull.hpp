@@ -97,7 +97,8 @@ template <int AS> struct HullWS {
     asptr<AS, uint16_t> hb;    // [VC]
     asptr<AS, uint16_t> hnew;  // [VC] new face built on horizon edge i
     asptr<AS, uint16_t> vout;  // [VC] output index of a hull vertex
-    asptr<AS, uint8_t> vref;   // [VC] vertex referenced by a live face
+    asptr<AS, uint8_t> vref;   // [VC] vertex referenced by a live face / corner flag
+    asptr<AS, int> cv;         // [2*VC] canonical form: facet list head, triangle offset per c0
     // Always in LDS, whatever AS is: these are updated with atomics and then
     // read back; global atomics execute in L2 while plain loads may hit a
     // stale line of the CU's L1, so atomic-then-load state stays in LDS.
@@ -143,6 +144,7 @@ __host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int
     w.hnew = (asptr<AS, uint16_t>)take(v * 2u);
     w.vout = (asptr<AS, uint16_t>)take(v * 2u);
     w.vref = (asptr<AS, uint8_t>)take(v);
+    w.cv = (asptr<AS, int>)take(v * 8u);
     (void)nw;  // bm / wpre / ctr live in HullBlockLDS (hull_ws_attach)
     w.VC = vc;
     w.FC = (int)fc;
@@ -787,6 +789,310 @@ __device__ __forceinline__ double hull_volume_f64(const HullWS<AS> w, HullBlockL
         acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
     }
     return block_sum_d<NW>(acc, sb) / 6.0;
+}
+
+// ---------------------------------------------------------------------------
+// Canonical form of a finished hull
+// ---------------------------------------------------------------------------
+// Where grid points are coplanar, the faces the quickhull leaves (and the
+// non-extreme points it keeps as vertices, inside a facet or on an edge)
+// depend on the insertion order. The canonical form depends on the point set
+// alone:
+//   * facets = maximal planar faces (adjacent faces merged when exactly
+//     coplanar on the grid);
+//   * corners = extreme points: facet boundary vertices where the boundary
+//     turns (exact grid cross product); output vertices = the corners in
+//     ascending local point order;
+//   * each facet is the fan (c0, c_k, c_k+1), k = 1..m-2, of its corners in
+//     counter-clockwise order (seen from outside) starting at its corner of
+//     lowest output index c0; facets are listed by (c0, c1);
+//   * volume: FP64 divergence sum over the fan triangles, original fp32
+//     coordinates, anchored at output vertex 0.
+// Scratch reused after block_quickhull, per face f: fvis = coplanar-edge bits
+// (bit 3: facet leader), fstamp = facet label (lowest face id of the facet),
+// then a leader's triangle offset; fpl.xyz = successor of each boundary edge
+// (3*face + slot, -1 inside a facet), fpl.w = a leader's edge ending at c0;
+// fv.w = a leader's boundary face, then its corner count; ffree = a leader's
+// c1; vis = next leader in the list of facets sharing c0. Per vertex: vref =
+// corner flag, vout = output index, cv = list head and triangle offset per c0.
+struct HullCanon {
+    int nv, nt;
+    double vol;
+};
+
+template <int AS> __device__ __forceinline__ int canon_ld(asptr<AS, int> p) {
+    if constexpr (AS == AS_LDS) return *p;
+    else return __hip_atomic_load((int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int AS> __device__ __forceinline__ int canon_exch(asptr<AS, int> p, int v) {
+    if constexpr (AS == AS_LDS) return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else return __hip_atomic_exchange((int*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int fpl_geti(f4v v, int j) {
+    return __float_as_int(j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w)));
+}
+__device__ __forceinline__ int u4_get(u4v v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w)); }
+
+template <int NW> __device__ __forceinline__ bool block_any(bool p, HullBlockLDS* sb) {
+    if (NW == 1) return ballot(p) != 0ull;
+    return block_sum_i<NW>(p ? 1 : 0, sb) != 0;
+}
+
+// Builds the canonical form's vertex set (vref/vout), per-facet fans and
+// offsets, and its volume. Block-uniform result; nt = 0 on a topology fault.
+template <int AS, int NW, class PosFn>
+__device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const HullResult& r, int n, PosFn pos) {
+    constexpr int T = 64 * NW;
+    const int tid = threadIdx.x;
+    const int ftop = r.ftop, FC = w.FC;
+    HullCanon out{0, 0, 0.0};
+    auto alive = [&](int f) { return w.fn[f].w != 0; };
+    auto opp_of = [&](int g, int a, int b) {
+        const u4v gv = w.fv[g];
+        return (gv.x != a && gv.x != b) ? (int)gv.x : ((gv.y != a && gv.y != b) ? (int)gv.y : (int)gv.z);
+    };
+    // A: exact coplanarity of each face with its neighbour across each edge
+    for (int f = tid; f < ftop; f += T) {
+        w.fstamp[f] = f;
+        if (!alive(f)) continue;
+        const u4v v = w.fv[f], nb = w.fn[f];
+        const s4v qa = w.vq[v.x], qb = w.vq[v.y], qc = w.vq[v.z];
+        int bits = 0;
+        for (int e = 0; e < 3; ++e) {
+            const int a = u4_get(v, e), b = u4_get(v, e == 2 ? 0 : e + 1);
+            const s4v q = w.vq[opp_of(u4_get(nb, e), a, b)];
+            if (orient_exact(qa, qb, qc, q.x, q.y, q.z) == 0.0) bits |= 1 << e;
+        }
+        w.fvis[f] = (uint8_t)bits;
+    }
+    for (int v = tid; v < r.nvert; v += T) w.vref[v] = 0;
+    __syncthreads();
+    // B: facet labels = lowest face id over coplanar adjacency (min propagation
+    // with pointer jumping; a round that changes nothing ends it)
+    for (int round = 0; round < FC; ++round) {
+        bool changed = false;
+        for (int f = tid; f < ftop; f += T) {
+            if (!alive(f)) continue;
+            const int bits = w.fvis[f] & 7;
+            if (!bits) continue;
+            const u4v nb = w.fn[f];
+            int l = w.fstamp[f];
+            const int l0 = l;
+            for (int e = 0; e < 3; ++e)
+                if (bits & (1 << e)) l = min(l, w.fstamp[u4_get(nb, e)]);
+            l = min(l, w.fstamp[l]);
+            if (l < l0) { w.fstamp[f] = l; changed = true; }
+        }
+        if (!block_any<NW>(changed, sb)) break;
+        __syncthreads();
+    }
+    __syncthreads();
+    // C: successor of every boundary edge inside its facet, corner flags, a
+    // boundary face per facet (any one: the walks cover the whole cycle)
+    int fault = 0;
+    for (int f = tid; f < ftop; f += T) {
+        if (!alive(f)) continue;
+        const int bits = w.fvis[f] & 7;
+        const u4v v = w.fv[f];
+        f4v sc = w.fpl[f];
+        int succ[3] = {-1, -1, -1};
+        for (int e = 0; e < 3; ++e) {
+            if (bits & (1 << e)) continue;
+            // rotate around the edge's end vertex b, staying inside the facet
+            const int b = u4_get(v, e == 2 ? 0 : e + 1);
+            int h = f, j = e == 2 ? 0 : e + 1;
+            int guard = 0;
+            while (((w.fvis[h] >> j) & 1) && guard < FC) {
+                const int g = u4_get(w.fn[h], j);
+                const u4v gv = w.fv[g];
+                j = gv.x == b ? 0 : (gv.y == b ? 1 : 2);
+                h = g;
+                ++guard;
+            }
+            if (guard >= FC) { fault = 1; break; }
+            succ[e] = 3 * h + j;
+            const int a = u4_get(v, e), c = u4_get(w.fv[h], j == 2 ? 0 : j + 1);
+            const s4v qa = w.vq[a], qb = w.vq[b], qc = w.vq[c];
+            const int ux = qb.x - qa.x, uy = qb.y - qa.y, uz = qb.z - qa.z;
+            const int vx = qc.x - qb.x, vy = qc.y - qb.y, vz = qc.z - qb.z;
+            if (uy * vz - uz * vy != 0 || uz * vx - ux * vz != 0 || ux * vy - uy * vx != 0) w.vref[b] = 1;
+            asptr<AS, uint16_t> fw = (asptr<AS, uint16_t>)(w.fv + w.fstamp[f]);
+            fw[3] = (uint16_t)f;
+        }
+        sc.x = __int_as_float(succ[0]);
+        sc.y = __int_as_float(succ[1]);
+        sc.z = __int_as_float(succ[2]);
+        w.fpl[f] = sc;
+    }
+    if (block_any<NW>(fault != 0, sb)) {
+        out.nt = -1;
+        return out;
+    }
+    __syncthreads();
+    // D: output indices of the corners, ascending local point order
+    {
+        const int nw = (n + 31) >> 5;
+        for (int i = tid; i < nw; i += T) w.bm[i] = 0;
+        for (int v = tid; v < r.nvert; v += T) { w.cv[2 * v] = -1; w.cv[2 * v + 1] = 0; }
+        __syncthreads();
+        for (int v = tid; v < r.nvert; v += T)
+            if (w.vref[v]) {
+                const int p = w.vpid[v];
+                atom_or(&w.bm[p >> 5], 1u << (p & 31));
+            }
+        __syncthreads();
+        int carry = 0;
+        for (int b = 0; b < nw; b += 64) {
+            int c = 0;
+            if (tid < 64 && b + tid < nw) c = __popc(w.bm[b + tid]);
+            const int incl = wave_incl_scan(c);
+            if (tid < 64 && b + tid < nw) w.wpre[b + tid] = carry + incl - c;
+            carry += __shfl(incl, 63);
+        }
+        if (tid == 0) sb->bc[7] = carry;
+        __syncthreads();
+        out.nv = sb->bc[7];
+        for (int v = tid; v < r.nvert; v += T)
+            if (w.vref[v]) {
+                const int p = w.vpid[v];
+                const int o = w.wpre[p >> 5] + __popc(w.bm[p >> 5] & ((1u << (p & 31)) - 1u));
+                w.vout[v] = (uint16_t)o;
+                if (o == 0) sb->bc[6] = v;
+            }
+        __syncthreads();
+    }
+    if (out.nv < 4) return out;
+    float an[3];
+    pos(w.vpid[sb->bc[6]], an);
+    const double ax = an[0], ay = an[1], az = an[2];
+    auto succ_of = [&](int e) { return fpl_geti(w.fpl[e / 3], e % 3); };
+    auto end_of = [&](int e) { const int h = e / 3, j = e % 3; return u4_get(w.fv[h], j == 2 ? 0 : j + 1); };
+    // E: per facet (its leader's thread): corners, c0, c1, fan volume
+    double acc = 0.0;
+    for (int f = tid; f < ftop; f += T) {
+        if (!alive(f) || w.fstamp[f] != f) continue;
+        const int h0 = u4_get(w.fv[f], 3);
+        const int b0 = w.fvis[h0] & 7;
+        const int j0 = !(b0 & 1) ? 0 : (!(b0 & 2) ? 1 : 2);
+        const int start = 3 * h0 + j0;
+        int m = 0, best = 0x7fffffff, ebest = start;
+        int e = start, guard = 0;
+        do {
+            const int b = end_of(e);
+            if (w.vref[b]) {
+                ++m;
+                if (w.vout[b] < best) { best = w.vout[b]; ebest = e; }
+            }
+            e = succ_of(e);
+        } while (e != start && ++guard < 3 * FC);
+        const int c0v = end_of(ebest);
+        float p0[3];
+        pos(w.vpid[c0v], p0);
+        const double x0 = p0[0] - ax, y0 = p0[1] - ay, z0 = p0[2] - az;
+        int c1 = -1, prev = -1;
+        e = succ_of(ebest);
+        guard = 0;
+        while (e != ebest && ++guard < 3 * FC) {
+            const int b = end_of(e);
+            if (w.vref[b]) {
+                if (c1 < 0) c1 = w.vout[b];
+                if (prev >= 0) {
+                    float p1[3], p2[3];
+                    pos(w.vpid[prev], p1);
+                    pos(w.vpid[b], p2);
+                    const double x1 = p1[0] - ax, y1 = p1[1] - ay, z1 = p1[2] - az;
+                    const double x2 = p2[0] - ax, y2 = p2[1] - ay, z2 = p2[2] - az;
+                    acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
+                }
+                prev = b;
+            }
+            e = succ_of(e);
+        }
+        w.fvis[f] = (uint8_t)(w.fvis[f] | 8);
+        f4v sc = w.fpl[f];
+        sc.w = __int_as_float(ebest);
+        w.fpl[f] = sc;
+        asptr<AS, uint16_t> fw = (asptr<AS, uint16_t>)(w.fv + f);
+        fw[3] = (uint16_t)m;
+        w.ffree[f] = (uint16_t)(c1 < 0 ? 0 : c1);
+        w.vis[f] = (uint16_t)canon_exch<AS>(w.cv + 2 * best, f);
+    }
+    out.vol = block_sum_d<NW>(acc, sb) / 6.0;
+    __syncthreads();
+    // F: triangle offsets: per c0 bucket (scan over output vertices), then by c1 inside it
+    auto tris_of = [&](int L) { return max(u4_get(w.fv[L], 3) - 2, 0); };
+    {
+        int carry = 0;
+        for (int b = 0; b < out.nv; b += T) {
+            const int o = b + tid;
+            int c = 0;
+            if (o < out.nv)
+                for (int L = canon_ld<AS>(w.cv + 2 * o), guard = 0; L >= 0 && L != 0xFFFF && guard < FC; ++guard) {
+                    c += tris_of(L);
+                    L = w.vis[L];
+                }
+            int incl = wave_incl_scan(c);
+            int wbase = 0, tot = __shfl(incl, 63);
+            if (NW > 1) {
+                const int wv = tid >> 6;
+                if (lane_id() == 63) sb->bc[wv] = incl;
+                __syncthreads();
+                tot = 0;
+                for (int q = 0; q < NW; ++q) {
+                    if (q < wv) wbase += sb->bc[q];
+                    tot += sb->bc[q];
+                }
+                __syncthreads();
+            }
+            if (o < out.nv) w.cv[2 * o + 1] = carry + wbase + incl - c;
+            carry += tot;
+        }
+        out.nt = carry;
+    }
+    __syncthreads();
+    for (int f = tid; f < ftop; f += T) {
+        if (!alive(f) || !(w.fvis[f] & 8)) continue;
+        const int c0 = w.vout[end_of(fpl_geti(w.fpl[f], 3))];
+        const int c1 = w.ffree[f];
+        int off = w.cv[2 * c0 + 1];
+        for (int L = canon_ld<AS>(w.cv + 2 * c0), guard = 0; L >= 0 && L != 0xFFFF && guard < FC; ++guard) {
+            if (L != f && w.ffree[L] < c1) off += tris_of(L);
+            L = w.vis[L];
+        }
+        w.fstamp[f] = off;
+    }
+    __syncthreads();
+    return out;
+}
+
+// Writes the canonical triangles (output vertex indices) after hull_canon.
+template <int AS, int NW>
+__device__ void hull_canon_write(const HullWS<AS> w, const HullResult& r, int* __restrict__ out_tris) {
+    constexpr int T = 64 * NW;
+    const int FC = w.FC;
+    auto succ_of = [&](int e) { return fpl_geti(w.fpl[e / 3], e % 3); };
+    auto end_of = [&](int e) { const int h = e / 3, j = e % 3; return u4_get(w.fv[h], j == 2 ? 0 : j + 1); };
+    for (int f = threadIdx.x; f < r.ftop; f += T) {
+        if (!w.fn[f].w || !(w.fvis[f] & 8)) continue;
+        const int ebest = fpl_geti(w.fpl[f], 3);
+        const int c0 = w.vout[end_of(ebest)];
+        int t = w.fstamp[f];
+        int prev = -1, e = succ_of(ebest), guard = 0;
+        while (e != ebest && ++guard < 3 * FC) {
+            const int b = end_of(e);
+            if (w.vref[b]) {
+                const int o = w.vout[b];
+                if (prev >= 0) {
+                    out_tris[3 * t] = c0;
+                    out_tris[3 * t + 1] = prev;
+                    out_tris[3 * t + 2] = o;
+                    ++t;
+                }
+                prev = o;
+            }
+            e = succ_of(e);
+        }
+    }
 }
 
 }  // namespace cuacd

@@ -168,15 +168,27 @@ __device__ __forceinline__ void hull_item(const HullItemArgs& a, int mode, int i
         }
         return;
     }
-    const int nv = hull_rank_vertices<AS, NW>(w, sb, r, n);
     if (mode == 1) {
+        // survivors: every hull vertex (a superset of the corners is enough)
+        const int nv = hull_rank_vertices<AS, NW>(w, sb, r, n);
         int* dst = a.out_sel + a.sel_off[item];
         for (int v = tid; v < r.nvert; v += T)
             if (w.vref[v]) dst[w.vout[v]] = orig(w.vpid[v]);
         if (tid == 0) a.out_cnt[item] = nv;
         return;
     }
-    const int nt = hull_count_faces<AS, NW>(w, sb, r);
+    auto pos = [&](int p, float* o3) {
+        const float* q = cpts + 3ll * orig(p);
+        o3[0] = q[0];
+        o3[1] = q[1];
+        o3[2] = q[2];
+    };
+    const HullCanon hc = hull_canon<AS, NW>(w, sb, r, n, pos);
+    if (hc.nt < 0) {
+        if (tid == 0) a.out_err[cloud] = KERR_HULL_CAPACITY;
+        return;
+    }
+    const int nv = hc.nv, nt = hc.nt;
     if (tid == 0) {
         a.out_nv[cloud] = nv;
         a.out_nt[cloud] = nt;
@@ -192,17 +204,8 @@ __device__ __forceinline__ void hull_item(const HullItemArgs& a, int mode, int i
                 dv[3 * o + 2] = p[2];
             }
     }
-    if (nt <= a.max_ht) hull_write_tris<AS, NW>(w, sb, r, a.out_tris + 3ll * a.max_ht * cloud);
-    if (a.out_vol) {
-        auto pos = [&](int p, float* o3) {
-            const float* q = cpts + 3ll * orig(p);
-            o3[0] = q[0];
-            o3[1] = q[1];
-            o3[2] = q[2];
-        };
-        const double vol = hull_volume_f64<AS, NW>(w, sb, r, pos);
-        if (tid == 0) a.out_vol[cloud] = vol;
-    }
+    if (nt <= a.max_ht) hull_canon_write<AS, NW>(w, r, a.out_tris + 3ll * a.max_ht * cloud);
+    if (a.out_vol && tid == 0) a.out_vol[cloud] = hc.vol;
 }
 
 template <int PPL>

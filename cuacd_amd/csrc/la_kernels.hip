@@ -104,11 +104,22 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         o3[1] = P[3ll * p + 1];
         o3[2] = P[3ll * p + 2];
     };
-    const double vol = hull_volume_f64<AS, NW>(w, sb, r, pos);
-    if (tid == 0) t.vol = vol;
+    const HullCanon hc = hull_canon<AS, NW>(w, sb, r, n, pos);
+    if (hc.nt < 0) {
+        if (tid == 0) {
+            t.vol = 0.0;
+            t.hnv = t.hnt = 0;
+            t.hv = nullptr;
+            t.ht = nullptr;
+            t.err = KERR_HULL_CAPACITY;
+            err_or(err, KERR_HULL_CAPACITY);
+        }
+        return;
+    }
+    if (tid == 0) t.vol = hc.vol;
     if (!t.emit) return;
-    const int nv = hull_rank_vertices<AS, NW>(w, sb, r, n);
-    const int nt = hull_count_faces<AS, NW>(w, sb, r);
+    const int nv = hc.nv;
+    const int nt = hc.nt;
     const unsigned long long vb = al16(12ull * nv);
     if (tid == 0) sb->red[0] = (unsigned long long)heap_alloc(pool, 0, vb + 12ull * nt, err);
     __syncthreads();
@@ -126,7 +137,7 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
             chunk[3 * o + 1] = P[3ll * p + 1];
             chunk[3 * o + 2] = P[3ll * p + 2];
         }
-    hull_write_tris<AS, NW>(w, sb, r, tris);
+    hull_canon_write<AS, NW>(w, r, tris);
     if (tid == 0) { t.hv = chunk; t.ht = tris; t.hnv = nv; t.hnt = nt; }
 }
 
@@ -157,8 +168,8 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
 // k-DOP prefilter (reference cuda/kdop_hull.cuh:116-337): extreme points along
 // the 40 axes -> rough inner hull -> optional bound pruning on its volume ->
 // keep the points strictly outside some rough-hull face, plus the rough hull's
-// vertices. The filtered set keeps the full set's quantization frame, so its
-// exact hull equals the full set's.
+// corners, in input order. The filtered set keeps the full set's quantization
+// frame and the input order, so its canonical hull equals the full set's.
 constexpr int KD_T = 256;
 constexpr int KD_NW = KD_T / 64;
 
@@ -167,6 +178,8 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
     __shared__ HullBlockLDS sb;
     __shared__ __align__(16) char lds[20480];
     __shared__ float ep[2 * KDOP_AXES_N][3];
+    __shared__ int epi[2 * KDOP_AXES_N];
+    __shared__ int rvi[2 * KDOP_AXES_N + 1];
     __shared__ float4 planes[2 * 128 + 8];
     __shared__ unsigned long long kred[KD_NW][16];
     __shared__ int s_cnt[KD_NW + 2];
@@ -232,6 +245,7 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             ep[slot][0] = P[3ll * idx];
             ep[slot][1] = P[3ll * idx + 1];
             ep[slot][2] = P[3ll * idx + 2];
+            epi[slot] = idx;
         }
         __syncthreads();
     }
@@ -256,7 +270,13 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         return;
     }
     auto epos = [&](int p, float* o3) { o3[0] = ep[p][0]; o3[1] = ep[p][1]; o3[2] = ep[p][2]; };
-    const double rough = hull_volume_f64<AS_LDS, KD_NW>(w, &sb, r, epos);
+    const HullCanon rc = hull_canon<AS_LDS, KD_NW>(w, &sb, r, NE, epos);
+    if (rc.nt <= 0) {
+        // no usable rough hull: no prefilter, hull the full set
+        if (tid == 0) { res[blockIdx.x] = make_int2(np, 0); t.use_frame = 1; t.frame = full; t.filt = nullptr; t.nf = 0; }
+        return;
+    }
+    const double rough = rc.vol;
     if (t.prune_gap < 1e29) {
         const double gap = fmax(rough - t.mesh_vol, 0.0);
         if (gap > t.prune_gap) {
@@ -270,17 +290,29 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             return;
         }
     }
-    const int nrv = hull_rank_vertices<AS_LDS, KD_NW>(w, &sb, r, NE);
-    // interior reference: centroid of the rough hull's vertices (output order)
+    const int nrv = rc.nv;
+    // interior reference: centroid of the rough hull's corners (ascending
+    // extreme slot); the corners' input indices, ascending, for the filter
     if (tid == 0) {
         float rx = 0.f, ry = 0.f, rz = 0.f;
+        int k = 0;
         for (int p = 0; p < NE; ++p) {
-            // output order = ascending local id among referenced vertices
             bool used = false;
             for (int v = 0; v < r.nvert; ++v)
                 if (w.vref[v] && w.vpid[v] == p) { used = true; break; }
-            if (used) { rx += ep[p][0]; ry += ep[p][1]; rz += ep[p][2]; }
+            if (used) {
+                rx += ep[p][0]; ry += ep[p][1]; rz += ep[p][2];
+                rvi[k++] = epi[p];
+            }
         }
+        // insertion sort (at most 80 entries), then a sentinel
+        for (int i = 1; i < k; ++i) {
+            const int x = rvi[i];
+            int j = i - 1;
+            while (j >= 0 && rvi[j] > x) { rvi[j + 1] = rvi[j]; --j; }
+            rvi[j + 1] = x;
+        }
+        rvi[k] = 0x7fffffff;
         const float inv = 1.f / (float)nrv;
         s_ref[0] = rx * inv; s_ref[1] = ry * inv; s_ref[2] = rz * inv;
     }
@@ -306,7 +338,7 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
     }
     __syncthreads();
     const int npl = s_cnt[KD_NW];
-    // filtered buffer: the points outside the rough hull, then its vertices
+    // filtered buffer: the points outside the rough hull and its corners, in input order
     if (tid == 0) sb.red[0] = (unsigned long long)heap_alloc(pool, 1, 12ull * (np + NE), err);
     __syncthreads();
     float* filt = (float*)sb.red[0];
@@ -326,6 +358,9 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
                 const float4 pl = planes[f];
                 if (pl.x * x + pl.y * y + pl.z * z > pl.w) { keep = true; break; }
             }
+            if (!keep)
+                for (int q = 0; rvi[q] <= i; ++q)
+                    if (rvi[q] == i) { keep = true; break; }
         }
         const uint64_t m = ballot(keep);
         if (lane == 0) s_cnt[wv] = __popcll(m);
@@ -339,23 +374,12 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         for (int q = 0; q < KD_NW; ++q) base += s_cnt[q];
         __syncthreads();
     }
-    // rough hull vertices in ascending extreme-slot order
     if (tid == 0) {
-        int o = base;
-        for (int p = 0; p < NE; ++p) {
-            bool used = false;
-            for (int v = 0; v < r.nvert; ++v)
-                if (w.vref[v] && w.vpid[v] == p) { used = true; break; }
-            if (used) {
-                filt[3 * o] = ep[p][0]; filt[3 * o + 1] = ep[p][1]; filt[3 * o + 2] = ep[p][2];
-                ++o;
-            }
-        }
         t.filt = filt;
-        t.nf = o;
+        t.nf = base;
         t.frame = full;
         t.use_frame = 1;
-        res[blockIdx.x] = make_int2(o, 0);
+        res[blockIdx.x] = make_int2(base, 0);
     }
 }
 

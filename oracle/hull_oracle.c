@@ -204,6 +204,148 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
     return failed ? -1 : out;
 }
 
+/* Canonical form of the exact hull (the GPU library emits the same):
+ *   facets = maximal planar faces (adjacent faces merged when exactly
+ *   coplanar); corners = facet boundary vertices where the boundary turns;
+ *   each facet is the fan (c0, c_k, c_k+1) over its corners in CCW order from
+ *   its lowest-index corner c0; facets ordered by (c0, c1).
+ * Writes is_vertex[n] = corner flags and the triangles (input indices);
+ * returns the triangle count, 0 for a flat input, -1 on failure. */
+static int ocn_find(int* uf, int x) {
+    while (uf[x] != x) { uf[x] = uf[uf[x]]; x = uf[x]; }
+    return x;
+}
+typedef struct { int c0, c1, f, e, m; } OFacet;
+static int ofacet_cmp(const void* a, const void* b) {
+    const OFacet* x = (const OFacet*)a;
+    const OFacet* y = (const OFacet*)b;
+    if (x->c0 != y->c0) return x->c0 < y->c0 ? -1 : 1;
+    if (x->c1 != y->c1) return x->c1 < y->c1 ? -1 : 1;
+    return 0;
+}
+int orc_hull_canon(const int* q, int n, unsigned char* is_vertex, int* out, int cap) {
+    memset(is_vertex, 0, (size_t)(n > 0 ? n : 1));
+    if (n < 4) return 0;
+    int fcap = 2 * n + 8;
+    int* F = (int*)malloc(sizeof(int) * 3 * (size_t)fcap);
+    unsigned char* tmpv = (unsigned char*)malloc((size_t)n);
+    if (!F || !tmpv) { free(F); free(tmpv); return -1; }
+    int nf = orc_hull(q, n, tmpv, F, fcap);
+    free(tmpv);
+    if (nf <= 0) { free(F); return nf; }
+    /* directed edge -> face */
+    OEdgeMap em;
+    uint64_t H = 64;
+    while (H < 2ull * 3ull * (uint64_t)nf + 2) H <<= 1;
+    em.k = (uint64_t*)malloc(8ull * H);
+    em.v = (int*)malloc(4ull * H);
+    em.mask = H - 1;
+    int* uf = (int*)malloc(sizeof(int) * (size_t)nf);
+    unsigned char* cop = (unsigned char*)malloc((size_t)nf);
+    int* succ = (int*)malloc(sizeof(int) * 3 * (size_t)nf);
+    unsigned char* corner = (unsigned char*)calloc((size_t)n, 1);
+    OFacet* fl = (OFacet*)malloc(sizeof(OFacet) * (size_t)nf);
+    int* start = (int*)malloc(sizeof(int) * (size_t)nf);
+    if (!em.k || !em.v || !uf || !cop || !succ || !corner || !fl || !start) {
+        free(F); free(em.k); free(em.v); free(uf); free(cop); free(succ); free(corner); free(fl); free(start);
+        return -1;
+    }
+    for (uint64_t k = 0; k < H; ++k) em.k[k] = ~0ull;
+    for (int f = 0; f < nf; ++f)
+        for (int e = 0; e < 3; ++e) *oem_slot(&em, F[3 * f + e], F[3 * f + (e + 1) % 3], 1) = f;
+    int fail = 0;
+    for (int f = 0; f < nf; ++f) {
+        uf[f] = f;
+        cop[f] = 0;
+        for (int e = 0; e < 3; ++e) {
+            int a = F[3 * f + e], b = F[3 * f + (e + 1) % 3];
+            int g = oem_get(&em, b, a);
+            if (g < 0) { fail = 1; continue; }
+            int o = F[3 * g];
+            if (o == a || o == b) o = F[3 * g + 1];
+            if (o == a || o == b) o = F[3 * g + 2];
+            if (orient64(q + 3 * F[3 * f], q + 3 * F[3 * f + 1], q + 3 * F[3 * f + 2], q + 3 * o) == 0)
+                cop[f] |= (unsigned char)(1 << e);
+        }
+    }
+    for (int f = 0; f < nf && !fail; ++f)
+        for (int e = 0; e < 3; ++e)
+            if (cop[f] & (1 << e)) {
+                int g = oem_get(&em, F[3 * f + (e + 1) % 3], F[3 * f + e]);
+                int a = ocn_find(uf, f), b = ocn_find(uf, g);
+                if (a != b) { if (a < b) uf[b] = a; else uf[a] = b; }
+            }
+    /* boundary successors (rotate around the end vertex inside the facet), corners */
+    for (int f = 0; f < nf && !fail; ++f)
+        for (int e = 0; e < 3; ++e) {
+            succ[3 * f + e] = -1;
+            if (cop[f] & (1 << e)) continue;
+            int b = F[3 * f + (e + 1) % 3];
+            int h = f, j = (e + 1) % 3, guard = 0;
+            while ((cop[h] >> j) & 1) {
+                int c = F[3 * h + (j + 1) % 3];
+                int g = oem_get(&em, c, b);
+                if (g < 0 || ++guard > nf) { fail = 1; break; }
+                j = F[3 * g] == b ? 0 : (F[3 * g + 1] == b ? 1 : 2);
+                h = g;
+            }
+            if (fail) break;
+            succ[3 * f + e] = 3 * h + j;
+            int a = F[3 * f + e], c = F[3 * h + (j + 1) % 3];
+            int64_t ux = q[3 * b] - q[3 * a], uy = q[3 * b + 1] - q[3 * a + 1], uz = q[3 * b + 2] - q[3 * a + 2];
+            int64_t vx = q[3 * c] - q[3 * b], vy = q[3 * c + 1] - q[3 * b + 1], vz = q[3 * c + 2] - q[3 * b + 2];
+            if (uy * vz - uz * vy || uz * vx - ux * vz || ux * vy - uy * vx) corner[b] = 1;
+        }
+    /* one boundary edge per facet */
+    for (int f = 0; f < nf; ++f) start[f] = -1;
+    for (int f = 0; f < nf && !fail; ++f)
+        for (int e = 0; e < 3; ++e)
+            if (!(cop[f] & (1 << e))) {
+                int L = ocn_find(uf, f);
+                if (start[L] < 0) start[L] = 3 * f + e;
+            }
+    int nfac = 0, nt = 0;
+    for (int L = 0; L < nf && !fail; ++L) {
+        if (ocn_find(uf, L) != L || start[L] < 0) continue;
+        int s0 = start[L], e = s0, m = 0, best = 0x7fffffff, eb = s0, guard = 0;
+        do {
+            int b = F[3 * (e / 3) + (e % 3 + 1) % 3];
+            if (corner[b]) { ++m; if (b < best) { best = b; eb = e; } }
+            e = succ[e];
+            if (e < 0 || ++guard > 3 * nf) { fail = 1; break; }
+        } while (e != s0);
+        if (fail) break;
+        int c1 = -1;
+        for (e = succ[eb]; e != eb; e = succ[e]) {
+            int b = F[3 * (e / 3) + (e % 3 + 1) % 3];
+            if (corner[b]) { c1 = b; break; }
+        }
+        fl[nfac].c0 = best; fl[nfac].c1 = c1; fl[nfac].f = L; fl[nfac].e = eb; fl[nfac].m = m;
+        ++nfac;
+        if (m >= 3) nt += m - 2;
+    }
+    if (!fail) {
+        qsort(fl, (size_t)nfac, sizeof(OFacet), ofacet_cmp);
+        int t = 0;
+        for (int k = 0; k < nfac; ++k) {
+            int eb = fl[k].e, c0 = fl[k].c0, prev = -1;
+            for (int e = succ[eb]; e != eb; e = succ[e]) {
+                int b = F[3 * (e / 3) + (e % 3 + 1) % 3];
+                if (!corner[b]) continue;
+                if (prev >= 0) {
+                    if (t < cap) { out[3 * t] = c0; out[3 * t + 1] = prev; out[3 * t + 2] = b; }
+                    ++t;
+                }
+                prev = b;
+            }
+        }
+        memcpy(is_vertex, corner, (size_t)n);
+        nt = t;
+    }
+    free(F); free(em.k); free(em.v); free(uf); free(cop); free(succ); free(corner); free(fl); free(start);
+    return fail ? -1 : nt;
+}
+
 /* 1 if the hull is in general position: no duplicate quantized points and no
  * input point other than a facet's three vertices lies on that facet's plane.
  * Then the vertex set and the facets are unique. */
@@ -281,9 +423,11 @@ int orc_hull_batch(const float* pts, const int* offsets, int n_clouds, unsigned 
         orc_quantize(pts + 3 * (size_t)s, n, q);
         int nf = orc_hull(q, n, is_vertex + s, faces, cap);
         if (nf < 0) { ++bad; nf = 0; }
+        general[c] = check_general ? orc_general_position(q, n, faces, nf) : -1;
+        nf = orc_hull_canon(q, n, is_vertex + s, faces, cap);
+        if (nf < 0) { ++bad; nf = 0; }
         n_faces[c] = nf;
         volumes[c] = orc_mesh_volume(pts + 3 * (size_t)s, faces, nf);
-        general[c] = check_general ? orc_general_position(q, n, faces, nf) : -1;
         free(q); free(faces);
     }
     return bad;
@@ -300,7 +444,7 @@ int orc_hull_batch_serial(const float* pts, const int* offsets, int n_clouds, un
         int* faces = (int*)malloc(sizeof(int) * 3 * (size_t)cap);
         if (!q || !faces) { ++bad; free(q); free(faces); continue; }
         orc_quantize(pts + 3 * (size_t)s, n, q);
-        int nf = orc_hull(q, n, is_vertex + s, faces, cap);
+        int nf = orc_hull_canon(q, n, is_vertex + s, faces, cap);
         if (nf < 0) { ++bad; nf = 0; }
         n_faces[c] = nf;
         volumes[c] = orc_mesh_volume(pts + 3 * (size_t)s, faces, nf);

@@ -23,6 +23,8 @@ def lib():
         L.orc_quantize.argtypes = [P, C.c_int, P]
         L.orc_hull.argtypes = [P, C.c_int, P, P, C.c_int]
         L.orc_hull.restype = C.c_int
+        L.orc_hull_canon.argtypes = [P, C.c_int, P, P, C.c_int]
+        L.orc_hull_canon.restype = C.c_int
         L.orc_general_position.argtypes = [P, C.c_int, P, C.c_int]
         L.orc_general_position.restype = C.c_int
         L.orc_mesh_volume.argtypes = [P, P, C.c_int]
@@ -54,14 +56,18 @@ def quantize(pts):
     return q
 
 
-def hull(q):
-    """Exact hull of integer points: (vertex flags bool[n], faces int[nf,3])."""
+def hull(q, canonical=True):
+    """Exact hull of integer points: (vertex flags bool[n], faces int[nf,3]).
+    canonical: corners only, facets as fans from their lowest-index corner,
+    ordered by (c0, c1) (the form the GPU library emits); otherwise the
+    incremental construction's own faces and vertices."""
     q = np.ascontiguousarray(q, np.int32).reshape(-1, 3)
     n = len(q)
     flags = np.zeros(max(n, 1), np.uint8)
     cap = 2 * n + 8
     faces = np.zeros((cap, 3), np.int32)
-    nf = lib().orc_hull(_p(q), n, _p(flags), _p(faces), cap)
+    fn = lib().orc_hull_canon if canonical else lib().orc_hull
+    nf = fn(_p(q), n, _p(flags), _p(faces), cap)
     if nf < 0:
         raise RuntimeError("oracle hull failed")
     return flags[:n].astype(bool), faces[:nf].copy()

@@ -148,3 +148,46 @@ def test_error_injection_propagates(ctx):
     assert errs[0] & (1 << 16)
     _, errs = ctx.batch_hull_volume([cube])
     assert errs[0] == 0
+
+
+def _canonical_expect(p):
+    q = oracle.quantize(p)
+    flags, tris = oracle.hull(q)
+    rank = -np.ones(len(p), np.int64)
+    idx = np.nonzero(flags)[0]
+    rank[idx] = np.arange(len(idx))
+    return p[idx], rank[tris].astype(np.int32) if len(tris) else np.zeros((0, 3), np.int32), tris
+
+
+@pytest.mark.parametrize("n,span", [(60, 2), (300, 3), (1000, 4), (1024, 6), (4000, 5), (12000, 8), (30000, 9)])
+def test_canonical_mesh_equals_oracle_on_lattices(ctx, n, span):
+    """Coplanar and collinear points everywhere (small integer lattices plus
+    duplicates): the emitted mesh - corners, fan triangles, their order - is
+    the oracle's canonical hull exactly, in every tier."""
+    rng = np.random.default_rng(n + span)
+    p = (rng.integers(0, span + 1, size=(n, 3)) * 0.25 - 0.3).astype(np.float32)
+    (hv, ht, vol), = ctx.batch_hull_dandc_mesh([p])
+    ev, et, ot = _canonical_expect(p)
+    assert hv.tobytes() == ev.tobytes()
+    assert np.array_equal(ht, et)
+    assert abs(vol - oracle.mesh_volume(p, ot)) <= 1e-12 * max(abs(vol), 1.0)
+
+
+def test_canonical_mesh_equals_oracle_on_cut_like_sets(ctx):
+    """Half-ball point sets with a planar cap (the lookahead's typical hull
+    input): the cap facet's points are coplanar on the grid."""
+    rng = np.random.default_rng(11)
+    outs = []
+    for n in (200, 900, 3000):
+        s = rng.standard_normal((n, 3))
+        s /= np.linalg.norm(s, axis=1, keepdims=True)
+        s = s[s[:, 2] > 0]
+        ang = rng.random(n // 4) * 2 * np.pi
+        cap = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], 1) * rng.random((n // 4, 1))
+        outs.append(np.concatenate([s, cap]).astype(np.float32))
+    res = ctx.batch_hull_dandc_mesh(outs)
+    kres = ctx.batch_kdop_hull_mesh(outs)
+    for p, (hv, ht, vol), (kv, kt, kvol) in zip(outs, res, kres):
+        ev, et, ot = _canonical_expect(p)
+        assert hv.tobytes() == ev.tobytes() and np.array_equal(ht, et)
+        assert kv.tobytes() == ev.tobytes() and np.array_equal(kt, et)

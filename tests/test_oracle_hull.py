@@ -147,3 +147,99 @@ def test_planted_mistake_is_caught():
     _, f = oracle.hull(oracle.quantize(p))
     assert oracle.mesh_volume(p, f) > 0
     assert oracle.mesh_volume(p, f[:, ::-1].copy()) < 0
+
+
+# ---- canonical form (what the GPU library emits) -----------------------------
+
+def _facets(q, tris):
+    """Group triangles by exact supporting plane: {plane key: [triangles]}."""
+    out = {}
+    for t in tris:
+        a, b, c = (q[i].astype(np.int64) for i in t)
+        n = np.cross(b - a, c - a)
+        g = np.gcd.reduce(np.abs(n))
+        n = n // g
+        out.setdefault((*n.tolist(), int(n @ a)), []).append(tuple(int(i) for i in t))
+    return out
+
+
+def _degenerate_cloud(rng, n=120, span=3):
+    """Points on a tiny integer lattice: many coplanar and collinear points,
+    duplicates included."""
+    return rng.integers(0, span + 1, size=(n, 3)).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_canonical_hull_on_degenerate_lattices(seed):
+    """Corners = Qhull's extreme points; every facet a fan from its lowest-index
+    corner with no degenerate triangle; all points on or below every facet;
+    closed and outward; volume = Qhull's; facets ordered by (c0, c1)."""
+    rng = np.random.default_rng(seed)
+    q = _degenerate_cloud(rng, span=2 + seed % 3)
+    flags, tris = oracle.hull(q)
+    if len(tris) == 0:
+        pytest.skip("flat sample")
+    # duplicates resolve to the lowest index: compare corner coordinates
+    qh = scipy.spatial.ConvexHull(q.astype(np.float64))
+    want = sorted(set(map(tuple, q[qh.vertices].tolist())))
+    got = sorted(map(tuple, q[flags].tolist()))
+    assert got == want
+    for i in np.nonzero(flags)[0]:
+        assert not any((q[j] == q[i]).all() for j in range(i)), "a corner must be the lowest-index duplicate"
+    assert oracle.edge_manifold_report(tris) == (0, 0)
+    assert abs(oracle.mesh_volume(q.astype(np.float32), tris) - qh.volume) < 1e-9
+    fac = _facets(q, tris)
+    for (nx, ny, nz, d), ts in fac.items():
+        nrm = np.array([nx, ny, nz], np.int64)
+        assert (q.astype(np.int64) @ nrm <= d).all()
+        c0 = ts[0][0]
+        assert all(t[0] == c0 for t in ts)
+        corners = {i for t in ts for i in t}
+        assert c0 == min(corners)
+        assert len(ts) == len(corners) - 2
+        for t in ts:
+            a, b, c = (q[i].astype(np.int64) for i in t)
+            assert np.cross(b - a, c - a).any()
+    # facets ordered by (c0, c1): triangles sorted by their first two vertices
+    # at facet starts
+    starts, seen = [], set()
+    for t in tris:
+        a, b, c = (q[i].astype(np.int64) for i in t)
+        key = None
+        for k, ts in fac.items():
+            if tuple(int(i) for i in t) in ts:
+                key = k
+        if key not in seen:
+            seen.add(key)
+            starts.append((t[0], t[1]))
+    assert starts == sorted(starts)
+
+
+def test_canonical_hull_cube_with_face_and_edge_points():
+    """A cube with extra points in face interiors and on edges: 8 corners,
+    12 triangles, each square a 2-triangle fan from its lowest corner."""
+    cube = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0],
+                     [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2]], np.int32)
+    extra = np.array([[1, 1, 0], [1, 0, 0], [2, 1, 1], [1, 2, 2], [0, 1, 1], [1, 1, 2]], np.int32)
+    for order in (np.arange(14), np.random.default_rng(5).permutation(14)):
+        q = np.concatenate([extra, cube])[order]
+        flags, tris = oracle.hull(q)
+        assert sorted(map(tuple, q[flags].tolist())) == sorted(map(tuple, cube.tolist()))
+        assert len(tris) == 12
+        assert abs(oracle.mesh_volume(q.astype(np.float32), tris) - 8.0) < 1e-12
+        for ts in _facets(q, tris).values():
+            corners = {i for t in ts for i in t}
+            assert len(ts) == 2 and ts[0][0] == ts[1][0] == min(corners)
+
+
+def test_canonical_hull_equals_raw_in_general_position():
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((300, 3)).astype(np.float32)
+    q = oracle.quantize(p)
+    f_raw, t_raw = oracle.hull(q, canonical=False)
+    f_can, t_can = oracle.hull(q)
+    assert oracle.general_position(q, t_raw)
+    assert (f_raw == f_can).all()
+    assert sorted(map(lambda t: tuple(np.roll(t, -int(np.argmin(t)))), t_raw.tolist())) == \
+        sorted(map(tuple, t_can.tolist()))
+    assert abs(oracle.mesh_volume(p, t_raw) - oracle.mesh_volume(p, t_can)) < 1e-12
