@@ -26,6 +26,22 @@ PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_
               n_concave_edges=16, merge_hulls=True)
 
 
+def _hull_kernel(ppl):
+    def name(wide_min):
+        if wide_min > 0 and ppl >= 4 and 64 * ppl > wide_min:
+            return f"la_hull_lds4<{ppl // 4}>"
+        return f"la_hull_wave<{ppl}>"
+    return name
+
+
+# hull kernel classes reported by the library -> kernel names
+KERNEL_OF = {"hull_w64": _hull_kernel(1), "hull_w128": _hull_kernel(2), "hull_w256": _hull_kernel(4),
+             "hull_w512": _hull_kernel(8), "hull_w1024": _hull_kernel(16),
+             "hull_b2048": lambda w: "la_hull_block<4>", "hull_b4096": lambda w: "la_hull_block<8>",
+             "hull_b8192": lambda w: "la_hull_block<16>", "hull_b16384": lambda w: "la_hull_block<32>",
+             "kdop_prefilter": lambda w: "la_kdop_kernel"}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -112,8 +128,19 @@ def main():
     sec = ctx.last_lookahead_sections
     log("  host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()) +
         f" (sum {sum(sec.values()) / 1e3:.2f} s)")
-    dom = max(stats, key=lambda k: stats[k]["ms"])
-    d = stats[dom]
+    # single-kernel timings: HIP events around each launch of that kernel alone
+    kern = {}
+    for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel"),
+                     ("hausdorff", "hd_task_kernel")):
+        kern[name] = stats[ph]
+    wide_min = int(os.environ.get("CUACD_HULL_WIDE_MIN", "512"))
+    for cls, k in ctx.last_lookahead_kernels.items():
+        kern[KERNEL_OF[cls](wide_min)] = k
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
+        log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
+            f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
 
     cpu = None
@@ -143,7 +170,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],
-                     "alg_bytes_total": d["alg_bytes"]},
+                     "avg_launch_ms": d["ms"] / max(d["launches"], 1),
+                     "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -35,6 +35,9 @@ LOOKAHEAD_PHASES = ("components", "part_hulls", "hausdorff", "concave_edges", "l
 # host wall-time sections of the same run (Context.last_lookahead_sections, ms)
 LOOKAHEAD_SECTIONS = ("setup", "refresh_select", "concave_planes", "level0", "deeper_levels", "evaluate_apply",
                       "post_merge", "readback")
+# hull kernel classes timed per launch (Context.last_lookahead_kernels)
+LOOKAHEAD_KERNELS = ("hull_w64", "hull_w128", "hull_w256", "hull_w512", "hull_w1024", "hull_b2048", "hull_b4096",
+                     "hull_b8192", "hull_b16384", "kdop_prefilter")
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
@@ -425,7 +428,7 @@ class Context:
         if width + 4 * n_concave_edges >= 512:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
-        stats = np.zeros(32, np.float64)
+        stats = np.zeros(62, np.float64)
         raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
                                    hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
                                    max_iters, width, width2, float(threshold), depth, max_n_cutting,
@@ -437,6 +440,10 @@ class Context:
                                             "alg_bytes": float(ph[i, 2])}
                                      for i, name in enumerate(LOOKAHEAD_PHASES)}
         self.last_lookahead_sections = {name: float(stats[24 + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
+        kt = stats[32:62].reshape(10, 3)
+        self.last_lookahead_kernels = {name: {"ms": float(kt[i, 0]), "launches": int(kt[i, 1]),
+                                              "alg_bytes": float(kt[i, 2])}
+                                       for i, name in enumerate(LOOKAHEAD_KERNELS)}
         out = [[_part_tuple(r)[:4] for r in parts] for parts in raw]
         if not return_info:
             return out

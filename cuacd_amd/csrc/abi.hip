@@ -39,6 +39,52 @@ void* cuacd::stage_buf(gpu_ctx_t ctx, int slot, size_t bytes) {
     return ctx->stage[slot];
 }
 
+hipEvent_t cuacd::kt_begin(gpu_ctx_t ctx) {
+    KTimers* k = ctx->kt;
+    if (!k) return nullptr;
+    hipEvent_t e = nullptr;
+    if (!k->free_ev.empty()) {
+        e = k->free_ev.back();
+        k->free_ev.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    (void)hipEventRecord(e, ctx->stream);
+    return e;
+}
+
+void cuacd::kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes) {
+    KTimers* k = ctx->kt;
+    if (!k || !a) return;
+    hipEvent_t b = nullptr;
+    if (!k->free_ev.empty()) {
+        b = k->free_ev.back();
+        k->free_ev.pop_back();
+    } else if (hipEventCreate(&b) != hipSuccess) {
+        k->free_ev.push_back(a);
+        return;
+    }
+    (void)hipEventRecord(b, ctx->stream);
+    k->pending.push_back({kid, a, b, bytes});
+}
+
+int cuacd::kt_resolve(gpu_ctx_t ctx) {
+    KTimers* k = ctx->kt;
+    if (!k) return 0;
+    for (auto& r : k->pending) {
+        float ms = 0.f;
+        HIP_TRY(hipEventSynchronize(r.b));
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        k->stats[r.kid][0] += ms;
+        k->stats[r.kid][1] += 1;
+        k->stats[r.kid][2] += r.bytes;
+        k->free_ev.push_back(r.a);
+        k->free_ev.push_back(r.b);
+    }
+    k->pending.clear();
+    return 0;
+}
+
 int cuacd::read_and_clear_err(gpu_ctx_t ctx) {
     int h = 0;
     if (hipMemcpyAsync(&h, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)

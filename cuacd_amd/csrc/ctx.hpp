@@ -7,6 +7,29 @@
 #include "cuacd_amd.h"
 #include "dheap.hpp"
 
+#include <vector>
+
+namespace cuacd {
+// Per-kernel timing (HIP events around individual launches on the context's
+// stream) for the benchmark's roofline figures. Enabled by the lookahead
+// search while it runs; ids name kernel classes.
+enum KtId {
+    KT_HULL_W1, KT_HULL_W2, KT_HULL_W4, KT_HULL_W8, KT_HULL_W16,  // one-workgroup LDS hulls by size class
+    KT_HULL_B4, KT_HULL_B8, KT_HULL_B16, KT_HULL_B32,              // multi-wave hulls, workspace in HBM
+    KT_KDOP, KT_COUNT
+};
+struct KTimers {
+    struct Rec {
+        int kid;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> free_ev;
+    double stats[KT_COUNT][3] = {};  // ms, launches, algorithmic bytes
+};
+}  // namespace cuacd
+
 struct gpu_ctx {
     int device;
     int n_cu;
@@ -21,6 +44,7 @@ struct gpu_ctx {
     void* stage[8];
     size_t stage_cap[8];
     char last_error[512];
+    cuacd::KTimers* kt;      // per-kernel timers, null when off
 };
 
 namespace cuacd {
@@ -33,6 +57,13 @@ void* stage_buf(gpu_ctx_t ctx, int slot, size_t bytes);
 // Read the kernel error word written by the last launches (synchronises) and
 // clear it. Applies host-injected bits. Returns the word.
 int read_and_clear_err(gpu_ctx_t ctx);
+
+// Per-kernel timers: kt_begin records the start event (a = nullptr when off),
+// kt_end the stop event with the launch's algorithmic bytes; kt_resolve
+// (after the stream has passed them) accumulates the elapsed times.
+hipEvent_t kt_begin(gpu_ctx_t ctx);
+void kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes);
+int kt_resolve(gpu_ctx_t ctx);
 
 }  // namespace cuacd
 

@@ -74,23 +74,33 @@ __device__ __forceinline__ int dh_class_of(unsigned long long total) {
 
 __device__ __forceinline__ int dh_stripe() { return (int)(blockIdx.x & (DH_STRIPES - 1)); }
 
-// Pop one block of class c from stripe s; nullptr if that stripe is empty.
-__device__ inline char* dh_pop(DPool* pool, DHeap* hp, int c, int s) {
+// Heads and links are read with relaxed agent-scope atomics: they live in L2,
+// the point of coherence, and the link's address depends on the head just
+// read. Only a successful pop acquires (one L1 invalidation per block taken,
+// not one per stripe looked at).
+__device__ __forceinline__ unsigned long long dh_head(DHeap* hp, int c, int s) {
+    return __hip_atomic_load(&hp->heads[c][s].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pop one block of class c from stripe s starting from head value `old`;
+// nullptr if the stripe is (or becomes) empty.
+__device__ inline char* dh_pop_from(DPool* pool, DHeap* hp, int c, int s, unsigned long long old) {
     char* base = pool->base;
-    unsigned long long old = ld_acquire(&hp->heads[c][s].v);
     while ((uint32_t)old != DH_EMPTY) {
         uint32_t off = (uint32_t)old;
         DBlockHdr* b = (DBlockHdr*)(base + ((unsigned long long)off << 8));
         uint32_t nxt = __hip_atomic_load(&b->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned long long neu = (((old >> 32) + 1ull) << 32) | nxt;
-        if (cas_acq_rel(&hp->heads[c][s].v, old, neu)) return (char*)b;
+        if (__hip_atomic_compare_exchange_strong(&hp->heads[c][s].v, &old, neu, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return (char*)b;
     }
     return nullptr;
 }
 
 // Push the chain first..last (linked through next, offsets / 256) onto stripe s.
 __device__ inline void dh_push_chain(DHeap* hp, int c, int s, DBlockHdr* last, uint32_t first_off) {
-    unsigned long long old = ld_acquire(&hp->heads[c][s].v);
+    unsigned long long old = dh_head(hp, c, s);
     for (;;) {
         __hip_atomic_store(&last->next, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned long long neu = (((old >> 32) + 1ull) << 32) | first_off;
@@ -105,8 +115,15 @@ __device__ inline void* heap_alloc(DPool* pool, int h, unsigned long long bytes,
     const int c = dh_class_of(bytes + DH_HDR);
     if (c >= DH_NCLASS) { err_or(err, KERR_HEAP_OOM); return nullptr; }
     const int s0 = dh_stripe();
-    char* blk = dh_pop(pool, hp, c, s0);
-    for (int k = 1; !blk && k < DH_STRIPES; ++k) blk = dh_pop(pool, hp, c, (s0 + k) & (DH_STRIPES - 1));
+    char* blk = dh_pop_from(pool, hp, c, s0, dh_head(hp, c, s0));
+    if (!blk) {
+        // look at every other stripe at once (independent loads), then try the non-empty ones
+        unsigned long long hv[DH_STRIPES];
+#pragma unroll
+        for (int k = 1; k < DH_STRIPES; ++k) hv[k] = dh_head(hp, c, (s0 + k) & (DH_STRIPES - 1));
+        for (int k = 1; !blk && k < DH_STRIPES; ++k)
+            if ((uint32_t)hv[k] != DH_EMPTY) blk = dh_pop_from(pool, hp, c, (s0 + k) & (DH_STRIPES - 1), hv[k]);
+    }
     if (!blk) {
         unsigned long long sz = 256ull << c;
         unsigned long long o = fetch_add_relaxed(&pool->bump, sz);

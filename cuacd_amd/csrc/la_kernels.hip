@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 #include "lookahead.hpp"
 #include "plane_cut.hpp"
@@ -149,6 +150,22 @@ __global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int*
     LaHullTask& t = tasks[ids[blockIdx.x]];
     const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
     la_hull_item<PPL, AS_LDS, 1>(t, w, &sb, pool, err);
+    if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+}
+
+// Four waves per hull, workspace in LDS: for the larger one-workgroup sets,
+// where the LDS footprint allows one hull per CU anyway, all four SIMDs work.
+constexpr int LA_LDS4_NW = 4;
+template <int PPL>
+__global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* tasks, const int* ids, int n, DPool* pool,
+                                                               int* err) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ HullBlockLDS sb;
+    if ((int)blockIdx.x >= n) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    constexpr int CAP = 64 * LA_LDS4_NW * PPL;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP), &sb);
+    la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err);
     if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
 
@@ -441,28 +458,35 @@ __global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, D
 // ---------------------------------------------------------------------------
 // Leaf scoring: half-space volumes and hull point sets of both sides
 // ---------------------------------------------------------------------------
+// One pass over the triangles accumulates both sides' volumes, marks the
+// vertices of each side and lists the triangles with vertices strictly on
+// both sides (in index order); the crossing points - one per undirected edge
+// with strictly opposite signs, in order of first occurrence over (triangle,
+// edge) - come from that short list alone.
 __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int n, DPool* pool, int* err) {
     __shared__ PcLDS s;
     __shared__ double dsum[2][PC_T / 64];
+    __shared__ int wcnt[PC_T / 64];
     if ((int)blockIdx.x >= n) return;
     LaScoreTask& T = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6;
     const float* V = T.v;
     const int* Tr = T.t;
     const int nv = T.nv, nt = T.nt;
     const float pa = T.plane[0], pb = T.plane[1], pc = T.plane[2], pd = T.plane[3];
-    // A: signs (int8), side marks (int), crossing-instance offsets, new-point offsets
-    const unsigned long long szA = al16(nv) + 4ull * nv + 8ull * nt + 64;
+    // A: signs (int8), side flags (two byte arrays), crossing-triangle list
+    const unsigned long long szA = 3ull * al16(nv) + 4ull * nt + 64;
     char* A = (char*)pc_alloc(pool, szA, &s, 1, err, 1);
     if (!A) return;
     int8_t* sg = (int8_t*)A;
-    int* mark = (int*)(A + al16(nv));
-    int* toff = mark + nv;
-    int* tnew = toff + nt;
+    uint8_t* m1 = (uint8_t*)(A + al16(nv));
+    uint8_t* m2 = m1 + al16(nv);
+    int* xl = (int*)(m2 + al16(nv));
     for (int i = tid; i < nv; i += PC_T) {
         const float d = pc_side(pa, pb, pc, pd, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
         sg[i] = d > PC_EPS ? 1 : (d < -PC_EPS ? -1 : 0);
-        mark[i] = 0;
+        m1[i] = 0;
+        m2[i] = 0;
     }
     __syncthreads();
     // divergence origin: the point of the plane nearest the coordinate origin
@@ -485,76 +509,98 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         o[1] = ay + t * (by - ay);
         o[2] = az + t * (bz - az);
     };
+    auto mark = [&](int i, int side) {
+        if (side & 1) m1[i] = 1;
+        if (side & 2) m2[i] = 1;
+    };
     double vp = 0.0, vn = 0.0;
     int cntp = 0, cntn = 0;
-    for (int k = tid; k < nt; k += PC_T) {
-        const int i0 = Tr[3 * k], i1 = Tr[3 * k + 1], i2 = Tr[3 * k + 2];
-        const int s0 = sg[i0], s1 = sg[i1], s2 = sg[i2];
-        const float* p0 = V + 3 * i0;
-        const float* p1 = V + 3 * i1;
-        const float* p2 = V + 3 * i2;
-        const int hp = (s0 > 0) | (s1 > 0) | (s2 > 0), hn = (s0 < 0) | (s1 < 0) | (s2 < 0);
-        if (!hp || !hn) {
-            int side;
-            if (hn) side = 2;
-            else if (hp) side = 1;
-            else {
-                const float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
-                const float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
-                const float dt = pa * (e1y * e2z - e1z * e2y) + pb * (e1z * e2x - e1x * e2z) + pc * (e1x * e2y - e1y * e2x);
-                side = dt > 0 ? 1 : 2;
+    int nx = 0;
+    for (int b0 = 0; b0 < nt; b0 += PC_T) {
+        const int k = b0 + tid;
+        bool crossing = false;
+        if (k < nt) {
+            const int i0 = Tr[3 * k], i1 = Tr[3 * k + 1], i2 = Tr[3 * k + 2];
+            const int s0 = sg[i0], s1 = sg[i1], s2 = sg[i2];
+            const float* p0 = V + 3 * i0;
+            const float* p1 = V + 3 * i1;
+            const float* p2 = V + 3 * i2;
+            const int hp = (s0 > 0) | (s1 > 0) | (s2 > 0), hn = (s0 < 0) | (s1 < 0) | (s2 < 0);
+            if (!hp || !hn) {
+                int side;
+                if (hn) side = 2;
+                else if (hp) side = 1;
+                else {
+                    const float e1x = p1[0] - p0[0], e1y = p1[1] - p0[1], e1z = p1[2] - p0[2];
+                    const float e2x = p2[0] - p0[0], e2y = p2[1] - p0[1], e2z = p2[2] - p0[2];
+                    const float dt = pa * (e1y * e2z - e1z * e2y) + pb * (e1z * e2x - e1x * e2z) + pc * (e1x * e2y - e1y * e2x);
+                    side = dt > 0 ? 1 : 2;
+                }
+                const double v = tet(p0, p1, p2);
+                if (side == 1) { vp += v; ++cntp; } else { vn += v; ++cntn; }
+                mark(i0, side); mark(i1, side); mark(i2, side);
+            } else {
+                crossing = true;
+                const int vi[3] = {i0, i1, i2}, si[3] = {s0, s1, s2};
+                int on = -1;
+                for (int q = 0; q < 3; ++q) if (si[q] == 0) { on = q; break; }
+                if (on >= 0) {
+                    const int ov = vi[on], a = vi[(on + 1) % 3], b = vi[(on + 2) % 3], sa = si[(on + 1) % 3];
+                    float x[3];
+                    xpt(a, b, x);
+                    const float* O = V + 3 * ov;
+                    const float* A3 = V + 3 * a;
+                    const float* B3 = V + 3 * b;
+                    if (sa > 0) { vp += tet(O, A3, x); vn += tet(O, x, B3); }
+                    else { vn += tet(O, A3, x); vp += tet(O, x, B3); }
+                    ++cntp; ++cntn;
+                    mark(ov, 3);
+                    mark(a, sa > 0 ? 1 : 2);
+                    mark(b, sa > 0 ? 2 : 1);
+                } else {
+                    int lone = 0;
+                    for (int q = 0; q < 3; ++q)
+                        if (si[q] != si[(q + 1) % 3] && si[q] != si[(q + 2) % 3]) { lone = q; break; }
+                    const int lv = vi[lone], o1 = vi[(lone + 1) % 3], o2 = vi[(lone + 2) % 3];
+                    float x1[3], x2[3];
+                    xpt(lv, o1, x1);
+                    xpt(lv, o2, x2);
+                    const float* L = V + 3 * lv;
+                    const float* Q1 = V + 3 * o1;
+                    const float* Q2 = V + 3 * o2;
+                    const double a1 = tet(L, x1, x2);
+                    const double a2 = tet(Q1, x2, x1) + tet(Q1, Q2, x2);
+                    const int ls = si[lone] > 0 ? 1 : 2;
+                    if (ls == 1) { vp += a1; vn += a2; } else { vn += a1; vp += a2; }
+                    ++cntp; ++cntn;
+                    mark(lv, ls);
+                    mark(o1, 3 - ls);
+                    mark(o2, 3 - ls);
+                }
             }
-            const double v = tet(p0, p1, p2);
-            if (side == 1) { vp += v; ++cntp; } else { vn += v; ++cntn; }
-            atomicOr(&mark[i0], side); atomicOr(&mark[i1], side); atomicOr(&mark[i2], side);
-            continue;
         }
-        const int vi[3] = {i0, i1, i2}, si[3] = {s0, s1, s2};
-        int on = -1;
-        for (int q = 0; q < 3; ++q) if (si[q] == 0) { on = q; break; }
-        if (on >= 0) {
-            const int ov = vi[on], a = vi[(on + 1) % 3], b = vi[(on + 2) % 3], sa = si[(on + 1) % 3];
-            float x[3];
-            xpt(a, b, x);
-            const float* O = V + 3 * ov;
-            const float* A3 = V + 3 * a;
-            const float* B3 = V + 3 * b;
-            if (sa > 0) { vp += tet(O, A3, x); vn += tet(O, x, B3); }
-            else { vn += tet(O, A3, x); vp += tet(O, x, B3); }
-            ++cntp; ++cntn;
-            atomicOr(&mark[ov], 3);
-            atomicOr(&mark[a], sa > 0 ? 1 : 2);
-            atomicOr(&mark[b], sa > 0 ? 2 : 1);
-        } else {
-            int lone = 0;
-            for (int q = 0; q < 3; ++q)
-                if (si[q] != si[(q + 1) % 3] && si[q] != si[(q + 2) % 3]) { lone = q; break; }
-            const int lv = vi[lone], o1 = vi[(lone + 1) % 3], o2 = vi[(lone + 2) % 3];
-            float x1[3], x2[3];
-            xpt(lv, o1, x1);
-            xpt(lv, o2, x2);
-            const float* L = V + 3 * lv;
-            const float* Q1 = V + 3 * o1;
-            const float* Q2 = V + 3 * o2;
-            const double a1 = tet(L, x1, x2);
-            const double a2 = tet(Q1, x2, x1) + tet(Q1, Q2, x2);
-            const int ls = si[lone] > 0 ? 1 : 2;
-            if (ls == 1) { vp += a1; vn += a2; } else { vn += a1; vp += a2; }
-            ++cntp; ++cntn;
-            atomicOr(&mark[lv], ls);
-            atomicOr(&mark[o1], 3 - ls);
-            atomicOr(&mark[o2], 3 - ls);
+        // ordered append of the crossing triangles
+        const uint64_t m = ballot(crossing);
+        if (lane_id() == 0) wcnt[wv] = __popcll(m);
+        __syncthreads();
+        int wbase = nx, tot = 0;
+        for (int q = 0; q < PC_T / 64; ++q) {
+            if (q < wv) wbase += wcnt[q];
+            tot += wcnt[q];
         }
+        if (crossing) xl[wbase + mask_rank(m)] = k;
+        nx += tot;
+        __syncthreads();
     }
     vp = wave_sum_d(vp);
     vn = wave_sum_d(vn);
     cntp = wave_sum_i(cntp);
     cntn = wave_sum_i(cntn);
     if (lane_id() == 0) {
-        dsum[0][tid >> 6] = vp;
-        dsum[1][tid >> 6] = vn;
-        s.cnt[tid >> 6] = cntp;
-        s.cnt[4 + (tid >> 6)] = cntn;
+        dsum[0][wv] = vp;
+        dsum[1][wv] = vn;
+        s.cnt[wv] = cntp;
+        s.cnt[4 + wv] = cntn;
     }
     __syncthreads();
     double VP = 0.0, VN = 0.0;
@@ -566,29 +612,34 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         totn += s.cnt[4 + q];
     }
     __syncthreads();
-    // crossing instances and their owners (lowest instance of each undirected edge)
-    auto ncr = [&](int k) {
+    // instances of the strictly crossing edges over the listed triangles
+    auto strict = [&](int a, int b) { return (sg[a] > 0 && sg[b] < 0) || (sg[a] < 0 && sg[b] > 0); };
+    auto ninst = [&](int x) {
+        const int k = xl[x];
         const int a = Tr[3 * k], b = Tr[3 * k + 1], c = Tr[3 * k + 2];
-        return (int)((sg[a] >= 0) != (sg[b] >= 0)) + (int)((sg[b] >= 0) != (sg[c] >= 0)) +
-               (int)((sg[c] >= 0) != (sg[a] >= 0));
+        return (int)strict(a, b) + (int)strict(b, c) + (int)strict(c, a);
     };
-    const int ncross = pc_block_scan(nt, toff, &s, ncr);
     uint32_t H = 16;
-    while ((int)H < 2 * ncross + 2) H <<= 1;
-    char* B = (char*)pc_alloc(pool, 12ull * H + 64, &s, 2, err, 1);
+    while ((int)H < 4 * nx + 2) H <<= 1;
+    char* B = (char*)pc_alloc(pool, 12ull * H + 8ull * nx + 64, &s, 2, err, 1);
     if (!B) {
         if (tid == 0) heap_free(pool, A, err);
         return;
     }
     unsigned long long* hk = (unsigned long long*)B;
     int* hmin = (int*)(B + 8ull * H);
+    int* xoff = hmin + H;
+    int* xnew = xoff + nx;
+    const int ncross = pc_block_scan(nx, xoff, &s, ninst);
+    (void)ncross;
     for (uint32_t i = tid; i < H; i += PC_T) { hk[i] = PC_EMPTY; hmin[i] = 0x7fffffff; }
     __syncthreads();
-    for (int k = tid; k < nt; k += PC_T) {
-        int inst = toff[k];
+    for (int x = tid; x < nx; x += PC_T) {
+        const int k = xl[x];
+        int inst = xoff[x];
         for (int e = 0; e < 3; ++e) {
             const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
-            if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+            if (!strict(a, b)) continue;
             int fresh;
             const int slot = pc_ht_insert(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b), &fresh);
             atomicMin(&hmin[slot], inst);
@@ -596,21 +647,29 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         }
     }
     __syncthreads();
-    auto owned_new = [&](int k) {
-        int inst = toff[k], c = 0;
+    auto owned_new = [&](int x) {
+        const int k = xl[x];
+        int inst = xoff[x], c = 0;
         for (int e = 0; e < 3; ++e) {
             const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
-            if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+            if (!strict(a, b)) continue;
             const int slot = pc_ht_find(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b));
-            if (hmin[slot] == inst && sg[a] != 0 && sg[b] != 0) ++c;
+            if (hmin[slot] == inst) ++c;
             ++inst;
         }
         return c;
     };
-    const int n_new = pc_block_scan(nt, tnew, &s, owned_new);
+    const int n_new = pc_block_scan(nx, xnew, &s, owned_new);
     // point sets: marked original vertices in index order, then the new points
-    const int nposo = pc_block_scan(nv, nullptr, &s, [&](int i) { return mark[i] & 1; });
-    const int nnego = pc_block_scan(nv, nullptr, &s, [&](int i) { return (mark[i] >> 1) & 1; });
+    int c1 = 0, c2 = 0;
+    for (int i = tid; i < nv; i += PC_T) { c1 += m1[i]; c2 += m2[i]; }
+    c1 = wave_sum_i(c1);
+    c2 = wave_sum_i(c2);
+    if (lane_id() == 0) { s.cnt[wv] = c1; s.cnt[4 + wv] = c2; }
+    __syncthreads();
+    int nposo = 0, nnego = 0;
+    for (int q = 0; q < PC_T / 64; ++q) { nposo += s.cnt[q]; nnego += s.cnt[4 + q]; }
+    __syncthreads();
     float* pp = nullptr;
     float* npp = nullptr;
     if (totp > 0 && totn > 0) {
@@ -621,38 +680,39 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
             int bp = 0, bn = 0;
             for (int b0 = 0; b0 < nv; b0 += PC_T) {
                 const int i = b0 + tid;
-                const int m = i < nv ? mark[i] : 0;
-                const uint64_t mp = ballot(m & 1), mn_ = ballot(m & 2);
-                if (lane_id() == 0) { s.cnt[tid >> 6] = __popcll(mp); s.cnt[4 + (tid >> 6)] = __popcll(mn_); }
+                const int f1 = i < nv ? m1[i] : 0, f2 = i < nv ? m2[i] : 0;
+                const uint64_t mp = ballot(f1), mn_ = ballot(f2);
+                if (lane_id() == 0) { s.cnt[wv] = __popcll(mp); s.cnt[4 + wv] = __popcll(mn_); }
                 __syncthreads();
                 int op = bp, on_ = bn;
-                for (int q = 0; q < (tid >> 6); ++q) { op += s.cnt[q]; on_ += s.cnt[4 + q]; }
-                if (m & 1) {
+                for (int q = 0; q < wv; ++q) { op += s.cnt[q]; on_ += s.cnt[4 + q]; }
+                if (f1) {
                     const int o = op + mask_rank(mp);
                     pp[3 * o] = V[3 * i]; pp[3 * o + 1] = V[3 * i + 1]; pp[3 * o + 2] = V[3 * i + 2];
                 }
-                if (m & 2) {
+                if (f2) {
                     const int o = on_ + mask_rank(mn_);
                     npp[3 * o] = V[3 * i]; npp[3 * o + 1] = V[3 * i + 1]; npp[3 * o + 2] = V[3 * i + 2];
                 }
                 for (int q = 0; q < PC_T / 64; ++q) { bp += s.cnt[q]; bn += s.cnt[4 + q]; }
                 __syncthreads();
             }
-            for (int k = tid; k < nt; k += PC_T) {
-                int inst = toff[k], c = 0;
+            for (int x = tid; x < nx; x += PC_T) {
+                const int k = xl[x];
+                int inst = xoff[x], c = 0;
                 for (int e = 0; e < 3; ++e) {
                     const int a = Tr[3 * k + e], b = Tr[3 * k + (e == 2 ? 0 : e + 1)];
-                    if ((sg[a] >= 0) == (sg[b] >= 0)) continue;
+                    if (!strict(a, b)) continue;
                     const int slot =
                         pc_ht_find(hk, H - 1, ((unsigned long long)min(a, b) << 32) | (unsigned)max(a, b));
-                    if (hmin[slot] == inst && sg[a] != 0 && sg[b] != 0) {
-                        float x[3];
-                        xpt(a, b, x);
-                        const int j = tnew[k] + c;
+                    if (hmin[slot] == inst) {
+                        float q3[3];
+                        xpt(a, b, q3);
+                        const int j = xnew[x] + c;
                         float* dp = pp + 3ll * (nposo + j);
                         float* dn = npp + 3ll * (nnego + j);
-                        dp[0] = x[0]; dp[1] = x[1]; dp[2] = x[2];
-                        dn[0] = x[0]; dn[1] = x[1]; dn[2] = x[2];
+                        dp[0] = q3[0]; dp[1] = q3[1]; dp[2] = q3[2];
+                        dn[0] = q3[0]; dn[1] = q3[1]; dn[2] = q3[2];
                         ++c;
                     }
                     ++inst;
@@ -1047,8 +1107,39 @@ int ensure_kdop_axes(gpu_ctx_t ctx) {
 using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
 using BlockFn = void (*)(LaHullTask*, const int*, int, char*, unsigned long long, int, int, DPool*, int*);
 
-int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl) {
+// Wave-tier sets of more than `wide_min` points run four waves per hull
+// (CUACD_HULL_WIDE_MIN overrides; 0 disables).
+int hull_wide_min() {
+    static int v = -2;
+    if (v == -2) {
+        const char* e = getenv("CUACD_HULL_WIDE_MIN");
+        v = e ? atoi(e) : 512;
+    }
+    return v;
+}
+
+int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl, double bytes) {
     if (ids.empty()) return 0;
+    const int kid = ppl == 1 ? KT_HULL_W1 : ppl == 2 ? KT_HULL_W2 : ppl == 4 ? KT_HULL_W4 : ppl == 8 ? KT_HULL_W8
+                                                                                                     : KT_HULL_W16;
+    const int wm = hull_wide_min();
+    if (wm > 0 && ppl >= 4 && 64 * ppl > wm) {
+        // same capacity (64 * ppl points and vertices), four waves
+        WaveFn k = ppl == 4 ? la_hull_lds4<1> : ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
+        const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
+        if (lds > 65536u)
+            HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int* d_ids = dev_tmp<int>(ctx, ids.size());
+        REQUIRE(d_ids, "lookahead: id list allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        hipEvent_t t0 = kt_begin(ctx);
+        hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), lds, ctx->stream, d_tasks, d_ids,
+                           (int)ids.size(), ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, kid, t0, bytes);
+        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+        return 0;
+    }
     WaveFn k = ppl == 1 ? la_hull_wave<1> : ppl == 2 ? la_hull_wave<2> : ppl == 4 ? la_hull_wave<4>
              : ppl == 8 ? la_hull_wave<8> : la_hull_wave<16>;
     const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
@@ -1057,21 +1148,24 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
     int* d_ids = dev_tmp<int>(ctx, ids.size());
     REQUIRE(d_ids, "lookahead: id list allocation failed");
     HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t t0 = kt_begin(ctx);
     hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64), lds, ctx->stream, d_tasks, d_ids, (int)ids.size(),
                        ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
+    kt_end(ctx, kid, t0, bytes);
     HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
     return 0;
 }
 
-int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn) {
+int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn, double bytes) {
     if (ids.empty()) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
     BlockFn k;
-    if (maxn <= 4 * T) k = la_hull_block<4>;
-    else if (maxn <= 8 * T) k = la_hull_block<8>;
-    else if (maxn <= 16 * T) k = la_hull_block<16>;
-    else k = la_hull_block<32>;
+    int kid;
+    if (maxn <= 4 * T) { k = la_hull_block<4>; kid = KT_HULL_B4; }
+    else if (maxn <= 8 * T) { k = la_hull_block<8>; kid = KT_HULL_B8; }
+    else if (maxn <= 16 * T) { k = la_hull_block<16>; kid = KT_HULL_B16; }
+    else { k = la_hull_block<32>; kid = KT_HULL_B32; }
     const int np = (maxn + 63) / 64 * 64;
     const int vc = np < 32704 ? np : 32704;
     const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull;
@@ -1085,9 +1179,11 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
     HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
     for (size_t b = 0; b < ids.size(); b += slice) {
         const int cnt = (int)std::min(slice, ids.size() - b);
+        hipEvent_t t0 = kt_begin(ctx);
         hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, ctx->stream, d_tasks, d_ids + b, cnt, gws, stride, vc, np,
                            ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
+        kt_end(ctx, kid, t0, bytes * (double)cnt / (double)ids.size());
     }
     HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
     HIP_TRY(hipFreeAsync(gws, ctx->stream));
@@ -1098,13 +1194,16 @@ int ppl_class(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 5
 
 int launch_by_size(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn) {
     std::vector<int> cls[5];
+    double cbytes[5] = {0, 0, 0, 0, 0};
     std::vector<int> big;
     int bigmax = 0;
     for (auto& q : idn) {
         const int n = q.second;
         if (n <= LA_WAVE_MAX) {
             const int p = ppl_class(n);
-            cls[p == 1 ? 0 : p == 2 ? 1 : p == 4 ? 2 : p == 8 ? 3 : 4].push_back(q.first);
+            const int c = p == 1 ? 0 : p == 2 ? 1 : p == 4 ? 2 : p == 8 ? 3 : 4;
+            cls[c].push_back(q.first);
+            cbytes[c] += 12.0 * n + 8.0;  // points read, volume written
         } else {
             big.push_back(q.first);
             bigmax = std::max(bigmax, n);
@@ -1112,24 +1211,26 @@ int launch_by_size(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<std::pa
     }
     const int ppls[5] = {1, 2, 4, 8, 16};
     for (int c = 0; c < 5; ++c)
-        if (int rc = launch_wave_ids(ctx, d_tasks, cls[c], ppls[c])) return rc;
+        if (int rc = launch_wave_ids(ctx, d_tasks, cls[c], ppls[c], cbytes[c])) return rc;
     if (!big.empty()) {
         // group the block tier by size class to bound the workspaces
         std::vector<int> b4, b8, b16, b32;
         int m4 = 0, m8 = 0, m16 = 0, m32 = 0;
+        double y4 = 0, y8 = 0, y16 = 0, y32 = 0;
         std::vector<std::pair<int, int>> bn;
         for (auto& q : idn) if (q.second > LA_WAVE_MAX) bn.push_back(q);
         for (auto& q : bn) {
             const int n = q.second;
-            if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); }
-            else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); }
-            else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); }
-            else { b32.push_back(q.first); m32 = std::max(m32, n); }
+            const double y = 12.0 * n + 8.0;
+            if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); y4 += y; }
+            else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); y8 += y; }
+            else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); y16 += y; }
+            else { b32.push_back(q.first); m32 = std::max(m32, n); y32 += y; }
         }
-        if (int rc = launch_block_ids(ctx, d_tasks, b4, m4)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b8, m8)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b16, m16)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b32, m32)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b4, m4, y4)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b8, m8, y8)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b16, m16, y16)) return rc;
+        if (int rc = launch_block_ids(ctx, d_tasks, b32, m32, y32)) return rc;
     }
     return 0;
 }
@@ -1156,9 +1257,13 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
     int2* d_res = dev_tmp<int2>(ctx, big.size());
     REQUIRE(d_ids && d_res, "lookahead: k-DOP buffers allocation failed");
     HIP_TRY(hipMemcpyAsync(d_ids, big.data(), 4 * big.size(), hipMemcpyHostToDevice, ctx->stream));
+    double kbytes = 0;
+    for (int i : big) kbytes += 12.0 * h_tasks[i].n;  // points read (the filtered copy is extra)
+    hipEvent_t t0 = kt_begin(ctx);
     hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
                        (int)big.size(), d_res, ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
+    kt_end(ctx, KT_KDOP, t0, kbytes);
     std::vector<int2> res(big.size());
     HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipFreeAsync(d_ids, ctx->stream));

@@ -187,6 +187,7 @@ struct Engine {
         HIP_TRY(hipEventSynchronize(ev1));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        LA_TRY(kt_resolve(ctx));
         stats[ph][0] += ms;
         stats[ph][1] += launches;
         stats[ph][2] += bytes;
@@ -293,14 +294,21 @@ struct Engine {
         return flush_frees();
     }
 
-    // ---- phase: hulls of the parts without one (cuda/la_lifecycle.cu:268-300)
-    int hull_parts(const std::vector<int>& meshes) {
+    // ---- phase: hulls of the parts without one (cuda/la_lifecycle.cu:268-300).
+    // A part adopted from a level-0 cut already carries that half's hull
+    // volume (the same point set, the same canonical hull); its mesh is built
+    // only when the Hausdorff stage needs it (rv under the threshold) or, with
+    // all_meshes, for the results.
+    int hull_parts(const std::vector<int>& meshes, bool all_meshes = false) {
         std::vector<LaHullTask> h;
         std::vector<std::pair<int, int>> who;
         for (int m : meshes)
             for (int i = 0; i < (int)M[m].parts.size(); ++i) {
                 Part& p = M[m].parts[i];
-                if (p.hvol > 0.0 || p.nv == 0) continue;
+                if (p.nv == 0) continue;
+                const bool need_vol = !(p.hvol > 0.0);
+                const bool need_mesh = !p.hv && (all_meshes || cost_rv(p.hvol, p.mvol) < P.threshold);
+                if (!need_vol && !need_mesh) continue;
                 LaHullTask t{};
                 t.pts = p.v; t.n = p.nv; t.emit = 1; t.prune_gap = 1e30;
                 h.push_back(t);
@@ -313,11 +321,8 @@ struct Engine {
         LA_TRY(tbeg());
         LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-        {
-            double by = 0;
-            for (auto& x : h) by += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
-            LA_TRY(tend(PH_PHULL, by));
-        }
+        LA_TRY(tend(PH_PHULL, 0.0));
+        for (auto& x : h) stats[PH_PHULL][2] += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
         LA_TRY(check_err("part hulls"));
         for (size_t k = 0; k < h.size(); ++k) {
             Part& p = M[who[k].first].parts[who[k].second];
@@ -461,11 +466,8 @@ struct Engine {
         LA_TRY(tbeg());
         LA_TRY(la_launch_hull(ctx, d.p, h.data(), (int)h.size()));
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-        {
-            double by = 0;
-            for (auto& x : h) by += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
-            LA_TRY(tend(hull_phase, by));
-        }
+        LA_TRY(tend(hull_phase, 0.0));
+        for (auto& x : h) stats[hull_phase][2] += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
         return check_err("item hulls");
     }
 
@@ -913,12 +915,15 @@ struct Engine {
                                            ctx->stream));
                     LA_TRY(tbeg());
                     LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
+                    LA_TRY(tend(PH_CUT, 0.0));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));
+                    HIP_TRY(hipStreamSynchronize(ctx->stream));
                     {
+                        // mesh read, both halves written, volumes
                         double by = 0;
                         for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
-                        LA_TRY(tend(PH_CUT, by));
+                        stats[PH_CUT][2] += by;
                     }
                     LA_TRY(check_err("cuts"));
                 }
@@ -971,12 +976,15 @@ struct Engine {
                                            ctx->stream));
                     LA_TRY(tbeg());
                     LA_TRY(la_launch_score(ctx, dt.p, (int)h.size()));
+                    LA_TRY(tend(PH_SCORE, 0.0));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaScoreTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));
+                    HIP_TRY(hipStreamSynchronize(ctx->stream));
                     {
+                        // mesh read, both point sets written, volumes and counts
                         double by = 0;
                         for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.np_pts + x.nn_pts) + 16.0;
-                        LA_TRY(tend(PH_SCORE, by));
+                        stats[PH_SCORE][2] += by;
                     }
                     LA_TRY(check_err("leaf scoring"));
                 }
@@ -1137,7 +1145,8 @@ struct Engine {
                 const IPart& x = *hp[s2];
                 store[x.store].keep = true;
                 Part q;
-                q.v = x.v; q.t = x.t; q.nv = x.nv; q.nt = x.nt; q.mvol = x.mvol; q.hvol = 0.0; q.haus = 0.0f;
+                q.v = x.v; q.t = x.t; q.nv = x.nv; q.nt = x.nt; q.mvol = x.mvol; q.hvol = x.hvol; q.haus = 0.0f;
+                if (q.mvol > q.hvol) q.mvol = q.hvol;  // cuda/la_lifecycle.cu:292-296
                 q.cc = cc; q.own = true;
                 for (int j = 0; j < 3; ++j) { q.lo[j] = x.lo[j]; q.hi[j] = x.hi[j]; }
                 if (s2 == 0) old = q;
@@ -1178,6 +1187,17 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     }
     Engine E;
     E.ctx = ctx;
+    // per-kernel timers for the run; detached on every exit path
+    struct KtGuard {
+        gpu_ctx_t c;
+        KTimers k;
+        explicit KtGuard(gpu_ctx_t c_) : c(c_) { c->kt = &k; }
+        ~KtGuard() {
+            for (auto& r : k.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+            for (hipEvent_t e : k.free_ev) (void)hipEventDestroy(e);
+            c->kt = nullptr;
+        }
+    } ktg(ctx);
     E.P = P;
     E.M.resize(n_meshes);
     const size_t tv = (size_t)voff[n_meshes], tt = (size_t)toff[n_meshes];
@@ -1249,10 +1269,11 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         rc = E.components(all);
         if (rc == 0) rc = E.hull_parts(all);
     }
-    if (rc == 0 && P.merge) {
+    if (rc == 0) {
         std::vector<int> all;
         for (int m = 0; m < n_meshes; ++m) all.push_back(m);
-        rc = E.merge(all);
+        rc = E.hull_parts(all, true);
+        if (rc == 0 && P.merge) rc = E.merge(all);
     }
     E.lap(S_POST);
     // results
@@ -1304,6 +1325,9 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         for (int ph = 0; ph < PH_COUNT; ++ph)
             for (int j = 0; j < 3; ++j) stats_out[3 * ph + j] = E.stats[ph][j];
         for (int sc = 0; sc < S_COUNT; ++sc) stats_out[3 * PH_COUNT + sc] = E.sect[sc];
+        (void)kt_resolve(ctx);
+        for (int kc = 0; kc < KT_COUNT; ++kc)
+            for (int j = 0; j < 3; ++j) stats_out[3 * PH_COUNT + S_COUNT + 3 * kc + j] = ktg.k.stats[kc][j];
     }
     if (rc) {
         for (int m = 0; m < n_meshes; ++m) gpu_result_free(&out[m]);
