@@ -26,20 +26,24 @@ PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_
               n_concave_edges=16, merge_hulls=True)
 
 
-def _hull_kernel(ppl):
-    def name(wide_min):
-        if wide_min > 0 and ppl >= 4 and 64 * ppl > wide_min:
-            return f"la_hull_lds4<{ppl // 4}>"
-        return f"la_hull_wave<{ppl}>"
-    return name
-
-
-# hull kernel classes reported by the library -> kernel names
-KERNEL_OF = {"hull_w64": _hull_kernel(1), "hull_w128": _hull_kernel(2), "hull_w256": _hull_kernel(4),
-             "hull_w512": _hull_kernel(8), "hull_w1024": _hull_kernel(16),
-             "hull_b2048": lambda w: "la_hull_block<4>", "hull_b4096": lambda w: "la_hull_block<8>",
-             "hull_b8192": lambda w: "la_hull_block<16>", "hull_b16384": lambda w: "la_hull_block<32>",
-             "kdop_prefilter": lambda w: "la_kdop_kernel"}
+def kernel_name(cls, wide_min, half_cap):
+    """Kernel(s) behind a hull class reported by the library (the first pass
+    and, after '+', the full-capacity retry)."""
+    if cls == "kdop_prefilter":
+        return "la_kdop_kernel"
+    if cls.startswith("hull_b"):
+        n = int(cls[6:])
+        ppl = n // 512
+        if n == 2048 and half_cap:
+            return "la_hull_ldsv<4,8,1024>+la_hull_block<4>"
+        return f"la_hull_block<{ppl}>"
+    n = int(cls[6:])
+    ppl = n // 64
+    if wide_min > 0 and ppl >= 4 and n > wide_min:
+        if ppl >= 8 and half_cap:
+            return f"la_hull_ldsv<{ppl // 4},4,{n // 2}>+la_hull_lds4<{ppl // 4}>"
+        return f"la_hull_lds4<{ppl // 4}>"
+    return f"la_hull_wave<{ppl}>"
 
 
 def log(*a):
@@ -133,9 +137,10 @@ def main():
     for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel"),
                      ("hausdorff", "hd_task_kernel")):
         kern[name] = stats[ph]
-    wide_min = int(os.environ.get("CUACD_HULL_WIDE_MIN", "512"))
+    wide_min = int(os.environ.get("CUACD_HULL_WIDE_MIN", "256"))
+    half_cap = int(os.environ.get("CUACD_HULL_HALFCAP", "1")) != 0
     for cls, k in ctx.last_lookahead_kernels.items():
-        kern[KERNEL_OF[cls](wide_min)] = k
+        kern[kernel_name(cls, wide_min, half_cap)] = k
     for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
         log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
             f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")

@@ -806,15 +806,21 @@ __device__ __forceinline__ double hull_volume_f64(const HullWS<AS> w, HullBlockL
 //   * each facet is the fan (c0, c_k, c_k+1), k = 1..m-2, of its corners in
 //     counter-clockwise order (seen from outside) starting at its corner of
 //     lowest output index c0; facets are listed by (c0, c1);
-//   * volume: FP64 divergence sum over the fan triangles, original fp32
-//     coordinates, anchored at output vertex 0.
+//   * volume: FP64 divergence terms of the fan triangles (original fp32
+//     coordinates, anchored at output vertex 0, no fused multiply-add),
+//     summed in a fixed order: the triangle list in canonical order is cut
+//     into 64 contiguous chunks of ceil(nt / 64), each summed in order, then
+//     the 64 partial sums in order - so the value does not depend on how the
+//     hull was built.
 // Scratch reused after block_quickhull, per face f: fvis = coplanar-edge bits
-// (bit 3: facet leader), fstamp = facet label (lowest face id of the facet),
+// (bit 3: facet leader, bit 4: live face), fstamp = facet label (lowest face id of the facet),
 // then a leader's triangle offset; fpl.xyz = successor of each boundary edge
 // (3*face + slot, -1 inside a facet), fpl.w = a leader's edge ending at c0;
 // fv.w = a leader's boundary face, then its corner count; ffree = a leader's
-// c1; vis = next leader in the list of facets sharing c0. Per vertex: vref =
-// corner flag, vout = output index, cv = list head and triangle offset per c0.
+// c1; vis = next leader in the list of facets sharing c0; fn (no longer
+// needed once the boundaries are traced) = the triangles' volume terms. Per
+// vertex: vref = corner flag, vout = output index, cv = list head and
+// triangle offset per c0.
 struct HullCanon {
     int nv, nt;
     double vol;
@@ -854,7 +860,7 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
     // A: exact coplanarity of each face with its neighbour across each edge
     for (int f = tid; f < ftop; f += T) {
         w.fstamp[f] = f;
-        if (!alive(f)) continue;
+        if (!alive(f)) { w.fvis[f] = 0; continue; }
         const u4v v = w.fv[f], nb = w.fn[f];
         const s4v qa = w.vq[v.x], qb = w.vq[v.y], qc = w.vq[v.z];
         int bits = 0;
@@ -863,7 +869,7 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
             const s4v q = w.vq[opp_of(u4_get(nb, e), a, b)];
             if (orient_exact(qa, qb, qc, q.x, q.y, q.z) == 0.0) bits |= 1 << e;
         }
-        w.fvis[f] = (uint8_t)bits;
+        w.fvis[f] = (uint8_t)(bits | 16);
     }
     for (int v = tid; v < r.nvert; v += T) w.vref[v] = 0;
     __syncthreads();
@@ -967,10 +973,10 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
     const double ax = an[0], ay = an[1], az = an[2];
     auto succ_of = [&](int e) { return fpl_geti(w.fpl[e / 3], e % 3); };
     auto end_of = [&](int e) { const int h = e / 3, j = e % 3; return u4_get(w.fv[h], j == 2 ? 0 : j + 1); };
-    // E: per facet (its leader's thread): corners, c0, c1, fan volume
-    double acc = 0.0;
+    // E: per facet (its leader's thread): corners, c0, c1
+    auto live = [&](int f) { return (w.fvis[f] & 16) != 0; };
     for (int f = tid; f < ftop; f += T) {
-        if (!alive(f) || w.fstamp[f] != f) continue;
+        if (!live(f) || w.fstamp[f] != f) continue;
         const int h0 = u4_get(w.fv[f], 3);
         const int b0 = w.fvis[h0] & 7;
         const int j0 = !(b0 & 1) ? 0 : (!(b0 & 2) ? 1 : 2);
@@ -985,27 +991,12 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
             }
             e = succ_of(e);
         } while (e != start && ++guard < 3 * FC);
-        const int c0v = end_of(ebest);
-        float p0[3];
-        pos(w.vpid[c0v], p0);
-        const double x0 = p0[0] - ax, y0 = p0[1] - ay, z0 = p0[2] - az;
-        int c1 = -1, prev = -1;
+        int c1 = -1;
         e = succ_of(ebest);
         guard = 0;
         while (e != ebest && ++guard < 3 * FC) {
             const int b = end_of(e);
-            if (w.vref[b]) {
-                if (c1 < 0) c1 = w.vout[b];
-                if (prev >= 0) {
-                    float p1[3], p2[3];
-                    pos(w.vpid[prev], p1);
-                    pos(w.vpid[b], p2);
-                    const double x1 = p1[0] - ax, y1 = p1[1] - ay, z1 = p1[2] - az;
-                    const double x2 = p2[0] - ax, y2 = p2[1] - ay, z2 = p2[2] - az;
-                    acc += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
-                }
-                prev = b;
-            }
+            if (w.vref[b]) { c1 = w.vout[b]; break; }
             e = succ_of(e);
         }
         w.fvis[f] = (uint8_t)(w.fvis[f] | 8);
@@ -1017,7 +1008,6 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
         w.ffree[f] = (uint16_t)(c1 < 0 ? 0 : c1);
         w.vis[f] = (uint16_t)canon_exch<AS>(w.cv + 2 * best, f);
     }
-    out.vol = block_sum_d<NW>(acc, sb) / 6.0;
     __syncthreads();
     // F: triangle offsets: per c0 bucket (scan over output vertices), then by c1 inside it
     auto tris_of = [&](int L) { return max(u4_get(w.fv[L], 3) - 2, 0); };
@@ -1051,7 +1041,7 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
     }
     __syncthreads();
     for (int f = tid; f < ftop; f += T) {
-        if (!alive(f) || !(w.fvis[f] & 8)) continue;
+        if (!(w.fvis[f] & 8)) continue;
         const int c0 = w.vout[end_of(fpl_geti(w.fpl[f], 3))];
         const int c1 = w.ffree[f];
         int off = w.cv[2 * c0 + 1];
@@ -1061,6 +1051,46 @@ __device__ HullCanon hull_canon(const HullWS<AS> w, HullBlockLDS* sb, const Hull
         }
         w.fstamp[f] = off;
     }
+    __syncthreads();
+    // V: volume terms at the triangles' canonical positions, fixed-order sum
+    asptr<AS, double> term = (asptr<AS, double>)w.fn;
+    for (int f = tid; f < ftop; f += T) {
+#pragma clang fp contract(off)
+        if (!(w.fvis[f] & 8)) continue;
+        const int ebest = fpl_geti(w.fpl[f], 3);
+        float p0[3];
+        pos(w.vpid[end_of(ebest)], p0);
+        const double x0 = (double)p0[0] - ax, y0 = (double)p0[1] - ay, z0 = (double)p0[2] - az;
+        int t = w.fstamp[f];
+        int prev = -1, e = succ_of(ebest), guard = 0;
+        while (e != ebest && ++guard < 3 * FC) {
+            const int b = end_of(e);
+            if (w.vref[b]) {
+                if (prev >= 0) {
+                    float p1[3], p2[3];
+                    pos(w.vpid[prev], p1);
+                    pos(w.vpid[b], p2);
+                    const double x1 = (double)p1[0] - ax, y1 = (double)p1[1] - ay, z1 = (double)p1[2] - az;
+                    const double x2 = (double)p2[0] - ax, y2 = (double)p2[1] - ay, z2 = (double)p2[2] - az;
+                    term[t++] = x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
+                }
+                prev = b;
+            }
+            e = succ_of(e);
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int chunk = (out.nt + 63) / 64;
+        const int b = min(out.nt, tid * chunk), e = min(out.nt, b + chunk);
+        double part = 0.0;
+        for (int t = b; t < e; ++t) part += term[t];
+        double acc = 0.0;
+        for (int l = 0; l < 64; ++l) acc += __shfl(part, l);
+        if (tid == 0) sb->dred[0] = acc / 6.0;
+    }
+    __syncthreads();
+    out.vol = sb->dred[0];
     __syncthreads();
     return out;
 }
@@ -1073,7 +1103,7 @@ __device__ void hull_canon_write(const HullWS<AS> w, const HullResult& r, int* _
     auto succ_of = [&](int e) { return fpl_geti(w.fpl[e / 3], e % 3); };
     auto end_of = [&](int e) { const int h = e / 3, j = e % 3; return u4_get(w.fv[h], j == 2 ? 0 : j + 1); };
     for (int f = threadIdx.x; f < r.ftop; f += T) {
-        if (!w.fn[f].w || !(w.fvis[f] & 8)) continue;
+        if (!(w.fvis[f] & 8)) continue;
         const int ebest = fpl_geti(w.fpl[f], 3);
         const int c0 = w.vout[end_of(ebest)];
         int t = w.fstamp[f];

@@ -26,16 +26,19 @@ __constant__ float c_kdop_axes[KDOP_AXES_N][3];
 // ---------------------------------------------------------------------------
 // Hull of one task's point set
 // ---------------------------------------------------------------------------
+// Returns false when the workspace's vertex capacity was too small and the
+// task was queued on `retry` (count at retry[0], task ids after it) to be
+// redone by a full-capacity kernel; the task is then left untouched.
 template <int PPL, int AS, int NW>
-__device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, HullBlockLDS* sb, DPool* pool,
-                                             int* err) {
+__device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, HullBlockLDS* sb, DPool* pool,
+                                             int* err, int* retry = nullptr, int task_id = -1) {
     constexpr int T = 64 * NW;
     const int tid = threadIdx.x;
     const float* P = t.filt ? t.filt : t.pts;
     const int n = t.filt ? t.nf : t.n;
     if (n < 4) {
         if (tid == 0) { t.vol = 0.0; t.hnv = t.hnt = 0; t.hv = nullptr; t.ht = nullptr; }
-        return;
+        return true;
     }
     float xs[PPL], ys[PPL], zs[PPL];
 #pragma unroll
@@ -90,6 +93,13 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
     }
     HullResult r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
+    if (r.status == HULL_OVERFLOW && retry) {
+        if (tid == 0) {
+            const int k = atomicAdd(retry, 1);
+            retry[1 + k] = task_id;
+        }
+        return false;
+    }
     if (r.status != HULL_OK) {
         if (tid == 0) {
             t.vol = 0.0;
@@ -98,7 +108,7 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
             t.ht = nullptr;
             if (r.status == HULL_OVERFLOW) { t.err = KERR_HULL_CAPACITY; err_or(err, KERR_HULL_CAPACITY); }
         }
-        return;
+        return true;
     }
     auto pos = [&](int p, float* o3) {
         o3[0] = P[3ll * p];
@@ -115,10 +125,10 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
             t.err = KERR_HULL_CAPACITY;
             err_or(err, KERR_HULL_CAPACITY);
         }
-        return;
+        return true;
     }
     if (tid == 0) t.vol = hc.vol;
-    if (!t.emit) return;
+    if (!t.emit) return true;
     const int nv = hc.nv;
     const int nt = hc.nt;
     const unsigned long long vb = al16(12ull * nv);
@@ -128,7 +138,7 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
     __syncthreads();
     if (!chunk) {
         if (tid == 0) { t.hv = nullptr; t.ht = nullptr; t.hnv = t.hnt = 0; t.err = KERR_HULL_HEAP_OUTPUT; }
-        return;
+        return true;
     }
     int* tris = (int*)((char*)chunk + vb);
     for (int v = tid; v < r.nvert; v += T)
@@ -140,6 +150,7 @@ __device__ __forceinline__ void la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
     hull_canon_write<AS, NW>(w, r, tris);
     if (tid == 0) { t.hv = chunk; t.ht = tris; t.hnv = nv; t.hnt = nt; }
+    return true;
 }
 
 template <int PPL>
@@ -169,6 +180,23 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
     if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
 
+// Same, with the hull-vertex capacity reduced to VCAP (a smaller LDS
+// footprint, so two hulls share a CU); a set whose hull has more vertices is
+// queued on `retry` for the full-capacity kernel.
+template <int PPL, int NW, int VCAP>
+__global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n, int* retry,
+                                                       DPool* pool, int* err) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ HullBlockLDS sb;
+    if ((int)blockIdx.x >= n) return;
+    const int id = ids[blockIdx.x];
+    LaHullTask& t = tasks[id];
+    constexpr int CAP = 64 * NW * PPL;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, VCAP, CAP), &sb);
+    const bool done = la_hull_item<PPL, AS_LDS, NW>(t, w, &sb, pool, err, retry, id);
+    if (done && threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+}
+
 template <int PPL>
 __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* tasks, const int* ids, int n, char* gws,
                                                                   unsigned long long stride, int VC, int NP,
@@ -184,9 +212,10 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
 
 // k-DOP prefilter (reference cuda/kdop_hull.cuh:116-337): extreme points along
 // the 40 axes -> rough inner hull -> optional bound pruning on its volume ->
-// keep the points strictly outside some rough-hull face, plus the rough hull's
-// corners, in input order. The filtered set keeps the full set's quantization
-// frame and the input order, so its canonical hull equals the full set's.
+// keep the points on or outside some rough-hull face (exactly, on the full
+// set's grid), in input order. The filtered set keeps the full set's
+// quantization frame and the input order, so its canonical hull equals the
+// full set's.
 constexpr int KD_T = 256;
 constexpr int KD_NW = KD_T / 64;
 
@@ -195,12 +224,9 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
     __shared__ HullBlockLDS sb;
     __shared__ __align__(16) char lds[20480];
     __shared__ float ep[2 * KDOP_AXES_N][3];
-    __shared__ int epi[2 * KDOP_AXES_N];
-    __shared__ int rvi[2 * KDOP_AXES_N + 1];
-    __shared__ float4 planes[2 * 128 + 8];
+    __shared__ double4 dpl[2 * 128 + 8];
     __shared__ unsigned long long kred[KD_NW][16];
     __shared__ int s_cnt[KD_NW + 2];
-    __shared__ float s_ref[3];
     if ((int)blockIdx.x >= n) return;
     LaHullTask& t = tasks[ids[blockIdx.x]];
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
@@ -262,7 +288,6 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             ep[slot][0] = P[3ll * idx];
             ep[slot][1] = P[3ll * idx + 1];
             ep[slot][2] = P[3ll * idx + 2];
-            epi[slot] = idx;
         }
         __syncthreads();
     }
@@ -294,7 +319,8 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         return;
     }
     const double rough = rc.vol;
-    if (t.prune_gap < 1e29) {
+    // bound pruning on sets above one wave's reach only (reference cuda/kdop_hull.cuh:49-115)
+    if (t.prune_gap < 1e29 && np > LA_WAVE_MAX) {
         const double gap = fmax(rough - t.mesh_vol, 0.0);
         if (gap > t.prune_gap) {
             if (tid == 0) {
@@ -307,55 +333,36 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             return;
         }
     }
-    const int nrv = rc.nv;
-    // interior reference: centroid of the rough hull's corners (ascending
-    // extreme slot); the corners' input indices, ascending, for the filter
-    if (tid == 0) {
-        float rx = 0.f, ry = 0.f, rz = 0.f;
-        int k = 0;
-        for (int p = 0; p < NE; ++p) {
-            bool used = false;
-            for (int v = 0; v < r.nvert; ++v)
-                if (w.vref[v] && w.vpid[v] == p) { used = true; break; }
-            if (used) {
-                rx += ep[p][0]; ry += ep[p][1]; rz += ep[p][2];
-                rvi[k++] = epi[p];
-            }
-        }
-        // insertion sort (at most 80 entries), then a sentinel
-        for (int i = 1; i < k; ++i) {
-            const int x = rvi[i];
-            int j = i - 1;
-            while (j >= 0 && rvi[j] > x) { rvi[j + 1] = rvi[j]; --j; }
-            rvi[j + 1] = x;
-        }
-        rvi[k] = 0x7fffffff;
-        const float inv = 1.f / (float)nrv;
-        s_ref[0] = rx * inv; s_ref[1] = ry * inv; s_ref[2] = rz * inv;
-    }
-    __syncthreads();
-    // oriented planes of the live rough faces, in face-id order
-    if (tid == 0) s_cnt[KD_NW] = 0;
-    __syncthreads();
+    // exact face planes of the rough hull on the full set's grid: a point is
+    // dropped only when it lies strictly inside every face, i.e. strictly
+    // inside a solid whose vertices are points of the set, so it cannot be an
+    // extreme point; quantized duplicates share the decision. (Both grids are
+    // right-handed, so the faces keep their outward orientation.)
+    auto fq = [&](int p, double* o) {
+        const float v[3] = {ep[p][0], ep[p][1], ep[p][2]};
+        o[0] = (double)qcoord(full, 0, sel3(v, full.ax[0]));
+        o[1] = (double)qcoord(full, 1, sel3(v, full.ax[1]));
+        o[2] = (double)qcoord(full, 2, sel3(v, full.ax[2]));
+    };
     if (tid == 0) {
         int c = 0;
-        for (int f = 0; f < r.ftop; ++f) {
-            if (!w.fn[f].w) continue;
+        for (int f = 0; f < r.ftop && c < 2 * 128 + 8; ++f) {
+            if (!(w.fvis[f] & 16)) continue;  // live face (hull_canon reused fn)
             const u4v fv = w.fv[f];
-            const int ia = w.vpid[fv.x], ib = w.vpid[fv.y], ic = w.vpid[fv.z];
-            const float ax = ep[ia][0], ay = ep[ia][1], az = ep[ia][2];
-            const float e1x = ep[ib][0] - ax, e1y = ep[ib][1] - ay, e1z = ep[ib][2] - az;
-            const float e2x = ep[ic][0] - ax, e2y = ep[ic][1] - ay, e2z = ep[ic][2] - az;
-            float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-            float d = nx * ax + ny * ay + nz * az;
-            if (nx * s_ref[0] + ny * s_ref[1] + nz * s_ref[2] > d) { nx = -nx; ny = -ny; nz = -nz; d = -d; }
-            planes[c++] = make_float4(nx, ny, nz, d);
+            double a[3], b[3], cc[3];
+            fq(w.vpid[fv.x], a);
+            fq(w.vpid[fv.y], b);
+            fq(w.vpid[fv.z], cc);
+            const double e1x = b[0] - a[0], e1y = b[1] - a[1], e1z = b[2] - a[2];
+            const double e2x = cc[0] - a[0], e2y = cc[1] - a[1], e2z = cc[2] - a[2];
+            const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+            dpl[c++] = make_double4(nx, ny, nz, nx * a[0] + ny * a[1] + nz * a[2]);
         }
         s_cnt[KD_NW] = c;
     }
     __syncthreads();
     const int npl = s_cnt[KD_NW];
-    // filtered buffer: the points outside the rough hull and its corners, in input order
+    // filtered buffer: the points on or outside some rough face, in input order
     if (tid == 0) sb.red[0] = (unsigned long long)heap_alloc(pool, 1, 12ull * (np + NE), err);
     __syncthreads();
     float* filt = (float*)sb.red[0];
@@ -371,13 +378,15 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         float x = 0.f, y = 0.f, z = 0.f;
         if (i < np) {
             x = P[3ll * i]; y = P[3ll * i + 1]; z = P[3ll * i + 2];
+            const float v[3] = {x, y, z};
+            const double qx = (double)qcoord(full, 0, sel3(v, full.ax[0]));
+            const double qy = (double)qcoord(full, 1, sel3(v, full.ax[1]));
+            const double qz = (double)qcoord(full, 2, sel3(v, full.ax[2]));
+            // every product and sum is an integer below 2^53: exact in FP64
             for (int f = 0; f < npl; ++f) {
-                const float4 pl = planes[f];
-                if (pl.x * x + pl.y * y + pl.z * z > pl.w) { keep = true; break; }
+                const double4 pl = dpl[f];
+                if (pl.x * qx + pl.y * qy + pl.z * qz - pl.w >= 0.0) { keep = true; break; }
             }
-            if (!keep)
-                for (int q = 0; rvi[q] <= i; ++q)
-                    if (rvi[q] == i) { keep = true; break; }
         }
         const uint64_t m = ballot(keep);
         if (lane == 0) s_cnt[wv] = __popcll(m);
@@ -1113,7 +1122,17 @@ int hull_wide_min() {
     static int v = -2;
     if (v == -2) {
         const char* e = getenv("CUACD_HULL_WIDE_MIN");
-        v = e ? atoi(e) : 512;
+        v = e ? atoi(e) : 256;
+    }
+    return v;
+}
+
+// Half-capacity first pass for the four-wave tiers (CUACD_HULL_HALFCAP=0 disables).
+int hull_half_cap() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("CUACD_HULL_HALFCAP");
+        v = e ? (atoi(e) != 0) : 1;
     }
     return v;
 }
@@ -1123,6 +1142,41 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
     const int kid = ppl == 1 ? KT_HULL_W1 : ppl == 2 ? KT_HULL_W2 : ppl == 4 ? KT_HULL_W4 : ppl == 8 ? KT_HULL_W8
                                                                                                      : KT_HULL_W16;
     const int wm = hull_wide_min();
+    if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
+        // first pass with half the vertex capacity (two hulls per CU), then
+        // the sets whose hulls did not fit at full capacity
+        using CapFn = void (*)(LaHullTask*, const int*, int, int*, DPool*, int*);
+        CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW, 256> : la_hull_ldsv<4, LA_LDS4_NW, 512>;
+        WaveFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(32 * ppl, 64 * ppl);
+        const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
+        HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
+        HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
+        int* d_ids = dev_tmp<int>(ctx, ids.size());
+        int* d_retry = dev_tmp<int>(ctx, ids.size() + 1);
+        REQUIRE(d_ids && d_retry, "lookahead: id list allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
+        hipEvent_t t0 = kt_begin(ctx);
+        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, ctx->stream, d_tasks, d_ids,
+                           (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, kid, t0, bytes);
+        int nre = 0;
+        HIP_TRY(hipMemcpyAsync(&nre, d_retry, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        REQUIRE(nre >= 0 && nre <= (int)ids.size(), "lookahead: hull retry count %d out of range", nre);
+        if (nre > 0) {
+            hipEvent_t t1 = kt_begin(ctx);
+            hipLaunchKernelGGL(kf, dim3((unsigned)nre), dim3(64 * LA_LDS4_NW), ldf, ctx->stream, d_tasks, d_retry + 1,
+                               nre, ctx->d_pool, ctx->d_err);
+            HIP_TRY(hipGetLastError());
+            kt_end(ctx, kid, t1, 0.0);
+        }
+        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_retry, ctx->stream));
+        return 0;
+    }
     if (wm > 0 && ppl >= 4 && 64 * ppl > wm) {
         // same capacity (64 * ppl points and vertices), four waves
         WaveFn k = ppl == 4 ? la_hull_lds4<1> : ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
@@ -1157,9 +1211,41 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
     return 0;
 }
 
-int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn, double bytes) {
-    if (ids.empty()) return 0;
+int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids_in, int maxn, double bytes) {
+    if (ids_in.empty()) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
+    std::vector<int> ids = ids_in;
+    if (maxn <= 4 * T && hull_half_cap()) {
+        // first pass with the workspace in LDS (1024 hull vertices), then the
+        // sets whose hulls did not fit, with the workspace in HBM
+        using CapFn = void (*)(LaHullTask*, const int*, int, int*, DPool*, int*);
+        CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW, 1024>;
+        const int kidc = KT_HULL_B4;
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T);
+        HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
+        int* d_ids = dev_tmp<int>(ctx, ids.size());
+        int* d_retry = dev_tmp<int>(ctx, ids.size() + 1);
+        REQUIRE(d_ids && d_retry, "lookahead: id list allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
+        hipEvent_t t0 = kt_begin(ctx);
+        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(T), ldc, ctx->stream, d_tasks, d_ids,
+                           (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, kidc, t0, bytes);
+        int nre = 0;
+        HIP_TRY(hipMemcpyAsync(&nre, d_retry, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        REQUIRE(nre >= 0 && nre <= (int)ids.size(), "lookahead: hull retry count %d out of range", nre);
+        ids.resize(nre);
+        if (nre) HIP_TRY(hipMemcpyAsync(ids.data(), d_retry + 1, 4ull * nre, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_retry, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ids.empty()) return 0;
+        std::sort(ids.begin(), ids.end());
+        bytes = 0.0;
+    }
     BlockFn k;
     int kid;
     if (maxn <= 4 * T) { k = la_hull_block<4>; kid = KT_HULL_B4; }
@@ -1241,13 +1327,25 @@ namespace cuacd {
 
 // Hulls of n tasks. Host copy h_tasks gives the point counts; sets above the
 // wave tier are prefiltered (k-DOP) first, which needs one readback.
+// Sets above this many points are k-DOP prefiltered before their exact hull
+// (CUACD_KDOP_MIN overrides); bound pruning still applies only above 1024.
+static int kdop_min_points() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("CUACD_KDOP_MIN");
+        v = e ? std::max(4, atoi(e)) : 256;
+    }
+    return v;
+}
+
 int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n) {
     if (n <= 0) return 0;
     if (int rc = ensure_kdop_axes(ctx)) return rc;
     std::vector<std::pair<int, int>> small;
     std::vector<int> big;
+    const int kmin = std::min(kdop_min_points(), LA_WAVE_MAX);
     for (int i = 0; i < n; ++i) {
-        if (h_tasks[i].n <= LA_WAVE_MAX) small.push_back({i, h_tasks[i].n});
+        if (h_tasks[i].n <= kmin) small.push_back({i, h_tasks[i].n});
         else big.push_back(i);
     }
     if (int rc = launch_by_size(ctx, d_tasks, small)) return rc;

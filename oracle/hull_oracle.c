@@ -383,6 +383,33 @@ double orc_mesh_volume(const float* pts, const int* tris, int nt) {
     return acc / 6.0;
 }
 
+/* Volume of a canonical hull mesh (orc_hull_canon's triangle order), summed
+ * in the order the GPU library fixes: FP64 divergence terms anchored at the
+ * first vertex of the first triangle, the triangle list cut into 64
+ * contiguous chunks of ceil(nt / 64), each chunk summed in order, then the 64
+ * partial sums in order. */
+double orc_hull_volume(const float* pts, const int* tris, int nt) {
+    if (nt <= 0) return 0.0;
+    const float* o = pts + 3 * tris[0];
+    double ax = o[0], ay = o[1], az = o[2], acc = 0.0;
+    int chunk = (nt + 63) / 64;
+    for (int l = 0; l < 64; ++l) {
+        int b = l * chunk < nt ? l * chunk : nt, e = b + chunk < nt ? b + chunk : nt;
+        double part = 0.0;
+        for (int t = b; t < e; ++t) {
+            const float* p0 = pts + 3 * tris[3 * t];
+            const float* p1 = pts + 3 * tris[3 * t + 1];
+            const float* p2 = pts + 3 * tris[3 * t + 2];
+            double x0 = p0[0] - ax, y0 = p0[1] - ay, z0 = p0[2] - az;
+            double x1 = p1[0] - ax, y1 = p1[1] - ay, z1 = p1[2] - az;
+            double x2 = p2[0] - ax, y2 = p2[1] - ay, z2 = p2[2] - az;
+            part += x0 * (y1 * z2 - z1 * y2) + y0 * (z1 * x2 - x1 * z2) + z0 * (x1 * y2 - y1 * x2);
+        }
+        acc += part;
+    }
+    return acc / 6.0;
+}
+
 /* Brute force (n <= ~60): vertex flags from every facet triple (i<j<k) whose
  * plane has all other points strictly on one side. Valid in general position. */
 int orc_hull_bruteforce(const int* q, int n, unsigned char* is_vertex) {
@@ -427,7 +454,7 @@ int orc_hull_batch(const float* pts, const int* offsets, int n_clouds, unsigned 
         nf = orc_hull_canon(q, n, is_vertex + s, faces, cap);
         if (nf < 0) { ++bad; nf = 0; }
         n_faces[c] = nf;
-        volumes[c] = orc_mesh_volume(pts + 3 * (size_t)s, faces, nf);
+        volumes[c] = orc_hull_volume(pts + 3 * (size_t)s, faces, nf);
         free(q); free(faces);
     }
     return bad;
@@ -447,7 +474,7 @@ int orc_hull_batch_serial(const float* pts, const int* offsets, int n_clouds, un
         int nf = orc_hull_canon(q, n, is_vertex + s, faces, cap);
         if (nf < 0) { ++bad; nf = 0; }
         n_faces[c] = nf;
-        volumes[c] = orc_mesh_volume(pts + 3 * (size_t)s, faces, nf);
+        volumes[c] = orc_hull_volume(pts + 3 * (size_t)s, faces, nf);
         free(q); free(faces);
     }
     return bad;

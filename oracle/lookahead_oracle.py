@@ -115,13 +115,13 @@ def hull_volume(pts, prune_gap=1e30, mesh_vol=0.0):
         ep[1::2] = pts[imin]
         flags, faces = O.hull(O.quantize(ep))
         if len(faces) and prune_gap < 1e29:
-            rough = abs(O.mesh_volume(ep, faces))
+            rough = abs(O.hull_volume(ep, faces))
             if max(rough - mesh_vol, 0.0) > prune_gap:
                 return rough, True
     flags, faces = O.hull(O.quantize(pts))
     if len(faces) == 0:
         return 0.0, False
-    return abs(O.mesh_volume(pts, faces)), False
+    return abs(O.hull_volume(pts, faces)), False
 
 
 def hull_mesh(pts):
@@ -134,7 +134,7 @@ def hull_mesh(pts):
     idx = np.nonzero(flags)[0]
     remap = -np.ones(len(pts), np.int64)
     remap[idx] = np.arange(len(idx))
-    return pts[idx].copy(), remap[faces].astype(np.int32), abs(O.mesh_volume(pts, faces))
+    return pts[idx].copy(), remap[faces].astype(np.int32), abs(O.hull_volume(pts, faces))
 
 
 def hausdorff(hv, ht, mv, mt):

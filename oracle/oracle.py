@@ -29,6 +29,8 @@ def lib():
         L.orc_general_position.restype = C.c_int
         L.orc_mesh_volume.argtypes = [P, P, C.c_int]
         L.orc_mesh_volume.restype = C.c_double
+        L.orc_hull_volume.argtypes = [P, P, C.c_int]
+        L.orc_hull_volume.restype = C.c_double
         L.orc_hull_bruteforce.argtypes = [P, C.c_int, P]
         L.orc_hull_bruteforce.restype = C.c_int
         L.orc_hull_batch.argtypes = [P, P, C.c_int, P, P, P, P, C.c_int]
@@ -83,6 +85,14 @@ def mesh_volume(pts, tris):
     pts = np.ascontiguousarray(pts, np.float32)
     tris = np.ascontiguousarray(tris, np.int32)
     return lib().orc_mesh_volume(_p(pts), _p(tris), len(tris))
+
+
+def hull_volume(pts, tris):
+    """FP64 volume of a canonical hull mesh in the GPU library's fixed
+    summation order (64 contiguous chunks, then their partial sums)."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    return lib().orc_hull_volume(_p(pts), _p(tris), len(tris))
 
 
 def hull_bruteforce(q):

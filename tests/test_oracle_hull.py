@@ -135,7 +135,7 @@ def test_batch_equals_single_and_reports_general_position():
         f1, faces = oracle.hull(q)
         assert np.array_equal(flags[off[i]:off[i + 1]], f1)
         assert nf[i] == len(faces)
-        assert vol[i] == oracle.mesh_volume(p, faces)
+        assert vol[i] == oracle.hull_volume(p, faces)
         assert gen[i] == int(oracle.general_position(q, faces))
     assert gen.sum() >= 24  # the uniform-ball protocol is overwhelmingly in general position
 
