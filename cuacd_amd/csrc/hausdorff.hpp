@@ -353,6 +353,9 @@ struct HdTask {
     float* out_d2;              // max squared nearest distance
     int* out_ns;                // samples used (may be null)
     float* out_samples;         // optional sample dump (K*3 floats)
+    // filled by the preparation kernel for the query kernel
+    int ns = 0;                 // samples drawn
+    float area = 0.0f;          // source area (fp32 as the sampler uses it)
 };
 
 }  // namespace cuacd

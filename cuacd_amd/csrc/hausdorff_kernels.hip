@@ -91,15 +91,18 @@ __device__ __forceinline__ V3 bary(V3 a, V3 b, V3 c, uint32_t ha, uint32_t hb) {
     return {wa * a.x + wb * b.x + wc * c.x, wa * a.y + wb * b.y + wc * c.y, wa * a.z + wb * b.z + wc * c.z};
 }
 
-__global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict__ tasks, int n_tasks,
-                                                       char* __restrict__ ws_base) {
+// Preparation, one workgroup per task: the target's BVH, the source's
+// sample layout (mode 0: per-triangle counts and their exclusive prefix;
+// mode 1: the cumulative area), the sample count; clears the result.
+__global__ __launch_bounds__(HD_T) void hd_prep_kernel(HdTask* tasks, int n_tasks, char* __restrict__ ws_base) {
     __shared__ HdLDS s;
     if ((int)blockIdx.x >= n_tasks) return;
-    const HdTask tk = tasks[blockIdx.x];
+    HdTask& tk = tasks[blockIdx.x];
     const int tid = threadIdx.x, wv = tid >> 6;
     if (tk.snt <= 0 || tk.tnt <= 0) {
         if (tid == 0) {
             *tk.out_d2 = 0.0f;
+            tk.ns = 0;
             if (tk.out_ns) *tk.out_ns = 0;
         }
         return;
@@ -107,18 +110,17 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
     const MeshView src{tk.sv, tk.st, tk.snv, tk.snt};
     const MeshView tgt{tk.tv, tk.tt, tk.tnv, tk.tnt};
     char* ws = ws_base + tk.ws_off;
-    const HdBvh bvh = hd_build(tgt, ws, &s);
+    (void)hd_build(tgt, ws, &s);
     int nbox = 0;
     hd_bvh_levels(tgt.nt, nullptr, nullptr, &nbox);
     char* ws2 = ws + (unsigned long long)((tgt.nt + 3) / 4 * 4) * 48ull + (unsigned long long)nbox * 32ull;
-
     auto area_of = [&](int k) {
         V3 a, b, c;
         src.tri(k, a, b, c);
         return tri_area(a, b, c);
     };
-    float local = 0.0f;
     int ns = 0;
+    float area = 0.0f;
     if (tk.mode == 0) {
         // reference sampler: total area (FP64 sum of the fp32 triangle areas,
         // rounded once, so the counts do not depend on the summation order),
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
         double tsum = 0.0;
         for (int q = 0; q < HD_T / 64; ++q) tsum += s.dred[q];
         __syncthreads();
-        const float area = (float)tsum;
+        area = (float)tsum;
         int res = (int)((float)HD_RESOLUTION * area);
         if (res < HD_MIN_SAMPLES) res = HD_MIN_SAMPLES;
         const int nt = src.nt;
@@ -141,9 +143,59 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
             const int ac = area > 1e-20f ? (int)(dens * area_of(k)) : 0;
             return ac > 0 ? ac : ((k % step == 0) ? 1 : 0);
         };
-        int* pre = (int*)ws2;
-        ns = block_excl_scan_int(nt, pre, &s, count_of);
-        for (int i = tid; i < ns; i += HD_T) {
+        ns = block_excl_scan_int(nt, (int*)ws2, &s, count_of);
+    } else {
+        area = block_incl_scan_area(src.nt, (float*)ws2, &s, area_of);
+        ns = tk.K;
+    }
+    if (tid == 0) {
+        *tk.out_d2 = 0.0f;
+        tk.ns = ns;
+        tk.area = area;
+        if (tk.out_ns) *tk.out_ns = ns;
+    }
+}
+
+// Queries: `chunks` workgroups per task, each answering a contiguous share
+// of the task's samples; the maximum is merged with an atomic max on the
+// result's bits (non-negative floats order as unsigned integers).
+__global__ __launch_bounds__(HD_T) void hd_query_kernel(const HdTask* __restrict__ tasks, int n_tasks, int chunks,
+                                                        char* __restrict__ ws_base) {
+    __shared__ int qoff[HD_MAX_LEVELS], qcnt[HD_MAX_LEVELS];
+    __shared__ float qred[HD_T / 64];
+    const int ti = (int)blockIdx.x / chunks, ch = (int)blockIdx.x % chunks;
+    if (ti >= n_tasks) return;
+    const HdTask tk = tasks[ti];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const int ns = tk.ns;
+    if (ns <= 0 || tk.snt <= 0 || tk.tnt <= 0) return;
+    const int per = (ns + chunks - 1) / chunks;
+    const int i0 = ch * per, i1 = min(ns, i0 + per);
+    if (i0 >= i1) return;
+    const MeshView src{tk.sv, tk.st, tk.snv, tk.snt};
+    char* ws = ws_base + tk.ws_off;
+    HdBvh bvh;
+    {
+        int lo[HD_MAX_LEVELS], lc[HD_MAX_LEVELS], nbox = 0;
+        bvh.top = hd_bvh_levels(tk.tnt, lo, lc, &nbox);
+        if (tid == 0)
+            for (int L = 0; L <= bvh.top; ++L) { qoff[L] = lo[L]; qcnt[L] = lc[L]; }
+        bvh.nt = tk.tnt;
+        bvh.off = (asptr<AS_LDS, int>)qoff;
+        bvh.cnt = (asptr<AS_LDS, int>)qcnt;
+        const int ntp = (tk.tnt + 3) / 4 * 4;
+        bvh.tri = (f4v*)ws;
+        bvh.box = (f4v*)(ws + (unsigned long long)ntp * 48ull);
+        __syncthreads();
+    }
+    int nbox = 0;
+    hd_bvh_levels(tk.tnt, nullptr, nullptr, &nbox);
+    char* ws2 = ws + (unsigned long long)((tk.tnt + 3) / 4 * 4) * 48ull + (unsigned long long)nbox * 32ull;
+    float local = 0.0f;
+    if (tk.mode == 0) {
+        const int* pre = (const int*)ws2;
+        const int nt = src.nt;
+        for (int i = i0 + tid; i < i1; i += HD_T) {
             int lo = 0, hi = nt - 1;  // last t with pre[t] <= i
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
@@ -159,11 +211,9 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
             local = fmaxf(local, hd_nearest_d2(bvh, p));
         }
     } else {
-        float* cum = (float*)ws2;
-        const float area = block_incl_scan_area(src.nt, cum, &s, area_of);
-        ns = tk.K;
-        const float scale = area / (float)tk.K;
-        for (int i = tid; i < ns; i += HD_T) {
+        const float* cum = (const float*)ws2;
+        const float scale = tk.area / (float)tk.K;
+        for (int i = i0 + tid; i < i1; i += HD_T) {
             const uint32_t h1 = wang_hash((uint32_t)i * 0x9E3779B1u + 0x632BE5ABu);
             const uint32_t h2 = wang_hash(h1 + 0x9e3779b9u);
             const uint32_t h3 = wang_hash(h2 + 0x9e3779b9u);
@@ -186,19 +236,25 @@ __global__ __launch_bounds__(HD_T) void hd_task_kernel(const HdTask* __restrict_
         }
     }
     local = wave_max_f(local);
-    if (lane_id() == 0) s.red[wv][6] = local;
+    if (lane_id() == 0) qred[wv] = local;
     __syncthreads();
     if (tid == 0) {
         float m = 0.0f;
-        for (int q = 0; q < HD_T / 64; ++q) m = fmaxf(m, s.red[q][6]);
-        *tk.out_d2 = m;
-        if (tk.out_ns) *tk.out_ns = ns;
+        for (int q = 0; q < HD_T / 64; ++q) m = fmaxf(m, qred[q]);
+        atomicMax((unsigned int*)tk.out_d2, __float_as_uint(m));
     }
 }
 
 int launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(hd_task_kernel, dim3(n), dim3(HD_T), 0, ctx->stream, d_tasks, n, d_ws);
+    hipLaunchKernelGGL(hd_prep_kernel, dim3(n), dim3(HD_T), 0, ctx->stream, (HdTask*)d_tasks, n, d_ws);
+    HIP_TRY(hipGetLastError());
+    // about 4096 query workgroups per launch, at least one per task
+    int chunks = (4096 + n - 1) / n;
+    if (chunks > 64) chunks = 64;
+    if (chunks < 1) chunks = 1;
+    hipLaunchKernelGGL(hd_query_kernel, dim3((unsigned)(n * chunks)), dim3(HD_T), 0, ctx->stream, d_tasks, n, chunks,
+                       d_ws);
     HIP_TRY(hipGetLastError());
     return 0;
 }
