@@ -27,21 +27,24 @@ PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_
 
 
 def kernel_name(cls, wide_min, half_cap):
-    """Kernel(s) behind a hull class reported by the library (the first pass
-    and, after '+', the full-capacity retry)."""
+    """Kernel behind a hull class reported by the library."""
     if cls == "kdop_prefilter":
         return "la_kdop_kernel"
+    if cls.endswith("_retry"):
+        base = cls[:-6]
+        n = int(base[6:])
+        return "la_hull_block<4>" if base.startswith("hull_b") else f"la_hull_lds4<{n // 256}>"
     if cls.startswith("hull_b"):
         n = int(cls[6:])
         ppl = n // 512
         if n == 2048 and half_cap:
-            return "la_hull_ldsv<4,8,1024>+la_hull_block<4>"
+            return "la_hull_ldsv<4,8,1024>"
         return f"la_hull_block<{ppl}>"
     n = int(cls[6:])
     ppl = n // 64
     if wide_min > 0 and ppl >= 4 and n > wide_min:
         if ppl >= 8 and half_cap:
-            return f"la_hull_ldsv<{ppl // 4},4,{n // 2}>+la_hull_lds4<{ppl // 4}>"
+            return f"la_hull_ldsv<{ppl // 4},4,{n // 2}>"
         return f"la_hull_lds4<{ppl // 4}>"
     return f"la_hull_wave<{ppl}>"
 

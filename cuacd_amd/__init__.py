@@ -37,7 +37,8 @@ LOOKAHEAD_SECTIONS = ("setup", "refresh_select", "concave_planes", "level0", "de
                       "post_merge", "readback")
 # hull kernel classes timed per launch (Context.last_lookahead_kernels)
 LOOKAHEAD_KERNELS = ("hull_w64", "hull_w128", "hull_w256", "hull_w512", "hull_w1024", "hull_b2048", "hull_b4096",
-                     "hull_b8192", "hull_b16384", "kdop_prefilter")
+                     "hull_b8192", "hull_b16384", "kdop_prefilter", "hull_w512_retry", "hull_w1024_retry",
+                     "hull_b2048_retry")
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
@@ -428,7 +429,7 @@ class Context:
         if width + 4 * n_concave_edges >= 512:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
-        stats = np.zeros(62, np.float64)
+        stats = np.zeros(71, np.float64)
         raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
                                    hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
                                    max_iters, width, width2, float(threshold), depth, max_n_cutting,
@@ -440,7 +441,7 @@ class Context:
                                             "alg_bytes": float(ph[i, 2])}
                                      for i, name in enumerate(LOOKAHEAD_PHASES)}
         self.last_lookahead_sections = {name: float(stats[24 + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
-        kt = stats[32:62].reshape(10, 3)
+        kt = stats[32:71].reshape(13, 3)
         self.last_lookahead_kernels = {name: {"ms": float(kt[i, 0]), "launches": int(kt[i, 1]),
                                               "alg_bytes": float(kt[i, 2])}
                                        for i, name in enumerate(LOOKAHEAD_KERNELS)}

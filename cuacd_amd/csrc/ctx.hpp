@@ -16,7 +16,9 @@ namespace cuacd {
 enum KtId {
     KT_HULL_W1, KT_HULL_W2, KT_HULL_W4, KT_HULL_W8, KT_HULL_W16,  // one-workgroup LDS hulls by size class
     KT_HULL_B4, KT_HULL_B8, KT_HULL_B16, KT_HULL_B32,              // multi-wave hulls, workspace in HBM
-    KT_KDOP, KT_COUNT
+    KT_KDOP,
+    KT_HULL_R8, KT_HULL_R16, KT_HULL_RB4,  // full-capacity retries of the half-capacity first passes
+    KT_COUNT
 };
 struct KTimers {
     struct Rec {

@@ -1006,6 +1006,15 @@ __global__ __launch_bounds__(PC_T) void la_bbox_kernel(LaBoxTask* tasks, int n) 
         for (int j = 0; j < 3; ++j) { T.lo[j] = lo[j]; T.hi[j] = hi[j]; }
 }
 
+__global__ __launch_bounds__(256) void la_segcopy_kernel(const LaSeg* segs, int n) {
+    if ((int)blockIdx.x >= n) return;
+    const LaSeg g = segs[blockIdx.x];
+    const uint32_t* src = (const uint32_t*)g.src;
+    uint32_t* dst = (uint32_t*)g.dst;
+    const unsigned long long words = g.bytes / 4;
+    for (unsigned long long i = threadIdx.x; i < words; i += 256) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(PC_T) void la_gather_kernel(LaGatherTask* tasks, int n) {
     __shared__ int wc[PC_T / 64];
     if ((int)blockIdx.x >= n) return;
@@ -1171,7 +1180,7 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
             hipLaunchKernelGGL(kf, dim3((unsigned)nre), dim3(64 * LA_LDS4_NW), ldf, ctx->stream, d_tasks, d_retry + 1,
                                nre, ctx->d_pool, ctx->d_err);
             HIP_TRY(hipGetLastError());
-            kt_end(ctx, kid, t1, 0.0);
+            kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, bytes * (double)nre / (double)ids.size());
         }
         HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
         HIP_TRY(hipFreeAsync(d_retry, ctx->stream));
@@ -1215,6 +1224,7 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
     if (ids_in.empty()) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
     std::vector<int> ids = ids_in;
+    int retry_kid = -1;
     if (maxn <= 4 * T && hull_half_cap()) {
         // first pass with the workspace in LDS (1024 hull vertices), then the
         // sets whose hulls did not fit, with the workspace in HBM
@@ -1237,6 +1247,7 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
         HIP_TRY(hipMemcpyAsync(&nre, d_retry, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         REQUIRE(nre >= 0 && nre <= (int)ids.size(), "lookahead: hull retry count %d out of range", nre);
+        const double per_task = bytes / (double)ids.size();
         ids.resize(nre);
         if (nre) HIP_TRY(hipMemcpyAsync(ids.data(), d_retry + 1, 4ull * nre, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
@@ -1244,7 +1255,8 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ids.empty()) return 0;
         std::sort(ids.begin(), ids.end());
-        bytes = 0.0;
+        bytes = per_task * (double)nre;
+        retry_kid = KT_HULL_RB4;
     }
     BlockFn k;
     int kid;
@@ -1269,7 +1281,7 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
         hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, ctx->stream, d_tasks, d_ids + b, cnt, gws, stride, vc, np,
                            ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, kid, t0, bytes * (double)cnt / (double)ids.size());
+        kt_end(ctx, retry_kid >= 0 ? retry_kid : kid, t0, bytes * (double)cnt / (double)ids.size());
     }
     HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
     HIP_TRY(hipFreeAsync(gws, ctx->stream));
@@ -1411,6 +1423,13 @@ int la_launch_edges(gpu_ctx_t ctx, LaEdgeTask* d_tasks, int n, float cos_pos, fl
 int la_launch_bbox(gpu_ctx_t ctx, LaBoxTask* d_tasks, int n) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(la_bbox_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int la_launch_segcopy(gpu_ctx_t ctx, const LaSeg* d_segs, int n) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(la_segcopy_kernel, dim3(n), dim3(256), 0, ctx->stream, d_segs, n);
     HIP_TRY(hipGetLastError());
     return 0;
 }

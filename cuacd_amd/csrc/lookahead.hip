@@ -168,6 +168,60 @@ struct Engine {
     std::chrono::steady_clock::time_point t_mark = std::chrono::steady_clock::now();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int hull_phase = PH_IHULL;
+    char* pinned = nullptr;  // grow-only pinned staging for batched readbacks
+    size_t pinned_cap = 0;
+
+    // Device -> host for many small device ranges at once: one segmented
+    // device copy into a packed staging buffer, one transfer into pinned
+    // memory. Returns the packed host copy (range k at offs[k], 16-byte
+    // aligned), valid until the next call.
+    const char* gather_to_host(const std::vector<std::pair<const void*, size_t>>& src, std::vector<size_t>& offs,
+                               int* rc) {
+        *rc = 0;
+        offs.resize(src.size());
+        size_t total = 0;
+        for (size_t k = 0; k < src.size(); ++k) {
+            offs[k] = total;
+            total += (src[k].second + 15) / 16 * 16;
+        }
+        if (total == 0) return pinned;
+        if (total > pinned_cap) {
+            if (pinned) (void)hipHostFree(pinned);
+            pinned = nullptr;
+            pinned_cap = 0;
+            const size_t cap = std::max(total + total / 2, (size_t)(1 << 20));
+            if (hipHostMalloc((void**)&pinned, cap, hipHostMallocDefault) != hipSuccess) {
+                set_error(ctx, "lookahead: pinned staging allocation failed");
+                *rc = -1;
+                return nullptr;
+            }
+            pinned_cap = cap;
+        }
+        DevBuf<char> stage;
+        DevBuf<LaSeg> dsegs;
+        std::vector<LaSeg> segs;
+        for (size_t k = 0; k < src.size(); ++k)
+            if (src[k].second) segs.push_back({src[k].first, nullptr, src[k].second});
+        if (stage.alloc(ctx, total) != 0 || dsegs.alloc(ctx, segs.size() + 1) != 0) {
+            set_error(ctx, "lookahead: readback staging allocation failed");
+            *rc = -1;
+            return nullptr;
+        }
+        for (size_t k = 0, j = 0; k < src.size(); ++k)
+            if (src[k].second) segs[j++].dst = stage.p + offs[k];
+        if (!segs.empty()) {
+            if (hipMemcpyAsync(dsegs.p, segs.data(), sizeof(LaSeg) * segs.size(), hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess ||
+                la_launch_segcopy(ctx, dsegs.p, (int)segs.size()) != 0 ||
+                hipMemcpyAsync(pinned, stage.p, total, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess) {
+                set_error(ctx, "lookahead: batched readback failed");
+                *rc = -1;
+                return nullptr;
+            }
+        }
+        return pinned;
+    }
 
     void lap(int s) {
         const auto t = std::chrono::steady_clock::now();
@@ -196,6 +250,7 @@ struct Engine {
     ~Engine() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (pinned) (void)hipHostFree(pinned);
     }
 
     int check_err(const char* where) {
@@ -259,6 +314,12 @@ struct Engine {
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaCompTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         LA_TRY(check_err("components"));
         // split parts (ascending mesh, part order: new pieces append in that order)
+        std::vector<std::pair<const void*, size_t>> osrc(h.size());
+        for (size_t k = 0; k < h.size(); ++k) osrc[k] = {h[k].out, sizeof(LaCompOut) * (size_t)h[k].n_comp};
+        std::vector<size_t> ooff;
+        int grc = 0;
+        const char* oh = gather_to_host(osrc, ooff, &grc);
+        if (grc) return grc;
         std::vector<LaCompOut> o(LA_MAX_COMP);
         for (size_t k = 0; k < h.size(); ++k) {
             const int m = who[k].first, i = who[k].second;
@@ -269,7 +330,7 @@ struct Engine {
                 S.parts[i].cc = ++S.cc_counter;  // first pass over this lineage: fresh id
                 continue;
             }
-            HIP_TRY(hipMemcpy(o.data(), h[k].out, sizeof(LaCompOut) * nc, hipMemcpyDeviceToHost));
+            memcpy(o.data(), oh + ooff[k], sizeof(LaCompOut) * nc);
             Part old = S.parts[i];
             int cc_base = 0;
             const bool fresh = old.cc == 0;
@@ -412,10 +473,17 @@ struct Engine {
         LA_TRY(check_err("concave edges"));
         std::vector<LaEdgeRec> r;
         const int K = P.n_concave;
+        std::vector<std::pair<const void*, size_t>> rsrc(h.size());
+        for (size_t c = 0; c < h.size(); ++c)
+            rsrc[c] = {h[c].out, sizeof(LaEdgeRec) * (size_t)std::min(h[c].count, h[c].cap)};
+        std::vector<size_t> roff;
+        int grc = 0;
+        const char* rh = gather_to_host(rsrc, roff, &grc);
+        if (grc) return grc;
         for (size_t c = 0; c < h.size(); ++c) {
             const int cnt = std::min(h[c].count, h[c].cap);
             r.resize(cnt);
-            if (cnt) HIP_TRY(hipMemcpy(r.data(), h[c].out, sizeof(LaEdgeRec) * cnt, hipMemcpyDeviceToHost));
+            if (cnt) memcpy(r.data(), rh + roff[c], sizeof(LaEdgeRec) * cnt);
             // edges in (lo, hi) order, reservoir sampling with the reference's LCG
             std::sort(r.begin(), r.end(), [](const LaEdgeRec& a, const LaEdgeRec& b) {
                 return a.lo != b.lo ? a.lo < b.lo : (a.hi != b.hi ? a.hi < b.hi : a.f0 < b.f0);
@@ -556,7 +624,20 @@ struct Engine {
             hd_pairs.push_back((int)i);
         }
         if (!hd_pairs.empty()) {
-            // host copies of the parts' hulls for the shared-plane search
+            // host copies of the parts' hulls for the shared-plane search (one readback)
+            std::vector<std::pair<const void*, size_t>> hsrc;
+            for (size_t k = 0; k < hd_pairs.size(); ++k) {
+                const MPair& q = pairs[hd_pairs[k]];
+                const Part& A = M[q.m].parts[q.p1];
+                const Part& B = M[q.m].parts[q.p2];
+                hsrc.push_back({A.hv, 12ull * A.hnv});
+                hsrc.push_back({A.ht, 12ull * A.hnt});
+                hsrc.push_back({B.hv, 12ull * B.hnv});
+            }
+            std::vector<size_t> hoff;
+            int grc = 0;
+            const char* hh = gather_to_host(hsrc, hoff, &grc);
+            if (grc) return grc;
             std::vector<LaGatherTask> mg(hd_pairs.size());
             size_t vtot = 0, ttot = 0;
             for (size_t k = 0; k < hd_pairs.size(); ++k) {
@@ -567,7 +648,8 @@ struct Engine {
                 g.v1 = A.v; g.n1 = A.nv; g.v2 = B.v; g.n2 = B.nv;
                 g.t1 = A.t; g.m1 = A.nt; g.t2 = B.t; g.m2 = B.nt;
                 float pl[4];
-                g.has_plane = shared_plane(A, B, pl);
+                g.has_plane = shared_plane((const float*)(hh + hoff[3 * k]), A.hnv, (const int*)(hh + hoff[3 * k + 1]),
+                                           A.hnt, (const float*)(hh + hoff[3 * k + 2]), B.hnv, pl);
                 for (int j = 0; j < 4; ++j) g.plane[j] = pl[j];
                 g.dv = (float*)(uintptr_t)vtot;
                 g.dt = (int*)(uintptr_t)ttot;
@@ -714,14 +796,10 @@ struct Engine {
 
     // Shared plane (reference cuda/merge.cuh:111-165): the first face of A's
     // hull with A's hull on its inner side and B's hull within 1e-3 of it.
-    bool shared_plane(const Part& A, const Part& B, float* out) {
-        std::vector<float> av(3ull * A.hnv), bv(3ull * B.hnv);
-        std::vector<int> at(3ull * A.hnt);
-        if (A.hnv) (void)hipMemcpy(av.data(), A.hv, 12ull * A.hnv, hipMemcpyDeviceToHost);
-        if (A.hnt) (void)hipMemcpy(at.data(), A.ht, 12ull * A.hnt, hipMemcpyDeviceToHost);
-        if (B.hnv) (void)hipMemcpy(bv.data(), B.hv, 12ull * B.hnv, hipMemcpyDeviceToHost);
+    // av/at: A's hull (host copy), bv: B's hull vertices.
+    bool shared_plane(const float* av, int a_hnv, const int* at, int a_hnt, const float* bv, int b_hnv, float* out) {
         const float tol = 1e-3f;
-        for (int t = 0; t < A.hnt; ++t) {
+        for (int t = 0; t < a_hnt; ++t) {
             const float* a = &av[3 * at[3 * t]];
             const float* bq = &av[3 * at[3 * t + 1]];
             const float* c = &av[3 * at[3 * t + 2]];
@@ -734,9 +812,9 @@ struct Engine {
             nx *= inv; ny *= inv; nz *= inv;
             const float d = nx * a[0] + ny * a[1] + nz * a[2];
             bool ok = true;
-            for (int j = 0; j < A.hnv && ok; ++j)
+            for (int j = 0; j < a_hnv && ok; ++j)
                 if (nx * av[3 * j] + ny * av[3 * j + 1] + nz * av[3 * j + 2] - d > tol) ok = false;
-            for (int j = 0; j < B.hnv && ok; ++j)
+            for (int j = 0; j < b_hnv && ok; ++j)
                 if (std::fabs(nx * bv[3 * j] + ny * bv[3 * j + 1] + nz * bv[3 * j + 2] - d) > tol) ok = false;
             if (ok) {
                 out[0] = nx; out[1] = ny; out[2] = nz; out[3] = d;
@@ -1276,34 +1354,46 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         if (rc == 0 && P.merge) rc = E.merge(all);
     }
     E.lap(S_POST);
-    // results
+    // results: every part's mesh and hull in one batched readback
     if (rc == 0) {
+        std::vector<std::pair<const void*, size_t>> src;
+        for (int m = 0; m < n_meshes; ++m)
+            for (Part& p : E.M[m].parts) {
+                src.push_back({p.v, 12ull * p.nv});
+                src.push_back({p.t, 12ull * p.nt});
+                src.push_back({p.hv, p.hv ? 12ull * p.hnv : 0});
+                src.push_back({p.ht, p.ht ? 12ull * p.hnt : 0});
+            }
+        std::vector<size_t> offs;
+        int grc = 0;
+        const char* hb = E.gather_to_host(src, offs, &grc);
+        rc = grc;
+        size_t q = 0;
         for (int m = 0; m < n_meshes && rc == 0; ++m) {
             MeshState& S = E.M[m];
             if (iters_out) iters_out[m] = S.iters;
             const int np = (int)S.parts.size();
             out[m].nparts = np;
             out[m].parts = (struct gpu_part_result*)calloc(np > 0 ? np : 1, sizeof(struct gpu_part_result));
-            for (int i = 0; i < np; ++i) {
+            for (int i = 0; i < np; ++i, q += 4) {
                 Part& p = S.parts[i];
                 gpu_part_result& r = out[m].parts[i];
-                r.nv = p.nv; r.nt = p.nt; r.hull_nv = p.hnv; r.hull_nt = p.hnt;
+                r.nv = p.nv; r.nt = p.nt;
+                r.hull_nv = p.hv ? p.hnv : 0;
+                r.hull_nt = p.ht ? p.hnt : 0;
                 r.mesh_vol = (float)p.mvol;
                 r.hull_vol = (float)p.hvol;
                 r.hausdorff = p.haus;
                 r.verts = (float*)malloc(12ull * (p.nv ? p.nv : 1));
                 r.tris = (int*)malloc(12ull * (p.nt ? p.nt : 1));
-                r.hull_verts = (float*)malloc(12ull * (p.hnv ? p.hnv : 1));
-                r.hull_tris = (int*)malloc(12ull * (p.hnt ? p.hnt : 1));
-                if (p.nv) HIP_TRY(hipMemcpyAsync(r.verts, p.v, 12ull * p.nv, hipMemcpyDeviceToHost, ctx->stream));
-                if (p.nt) HIP_TRY(hipMemcpyAsync(r.tris, p.t, 12ull * p.nt, hipMemcpyDeviceToHost, ctx->stream));
-                if (p.hnv && p.hv)
-                    HIP_TRY(hipMemcpyAsync(r.hull_verts, p.hv, 12ull * p.hnv, hipMemcpyDeviceToHost, ctx->stream));
-                if (p.hnt && p.ht)
-                    HIP_TRY(hipMemcpyAsync(r.hull_tris, p.ht, 12ull * p.hnt, hipMemcpyDeviceToHost, ctx->stream));
+                r.hull_verts = (float*)malloc(12ull * (r.hull_nv ? r.hull_nv : 1));
+                r.hull_tris = (int*)malloc(12ull * (r.hull_nt ? r.hull_nt : 1));
+                if (src[q].second) memcpy(r.verts, hb + offs[q], src[q].second);
+                if (src[q + 1].second) memcpy(r.tris, hb + offs[q + 1], src[q + 1].second);
+                if (src[q + 2].second) memcpy(r.hull_verts, hb + offs[q + 2], src[q + 2].second);
+                if (src[q + 3].second) memcpy(r.hull_tris, hb + offs[q + 3], src[q + 3].second);
             }
         }
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     // free everything the search still holds
     for (auto& S : E.M)

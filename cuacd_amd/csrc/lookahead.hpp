@@ -128,6 +128,14 @@ struct LaGatherTask {
 
 // ---- launchers (la_kernels.hip); all stream-ordered on ctx->stream --------------
 int la_launch_gather(gpu_ctx_t ctx, LaGatherTask* d_tasks, int n);
+// Segmented device copy: segment i copies bytes[i] (a multiple of 4) from
+// src[i] to dst[i]; one workgroup per segment.
+struct LaSeg {
+    const void* src;
+    void* dst;
+    unsigned long long bytes;
+};
+int la_launch_segcopy(gpu_ctx_t ctx, const LaSeg* d_segs, int n);
 int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n);
 int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n);
 int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n);

@@ -137,14 +137,15 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * convex hulls (hull_vert_off/hull_tri_off), decomposed independently with the
  * same parameters in one search whose launches span all meshes. out[m] gets
  * mesh m's parts (free each with gpu_result_free); out_iters[m] (optional) the
- * iterations it ran; out_stats (optional, 62 doubles): [0, 24) per phase -
+ * iterations it ran; out_stats (optional, 71 doubles): [0, 24) per phase -
  * components, part hulls, Hausdorff, concave edges, level-0 cuts, leaf
  * scoring, item hulls, merge - the GPU milliseconds, launches and algorithmic
  * bytes; [24, 32) host wall milliseconds by section - set-up, per-iteration
  * refresh and selection, concave planes, level 0, deeper levels, evaluate and
- * apply, post-loop passes and merge, result readback; [32, 62) per hull
+ * apply, post-loop passes and merge, result readback; [32, 71) per hull
  * kernel class - one-workgroup hulls of <= 64, 128, 256, 512, 1024 points,
- * multi-wave hulls of <= 2048, 4096, 8192, 16384 points, k-DOP prefilter -
+ * multi-wave hulls of <= 2048, 4096, 8192, 16384 points, k-DOP prefilter,
+ * full-capacity retries of the 512-, 1024- and 2048-point first passes -
  * the milliseconds between HIP events around each launch, launches and
  * algorithmic bytes (points read, volumes written). quick_depth is 0. */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
