@@ -38,13 +38,13 @@ def kernel_name(cls, wide_min, half_cap):
         n = int(cls[6:])
         ppl = n // 512
         if n == 2048 and half_cap:
-            return "la_hull_ldsv<4,8,1024>"
+            return "la_hull_ldsv<4,8>"
         return f"la_hull_block<{ppl}>"
     n = int(cls[6:])
     ppl = n // 64
     if wide_min > 0 and ppl >= 4 and n > wide_min:
         if ppl >= 8 and half_cap:
-            return f"la_hull_ldsv<{ppl // 4},4,{n // 2}>"
+            return f"la_hull_ldsv<{ppl // 4},4>"
         return f"la_hull_lds4<{ppl // 4}>"
     return f"la_hull_wave<{ppl}>"
 

@@ -127,7 +127,7 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
         return true;
     }
-    if (tid == 0) t.vol = hc.vol;
+    if (tid == 0) { t.vol = hc.vol; t.ncorner = hc.nv; }
     if (!t.emit) return true;
     const int nv = hc.nv;
     const int nt = hc.nt;
@@ -180,19 +180,19 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
     if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
 
-// Same, with the hull-vertex capacity reduced to VCAP (a smaller LDS
-// footprint, so two hulls share a CU); a set whose hull has more vertices is
-// queued on `retry` for the full-capacity kernel.
-template <int PPL, int NW, int VCAP>
-__global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n, int* retry,
-                                                       DPool* pool, int* err) {
+// Same, with the hull-vertex capacity reduced to vcap (a smaller LDS
+// footprint, so several hulls share a CU); a set whose hull needs more
+// vertices is queued on `retry` for the full-capacity kernel.
+template <int PPL, int NW>
+__global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n, int vcap,
+                                                       int* retry, DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
     if ((int)blockIdx.x >= n) return;
     const int id = ids[blockIdx.x];
     LaHullTask& t = tasks[id];
     constexpr int CAP = 64 * NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, VCAP, CAP), &sb);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, vcap, CAP), &sb);
     const bool done = la_hull_item<PPL, AS_LDS, NW>(t, w, &sb, pool, err, retry, id);
     if (done && threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
@@ -1136,7 +1136,18 @@ int hull_wide_min() {
     return v;
 }
 
-// Half-capacity first pass for the four-wave tiers (CUACD_HULL_HALFCAP=0 disables).
+// First-pass vertex capacity of the four-wave tiers in eighths of their point
+// capacity (CUACD_HULL_VCAP8, 1..8).
+int hull_vcap_eighths() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("CUACD_HULL_VCAP8");
+        v = e ? std::min(8, std::max(1, atoi(e))) : 3;
+    }
+    return v;
+}
+
+// Reduced-capacity first pass for the four-wave tiers (CUACD_HULL_HALFCAP=0 disables).
 int hull_half_cap() {
     static int v = -1;
     if (v < 0) {
@@ -1154,10 +1165,13 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
     if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
         // first pass with half the vertex capacity (two hulls per CU), then
         // the sets whose hulls did not fit at full capacity
-        using CapFn = void (*)(LaHullTask*, const int*, int, int*, DPool*, int*);
-        CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW, 256> : la_hull_ldsv<4, LA_LDS4_NW, 512>;
+        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
+        CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>;
         WaveFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
-        const unsigned ldc = hull_ws_bytes<AS_LDS>(32 * ppl, 64 * ppl);
+        // the canonical hulls of these sets have few corners (mostly under a
+        // quarter of the points), so 3/8 of the points fits nearly all
+        const int vcap = (64 * ppl * hull_vcap_eighths()) / 8;
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(vcap, 64 * ppl);
         const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
@@ -1168,7 +1182,7 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
         hipEvent_t t0 = kt_begin(ctx);
         hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, ctx->stream, d_tasks, d_ids,
-                           (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+                           (int)ids.size(), vcap, d_retry, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes);
         int nre = 0;
@@ -1228,8 +1242,9 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
     if (maxn <= 4 * T && hull_half_cap()) {
         // first pass with the workspace in LDS (1024 hull vertices), then the
         // sets whose hulls did not fit, with the workspace in HBM
-        using CapFn = void (*)(LaHullTask*, const int*, int, int*, DPool*, int*);
-        CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW, 1024>;
+        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
+        CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW>;
+        const int tb = T;
         const int kidc = KT_HULL_B4;
         const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
@@ -1239,8 +1254,8 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
         HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
         hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(T), ldc, ctx->stream, d_tasks, d_ids,
-                           (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(tb), ldc, ctx->stream, d_tasks, d_ids,
+                           (int)ids.size(), 1024, d_retry, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kidc, t0, bytes);
         int nre = 0;

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "lookahead.hpp"
@@ -248,6 +249,7 @@ struct Engine {
         return 0;
     }
     ~Engine() {
+        print_hull_stats();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (pinned) (void)hipHostFree(pinned);
@@ -536,7 +538,51 @@ struct Engine {
         HIP_TRY(hipMemcpyAsync(h.data(), d.p, sizeof(LaHullTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         LA_TRY(tend(hull_phase, 0.0));
         for (auto& x : h) stats[hull_phase][2] += 12.0 * (x.n + x.hnv + x.hnt) + 8.0;
+        if (hull_stats_on())
+            for (auto& x : h) {
+                if (x.pruned) { ++hstat[5][0]; continue; }
+                const int m = x.use_frame && x.nf > 0 ? x.nf : x.n;  // points the exact hull saw
+                const int c = m <= 256 ? 0 : m <= 512 ? 1 : m <= 1024 ? 2 : m <= 2048 ? 3 : 4;
+                const int b = m > 0 ? std::min(7, (8 * x.ncorner) / std::max(m, 1)) : 0;
+                ++hstat[c][b];
+                hin[c] += x.n;
+            }
         return check_err("item hulls");
+    }
+
+    // Optional statistics of the item hulls (CUACD_HULL_STATS=1): per class
+    // of points seen by the exact hull, the corners-per-point ratio in eighths.
+    static bool hull_stats_on() {
+        static int v = -1;
+        if (v < 0) v = getenv("CUACD_HULL_STATS") ? 1 : 0;
+        return v == 1;
+    }
+    long long hstat[6][8] = {};
+    double hin[5] = {};
+    long long bnb_scored = 0, bnb_skipped = 0;
+    // Two-round leaf scoring with the decision-preserving bound
+    // (CUACD_LEAF_BNB=0 scores every leaf).
+    static bool leaf_bnb_on() {
+        static int v = -1;
+        if (v < 0) {
+            const char* e = getenv("CUACD_LEAF_BNB");
+            v = e ? (atoi(e) != 0) : 1;
+        }
+        return v == 1;
+    }
+    void print_hull_stats() const {
+        if (!hull_stats_on()) return;
+        const char* nm[5] = {"<=256", "<=512", "<=1024", "<=2048", ">2048"};
+        for (int c = 0; c < 5; ++c) {
+            long long tot = 0;
+            for (int b = 0; b < 8; ++b) tot += hstat[c][b];
+            fprintf(stderr, "[hull-stats] %-7s %9lld hulls (mean input %.0f) corner/point eighths:", nm[c], tot,
+                    tot ? hin[c] / (double)tot : 0.0);
+            for (int b = 0; b < 8; ++b) fprintf(stderr, " %lld", hstat[c][b]);
+            fprintf(stderr, "\n");
+        }
+        fprintf(stderr, "[hull-stats] pruned %lld; leaves scored %lld, skipped by the bound %lld\n", hstat[5][0],
+                bnb_scored, bnb_skipped);
     }
 
     std::vector<void*> ext_bufs;  // hipMallocAsync buffers owned by merged parts
@@ -1039,15 +1085,42 @@ struct Engine {
                     ht_who.push_back({ci, 1});
                 }
             } else {
-                std::vector<LaScoreTask> h(cands.size());
-                for (size_t q = 0; q < cands.size(); ++q) {
-                    const IPart& wp = cur[cands[q].item].parts[cur[cands[q].item].np - 1];
-                    LaScoreTask t{};
-                    t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
-                    for (int j = 0; j < 4; ++j) t.plane[j] = cands[q].pl[j];
-                    h[q] = t;
+                // branch-and-bound bound per mesh (cuda/la_expand.cu:434-457): min over the
+                // items entering this level of sum(lc) + worst * (levels left)
+                std::vector<float> best_ub(M.size(), 1e30f);
+                for (const Item& it : cur) {
+                    if (it.nl <= 0) continue;
+                    float ps = 0.0f;
+                    for (int l = 0; l < it.nl; ++l) ps += it.lc[l];
+                    const float ub = ps + it.lc[it.nl - 1] * (float)(total_levels - it.nl);
+                    if (ub < best_ub[it.mesh]) best_ub[it.mesh] = ub;
                 }
-                if (!h.empty()) {
+                // Lower bound of every leaf of an item: the leaf's path cost with
+                // the two new halves left out of its worst part (same float
+                // operations, max only grows, so the bound never exceeds the cost).
+                std::vector<float> item_lb(cur.size());
+                for (size_t i = 0; i < cur.size(); ++i) {
+                    const Item& par = cur[i];
+                    float wlb = 0.0f;
+                    for (int j = 0; j < par.np - 1; ++j) wlb = std::max(wlb, cost_rv(par.parts[j].hvol, par.parts[j].mvol));
+                    float sum = 0.0f;
+                    for (int l = 0; l < par.nl; ++l) sum += par.lc[l];
+                    sum += wlb;
+                    item_lb[i] = sum / (float)(par.nl + 1);
+                }
+                std::vector<char> mesh_has(M.size(), 0);
+                // score a set of candidates: volumes and point sets, their hulls, the leaf records
+                auto run_round = [&](const std::vector<int>& qs) -> int {
+                    if (qs.empty()) return 0;
+                    std::vector<LaScoreTask> h(qs.size());
+                    for (size_t u = 0; u < qs.size(); ++u) {
+                        const Cand& cd = cands[qs[u]];
+                        const IPart& wp = cur[cd.item].parts[cur[cd.item].np - 1];
+                        LaScoreTask t{};
+                        t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
+                        for (int j = 0; j < 4; ++j) t.plane[j] = cd.pl[j];
+                        h[u] = t;
+                    }
                     DevBuf<LaScoreTask> dt;
                     REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
                     HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaScoreTask) * h.size(), hipMemcpyHostToDevice,
@@ -1065,82 +1138,107 @@ struct Engine {
                         stats[PH_SCORE][2] += by;
                     }
                     LA_TRY(check_err("leaf scoring"));
-                }
-                // branch-and-bound bound per mesh (cuda/la_expand.cu:434-457): min over the
-                // items entering this level of sum(lc) + worst * (levels left)
-                std::vector<float> best_ub(M.size(), 1e30f);
-                for (const Item& it : cur) {
-                    if (it.nl <= 0) continue;
-                    float ps = 0.0f;
-                    for (int l = 0; l < it.nl; ++l) ps += it.lc[l];
-                    const float ub = ps + it.lc[it.nl - 1] * (float)(total_levels - it.nl);
-                    if (ub < best_ub[it.mesh]) best_ub[it.mesh] = ub;
-                }
-                for (size_t q = 0; q < cands.size(); ++q) {
-                    const LaScoreTask& t = h[q];
-                    if (t.ppts) pts_free.push_back(t.ppts);
-                    if (t.npts) pts_free.push_back(t.npts);
-                    if (t.npt == 0 || t.nnt == 0 || !t.ppts || !t.npts) continue;
-                    const Item& par = cur[cands[q].item];
-                    // pruning threshold (cuda/la_expand.cu:380-399)
-                    double gap = 1e30;
-                    const float ub = best_ub[par.mesh];
-                    const int nl = par.nl;
-                    if (ub < 1e29f && nl > 0 && nl < total_levels) {
-                        float ps = 0.0f;
-                        for (int l = 0; l < nl; ++l) ps += par.lc[l];
-                        const float rv_thr = ub - ps;
-                        if (rv_thr <= 0.0f) gap = -1.0;
-                        else {
-                            const float r = rv_thr / 0.3f;
-                            gap = (double)(r * r * r * (4.0f * LA_PI_F / 3.0f));
+                    std::vector<LaHullTask> rh;
+                    std::vector<std::pair<int, int>> rwho;
+                    const size_t s0 = srec.size();
+                    for (size_t u = 0; u < qs.size(); ++u) {
+                        const LaScoreTask& t = h[u];
+                        const Cand& cd = cands[qs[u]];
+                        if (t.ppts) pts_free.push_back(t.ppts);
+                        if (t.npts) pts_free.push_back(t.npts);
+                        if (t.npt == 0 || t.nnt == 0 || !t.ppts || !t.npts) continue;
+                        const Item& par = cur[cd.item];
+                        // pruning threshold (cuda/la_expand.cu:380-399)
+                        double gap = 1e30;
+                        const float ub = best_ub[par.mesh];
+                        const int nl = par.nl;
+                        if (ub < 1e29f && nl > 0 && nl < total_levels) {
+                            float ps = 0.0f;
+                            for (int l = 0; l < nl; ++l) ps += par.lc[l];
+                            const float rv_thr = ub - ps;
+                            if (rv_thr <= 0.0f) gap = -1.0;
+                            else {
+                                const float r = rv_thr / 0.3f;
+                                gap = (double)(r * r * r * (4.0f * LA_PI_F / 3.0f));
+                            }
                         }
+                        const int ci = (int)srec.size();
+                        srec.push_back({cd.item, cd.k, t.pvol, t.nvol, 0.0, 0.0, false});
+                        LaHullTask a{};
+                        a.pts = t.ppts; a.n = t.np_pts; a.prune_gap = gap; a.mesh_vol = t.pvol;
+                        LaHullTask b{};
+                        b.pts = t.npts; b.n = t.nn_pts; b.prune_gap = gap; b.mesh_vol = t.nvol;
+                        rh.push_back(a);
+                        rwho.push_back({ci, 0});
+                        rh.push_back(b);
+                        rwho.push_back({ci, 1});
                     }
-                    const int ci = (int)srec.size();
-                    srec.push_back({cands[q].item, cands[q].k, t.pvol, t.nvol, 0.0, 0.0, false});
-                    LaHullTask a{};
-                    a.pts = t.ppts; a.n = t.np_pts; a.prune_gap = gap; a.mesh_vol = t.pvol;
-                    LaHullTask b{};
-                    b.pts = t.npts; b.n = t.nn_pts; b.prune_gap = gap; b.mesh_vol = t.nvol;
-                    ht.push_back(a);
-                    ht_who.push_back({ci, 0});
-                    ht.push_back(b);
-                    ht_who.push_back({ci, 1});
+                    LA_TRY(hull_volumes(rh));
+                    to_free.insert(to_free.end(), pts_free.begin(), pts_free.end());
+                    pts_free.clear();
+                    // leaf costs: the worst part is the maximum rv cost over the
+                    // inherited parts and both halves; path cost = mean level cost
+                    for (size_t u = 0; u < rh.size(); ++u) {
+                        SRec& r = srec[rwho[u].first];
+                        (rwho[u].second ? r.nhv : r.phv) = rh[u].vol;
+                        if (rh[u].pruned) r.pruned = true;
+                    }
+                    for (size_t ri = s0; ri < srec.size(); ++ri) {
+                        const SRec& r = srec[ri];
+                        const Item& par = cur[r.item];
+                        mesh_has[par.mesh] = 1;
+                        float worst = std::max(cost_rv(r.phv, r.pvol), cost_rv(r.nhv, r.nvol));
+                        for (int i = 0; i < par.np - 1; ++i) worst = std::max(worst, cost_rv(par.parts[i].hvol, par.parts[i].mvol));
+                        float sum = 0.0f;
+                        for (int l = 0; l < par.nl; ++l) sum += par.lc[l];
+                        sum += worst;
+                        Leaf L;
+                        L.c = cur_c[r.item];
+                        L.cut0 = par.nl == 0 ? r.k : par.cut0;
+                        L.path = sum / (float)(par.nl + 1);
+                        L.pruned = r.pruned;
+                        leaf_recs.push_back(L);
+                    }
+                    return 0;
+                };
+                // Round 1: per cutting part, the candidates of its items with the
+                // lowest bounds; round 2: the others, except those whose bound is
+                // above the best leaf cost found for their cutting part. Such a
+                // leaf can never be the part's argmin, so the decisions are the
+                // same as scoring every leaf.
+                std::vector<int> r1, r2;
+                if (leaf_bnb_on()) {
+                    constexpr int K1 = 3;
+                    std::vector<std::vector<int>> by_c(cutting.size());
+                    for (size_t i = 0; i < cur.size(); ++i) by_c[cur_c[i]].push_back((int)i);
+                    std::vector<char> first(cur.size(), 0);
+                    for (auto& v : by_c) {
+                        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return item_lb[a] < item_lb[b]; });
+                        for (int j = 0; j < (int)v.size() && j < K1; ++j) first[v[j]] = 1;
+                    }
+                    for (size_t q = 0; q < cands.size(); ++q) (first[cands[q].item] ? r1 : r2).push_back((int)q);
+                } else {
+                    for (size_t q = 0; q < cands.size(); ++q) r1.push_back((int)q);
                 }
-            }
-            LA_TRY(hull_volumes(ht));
-            to_free.insert(to_free.end(), pts_free.begin(), pts_free.end());
-            if (full)
-                for (size_t q = 0; q < ht.size(); ++q) {
-                    Item& ch = nxt[ht_who[q].first];
-                    ch.parts[ch.np - 2 + ht_who[q].second].hvol = ht[q].vol;
-                }
-            if (!full) {
-                // leaf costs: the worst part is the maximum rv cost over the
-                // inherited parts and both halves; path cost = mean level cost
-                for (size_t q = 0; q < ht.size(); ++q) {
-                    SRec& r = srec[ht_who[q].first];
-                    (ht_who[q].second ? r.nhv : r.phv) = ht[q].vol;
-                    if (ht[q].pruned) r.pruned = true;
-                }
-                std::vector<char> has_child(M.size(), 0);
-                for (const SRec& r : srec) {
-                    const Item& par = cur[r.item];
-                    has_child[par.mesh] = 1;
-                    float worst = std::max(cost_rv(r.phv, r.pvol), cost_rv(r.nhv, r.nvol));
-                    for (int i = 0; i < par.np - 1; ++i) worst = std::max(worst, cost_rv(par.parts[i].hvol, par.parts[i].mvol));
-                    float sum = 0.0f;
-                    for (int l = 0; l < par.nl; ++l) sum += par.lc[l];
-                    sum += worst;
-                    Leaf L;
-                    L.c = cur_c[r.item];
-                    L.cut0 = par.nl == 0 ? r.k : par.cut0;
-                    L.path = sum / (float)(par.nl + 1);
-                    L.pruned = r.pruned;
-                    leaf_recs.push_back(L);
+                LA_TRY(run_round(r1));
+                if (!r2.empty()) {
+                    std::vector<float> ubc(cutting.size(), 1e30f);
+                    for (const Leaf& L : leaf_recs)
+                        if (!L.pruned && L.path < ubc[L.c]) ubc[L.c] = L.path;
+                    std::vector<int> keep;
+                    keep.reserve(r2.size());
+                    for (int q : r2) {
+                        const int it = cands[q].item;
+                        if (mesh_has[cur[it].mesh] && item_lb[it] > ubc[cur_c[it]]) { ++bnb_skipped; continue; }
+                        keep.push_back(q);
+                    }
+                    bnb_scored += (long long)(r1.size() + keep.size());
+                    LA_TRY(run_round(keep));
+                } else {
+                    bnb_scored += (long long)r1.size();
                 }
                 for (size_t i = 0; i < cur.size(); ++i)
-                    if (!has_child[cur[i].mesh]) {
+                    if (!mesh_has[cur[i].mesh]) {
                         leaves.push_back(cur[i]);
                         leaves_c.push_back(cur_c[i]);
                     }
@@ -1149,6 +1247,13 @@ struct Engine {
                 lap(d == 0 ? S_LEVEL0 : S_LEVELN);
                 break;
             }
+            LA_TRY(hull_volumes(ht));
+            to_free.insert(to_free.end(), pts_free.begin(), pts_free.end());
+            if (full)
+                for (size_t q = 0; q < ht.size(); ++q) {
+                    Item& ch = nxt[ht_who[q].first];
+                    ch.parts[ch.np - 2 + ht_who[q].second].hvol = ht[q].vol;
+                }
             for (Item& ch : nxt) {
                 sort_item_parts(ch);
                 if (ch.nl < LA_MAX_LEVELS) {

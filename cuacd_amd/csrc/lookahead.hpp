@@ -16,8 +16,8 @@ namespace cuacd {
 constexpr int LA_MAX_COMP = 256;  // components kept per part (more merge into the last)
 
 // Hull of a point set: FP64 volume, optionally the hull mesh in a heap chunk.
-// Sets above LA_HULL_WAVE_MAX points go through the k-DOP prefilter first
-// (reference cuda/kdop_hull.cuh:49-398), which may also prune the set.
+// Sets above 256 points go through the k-DOP prefilter first (reference
+// cuda/kdop_hull.cuh:49-398), which also prunes sets above 1024 points.
 struct LaHullTask {
     const float* pts;    // xyz
     int n;
@@ -35,6 +35,7 @@ struct LaHullTask {
     float* hv;           // hull mesh chunk: verts, then tris at ht
     int* ht;
     int hnv, hnt;
+    int ncorner;         // corners of the canonical hull (emitted or not)
 };
 
 // Full plane cut of one part (level 0 of the search): both closed halves.

@@ -131,3 +131,24 @@ def test_merge_hulls_pass(ctx):
         rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(info[m]["hull_vol"] - info[m]["mesh_vol"], 0))
         if info[m]["iters"] < kw["max_iters"]:
             assert (rv < kw["threshold"]).all()
+
+
+def test_parity_with_serial_oracle_bench_parameters(ctx):
+    """The benchmark's search parameters (width/width2 30/5, 16 concave edges,
+    16 parts cut per iteration) on two SDF solids, three iterations: the GPU
+    search and the serial oracle make the same decisions."""
+    import lookahead_oracle as LO
+    kw = dict(max_iters=3, width=30, width2=5, threshold=0.05, depth=2, n_concave_edges=16, max_n_cutting=16)
+    for ci, (v, t) in enumerate(sdf.sdf_meshes(2, seed=5, res=20)):
+        v = _normalize(v)
+        hv, ht = ctx._input_hull(v)
+        gpu, info = ctx.batch_lookahead_decompose([(v, t)], return_info=True, **kw)
+        ref, iters = LO.lookahead(v, t, hv, ht, **kw)
+        assert info[0]["iters"] == iters, (ci, info[0]["iters"], iters)
+        key = lambda p: (len(p[1]), p[0].tobytes())
+        g = sorted(gpu[0], key=key)
+        r = sorted(ref, key=key)
+        assert len(g) == len(r), (ci, len(g), len(r))
+        for (gv, gt, ghv, ght), rp in zip(g, r):
+            assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
+            assert ghv.tobytes() == rp[2].tobytes() and np.array_equal(ght, rp[3]), ci
