@@ -117,12 +117,18 @@ __device__ inline void* heap_alloc(DPool* pool, int h, unsigned long long bytes,
     const int s0 = dh_stripe();
     char* blk = dh_pop_from(pool, hp, c, s0, dh_head(hp, c, s0));
     if (!blk) {
-        // look at every other stripe at once (independent loads), then try the non-empty ones
-        unsigned long long hv[DH_STRIPES];
+        // look at every other stripe at once (independent loads, a register
+        // bit mask - no private array), then try the non-empty ones in order
+        uint32_t live = 0;
 #pragma unroll
-        for (int k = 1; k < DH_STRIPES; ++k) hv[k] = dh_head(hp, c, (s0 + k) & (DH_STRIPES - 1));
-        for (int k = 1; !blk && k < DH_STRIPES; ++k)
-            if ((uint32_t)hv[k] != DH_EMPTY) blk = dh_pop_from(pool, hp, c, (s0 + k) & (DH_STRIPES - 1), hv[k]);
+        for (int k = 1; k < DH_STRIPES; ++k)
+            if ((uint32_t)dh_head(hp, c, (s0 + k) & (DH_STRIPES - 1)) != DH_EMPTY) live |= 1u << k;
+        while (!blk && live) {
+            const int k = __builtin_ctz(live);
+            live &= live - 1;
+            const int st = (s0 + k) & (DH_STRIPES - 1);
+            blk = dh_pop_from(pool, hp, c, st, dh_head(hp, c, st));
+        }
     }
     if (!blk) {
         unsigned long long sz = 256ull << c;

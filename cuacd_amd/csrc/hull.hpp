@@ -333,22 +333,23 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
     HullResult res{HULL_OK, 0, 0};
 
     // coordinates of local point ip, broadcast to the whole block
+    // Slot `slot` of a per-lane array, selected with masks: written as
+    // `if (k == slot) v = a[k]` the compiler turns the loop into an indexed
+    // access and moves the whole array to scratch memory.
+    auto pick = [&](const int (&a)[PPL], int slot) {
+        int v = 0;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) v |= a[k] & -(int)(k == slot);
+        return v;
+    };
     auto fetch = [&](int ip, int& x, int& y, int& z) {
         const int own = ip % T, slot = ip / T;
         if (NW == 1) {
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if (k == slot) {
-                    x = read_lane(px[k], own);
-                    y = read_lane(py[k], own);
-                    z = read_lane(pz[k], own);
-                }
+            x = read_lane(pick(px, slot), own);
+            y = read_lane(pick(py, slot), own);
+            z = read_lane(pick(pz, slot), own);
         } else {
-            if (tid == own) {
-#pragma unroll
-                for (int k = 0; k < PPL; ++k)
-                    if (k == slot) { sb->bc[0] = px[k]; sb->bc[1] = py[k]; sb->bc[2] = pz[k]; }
-            }
+            if (tid == own) { sb->bc[0] = pick(px, slot); sb->bc[1] = pick(py, slot); sb->bc[2] = pick(pz, slot); }
             __syncthreads();
             x = sb->bc[0]; y = sb->bc[1]; z = sb->bc[2];
             __syncthreads();
@@ -497,26 +498,23 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
         {
             const int own = ip % T, slot = ip / T;
             if (NW == 1) {
-#pragma unroll
-                for (int k = 0; k < PPL; ++k)
-                    if (k == slot) {
-                        qx = read_lane(px[k], own);
-                        qy = read_lane(py[k], own);
-                        qz = read_lane(pz[k], own);
-                        f0 = read_lane(conf[k], own);
-                        if (tid == own) conf[k] = -2;
-                    }
+                qx = read_lane(pick(px, slot), own);
+                qy = read_lane(pick(py, slot), own);
+                qz = read_lane(pick(pz, slot), own);
+                f0 = read_lane(pick(conf, slot), own);
             } else {
                 if (tid == own) {
-#pragma unroll
-                    for (int k = 0; k < PPL; ++k)
-                        if (k == slot) {
-                            sb->bc[0] = px[k]; sb->bc[1] = py[k]; sb->bc[2] = pz[k]; sb->bc[3] = conf[k];
-                            conf[k] = -2;
-                        }
+                    sb->bc[0] = pick(px, slot); sb->bc[1] = pick(py, slot); sb->bc[2] = pick(pz, slot);
+                    sb->bc[3] = pick(conf, slot);
                 }
                 __syncthreads();
                 qx = sb->bc[0]; qy = sb->bc[1]; qz = sb->bc[2]; f0 = sb->bc[3];
+            }
+            // the apex leaves the conflict sets
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                const int m = -(int)(tid == own && k == slot);
+                conf[k] = (conf[k] & ~m) | (-2 & m);
             }
         }
         if (tid == 0) {
