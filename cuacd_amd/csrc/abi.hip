@@ -39,7 +39,7 @@ void* cuacd::stage_buf(gpu_ctx_t ctx, int slot, size_t bytes) {
     return ctx->stage[slot];
 }
 
-hipEvent_t cuacd::kt_begin(gpu_ctx_t ctx) {
+hipEvent_t cuacd::kt_begin(gpu_ctx_t ctx, hipStream_t s) {
     KTimers* k = ctx->kt;
     if (!k) return nullptr;
     hipEvent_t e = nullptr;
@@ -49,11 +49,11 @@ hipEvent_t cuacd::kt_begin(gpu_ctx_t ctx) {
     } else if (hipEventCreate(&e) != hipSuccess) {
         return nullptr;
     }
-    (void)hipEventRecord(e, ctx->stream);
+    (void)hipEventRecord(e, s ? s : ctx->stream);
     return e;
 }
 
-void cuacd::kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes) {
+void cuacd::kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes, hipStream_t s) {
     KTimers* k = ctx->kt;
     if (!k || !a) return;
     hipEvent_t b = nullptr;
@@ -64,7 +64,7 @@ void cuacd::kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes) {
         k->free_ev.push_back(a);
         return;
     }
-    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, s ? s : ctx->stream);
     k->pending.push_back({kid, a, b, bytes});
 }
 
@@ -134,6 +134,12 @@ void gpu_destroy(gpu_ctx_t ctx) {
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
     if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    for (int i = 0; i < 3; ++i) {
+        if (ctx->aux[i]) (void)hipStreamSynchronize(ctx->aux[i]);
+        if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     free(ctx);
 }
@@ -165,6 +171,11 @@ int gpu_init(gpu_ctx_t* out, int device, size_t pool_bytes) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 3; ++i) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreate(&ctx->ev_a));
     HIP_TRY(hipEventCreate(&ctx->ev_b));
     size_t free_b = 0, total_b = 0;

@@ -47,6 +47,11 @@ struct gpu_ctx {
     size_t stage_cap[8];
     char last_error[512];
     cuacd::KTimers* kt;      // per-kernel timers, null when off
+    // side streams (default priority) for independent launches of one call:
+    // they fork from `stream` after ev_fork and join back through ev_join
+    hipStream_t aux[3];
+    hipEvent_t ev_fork;
+    hipEvent_t ev_join[3];
 };
 
 namespace cuacd {
@@ -61,10 +66,11 @@ void* stage_buf(gpu_ctx_t ctx, int slot, size_t bytes);
 int read_and_clear_err(gpu_ctx_t ctx);
 
 // Per-kernel timers: kt_begin records the start event (a = nullptr when off),
-// kt_end the stop event with the launch's algorithmic bytes; kt_resolve
+// kt_end the stop event with the launch's algorithmic bytes, both on stream s
+// (the context's stream when null); kt_resolve
 // (after the stream has passed them) accumulates the elapsed times.
-hipEvent_t kt_begin(gpu_ctx_t ctx);
-void kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes);
+hipEvent_t kt_begin(gpu_ctx_t ctx, hipStream_t s = nullptr);
+void kt_end(gpu_ctx_t ctx, int kid, hipEvent_t a, double bytes, hipStream_t s = nullptr);
 int kt_resolve(gpu_ctx_t ctx);
 
 }  // namespace cuacd

@@ -168,16 +168,21 @@ __global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int*
 // where the LDS footprint allows one hull per CU anyway, all four SIMDs work.
 constexpr int LA_LDS4_NW = 4;
 template <int PPL>
-__global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* tasks, const int* ids, int n, DPool* pool,
-                                                               int* err) {
+__global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* tasks, const int* ids, int n,
+                                                               const int* dcount, DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
-    if ((int)blockIdx.x >= n) return;
-    LaHullTask& t = tasks[ids[blockIdx.x]];
+    // ids[0, n), or the first min(*dcount, n) of them when dcount is given
+    // (a retry list filled by the previous launch); the grid strides over them
+    const int nn = dcount ? min(*dcount, n) : n;
     constexpr int CAP = 64 * LA_LDS4_NW * PPL;
     const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP), &sb);
-    la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err);
-    if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+    for (int j = blockIdx.x; j < nn; j += gridDim.x) {
+        __syncthreads();
+        LaHullTask& t = tasks[ids[j]];
+        la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err);
+        if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+    }
 }
 
 // Same, with the hull-vertex capacity reduced to vcap (a smaller LDS
@@ -197,17 +202,23 @@ __global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const
     if (done && threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
 
+// Workspace slot blockIdx.x of gws; ids[0, n), or the first min(*dcount, n)
+// when dcount is given, strided over the grid.
 template <int PPL>
-__global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* tasks, const int* ids, int n, char* gws,
+__global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* tasks, const int* ids, int n,
+                                                                  const int* dcount, char* gws,
                                                                   unsigned long long stride, int VC, int NP,
                                                                   DPool* pool, int* err) {
     __shared__ HullBlockLDS sb;
-    if ((int)blockIdx.x >= n) return;
-    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const int nn = dcount ? min(*dcount, n) : n;
     const HullWS<AS_GLOBAL> w =
         hull_ws_attach(hull_ws_carve<AS_GLOBAL>(gws + (unsigned long long)blockIdx.x * stride, VC, NP), &sb);
-    la_hull_item<PPL, AS_GLOBAL, LA_BLOCK_NW>(t, w, &sb, pool, err);
-    if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+    for (int j = blockIdx.x; j < nn; j += gridDim.x) {
+        __syncthreads();
+        LaHullTask& t = tasks[ids[j]];
+        la_hull_item<PPL, AS_GLOBAL, LA_BLOCK_NW>(t, w, &sb, pool, err);
+        if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+    }
 }
 
 // k-DOP prefilter (reference cuda/kdop_hull.cuh:116-337): extreme points along
@@ -1123,7 +1134,6 @@ int ensure_kdop_axes(gpu_ctx_t ctx) {
 }
 
 using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
-using BlockFn = void (*)(LaHullTask*, const int*, int, char*, unsigned long long, int, int, DPool*, int*);
 
 // Wave-tier sets of more than `wide_min` points run four waves per hull
 // (CUACD_HULL_WIDE_MIN overrides; 0 disables).
@@ -1157,17 +1167,77 @@ int hull_half_cap() {
     return v;
 }
 
-int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl, double bytes) {
+// One hull call: its size tiers are independent, so they run on separate
+// streams - the latency-bound large-set tiers (a few hulls of thousands of
+// points, each a long sequential insertion chain) beside the
+// throughput-bound small-set ones, instead of after them. Buffers are
+// allocated on the context's stream, which each side stream waits for before
+// its launches (fork), and freed there after the side streams have joined
+// back. Retry passes read their task counts on the device, so no launch waits
+// for the host.
+struct HullCall {
+    gpu_ctx_t ctx;
+    std::vector<void*> tmp;
+    bool used[3] = {false, false, false};
+    bool joined = false;
+    explicit HullCall(gpu_ctx_t c) : ctx(c) {}
+    ~HullCall() {
+        if (!joined) (void)join();
+    }
+    // stream k (-1: the context's stream); a side stream is first made to
+    // wait for everything issued on the context's stream so far
+    int fork(int k, hipStream_t* s) {
+        if (k < 0) {
+            *s = ctx->stream;
+            return 0;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
+        used[k] = true;
+        *s = ctx->aux[k];
+        return 0;
+    }
+    template <class T> T* alloc(size_t n) {
+        T* p = dev_tmp<T>(ctx, n);
+        if (p) tmp.push_back(p);
+        return p;
+    }
+    int join() {
+        joined = true;
+        for (int k = 0; k < 3; ++k)
+            if (used[k]) {
+                HIP_TRY(hipEventRecord(ctx->ev_join[k], ctx->aux[k]));
+                HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join[k], 0));
+                used[k] = false;
+            }
+        for (void* p : tmp) HIP_TRY(hipFreeAsync(p, ctx->stream));
+        tmp.clear();
+        return 0;
+    }
+};
+
+// grid of a retry pass (its count is only known on the device)
+int retry_grid(gpu_ctx_t ctx, size_t cap) {
+    return (int)std::min(cap, (size_t)std::max(1, ctx->n_cu));
+}
+
+int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl,
+                    double bytes) {
+    gpu_ctx_t ctx = hc.ctx;
     if (ids.empty()) return 0;
     const int kid = ppl == 1 ? KT_HULL_W1 : ppl == 2 ? KT_HULL_W2 : ppl == 4 ? KT_HULL_W4 : ppl == 8 ? KT_HULL_W8
                                                                                                      : KT_HULL_W16;
     const int wm = hull_wide_min();
+    int* d_ids = hc.alloc<int>(ids.size());
+    REQUIRE(d_ids, "lookahead: id list allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
     if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
-        // first pass with half the vertex capacity (two hulls per CU), then
-        // the sets whose hulls did not fit at full capacity
+        // first pass with a reduced vertex capacity (several hulls per CU),
+        // then the sets whose hulls did not fit, at full capacity
         using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
+        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
         CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>;
-        WaveFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
+        FullFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
         // the canonical hulls of these sets have few corners (mostly under a
         // quarter of the points), so 3/8 of the points fits nearly all
         const int vcap = (64 * ppl * hull_vcap_eighths()) / 8;
@@ -1175,46 +1245,37 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
         const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
-        int* d_ids = dev_tmp<int>(ctx, ids.size());
-        int* d_retry = dev_tmp<int>(ctx, ids.size() + 1);
-        REQUIRE(d_ids && d_retry, "lookahead: id list allocation failed");
-        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        int* d_retry = hc.alloc<int>(ids.size() + 1);
+        REQUIRE(d_retry, "lookahead: id list allocation failed");
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
-        hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, ctx->stream, d_tasks, d_ids,
+        hipStream_t s;
+        if (int rc = hc.fork(sk, &s)) return rc;
+        hipEvent_t t0 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
                            (int)ids.size(), vcap, d_retry, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, kid, t0, bytes);
-        int nre = 0;
-        HIP_TRY(hipMemcpyAsync(&nre, d_retry, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        REQUIRE(nre >= 0 && nre <= (int)ids.size(), "lookahead: hull retry count %d out of range", nre);
-        if (nre > 0) {
-            hipEvent_t t1 = kt_begin(ctx);
-            hipLaunchKernelGGL(kf, dim3((unsigned)nre), dim3(64 * LA_LDS4_NW), ldf, ctx->stream, d_tasks, d_retry + 1,
-                               nre, ctx->d_pool, ctx->d_err);
-            HIP_TRY(hipGetLastError());
-            kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, bytes * (double)nre / (double)ids.size());
-        }
-        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
-        HIP_TRY(hipFreeAsync(d_retry, ctx->stream));
+        kt_end(ctx, kid, t0, bytes, s);
+        hipEvent_t t1 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, ids.size())), dim3(64 * LA_LDS4_NW), ldf, s, d_tasks,
+                           d_retry + 1, (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, 0.0, s);
         return 0;
     }
     if (wm > 0 && ppl >= 4 && 64 * ppl > wm) {
         // same capacity (64 * ppl points and vertices), four waves
-        WaveFn k = ppl == 4 ? la_hull_lds4<1> : ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
+        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
+        FullFn k = ppl == 4 ? la_hull_lds4<1> : ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
         const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
         if (lds > 65536u)
             HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        int* d_ids = dev_tmp<int>(ctx, ids.size());
-        REQUIRE(d_ids, "lookahead: id list allocation failed");
-        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
-        hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), lds, ctx->stream, d_tasks, d_ids,
-                           (int)ids.size(), ctx->d_pool, ctx->d_err);
+        hipStream_t s;
+        if (int rc = hc.fork(sk, &s)) return rc;
+        hipEvent_t t0 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), lds, s, d_tasks, d_ids,
+                           (int)ids.size(), (const int*)nullptr, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, kid, t0, bytes);
-        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+        kt_end(ctx, kid, t0, bytes, s);
         return 0;
     }
     WaveFn k = ppl == 1 ? la_hull_wave<1> : ppl == 2 ? la_hull_wave<2> : ppl == 4 ? la_hull_wave<4>
@@ -1222,56 +1283,56 @@ int launch_wave_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& 
     const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
     if (lds > 65536u)
         HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int* d_ids = dev_tmp<int>(ctx, ids.size());
-    REQUIRE(d_ids, "lookahead: id list allocation failed");
-    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
-    hipEvent_t t0 = kt_begin(ctx);
-    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64), lds, ctx->stream, d_tasks, d_ids, (int)ids.size(),
+    hipStream_t s;
+    if (int rc = hc.fork(sk, &s)) return rc;
+    hipEvent_t t0 = kt_begin(ctx, s);
+    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64), lds, s, d_tasks, d_ids, (int)ids.size(),
                        ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
-    kt_end(ctx, kid, t0, bytes);
-    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+    kt_end(ctx, kid, t0, bytes, s);
     return 0;
 }
 
-int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>& ids_in, int maxn, double bytes) {
-    if (ids_in.empty()) return 0;
+using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, DPool*,
+                         int*);
+
+int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn,
+                     double bytes) {
+    gpu_ctx_t ctx = hc.ctx;
+    if (ids.empty()) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
-    std::vector<int> ids = ids_in;
-    int retry_kid = -1;
+    int* d_ids = hc.alloc<int>(ids.size());
+    REQUIRE(d_ids, "lookahead: id list allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+    const int n = (int)ids.size();
     if (maxn <= 4 * T && hull_half_cap()) {
         // first pass with the workspace in LDS (1024 hull vertices), then the
         // sets whose hulls did not fit, with the workspace in HBM
         using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
         CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW>;
-        const int tb = T;
-        const int kidc = KT_HULL_B4;
         const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
-        int* d_ids = dev_tmp<int>(ctx, ids.size());
-        int* d_retry = dev_tmp<int>(ctx, ids.size() + 1);
-        REQUIRE(d_ids && d_retry, "lookahead: id list allocation failed");
-        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        int* d_retry = hc.alloc<int>(ids.size() + 1);
+        REQUIRE(d_retry, "lookahead: id list allocation failed");
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
-        hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(tb), ldc, ctx->stream, d_tasks, d_ids,
-                           (int)ids.size(), 1024, d_retry, ctx->d_pool, ctx->d_err);
+        const int np = (maxn + 63) / 64 * 64;
+        const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(np, np) + 255ull) & ~255ull;
+        const int g = retry_grid(ctx, ids.size());
+        char* gws = hc.alloc<char>(stride * (size_t)g + 256);
+        REQUIRE(gws, "lookahead: hull workspace allocation failed");
+        hipStream_t s;
+        if (int rc = hc.fork(sk, &s)) return rc;
+        hipEvent_t t0 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(kc, dim3((unsigned)n), dim3(T), ldc, s, d_tasks, d_ids, n, 1024, d_retry, ctx->d_pool,
+                           ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, kidc, t0, bytes);
-        int nre = 0;
-        HIP_TRY(hipMemcpyAsync(&nre, d_retry, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        REQUIRE(nre >= 0 && nre <= (int)ids.size(), "lookahead: hull retry count %d out of range", nre);
-        const double per_task = bytes / (double)ids.size();
-        ids.resize(nre);
-        if (nre) HIP_TRY(hipMemcpyAsync(ids.data(), d_retry + 1, 4ull * nre, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
-        HIP_TRY(hipFreeAsync(d_retry, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (ids.empty()) return 0;
-        std::sort(ids.begin(), ids.end());
-        bytes = per_task * (double)nre;
-        retry_kid = KT_HULL_RB4;
+        kt_end(ctx, KT_HULL_B4, t0, bytes, s);
+        hipEvent_t t1 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(la_hull_block<4>, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_retry + 1, n,
+                           (const int*)d_retry, gws, stride, np, np, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, KT_HULL_RB4, t1, 0.0, s);
+        return 0;
     }
     BlockFn k;
     int kid;
@@ -1283,68 +1344,57 @@ int launch_block_ids(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<int>&
     const int vc = np < 32704 ? np : 32704;
     const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull;
     // bounded workspace: launch in slices of at most 2048 tasks
-    const size_t slice = 2048;
-    char* gws = nullptr;
-    const size_t nslice = std::min(slice, ids.size());
-    HIP_TRY(hipMallocAsync((void**)&gws, stride * nslice + 256, ctx->stream));
-    int* d_ids = dev_tmp<int>(ctx, ids.size());
-    REQUIRE(d_ids, "lookahead: id list allocation failed");
-    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
-    for (size_t b = 0; b < ids.size(); b += slice) {
-        const int cnt = (int)std::min(slice, ids.size() - b);
-        hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, ctx->stream, d_tasks, d_ids + b, cnt, gws, stride, vc, np,
-                           ctx->d_pool, ctx->d_err);
+    const int slice = 2048;
+    const int g = std::min(slice, n);
+    char* gws = hc.alloc<char>(stride * (size_t)g + 256);
+    REQUIRE(gws, "lookahead: hull workspace allocation failed");
+    hipStream_t s;
+    if (int rc = hc.fork(sk, &s)) return rc;
+    for (int b = 0; b < n; b += slice) {
+        const int cnt = std::min(slice, n - b);
+        hipEvent_t t0 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, s, d_tasks, d_ids + b, cnt, (const int*)nullptr, gws, stride,
+                           vc, np, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, retry_kid >= 0 ? retry_kid : kid, t0, bytes * (double)cnt / (double)ids.size());
+        kt_end(ctx, kid, t0, bytes * (double)cnt / (double)n, s);
     }
-    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
-    HIP_TRY(hipFreeAsync(gws, ctx->stream));
     return 0;
 }
 
 int ppl_class(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16; }
 
-int launch_by_size(gpu_ctx_t ctx, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn) {
+// Launch streams by tier: the large-set global-workspace tiers (longest
+// single-hull latency) first and on their own stream, then the 1025-2048
+// and 513-1024 LDS tiers on one stream each, the small-set tiers last.
+// `pre`: the sets of the call that skip the k-DOP prefilter, launched while
+// it runs on the context's stream.
+int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn, bool pre) {
     std::vector<int> cls[5];
     double cbytes[5] = {0, 0, 0, 0, 0};
-    std::vector<int> big;
-    int bigmax = 0;
+    std::vector<int> b4, b8, b16, b32;
+    int m4 = 0, m8 = 0, m16 = 0, m32 = 0;
+    double y4 = 0, y8 = 0, y16 = 0, y32 = 0;
     for (auto& q : idn) {
         const int n = q.second;
+        const double y = 12.0 * n + 8.0;  // points read, volume written
         if (n <= LA_WAVE_MAX) {
             const int p = ppl_class(n);
             const int c = p == 1 ? 0 : p == 2 ? 1 : p == 4 ? 2 : p == 8 ? 3 : 4;
             cls[c].push_back(q.first);
-            cbytes[c] += 12.0 * n + 8.0;  // points read, volume written
-        } else {
-            big.push_back(q.first);
-            bigmax = std::max(bigmax, n);
-        }
+            cbytes[c] += y;
+        } else if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); y4 += y; }
+        else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); y8 += y; }
+        else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); y16 += y; }
+        else { b32.push_back(q.first); m32 = std::max(m32, n); y32 += y; }
     }
+    if (int rc = launch_block_ids(hc, 0, d_tasks, b32, m32, y32)) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, b16, m16, y16)) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, b8, m8, y8)) return rc;
+    if (int rc = launch_block_ids(hc, 1, d_tasks, b4, m4, y4)) return rc;
     const int ppls[5] = {1, 2, 4, 8, 16};
-    for (int c = 0; c < 5; ++c)
-        if (int rc = launch_wave_ids(ctx, d_tasks, cls[c], ppls[c], cbytes[c])) return rc;
-    if (!big.empty()) {
-        // group the block tier by size class to bound the workspaces
-        std::vector<int> b4, b8, b16, b32;
-        int m4 = 0, m8 = 0, m16 = 0, m32 = 0;
-        double y4 = 0, y8 = 0, y16 = 0, y32 = 0;
-        std::vector<std::pair<int, int>> bn;
-        for (auto& q : idn) if (q.second > LA_WAVE_MAX) bn.push_back(q);
-        for (auto& q : bn) {
-            const int n = q.second;
-            const double y = 12.0 * n + 8.0;
-            if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); y4 += y; }
-            else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); y8 += y; }
-            else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); y16 += y; }
-            else { b32.push_back(q.first); m32 = std::max(m32, n); y32 += y; }
-        }
-        if (int rc = launch_block_ids(ctx, d_tasks, b4, m4, y4)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b8, m8, y8)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b16, m16, y16)) return rc;
-        if (int rc = launch_block_ids(ctx, d_tasks, b32, m32, y32)) return rc;
-    }
+    if (int rc = launch_wave_ids(hc, 2, d_tasks, cls[4], ppls[4], cbytes[4])) return rc;
+    for (int c = 3; c >= 0; --c)
+        if (int rc = launch_wave_ids(hc, pre ? 2 : -1, d_tasks, cls[c], ppls[c], cbytes[c])) return rc;
     return 0;
 }
 
@@ -1375,34 +1425,36 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
         if (h_tasks[i].n <= kmin) small.push_back({i, h_tasks[i].n});
         else big.push_back(i);
     }
-    if (int rc = launch_by_size(ctx, d_tasks, small)) return rc;
-    if (big.empty()) return 0;
-    REQUIRE(big.size() < (1u << 31), "too many hull tasks");
-    int* d_ids = dev_tmp<int>(ctx, big.size());
-    int2* d_res = dev_tmp<int2>(ctx, big.size());
-    REQUIRE(d_ids && d_res, "lookahead: k-DOP buffers allocation failed");
-    HIP_TRY(hipMemcpyAsync(d_ids, big.data(), 4 * big.size(), hipMemcpyHostToDevice, ctx->stream));
-    double kbytes = 0;
-    for (int i : big) kbytes += 12.0 * h_tasks[i].n;  // points read (the filtered copy is extra)
-    hipEvent_t t0 = kt_begin(ctx);
-    hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
-                       (int)big.size(), d_res, ctx->d_pool, ctx->d_err);
-    HIP_TRY(hipGetLastError());
-    kt_end(ctx, KT_KDOP, t0, kbytes);
-    std::vector<int2> res(big.size());
-    HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
-    HIP_TRY(hipFreeAsync(d_res, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    std::vector<std::pair<int, int>> fin;
-    for (size_t i = 0; i < big.size(); ++i) {
-        if (res[i].y) continue;          // pruned: volume is the rough bound
-        if (res[i].x < 0) continue;      // allocation failure, flagged in the task
-        REQUIRE(res[i].x <= HULL_NP_MAX, "lookahead: filtered hull set of %d points exceeds %d", res[i].x,
-                HULL_NP_MAX);
-        fin.push_back({big[i], res[i].x});
+    HullCall hc(ctx);
+    // the sets that skip the prefilter run beside it
+    if (int rc = launch_by_size(hc, d_tasks, small, true)) return rc;
+    if (!big.empty()) {
+        REQUIRE(big.size() < (1u << 31), "too many hull tasks");
+        int* d_ids = hc.alloc<int>(big.size());
+        int2* d_res = hc.alloc<int2>(big.size());
+        REQUIRE(d_ids && d_res, "lookahead: k-DOP buffers allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ids, big.data(), 4 * big.size(), hipMemcpyHostToDevice, ctx->stream));
+        double kbytes = 0;
+        for (int i : big) kbytes += 12.0 * h_tasks[i].n;  // points read (the filtered copy is extra)
+        hipEvent_t t0 = kt_begin(ctx);
+        hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
+                           (int)big.size(), d_res, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, KT_KDOP, t0, kbytes);
+        std::vector<int2> res(big.size());
+        HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<std::pair<int, int>> fin;
+        for (size_t i = 0; i < big.size(); ++i) {
+            if (res[i].y) continue;          // pruned: volume is the rough bound
+            if (res[i].x < 0) continue;      // allocation failure, flagged in the task
+            REQUIRE(res[i].x <= HULL_NP_MAX, "lookahead: filtered hull set of %d points exceeds %d", res[i].x,
+                    HULL_NP_MAX);
+            fin.push_back({big[i], res[i].x});
+        }
+        if (int rc = launch_by_size(hc, d_tasks, fin, false)) return rc;
     }
-    return launch_by_size(ctx, d_tasks, fin);
+    return hc.join();
 }
 
 int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n) {
