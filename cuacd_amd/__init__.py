@@ -362,6 +362,46 @@ class Context:
                                  vp.ctypes.data, vn.ctypes.data, sz.ctypes.data, st.ctypes.data)
         return vp, vn, sz, st
 
+    def batch_score_cuts(self, meshes, mesh_of, planes, threshold=0.05, return_detail=False):
+        """Exact score of many candidate cuts in one batch.
+
+        Cut i cuts meshes[mesh_of[i]] by planes[i] = (a, b, c, d); both closed
+        halves get their exact hulls, a half whose volume cost 0.3*rv is under
+        `threshold` also its Hausdorff distance H to its hull; a half's cost is
+        max(0.3*rv, H) (the stop cost of reference cuda/la_common.cuh:21-34,
+        measured lazily as in cuda/la_refine.cu:19-45) and the cut's score the
+        larger cost of its two halves (1e30 when the cut fails or leaves a side
+        empty). Returns float32 scores, plus with return_detail a float64
+        [n, 10] array (status, mesh volume pos/neg, hull volume pos/neg, rv
+        cost pos/neg, H pos/neg or -1, scored flag) and the stage statistics
+        (GPU ms of cuts, hulls, Hausdorff; bytes of cuts and hulls; hull tasks).
+        """
+        mesh_of = _i32(mesh_of).reshape(-1)
+        pl = _f32(planes).reshape(-1, 4)
+        n = len(mesh_of)
+        if len(pl) != n:
+            raise ValueError("mesh_of and planes must have the same length")
+        if n and (mesh_of.min() < 0 or mesh_of.max() >= len(meshes)):
+            raise ValueError("mesh_of index out of range")
+        vs = [_f32(v).reshape(-1, 3) for v, _ in meshes]
+        ts = [_i32(t).reshape(-1, 3) for _, t in meshes]
+        voff = np.zeros(len(meshes) + 1, np.int32)
+        toff = np.zeros(len(meshes) + 1, np.int32)
+        voff[1:] = np.cumsum([len(x) for x in vs])
+        toff[1:] = np.cumsum([len(x) for x in ts])
+        allv = np.ascontiguousarray(np.concatenate(vs)) if vs else np.zeros((0, 3), np.float32)
+        allt = np.ascontiguousarray(np.concatenate(ts)) if ts else np.zeros((0, 3), np.int32)
+        score = np.zeros(n, np.float32)
+        detail = np.zeros((n, 10), np.float64)
+        stats = np.zeros(6, np.float64)
+        if n:
+            _gpu.score_cuts(allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data, len(meshes),
+                            mesh_of.ctypes.data, pl.ctypes.data, n, float(threshold), score.ctypes.data,
+                            detail.ctypes.data, stats.ctypes.data)
+        if return_detail:
+            return score, detail, stats
+        return score
+
     # ---- lookahead decomposition ------------------------------------------------------
     def _input_hull(self, verts):
         (hv, ht, _), = self.batch_hull_dandc_mesh([verts])

@@ -190,6 +190,20 @@ static PyObject* m_plane_cut_batch(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* m_score_cuts(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, mo, pl, osc, odt, ost;
+    int nm, nc;
+    float tau;
+    if (!PyArg_ParseTuple(args, "KKKKiKKifKKK", &v, &vo, &t, &to, &nm, &mo, &pl, &nc, &tau, &osc, &odt, &ost))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_score_cuts(g_ctx, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to), nm,
+                            P(const int*, mo), P(const float*, pl), nc, tau, P(float*, osc), P(double*, odt),
+                            P(double*, ost));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
 static PyObject* part_list(struct gpu_result* res) {
     PyObject* list = PyList_New(res->nparts);
     if (!list) return NULL;
@@ -530,6 +544,7 @@ static PyMethodDef methods[] = {
     {"batch_mesh_volume", m_batch_mesh_volume, METH_VARARGS, "Volumes of a batch of meshes."},
     {"test_plane_cut", m_test_plane_cut, METH_VARARGS, "Plane cut with cap triangulation."},
     {"plane_cut_batch", m_plane_cut_batch, METH_VARARGS, "Batched plane cuts: volumes, sizes, status."},
+    {"score_cuts", m_score_cuts, METH_VARARGS, "Batched exact cut scores: cut, hulls, max(0.3 rv, H) per half."},
     {"kdop_hull", m_kdop_hull, METH_VARARGS, "k-DOP prefiltered exact hull."},
     {"test_hausdorff", m_test_hausdorff, METH_VARARGS, "Bidirectional sampled Hausdorff distance."},
     {"test_postprocess_dc", m_test_postprocess_dc, METH_VARARGS, "Split a mesh into connected components."},

@@ -167,3 +167,59 @@ def l_shape_union():
     occ[0, :, 0] = True
     occ[1, 0, 0] = True
     return voxel_solid(occ, (0.0, 0.0, 0.0), 1.0)
+
+
+def cut_candidates(verts, tris, n_edges=16, n_total=30, threshold=3.49, seed=0):
+    """Candidate planes (a, b, c, d) for one closed mesh, float32 [n_total, 4]:
+    up to n_edges planes through sampled concave edges - edges whose interior
+    dihedral angle exceeds `threshold` radians (pi + the angle between the two
+    face normals when the second face's far vertex lies above the first face;
+    reference rule cuda/la_concave.cu:213-233) - each the plane of one of the
+    edge's two faces (alternately) through the edge midpoint, then axis cuts
+    spaced evenly inside the bounding box (x, y, z in turn) up to n_total."""
+    v = np.asarray(verts, np.float64).reshape(-1, 3)
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    planes = []
+    if len(t):
+        p0, p1, p2 = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+        fn = np.cross(p1 - p0, p2 - p0)
+        ln = np.linalg.norm(fn, axis=1)
+        ok = ln > 0
+        fn[ok] /= ln[ok, None]
+        # directed edges a->b of face f; the twin b->a names the neighbour
+        a = t.reshape(-1)
+        b = t[:, [1, 2, 0]].reshape(-1)
+        f = np.repeat(np.arange(len(t)), 3)
+        third = t[:, [2, 0, 1]].reshape(-1)
+        key = a * (len(v) + 1) + b
+        twin = b * (len(v) + 1) + a
+        order = np.argsort(key)
+        pos = np.searchsorted(key[order], twin)
+        pos = np.minimum(pos, len(order) - 1)
+        has = key[order][pos] == twin
+        e = np.nonzero(has & (a < b))[0]
+        g = order[pos[e]]
+        f0, f1 = f[e], f[g]
+        n0, n1 = fn[f0], fn[f1]
+        up = np.einsum("ij,ij->i", v[third[g]] - v[a[e]], n0) > 0
+        ang = np.arccos(np.clip(np.einsum("ij,ij->i", n0, n1), -1.0, 1.0))
+        dih = np.where(up, np.pi + ang, np.pi - ang)
+        cand = np.nonzero((dih > threshold) & ok[f0] & ok[f1])[0]
+        if len(cand) > n_edges:
+            cand = rng.choice(cand, n_edges, replace=False)
+        for j, c in enumerate(cand):
+            k = e[c]
+            n = n0[c] if j % 2 == 0 else n1[c]
+            mid = 0.5 * (v[a[k]] + v[b[k]])
+            planes.append([n[0], n[1], n[2], -float(n @ mid)])
+    lo, hi = v.min(0), v.max(0)
+    n_axis = n_total - len(planes)
+    for k in range(n_axis):
+        ax = k % 3
+        slot, per = k // 3, (n_axis - ax + 2) // 3
+        pos = lo[ax] + (hi[ax] - lo[ax]) * (slot + 1) / (per + 1)
+        pl = [0.0, 0.0, 0.0, -pos]
+        pl[ax] = 1.0
+        planes.append(pl)
+    return np.asarray(planes, np.float32).reshape(-1, 4)

@@ -204,6 +204,23 @@ int gpu_plane_cut_batch(gpu_ctx_t ctx, const float* verts, const int* vert_off, 
                         const int* tri_off, int n_meshes, const int* mesh_of, const float* planes, int n_cuts,
                         double* out_vol_pos, double* out_vol_neg, int* out_sizes, int* out_status);
 
+/* Batched exact cut scoring (no reference entry point: the per-candidate work
+ * of the lookahead's level 0, cuda/la_expand.cu:65-424, with the stop cost of
+ * cuda/la_common.cuh:21-34). Cut i cuts mesh mesh_of[i] (packed as in
+ * gpu_plane_cut_batch) by planes[4i..4i+3]; both closed halves get their
+ * exact hulls; a half whose volume cost 0.3*rv is under tau also gets its
+ * Hausdorff distance H to its hull (reference sampler, both directions,
+ * cuda/la_refine.cu:19-45); its cost is max(0.3*rv, H). out_score[i] = the
+ * larger cost of the two halves, 1e30 when the cut fails or leaves a side
+ * empty. out_detail (optional, 10 doubles per cut): status, mesh volume
+ * pos/neg (clamped to the hull volume), hull volume pos/neg, rv cost pos/neg,
+ * H pos/neg (-1 when not measured), 1 if scored. out_stats (optional, 6
+ * doubles): GPU ms of the cut, hull and Hausdorff stages, algorithmic bytes of
+ * the cut and hull stages, hull tasks. */
+int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris, const int* tri_off,
+                   int n_meshes, const int* mesh_of, const float* planes, int n_cuts, float tau, float* out_score,
+                   double* out_detail, double* out_stats);
+
 /* Plans: the same computation on device-resident meshes, prepared once and
  * launched many times (stream-ordered). Outputs are SQUARED distances. */
 typedef struct hd_plan* hd_plan_t;

@@ -268,6 +268,32 @@ def concave_planes(p, n_concave, eps, thr):
     return [np.array(pl, np.float32) for pl in planes]
 
 
+def cut_cost(v, t, plane, tau):
+    """Exact cost of one candidate cut, serially: the full plane cut, the exact
+    hull of each closed half, cost_rv of each half (mesh volume clamped to the
+    hull volume, cuda/la_lifecycle.cu:292-296), its Hausdorff distance to its
+    hull when cost_rv < tau (cuda/la_refine.cu:19-45), the half's cost
+    max(cost_rv, H) (cuda/la_common.cuh:21-34) and the cut's score as the
+    larger of the two. Returns (score, [(mvol, hvol, crv, H or -1)] * 2);
+    score 1e30 when the cut fails or leaves a side empty."""
+    st, pv, pt, nv, nt = O.plane_cut(v, t, plane)
+    if st != 0 or len(pv) == 0 or len(nv) == 0:
+        return np.float32(1e30), None
+    halves = []
+    for hv_, ht_ in ((pv, pt), (nv, nt)):
+        mv = mesh_volume_abs(hv_, ht_)
+        hmv, hmt, hvol = hull_mesh(hv_)
+        if mv > hvol:
+            mv = hvol
+        crv = cost_rv(hvol, mv)
+        h = np.float32(-1.0)
+        if crv < np.float32(tau) and len(hmt):
+            h = hausdorff(hmv, hmt, hv_, ht_)
+        halves.append((mv, hvol, crv, h))
+    cost = [max(c, h if h > 0 else np.float32(0.0)) for _, _, c, h in halves]
+    return np.float32(max(cost)), halves
+
+
 def score_cut(v, t, plane):
     import ctypes as C
     L = O.lib()

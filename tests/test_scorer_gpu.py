@@ -1,0 +1,83 @@
+"""GPU tests of the batched exact cut scorer (Context.batch_score_cuts).
+
+The scorer is the lookahead's level-0 work per (part, plane) pair - full cut
+(reference cuda/plane_cut.cuh), exact hulls of both halves, the stop cost
+max(0.3 rv, H) of each half with H measured when 0.3 rv < tau
+(cuda/la_common.cuh:21-34, cuda/la_refine.cu:19-45). The serial oracle
+(lookahead_oracle.cut_cost) composes the C oracle's cut, exact hull and the
+reference-sampler Hausdorff distance with exact FP64 distances.
+"""
+import numpy as np
+import pytest
+
+from cuacd_amd import datasets, sdf
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    meshes = sdf.sdf_meshes(3, seed=31, res=18)
+    out = []
+    for m, (v, t) in enumerate(meshes):
+        pl = datasets.cut_candidates(v, t, n_edges=6, n_total=12, seed=m)
+        out.append((v, t, pl))
+    return out
+
+
+def _check(ctx, tau):
+    import lookahead_oracle as LO
+    cases = _cases()
+    meshes = [(v, t) for v, t, _ in cases]
+    mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, (_, _, p) in enumerate(cases)])
+    planes = np.concatenate([p for _, _, p in cases])
+    score, detail, stats = ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=tau, return_detail=True)
+    assert stats[5] > 0
+    n_h = 0
+    for i in range(len(planes)):
+        v, t = meshes[mesh_of[i]]
+        ref, halves = LO.cut_cost(v, t, planes[i], tau)
+        if halves is None:
+            assert score[i] == np.float32(1e30) and detail[i, 9] == 0, i
+            continue
+        assert detail[i, 9] == 1, i
+        for s, (mv, hv, crv, h) in enumerate(halves):
+            assert abs(detail[i, 1 + s] - mv) <= 1e-9 * max(abs(mv), 1e-12), (i, s)
+            assert abs(detail[i, 3 + s] - hv) <= 1e-6 * max(hv, 1e-12), (i, s)
+            assert abs(detail[i, 5 + s] - crv) <= 1e-5 * max(crv, 1e-6), (i, s)
+            if h >= 0:
+                n_h += 1
+                # Hausdorff to 1e-5 absolute (the reference's test tolerance)
+                assert abs(detail[i, 7 + s] - h) <= 1e-5, (i, s, detail[i, 7 + s], h)
+            else:
+                assert detail[i, 7 + s] == -1.0, (i, s)
+        assert abs(score[i] - ref) <= 1e-5 + 1e-5 * ref, (i, score[i], ref)
+    return n_h
+
+
+def test_scores_match_oracle(ctx):
+    _check(ctx, 0.05)
+
+
+def test_scores_match_oracle_all_hausdorff(ctx):
+    # a threshold above every volume cost measures H on every half
+    assert _check(ctx, 10.0) > 0
+
+
+def test_missed_plane_and_empty_batch(ctx):
+    v, t = datasets.cube_mesh()
+    v = np.asarray(v, np.float32)
+    planes = np.array([[1, 0, 0, -5.0], [1, 0, 0, -0.5]], np.float32)
+    score, detail, _ = ctx.batch_score_cuts([(v, t)], [0, 0], planes, threshold=0.05, return_detail=True)
+    assert score[0] == np.float32(1e30) and detail[0, 0] == 2
+    # two boxes: both halves convex, cost 0
+    assert detail[1, 9] == 1 and score[1] <= 1e-6
+    assert len(ctx.batch_score_cuts([(v, t)], [], np.zeros((0, 4), np.float32))) == 0
+
+
+def test_heap_balanced_after_scoring(ctx):
+    cases = _cases()[:1]
+    v, t, pl = cases[0]
+    before = ctx.heap_stats()
+    ctx.batch_score_cuts([(v, t)], np.zeros(len(pl), np.int32), pl, threshold=0.05)
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)
