@@ -9,6 +9,14 @@ over all 64 meshes (the meshes are independent; every launch spans all of
 them). Metric: meshes per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--meshes N]
+
+`--workload score` measures the batched exact cut scorer instead (candidate
+cuts scored per second): 256 SDF meshes (marching tetrahedra on a 64^3 grid,
+500-60k triangles, seed 3), 30 candidate planes each (up to 16 through
+sampled concave edges, the rest axis cuts), tau 0.05; a step scores all 7680
+cuts (cut, hulls of both halves, Hausdorff where the volume cost is under
+tau). `--autotune` tunes the hull launch shapes on the workload first
+(untimed).
 """
 import argparse
 import json
@@ -89,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--meshes", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["lookahead", "score"], default="lookahead")
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: one GPU only (asked for {args.gpus}); the meshes of the batch are independent and "
@@ -97,6 +107,8 @@ def main():
 
     import cuacd_amd
 
+    if args.workload == "score":
+        return main_score(args)
     t0 = time.perf_counter()
     meshes = workload(args.meshes)
     tris = np.array([len(t) for _, t in meshes])
@@ -140,8 +152,8 @@ def main():
     for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel"),
                      ("hausdorff", "hd_task_kernel")):
         kern[name] = stats[ph]
-    wide_min = int(os.environ.get("CUACD_HULL_WIDE_MIN", "256"))
-    half_cap = int(os.environ.get("CUACD_HULL_HALFCAP", "1")) != 0
+    wide_min = ctx.get_tuning("hull_wide_min")
+    half_cap = ctx.get_tuning("hull_half_cap") != 0
     for cls, k in ctx.last_lookahead_kernels.items():
         kern[kernel_name(cls, wide_min, half_cap)] = k
     for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
@@ -175,6 +187,109 @@ def main():
                                "merge_hulls on, n_concave_edges 16, max_n_cutting 16",
                    "meshes": len(meshes), "tris_total": int(tris.sum()), "parts_total": int(n_parts.sum()),
                    "max_part_concavity": max_conc, "iterations_max": int(iters.max())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],
+                     "avg_launch_ms": d["ms"] / max(d["launches"], 1),
+                     "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def score_workload(n):
+    from cuacd_amd import datasets, sdf
+    meshes = sdf.sdf_meshes(n, seed=3, res=64, min_tris=500, max_tris=60000)
+    planes = [datasets.cut_candidates(v, t, n_edges=16, n_total=30, seed=m) for m, (v, t) in enumerate(meshes)]
+    return meshes, planes
+
+
+def cpu_score_baseline(meshes, planes, tau, budget_s=20.0):
+    """The serial oracle scorer (one core) on the cuts of the smallest mesh,
+    as many as fit in the time budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lookahead_oracle as LO
+    m = int(np.argmin([len(t) for _, t in meshes]))
+    v, t = meshes[m]
+    done = 0
+    t0 = time.perf_counter()
+    for pl in planes[m]:
+        LO.cut_cost(v, t, pl, tau)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "cuts/s", "cores": 1, "kind": "port",
+            "sample": f"{done} cuts of the smallest mesh ({len(t)} tris) by the serial oracle scorer "
+                      f"(C cut, exact hull, reference-sampler Hausdorff) in {dt:.1f} s"}
+
+
+def main_score(args):
+    import cuacd_amd
+    from cuacd_amd import autotune
+    tau = 0.05
+    n = 256 if args.meshes == 64 else args.meshes
+    t0 = time.perf_counter()
+    meshes, planes = score_workload(n)
+    tris = np.array([len(t) for _, t in meshes])
+    mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, p in enumerate(planes)])
+    pl = np.concatenate(planes)
+    log(f"score workload: {len(meshes)} meshes, {tris.min()}-{tris.max()} tris (mean {tris.mean():.0f}), "
+        f"{len(pl)} cuts, generated in {time.perf_counter() - t0:.1f} s")
+    ctx = cuacd_amd.Context(device=0)
+    name, n_cu, hbm = ctx.device_info()
+    log(f"device {name}, {n_cu} CUs, {hbm / 2**30:.0f} GiB")
+    ctx.batch_score_cuts(meshes[:1], mesh_of[:30], pl[:30], threshold=tau)  # first touch
+    if args.autotune:
+        best, trials = autotune.tune(ctx, lambda: ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau), log=log)
+        log(f"autotune: {best} ({len(trials)} trials)")
+    for _ in range(args.warmup):
+        ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        score, detail, st = ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau, return_detail=True)
+        times.append(time.perf_counter() - t0)
+    ms_per_step = 1e3 * float(np.mean(times))
+    scored = int(detail[:, 9].sum())
+    n_h = int((detail[:, 7:9] >= 0).sum())
+    log(f"step {ms_per_step:.1f} ms; {scored} of {len(pl)} cuts scored, {n_h} halves with Hausdorff; "
+        f"stages: cuts {st[0]:.1f} ms, hulls {st[1]:.1f} ms, hausdorff {st[2]:.1f} ms")
+    kern = {"la_cut_kernel": {"ms": st[0], "launches": 1, "alg_bytes": st[3]}}
+    wide_min = ctx.get_tuning("hull_wide_min")
+    half_cap = ctx.get_tuning("hull_half_cap") != 0
+    for i, cls in enumerate(cuacd_amd.LOOKAHEAD_KERNELS):
+        ms, la, by = st[6 + 3 * i: 9 + 3 * i]
+        kern[kernel_name(cls, wide_min, half_cap)] = {"ms": ms, "launches": int(la), "alg_bytes": by}
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
+        log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
+            f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_score_baseline(meshes, planes, tau)
+        log(f"cpu baseline {cpu['value']:.3f} cuts/s ({cpu['sample']})")
+    line = {
+        "metric": "candidate cuts scored/sec (plain, batched exact scorer)",
+        "value": len(pl) / (ms_per_step * 1e-3),
+        "unit": "cuts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid",
+        "data": "synthetic",
+        "config": {"workload": f"batched cut scoring: {len(meshes)} SDF meshes (marching tetrahedra 64^3, "
+                               f"500-60k tris, extent 2, seed 3), 30 candidate planes each (16 concave-edge "
+                               f"planes + axis cuts), tau 0.05", "cuts": int(len(pl)),
+                   "tris_total": int(tris.sum()), "autotuned": bool(args.autotune),
+                   "tuning": ctx.get_tuning()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],

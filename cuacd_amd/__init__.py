@@ -40,6 +40,10 @@ LOOKAHEAD_KERNELS = ("hull_w64", "hull_w128", "hull_w256", "hull_w512", "hull_w1
                      "hull_b8192", "hull_b16384", "kdop_prefilter", "hull_w512_retry", "hull_w1024_retry",
                      "hull_b2048_retry")
 
+# launch-shape knobs of the hull tiers (gpu_set_tuning keys)
+TUNING_KEYS = {"hull_wide_min": 0, "hull_vcap_eighths": 1, "hull_half_cap": 2, "kdop_min_points": 3,
+               "hull_chunk_above": 4}
+
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
 
@@ -374,7 +378,8 @@ class Context:
         empty). Returns float32 scores, plus with return_detail a float64
         [n, 10] array (status, mesh volume pos/neg, hull volume pos/neg, rv
         cost pos/neg, H pos/neg or -1, scored flag) and the stage statistics
-        (GPU ms of cuts, hulls, Hausdorff; bytes of cuts and hulls; hull tasks).
+        (GPU ms of cuts, hulls, Hausdorff; bytes of cuts and hulls; hull tasks;
+        then ms, launches, bytes per hull kernel class of LOOKAHEAD_KERNELS).
         """
         mesh_of = _i32(mesh_of).reshape(-1)
         pl = _f32(planes).reshape(-1, 4)
@@ -393,7 +398,7 @@ class Context:
         allt = np.ascontiguousarray(np.concatenate(ts)) if ts else np.zeros((0, 3), np.int32)
         score = np.zeros(n, np.float32)
         detail = np.zeros((n, 10), np.float64)
-        stats = np.zeros(6, np.float64)
+        stats = np.zeros(6 + 3 * len(LOOKAHEAD_KERNELS), np.float64)
         if n:
             _gpu.score_cuts(allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data, len(meshes),
                             mesh_of.ctypes.data, pl.ctypes.data, n, float(threshold), score.ctypes.data,
@@ -401,6 +406,17 @@ class Context:
         if return_detail:
             return score, detail, stats
         return score
+
+    # ---- launch-shape knobs ---------------------------------------------------------
+    def set_tuning(self, name, value):
+        """Set a hull launch-shape knob (TUNING_KEYS); results never depend on it."""
+        _gpu.set_tuning(TUNING_KEYS[name], int(value))
+
+    def get_tuning(self, name=None):
+        """One knob's value, or all of them as a dict."""
+        if name is None:
+            return {k: _gpu.get_tuning(v) for k, v in TUNING_KEYS.items()}
+        return _gpu.get_tuning(TUNING_KEYS[name])
 
     # ---- lookahead decomposition ------------------------------------------------------
     def _input_hull(self, verts):

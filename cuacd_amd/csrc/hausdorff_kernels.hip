@@ -15,10 +15,14 @@
 
 using namespace cuacd;
 
+// A plan's launches (preparation + query) are captured once as a HIP graph;
+// each run replays it. The kernels clear their own outputs (no memset nodes).
 struct hd_plan {
     HdTask* d_tasks;
     char* d_ws;
     int n_tasks;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
 };
 
 namespace {
@@ -306,10 +310,10 @@ int gpu_test_hausdorff(gpu_ctx_t ctx, const float* hv, int hnv, const int* ht, i
 
 // Plan: device task table + workspaces for n_pairs pairs (A_i, B_i) whose data
 // already lives on the device. Two tasks per pair: A samples -> B, B samples -> A.
-int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_av, const int* d_at, const float* d_bv,
-                              const int* d_bt, const int* h_avoff, const int* h_atoff, const int* h_bvoff,
-                              const int* h_btoff, int n_pairs, int mode, int n_samples, float* d_out_ab,
-                              float* d_out_ba, float* d_samples_a, float* d_samples_b, hd_plan_t* out_plan) {
+static int plan_create(gpu_ctx_t ctx, const float* d_av, const int* d_at, const float* d_bv, const int* d_bt,
+                       const int* h_avoff, const int* h_atoff, const int* h_bvoff, const int* h_btoff, int n_pairs,
+                       int mode, int n_samples, float* d_out_ab, float* d_out_ba, float* d_samples_a,
+                       float* d_samples_b, bool capture, hd_plan_t* out_plan) {
     *out_plan = nullptr;
     HIP_TRY(hipSetDevice(ctx->device));
     REQUIRE(n_pairs >= 0 && (mode == 0 || mode == 1), "invalid hausdorff plan arguments");
@@ -344,19 +348,49 @@ int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_av, const int* d_at,
     }
     HIP_TRY(hipMemcpyAsync(p->d_tasks, t.data(), sizeof(HdTask) * t.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (capture && p->n_tasks > 0) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_tasks(ctx, p->d_tasks, p->n_tasks, p->d_ws);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        hipError_t ei = hipErrorUnknown;
+        if (rc == 0 && e == hipSuccess) ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        if (rc != 0 || e != hipSuccess || ei != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipFree(p->d_tasks);
+            (void)hipFree(p->d_ws);
+            free(p);
+            set_error(ctx, "hausdorff plan graph capture failed (%s)",
+                      hipGetErrorString(e != hipSuccess ? e : ei));
+            return rc ? rc : -1;
+        }
+        p->graph = g;
+    }
     *out_plan = p;
     return 0;
+}
+
+int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_av, const int* d_at, const float* d_bv,
+                              const int* d_bt, const int* h_avoff, const int* h_atoff, const int* h_bvoff,
+                              const int* h_btoff, int n_pairs, int mode, int n_samples, float* d_out_ab,
+                              float* d_out_ba, float* d_samples_a, float* d_samples_b, hd_plan_t* out_plan) {
+    return plan_create(ctx, d_av, d_at, d_bv, d_bt, h_avoff, h_atoff, h_bvoff, h_btoff, n_pairs, mode, n_samples,
+                       d_out_ab, d_out_ba, d_samples_a, d_samples_b, true, out_plan);
 }
 
 int gpu_hausdorff_plan_run(gpu_ctx_t ctx, hd_plan_t plan) {
     HIP_TRY(hipSetDevice(ctx->device));
     REQUIRE(plan, "null plan");
-    return launch_tasks(ctx, plan->d_tasks, plan->n_tasks, plan->d_ws);
+    if (plan->n_tasks == 0) return 0;
+    HIP_TRY(hipGraphLaunch(plan->exec, ctx->stream));
+    return 0;
 }
 
 void gpu_hausdorff_plan_destroy(gpu_ctx_t ctx, hd_plan_t plan) {
     if (!plan) return;
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
     (void)hipFree(plan->d_tasks);
     (void)hipFree(plan->d_ws);
     free(plan);
@@ -388,10 +422,11 @@ int gpu_hausdorff_batch(gpu_ctx_t ctx, const float* av, const int* avoff, const 
     HIP_TRY(hipMemcpyAsync(d_bv, bv, 12 * nbv, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(d_bt, bt, 12 * nbt, hipMemcpyHostToDevice, ctx->stream));
     hd_plan_t plan = nullptr;
-    int rc = gpu_hausdorff_plan_create(ctx, d_av, d_at, d_bv, d_bt, avoff, atoff, bvoff, btoff, n_pairs, mode,
-                                       n_samples, d_ab, d_ba, d_sa, d_sb, &plan);
+    // one run: launched directly, no graph
+    int rc = plan_create(ctx, d_av, d_at, d_bv, d_bt, avoff, atoff, bvoff, btoff, n_pairs, mode, n_samples, d_ab,
+                         d_ba, d_sa, d_sb, false, &plan);
     if (rc) return rc;
-    rc = gpu_hausdorff_plan_run(ctx, plan);
+    rc = launch_tasks(ctx, plan->d_tasks, plan->n_tasks, plan->d_ws);
     if (rc == 0) {
         HIP_TRY(hipMemcpyAsync(out_ab, d_ab, 4ull * n_pairs, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(out_ba, d_ba, 4ull * n_pairs, hipMemcpyDeviceToHost, ctx->stream));

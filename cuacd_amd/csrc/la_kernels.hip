@@ -420,6 +420,56 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
     }
 }
 
+// Sets above one workgroup's reach (HULL_NP_MAX points after the prefilter)
+// are hulled in chunks first: the hull of a set is the hull of the union of
+// its chunks' hull corners. Chunks are contiguous runs of the set, each
+// chunk's canonical corners come out in ascending point order, so the union
+// is an order-preserving subsequence of the set under the set's own
+// quantization frame - its canonical hull is the set's. One workgroup per
+// oversized set: the chunk hulls' vertex arrays (segs[seg_off[i]..seg_off[i+1]),
+// in chunk order) are concatenated into a new scratch chunk that replaces the
+// task's filtered set; the chunk hulls are freed. A chunk without a hull
+// (fewer than four points, or all coplanar - e.g. a run of a cut's cap
+// vertices) passes all its points through (own = 0: not a heap chunk).
+struct LaUnionSeg {
+    const float* v;
+    int n;
+    int own;
+};
+__global__ __launch_bounds__(256) void la_union_kernel(LaHullTask* tasks, const int* ids, const int* seg_off,
+                                                       const LaUnionSeg* segs, int n, DPool* pool, int* err) {
+    __shared__ unsigned long long sp;
+    if ((int)blockIdx.x >= n) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const int s0 = seg_off[blockIdx.x], s1 = seg_off[blockIdx.x + 1];
+    int total = 0;
+    for (int q = s0; q < s1; ++q) total += segs[q].n;
+    if (threadIdx.x == 0) sp = (unsigned long long)heap_alloc(pool, 1, 12ull * total + 16, err);
+    __syncthreads();
+    float* dst = (float*)sp;
+    if (dst) {
+        int o = 0;
+        for (int q = s0; q < s1; ++q) {
+            const LaUnionSeg g = segs[q];
+            for (int i = threadIdx.x; i < 3 * g.n; i += 256) dst[3 * o + i] = g.v[i];
+            o += g.n;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = s0; q < s1; ++q)
+            if (segs[q].own) heap_free(pool, (void*)segs[q].v, err);
+        if (!dst) {
+            t.err = KERR_KDOP_SCRATCH_OOM;
+            err_or(err, KERR_KDOP_SCRATCH_OOM);
+            return;
+        }
+        if (t.filt) heap_free(pool, t.filt, err);
+        t.filt = dst;
+        t.nf = total;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Level-0 cut: both closed halves (kept in heap 0), volumes and boxes
 // ---------------------------------------------------------------------------
@@ -1135,37 +1185,49 @@ int ensure_kdop_axes(gpu_ctx_t ctx) {
 
 using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
 
-// Wave-tier sets of more than `wide_min` points run four waves per hull
-// (CUACD_HULL_WIDE_MIN overrides; 0 disables).
-int hull_wide_min() {
-    static int v = -2;
-    if (v == -2) {
-        const char* e = getenv("CUACD_HULL_WIDE_MIN");
-        v = e ? atoi(e) : 256;
+// Launch-shape knobs of the hull tiers (they choose kernels and capacities,
+// never results: every tier computes the same canonical hull). Process-wide;
+// initial values from the environment, changed at run time through
+// gpu_set_tuning (the autotuner).
+//   0 wide_min: wave-tier sets above this many points run four waves per hull
+//     (CUACD_HULL_WIDE_MIN, 0 disables; default 256)
+//   1 vcap8: first-pass vertex capacity of the four-wave tiers, in eighths of
+//     their point capacity (CUACD_HULL_VCAP8, 1..8; default 3)
+//   2 half_cap: reduced-capacity first passes on/off (CUACD_HULL_HALFCAP; default 1)
+//   3 kdop_min: sets above this many points are k-DOP prefiltered
+//     (CUACD_KDOP_MIN, >= 4; default 256)
+//   4 chunk_above: prefiltered sets above this many points are hulled in
+//     chunks of half that size first (CUACD_HULL_CHUNK_ABOVE, 1024..16384;
+//     default 16384, the most one workgroup hulls)
+constexpr int LA_TUNE_N = 5;
+int g_tune[LA_TUNE_N];
+bool g_tune_init = false;
+
+int tune_clamp(int key, int v) {
+    switch (key) {
+        case 0: return std::max(0, v);
+        case 1: return std::min(8, std::max(1, v));
+        case 2: return v != 0;
+        case 3: return std::max(4, v);
+        default: return std::min(HULL_NP_MAX, std::max(1024, v));
     }
-    return v;
 }
 
-// First-pass vertex capacity of the four-wave tiers in eighths of their point
-// capacity (CUACD_HULL_VCAP8, 1..8).
-int hull_vcap_eighths() {
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("CUACD_HULL_VCAP8");
-        v = e ? std::min(8, std::max(1, atoi(e))) : 3;
+void tune_init() {
+    if (g_tune_init) return;
+    const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN", "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP", "CUACD_KDOP_MIN",
+                                  "CUACD_HULL_CHUNK_ABOVE"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX};
+    for (int k = 0; k < LA_TUNE_N; ++k) {
+        const char* e = getenv(env[k]);
+        g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
     }
-    return v;
+    g_tune_init = true;
 }
 
-// Reduced-capacity first pass for the four-wave tiers (CUACD_HULL_HALFCAP=0 disables).
-int hull_half_cap() {
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("CUACD_HULL_HALFCAP");
-        v = e ? (atoi(e) != 0) : 1;
-    }
-    return v;
-}
+int hull_wide_min() { tune_init(); return g_tune[0]; }
+int hull_vcap_eighths() { tune_init(); return g_tune[1]; }
+int hull_half_cap() { tune_init(); return g_tune[2]; }
 
 // One hull call: its size tiers are independent, so they run on separate
 // streams - the latency-bound large-set tiers (a few hulls of thousands of
@@ -1404,15 +1466,105 @@ namespace cuacd {
 
 // Hulls of n tasks. Host copy h_tasks gives the point counts; sets above the
 // wave tier are prefiltered (k-DOP) first, which needs one readback.
-// Sets above this many points are k-DOP prefiltered before their exact hull
-// (CUACD_KDOP_MIN overrides); bound pruning still applies only above 1024.
 static int kdop_min_points() {
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("CUACD_KDOP_MIN");
-        v = e ? std::max(4, atoi(e)) : 256;
+    tune_init();
+    return g_tune[3];
+}
+
+// Oversized sets (more than chunk_above points after the prefilter, `over`
+// as (task, points)): hulls of chunks of chunk_above / 2 points, then the
+// union of their corners replaces the set (la_union_kernel), repeated while a
+// union is still too large; the shrunken sets join `fin`.
+static int hull_oversized(gpu_ctx_t ctx, LaHullTask* d_tasks, std::vector<std::pair<int, int>> over,
+                          std::vector<std::pair<int, int>>& fin) {
+    tune_init();
+    const int limit = g_tune[4], chunk = limit / 2;
+    while (!over.empty()) {
+        // the sets' current point arrays and frames (written by the prefilter)
+        std::vector<LaHullTask> par(over.size());
+        for (size_t i = 0; i < over.size(); ++i)
+            HIP_TRY(hipMemcpyAsync(&par[i], d_tasks + over[i].first, sizeof(LaHullTask), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<LaHullTask> sub;
+        std::vector<int> sub_of;
+        for (size_t i = 0; i < over.size(); ++i) {
+            const LaHullTask& p = par[i];
+            const float* P = p.filt ? p.filt : p.pts;
+            const int n = over[i].second;
+            for (int o = 0; o < n; o += chunk) {
+                LaHullTask c{};
+                c.pts = P + 3ll * o;
+                c.n = std::min(chunk, n - o);
+                c.emit = 1;
+                c.prune_gap = 1e30;
+                c.frame = p.frame;
+                c.use_frame = 1;
+                sub.push_back(c);
+                sub_of.push_back((int)i);
+            }
+        }
+        int* d_ids = dev_tmp<int>(ctx, over.size());
+        int* d_off = dev_tmp<int>(ctx, over.size() + 1);
+        LaHullTask* d_sub = dev_tmp<LaHullTask>(ctx, sub.size());
+        LaUnionSeg* d_segs = dev_tmp<LaUnionSeg>(ctx, sub.size());
+        REQUIRE(d_ids && d_off && d_sub && d_segs, "lookahead: chunked hull buffers allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_sub, sub.data(), sizeof(LaHullTask) * sub.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        {
+            HullCall hc(ctx);
+            std::vector<std::pair<int, int>> idn;
+            for (size_t k = 0; k < sub.size(); ++k) idn.push_back({(int)k, sub[k].n});
+            if (int rc = launch_by_size(hc, d_sub, idn, false)) return rc;
+            if (int rc = hc.join()) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(sub.data(), d_sub, sizeof(LaHullTask) * sub.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<LaUnionSeg> segs;
+        std::vector<int> ids, off(1, 0), tot(over.size(), 0);
+        for (size_t i = 0, k = 0; i < over.size(); ++i) {
+            ids.push_back(over[i].first);
+            for (; k < sub.size() && sub_of[k] == (int)i; ++k) {
+                REQUIRE(sub[k].hv || sub[k].err == 0, "lookahead: chunk hull failed (error %d)", sub[k].err);
+                if (sub[k].hv) {
+                    segs.push_back({sub[k].hv, sub[k].hnv, 1});
+                    tot[i] += sub[k].hnv;
+                } else {
+                    segs.push_back({sub[k].pts, sub[k].n, 0});
+                    tot[i] += sub[k].n;
+                }
+            }
+            off.push_back((int)segs.size());
+        }
+        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d_off, off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (!segs.empty())
+            HIP_TRY(hipMemcpyAsync(d_segs, segs.data(), sizeof(LaUnionSeg) * segs.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        hipLaunchKernelGGL(la_union_kernel, dim3((unsigned)over.size()), dim3(256), 0, ctx->stream, d_tasks, d_ids,
+                           d_off, d_segs, (int)over.size(), ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipFreeAsync(d_ids, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_off, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_sub, ctx->stream));
+        HIP_TRY(hipFreeAsync(d_segs, ctx->stream));
+        std::vector<std::pair<int, int>> next;
+        for (size_t i = 0; i < over.size(); ++i) {
+            if (tot[i] <= limit) {
+                fin.push_back({over[i].first, tot[i]});
+            } else if (tot[i] < over[i].second) {
+                next.push_back({over[i].first, tot[i]});  // shrinking: chunk again
+            } else {
+                // no chunk lost a point: every point is a corner
+                REQUIRE(tot[i] <= HULL_NP_MAX, "lookahead: hull set of %d points with %d chunk corners exceeds %d",
+                        over[i].second, tot[i], HULL_NP_MAX);
+                fin.push_back({over[i].first, tot[i]});
+            }
+        }
+        over.swap(next);
     }
-    return v;
+    return 0;
 }
 
 int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n) {
@@ -1444,14 +1596,15 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
         std::vector<int2> res(big.size());
         HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        std::vector<std::pair<int, int>> fin;
+        std::vector<std::pair<int, int>> fin, over;
         for (size_t i = 0; i < big.size(); ++i) {
             if (res[i].y) continue;          // pruned: volume is the rough bound
             if (res[i].x < 0) continue;      // allocation failure, flagged in the task
-            REQUIRE(res[i].x <= HULL_NP_MAX, "lookahead: filtered hull set of %d points exceeds %d", res[i].x,
-                    HULL_NP_MAX);
-            fin.push_back({big[i], res[i].x});
+            if (res[i].x > g_tune[4]) over.push_back({big[i], res[i].x});
+            else fin.push_back({big[i], res[i].x});
         }
+        if (!over.empty())
+            if (int rc = hull_oversized(ctx, d_tasks, over, fin)) return rc;
         if (int rc = launch_by_size(hc, d_tasks, fin, false)) return rc;
     }
     return hc.join();
@@ -1517,3 +1670,21 @@ int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n) {
 }
 
 }  // namespace cuacd
+
+extern "C" {
+
+int gpu_set_tuning(gpu_ctx_t ctx, int key, int value) {
+    REQUIRE(key >= 0 && key < LA_TUNE_N, "unknown tuning key %d", key);
+    tune_init();
+    g_tune[key] = tune_clamp(key, value);
+    return 0;
+}
+
+int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value) {
+    REQUIRE(key >= 0 && key < LA_TUNE_N, "unknown tuning key %d", key);
+    tune_init();
+    *value = g_tune[key];
+    return 0;
+}
+
+}  // extern "C"

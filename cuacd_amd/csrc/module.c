@@ -204,6 +204,22 @@ static PyObject* m_score_cuts(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* m_set_tuning(PyObject* self, PyObject* args) {
+    int key, value;
+    if (!PyArg_ParseTuple(args, "ii", &key, &value)) return NULL;
+    int rc = gpu_set_tuning(g_ctx, key, value);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_get_tuning(PyObject* self, PyObject* args) {
+    int key, value = 0;
+    if (!PyArg_ParseTuple(args, "i", &key)) return NULL;
+    int rc = gpu_get_tuning(g_ctx, key, &value);
+    if (rc) return raise_error(rc);
+    return PyLong_FromLong(value);
+}
+
 static PyObject* part_list(struct gpu_result* res) {
     PyObject* list = PyList_New(res->nparts);
     if (!list) return NULL;
@@ -544,6 +560,8 @@ static PyMethodDef methods[] = {
     {"batch_mesh_volume", m_batch_mesh_volume, METH_VARARGS, "Volumes of a batch of meshes."},
     {"test_plane_cut", m_test_plane_cut, METH_VARARGS, "Plane cut with cap triangulation."},
     {"plane_cut_batch", m_plane_cut_batch, METH_VARARGS, "Batched plane cuts: volumes, sizes, status."},
+    {"set_tuning", m_set_tuning, METH_VARARGS, "Set a hull launch-shape knob (key, value)."},
+    {"get_tuning", m_get_tuning, METH_VARARGS, "Read a hull launch-shape knob."},
     {"score_cuts", m_score_cuts, METH_VARARGS, "Batched exact cut scores: cut, hulls, max(0.3 rv, H) per half."},
     {"kdop_hull", m_kdop_hull, METH_VARARGS, "k-DOP prefiltered exact hull."},
     {"test_hausdorff", m_test_hausdorff, METH_VARARGS, "Bidirectional sampled Hausdorff distance."},

@@ -67,6 +67,17 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
         REQUIRE(vert_off[m + 1] >= vert_off[m] && tri_off[m + 1] >= tri_off[m], "score_cuts: bad offsets");
     for (int i = 0; i < n_cuts; ++i) REQUIRE(mesh_of[i] >= 0 && mesh_of[i] < n_meshes, "score_cuts: bad mesh index");
     const size_t tv = (size_t)vert_off[n_meshes], tt = (size_t)tri_off[n_meshes];
+    // per-kernel timers of the hull launches (for the stage statistics)
+    struct KtGuard {
+        gpu_ctx_t c;
+        KTimers k;
+        explicit KtGuard(gpu_ctx_t c_) : c(c_) { c->kt = &k; }
+        ~KtGuard() {
+            for (auto& r : k.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+            for (hipEvent_t e : k.free_ev) (void)hipEventDestroy(e);
+            c->kt = nullptr;
+        }
+    } ktg(ctx);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -159,6 +170,7 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
         if (rc == 0 && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = -1;
         if (rc == 0) rc = la_launch_hull(ctx, dht.p, ht.data(), (int)ht.size());
         if (rc == 0) rc = phase_end(1);
+        if (rc == 0) rc = kt_resolve(ctx);
         if (rc == 0 && hipMemcpyAsync(ht.data(), dht.p, sizeof(LaHullTask) * ht.size(), hipMemcpyDeviceToHost,
                                       ctx->stream) != hipSuccess)
             rc = -1;
@@ -266,6 +278,8 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
         out_stats[3] = cut_bytes;
         out_stats[4] = hull_bytes;
         out_stats[5] = (double)ht.size();
+        for (int kc = 0; kc < KT_COUNT; ++kc)
+            for (int j = 0; j < 3; ++j) out_stats[6 + 3 * kc + j] = ctx->kt ? ctx->kt->stats[kc][j] : 0.0;
     }
     return 0;
 }

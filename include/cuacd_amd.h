@@ -214,15 +214,28 @@ int gpu_plane_cut_batch(gpu_ctx_t ctx, const float* verts, const int* vert_off, 
  * larger cost of the two halves, 1e30 when the cut fails or leaves a side
  * empty. out_detail (optional, 10 doubles per cut): status, mesh volume
  * pos/neg (clamped to the hull volume), hull volume pos/neg, rv cost pos/neg,
- * H pos/neg (-1 when not measured), 1 if scored. out_stats (optional, 6
+ * H pos/neg (-1 when not measured), 1 if scored. out_stats (optional, 45
  * doubles): GPU ms of the cut, hull and Hausdorff stages, algorithmic bytes of
- * the cut and hull stages, hull tasks. */
+ * the cut and hull stages, hull tasks, then per hull kernel class (the 13
+ * classes of gpu_lookahead_batch's statistics) ms, launches, bytes. */
 int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris, const int* tri_off,
                    int n_meshes, const int* mesh_of, const float* planes, int n_cuts, float tau, float* out_score,
                    double* out_detail, double* out_stats);
 
-/* Plans: the same computation on device-resident meshes, prepared once and
- * launched many times (stream-ordered). Outputs are SQUARED distances. */
+/* Launch-shape knobs of the hull tiers (no reference counterpart; the
+ * autotuner sets them). They select kernels and capacities only - every
+ * setting computes the same canonical hulls. Process-wide. Keys: 0 wide_min
+ * (wave-tier sets above it run four waves per hull; 0 disables), 1 vcap8
+ * (first-pass vertex capacity in eighths, 1..8), 2 half_cap (reduced-capacity
+ * first passes, 0/1), 3 kdop_min (k-DOP prefilter above this many points,
+ * >= 4), 4 chunk_above (prefiltered sets above it are hulled in chunks of
+ * half that size first, 1024..16384). Values are clamped to their ranges. */
+int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
+int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);
+
+/* Plans: the same computation on device-resident meshes, prepared once (task
+ * table, workspaces, and the launches captured as a HIP graph) and replayed
+ * many times (stream-ordered). Outputs are SQUARED distances. */
 typedef struct hd_plan* hd_plan_t;
 int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_a_verts, const int* d_a_tris, const float* d_b_verts,
                               const int* d_b_tris, const int* h_a_vert_off, const int* h_a_tri_off,

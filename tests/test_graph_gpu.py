@@ -1,0 +1,93 @@
+"""GPU test of the Hausdorff plan's HIP graph, called through the C-ABI.
+
+gpu_hausdorff_plan_create captures the preparation and query launches as a
+graph; every gpu_hausdorff_plan_run replays it. Each replay (with eager work
+and synchronisations of the program in between) must reproduce the eager
+result of gpu_hausdorff_batch bit for bit.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from cuacd_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _pack(ms):
+    vs = [np.asarray(v, np.float32).reshape(-1, 3) for v, _ in ms]
+    ts = [np.asarray(t, np.int32).reshape(-1, 3) for _, t in ms]
+    voff = np.zeros(len(ms) + 1, np.int32)
+    toff = np.zeros(len(ms) + 1, np.int32)
+    voff[1:] = np.cumsum([len(v) for v in vs])
+    toff[1:] = np.cumsum([len(t) for t in ts])
+    return np.ascontiguousarray(np.concatenate(vs)), voff, np.ascontiguousarray(np.concatenate(ts)), toff
+
+
+def test_plan_graph_replays_match_eager(ctx):
+    lib = C.CDLL(os.path.join(ROOT, "cuacd_amd", "libcuacd_hip.so"))
+    lib.gpu_init.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_size_t]
+    lib.gpu_dev_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+    lib.gpu_dev_free.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gpu_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.gpu_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.gpu_memset_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]
+    lib.gpu_sync.argtypes = [C.c_void_p]
+    lib.gpu_hausdorff_plan_create.argtypes = [C.c_void_p] + [C.c_void_p] * 8 + [C.c_int, C.c_int, C.c_int] + \
+        [C.c_void_p] * 4 + [C.POINTER(C.c_void_p)]
+    lib.gpu_hausdorff_plan_run.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gpu_hausdorff_plan_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gpu_destroy.argtypes = [C.c_void_p]
+    lib.gpu_last_error.argtypes = [C.c_void_p]
+    lib.gpu_last_error.restype = C.c_char_p
+
+    a = datasets.noisy_icospheres(24, 3, 0.15, seed=5)
+    b = datasets.noisy_icospheres(24, 2, 0.05, seed=6)
+    n = len(a)
+    eager_ab, eager_ba = ctx.batch_hausdorff(a, b, sampler="reference")
+
+    g = C.c_void_p()
+    assert lib.gpu_init(C.byref(g), 0, C.c_size_t(1 << 28)) == 0
+    bufs = []
+    try:
+        def up(arr):
+            p = C.c_void_p()
+            assert lib.gpu_dev_alloc(g, C.c_size_t(max(arr.nbytes, 4)), C.byref(p)) == 0
+            bufs.append(p)
+            assert lib.gpu_memcpy_h2d(g, p, arr.ctypes.data, C.c_size_t(arr.nbytes)) == 0
+            return p
+
+        av, avo, at, ato = _pack(a)
+        bv, bvo, bt, bto = _pack(b)
+        d_av, d_at, d_bv, d_bt = up(av), up(at), up(bv), up(bt)
+        out = np.full(2 * n, 7.0, np.float32)
+        d_out = up(out)  # stale values: the kernels clear their own outputs
+        d_ab = C.c_void_p(d_out.value)
+        d_ba = C.c_void_p(d_out.value + 4 * n)
+        plan = C.c_void_p()
+        rc = lib.gpu_hausdorff_plan_create(g, d_av, d_at, d_bv, d_bt, avo.ctypes.data, ato.ctypes.data,
+                                           bvo.ctypes.data, bto.ctypes.data, n, 0, 0, d_ab, d_ba, None, None,
+                                           C.byref(plan))
+        assert rc == 0, lib.gpu_last_error(g)
+        try:
+            for rep in range(4):
+                assert lib.gpu_hausdorff_plan_run(g, plan) == 0, lib.gpu_last_error(g)
+                res = np.empty(2 * n, np.float32)
+                assert lib.gpu_memcpy_d2h(g, res.ctypes.data, d_out, C.c_size_t(res.nbytes)) == 0
+                assert np.array_equal(np.sqrt(res[:n]), eager_ab), rep
+                assert np.array_equal(np.sqrt(res[n:]), eager_ba), rep
+                # eager work of the program between replays
+                e_ab, e_ba = ctx.batch_hausdorff(a[:3], b[:3], sampler="reference")
+                assert np.array_equal(e_ab, eager_ab[:3])
+                assert lib.gpu_memset_dev(g, d_out, 0x7f, C.c_size_t(res.nbytes)) == 0
+                assert lib.gpu_sync(g) == 0
+        finally:
+            lib.gpu_hausdorff_plan_destroy(g, plan)
+    finally:
+        for p in bufs:
+            lib.gpu_dev_free(g, p)
+        lib.gpu_destroy(g)

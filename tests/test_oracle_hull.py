@@ -243,3 +243,40 @@ def test_canonical_hull_equals_raw_in_general_position():
     assert sorted(map(lambda t: tuple(np.roll(t, -int(np.argmin(t)))), t_raw.tolist())) == \
         sorted(map(tuple, t_can.tolist()))
     assert abs(oracle.mesh_volume(p, t_raw) - oracle.mesh_volume(p, t_can)) < 1e-12
+
+
+def test_chunked_hull_union_keeps_the_canonical_hull():
+    """The property the GPU's chunked path for oversized sets rests on: with
+    one quantization frame, the canonical hull of the union of the chunks'
+    canonical corners (chunks contiguous, corners in point order, a flat
+    chunk passed through whole) is the canonical hull of the whole set."""
+    rng = np.random.default_rng(8)
+    p = rng.normal(size=(3000, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    p = p[p[:, 0] >= 0.1] * (1 + 0.002 * rng.normal(size=(1, 1)))
+    # a planar cap on x = 0.1 appended last, as a cut emits it
+    cap = np.c_[np.full(400, 0.1), rng.uniform(-0.9, 0.9, (400, 2))]
+    cap = cap[np.linalg.norm(cap[:, 1:], axis=1) <= 0.99]
+    pts = np.concatenate([p, cap]).astype(np.float32)
+    q = oracle.quantize(pts)
+    flags, faces = oracle.hull(q)
+    keep = []
+    for o in range(0, len(q), 256):
+        c = q[o:o + 256]
+        try:
+            f, fc = oracle.hull(c)
+            flat = len(fc) == 0
+        except RuntimeError:
+            flat = True
+        idx = np.arange(o, o + len(c))
+        keep.append(idx if flat else idx[f])
+    u = np.concatenate(keep)
+    assert len(u) < len(q)
+    f2, faces2 = oracle.hull(q[u])
+    # same corners (as original indices) and the same facets
+    assert np.array_equal(u[f2], np.nonzero(flags)[0])
+    assert np.array_equal(u[faces2], faces)
+    # a dropped flat chunk (the mistake the pass-through avoids) loses corners
+    bad = np.concatenate([k for k, o in zip(keep, range(0, len(q), 256)) if o < len(p) - 256])
+    fb, _ = oracle.hull(q[bad])
+    assert not np.array_equal(bad[fb], np.nonzero(flags)[0])

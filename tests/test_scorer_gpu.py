@@ -63,6 +63,31 @@ def test_scores_match_oracle_all_hausdorff(ctx):
     assert _check(ctx, 10.0) > 0
 
 
+def test_scores_match_oracle_chunked_hulls(ctx):
+    # sets above 1024 points after the prefilter are hulled in chunks of 512
+    # first; the union of the chunk corners must give the same hulls
+    import lookahead_oracle as LO
+    old = ctx.get_tuning("hull_chunk_above")
+    try:
+        ctx.set_tuning("hull_chunk_above", 1024)
+        assert ctx.get_tuning("hull_chunk_above") == 1024
+        _check(ctx, 10.0)
+        # a nearly spherical mesh: halves of ~5k points that nearly all pass
+        # the prefilter, several chunking rounds
+        (v, t), = datasets.noisy_icospheres(1, 5, 0.003, seed=2)
+        v = np.asarray(v, np.float32)
+        planes = np.array([[1, 0, 0, -0.1], [0.6, 0.8, 0, 0.2]], np.float32)
+        score, detail, _ = ctx.batch_score_cuts([(v, t)], [0, 0], planes, threshold=0.05, return_detail=True)
+        for i, pl in enumerate(planes):
+            ref, halves = LO.cut_cost(v, t, pl, 0.05)
+            for s_, (mv, hv, crv, h) in enumerate(halves):
+                assert abs(detail[i, 3 + s_] - hv) <= 1e-6 * hv, (i, s_)
+                assert abs(detail[i, 7 + s_] - h) <= 1e-5, (i, s_)
+            assert abs(score[i] - ref) <= 1e-5, (i, score[i], ref)
+    finally:
+        ctx.set_tuning("hull_chunk_above", old)
+
+
 def test_missed_plane_and_empty_batch(ctx):
     v, t = datasets.cube_mesh()
     v = np.asarray(v, np.float32)
@@ -81,3 +106,21 @@ def test_heap_balanced_after_scoring(ctx):
     ctx.batch_score_cuts([(v, t)], np.zeros(len(pl), np.int32), pl, threshold=0.05)
     after = ctx.heap_stats()
     assert after[0] == before[0] and after[3] == before[3], (before, after)
+
+
+def test_autotuner_keeps_scores(ctx):
+    from cuacd_amd import autotune
+    cases = _cases()
+    meshes = [(v, t) for v, t, _ in cases]
+    mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, (_, _, p) in enumerate(cases)])
+    planes = np.concatenate([p for _, _, p in cases])
+    start = ctx.get_tuning()
+    ref = ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=10.0)
+    try:
+        best, trials = autotune.tune(ctx, lambda: ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=10.0))
+        # every knob only picks kernels: no setting may change a score
+        assert all(ok for _, _, ok in trials), [s for s, _, ok in trials if not ok]
+        assert np.array_equal(ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=10.0), ref)
+    finally:
+        for k, v in start.items():
+            ctx.set_tuning(k, v)
