@@ -31,7 +31,7 @@ MAV_FLIPPED = 1 << 4
 
 # phases reported by batch_lookahead_decompose (Context.last_lookahead_stats)
 LOOKAHEAD_PHASES = ("components", "part_hulls", "hausdorff", "concave_edges", "level0_cuts", "leaf_scoring",
-                    "item_hulls", "merge")
+                    "item_hulls", "merge", "proposal")
 # host wall-time sections of the same run (Context.last_lookahead_sections, ms)
 LOOKAHEAD_SECTIONS = ("setup", "refresh_select", "concave_planes", "level0", "deeper_levels", "evaluate_apply",
                       "post_merge", "readback")
@@ -458,12 +458,16 @@ class Context:
                                   depth=2, max_n_cutting=16, decompose_components=False,
                                   no_decompose_components_per_iter=False, n_concave_edges=32,
                                   concave_eps=0.005, concave_threshold=3.49, concave_iters=10,
-                                  merge_hulls=False, return_info=False):
+                                  merge_hulls=False, return_info=False, proposal=None, top_k=8):
         """lookahead_decompose over many meshes at once: one search whose
         launches span every mesh of the batch; each mesh's result is the same
         as its own lookahead_decompose call. Returns a list (per mesh) of part
         lists; with return_info also a list of dicts (iterations, per-part
-        mesh/hull volumes and Hausdorff distance)."""
+        mesh/hull volumes and Hausdorff distance).
+
+        proposal: a cuacd_amd.proposal.ProposalNet; level 0 of every iteration
+        then evaluates exactly only each cutting part's top_k candidates by
+        predicted score (the others when those yield no cut)."""
         n = len(meshes)
         if n == 0:
             return ([], []) if return_info else []
@@ -485,19 +489,27 @@ class Context:
         if width + 4 * n_concave_edges >= 512:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
-        stats = np.zeros(71, np.float64)
-        raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
-                                   hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
-                                   max_iters, width, width2, float(threshold), depth, max_n_cutting,
-                                   int(decompose_components), int(not no_decompose_components_per_iter),
-                                   n_concave_edges, float(concave_eps), float(concave_threshold),
-                                   concave_iters, int(merge_hulls), stats.ctypes.data, iters.ctypes.data)
-        ph = stats[:24].reshape(8, 3)
+        NP_, NS_, NK_ = len(LOOKAHEAD_PHASES), len(LOOKAHEAD_SECTIONS), len(LOOKAHEAD_KERNELS)
+        stats = np.zeros(3 * NP_ + NS_ + 3 * NK_ + 3, np.float64)
+        _gpu.lookahead_set_proposal(proposal.h if proposal is not None else 0, int(top_k))
+        try:
+            raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
+                                       hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
+                                       max_iters, width, width2, float(threshold), depth, max_n_cutting,
+                                       int(decompose_components), int(not no_decompose_components_per_iter),
+                                       n_concave_edges, float(concave_eps), float(concave_threshold),
+                                       concave_iters, int(merge_hulls), stats.ctypes.data, iters.ctypes.data)
+        finally:
+            _gpu.lookahead_set_proposal(0, 0)
+        ph = stats[:3 * NP_].reshape(NP_, 3)
         self.last_lookahead_stats = {name: {"ms": float(ph[i, 0]), "launches": int(ph[i, 1]),
                                             "alg_bytes": float(ph[i, 2])}
                                      for i, name in enumerate(LOOKAHEAD_PHASES)}
-        self.last_lookahead_sections = {name: float(stats[24 + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
-        kt = stats[32:71].reshape(13, 3)
+        self.last_lookahead_sections = {name: float(stats[3 * NP_ + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
+        k0 = 3 * NP_ + NS_
+        kt = stats[k0:k0 + 3 * NK_].reshape(NK_, 3)
+        pr = stats[k0 + 3 * NK_:]
+        self.last_lookahead_proposal = {"kept": int(pr[0]), "ranked": int(pr[1]), "fallbacks": int(pr[2])}
         self.last_lookahead_kernels = {name: {"ms": float(kt[i, 0]), "launches": int(kt[i, 1]),
                                               "alg_bytes": float(kt[i, 2])}
                                        for i, name in enumerate(LOOKAHEAD_KERNELS)}

@@ -32,6 +32,8 @@ struct KTimers {
 };
 }  // namespace cuacd
 
+struct gpu_mlp;
+
 struct gpu_ctx {
     int device;
     int n_cu;
@@ -52,6 +54,9 @@ struct gpu_ctx {
     hipStream_t aux[3];
     hipEvent_t ev_fork;
     hipEvent_t ev_join[3];
+    // learned proposal for the lookahead (null: plain search)
+    struct gpu_mlp* proposal;
+    int proposal_topk;
 };
 
 namespace cuacd {

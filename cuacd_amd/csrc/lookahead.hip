@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 #include "lookahead.hpp"
+#include "mlp.hpp"
 
 #pragma clang fp contract(off)
 
@@ -153,7 +154,7 @@ template <class T> struct DevBuf {
 
 // Per-phase accounting (GPU time from events around each phase's launches,
 // launch count, algorithmic bytes: what the phase must read and write).
-enum LaPhase { PH_COMP, PH_PHULL, PH_HD, PH_EDGE, PH_CUT, PH_SCORE, PH_IHULL, PH_MERGE, PH_COUNT };
+enum LaPhase { PH_COMP, PH_PHULL, PH_HD, PH_EDGE, PH_CUT, PH_SCORE, PH_IHULL, PH_MERGE, PH_PROP, PH_COUNT };
 // Host wall time by section of the search (launches, waits and host work):
 // input set-up, per-iteration refresh + selection, concave planes, level 0,
 // deeper levels, evaluate + apply, post-loop passes and merge, result readback.
@@ -169,6 +170,7 @@ struct Engine {
     std::chrono::steady_clock::time_point t_mark = std::chrono::steady_clock::now();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int hull_phase = PH_IHULL;
+    long long proposal_kept = 0, proposal_total = 0, proposal_fallbacks = 0;
     char* pinned = nullptr;  // grow-only pinned staging for batched readbacks
     size_t pinned_cap = 0;
 
@@ -1010,6 +1012,69 @@ struct Engine {
                     cands.push_back(cd);
                 }
             }
+            // learned proposal: per cutting part, only the top_k candidates by
+            // predicted score are evaluated exactly; the others are the
+            // fallback for a part whose kept candidates yield no cut
+            std::vector<Cand> rest;
+            if (d == 0 && ctx->proposal && ctx->proposal_topk > 0 && !cands.empty()) {
+                const int P = (int)cur.size(), C = (int)cands.size();
+                std::vector<MlpSamplePart> sp(P);
+                std::vector<float> fc(3 * P), fs(P);
+                for (int i = 0; i < P; ++i) {
+                    const IPart& wp = cur[i].parts[cur[i].np - 1];
+                    sp[i] = MlpSamplePart{wp.v, wp.t, wp.nv, wp.nt, {wp.lo[0], wp.lo[1], wp.lo[2]},
+                                          {wp.hi[0], wp.hi[1], wp.hi[2]}};
+                    float sc = 0.0f;
+                    for (int j = 0; j < 3; ++j) {
+                        fc[3 * i + j] = 0.5f * (wp.lo[j] + wp.hi[j]);
+                        sc = std::max(sc, 0.5f * (wp.hi[j] - wp.lo[j]));
+                    }
+                    fs[i] = sc > 0.0f ? sc : 1.0f;
+                }
+                // planes in their part's frame: unit normal, offset
+                std::vector<float> pl(4 * (size_t)C);
+                std::vector<int> po(C);
+                for (int q = 0; q < C; ++q) {
+                    const Cand& cd = cands[q];
+                    const float* n = cd.pl;
+                    const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    const float il = len > 0.0f ? 1.0f / len : 0.0f;
+                    const int i = cd.item;
+                    float dd = n[3] * il;
+                    for (int j = 0; j < 3; ++j) {
+                        pl[4 * q + j] = n[j] * il;
+                        dd += n[j] * il * fc[3 * i + j];
+                    }
+                    pl[4 * q + 3] = dd / fs[i];
+                    po[q] = i;
+                }
+                DevBuf<float> dpts, dpl, dsc;
+                DevBuf<int> dpo;
+                REQUIRE(dpts.alloc(ctx, (size_t)P * MLP_SAMPLES * 3) == 0 && dpl.alloc(ctx, 4 * (size_t)C) == 0 &&
+                            dsc.alloc(ctx, C) == 0 && dpo.alloc(ctx, C) == 0,
+                        "lookahead: proposal buffers allocation failed");
+                HIP_TRY(hipMemcpyAsync(dpl.p, pl.data(), 16 * (size_t)C, hipMemcpyHostToDevice, ctx->stream));
+                HIP_TRY(hipMemcpyAsync(dpo.p, po.data(), 4 * (size_t)C, hipMemcpyHostToDevice, ctx->stream));
+                LA_TRY(tbeg());
+                LA_TRY(mlp_sample_dev(ctx, sp.data(), P, dpts.p));
+                LA_TRY(mlp_forward_dev(ctx, ctx->proposal, dpts.p, P, dpl.p, dpo.p, C, dsc.p));
+                LA_TRY(tend(PH_PROP, 12.0 * P * MLP_SAMPLES + 20.0 * C, 2));
+                std::vector<float> score(C);
+                HIP_TRY(hipMemcpyAsync(score.data(), dsc.p, 4 * (size_t)C, hipMemcpyDeviceToHost, ctx->stream));
+                LA_TRY(check_err("proposal"));
+                std::vector<std::vector<int>> by_item(P);
+                for (int q = 0; q < C; ++q) by_item[cands[q].item].push_back(q);
+                std::vector<Cand> kept;
+                for (int i = 0; i < P; ++i) {
+                    std::vector<int>& v = by_item[i];
+                    std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return score[x] < score[y]; });
+                    for (int j = 0; j < (int)v.size(); ++j)
+                        (j < ctx->proposal_topk ? kept : rest).push_back(cands[v[j]]);
+                }
+                proposal_kept += (long long)kept.size();
+                proposal_total += (long long)C;
+                cands.swap(kept);
+            }
             // build the children: halves + volumes
             std::vector<Item> nxt;
             std::vector<int> nxt_c;
@@ -1023,66 +1088,83 @@ struct Engine {
                 bool pruned;
             };
             std::vector<SRec> srec;
+            auto build_children = [&](const std::vector<Cand>& cs) -> int {
+                    std::vector<LaCutTask> h(cs.size());
+                    for (size_t q = 0; q < cs.size(); ++q) {
+                        const IPart& wp = cur[cs[q].item].parts[cur[cs[q].item].np - 1];
+                        LaCutTask t{};
+                        t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
+                        for (int j = 0; j < 4; ++j) t.plane[j] = cs[q].pl[j];
+                        h[q] = t;
+                    }
+                    if (!h.empty()) {
+                        DevBuf<LaCutTask> dt;
+                        REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+                        HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice,
+                                               ctx->stream));
+                        LA_TRY(tbeg());
+                        LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
+                        LA_TRY(tend(PH_CUT, 0.0));
+                        HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
+                                               ctx->stream));
+                        HIP_TRY(hipStreamSynchronize(ctx->stream));
+                        {
+                            // mesh read, both halves written, volumes
+                            double by = 0;
+                            for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
+                            stats[PH_CUT][2] += by;
+                        }
+                        LA_TRY(check_err("cuts"));
+                    }
+                    for (size_t q = 0; q < cs.size(); ++q) {
+                        const LaCutTask& t = h[q];
+                        if (t.status != 0 || !t.pv || !t.nvp) continue;
+                        const Item& par = cur[cs[q].item];
+                        Item ch = par;
+                        IPart pp, nn;
+                        pp.v = t.pv; pp.t = t.pt; pp.nv = t.pnv; pp.nt = t.pnt; pp.mvol = t.pvol;
+                        nn.v = t.nvp; nn.t = t.ntp; nn.nv = t.nnv; nn.nt = t.nnt; nn.mvol = t.nvol;
+                        pp.cc = nn.cc = par.parts[par.np - 1].cc;
+                        for (int j = 0; j < 3; ++j) {
+                            pp.lo[j] = t.plo[j]; pp.hi[j] = t.phi[j];
+                            nn.lo[j] = t.nlo[j]; nn.hi[j] = t.nhi[j];
+                        }
+                        pp.store = (int)store.size();
+                        store.push_back({t.pv, false});
+                        nn.store = (int)store.size();
+                        store.push_back({t.nvp, false});
+                        ch.parts[par.np - 1] = pp;
+                        ch.parts[par.np] = nn;
+                        ch.np = par.np + 1;
+                        if (par.nl == 0) ch.cut0 = cs[q].k;
+                        const int ci = (int)nxt.size();
+                        nxt.push_back(ch);
+                        nxt_c.push_back(cur_c[cs[q].item]);
+                        LaHullTask a{};
+                        a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30;
+                        LaHullTask b{};
+                        b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30;
+                        ht.push_back(a);
+                        ht_who.push_back({ci, 0});
+                        ht.push_back(b);
+                        ht_who.push_back({ci, 1});
+                    }
+                return 0;
+            };
             if (full) {
-                std::vector<LaCutTask> h(cands.size());
-                for (size_t q = 0; q < cands.size(); ++q) {
-                    const IPart& wp = cur[cands[q].item].parts[cur[cands[q].item].np - 1];
-                    LaCutTask t{};
-                    t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
-                    for (int j = 0; j < 4; ++j) t.plane[j] = cands[q].pl[j];
-                    h[q] = t;
-                }
-                if (!h.empty()) {
-                    DevBuf<LaCutTask> dt;
-                    REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
-                    HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice,
-                                           ctx->stream));
-                    LA_TRY(tbeg());
-                    LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
-                    LA_TRY(tend(PH_CUT, 0.0));
-                    HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
-                                           ctx->stream));
-                    HIP_TRY(hipStreamSynchronize(ctx->stream));
-                    {
-                        // mesh read, both halves written, volumes
-                        double by = 0;
-                        for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
-                        stats[PH_CUT][2] += by;
+                LA_TRY(build_children(cands));
+                // exact fallback of the learned proposal: a cutting part whose
+                // kept candidates produced no cut gets the others
+                if (d == 0 && !rest.empty()) {
+                    std::vector<char> got(cutting.size(), 0);
+                    for (int c : nxt_c) got[c] = 1;
+                    std::vector<Cand> fb;
+                    for (const Cand& cd : rest)
+                        if (!got[cur_c[cd.item]]) fb.push_back(cd);
+                    if (!fb.empty()) {
+                        proposal_fallbacks += 1;
+                        LA_TRY(build_children(fb));
                     }
-                    LA_TRY(check_err("cuts"));
-                }
-                for (size_t q = 0; q < cands.size(); ++q) {
-                    const LaCutTask& t = h[q];
-                    if (t.status != 0 || !t.pv || !t.nvp) continue;
-                    const Item& par = cur[cands[q].item];
-                    Item ch = par;
-                    IPart pp, nn;
-                    pp.v = t.pv; pp.t = t.pt; pp.nv = t.pnv; pp.nt = t.pnt; pp.mvol = t.pvol;
-                    nn.v = t.nvp; nn.t = t.ntp; nn.nv = t.nnv; nn.nt = t.nnt; nn.mvol = t.nvol;
-                    pp.cc = nn.cc = par.parts[par.np - 1].cc;
-                    for (int j = 0; j < 3; ++j) {
-                        pp.lo[j] = t.plo[j]; pp.hi[j] = t.phi[j];
-                        nn.lo[j] = t.nlo[j]; nn.hi[j] = t.nhi[j];
-                    }
-                    pp.store = (int)store.size();
-                    store.push_back({t.pv, false});
-                    nn.store = (int)store.size();
-                    store.push_back({t.nvp, false});
-                    ch.parts[par.np - 1] = pp;
-                    ch.parts[par.np] = nn;
-                    ch.np = par.np + 1;
-                    if (par.nl == 0) ch.cut0 = cands[q].k;
-                    const int ci = (int)nxt.size();
-                    nxt.push_back(ch);
-                    nxt_c.push_back(cur_c[cands[q].item]);
-                    LaHullTask a{};
-                    a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30;
-                    LaHullTask b{};
-                    b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30;
-                    ht.push_back(a);
-                    ht_who.push_back({ci, 0});
-                    ht.push_back(b);
-                    ht_who.push_back({ci, 1});
                 }
             } else {
                 // branch-and-bound bound per mesh (cuda/la_expand.cu:434-457): min over the
@@ -1523,6 +1605,10 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         (void)kt_resolve(ctx);
         for (int kc = 0; kc < KT_COUNT; ++kc)
             for (int j = 0; j < 3; ++j) stats_out[3 * PH_COUNT + S_COUNT + 3 * kc + j] = ktg.k.stats[kc][j];
+        double* pr = stats_out + 3 * PH_COUNT + S_COUNT + 3 * KT_COUNT;
+        pr[0] = (double)E.proposal_kept;
+        pr[1] = (double)E.proposal_total;
+        pr[2] = (double)E.proposal_fallbacks;
     }
     if (rc) {
         for (int m = 0; m < n_meshes; ++m) gpu_result_free(&out[m]);
@@ -1561,3 +1647,92 @@ int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const i
 }
 
 }  // extern "C"
+
+extern "C" int gpu_lookahead_set_proposal(gpu_ctx_t ctx, gpu_mlp_t m, int top_k) {
+    if (!ctx) return -1;
+    REQUIRE(!m || top_k > 0, "lookahead proposal: top_k must be positive");
+    ctx->proposal = m;
+    ctx->proposal_topk = m ? top_k : 0;
+    return 0;
+}
+
+// Level-0 candidate planes of whole meshes as the search builds them (axis
+// cuts of cuda/la_axis_cut.cuh:10-19 inside the box margins, then the planes
+// of sampled concave edges, cuda/la_concave.cu:48-363), for the proposal
+// network's training data. Mesh m's planes go to out_planes[4 * (m * cap + j)],
+// their count (at most cap) to out_count[m]; out_k[m * cap + j] is the
+// candidate's index k in the search (axis cuts first).
+extern "C" int gpu_lookahead_candidates(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* voff,
+                                        const int* tris, const int* toff, int width, float threshold, int n_concave,
+                                        float concave_eps, float concave_threshold, int cap, float* out_planes,
+                                        int* out_k, int* out_count) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    REQUIRE(n_meshes >= 0 && width >= 1 && cap > 0, "lookahead_candidates: bad arguments");
+    if (n_meshes == 0) return 0;
+    Engine E;
+    E.ctx = ctx;
+    E.P = Params{};
+    E.P.width = width;
+    E.P.threshold = threshold;
+    E.P.n_concave = n_concave;
+    E.P.concave_eps = concave_eps;
+    E.P.concave_threshold = concave_threshold;
+    E.M.resize(n_meshes);
+    const size_t tv = (size_t)voff[n_meshes], tt = (size_t)toff[n_meshes];
+    float* d_v = nullptr;
+    int* d_t = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_v, 12 * tv + 16, ctx->stream));
+    HIP_TRY(hipMallocAsync((void**)&d_t, 12 * tt + 16, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_v, verts, 12 * tv, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_t, tris, 12 * tt, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<std::pair<int, int>> cutting;
+    for (int m = 0; m < n_meshes; ++m) {
+        Part p;
+        p.v = d_v + 3ll * voff[m];
+        p.t = d_t + 3ll * toff[m];
+        p.nv = voff[m + 1] - voff[m];
+        p.nt = toff[m + 1] - toff[m];
+        const float* hv = verts + 3ll * voff[m];
+        for (int j = 0; j < 3; ++j) { p.lo[j] = 1e30f; p.hi[j] = -1e30f; }
+        for (int i = 0; i < p.nv; ++i)
+            for (int j = 0; j < 3; ++j) {
+                p.lo[j] = std::min(p.lo[j], hv[3 * i + j]);
+                p.hi[j] = std::max(p.hi[j], hv[3 * i + j]);
+            }
+        E.M[m].parts.push_back(p);
+        cutting.push_back({m, 0});
+    }
+    std::vector<std::vector<float>> ep(n_meshes);
+    int rc = n_concave > 0 ? E.concave_planes(cutting, ep) : 0;
+    float med = threshold * 0.25f;
+    if (med > 0.015f) med = 0.015f;
+    for (int m = 0; m < n_meshes && rc == 0; ++m) {
+        const Part& wp = E.M[m].parts[0];
+        int cnt = 0;
+        auto put = [&](const float* pl, int k) {
+            if (cnt >= cap) return;
+            for (int j = 0; j < 4; ++j) out_planes[4 * ((size_t)m * cap + cnt) + j] = pl[j];
+            out_k[(size_t)m * cap + cnt] = k;
+            ++cnt;
+        };
+        for (int k = 0; k < width; ++k) {
+            const AxisCut a = axis_cut(width, k);
+            const float lov = wp.lo[a.axis] + med, hiv = wp.hi[a.axis] - med;
+            if (lov >= hiv) continue;
+            const float frac = (a.slice + 1.f) / (a.cpa + 1.f);
+            const float pos = lov + frac * (hiv - lov);
+            float pl[4] = {a.axis == 0 ? 1.f : 0.f, a.axis == 1 ? 1.f : 0.f, a.axis == 2 ? 1.f : 0.f, -pos};
+            put(pl, k);
+        }
+        for (size_t e = 0; 4 * e + 3 < ep[m].size(); ++e) {
+            const float* pl = &ep[m][4 * e];
+            if (pl[0] == 0.f && pl[1] == 0.f && pl[2] == 0.f) continue;
+            put(pl, width + (int)e);
+        }
+        out_count[m] = cnt;
+    }
+    (void)hipFreeAsync(d_v, ctx->stream);
+    (void)hipFreeAsync(d_t, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+}

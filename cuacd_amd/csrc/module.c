@@ -220,6 +220,109 @@ static PyObject* m_get_tuning(PyObject* self, PyObject* args) {
     return PyLong_FromLong(value);
 }
 
+static PyObject* m_mlp_n_params(PyObject* self, PyObject* args) { return PyLong_FromLong(gpu_mlp_n_params()); }
+
+static PyObject* m_mlp_create(PyObject* self, PyObject* args) {
+    unsigned long long params;
+    int seed;
+    if (!PyArg_ParseTuple(args, "Ki", &params, &seed)) return NULL;
+    NEED_CTX();
+    gpu_mlp_t m = NULL;
+    int rc = gpu_mlp_create(g_ctx, params ? P(const float*, params) : NULL, seed, &m);
+    if (rc) return raise_error(rc);
+    return PyLong_FromUnsignedLongLong((unsigned long long)(uintptr_t)m);
+}
+
+static PyObject* m_mlp_destroy(PyObject* self, PyObject* args) {
+    unsigned long long h;
+    if (!PyArg_ParseTuple(args, "K", &h)) return NULL;
+    if (g_ctx && h) gpu_mlp_destroy(g_ctx, (gpu_mlp_t)(uintptr_t)h);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_mlp_get_params(PyObject* self, PyObject* args) {
+    unsigned long long h, out;
+    if (!PyArg_ParseTuple(args, "KK", &h, &out)) return NULL;
+    NEED_CTX();
+    int rc = gpu_mlp_get_params(g_ctx, (gpu_mlp_t)(uintptr_t)h, P(float*, out));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_mlp_forward(PyObject* self, PyObject* args) {
+    unsigned long long h, pts, pl, po, out;
+    int np, nc;
+    if (!PyArg_ParseTuple(args, "KKiKKiK", &h, &pts, &np, &pl, &po, &nc, &out)) return NULL;
+    NEED_CTX();
+    int rc = gpu_mlp_forward(g_ctx, (gpu_mlp_t)(uintptr_t)h, P(const float*, pts), np, P(const float*, pl),
+                             P(const int*, po), nc, P(float*, out));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_mlp_train_step(PyObject* self, PyObject* args) {
+    unsigned long long h, pts, pl, po, tg, gr;
+    int np, nc, apply;
+    float lr;
+    if (!PyArg_ParseTuple(args, "KKiKKiKfiK", &h, &pts, &np, &pl, &po, &nc, &tg, &lr, &apply, &gr)) return NULL;
+    NEED_CTX();
+    float loss = 0.0f;
+    int rc = gpu_mlp_train_step(g_ctx, (gpu_mlp_t)(uintptr_t)h, P(const float*, pts), np, P(const float*, pl),
+                                P(const int*, po), nc, P(const float*, tg), lr, apply, &loss,
+                                gr ? P(float*, gr) : NULL);
+    if (rc) return raise_error(rc);
+    return PyFloat_FromDouble(loss);
+}
+
+static PyObject* m_mlp_sample_parts(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, out;
+    int n;
+    if (!PyArg_ParseTuple(args, "KKKKiK", &v, &vo, &t, &to, &n, &out)) return NULL;
+    NEED_CTX();
+    int rc = gpu_mlp_sample_parts(g_ctx, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
+                                  n, P(float*, out));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_mlp_gemm_test(PyObject* self, PyObject* args) {
+    int M, N, K;
+    unsigned long long a, b, c;
+    long long sam, sak, alen, sbk, sbn, blen;
+    if (!PyArg_ParseTuple(args, "iiiKLLLKLLLK", &M, &N, &K, &a, &sam, &sak, &alen, &b, &sbk, &sbn, &blen, &c))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_mlp_gemm_test(g_ctx, M, N, K, P(const unsigned short*, a), sam, sak, alen,
+                               P(const unsigned short*, b), sbk, sbn, blen, P(float*, c));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_lookahead_set_proposal(PyObject* self, PyObject* args) {
+    unsigned long long h;
+    int k;
+    if (!PyArg_ParseTuple(args, "Ki", &h, &k)) return NULL;
+    NEED_CTX();
+    int rc = gpu_lookahead_set_proposal(g_ctx, (gpu_mlp_t)(uintptr_t)h, k);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_lookahead_candidates(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, op, ok, oc;
+    int n, width, nconc, cap;
+    float thr, eps, cthr;
+    if (!PyArg_ParseTuple(args, "iKKKKififfiKKK", &n, &v, &vo, &t, &to, &width, &thr, &nconc, &eps, &cthr, &cap, &op,
+                          &ok, &oc))
+        return NULL;
+    NEED_CTX();
+    int rc = gpu_lookahead_candidates(g_ctx, n, P(const float*, v), P(const int*, vo), P(const int*, t),
+                                      P(const int*, to), width, thr, nconc, eps, cthr, cap, P(float*, op),
+                                      P(int*, ok), P(int*, oc));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
 static PyObject* part_list(struct gpu_result* res) {
     PyObject* list = PyList_New(res->nparts);
     if (!list) return NULL;
@@ -560,6 +663,16 @@ static PyMethodDef methods[] = {
     {"batch_mesh_volume", m_batch_mesh_volume, METH_VARARGS, "Volumes of a batch of meshes."},
     {"test_plane_cut", m_test_plane_cut, METH_VARARGS, "Plane cut with cap triangulation."},
     {"plane_cut_batch", m_plane_cut_batch, METH_VARARGS, "Batched plane cuts: volumes, sizes, status."},
+    {"mlp_n_params", m_mlp_n_params, METH_NOARGS, "Parameter count of the proposal network (padded layout)."},
+    {"mlp_create", m_mlp_create, METH_VARARGS, "Create the proposal network (params pointer or 0, seed)."},
+    {"mlp_destroy", m_mlp_destroy, METH_VARARGS, "Destroy a proposal network."},
+    {"mlp_get_params", m_mlp_get_params, METH_VARARGS, "Copy the FP32 parameters out."},
+    {"mlp_forward", m_mlp_forward, METH_VARARGS, "Predicted scores of candidate planes."},
+    {"mlp_train_step", m_mlp_train_step, METH_VARARGS, "Forward, backward and Adam step; returns the loss."},
+    {"mlp_sample_parts", m_mlp_sample_parts, METH_VARARGS, "Surface samples of parts in their box frames."},
+    {"mlp_gemm_test", m_mlp_gemm_test, METH_VARARGS, "The network's BF16 MFMA GEMM on host operands."},
+    {"lookahead_candidates", m_lookahead_candidates, METH_VARARGS, "Level-0 candidate planes of whole meshes."},
+    {"lookahead_set_proposal", m_lookahead_set_proposal, METH_VARARGS, "Learned proposal for the lookahead."},
     {"set_tuning", m_set_tuning, METH_VARARGS, "Set a hull launch-shape knob (key, value)."},
     {"get_tuning", m_get_tuning, METH_VARARGS, "Read a hull launch-shape knob."},
     {"score_cuts", m_score_cuts, METH_VARARGS, "Batched exact cut scores: cut, hulls, max(0.3 rv, H) per half."},

@@ -137,17 +137,19 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * convex hulls (hull_vert_off/hull_tri_off), decomposed independently with the
  * same parameters in one search whose launches span all meshes. out[m] gets
  * mesh m's parts (free each with gpu_result_free); out_iters[m] (optional) the
- * iterations it ran; out_stats (optional, 71 doubles): [0, 24) per phase -
+ * iterations it ran; out_stats (optional, 77 doubles): [0, 27) per phase -
  * components, part hulls, Hausdorff, concave edges, level-0 cuts, leaf
- * scoring, item hulls, merge - the GPU milliseconds, launches and algorithmic
- * bytes; [24, 32) host wall milliseconds by section - set-up, per-iteration
- * refresh and selection, concave planes, level 0, deeper levels, evaluate and
- * apply, post-loop passes and merge, result readback; [32, 71) per hull
- * kernel class - one-workgroup hulls of <= 64, 128, 256, 512, 1024 points,
- * multi-wave hulls of <= 2048, 4096, 8192, 16384 points, k-DOP prefilter,
- * full-capacity retries of the 512-, 1024- and 2048-point first passes -
- * the milliseconds between HIP events around each launch, launches and
- * algorithmic bytes (points read, volumes written). quick_depth is 0. */
+ * scoring, item hulls, merge, learned proposal - the GPU milliseconds,
+ * launches and algorithmic bytes; [27, 35) host wall milliseconds by section
+ * - set-up, per-iteration refresh and selection, concave planes, level 0,
+ * deeper levels, evaluate and apply, post-loop passes and merge, result
+ * readback; [35, 74) per hull kernel class - one-workgroup hulls of <= 64,
+ * 128, 256, 512, 1024 points, multi-wave hulls of <= 2048, 4096, 8192, 16384
+ * points, k-DOP prefilter, full-capacity retries of the 512-, 1024- and
+ * 2048-point first passes - the milliseconds between HIP events around each
+ * launch, launches and algorithmic bytes (points read, volumes written);
+ * [74, 77) learned proposal: level-0 candidates kept, candidates ranked,
+ * fallbacks (gpu_lookahead_set_proposal).  * quick_depth is 0. */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off,
                         const int* hull_tris, const int* hull_tri_off, int max_iters, int width, int width2,
@@ -232,6 +234,48 @@ int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const
  * half that size first, 1024..16384). Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);
+
+/* Learned cut proposal (no reference counterpart): a point-set network that
+ * predicts the exact cut score of a part's candidate planes. Input: 256
+ * area-stratified surface samples per part in the part's box frame (center,
+ * half the largest extent; gpu_mlp_sample_parts computes them), planes as
+ * (unit normal, offset) in the same frame, part_of ascending. BF16 MFMA
+ * layers with FP32 accumulation; FP32 master weights and Adam state on the
+ * device. Parameters: gpu_mlp_n_params() floats (padded layout, see mlp.hip);
+ * params = NULL draws a seeded He-uniform initialisation. */
+typedef struct gpu_mlp* gpu_mlp_t;
+int gpu_mlp_n_params(void);
+int gpu_mlp_create(gpu_ctx_t ctx, const float* params, int seed, gpu_mlp_t* out);
+void gpu_mlp_destroy(gpu_ctx_t ctx, gpu_mlp_t m);
+int gpu_mlp_get_params(gpu_ctx_t ctx, gpu_mlp_t m, float* out);
+int gpu_mlp_forward(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_parts, const float* planes,
+                    const int* part_of, int n_cand, float* out_scores);
+/* One training step on the mean squared error of the predicted scores against
+ * targets: forward, backward (gradients into out_grads when given), Adam
+ * update with learning rate lr when apply != 0. */
+int gpu_mlp_train_step(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_parts, const float* planes,
+                       const int* part_of, int n_cand, const float* targets, float lr, int apply, float* out_loss,
+                       float* out_grads);
+int gpu_mlp_sample_parts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris, const int* tri_off,
+                         int n_parts, float* out_points);
+/* The network's BF16 MFMA GEMM on host operands (C = A . B, FP32 out; element
+ * strides sam/sak, sbk/sbn; a_len/b_len elements) - checks of its tile layout. */
+int gpu_mlp_gemm_test(gpu_ctx_t ctx, int M, int N, int K, const unsigned short* A, long long sam, long long sak,
+                      long long a_len, const unsigned short* B, long long sbk, long long sbn, long long b_len,
+                      float* C);
+/* The lookahead's learned proposal: with m set, level 0 of each search
+ * iteration evaluates exactly only the top_k candidates per cutting part by
+ * predicted score (all of them when the kept ones yield no cut). m = NULL
+ * restores the plain search. */
+int gpu_lookahead_set_proposal(gpu_ctx_t ctx, gpu_mlp_t m, int top_k);
+/* Level-0 candidate planes of whole meshes exactly as the search builds them
+ * (width axis cuts inside the box margins, then the planes of n_concave
+ * sampled concave edges; reference cuda/la_axis_cut.cuh:10-19,
+ * cuda/la_concave.cu:48-363): mesh m's first out_count[m] (<= cap) planes at
+ * out_planes[4 (m cap + j)], their search index k at out_k[m cap + j]. */
+int gpu_lookahead_candidates(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
+                             const int* tri_off, int width, float threshold, int n_concave, float concave_eps,
+                             float concave_threshold, int cap, float* out_planes, int* out_k, int* out_count);
 
 /* Plans: the same computation on device-resident meshes, prepared once (task
  * table, workspaces, and the launches captured as a HIP graph) and replayed

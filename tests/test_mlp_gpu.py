@@ -1,0 +1,112 @@
+"""GPU tests of the learned cut proposal (cuacd_amd/csrc/mlp.hip): the BF16
+MFMA GEMM against numpy, the network's forward pass and gradients against the
+FP64 oracle with the same BF16 rounding points (oracle/mlp_oracle.py), that
+training lowers the loss, and the learned search itself."""
+import numpy as np
+import pytest
+
+import cuacd_amd
+from cuacd_amd import proposal as PR
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(x):
+    f = np.asarray(x, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def _bits_to_f(b):
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(16, 16, 16, 0, 0), (64, 64, 32, 0, 0), (100, 70, 45, 0, 0),
+                                         (130, 128, 144, 1, 0), (128, 16, 300, 0, 1), (33, 129, 17, 1, 1)])
+def test_gemm_matches_numpy(ctx, M, N, K, ta, tb):
+    rng = np.random.default_rng(M * 7 + N)
+    A = _bf16_bits(rng.normal(size=(K, M) if ta else (M, K)))
+    B = _bf16_bits(rng.normal(size=(N, K) if tb else (K, N)))
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    C = np.empty((M, N), np.float32)
+    cuacd_amd._gpu.mlp_gemm_test(M, N, K, A.ctypes.data, sam, sak, A.size, B.ctypes.data, sbk, sbn, B.size,
+                                 C.ctypes.data)
+    a = _bits_to_f(A)
+    b = _bits_to_f(B)
+    ref = (a.T if ta else a) @ (b.T if tb else b)
+    assert np.allclose(C, ref, rtol=1e-5, atol=1e-4 * np.sqrt(K)), np.abs(C - ref).max()
+
+
+def _batch(seed=0, P=3, C=20):
+    import mlp_oracle as MO
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(-1, 1, (P, MO.NS, 3)).astype(np.float32)
+    planes = rng.normal(size=(C, 4)).astype(np.float32)
+    planes[:, :3] /= np.linalg.norm(planes[:, :3], axis=1, keepdims=True)
+    part_of = np.sort(rng.integers(0, P, C)).astype(np.int32)
+    t = rng.uniform(0, 0.5, C).astype(np.float32)
+    return pts, planes, part_of, t
+
+
+def test_forward_and_gradients_match_oracle(ctx):
+    import mlp_oracle as MO
+    net = PR.ProposalNet(ctx, seed=5)
+    try:
+        params = net.params()
+        assert len(params) == MO.N_PARAMS
+        pts, planes, part_of, t = _batch()
+        y = net.predict(pts, planes, part_of)
+        loss_ref, g_ref, y_ref = MO.loss_and_grads(params.astype(np.float64), pts, planes, part_of, t)
+        assert np.allclose(y, y_ref, rtol=2e-3, atol=2e-3), np.abs(y - y_ref).max()
+        loss, g = net.train_step(pts, planes, part_of, t, apply=False, return_grads=True)
+        assert abs(loss - loss_ref) <= 1e-3 * loss_ref + 1e-6
+        gd = MO.unpack(g.astype(np.float64))
+        rd = MO.unpack(g_ref)
+        for name, _ in MO.SHAPES:
+            a, b = gd[name].reshape(-1), rd[name].reshape(-1)
+            nb = np.linalg.norm(b)
+            if nb == 0:
+                assert np.linalg.norm(a) == 0, name
+                continue
+            rel = np.linalg.norm(a - b) / nb
+            assert rel < 2e-2, (name, rel)
+        # apply=False left the weights alone
+        assert np.array_equal(net.params(), params)
+    finally:
+        net.close()
+
+
+def test_training_lowers_the_loss(ctx):
+    rng = np.random.default_rng(4)
+    P, C = 24, 240
+    pts = rng.uniform(-1, 1, (P, 256, 3)).astype(np.float32)
+    planes = rng.normal(size=(C, 4)).astype(np.float32)
+    planes[:, :3] /= np.linalg.norm(planes[:, :3], axis=1, keepdims=True)
+    part_of = np.sort(rng.integers(0, P, C)).astype(np.int32)
+    # a learnable target: depends on the plane offset and the part's spread
+    spread = pts.std(axis=1).mean(axis=1)
+    t = (0.2 * np.abs(planes[:, 3]) + spread[part_of]).astype(np.float32)
+    net = PR.ProposalNet(ctx, seed=1)
+    try:
+        l0 = net.train_step(pts, planes, part_of, t, apply=False)
+        for _ in range(150):
+            loss = net.train_step(pts, planes, part_of, t, lr=2e-3)
+        assert loss < 0.2 * l0, (l0, loss)
+    finally:
+        net.close()
+
+
+def test_samples_lie_on_the_part_in_its_frame(ctx):
+    from cuacd_amd import datasets
+    v, t = datasets.box_mesh((0, 0, 0), (2, 1, 1))
+    s = PR.sample_parts(ctx, [(v, t)])[0]
+    # frame: center (1, .5, .5), scale 1 -> x in [-1, 1], y, z in [-.5, .5], on the box surface
+    assert s.shape == (256, 3)
+    assert np.all(np.abs(s[:, 0]) <= 1 + 1e-6) and np.all(np.abs(s[:, 1:]) <= 0.5 + 1e-6)
+    on = (np.isclose(np.abs(s[:, 0]), 1, atol=1e-5) | np.isclose(np.abs(s[:, 1]), 0.5, atol=1e-5) |
+          np.isclose(np.abs(s[:, 2]), 0.5, atol=1e-5))
+    assert on.all()
+    # area-stratified: the two 1x1 end faces hold ~2/10 of the area
+    ends = np.isclose(np.abs(s[:, 0]), 1, atol=1e-5).mean()
+    assert 0.14 < ends < 0.26, ends
