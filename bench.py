@@ -6,7 +6,9 @@ extent 2, seed 4), decomposed with lookahead_decompose at tau = 0.05,
 width/width2 30/5, depth 2, quick depth 0, 16 concave edges, 16 parts cut in
 parallel, at most 100 iterations, hull merge on. A step is one batched search
 over all 64 meshes (the meshes are independent; every launch spans all of
-them). Metric: meshes per second.
+them). Metric: meshes per second. After the timed plain steps, one step of
+the learned-proposal form (the shipped network keeps the top 24 level-0
+candidates of each cutting part) is timed and reported under "learned".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--meshes N]
 
@@ -99,6 +101,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["lookahead", "score"], default="lookahead")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--top-k", type=int, default=24,
+                    help="learned-proposal step after the plain steps (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: one GPU only (asked for {args.gpus}); the meshes of the batch are independent and "
@@ -163,6 +167,10 @@ def main():
     d = kern[dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
 
+    learned = None
+    if args.top_k > 0:
+        learned = learned_run(ctx, meshes, ms_per_step, int(n_parts.sum()), args.top_k)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, meshes, iters)
@@ -193,9 +201,45 @@ def main():
                      "avg_launch_ms": d["ms"] / max(d["launches"], 1),
                      "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
         "cpu_baseline": cpu,
+        "learned": learned,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def max_concavity(info):
+    m = 0.0
+    for i in info:
+        rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(i["hull_vol"] - i["mesh_vol"], 0))
+        c = np.maximum(rv, i["hausdorff"])
+        m = max(m, float(c.max()) if len(c) else 0.0)
+    return m
+
+
+def learned_run(ctx, meshes, plain_ms, plain_parts, top_k):
+    """One step of the learned-proposal form (the shipped network ranks each
+    cutting part's level-0 candidates; only the top_k are evaluated exactly)."""
+    from cuacd_amd import proposal as PR
+    net = PR.ProposalNet.load_default(ctx)
+    try:
+        t0 = time.perf_counter()
+        parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, proposal=net, top_k=top_k, **PARAMS)
+        ms = 1e3 * (time.perf_counter() - t0)
+    finally:
+        net.close()
+    pr = ctx.last_lookahead_proposal
+    n_parts = sum(len(p) for p in parts)
+    iters = max(i["iters"] for i in info)
+    out = {"metric": "lookahead meshes/sec at tau=0.05 (learned proposal)", "value": len(meshes) / (ms * 1e-3),
+           "unit": "meshes/s", "ms_per_step": ms, "top_k": top_k, "speedup_vs_plain": plain_ms / ms,
+           "parts_total": n_parts, "parts_vs_plain": n_parts / plain_parts - 1.0,
+           "max_part_concavity": max_concavity(info), "iterations_max": int(iters),
+           "candidates_ranked": pr["ranked"], "candidates_kept": pr["kept"], "fallbacks": pr["fallbacks"],
+           "proposal_ms": ctx.last_lookahead_stats["proposal"]["ms"]}
+    log(f"learned top{top_k}: {ms / 1e3:.2f} s ({out['value']:.1f} meshes/s, x{out['speedup_vs_plain']:.2f}), "
+        f"parts {n_parts} ({100 * out['parts_vs_plain']:+.1f}%), max concavity {out['max_part_concavity']:.4f}, "
+        f"kept {pr['kept']} of {pr['ranked']} candidates, network {out['proposal_ms']:.1f} ms")
+    return out
 
 
 def score_workload(n):

@@ -57,6 +57,7 @@ struct gpu_ctx {
     // learned proposal for the lookahead (null: plain search)
     struct gpu_mlp* proposal;
     int proposal_topk;
+    void* record;  // level-0 training record of the search (lookahead.hip), null when off
 };
 
 namespace cuacd {

@@ -120,6 +120,34 @@ void sort_item_parts(Item& it) {
     }
 }
 
+// Level-0 record of the search for the proposal network's training data:
+// per cutting part its 256 surface samples (box frame) and, per level-0
+// candidate, the plane in the same frame and the candidate's path cost (the
+// value the search minimises; 1e30 when the cut yielded no leaf).
+struct LaRecord {
+    std::vector<float> pts, planes, target;
+    std::vector<int> part_of;
+    int n_parts = 0;
+};
+
+// plane (a, b, c, d) in the box frame of [lo, hi]: unit normal, offset / scale
+void frame_plane(const float lo[3], const float hi[3], const float pl[4], float out[4]) {
+    float sc = 0.0f, ctr[3];
+    for (int j = 0; j < 3; ++j) {
+        ctr[j] = 0.5f * (lo[j] + hi[j]);
+        sc = std::max(sc, 0.5f * (hi[j] - lo[j]));
+    }
+    if (!(sc > 0.0f)) sc = 1.0f;
+    const float len = std::sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+    const float il = len > 0.0f ? 1.0f / len : 0.0f;
+    float dd = pl[3] * il;
+    for (int j = 0; j < 3; ++j) {
+        out[j] = pl[j] * il;
+        dd += pl[j] * il * ctr[j];
+    }
+    out[3] = dd / sc;
+}
+
 struct MeshState {
     std::vector<Part> parts;
     int cc_counter = 0;
@@ -873,6 +901,44 @@ struct Engine {
         return false;
     }
 
+    // append the iteration's level-0 record: samples of each cutting part with
+    // at least two candidates, its candidates' frame planes and path costs
+    int record_level0(const std::vector<std::pair<int, int>>& cutting, const std::vector<std::vector<int>>& rk,
+                      const std::vector<std::vector<float>>& rpl, const std::vector<std::vector<float>>& best) {
+        LaRecord* R = (LaRecord*)ctx->record;
+        std::vector<int> sel;
+        for (size_t c = 0; c < cutting.size(); ++c)
+            if (rk[c].size() >= 2) sel.push_back((int)c);
+        if (sel.empty()) return 0;
+        std::vector<MlpSamplePart> sp(sel.size());
+        for (size_t i = 0; i < sel.size(); ++i) {
+            const Part& p = M[cutting[sel[i]].first].parts[cutting[sel[i]].second];
+            sp[i] = MlpSamplePart{p.v, p.t, p.nv, p.nt, {p.lo[0], p.lo[1], p.lo[2]}, {p.hi[0], p.hi[1], p.hi[2]}};
+        }
+        DevBuf<float> d;
+        REQUIRE(d.alloc(ctx, sel.size() * MLP_SAMPLES * 3) == 0, "lookahead: record buffer allocation failed");
+        LA_TRY(mlp_sample_dev(ctx, sp.data(), (int)sel.size(), d.p));
+        const size_t base = R->pts.size();
+        R->pts.resize(base + sel.size() * MLP_SAMPLES * 3);
+        HIP_TRY(hipMemcpyAsync(R->pts.data() + base, d.p, sizeof(float) * sel.size() * MLP_SAMPLES * 3,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < sel.size(); ++i) {
+            const int c = sel[i];
+            const Part& p = M[cutting[c].first].parts[cutting[c].second];
+            for (size_t j = 0; j < rk[c].size(); ++j) {
+                float fp[4];
+                frame_plane(p.lo, p.hi, &rpl[c][4 * j], fp);
+                R->planes.insert(R->planes.end(), fp, fp + 4);
+                const int k = rk[c][j];
+                R->target.push_back(k < (int)best[c].size() ? best[c][k] : 1e30f);
+                R->part_of.push_back(R->n_parts);
+            }
+            R->n_parts++;
+        }
+        return 0;
+    }
+
     // ---- one search iteration over the meshes in `act`; returns meshes still active
     int iterate(std::vector<int>& act, int iter) {
         if (P.dc_per_iter) LA_TRY(components(act));
@@ -917,6 +983,9 @@ struct Engine {
         }
         lap(S_CONC);
         const int total_levels = P.depth;
+        // level-0 candidates per cutting part, for the training record
+        std::vector<std::vector<int>> rec_k(cutting.size());
+        std::vector<std::vector<float>> rec_pl(cutting.size());
         // the iteration's store of halves built by full cuts
         struct Half {
             float* v;
@@ -1016,37 +1085,26 @@ struct Engine {
             // predicted score are evaluated exactly; the others are the
             // fallback for a part whose kept candidates yield no cut
             std::vector<Cand> rest;
+            if (d == 0 && ctx->record)
+                for (const Cand& cd : cands) {
+                    rec_k[cur_c[cd.item]].push_back(cd.k);
+                    rec_pl[cur_c[cd.item]].insert(rec_pl[cur_c[cd.item]].end(), cd.pl, cd.pl + 4);
+                }
             if (d == 0 && ctx->proposal && ctx->proposal_topk > 0 && !cands.empty()) {
                 const int P = (int)cur.size(), C = (int)cands.size();
                 std::vector<MlpSamplePart> sp(P);
-                std::vector<float> fc(3 * P), fs(P);
                 for (int i = 0; i < P; ++i) {
                     const IPart& wp = cur[i].parts[cur[i].np - 1];
                     sp[i] = MlpSamplePart{wp.v, wp.t, wp.nv, wp.nt, {wp.lo[0], wp.lo[1], wp.lo[2]},
                                           {wp.hi[0], wp.hi[1], wp.hi[2]}};
-                    float sc = 0.0f;
-                    for (int j = 0; j < 3; ++j) {
-                        fc[3 * i + j] = 0.5f * (wp.lo[j] + wp.hi[j]);
-                        sc = std::max(sc, 0.5f * (wp.hi[j] - wp.lo[j]));
-                    }
-                    fs[i] = sc > 0.0f ? sc : 1.0f;
                 }
                 // planes in their part's frame: unit normal, offset
                 std::vector<float> pl(4 * (size_t)C);
                 std::vector<int> po(C);
                 for (int q = 0; q < C; ++q) {
-                    const Cand& cd = cands[q];
-                    const float* n = cd.pl;
-                    const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-                    const float il = len > 0.0f ? 1.0f / len : 0.0f;
-                    const int i = cd.item;
-                    float dd = n[3] * il;
-                    for (int j = 0; j < 3; ++j) {
-                        pl[4 * q + j] = n[j] * il;
-                        dd += n[j] * il * fc[3 * i + j];
-                    }
-                    pl[4 * q + 3] = dd / fs[i];
-                    po[q] = i;
+                    const IPart& wp = cur[cands[q].item].parts[cur[cands[q].item].np - 1];
+                    frame_plane(wp.lo, wp.hi, cands[q].pl, &pl[4 * q]);
+                    po[q] = cands[q].item;
                 }
                 DevBuf<float> dpts, dpl, dsc;
                 DevBuf<int> dpo;
@@ -1376,6 +1434,7 @@ struct Engine {
             if (!L.pruned && L.cut0 >= 0 && L.cut0 < (int)best[L.c].size() && L.path < best[L.c][L.cut0])
                 best[L.c][L.cut0] = L.path;
         for (const Item& x : extra) consider(x, cut_begin[x.mesh] + x.src);
+        if (ctx->record) LA_TRY(record_level0(cutting, rec_k, rec_pl, best));
         std::vector<int> adds_part;
         for (size_t c = 0; c < cutting.size(); ++c) {
             bool any = false;
@@ -1735,4 +1794,29 @@ extern "C" int gpu_lookahead_candidates(gpu_ctx_t ctx, int n_meshes, const float
     (void)hipFreeAsync(d_t, ctx->stream);
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
+}
+
+extern "C" int gpu_lookahead_record(gpu_ctx_t ctx, int on) {
+    if (!ctx) return -1;
+    delete (LaRecord*)ctx->record;
+    ctx->record = on ? new LaRecord() : nullptr;
+    return 0;
+}
+
+extern "C" int gpu_lookahead_record_size(gpu_ctx_t ctx, int* n_parts, int* n_cand) {
+    REQUIRE(ctx->record, "lookahead record: not recording");
+    const LaRecord* R = (const LaRecord*)ctx->record;
+    *n_parts = R->n_parts;
+    *n_cand = (int)R->target.size();
+    return 0;
+}
+
+extern "C" int gpu_lookahead_record_read(gpu_ctx_t ctx, float* pts, float* planes, int* part_of, float* targets) {
+    REQUIRE(ctx->record, "lookahead record: not recording");
+    const LaRecord* R = (const LaRecord*)ctx->record;
+    if (!R->pts.empty()) memcpy(pts, R->pts.data(), sizeof(float) * R->pts.size());
+    if (!R->planes.empty()) memcpy(planes, R->planes.data(), sizeof(float) * R->planes.size());
+    if (!R->part_of.empty()) memcpy(part_of, R->part_of.data(), sizeof(int) * R->part_of.size());
+    if (!R->target.empty()) memcpy(targets, R->target.data(), sizeof(float) * R->target.size());
+    return 0;
 }

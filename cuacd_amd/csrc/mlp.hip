@@ -1,20 +1,23 @@
 // mlp.hip - the learned cut proposal: a point-set network that ranks a part's
 // candidate planes, trained on exact scores the library computes itself.
 //
-// No reference counterpart. Network (PointNet-style): 256 area-stratified
-// surface samples of a part, in the part's box frame (center, half the largest
-// extent), through a shared point MLP 3 -> 128 -> 128 -> 128 (ReLU), max-pooled
-// to a 128-vector g; per candidate plane (unit normal n, offset d in the same
-// frame) a head [g, n, d] -> 128 -> 128 -> 1 (ReLU between) predicts the
-// plane's exact cut score. BF16 operands, FP32 accumulation: every layer is one
+// No reference counterpart. Network (PointNet-style, conditioned on the
+// plane): 256 area-stratified surface samples of a part in the part's box
+// frame (center, half the largest extent); for a candidate plane (unit normal
+// n, offset d in the same frame) every sample gets its signed distance
+// s = n.p + d, and (x, y, z, s) goes through a shared point MLP
+// 4 -> 128 -> 128 -> 128 (ReLU); the features are max-pooled separately over
+// the samples on each side of the plane (g+, g-: the two halves' shapes) and a
+// head [g+, g-, n, d] -> 128 -> 128 -> 1 (ReLU between) predicts the plane's
+// exact cut score. BF16 operands, FP32 accumulation: every layer is one
 // call of a wave64 MFMA GEMM (v_mfma_f32_16x16x16_bf16) with a fused epilogue
 // (bias, ReLU, backward ReLU mask, FP32 and BF16 stores). Training keeps FP32
 // master weights and Adam state on the device and runs forward, backward and
 // the update as kernels (mean squared error on the score).
 //
 // Parameter layout (FP32, padded to multiples of 16; the padding stays zero):
-//   W1 [16 x 128] (rows 0..2 used), b1 [128], W2 [128 x 128], b2, W3, b3,
-//   H1 [144 x 128] (rows 0..131 used), c1 [128], H2 [128 x 128], c2,
+//   W1 [16 x 128] (rows 0..3 used), b1 [128], W2 [128 x 128], b2, W3, b3,
+//   H1 [272 x 128] (rows 0..259 used), c1 [128], H2 [128 x 128], c2,
 //   H3 [128 x 16] (column 0 used), c3 [16] (entry 0 used).
 #include <algorithm>
 #include <cmath>
@@ -23,6 +26,9 @@
 #include "kerr.h"
 #include "mlp.hpp"
 #include "wave.hpp"
+
+// signed distances and sums round as written (the oracle repeats them)
+#pragma clang fp contract(off)
 
 using namespace cuacd;
 
@@ -35,7 +41,7 @@ typedef float f4v_t __attribute__((ext_vector_type(4)));
 constexpr int NS = MLP_SAMPLES;  // samples per part
 constexpr int HD = 128;   // hidden width
 constexpr int KIN = 16;   // padded point input width
-constexpr int KH = 144;   // padded head input width (128 + 4)
+constexpr int KH = 272;   // padded head input width (2 x 128 + 4)
 constexpr int NO = 16;    // padded output width
 
 // offsets of the parameter blocks (floats)
@@ -82,6 +88,8 @@ struct GemmArgs {
     long long ldc;
     bf16_t* Cb;          // BF16 out (row stride ldcb) or null
     long long ldcb;
+    int kchunk;          // split K: workgroup z covers k in [z kchunk, (z + 1) kchunk)
+    float* part;         // split K: raw partial sums [gridDim.z x M x N] (epilogue in the reduction)
 };
 
 constexpr int GM = 64, GN = 64, GK = 32, GKP = GK + 8;  // LDS rows padded by 16 bytes
@@ -98,19 +106,20 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f4v_t{0.f, 0.f, 0.f, 0.f};
     const int lr = tid >> 2, lk = (tid & 3) * 8;  // loader: one row (or column), 8 consecutive k
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
+    const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
+    for (int k0 = kb; k0 < ke; k0 += GK) {
         {
             const int m = m0 + lr;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int k = k0 + lk + j;
-                As[lr][lk + j] = (m < g.M && k < g.K) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (bf16_t)0;
+                As[lr][lk + j] = (m < g.M && k < ke) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (bf16_t)0;
             }
             const int n = n0 + lr;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int k = k0 + lk + j;
-                Bt[lr][lk + j] = (n < g.N && k < g.K) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : (bf16_t)0;
+                Bt[lr][lk + j] = (n < g.N && k < ke) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : (bf16_t)0;
             }
         }
         __syncthreads();
@@ -143,6 +152,10 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
                 if (m >= g.M) continue;
+                if (g.part) {
+                    g.part[((long long)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][r];
+                    continue;
+                }
                 float v = g.alpha * acc[i][j][r];
                 if (g.bias) v += g.bias[n];
                 if (g.relu) v = fmaxf(v, 0.0f);
@@ -153,41 +166,95 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
         }
 }
 
+// Split-K reduction with the GEMM's epilogue: C = alpha * sum_z part[z] ...
+__global__ void mlp_splitk_reduce(GemmArgs g, int nz) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)g.M * g.N) return;
+    const int m = (int)(i / g.N), n = (int)(i % g.N);
+    float v = 0.0f;
+    for (int z = 0; z < nz; ++z) v += g.part[(long long)z * g.M * g.N + i];
+    v *= g.alpha;
+    if (g.bias) v += g.bias[n];
+    if (g.relu) v = fmaxf(v, 0.0f);
+    if (g.mask && !(bf2f(g.mask[(long long)m * g.ldm + n]) > 0.0f)) v = 0.0f;
+    if (g.C) g.C[(long long)m * g.ldc + n] = v;
+    if (g.Cb) g.Cb[(long long)m * g.ldcb + n] = f2bf(v);
+}
+
+// out[n] = sum over rows of D [R x N] (BF16): per block a 1024-row chunk of
+// 128 columns into part[chunk][n], then a sum over the chunks
+__global__ __launch_bounds__(128) void mlp_colsum_part(const bf16_t* D, int R, int N, float* part) {
+    const int n = blockIdx.x * 128 + threadIdx.x, r0 = blockIdx.y * 1024;
+    if (n >= N) return;
+    float s = 0.0f;
+    const int r1 = min(R, r0 + 1024);
+    for (int r = r0; r < r1; ++r) s += bf2f(D[(long long)r * N + n]);
+    part[(long long)blockIdx.y * N + n] = s;
+}
+__global__ void mlp_colsum_fin(const float* part, int chunks, int N, float* out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int c = 0; c < chunks; ++c) s += part[(long long)c * N + n];
+    out[n] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise / reduction kernels
 // ---------------------------------------------------------------------------
-// points [R x 3] FP32 -> X [R x 16] BF16 (zero padded)
-__global__ void mlp_pack_points(const float* pts, int R, bf16_t* X) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)R * KIN) return;
-    const long long r = i / KIN;
-    const int c = (int)(i % KIN);
-    X[i] = c < 3 ? f2bf(pts[3 * r + c]) : (bf16_t)0;
+// X [C*NS x 16] BF16: candidate c's rows are its part's samples with their
+// signed distance to the plane (x, y, z, s, zeros); S [C*NS] the distances
+__global__ void mlp_pack_points(const float* pts, const float* planes, const int* part_of, int C, bf16_t* X,
+                                float* S) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long long)C * NS) return;
+    const long long c = r / NS;
+    const int i = (int)(r % NS);
+    const float* p = pts + ((long long)part_of[c] * NS + i) * 3;
+    const float* pl = planes + 4 * c;
+    const float sd = pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2] + pl[3];
+    S[r] = sd;
+    bf16_t* x = X + r * KIN;
+    x[0] = f2bf(p[0]);
+    x[1] = f2bf(p[1]);
+    x[2] = f2bf(p[2]);
+    x[3] = f2bf(sd);
+    for (int j = 4; j < KIN; ++j) x[j] = 0;
 }
 
-// max over each part's NS rows of A [P*NS x 128] -> G [P x 128] FP32, arg [P x 128]
-__global__ __launch_bounds__(HD) void mlp_maxpool(const bf16_t* A, int P, float* G, int* arg) {
-    const int p = blockIdx.x, j = threadIdx.x;
-    if (p >= P) return;
-    float best = -1e30f;
-    int bi = 0;
-    for (int r = 0; r < NS; ++r) {
-        const float v = bf2f(A[((long long)p * NS + r) * HD + j]);
-        if (v > best) { best = v; bi = r; }
+// per candidate and channel: max of A [C*NS x 128] over its rows with s >= 0
+// (G[c][j]) and with s < 0 (G[c][128 + j]); arg = the row, -1 (and 0) when a
+// side has no sample
+__global__ __launch_bounds__(HD) void mlp_maxpool(const bf16_t* A, const float* S, int C, float* G, int* arg) {
+    const int c = blockIdx.x, j = threadIdx.x;
+    if (c >= C) return;
+    float bp = -1e30f, bn = -1e30f;
+    int ip = -1, in = -1;
+    for (int i = 0; i < NS; ++i) {
+        const long long r = (long long)c * NS + i;
+        const float v = bf2f(A[r * HD + j]);
+        if (S[r] >= 0.0f) {
+            if (v > bp) { bp = v; ip = (int)r; }
+        } else if (v > bn) {
+            bn = v;
+            in = (int)r;
+        }
     }
-    G[(long long)p * HD + j] = best;
-    arg[(long long)p * HD + j] = p * NS + bi;
+    G[(long long)c * 2 * HD + j] = ip >= 0 ? bp : 0.0f;
+    G[(long long)c * 2 * HD + HD + j] = in >= 0 ? bn : 0.0f;
+    arg[(long long)c * 2 * HD + j] = ip;
+    arg[(long long)c * 2 * HD + HD + j] = in;
 }
 
-// head input [C x 144] BF16: g of the candidate's part, plane (n, d), zeros
-__global__ void mlp_head_input(const float* G, const float* planes, const int* part_of, int C, bf16_t* Hin) {
+// head input [C x 272] BF16: g+, g-, plane (n, d), zeros
+__global__ void mlp_head_input(const float* G, const float* planes, int C, bf16_t* Hin) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)C * KH) return;
     const long long c = i / KH;
     const int j = (int)(i % KH);
     float v = 0.0f;
-    if (j < HD) v = G[(long long)part_of[c] * HD + j];
-    else if (j < HD + 4) v = planes[4 * c + (j - HD)];
+    if (j < 2 * HD) v = G[c * 2 * HD + j];
+    else if (j < 2 * HD + 4) v = planes[4 * c + (j - 2 * HD)];
     Hin[i] = f2bf(v);
 }
 
@@ -206,34 +273,21 @@ __global__ __launch_bounds__(256) void mlp_loss(const float* Y, const float* t, 
     if (threadIdx.x == 0) loss[0] = (red[0] + red[1] + red[2] + red[3]) / (float)C;
 }
 
-// out[n] = sum over rows of D [R x N] (BF16), one thread per column
-__global__ void mlp_colsum(const bf16_t* D, int R, int N, float* out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.0f;
-    for (int r = 0; r < R; ++r) s += bf2f(D[(long long)r * N + n]);
-    out[n] = s;
-}
 
-// dG [P x 128] = sum over each part's candidates of dHin[:, :128] (FP32 dHin)
-__global__ __launch_bounds__(HD) void mlp_seg_grad(const float* dHin, const int* cand_off, int P, float* dG) {
-    const int p = blockIdx.x, j = threadIdx.x;
-    if (p >= P) return;
-    float s = 0.0f;
-    for (int c = cand_off[p]; c < cand_off[p + 1]; ++c) s += dHin[(long long)c * KH + j];
-    dG[(long long)p * HD + j] = s;
-}
-
-// dZ3 [P*NS x 128] BF16: dG scattered to the max rows (where the max was positive)
-__global__ void mlp_unpool(const float* dG, const int* arg, const bf16_t* A3, int P, bf16_t* dZ) {
+// dZ3 [C*NS x 128] BF16: each pooled gradient dHin[c][0, 256) back to its
+// max row, where the max was positive (ReLU)
+__global__ void mlp_unpool(const float* dHin, const int* arg, const bf16_t* A3, int C, bf16_t* dZ) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long R = (long long)P * NS;
+    const long long R = (long long)C * NS;
     if (i >= R * HD) return;
     const long long r = i / HD;
     const int j = (int)(i % HD);
-    const int p = (int)(r / NS);
+    const long long c = r / NS;
     float v = 0.0f;
-    if (arg[(long long)p * HD + j] == r && bf2f(A3[i]) > 0.0f) v = dG[(long long)p * HD + j];
+    if (bf2f(A3[i]) > 0.0f) {
+        if (arg[c * 2 * HD + j] == r) v += dHin[c * KH + j];
+        if (arg[c * 2 * HD + HD + j] == r) v += dHin[c * KH + HD + j];
+    }
     dZ[i] = f2bf(v);
 }
 
@@ -364,9 +418,11 @@ struct gpu_mlp {
     float* m;      // Adam moments
     float* v;
     int step;
-    // grow-only activations
+    // grow-only activations and split-K / column-sum partials
     char* act;
     size_t act_cap;
+    float* part;
+    size_t part_cap;
 };
 
 namespace {
@@ -377,32 +433,73 @@ template <class T> T* carve(char*& p, size_t n) {
     return r;
 }
 
+// network whose split-K workspace gemm() uses, set by each entry point (the
+// library's calls on one context are not concurrent)
+gpu_mlp* g_cur = nullptr;
+
+// split-K workspace of at least n floats (grow-only; a growth waits for the
+// stream so no launch still reads the old buffer)
+float* part_ws(gpu_ctx_t ctx, size_t n) {
+    gpu_mlp* mp = g_cur;
+    if (!mp) return nullptr;
+    if (n > mp->part_cap) {
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
+        if (mp->part) (void)hipFree(mp->part);
+        mp->part = nullptr;
+        mp->part_cap = 0;
+        if (hipMalloc((void**)&mp->part, n * sizeof(float)) != hipSuccess) return nullptr;
+        mp->part_cap = n;
+    }
+    return mp->part;
+}
+
 int gemm(gpu_ctx_t ctx, int M, int N, int K, const bf16_t* A, long long sam, long long sak, const bf16_t* B,
          long long sbk, long long sbn, const float* bias, int relu, const bf16_t* mask, long long ldm, float* C,
          long long ldc, bf16_t* Cb, long long ldcb, float alpha = 1.0f) {
     if (M <= 0 || N <= 0) return 0;
-    GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, alpha, bias, relu, mask, ldm, C, ldc, Cb, ldcb};
-    dim3 grid((unsigned)((N + GN - 1) / GN), (unsigned)((M + GM - 1) / GM));
-    hipLaunchKernelGGL(mlp_gemm_kernel, grid, dim3(256), 0, ctx->stream, g);
+    GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, alpha, bias, relu, mask, ldm, C, ldc, Cb, ldcb, K, nullptr};
+    const int tm = (M + GM - 1) / GM, tn = (N + GN - 1) / GN;
+    // few output tiles over a long K (the weight gradients): split K across
+    // workgroups, partial sums reduced by a second kernel with the epilogue
+    int nz = 1;
+    if (tm * tn < 128 && K >= 4096) {
+        nz = std::min((1024 + tm * tn - 1) / (tm * tn), (K + 511) / 512);
+        const int kc = ((K + nz - 1) / nz + GK - 1) / GK * GK;
+        nz = (K + kc - 1) / kc;
+        g.kchunk = kc;
+        g.part = part_ws(ctx, (size_t)nz * M * N);
+        REQUIRE(g.part, "mlp: split-K workspace allocation failed");
+    }
+    hipLaunchKernelGGL(mlp_gemm_kernel, dim3((unsigned)tn, (unsigned)tm, (unsigned)nz), dim3(256), 0, ctx->stream, g);
     HIP_TRY(hipGetLastError());
+    if (nz > 1) {
+        const long long n = (long long)M * N;
+        hipLaunchKernelGGL(mlp_splitk_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, g, nz);
+        HIP_TRY(hipGetLastError());
+    }
     return 0;
 }
 
 struct Act {
     bf16_t *X, *A1, *A2, *A3, *Hin, *B1, *B2, *dY, *dZ2h, *dZ1h, *dA3, *dZ2e, *dZ1e;
-    float *G, *Y, *dHin, *dG, *loss;
+    float *S, *G, *Y, *dHin, *loss, *csum;
+    size_t csum_cap;
     int* arg;
 };
 
-int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int P, int C, Act* a) {
-    const size_t R = (size_t)P * NS;
+// activations of C candidates (C * NS encoder rows); `train` adds the
+// backward buffers
+int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int C, bool train, Act* a) {
+    const size_t R = (size_t)C * NS;
     size_t need = 0;
     auto sz = [&](size_t n, size_t e) { need += (n * e + 255) / 256 * 256; };
     sz(R * KIN, 2); sz(R * HD, 2); sz(R * HD, 2); sz(R * HD, 2);
     sz((size_t)C * KH, 2); sz((size_t)C * HD, 2); sz((size_t)C * HD, 2); sz((size_t)C * NO, 2);
-    sz((size_t)C * HD, 2); sz((size_t)C * HD, 2); sz(R * HD, 2); sz(R * HD, 2); sz(R * HD, 2);
-    sz((size_t)P * HD, 4); sz((size_t)C * NO, 4); sz((size_t)C * KH, 4); sz((size_t)P * HD, 4); sz(64, 4);
-    sz((size_t)P * HD, 4);
+    if (train) { sz((size_t)C * HD, 2); sz((size_t)C * HD, 2); sz(R * HD, 2); sz(R * HD, 2); sz(R * HD, 2); }
+    sz(R, 4); sz((size_t)C * 2 * HD, 4); sz((size_t)C * NO, 4); sz((size_t)C * KH, 4); sz(64, 4);
+    sz((size_t)C * 2 * HD, 4);
+    const size_t csum = ((R + 1023) / 1024 + 1) * (size_t)HD;
+    if (train) sz(csum, 4);
     if (need > mp->act_cap) {
         if (mp->act) HIP_TRY(hipFree(mp->act));
         mp->act = nullptr;
@@ -411,6 +508,7 @@ int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int P, int C, Act* a) {
         mp->act_cap = need;
     }
     char* p = mp->act;
+    *a = Act{};
     a->X = carve<bf16_t>(p, R * KIN);
     a->A1 = carve<bf16_t>(p, R * HD);
     a->A2 = carve<bf16_t>(p, R * HD);
@@ -419,44 +517,47 @@ int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int P, int C, Act* a) {
     a->B1 = carve<bf16_t>(p, (size_t)C * HD);
     a->B2 = carve<bf16_t>(p, (size_t)C * HD);
     a->dY = carve<bf16_t>(p, (size_t)C * NO);
-    a->dZ2h = carve<bf16_t>(p, (size_t)C * HD);
-    a->dZ1h = carve<bf16_t>(p, (size_t)C * HD);
-    a->dA3 = carve<bf16_t>(p, R * HD);
-    a->dZ2e = carve<bf16_t>(p, R * HD);
-    a->dZ1e = carve<bf16_t>(p, R * HD);
-    a->G = carve<float>(p, (size_t)P * HD);
+    if (train) {
+        a->dZ2h = carve<bf16_t>(p, (size_t)C * HD);
+        a->dZ1h = carve<bf16_t>(p, (size_t)C * HD);
+        a->dA3 = carve<bf16_t>(p, R * HD);
+        a->dZ2e = carve<bf16_t>(p, R * HD);
+        a->dZ1e = carve<bf16_t>(p, R * HD);
+    }
+    a->S = carve<float>(p, R);
+    a->G = carve<float>(p, (size_t)C * 2 * HD);
     a->Y = carve<float>(p, (size_t)C * NO);
     a->dHin = carve<float>(p, (size_t)C * KH);
-    a->dG = carve<float>(p, (size_t)P * HD);
     a->loss = carve<float>(p, 64);
-    a->arg = carve<int>(p, (size_t)P * HD);
+    a->arg = carve<int>(p, (size_t)C * 2 * HD);
+    if (train) {
+        a->csum = carve<float>(p, csum);
+        a->csum_cap = csum;
+    }
     return 0;
 }
 
 // forward over device inputs: points [P*NS x 3], planes [C x 4], part_of [C]
-int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* d_planes, const int* d_part, int C,
+int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, const float* d_planes, const int* d_part, int C,
             Act& a) {
-    const int R = P * NS;
+    const int R = C * NS;
     const bf16_t* W = mp->wb;
     const float* Wf = mp->w;
-    {
-        const long long n = (long long)R * KIN;
-        hipLaunchKernelGGL(mlp_pack_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, d_pts, R,
-                           a.X);
-        HIP_TRY(hipGetLastError());
-    }
+    hipLaunchKernelGGL(mlp_pack_points, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, ctx->stream, d_pts,
+                       d_planes, d_part, C, a.X, a.S);
+    HIP_TRY(hipGetLastError());
     if (int rc = gemm(ctx, R, HD, KIN, a.X, KIN, 1, W + P_W1, HD, 1, Wf + P_B1, 1, nullptr, 0, nullptr, 0, a.A1, HD))
         return rc;
     if (int rc = gemm(ctx, R, HD, HD, a.A1, HD, 1, W + P_W2, HD, 1, Wf + P_B2, 1, nullptr, 0, nullptr, 0, a.A2, HD))
         return rc;
     if (int rc = gemm(ctx, R, HD, HD, a.A2, HD, 1, W + P_W3, HD, 1, Wf + P_B3, 1, nullptr, 0, nullptr, 0, a.A3, HD))
         return rc;
-    hipLaunchKernelGGL(mlp_maxpool, dim3((unsigned)P), dim3(HD), 0, ctx->stream, a.A3, P, a.G, a.arg);
+    hipLaunchKernelGGL(mlp_maxpool, dim3((unsigned)C), dim3(HD), 0, ctx->stream, a.A3, a.S, C, a.G, a.arg);
     HIP_TRY(hipGetLastError());
     {
         const long long n = (long long)C * KH;
         hipLaunchKernelGGL(mlp_head_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a.G,
-                           d_planes, d_part, C, a.Hin);
+                           d_planes, C, a.Hin);
         HIP_TRY(hipGetLastError());
     }
     if (int rc = gemm(ctx, C, HD, KH, a.Hin, KH, 1, W + P_H1, HD, 1, Wf + P_C1, 1, nullptr, 0, nullptr, 0, a.B1, HD))
@@ -467,14 +568,24 @@ int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* 
 }
 
 // gradients of the mean squared error into mp->g (after forward)
-int backward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_t, const int* d_cand_off, int P, int C, Act& a) {
-    const int R = P * NS;
+int backward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_t, int C, Act& a) {
+    const int R = C * NS;
+    // column sums use the part of the activation block reserved for them
+    float* colsum_ws = a.csum;
+    const size_t colsum_cap = a.csum_cap;
     const bf16_t* W = mp->wb;
     float* g = mp->g;
     hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, ctx->stream, a.Y, d_t, C, a.dY, a.loss);
     HIP_TRY(hipGetLastError());
     auto colsum = [&](const bf16_t* D, int rows, int N, float* out) -> int {
-        hipLaunchKernelGGL(mlp_colsum, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, ctx->stream, D, rows, N, out);
+        const int chunks = (rows + 1023) / 1024;
+        float* part = colsum_ws;
+        REQUIRE(part && (size_t)chunks * N <= colsum_cap, "mlp: column-sum workspace too small");
+        hipLaunchKernelGGL(mlp_colsum_part, dim3((unsigned)((N + 127) / 128), (unsigned)chunks), dim3(128), 0,
+                           ctx->stream, D, rows, N, part);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(mlp_colsum_fin, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, ctx->stream, part, chunks,
+                           N, out);
         HIP_TRY(hipGetLastError());
         return 0;
     };
@@ -494,13 +605,11 @@ int backward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_t, const int* d_cand_off
     if (int rc = colsum(a.dZ1h, C, HD, g + P_C1)) return rc;
     if (int rc = gemm(ctx, C, KH, HD, a.dZ1h, HD, 1, W + P_H1, 1, HD, nullptr, 0, nullptr, 0, a.dHin, KH, nullptr, 0))
         return rc;
-    hipLaunchKernelGGL(mlp_seg_grad, dim3((unsigned)P), dim3(HD), 0, ctx->stream, a.dHin, d_cand_off, P, a.dG);
-    HIP_TRY(hipGetLastError());
     // encoder
     {
         const long long n = (long long)R * HD;
-        hipLaunchKernelGGL(mlp_unpool, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a.dG, a.arg,
-                           a.A3, P, a.dA3);
+        hipLaunchKernelGGL(mlp_unpool, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a.dHin, a.arg,
+                           a.A3, C, a.dA3);
         HIP_TRY(hipGetLastError());
     }
     if (int rc = gemm(ctx, HD, HD, R, a.A2, 1, HD, a.dA3, HD, 1, nullptr, 0, nullptr, 0, g + P_W3, HD, nullptr, 0))
@@ -573,10 +682,17 @@ int stage_err(gpu_ctx_t ctx, int rc) {
 namespace cuacd {
 int mlp_forward_dev(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* d_planes, const int* d_part,
                     int C, float* d_out) {
+    g_cur = mp;
+    // bounded activations: slices of at most 8192 candidates (2M encoder rows)
+    (void)P;
+    const int slice = 8192;
     Act a;
-    if (int rc = act_layout(ctx, mp, P, C, &a)) return rc;
-    if (int rc = forward(ctx, mp, d_pts, P, d_planes, d_part, C, a)) return rc;
-    HIP_TRY(hipMemcpy2DAsync(d_out, 4, a.Y, 4 * NO, 4, C, hipMemcpyDeviceToDevice, ctx->stream));
+    if (int rc = act_layout(ctx, mp, std::min(slice, C), false, &a)) return rc;
+    for (int c0 = 0; c0 < C; c0 += slice) {
+        const int n = std::min(slice, C - c0);
+        if (int rc = forward(ctx, mp, d_pts, d_planes + 4ll * c0, d_part + c0, n, a)) return rc;
+        HIP_TRY(hipMemcpy2DAsync(d_out + c0, 4, a.Y, 4 * NO, 4, n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     return 0;
 }
 
@@ -629,10 +745,10 @@ int gpu_mlp_create(gpu_ctx_t ctx, const float* params, int seed, gpu_mlp_t* out)
                 for (int c = 0; c < cols; ++c)
                     w[off + r * cols + c] = (r < rows_used && c < cols_used) ? (float)((2.0 * rnd() - 1.0) * a) : 0.0f;
         };
-        fill(P_W1, 3, KIN, HD, HD, 3);
+        fill(P_W1, 4, KIN, HD, HD, 4);
         fill(P_W2, HD, HD, HD, HD, HD);
         fill(P_W3, HD, HD, HD, HD, HD);
-        fill(P_H1, HD + 4, KH, HD, HD, HD + 4);
+        fill(P_H1, 2 * HD + 4, KH, HD, HD, 2 * HD + 4);
         fill(P_H2, HD, HD, HD, HD, HD);
         fill(P_H3, HD, HD, 1, NO, HD);
     }
@@ -666,6 +782,8 @@ void gpu_mlp_destroy(gpu_ctx_t ctx, gpu_mlp_t m) {
     if (m->m) (void)hipFree(m->m);
     if (m->v) (void)hipFree(m->v);
     if (m->act) (void)hipFree(m->act);
+    if (m->part) (void)hipFree(m->part);
+    if (g_cur == m) g_cur = nullptr;
     free(m);
 }
 
@@ -684,12 +802,15 @@ int gpu_mlp_forward(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_parts
     Staged s;
     std::vector<int> coff;
     if (int rc = s.load(ctx, points, n_parts, planes, part_of, n_cand, nullptr, coff)) return stage_err(ctx, rc);
-    Act a;
-    if (int rc = act_layout(ctx, m, n_parts, n_cand, &a)) return rc;
-    if (int rc = forward(ctx, m, s.pts, n_parts, s.planes, s.part, n_cand, a)) return rc;
-    HIP_TRY(hipMemcpy2DAsync(out_scores, 4, a.Y, 4 * NO, 4, n_cand, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return 0;
+    float* d_out = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_out, 4ll * n_cand + 16, ctx->stream));
+    int rc = mlp_forward_dev(ctx, m, s.pts, n_parts, s.planes, s.part, n_cand, d_out);
+    if (rc == 0) {
+        HIP_TRY(hipMemcpyAsync(out_scores, d_out, 4ll * n_cand, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    (void)hipFreeAsync(d_out, ctx->stream);
+    return rc;
 }
 
 int gpu_mlp_train_step(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_parts, const float* planes,
@@ -700,10 +821,11 @@ int gpu_mlp_train_step(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_pa
     Staged s;
     std::vector<int> coff;
     if (int rc = s.load(ctx, points, n_parts, planes, part_of, n_cand, targets, coff)) return stage_err(ctx, rc);
+    g_cur = m;
     Act a;
-    if (int rc = act_layout(ctx, m, n_parts, n_cand, &a)) return rc;
-    if (int rc = forward(ctx, m, s.pts, n_parts, s.planes, s.part, n_cand, a)) return rc;
-    if (int rc = backward(ctx, m, s.t, s.coff, n_parts, n_cand, a)) return rc;
+    if (int rc = act_layout(ctx, m, n_cand, true, &a)) return rc;
+    if (int rc = forward(ctx, m, s.pts, s.planes, s.part, n_cand, a)) return rc;
+    if (int rc = backward(ctx, m, s.t, n_cand, a)) return rc;
     if (out_grads)
         HIP_TRY(hipMemcpyAsync(out_grads, m->g, sizeof(float) * P_TOTAL, hipMemcpyDeviceToHost, ctx->stream));
     if (apply) {
@@ -780,7 +902,11 @@ int gpu_mlp_gemm_test(gpu_ctx_t ctx, int M, int N, int K, const unsigned short* 
     HIP_TRY(hipMallocAsync((void**)&dC, 4ll * M * N, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dA, A, 2 * a_len, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dB, B, 2 * b_len, hipMemcpyHostToDevice, ctx->stream));
+    gpu_mlp tmp{};  // holds the split-K workspace of this call
+    gpu_mlp* prev = g_cur;
+    g_cur = &tmp;
     int rc = gemm(ctx, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, 0, nullptr, 0, dC, N, nullptr, 0);
+    g_cur = prev;
     if (rc == 0) {
         HIP_TRY(hipMemcpyAsync(C, dC, 4ll * M * N, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -788,6 +914,8 @@ int gpu_mlp_gemm_test(gpu_ctx_t ctx, int M, int N, int K, const unsigned short* 
     (void)hipFreeAsync(dA, ctx->stream);
     (void)hipFreeAsync(dB, ctx->stream);
     (void)hipFreeAsync(dC, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (tmp.part) (void)hipFree(tmp.part);
     return rc;
 }
 

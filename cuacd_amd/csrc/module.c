@@ -323,6 +323,32 @@ static PyObject* m_lookahead_candidates(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* m_lookahead_record(PyObject* self, PyObject* args) {
+    int on;
+    if (!PyArg_ParseTuple(args, "i", &on)) return NULL;
+    NEED_CTX();
+    int rc = gpu_lookahead_record(g_ctx, on);
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
+static PyObject* m_lookahead_record_size(PyObject* self, PyObject* args) {
+    NEED_CTX();
+    int np = 0, nc = 0;
+    int rc = gpu_lookahead_record_size(g_ctx, &np, &nc);
+    if (rc) return raise_error(rc);
+    return Py_BuildValue("ii", np, nc);
+}
+
+static PyObject* m_lookahead_record_read(PyObject* self, PyObject* args) {
+    unsigned long long a, b, c, d;
+    if (!PyArg_ParseTuple(args, "KKKK", &a, &b, &c, &d)) return NULL;
+    NEED_CTX();
+    int rc = gpu_lookahead_record_read(g_ctx, P(float*, a), P(float*, b), P(int*, c), P(float*, d));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+
 static PyObject* part_list(struct gpu_result* res) {
     PyObject* list = PyList_New(res->nparts);
     if (!list) return NULL;
@@ -671,6 +697,9 @@ static PyMethodDef methods[] = {
     {"mlp_train_step", m_mlp_train_step, METH_VARARGS, "Forward, backward and Adam step; returns the loss."},
     {"mlp_sample_parts", m_mlp_sample_parts, METH_VARARGS, "Surface samples of parts in their box frames."},
     {"mlp_gemm_test", m_mlp_gemm_test, METH_VARARGS, "The network's BF16 MFMA GEMM on host operands."},
+    {"lookahead_record", m_lookahead_record, METH_VARARGS, "Start (1) or discard (0) the level-0 training record."},
+    {"lookahead_record_size", m_lookahead_record_size, METH_NOARGS, "(parts, candidates) recorded."},
+    {"lookahead_record_read", m_lookahead_record_read, METH_VARARGS, "Copy the record out."},
     {"lookahead_candidates", m_lookahead_candidates, METH_VARARGS, "Level-0 candidate planes of whole meshes."},
     {"lookahead_set_proposal", m_lookahead_set_proposal, METH_VARARGS, "Learned proposal for the lookahead."},
     {"set_tuning", m_set_tuning, METH_VARARGS, "Set a hull launch-shape knob (key, value)."},

@@ -7,14 +7,16 @@ evaluates exactly only the top_k candidates of each cutting part, falling
 back to the others when the kept ones yield no cut. Every accepted cut is
 still the exact search's choice among the evaluated ones.
 
-Training data are triples (part, plane, exact score) the exact pipeline
-itself produces: parts are the pieces of plain lookahead runs stopped after
-0..k iterations on seeded SDF meshes, planes the search's own level-0
-candidates for them (Context.lookahead_candidates), scores the batched exact
-scorer's (Context.batch_score_cuts: the larger max(0.3 rv, H) of the two
-halves; failed cuts score 1). Training runs forward, backward and Adam on the
-GPU (mean squared error).
+Training data are triples (part, plane, exact concavity) the exact pipeline
+itself produces. record_dataset: plain lookahead runs on seeded SDF meshes
+record, for every cutting part of every iteration, its samples and each
+level-0 candidate's path cost - the exact value the search minimises when it
+picks the cut. make_dataset: parts and their level-0 candidates scored by the
+batched exact scorer (the larger max(0.3 rv, H) of the two halves). Failed
+cuts score 1. Training runs forward, backward and Adam on the GPU (mean
+squared error against each part's candidate ranks).
 """
+import os
 import time
 
 import numpy as np
@@ -22,6 +24,8 @@ import numpy as np
 from . import _gpu
 
 NS = 256            # surface samples per part
+# weights trained by `python -m cuacd_amd.train_proposal` (shipped with the package)
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "proposal_weights.npz")
 FAIL_SCORE = 1.0    # target of a cut that fails or leaves a side empty
 
 
@@ -59,6 +63,10 @@ class ProposalNet:
 
     def save(self, path):
         np.savez(path, params=self.params())
+
+    @classmethod
+    def load_default(cls, ctx):
+        return cls.load(ctx, DEFAULT_WEIGHTS)
 
     @classmethod
     def load(cls, ctx, path):
@@ -165,6 +173,25 @@ def training_parts(ctx, meshes, iters=(0, 1, 2), **search):
     return parts
 
 
+def record_dataset(ctx, meshes, **search):
+    """(points, planes, part_of, targets) recorded by plain lookahead runs:
+    every cutting part of every iteration with its level-0 candidates' path
+    costs."""
+    _gpu.lookahead_record(1)
+    try:
+        ctx.batch_lookahead_decompose(meshes, **search)
+        n_parts, n_cand = _gpu.lookahead_record_size()
+        pts = np.empty((n_parts, NS, 3), np.float32)
+        pl = np.empty((n_cand, 4), np.float32)
+        po = np.empty(n_cand, np.int32)
+        tg = np.empty(n_cand, np.float32)
+        _gpu.lookahead_record_read(pts.ctypes.data, pl.ctypes.data, po.ctypes.data, tg.ctypes.data)
+    finally:
+        _gpu.lookahead_record(0)
+    tg = np.where(tg >= 1e29, FAIL_SCORE, tg).astype(np.float32)
+    return pts, pl, po, tg
+
+
 def make_dataset(ctx, parts, threshold=0.05, width=30, n_concave_edges=16, chunk=256, log=None):
     """(points [P, 256, 3], planes [C, 4] in part frames, part_of [C], targets [C])."""
     pts, pls, pos, tgs = [], [], [], []
@@ -197,9 +224,27 @@ def _batches(part_of, n_parts, batch_parts, rng):
         yield sel, idx, remap[part_of[idx]]
 
 
-def train(ctx, data, epochs=30, batch_parts=64, lr=1e-3, seed=0, net=None, log=None):
-    """Adam on the mean squared error; returns (net, per-epoch mean losses)."""
+def rank_targets(part_of, scores):
+    """Per part, each candidate's rank by exact score scaled to [0, 1] (0 =
+    best; ties share the lower rank): what the proposal has to get right is
+    the order inside a part, not the part's overall cost level."""
+    t = np.zeros(len(scores), np.float32)
+    starts = np.searchsorted(part_of, np.arange(part_of.max() + 2)) if len(part_of) else [0]
+    for a, b in zip(starts[:-1], starts[1:]):
+        if b - a <= 1:
+            continue
+        s = scores[a:b]
+        r = np.searchsorted(np.sort(s, kind="stable"), s, side="left")
+        t[a:b] = r / float(b - a - 1)
+    return t
+
+
+def train(ctx, data, epochs=30, batch_parts=64, lr=1e-3, seed=0, net=None, log=None, target="rank"):
+    """Adam on the mean squared error against per-part rank targets (or the
+    raw scores with target="score"); returns (net, per-epoch mean losses)."""
     pts, pl, po, tg = data
+    if target == "rank":
+        tg = rank_targets(po, tg)
     rng = np.random.default_rng(seed)
     net = net or ProposalNet(ctx, seed=seed)
     hist = []

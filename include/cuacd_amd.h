@@ -268,6 +268,14 @@ int gpu_mlp_gemm_test(gpu_ctx_t ctx, int M, int N, int K, const unsigned short* 
  * predicted score (all of them when the kept ones yield no cut). m = NULL
  * restores the plain search. */
 int gpu_lookahead_set_proposal(gpu_ctx_t ctx, gpu_mlp_t m, int top_k);
+/* Training record of the search (no reference counterpart): while on, every
+ * iteration appends, per cutting part, its 256 surface samples (box frame) and
+ * per level-0 candidate the plane in that frame and the candidate's path cost
+ * (what the search minimises; 1e30 when the cut gave no leaf). record(ctx, 0)
+ * discards it; size/read copy it out (pts n_parts*256*3, planes n_cand*4). */
+int gpu_lookahead_record(gpu_ctx_t ctx, int on);
+int gpu_lookahead_record_size(gpu_ctx_t ctx, int* n_parts, int* n_cand);
+int gpu_lookahead_record_read(gpu_ctx_t ctx, float* pts, float* planes, int* part_of, float* targets);
 /* Level-0 candidate planes of whole meshes exactly as the search builds them
  * (width axis cuts inside the box margins, then the planes of n_concave
  * sampled concave edges; reference cuda/la_axis_cut.cuh:10-19,

@@ -12,8 +12,8 @@ def _instance(seed=0, P=2, C=7):
     p = MO.unpack(np.zeros(MO.N_PARAMS))
     for name, s in MO.SHAPES:
         p[name] = rng.normal(0, 0.3, s)
-    p["W1"][3:] = 0
-    p["H1"][132:] = 0
+    p["W1"][4:] = 0
+    p["H1"][260:] = 0
     p["H3"][:, 1:] = 0
     p["c3"][1:] = 0
     params = MO.pack(p)
