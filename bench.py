@@ -239,6 +239,11 @@ def learned_run(ctx, meshes, plain_ms, plain_parts, top_k):
     log(f"learned top{top_k}: {ms / 1e3:.2f} s ({out['value']:.1f} meshes/s, x{out['speedup_vs_plain']:.2f}), "
         f"parts {n_parts} ({100 * out['parts_vs_plain']:+.1f}%), max concavity {out['max_part_concavity']:.4f}, "
         f"kept {pr['kept']} of {pr['ranked']} candidates, network {out['proposal_ms']:.1f} ms")
+    st = ctx.last_lookahead_stats
+    log("  learned phases: " + ", ".join(f"{k} {v['ms']:.0f} ms" for k, v in st.items()) +
+        f" (GPU {sum(v['ms'] for v in st.values()) / 1e3:.2f} s)")
+    sec = ctx.last_lookahead_sections
+    log("  learned host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()))
     return out
 
 
