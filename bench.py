@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["lookahead", "score"], default="lookahead")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run the known-size device copy (HBM counter calibration) and exit")
     ap.add_argument("--top-k", type=int, default=24,
                     help="learned-proposal step after the plain steps (0 skips it)")
     args = ap.parse_args()
@@ -111,6 +113,14 @@ def main():
 
     import cuacd_amd
 
+    if args.copy_probe:
+        ctx = cuacd_amd.Context(device=0)
+        nb = args.copy_probe << 20
+        ms = cuacd_amd._gpu.copy_probe(nb, 3)
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 3, "ms_per_copy": ms,
+                          "gbs": 2 * nb / (ms * 1e-3) / 1e9}), flush=True)
+        ctx.close()
+        return
     if args.workload == "score":
         return main_score(args)
     t0 = time.perf_counter()

@@ -1688,3 +1688,45 @@ int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Plain device copy of `bytes` (a multiple of 4) through la_segcopy_kernel in
+// 1 MiB segments, `reps` times: the known-size copy that calibrates HBM
+// traffic counters. *ms = average GPU milliseconds per copy.
+int gpu_copy_probe(gpu_ctx_t ctx, size_t bytes, int reps, float* ms) {
+    REQUIRE(bytes >= 4 && bytes % 4 == 0 && reps >= 1, "copy_probe: bad arguments");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t seg = 1u << 20;
+    const size_t n = (bytes + seg - 1) / seg;
+    char* src = nullptr;
+    char* dst = nullptr;
+    LaSeg* d_segs = nullptr;
+    HIP_TRY(hipMalloc((void**)&src, bytes));
+    HIP_TRY(hipMalloc((void**)&dst, bytes));
+    HIP_TRY(hipMalloc((void**)&d_segs, sizeof(LaSeg) * n));
+    std::vector<LaSeg> h(n);
+    for (size_t i = 0; i < n; ++i)
+        h[i] = LaSeg{src + i * seg, dst + i * seg, (unsigned long long)std::min(seg, bytes - i * seg)};
+    HIP_TRY(hipMemcpy(d_segs, h.data(), sizeof(LaSeg) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(src, 1, bytes));
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, ctx->stream));
+    int rc = 0;
+    for (int r = 0; r < reps && rc == 0; ++r) rc = la_launch_segcopy(ctx, d_segs, (int)n);
+    HIP_TRY(hipEventRecord(b, ctx->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, a, b));
+    *ms = t / (float)reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    (void)hipFree(d_segs);
+    return rc;
+}
+
+}  // extern "C"

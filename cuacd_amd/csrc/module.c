@@ -349,6 +349,17 @@ static PyObject* m_lookahead_record_read(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* m_copy_probe(PyObject* self, PyObject* args) {
+    unsigned long long bytes;
+    int reps;
+    if (!PyArg_ParseTuple(args, "Ki", &bytes, &reps)) return NULL;
+    NEED_CTX();
+    float ms = 0.0f;
+    int rc = gpu_copy_probe(g_ctx, (size_t)bytes, reps, &ms);
+    if (rc) return raise_error(rc);
+    return PyFloat_FromDouble(ms);
+}
+
 static PyObject* part_list(struct gpu_result* res) {
     PyObject* list = PyList_New(res->nparts);
     if (!list) return NULL;
@@ -702,6 +713,7 @@ static PyMethodDef methods[] = {
     {"lookahead_record_read", m_lookahead_record_read, METH_VARARGS, "Copy the record out."},
     {"lookahead_candidates", m_lookahead_candidates, METH_VARARGS, "Level-0 candidate planes of whole meshes."},
     {"lookahead_set_proposal", m_lookahead_set_proposal, METH_VARARGS, "Learned proposal for the lookahead."},
+    {"copy_probe", m_copy_probe, METH_VARARGS, "Known-size device copy (bytes, reps) -> ms per copy."},
     {"set_tuning", m_set_tuning, METH_VARARGS, "Set a hull launch-shape knob (key, value)."},
     {"get_tuning", m_get_tuning, METH_VARARGS, "Read a hull launch-shape knob."},
     {"score_cuts", m_score_cuts, METH_VARARGS, "Batched exact cut scores: cut, hulls, max(0.3 rv, H) per half."},

@@ -297,6 +297,11 @@ int gpu_hausdorff_plan_create(gpu_ctx_t ctx, const float* d_a_verts, const int* 
 int gpu_hausdorff_plan_run(gpu_ctx_t ctx, hd_plan_t plan);
 void gpu_hausdorff_plan_destroy(gpu_ctx_t ctx, hd_plan_t plan);
 
+/* Known-size device copy (bytes, a multiple of 4; reps launches of the
+ * library's segmented copy kernel in 1 MiB segments) for calibrating HBM
+ * traffic counters; *ms = GPU ms per copy. No reference counterpart. */
+int gpu_copy_probe(gpu_ctx_t ctx, size_t bytes, int reps, float* ms);
+
 /* sets KERR_INJECTED in the next kernel error word read back by the library
  * (tests of error propagation; no GPU fault is involved) */
 void gpu_debug_inject_error(gpu_ctx_t ctx, int kerr_bits);
