@@ -433,15 +433,9 @@ template <class T> T* carve(char*& p, size_t n) {
     return r;
 }
 
-// network whose split-K workspace gemm() uses, set by each entry point (the
-// library's calls on one context are not concurrent)
-gpu_mlp* g_cur = nullptr;
-
 // split-K workspace of at least n floats (grow-only; a growth waits for the
 // stream so no launch still reads the old buffer)
-float* part_ws(gpu_ctx_t ctx, size_t n) {
-    gpu_mlp* mp = g_cur;
-    if (!mp) return nullptr;
+float* part_ws(gpu_ctx_t ctx, gpu_mlp* mp, size_t n) {
     if (n > mp->part_cap) {
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
         if (mp->part) (void)hipFree(mp->part);
@@ -453,7 +447,7 @@ float* part_ws(gpu_ctx_t ctx, size_t n) {
     return mp->part;
 }
 
-int gemm(gpu_ctx_t ctx, int M, int N, int K, const bf16_t* A, long long sam, long long sak, const bf16_t* B,
+int gemm(gpu_ctx_t ctx, gpu_mlp* mp, int M, int N, int K, const bf16_t* A, long long sam, long long sak, const bf16_t* B,
          long long sbk, long long sbn, const float* bias, int relu, const bf16_t* mask, long long ldm, float* C,
          long long ldc, bf16_t* Cb, long long ldcb, float alpha = 1.0f) {
     if (M <= 0 || N <= 0) return 0;
@@ -467,7 +461,7 @@ int gemm(gpu_ctx_t ctx, int M, int N, int K, const bf16_t* A, long long sam, lon
         const int kc = ((K + nz - 1) / nz + GK - 1) / GK * GK;
         nz = (K + kc - 1) / kc;
         g.kchunk = kc;
-        g.part = part_ws(ctx, (size_t)nz * M * N);
+        g.part = part_ws(ctx, mp, (size_t)nz * M * N);
         REQUIRE(g.part, "mlp: split-K workspace allocation failed");
     }
     hipLaunchKernelGGL(mlp_gemm_kernel, dim3((unsigned)tn, (unsigned)tm, (unsigned)nz), dim3(256), 0, ctx->stream, g);
@@ -546,11 +540,11 @@ int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, const float* d_plane
     hipLaunchKernelGGL(mlp_pack_points, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, ctx->stream, d_pts,
                        d_planes, d_part, C, a.X, a.S);
     HIP_TRY(hipGetLastError());
-    if (int rc = gemm(ctx, R, HD, KIN, a.X, KIN, 1, W + P_W1, HD, 1, Wf + P_B1, 1, nullptr, 0, nullptr, 0, a.A1, HD))
+    if (int rc = gemm(ctx, mp, R, HD, KIN, a.X, KIN, 1, W + P_W1, HD, 1, Wf + P_B1, 1, nullptr, 0, nullptr, 0, a.A1, HD))
         return rc;
-    if (int rc = gemm(ctx, R, HD, HD, a.A1, HD, 1, W + P_W2, HD, 1, Wf + P_B2, 1, nullptr, 0, nullptr, 0, a.A2, HD))
+    if (int rc = gemm(ctx, mp, R, HD, HD, a.A1, HD, 1, W + P_W2, HD, 1, Wf + P_B2, 1, nullptr, 0, nullptr, 0, a.A2, HD))
         return rc;
-    if (int rc = gemm(ctx, R, HD, HD, a.A2, HD, 1, W + P_W3, HD, 1, Wf + P_B3, 1, nullptr, 0, nullptr, 0, a.A3, HD))
+    if (int rc = gemm(ctx, mp, R, HD, HD, a.A2, HD, 1, W + P_W3, HD, 1, Wf + P_B3, 1, nullptr, 0, nullptr, 0, a.A3, HD))
         return rc;
     hipLaunchKernelGGL(mlp_maxpool, dim3((unsigned)C), dim3(HD), 0, ctx->stream, a.A3, a.S, C, a.G, a.arg);
     HIP_TRY(hipGetLastError());
@@ -560,11 +554,11 @@ int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, const float* d_plane
                            d_planes, C, a.Hin);
         HIP_TRY(hipGetLastError());
     }
-    if (int rc = gemm(ctx, C, HD, KH, a.Hin, KH, 1, W + P_H1, HD, 1, Wf + P_C1, 1, nullptr, 0, nullptr, 0, a.B1, HD))
+    if (int rc = gemm(ctx, mp, C, HD, KH, a.Hin, KH, 1, W + P_H1, HD, 1, Wf + P_C1, 1, nullptr, 0, nullptr, 0, a.B1, HD))
         return rc;
-    if (int rc = gemm(ctx, C, HD, HD, a.B1, HD, 1, W + P_H2, HD, 1, Wf + P_C2, 1, nullptr, 0, nullptr, 0, a.B2, HD))
+    if (int rc = gemm(ctx, mp, C, HD, HD, a.B1, HD, 1, W + P_H2, HD, 1, Wf + P_C2, 1, nullptr, 0, nullptr, 0, a.B2, HD))
         return rc;
-    return gemm(ctx, C, NO, HD, a.B2, HD, 1, W + P_H3, NO, 1, Wf + P_C3, 0, nullptr, 0, a.Y, NO, nullptr, 0);
+    return gemm(ctx, mp, C, NO, HD, a.B2, HD, 1, W + P_H3, NO, 1, Wf + P_C3, 0, nullptr, 0, a.Y, NO, nullptr, 0);
 }
 
 // gradients of the mean squared error into mp->g (after forward)
@@ -590,20 +584,20 @@ int backward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_t, int C, Act& a) {
         return 0;
     };
     // head
-    if (int rc = gemm(ctx, HD, NO, C, a.B2, 1, HD, a.dY, NO, 1, nullptr, 0, nullptr, 0, g + P_H3, NO, nullptr, 0))
+    if (int rc = gemm(ctx, mp, HD, NO, C, a.B2, 1, HD, a.dY, NO, 1, nullptr, 0, nullptr, 0, g + P_H3, NO, nullptr, 0))
         return rc;
     if (int rc = colsum(a.dY, C, NO, g + P_C3)) return rc;
-    if (int rc = gemm(ctx, C, HD, NO, a.dY, NO, 1, W + P_H3, 1, NO, nullptr, 0, a.B2, HD, nullptr, 0, a.dZ2h, HD))
+    if (int rc = gemm(ctx, mp, C, HD, NO, a.dY, NO, 1, W + P_H3, 1, NO, nullptr, 0, a.B2, HD, nullptr, 0, a.dZ2h, HD))
         return rc;
-    if (int rc = gemm(ctx, HD, HD, C, a.B1, 1, HD, a.dZ2h, HD, 1, nullptr, 0, nullptr, 0, g + P_H2, HD, nullptr, 0))
+    if (int rc = gemm(ctx, mp, HD, HD, C, a.B1, 1, HD, a.dZ2h, HD, 1, nullptr, 0, nullptr, 0, g + P_H2, HD, nullptr, 0))
         return rc;
     if (int rc = colsum(a.dZ2h, C, HD, g + P_C2)) return rc;
-    if (int rc = gemm(ctx, C, HD, HD, a.dZ2h, HD, 1, W + P_H2, 1, HD, nullptr, 0, a.B1, HD, nullptr, 0, a.dZ1h, HD))
+    if (int rc = gemm(ctx, mp, C, HD, HD, a.dZ2h, HD, 1, W + P_H2, 1, HD, nullptr, 0, a.B1, HD, nullptr, 0, a.dZ1h, HD))
         return rc;
-    if (int rc = gemm(ctx, KH, HD, C, a.Hin, 1, KH, a.dZ1h, HD, 1, nullptr, 0, nullptr, 0, g + P_H1, HD, nullptr, 0))
+    if (int rc = gemm(ctx, mp, KH, HD, C, a.Hin, 1, KH, a.dZ1h, HD, 1, nullptr, 0, nullptr, 0, g + P_H1, HD, nullptr, 0))
         return rc;
     if (int rc = colsum(a.dZ1h, C, HD, g + P_C1)) return rc;
-    if (int rc = gemm(ctx, C, KH, HD, a.dZ1h, HD, 1, W + P_H1, 1, HD, nullptr, 0, nullptr, 0, a.dHin, KH, nullptr, 0))
+    if (int rc = gemm(ctx, mp, C, KH, HD, a.dZ1h, HD, 1, W + P_H1, 1, HD, nullptr, 0, nullptr, 0, a.dHin, KH, nullptr, 0))
         return rc;
     // encoder
     {
@@ -612,17 +606,17 @@ int backward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_t, int C, Act& a) {
                            a.A3, C, a.dA3);
         HIP_TRY(hipGetLastError());
     }
-    if (int rc = gemm(ctx, HD, HD, R, a.A2, 1, HD, a.dA3, HD, 1, nullptr, 0, nullptr, 0, g + P_W3, HD, nullptr, 0))
+    if (int rc = gemm(ctx, mp, HD, HD, R, a.A2, 1, HD, a.dA3, HD, 1, nullptr, 0, nullptr, 0, g + P_W3, HD, nullptr, 0))
         return rc;
     if (int rc = colsum(a.dA3, R, HD, g + P_B3)) return rc;
-    if (int rc = gemm(ctx, R, HD, HD, a.dA3, HD, 1, W + P_W3, 1, HD, nullptr, 0, a.A2, HD, nullptr, 0, a.dZ2e, HD))
+    if (int rc = gemm(ctx, mp, R, HD, HD, a.dA3, HD, 1, W + P_W3, 1, HD, nullptr, 0, a.A2, HD, nullptr, 0, a.dZ2e, HD))
         return rc;
-    if (int rc = gemm(ctx, HD, HD, R, a.A1, 1, HD, a.dZ2e, HD, 1, nullptr, 0, nullptr, 0, g + P_W2, HD, nullptr, 0))
+    if (int rc = gemm(ctx, mp, HD, HD, R, a.A1, 1, HD, a.dZ2e, HD, 1, nullptr, 0, nullptr, 0, g + P_W2, HD, nullptr, 0))
         return rc;
     if (int rc = colsum(a.dZ2e, R, HD, g + P_B2)) return rc;
-    if (int rc = gemm(ctx, R, HD, HD, a.dZ2e, HD, 1, W + P_W2, 1, HD, nullptr, 0, a.A1, HD, nullptr, 0, a.dZ1e, HD))
+    if (int rc = gemm(ctx, mp, R, HD, HD, a.dZ2e, HD, 1, W + P_W2, 1, HD, nullptr, 0, a.A1, HD, nullptr, 0, a.dZ1e, HD))
         return rc;
-    if (int rc = gemm(ctx, KIN, HD, R, a.X, 1, KIN, a.dZ1e, HD, 1, nullptr, 0, nullptr, 0, g + P_W1, HD, nullptr, 0))
+    if (int rc = gemm(ctx, mp, KIN, HD, R, a.X, 1, KIN, a.dZ1e, HD, 1, nullptr, 0, nullptr, 0, g + P_W1, HD, nullptr, 0))
         return rc;
     return colsum(a.dZ1e, R, HD, g + P_B1);
 }
@@ -682,7 +676,6 @@ int stage_err(gpu_ctx_t ctx, int rc) {
 namespace cuacd {
 int mlp_forward_dev(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* d_planes, const int* d_part,
                     int C, float* d_out) {
-    g_cur = mp;
     // bounded activations: slices of at most 8192 candidates (2M encoder rows)
     (void)P;
     const int slice = 8192;
@@ -783,7 +776,6 @@ void gpu_mlp_destroy(gpu_ctx_t ctx, gpu_mlp_t m) {
     if (m->v) (void)hipFree(m->v);
     if (m->act) (void)hipFree(m->act);
     if (m->part) (void)hipFree(m->part);
-    if (g_cur == m) g_cur = nullptr;
     free(m);
 }
 
@@ -821,7 +813,6 @@ int gpu_mlp_train_step(gpu_ctx_t ctx, gpu_mlp_t m, const float* points, int n_pa
     Staged s;
     std::vector<int> coff;
     if (int rc = s.load(ctx, points, n_parts, planes, part_of, n_cand, targets, coff)) return stage_err(ctx, rc);
-    g_cur = m;
     Act a;
     if (int rc = act_layout(ctx, m, n_cand, true, &a)) return rc;
     if (int rc = forward(ctx, m, s.pts, s.planes, s.part, n_cand, a)) return rc;
@@ -903,10 +894,7 @@ int gpu_mlp_gemm_test(gpu_ctx_t ctx, int M, int N, int K, const unsigned short* 
     HIP_TRY(hipMemcpyAsync(dA, A, 2 * a_len, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dB, B, 2 * b_len, hipMemcpyHostToDevice, ctx->stream));
     gpu_mlp tmp{};  // holds the split-K workspace of this call
-    gpu_mlp* prev = g_cur;
-    g_cur = &tmp;
-    int rc = gemm(ctx, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, 0, nullptr, 0, dC, N, nullptr, 0);
-    g_cur = prev;
+    int rc = gemm(ctx, &tmp, M, N, K, dA, sam, sak, dB, sbk, sbn, nullptr, 0, nullptr, 0, dC, N, nullptr, 0);
     if (rc == 0) {
         HIP_TRY(hipMemcpyAsync(C, dC, 4ll * M * N, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));

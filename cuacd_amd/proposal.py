@@ -159,27 +159,30 @@ def candidates(ctx, meshes, width=30, threshold=0.05, n_concave_edges=16, concav
     return planes, part_of, k
 
 
-def training_parts(ctx, meshes, iters=(0, 1, 2), **search):
+def training_parts(ctx, meshes, iters=(0, 1, 2), chunk=128, **search):
     """Parts the search cuts: each mesh as it starts and the pieces of plain
-    lookahead runs stopped after `iters` iterations."""
+    lookahead runs stopped after `iters` iterations (meshes in batches of
+    `chunk`, which bounds the device heap one search uses)."""
     parts = []
     for k in iters:
         if k == 0:
             parts += [(np.asarray(v, np.float32), np.asarray(t, np.int32)) for v, t in meshes]
             continue
-        res = ctx.batch_lookahead_decompose(meshes, **dict(search, max_iters=k, merge_hulls=False))
-        for ps in res:
-            parts += [(pv, pt) for pv, pt, _, _ in ps if len(pt) >= 8]
+        for s in range(0, len(meshes), chunk):
+            res = ctx.batch_lookahead_decompose(meshes[s:s + chunk], **dict(search, max_iters=k, merge_hulls=False))
+            for ps in res:
+                parts += [(pv, pt) for pv, pt, _, _ in ps if len(pt) >= 8]
     return parts
 
 
-def record_dataset(ctx, meshes, **search):
+def record_dataset(ctx, meshes, chunk=128, **search):
     """(points, planes, part_of, targets) recorded by plain lookahead runs:
     every cutting part of every iteration with its level-0 candidates' path
     costs."""
     _gpu.lookahead_record(1)
     try:
-        ctx.batch_lookahead_decompose(meshes, **search)
+        for s in range(0, len(meshes), chunk):
+            ctx.batch_lookahead_decompose(meshes[s:s + chunk], **search)
         n_parts, n_cand = _gpu.lookahead_record_size()
         pts = np.empty((n_parts, NS, 3), np.float32)
         pl = np.empty((n_cand, 4), np.float32)

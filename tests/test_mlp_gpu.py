@@ -166,3 +166,30 @@ def test_trained_network_ranks_better_than_untrained(ctx):
     finally:
         trained.close()
         untrained.close()
+
+
+def test_search_record_and_error_paths(ctx):
+    from cuacd_amd import sdf
+    meshes = sdf.sdf_meshes(2, seed=13, res=20)
+    pts, pl, po, tg = PR.record_dataset(ctx, meshes, **dict(SEARCH, max_iters=2, merge_hulls=False))
+    assert len(pts) > 0 and len(pl) == len(po) == len(tg) and pts.shape[1:] == (256, 3)
+    assert np.all(np.diff(po) >= 0) and po.max() == len(pts) - 1
+    assert np.all(np.abs(np.linalg.norm(pl[:, :3], axis=1) - 1) < 1e-5)
+    # path costs are costs (>= 0) or the failure score; every part has a finite best
+    assert np.all(tg >= 0) and np.all(tg <= PR.FAIL_SCORE)
+    starts = np.searchsorted(po, np.arange(len(pts) + 1))
+    assert all(tg[a:b].min() < PR.FAIL_SCORE for a, b in zip(starts[:-1], starts[1:]))
+    # not recording: the size query fails loudly
+    with pytest.raises(RuntimeError):
+        cuacd_amd._gpu.lookahead_record_size()
+    # a proposal needs a positive top_k
+    net = PR.ProposalNet(ctx, seed=0)
+    try:
+        with pytest.raises(RuntimeError):
+            ctx.batch_lookahead_decompose(meshes[:1], proposal=net, top_k=0, **dict(SEARCH, max_iters=1))
+        with pytest.raises(ValueError):
+            net.predict(np.zeros((2, 256, 3), np.float32), np.zeros((2, 4), np.float32), [1, 0])
+    finally:
+        net.close()
+    # the failed call left no proposal behind: the plain search still runs
+    ctx.batch_lookahead_decompose(meshes[:1], **dict(SEARCH, max_iters=1))
