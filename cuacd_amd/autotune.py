@@ -22,6 +22,7 @@ DEFAULT_SPACE = {
     "hull_vcap_eighths": [2, 3, 4],
     "hull_half_cap": [0, 1],
     "kdop_min_points": [128, 256, 512],
+    "hull_multi": [0, 1],
 }
 
 CACHE_PATH = os.path.join(os.path.expanduser("~"), ".cache", "cuacd_amd", "tuning.json")

@@ -105,8 +105,19 @@ template <int AS> struct HullWS {
     asptr<AS_LDS, uint32_t> bm; // [NP/32] bitmap over local point ids (output order)
     asptr<AS_LDS, int> wpre;    // [NP/32] prefix popcounts of bm
     asptr<AS_LDS, int> ctr;     // [8] counters: 0 nvis, 1 nhz
+    // multi-apex scratch (LDS workspaces with several waves; null otherwise)
+    asptr<AS_LDS, uint32_t> mfm;    // [FC] per face: bit r visited by candidate r, bit 8+r visible from it
+    asptr<AS_LDS, uint32_t> mfmax;  // [FC] per face: best conflict key (quantised distance, point index)
+    asptr<AS_LDS, uint8_t> mfcone;  // [FC] candidate whose cone replaced the face this round
+    asptr<AS_LDS, uint16_t> mlist;  // [MA_W][MA_CAP][5]: visible faces, ha, hb, hface, hnew (ranks >= 1)
+    asptr<AS_LDS, uint8_t> mhedge;  // [MA_W][MA_CAP]
+    asptr<AS_LDS, int> mcand;       // [MA_W][MA_CI] candidate table, then [MA_W] conflict rows, misc
     int VC, FC, NP;
 };
+
+// Multi-apex rounds: at most MA_W candidates, MA_CAP visible faces and
+// horizon edges for each candidate but the first (which uses the main arrays).
+constexpr int MA_W = 8, MA_CAP = 128, MA_CI = 12;
 
 // FC = 2*VC + 8 faces bound the live faces of a hull with VC vertices plus the
 // transient pair created by one insertion (nhz <= nvis + 2). Face and vertex
@@ -117,7 +128,8 @@ __host__ __device__ constexpr int hull_fc(int vc) { return 2 * vc + 8; }
 // size; with p == nullptr it only sizes it. One routine for both, so the size
 // requested and the carve can never disagree. np = point capacity.
 template <int AS>
-__host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int np, HullWS<AS>* out) {
+__host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int np, HullWS<AS>* out,
+                                                            bool multi = false) {
     HullWS<AS> w{};
     const unsigned fc = (unsigned)hull_fc(vc), v = (unsigned)vc, nw = (unsigned)(np + 31) / 32u;
     unsigned off = 0;
@@ -145,6 +157,14 @@ __host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int
     w.vout = (asptr<AS, uint16_t>)take(v * 2u);
     w.vref = (asptr<AS, uint8_t>)take(v);
     w.cv = (asptr<AS, int>)take(v * 8u);
+    if (multi && AS == AS_LDS) {
+        w.mfm = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
+        w.mfmax = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
+        w.mfcone = (asptr<AS_LDS, uint8_t>)take(fc);
+        w.mlist = (asptr<AS_LDS, uint16_t>)take(MA_W * MA_CAP * 5 * 2u);
+        w.mhedge = (asptr<AS_LDS, uint8_t>)take(MA_W * MA_CAP);
+        w.mcand = (asptr<AS_LDS, int>)take((MA_W * MA_CI + MA_W + 16) * 4u);
+    }
     (void)nw;  // bm / wpre / ctr live in HullBlockLDS (hull_ws_attach)
     w.VC = vc;
     w.FC = (int)fc;
@@ -153,13 +173,13 @@ __host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int
     return off;
 }
 
-template <int AS> __host__ __device__ inline unsigned hull_ws_bytes(int vc, int np) {
-    return hull_ws_layout<AS>(nullptr, vc, np, nullptr);
+template <int AS> __host__ __device__ inline unsigned hull_ws_bytes(int vc, int np, bool multi = false) {
+    return hull_ws_layout<AS>(nullptr, vc, np, nullptr, multi);
 }
 
-template <int AS> __device__ __forceinline__ HullWS<AS> hull_ws_carve(char* p, int vc, int np) {
+template <int AS> __device__ __forceinline__ HullWS<AS> hull_ws_carve(char* p, int vc, int np, bool multi = false) {
     HullWS<AS> w;
-    hull_ws_layout<AS>(p, vc, np, &w);
+    hull_ws_layout<AS>(p, vc, np, &w, multi);
     return w;
 }
 
@@ -322,7 +342,7 @@ struct HullResult {
 
 // Cooperative exact quickhull over NW waves (blockDim.x == 64*NW).
 // px/py/pz: grid coords of local points i = k*T + tid (valid iff i < n), T = 64*NW.
-template <int PPL, int AS, int NW>
+template <int PPL, int AS, int NW, bool MULTI = false>
 __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBlockLDS* sb, int n, const int (&px)[PPL],
                                       const int (&py)[PPL], const int (&pz)[PPL]) {
     constexpr int T = 64 * NW;
@@ -478,6 +498,343 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
     }
 
     int nvert = 4, ftop = 4, nfree = 0;
+    if constexpr (MULTI) {
+        // ---- multi-apex rounds ---------------------------------------------
+        // Each round takes the farthest conflict point of every face, keeps
+        // the best face of each wave's share of the faces (ids interleaved
+        // over the waves; at most NW
+        // candidates, the first being the overall best), and lets wave r find
+        // the visible region of candidate r. Two candidates conflict when one's
+        // visible region meets the other's visited faces (its visible region
+        // and their neighbours); a conflict-free set of candidates (greedy by
+        // rank) is inserted at once. For such a set, inserting them together
+        // gives the hull of inserting them one after another: a candidate sees
+        // none of another's cone faces, since a walk over visible faces from
+        // its region to such a cone face would cross a horizon face of the
+        // other, which would then be in its own visible region. Candidates may
+        // be non-extreme points (their distances are quantised); the exact
+        // incremental construction still ends at the unique hull.
+        static_assert(AS == AS_LDS && NW > 1 && NW <= MA_W, "multi-apex rounds need an LDS workspace and several waves");
+        const int wv = tid >> 6, lane = tid & 63;
+        asptr<AS_LDS, int> cand = w.mcand;
+        asptr<AS_LDS, int> crow = w.mcand + MA_W * MA_CI;
+        asptr<AS_LDS, int> misc = w.mcand + MA_W * MA_CI + MA_W;
+        auto lvis = [&](int r) { return w.mlist + (r * 5 + 0) * MA_CAP; };
+        auto lha = [&](int r) { return w.mlist + (r * 5 + 1) * MA_CAP; };
+        auto lhb = [&](int r) { return w.mlist + (r * 5 + 2) * MA_CAP; };
+        auto lhf = [&](int r) { return w.mlist + (r * 5 + 3) * MA_CAP; };
+        auto lhn = [&](int r) { return w.mlist + (r * 5 + 4) * MA_CAP; };
+        auto wsync = [&]() {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        };
+        for (int it = 1;; ++it) {
+            // A: clear the per-face marks and best keys
+            for (int f = tid; f < ftop; f += T) { w.mfm[f] = 0u; w.mfmax[f] = 0u; }
+            if (tid < MA_W) crow[tid] = 0;
+            __syncthreads();
+            // B: best conflict key per face (quantised distance, lowest index)
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                if (conf[k] >= 0) {
+                    const uint32_t key = (fkey(dist[k]) & ~0x7FFu) | (0x7FFu - (uint32_t)(k * T + tid));
+                    __hip_atomic_fetch_max(&w.mfmax[conf[k]], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            __syncthreads();
+            // C: candidates - the best face of each wave's faces (face ids
+            // interleaved over the waves, so every wave has faces), ranked
+            uint64_t bk = 0;
+            for (int f = wv + NW * lane; f < ftop; f += NW * 64) {
+                const uint32_t k32 = w.mfmax[f];
+                const uint64_t v = ((uint64_t)k32 << 32) | (uint32_t)f;
+                if (k32 && v > bk) bk = v;
+            }
+            bk = wave_max_u64(bk);
+            if (lane == 0) sb->red[wv] = bk;
+            __syncthreads();
+            uint64_t rk[NW];
+#pragma unroll
+            for (int q = 0; q < NW; ++q) rk[q] = sb->red[q];
+            // rank order: descending keys (insertion sort, identical in every thread)
+#pragma unroll
+            for (int a = 1; a < NW; ++a)
+#pragma unroll
+                for (int b = a; b > 0; --b)
+                    if (rk[b] > rk[b - 1]) { const uint64_t t = rk[b]; rk[b] = rk[b - 1]; rk[b - 1] = t; }
+            if (rk[0] == 0) break;
+            int ncand = 0;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) ncand += rk[q] != 0;
+            // the owners of the candidate points publish them
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                if (!rk[q]) continue;
+                const int ip = (int)(0x7FFu - ((uint32_t)(rk[q] >> 32) & 0x7FFu));
+                if (tid == ip % T) {
+                    const int slot = ip / T;
+                    cand[q * MA_CI + 0] = (int)(uint32_t)(rk[q] >> 32);
+                    cand[q * MA_CI + 1] = (int)(uint32_t)rk[q];
+                    cand[q * MA_CI + 2] = ip;
+                    cand[q * MA_CI + 3] = pick(px, slot);
+                    cand[q * MA_CI + 4] = pick(py, slot);
+                    cand[q * MA_CI + 5] = pick(pz, slot);
+                }
+            }
+            __syncthreads();
+            // D: wave r finds candidate r's visible region and horizon
+            if (wv < ncand) {
+                const int r = wv;
+                const int f0 = cand[r * MA_CI + 1];
+                const int qx = cand[r * MA_CI + 3], qy = cand[r * MA_CI + 4], qz = cand[r * MA_CI + 5];
+                asptr<AS_LDS, uint16_t> V = r == 0 ? (asptr<AS_LDS, uint16_t>)w.vis : lvis(r);
+                const int vcap = r == 0 ? FC : MA_CAP;
+                const uint32_t vb = 1u << r, sbit = 1u << (8 + r);
+                if (lane == 0) {
+                    __hip_atomic_fetch_or(&w.mfm[f0], vb | sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    V[0] = (uint16_t)f0;
+                }
+                wsync();
+                int head = 0, nv = 1;
+                bool over = false;
+                while (head < nv) {
+                    const int cnt = min(nv - head, 64);
+                    int g3[3] = {0, 0, 0};
+                    bool ap[3] = {false, false, false};
+                    if (lane < cnt) {
+                        const int f = V[head + lane];
+                        const u4v nb = w.fn[f];
+                        g3[0] = nb.x; g3[1] = nb.y; g3[2] = nb.z;
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const uint32_t old = __hip_atomic_fetch_or(&w.mfm[g3[e]], vb, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (!(old & vb)) {
+                                float t;
+                                if (face_above(w, g3[e], qx, qy, qz, &t)) {
+                                    __hip_atomic_fetch_or(&w.mfm[g3[e]], sbit, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                                    ap[e] = true;
+                                }
+                            }
+                        }
+                    }
+                    head += cnt;
+                    int add = 0;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const uint64_t m = ballot(ap[e]);
+                        const int pos = nv + add + mask_rank(m);
+                        if (ap[e]) {
+                            if (pos < vcap) V[pos] = (uint16_t)g3[e];
+                            else over = true;
+                        }
+                        add += __popcll(m);
+                    }
+                    nv += add;
+                    wsync();
+                    if (nv > vcap) { over = true; nv = vcap; break; }
+                }
+                over = ballot(over) != 0;
+                // horizon edges: visible face -> neighbour not visible from this candidate
+                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
+                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
+                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
+                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
+                const int hcap = r == 0 ? VC : MA_CAP;
+                int nh = 0;
+                if (!over) {
+                    for (int j0 = 0; j0 < nv; j0 += 64) {
+                        const int j = j0 + lane;
+                        int f = 0;
+                        u4v fvl = {0, 0, 0, 0}, nb = {0, 0, 0, 0};
+                        if (j < nv) {
+                            f = V[j];
+                            fvl = w.fv[f];
+                            nb = w.fn[f];
+                        }
+                        const int vv[3] = {fvl.x, fvl.y, fvl.z};
+                        const int gg[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const bool h = j < nv && !(w.mfm[gg[e]] & sbit);
+                            const uint64_t m = ballot(h);
+                            const int pos = nh + mask_rank(m);
+                            if (h) {
+                                if (pos < hcap) {
+                                    const u4v gn = w.fn[gg[e]];
+                                    HA[pos] = (uint16_t)vv[e];
+                                    HB[pos] = (uint16_t)vv[e == 2 ? 0 : e + 1];
+                                    HF[pos] = (uint16_t)gg[e];
+                                    HE[pos] = (uint8_t)((gn.x == f) ? 0 : ((gn.y == f) ? 1 : 2));
+                                }
+                            }
+                            nh += __popcll(m);
+                        }
+                    }
+                }
+                if (lane == 0) {
+                    cand[r * MA_CI + 6] = nv;
+                    cand[r * MA_CI + 7] = nh;
+                    cand[r * MA_CI + 8] = (!over && nh >= 3 && nh <= hcap) ? 1 : 0;
+                }
+            }
+            __syncthreads();
+            // E: conflicts - candidate r's visible faces against the others' visited faces
+            for (int f = tid; f < ftop; f += T) {
+                const uint32_t m = w.mfm[f];
+                uint32_t sv = (m >> 8) & 0xFFu;
+                const uint32_t vis_by = m & 0xFFu;
+                while (sv) {
+                    const int r = __ffs(sv) - 1;
+                    sv &= sv - 1;
+                    const uint32_t o = vis_by & ~(1u << r);
+                    if (o) __hip_atomic_fetch_or(&crow[r], (int)o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            __syncthreads();
+            // F: one thread accepts greedily by rank and assigns vertex and face ids
+            if (tid == 0) {
+                uint32_t acc = 0;
+                int nacc = 0, nf = nfree, ft = ftop, st = HULL_OK;
+                for (int r = 0; r < ncand; ++r) {
+                    const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
+                    const bool ok = cand[r * MA_CI + 8] != 0;
+                    if (!ok) {
+                        if (r == 0) { st = HULL_OVERFLOW; break; }
+                        continue;
+                    }
+                    bool clash = ((uint32_t)crow[r] & acc) != 0;
+                    for (int a = 0; a < r && !clash; ++a)
+                        if (((acc >> a) & 1u) && ((uint32_t)crow[a] & (1u << r))) clash = true;
+                    if (clash) continue;
+                    const int extra = nh > nv ? nh - nv : 0;
+                    const int from_free = min(extra, nf);
+                    const int from_top = extra - from_free;
+                    if (nvert + nacc >= VC || ft + from_top > FC) {
+                        if (r == 0) { st = HULL_OVERFLOW; break; }
+                        continue;
+                    }
+                    asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                    for (int k = 0; k < from_free; ++k) HN[nv + k] = w.ffree[nf - 1 - k];
+                    nf -= from_free;
+                    for (int k = 0; k < from_top; ++k) HN[nv + from_free + k] = (uint16_t)(ft + k);
+                    ft += from_top;
+                    cand[r * MA_CI + 9] = nvert + nacc;
+                    ++nacc;
+                    acc |= 1u << r;
+                }
+                // the surplus visible faces of the accepted cones go back to the free list
+                for (int r = 0; r < ncand; ++r)
+                    if ((acc >> r) & 1u) {
+                        const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
+                        cand[r * MA_CI + 10] = nf;
+                        nf += nv > nh ? nv - nh : 0;
+                    }
+                misc[0] = (int)acc;
+                misc[1] = nacc;
+                misc[2] = ft;
+                misc[3] = nf;
+                misc[4] = st;
+            }
+            __syncthreads();
+            const uint32_t acc = (uint32_t)misc[0];
+            if (misc[4] != HULL_OK) { res.status = HULL_OVERFLOW; break; }
+            // G: wave r builds the cone of accepted candidate r
+            if (wv < ncand && ((acc >> wv) & 1u)) {
+                const int r = wv;
+                const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
+                const int vid = cand[r * MA_CI + 9], push = cand[r * MA_CI + 10];
+                const int qx = cand[r * MA_CI + 3], qy = cand[r * MA_CI + 4], qz = cand[r * MA_CI + 5];
+                asptr<AS_LDS, uint16_t> V = r == 0 ? (asptr<AS_LDS, uint16_t>)w.vis : lvis(r);
+                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
+                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
+                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
+                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
+                asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                if (lane == 0) {
+                    w.vq[vid] = mk_s4(qx, qy, qz, 0);
+                    w.vpid[vid] = cand[r * MA_CI + 2];
+                }
+                const int mx = max(nh, nv);
+                for (int j = lane; j < mx; j += 64) {
+                    if (j < nv) w.mfcone[V[j]] = (uint8_t)r;
+                    if (j < nh) {
+                        if (j < nv) HN[j] = V[j];
+                    } else {
+                        const int f = V[j];
+                        w.ffree[push + (j - nh)] = (uint16_t)f;
+                        u4v t = w.fn[f];
+                        t.w = 0;
+                        w.fn[f] = t;
+                        u4v u = w.fv[f];
+                        u.w = (uint16_t)it;
+                        w.fv[f] = u;
+                    }
+                }
+                wsync();
+                const s4v sp = mk_s4(qx, qy, qz, 0);
+                for (int j = lane; j < nh; j += 64) {
+                    const int a = HA[j], c = HB[j];
+                    int nx = 0, pv = 0;  // cone faces across (c, apex) and (apex, a)
+                    for (int k = 0; k < nh; ++k) {
+                        if (HA[k] == c) nx = k;
+                        if (HB[k] == a) pv = k;
+                    }
+                    const int id = HN[j];
+                    w.fv[id] = mk_u4(a, c, vid, (uint16_t)it);
+                    w.fn[id] = mk_u4(HF[j], HN[nx], HN[pv], 1);
+                    face_set_plane(w, id, w.vq[a], w.vq[c], sp);
+                }
+                wsync();
+                for (int j = lane; j < nh; j += 64) {
+                    asptr<AS, uint16_t> slotp = ((asptr<AS, uint16_t>)&w.fn[HF[j]]) + HE[j];
+                    *slotp = HN[j];
+                }
+            }
+            nvert += misc[1];
+            ftop = misc[2];
+            nfree = misc[3];
+            __syncthreads();
+            // H: accepted apexes leave the conflict sets; points of replaced
+            // faces move to the cone that replaced them
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                if (q >= ncand || !((acc >> q) & 1u)) continue;
+                const int ip = cand[q * MA_CI + 2];
+                const int own = ip % T, slot = ip / T;
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const int m = -(int)(tid == own && k == slot);
+                    conf[k] = (conf[k] & ~m) | (-2 & m);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                const int c = conf[k];
+                if (c >= 0 && w.fv[c].w == (uint16_t)it) {
+                    const int r = w.mfcone[c];
+                    const int nh = cand[r * MA_CI + 7];
+                    asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                    int best = -1;
+                    float bd = 0.0f;
+                    for (int j = 0; j < nh; ++j) {
+                        const int id = HN[j];
+                        float t;
+                        if (face_above(w, id, px[k], py[k], pz[k], &t) && (best < 0 || t > bd)) {
+                            best = id;
+                            bd = t;
+                        }
+                    }
+                    conf[k] = best;
+                    dist[k] = bd;
+                }
+            }
+        }
+        res.nvert = nvert;
+        res.ftop = ftop;
+        __syncthreads();
+        return res;
+    }
     for (int it = 1;; ++it) {
         // ---- select the farthest conflict point (ties: lowest index) -------
         uint64_t sk = 0;

@@ -31,7 +31,7 @@ __constant__ float c_kdop_axes[KDOP_AXES_N][3];
 // redone by a full-capacity kernel; the task is then left untouched.
 template <int PPL, int AS, int NW>
 __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, HullBlockLDS* sb, DPool* pool,
-                                             int* err, int* retry = nullptr, int task_id = -1) {
+                                             int* err, int* retry = nullptr, int task_id = -1, bool multi = false) {
     constexpr int T = 64 * NW;
     const int tid = threadIdx.x;
     const float* P = t.filt ? t.filt : t.pts;
@@ -92,7 +92,15 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
             pz[k] = qcoord(fr, 2, sel3(v, fr.ax[2]));
         }
     }
-    HullResult r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
+    HullResult r;
+    if constexpr (AS == AS_LDS && NW > 1) {
+        // multi-apex rounds (the workspace was carved with their scratch); the
+        // point index field of their keys holds 11 bits
+        if (multi && n <= 2048) r = block_quickhull<PPL, AS, NW, true>(w, sb, n, px, py, pz);
+        else r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
+    } else {
+        r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
+    }
     if (r.status == HULL_OVERFLOW && retry) {
         if (tid == 0) {
             const int k = atomicAdd(retry, 1);
@@ -169,18 +177,18 @@ __global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int*
 constexpr int LA_LDS4_NW = 4;
 template <int PPL>
 __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* tasks, const int* ids, int n,
-                                                               const int* dcount, DPool* pool, int* err) {
+                                                               const int* dcount, int multi, DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
     // ids[0, n), or the first min(*dcount, n) of them when dcount is given
     // (a retry list filled by the previous launch); the grid strides over them
     const int nn = dcount ? min(*dcount, n) : n;
     constexpr int CAP = 64 * LA_LDS4_NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP), &sb);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb);
     for (int j = blockIdx.x; j < nn; j += gridDim.x) {
         __syncthreads();
         LaHullTask& t = tasks[ids[j]];
-        la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err);
+        la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err, nullptr, -1, multi != 0);
         if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
     }
 }
@@ -190,15 +198,15 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
 // vertices is queued on `retry` for the full-capacity kernel.
 template <int PPL, int NW>
 __global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n, int vcap,
-                                                       int* retry, DPool* pool, int* err) {
+                                                       int* retry, int multi, DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
     if ((int)blockIdx.x >= n) return;
     const int id = ids[blockIdx.x];
     LaHullTask& t = tasks[id];
     constexpr int CAP = 64 * NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, vcap, CAP), &sb);
-    const bool done = la_hull_item<PPL, AS_LDS, NW>(t, w, &sb, pool, err, retry, id);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, vcap, CAP, multi != 0), &sb);
+    const bool done = la_hull_item<PPL, AS_LDS, NW>(t, w, &sb, pool, err, retry, id, multi != 0);
     if (done && threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
 
@@ -1199,7 +1207,9 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
 //   4 chunk_above: prefiltered sets above this many points are hulled in
 //     chunks of half that size first (CUACD_HULL_CHUNK_ABOVE, 1024..16384;
 //     default 16384, the most one workgroup hulls)
-constexpr int LA_TUNE_N = 5;
+//   5 multi: multi-apex rounds in the eight-wave LDS tier (1025-2048 points;
+//     CUACD_HULL_MULTI, 0/1; default 1)
+constexpr int LA_TUNE_N = 6;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1209,15 +1219,16 @@ int tune_clamp(int key, int v) {
         case 1: return std::min(8, std::max(1, v));
         case 2: return v != 0;
         case 3: return std::max(4, v);
-        default: return std::min(HULL_NP_MAX, std::max(1024, v));
+        case 4: return std::min(HULL_NP_MAX, std::max(1024, v));
+        default: return v != 0;
     }
 }
 
 void tune_init() {
     if (g_tune_init) return;
     const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN", "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP", "CUACD_KDOP_MIN",
-                                  "CUACD_HULL_CHUNK_ABOVE"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX};
+                                  "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1228,6 +1239,7 @@ void tune_init() {
 int hull_wide_min() { tune_init(); return g_tune[0]; }
 int hull_vcap_eighths() { tune_init(); return g_tune[1]; }
 int hull_half_cap() { tune_init(); return g_tune[2]; }
+int hull_multi() { tune_init(); return g_tune[5]; }
 
 // One hull call: its size tiers are independent, so they run on separate
 // streams - the latency-bound large-set tiers (a few hulls of thousands of
@@ -1296,15 +1308,19 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
     if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
         // first pass with a reduced vertex capacity (several hulls per CU),
         // then the sets whose hulls did not fit, at full capacity
-        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
-        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
+        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, int, DPool*, int*);
+        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, int, DPool*, int*);
         CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>;
         FullFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
         // the canonical hulls of these sets have few corners (mostly under a
         // quarter of the points), so 3/8 of the points fits nearly all
         const int vcap = (64 * ppl * hull_vcap_eighths()) / 8;
-        const unsigned ldc = hull_ws_bytes<AS_LDS>(vcap, 64 * ppl);
-        const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
+        // single-apex steps: on these sets (513-1024 points) the larger LDS
+        // footprint of the multi-apex rounds costs more occupancy than the
+        // rounds save
+        const int multi = 0;
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(vcap, 64 * ppl, multi != 0);
+        const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl, multi != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
         int* d_retry = hc.alloc<int>(ids.size() + 1);
@@ -1314,28 +1330,29 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
         hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
-                           (int)ids.size(), vcap, d_retry, ctx->d_pool, ctx->d_err);
+                           (int)ids.size(), vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         hipEvent_t t1 = kt_begin(ctx, s);
         hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, ids.size())), dim3(64 * LA_LDS4_NW), ldf, s, d_tasks,
-                           d_retry + 1, (int)ids.size(), d_retry, ctx->d_pool, ctx->d_err);
+                           d_retry + 1, (int)ids.size(), d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, 0.0, s);
         return 0;
     }
     if (wm > 0 && ppl >= 4 && 64 * ppl > wm) {
         // same capacity (64 * ppl points and vertices), four waves
-        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
+        using FullFn = void (*)(LaHullTask*, const int*, int, const int*, int, DPool*, int*);
         FullFn k = ppl == 4 ? la_hull_lds4<1> : ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
-        const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl);
+        const int multi = 0;
+        const unsigned lds = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl, multi != 0);
         if (lds > 65536u)
             HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
         hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), lds, s, d_tasks, d_ids,
-                           (int)ids.size(), (const int*)nullptr, ctx->d_pool, ctx->d_err);
+                           (int)ids.size(), (const int*)nullptr, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         return 0;
@@ -1370,9 +1387,10 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vecto
     if (maxn <= 4 * T && hull_half_cap()) {
         // first pass with the workspace in LDS (1024 hull vertices), then the
         // sets whose hulls did not fit, with the workspace in HBM
-        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, DPool*, int*);
+        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, int, DPool*, int*);
         CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW>;
-        const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T);
+        const int multi = hull_multi();
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T, multi != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         int* d_retry = hc.alloc<int>(ids.size() + 1);
         REQUIRE(d_retry, "lookahead: id list allocation failed");
@@ -1385,7 +1403,7 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vecto
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kc, dim3((unsigned)n), dim3(T), ldc, s, d_tasks, d_ids, n, 1024, d_retry, ctx->d_pool,
+        hipLaunchKernelGGL(kc, dim3((unsigned)n), dim3(T), ldc, s, d_tasks, d_ids, n, 1024, d_retry, multi, ctx->d_pool,
                            ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, KT_HULL_B4, t0, bytes, s);

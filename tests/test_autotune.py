@@ -9,7 +9,7 @@ import cuacd_amd
 class FakeCtx:
     def __init__(self):
         self.t = {"hull_wide_min": 256, "hull_vcap_eighths": 3, "hull_half_cap": 1, "kdop_min_points": 256,
-                  "hull_chunk_above": 16384}
+                  "hull_chunk_above": 16384, "hull_multi": 1}
 
     def get_tuning(self, name=None):
         return dict(self.t) if name is None else self.t[name]
