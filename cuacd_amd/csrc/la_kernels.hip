@@ -1295,16 +1295,13 @@ int retry_grid(gpu_ctx_t ctx, size_t cap) {
     return (int)std::min(cap, (size_t)std::max(1, ctx->n_cu));
 }
 
-int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector<int>& ids, int ppl,
+int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int nids, int ppl,
                     double bytes) {
     gpu_ctx_t ctx = hc.ctx;
-    if (ids.empty()) return 0;
+    if (nids <= 0) return 0;
     const int kid = ppl == 1 ? KT_HULL_W1 : ppl == 2 ? KT_HULL_W2 : ppl == 4 ? KT_HULL_W4 : ppl == 8 ? KT_HULL_W8
                                                                                                      : KT_HULL_W16;
     const int wm = hull_wide_min();
-    int* d_ids = hc.alloc<int>(ids.size());
-    REQUIRE(d_ids, "lookahead: id list allocation failed");
-    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
     if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
         // first pass with a reduced vertex capacity (several hulls per CU),
         // then the sets whose hulls did not fit, at full capacity
@@ -1323,19 +1320,19 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
         const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl, multi != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
-        int* d_retry = hc.alloc<int>(ids.size() + 1);
+        int* d_retry = hc.alloc<int>((size_t)nids + 1);
         REQUIRE(d_retry, "lookahead: id list allocation failed");
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kc, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
-                           (int)ids.size(), vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kc, dim3((unsigned)(size_t)nids), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
+                           (int)(size_t)nids, vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         hipEvent_t t1 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, ids.size())), dim3(64 * LA_LDS4_NW), ldf, s, d_tasks,
-                           d_retry + 1, (int)ids.size(), d_retry, multi, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, (size_t)nids)), dim3(64 * LA_LDS4_NW), ldf, s, d_tasks,
+                           d_retry + 1, (int)(size_t)nids, d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, 0.0, s);
         return 0;
@@ -1351,8 +1348,8 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64 * LA_LDS4_NW), lds, s, d_tasks, d_ids,
-                           (int)ids.size(), (const int*)nullptr, multi, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(k, dim3((unsigned)(size_t)nids), dim3(64 * LA_LDS4_NW), lds, s, d_tasks, d_ids,
+                           (int)(size_t)nids, (const int*)nullptr, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         return 0;
@@ -1365,7 +1362,7 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
     hipStream_t s;
     if (int rc = hc.fork(sk, &s)) return rc;
     hipEvent_t t0 = kt_begin(ctx, s);
-    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(64), lds, s, d_tasks, d_ids, (int)ids.size(),
+    hipLaunchKernelGGL(k, dim3((unsigned)(size_t)nids), dim3(64), lds, s, d_tasks, d_ids, (int)(size_t)nids,
                        ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
     kt_end(ctx, kid, t0, bytes, s);
@@ -1375,15 +1372,11 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector
 using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, DPool*,
                          int*);
 
-int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vector<int>& ids, int maxn,
+int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int n, int maxn,
                      double bytes) {
     gpu_ctx_t ctx = hc.ctx;
-    if (ids.empty()) return 0;
+    if (n <= 0) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
-    int* d_ids = hc.alloc<int>(ids.size());
-    REQUIRE(d_ids, "lookahead: id list allocation failed");
-    HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, ctx->stream));
-    const int n = (int)ids.size();
     if (maxn <= 4 * T && hull_half_cap()) {
         // first pass with the workspace in LDS (1024 hull vertices), then the
         // sets whose hulls did not fit, with the workspace in HBM
@@ -1392,12 +1385,12 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const std::vecto
         const int multi = hull_multi();
         const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T, multi != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
-        int* d_retry = hc.alloc<int>(ids.size() + 1);
+        int* d_retry = hc.alloc<int>((size_t)n + 1);
         REQUIRE(d_retry, "lookahead: id list allocation failed");
         HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
         const int np = (maxn + 63) / 64 * 64;
         const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(np, np) + 255ull) & ~255ull;
-        const int g = retry_grid(ctx, ids.size());
+        const int g = retry_grid(ctx, (size_t)n);
         char* gws = hc.alloc<char>(stride * (size_t)g + 256);
         REQUIRE(gws, "lookahead: hull workspace allocation failed");
         hipStream_t s;
@@ -1467,14 +1460,30 @@ int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pai
         else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); y16 += y; }
         else { b32.push_back(q.first); m32 = std::max(m32, n); y32 += y; }
     }
-    if (int rc = launch_block_ids(hc, 0, d_tasks, b32, m32, y32)) return rc;
-    if (int rc = launch_block_ids(hc, 0, d_tasks, b16, m16, y16)) return rc;
-    if (int rc = launch_block_ids(hc, 0, d_tasks, b8, m8, y8)) return rc;
-    if (int rc = launch_block_ids(hc, 1, d_tasks, b4, m4, y4)) return rc;
+    // every tier's id list in one upload, before any launch of this call on
+    // the context's stream (a pageable copy queued behind a running tier
+    // would hold the host until that tier finishes)
+    gpu_ctx_t ctx = hc.ctx;
+    const std::vector<int>* lists[9] = {&b32, &b16, &b8, &b4, &cls[4], &cls[3], &cls[2], &cls[1], &cls[0]};
+    size_t off[9], total = 0;
+    for (int k = 0; k < 9; ++k) { off[k] = total; total += lists[k]->size(); }
+    if (total == 0) return 0;
+    std::vector<int> all;
+    all.reserve(total);
+    for (int k = 0; k < 9; ++k) all.insert(all.end(), lists[k]->begin(), lists[k]->end());
+    int* d_all = hc.alloc<int>(total);
+    REQUIRE(d_all, "lookahead: id list allocation failed");
+    HIP_TRY(hipMemcpyAsync(d_all, all.data(), 4 * total, hipMemcpyHostToDevice, ctx->stream));
+    auto L = [&](int k) { return d_all + off[k]; };
+    auto N = [&](int k) { return (int)lists[k]->size(); };
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(0), N(0), m32, y32)) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(1), N(1), m16, y16)) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(2), N(2), m8, y8)) return rc;
+    if (int rc = launch_block_ids(hc, 1, d_tasks, L(3), N(3), m4, y4)) return rc;
     const int ppls[5] = {1, 2, 4, 8, 16};
-    if (int rc = launch_wave_ids(hc, 2, d_tasks, cls[4], ppls[4], cbytes[4])) return rc;
+    if (int rc = launch_wave_ids(hc, 2, d_tasks, L(4), N(4), ppls[4], cbytes[4])) return rc;
     for (int c = 3; c >= 0; --c)
-        if (int rc = launch_wave_ids(hc, pre ? 2 : -1, d_tasks, cls[c], ppls[c], cbytes[c])) return rc;
+        if (int rc = launch_wave_ids(hc, pre ? 2 : -1, d_tasks, L(8 - c), N(8 - c), ppls[c], cbytes[c])) return rc;
     return 0;
 }
 
