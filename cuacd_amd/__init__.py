@@ -455,7 +455,7 @@ class Context:
         return [_part_tuple(r)[:4] for r in raw]
 
     def batch_lookahead_decompose(self, meshes, *, max_iters=100, width=60, width2=5, threshold=0.05,
-                                  depth=2, max_n_cutting=16, decompose_components=False,
+                                  depth=2, quick_depth=0, max_n_cutting=16, decompose_components=False,
                                   no_decompose_components_per_iter=False, n_concave_edges=32,
                                   concave_eps=0.005, concave_threshold=3.49, concave_iters=10,
                                   merge_hulls=False, return_info=False, proposal=None, top_k=8):
@@ -495,7 +495,7 @@ class Context:
         try:
             raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
                                        hv.ctypes.data, hvo.ctypes.data, ht.ctypes.data, hto.ctypes.data,
-                                       max_iters, width, width2, float(threshold), depth, max_n_cutting,
+                                       max_iters, width, width2, float(threshold), depth, quick_depth, max_n_cutting,
                                        int(decompose_components), int(not no_decompose_components_per_iter),
                                        n_concave_edges, float(concave_eps), float(concave_threshold),
                                        concave_iters, int(merge_hulls), stats.ctypes.data, iters.ctypes.data)

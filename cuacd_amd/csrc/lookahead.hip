@@ -7,16 +7,18 @@
 // max_n_cutting parts whose full cost max(0.3 rv, H) is at or above the
 // threshold, and for each one expands a lookahead tree: level 0 tries `width`
 // axis cuts plus up to 4 planes per sampled concave edge, deeper levels cut
-// the worst part of each item with `width2` axis cuts. A leaf's path cost is
+// the worst part of each item with `width2` axis cuts, and optional quick
+// levels cut it once more at its box midpoint. A leaf's path cost is
 // the mean of the worst-part rv costs of its levels; the initial cut of the
 // best leaf is applied. Branch-and-bound pruning at the last level, extra
 // leaves for parts too small to cut, and the greedy level-0 fallback follow
 // the reference (cuda/la_expand.cu, cuda/la_refine.cu, cuda/la_lifecycle.cu).
 //
 // What is this library's own: every phase is one launch over all meshes of a
-// batch (a mesh drops out when it converges); the last level scores a cut
-// without building it (exact half-space volumes + the hull point sets of both
-// sides) since its halves are never used again; volumes are FP64 and the
+// batch (a mesh drops out when it converges); the last full level, unless
+// quick levels follow, scores a cut without building it (exact half-space
+// volumes + the hull point sets of both sides) since its halves are never
+// used again; volumes are FP64 and the
 // costs are rounded to float, the reference's cost precision; new parts are
 // appended in cutting order, so runs are deterministic.
 #include <algorithm>
@@ -558,6 +560,163 @@ struct Engine {
     }
 
     // Hull volumes for item halves. prune: per task gap (>=1e29 none).
+    // Full cuts with caps (halves, volumes, boxes) of a task list, read back.
+    int cut_batch(std::vector<LaCutTask>& h) {
+        if (h.empty()) return 0;
+        DevBuf<LaCutTask> dt;
+        REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
+        HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        LA_TRY(tbeg());
+        LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
+        LA_TRY(tend(PH_CUT, 0.0));
+        HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        // mesh read, both halves written, volumes
+        for (auto& x : h) stats[PH_CUT][2] += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
+        return check_err("cuts");
+    }
+
+    // ---- quick levels (csrc/lookahead.c:656-727, cuda/la_refine.cu:52-351): one
+    // child per item, its worst part cut at the box midpoint along the axis
+    // whose halves have the most balanced volumes (max / sum; the first axis
+    // on ties). An item whose worst part is too small along every axis becomes
+    // an extra leaf, one with no valid cut is dropped, and a mesh whose level
+    // produced no child keeps its items as they are. The last quick level
+    // prunes with the branch-and-bound bound over the items entering it.
+    // Quick halves are never applied: all of them go to the iteration's free
+    // list, which is flushed after the apply step.
+    int quick_levels(const Params& P, float med, int total_levels, std::vector<Item>& cur, std::vector<int>& cur_c,
+                     std::vector<Item>& extra) {
+        std::vector<Item> done;
+        std::vector<int> done_c;
+        const float lim = 2.f * med + 1e-6f;
+        for (int q = 0; q < P.quick_depth && !cur.empty(); ++q) {
+            const bool lastq = q == P.quick_depth - 1;
+            // bound per mesh (cuda/la_expand.cu:434-457), last quick level only
+            std::vector<float> best_ub(M.size(), 1e30f);
+            if (lastq)
+                for (const Item& it : cur) {
+                    if (it.nl <= 0) continue;
+                    float ps = 0.0f;
+                    for (int l = 0; l < it.nl; ++l) ps += it.lc[l];
+                    const float ub = ps + it.lc[it.nl - 1] * (float)(total_levels - it.nl);
+                    if (ub < best_ub[it.mesh]) best_ub[it.mesh] = ub;
+                }
+            // up to three midpoint cuts per item, in axis order
+            std::vector<LaCutTask> h;
+            std::vector<int> h_item;
+            for (size_t ii = 0; ii < cur.size(); ++ii) {
+                const Item& it = cur[ii];
+                const IPart& wp = it.parts[it.np - 1];
+                REQUIRE(wp.nv > 0 && wp.nt > 0 && wp.nv <= 1000000 && wp.nt <= 1000000,
+                        "lookahead: invalid part mesh");
+                bool any = false;
+                for (int a = 0; a < 3; ++a) any |= wp.hi[a] - wp.lo[a] > lim;
+                if (!any) {
+                    Item x = it;
+                    for (int l = x.nl; l < total_levels && l < LA_MAX_LEVELS; ++l) x.lc[l] = 0.0f;
+                    x.nl = std::min(total_levels, LA_MAX_LEVELS);
+                    extra.push_back(x);
+                    continue;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    if (wp.hi[a] - wp.lo[a] <= lim) continue;
+                    LaCutTask t{};
+                    t.v = wp.v; t.t = wp.t; t.nv = wp.nv; t.nt = wp.nt;
+                    t.plane[a] = 1.f;
+                    t.plane[3] = -((wp.lo[a] + wp.hi[a]) * 0.5f);
+                    h.push_back(t);
+                    h_item.push_back((int)ii);
+                }
+            }
+            LA_TRY(cut_batch(h));
+            std::vector<int> pick(cur.size(), -1);
+            std::vector<float> pick_cost(cur.size(), 1e30f);
+            for (size_t u = 0; u < h.size(); ++u) {
+                const LaCutTask& t = h[u];
+                if (t.status != 0 || !t.pv || !t.nvp) continue;
+                to_free.push_back(t.pv);
+                to_free.push_back(t.nvp);
+                const float pv = (float)t.pvol, nv = (float)t.nvol;
+                const float cost = std::max(pv, nv) / (pv + nv + 1e-10f);
+                if (cost < pick_cost[h_item[u]]) {
+                    pick_cost[h_item[u]] = cost;
+                    pick[h_item[u]] = (int)u;
+                }
+            }
+            // children and the hulls of their two new parts
+            std::vector<Item> nxt;
+            std::vector<int> nxt_c;
+            std::vector<LaHullTask> ht;
+            for (size_t ii = 0; ii < cur.size(); ++ii) {
+                if (pick[ii] < 0) continue;
+                const Item& par = cur[ii];
+                REQUIRE(par.np + 1 <= LA_MAX_PARTS, "lookahead: item part capacity exceeded");
+                const LaCutTask& t = h[pick[ii]];
+                Item ch = par;
+                IPart pp, nn;
+                pp.v = t.pv; pp.t = t.pt; pp.nv = t.pnv; pp.nt = t.pnt; pp.mvol = t.pvol;
+                nn.v = t.nvp; nn.t = t.ntp; nn.nv = t.nnv; nn.nt = t.nnt; nn.mvol = t.nvol;
+                pp.cc = nn.cc = par.parts[par.np - 1].cc;
+                for (int j = 0; j < 3; ++j) {
+                    pp.lo[j] = t.plo[j]; pp.hi[j] = t.phi[j];
+                    nn.lo[j] = t.nlo[j]; nn.hi[j] = t.nhi[j];
+                }
+                ch.parts[par.np - 1] = pp;
+                ch.parts[par.np] = nn;
+                ch.np = par.np + 1;
+                // pruning threshold (cuda/la_expand.cu:380-399)
+                double gap = 1e30;
+                const float ub = best_ub[par.mesh];
+                if (ub < 1e29f && par.nl > 0 && par.nl < total_levels) {
+                    float ps = 0.0f;
+                    for (int l = 0; l < par.nl; ++l) ps += par.lc[l];
+                    const float rv_thr = ub - ps;
+                    if (rv_thr <= 0.0f) gap = -1.0;
+                    else {
+                        const float r = rv_thr / 0.3f;
+                        gap = (double)(r * r * r * (4.0f * LA_PI_F / 3.0f));
+                    }
+                }
+                LaHullTask a{};
+                a.pts = t.pv; a.n = t.pnv; a.prune_gap = gap; a.mesh_vol = t.pvol;
+                LaHullTask b{};
+                b.pts = t.nvp; b.n = t.nnv; b.prune_gap = gap; b.mesh_vol = t.nvol;
+                ht.push_back(a);
+                ht.push_back(b);
+                nxt.push_back(ch);
+                nxt_c.push_back(cur_c[ii]);
+            }
+            LA_TRY(hull_volumes(ht));
+            for (size_t i = 0; i < nxt.size(); ++i) {
+                Item& ch = nxt[i];
+                ch.parts[ch.np - 2].hvol = ht[2 * i].vol;
+                ch.parts[ch.np - 1].hvol = ht[2 * i + 1].vol;
+                // a pruned hull's volume is a lower bound: the item is out of the
+                // evaluation (cuda/la_expand.cu:408-420)
+                if (ht[2 * i].pruned || ht[2 * i + 1].pruned) ch.pruned = true;
+                sort_item_parts(ch);
+                if (ch.nl < LA_MAX_LEVELS) {
+                    ch.lc[ch.nl] = cost_rv(ch.parts[ch.np - 1].hvol, ch.parts[ch.np - 1].mvol);
+                    ch.nl++;
+                }
+            }
+            std::vector<char> has_child(M.size(), 0);
+            for (const Item& ch : nxt) has_child[ch.mesh] = 1;
+            for (size_t i = 0; i < cur.size(); ++i)
+                if (!has_child[cur[i].mesh]) {
+                    done.push_back(cur[i]);
+                    done_c.push_back(cur_c[i]);
+                }
+            cur.swap(nxt);
+            cur_c.swap(nxt_c);
+            lap(S_LEVELN);
+        }
+        cur.insert(cur.end(), done.begin(), done.end());
+        cur_c.insert(cur_c.end(), done_c.begin(), done_c.end());
+        return 0;
+    }
+
     int hull_volumes(std::vector<LaHullTask>& h) {
         if (h.empty()) return 0;
         DevBuf<LaHullTask> d;
@@ -982,7 +1141,7 @@ struct Engine {
                 n_edge[cutting[c].first] = std::max(n_edge[cutting[c].first], (int)eplanes[c].size() / 4);
         }
         lap(S_CONC);
-        const int total_levels = P.depth;
+        const int total_levels = P.depth + P.quick_depth;
         // level-0 candidates per cutting part, for the training record
         std::vector<std::vector<int>> rec_k(cutting.size());
         std::vector<std::vector<float>> rec_pl(cutting.size());
@@ -1016,6 +1175,10 @@ struct Engine {
         // (reference: the level loop breaks with next_n == 0, csrc/lookahead.c:579-581)
         std::vector<Item> leaves;
         std::vector<int> leaves_c;
+        // items of a mesh whose level produced no child: the mesh's levels end
+        // there, and these items go on to the quick levels (or are its leaves)
+        std::vector<Item> stay;
+        std::vector<int> stay_c;
         struct Leaf {
             int c, cut0;
             float path;
@@ -1027,7 +1190,9 @@ struct Engine {
         for (size_t c = 0; c < cutting.size(); ++c) cur_c[c] = (int)c;
         for (int d = 0; d < P.depth; ++d) {
             const bool last = d == P.depth - 1;
-            const bool full = !last || P.depth == 1;
+            // the last full level is scored without building its children
+            // unless quick levels follow (they cut the children's parts)
+            const bool full = !last || P.depth == 1 || P.quick_depth > 0;
             // candidates
             struct Cand {
                 int item, k;
@@ -1155,25 +1320,7 @@ struct Engine {
                         for (int j = 0; j < 4; ++j) t.plane[j] = cs[q].pl[j];
                         h[q] = t;
                     }
-                    if (!h.empty()) {
-                        DevBuf<LaCutTask> dt;
-                        REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
-                        HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice,
-                                               ctx->stream));
-                        LA_TRY(tbeg());
-                        LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
-                        LA_TRY(tend(PH_CUT, 0.0));
-                        HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost,
-                                               ctx->stream));
-                        HIP_TRY(hipStreamSynchronize(ctx->stream));
-                        {
-                            // mesh read, both halves written, volumes
-                            double by = 0;
-                            for (auto& x : h) by += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
-                            stats[PH_CUT][2] += by;
-                        }
-                        LA_TRY(check_err("cuts"));
-                    }
+                    LA_TRY(cut_batch(h));
                     for (size_t q = 0; q < cs.size(); ++q) {
                         const LaCutTask& t = h[q];
                         if (t.status != 0 || !t.pv || !t.nvp) continue;
@@ -1408,15 +1555,18 @@ struct Engine {
             std::vector<char> has_child(M.size(), 0);
             for (const Item& ch : nxt) has_child[ch.mesh] = 1;
             for (size_t i = 0; i < cur.size(); ++i)
-                if (!has_child[cur[i].mesh] && d > 0) {
-                    leaves.push_back(cur[i]);
-                    leaves_c.push_back(cur_c[i]);
+                if (!has_child[cur[i].mesh]) {
+                    stay.push_back(cur[i]);
+                    stay_c.push_back(cur_c[i]);
                 }
             cur = nxt;
             cur_c = nxt_c;
             lap(d == 0 ? S_LEVEL0 : S_LEVELN);
             if (cur.empty()) break;
         }
+        cur.insert(cur.end(), stay.begin(), stay.end());
+        cur_c.insert(cur_c.end(), stay_c.begin(), stay_c.end());
+        LA_TRY(quick_levels(P, med, total_levels, cur, cur_c, extra));
         for (size_t i = 0; i < cur.size(); ++i) {
             leaves.push_back(cur[i]);
             leaves_c.push_back(cur_c[i]);
@@ -1497,7 +1647,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     HIP_TRY(hipSetDevice(ctx->device));
     for (int m = 0; m < n_meshes; ++m) { out[m].parts = nullptr; out[m].nparts = 0; }
     REQUIRE(P.depth >= 1 && P.depth <= LA_MAX_LEVELS, "lookahead: depth must be in [1, %d]", LA_MAX_LEVELS);
-    REQUIRE(P.quick_depth == 0, "lookahead: quick_depth > 0 is not supported");
+    REQUIRE(P.quick_depth >= 0, "lookahead: quick_depth must be >= 0");
     REQUIRE(P.width >= 1 && P.width2 >= 1, "lookahead: width and width2 must be positive");
     REQUIRE(P.width + 4 * P.n_concave < 512, "lookahead: width + 4*n_concave_edges must be < 512");
     REQUIRE(P.max_n_cutting >= 1, "lookahead: max_n_cutting must be positive");
@@ -1695,10 +1845,10 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off, const int* hull_tris,
                         const int* hull_tri_off, int max_iters, int width, int width2, float threshold, int depth,
-                        int max_n_cutting, int decompose_components, int decompose_components_per_iter,
+                        int quick_depth, int max_n_cutting, int decompose_components, int decompose_components_per_iter,
                         int n_concave_edges, float concave_eps, float concave_threshold, int concave_iters,
                         int merge_hulls, struct gpu_result* out, int* out_iters, double* out_stats) {
-    Params P{max_iters, width, width2, depth, 0, max_n_cutting, 0, threshold, decompose_components,
+    Params P{max_iters, width, width2, depth, quick_depth, max_n_cutting, 0, threshold, decompose_components,
              decompose_components_per_iter, n_concave_edges, concave_iters, merge_hulls, concave_eps,
              concave_threshold};
     return lookahead_run(ctx, n_meshes, verts, vert_off, tris, tri_off, hull_verts, hull_vert_off, hull_tris,

@@ -388,18 +388,18 @@ static PyObject* part_list(struct gpu_result* res) {
 
 static PyObject* m_lookahead_batch(PyObject* self, PyObject* args) {
     unsigned long long v, vo, t, to, hv, hvo, ht, hto, outp, itp;
-    int n, max_iters, width, width2, depth, max_cut, dcomp, dc_iter, n_conc, conc_iters, merge;
+    int n, max_iters, width, width2, depth, quick, max_cut, dcomp, dc_iter, n_conc, conc_iters, merge;
     float thr, conc_eps, conc_thr;
-    if (!PyArg_ParseTuple(args, "iKKKKKKKKiiifiiiiiffiiKK", &n, &v, &vo, &t, &to, &hv, &hvo, &ht, &hto, &max_iters,
-                          &width, &width2, &thr, &depth, &max_cut, &dcomp, &dc_iter, &n_conc, &conc_eps, &conc_thr,
-                          &conc_iters, &merge, &outp, &itp))
+    if (!PyArg_ParseTuple(args, "iKKKKKKKKiiifiiiiiiffiiKK", &n, &v, &vo, &t, &to, &hv, &hvo, &ht, &hto, &max_iters,
+                          &width, &width2, &thr, &depth, &quick, &max_cut, &dcomp, &dc_iter, &n_conc, &conc_eps,
+                          &conc_thr, &conc_iters, &merge, &outp, &itp))
         return NULL;
     NEED_CTX();
     struct gpu_result* res = (struct gpu_result*)calloc(n > 0 ? n : 1, sizeof(struct gpu_result));
     if (!res) return PyErr_NoMemory();
     int rc = gpu_lookahead_batch(g_ctx, n, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
                                  P(const float*, hv), P(const int*, hvo), P(const int*, ht), P(const int*, hto),
-                                 max_iters, width, width2, thr, depth, max_cut, dcomp, dc_iter, n_conc, conc_eps,
+                                 max_iters, width, width2, thr, depth, quick, max_cut, dcomp, dc_iter, n_conc, conc_eps,
                                  conc_thr, conc_iters, merge, res, P(int*, itp), P(double*, outp));
     if (rc) {
         free(res);

@@ -149,14 +149,14 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * 2048-point first passes - the milliseconds between HIP events around each
  * launch, launches and algorithmic bytes (points read, volumes written);
  * [74, 77) learned proposal: level-0 candidates kept, candidates ranked,
- * fallbacks (gpu_lookahead_set_proposal).  * quick_depth is 0. */
+ * fallbacks (gpu_lookahead_set_proposal). */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off,
                         const int* hull_tris, const int* hull_tri_off, int max_iters, int width, int width2,
-                        float threshold, int depth, int max_n_cutting, int decompose_components,
-                        int decompose_components_per_iter, int n_concave_edges, float concave_eps,
-                        float concave_threshold, int concave_iters, int merge_hulls, struct gpu_result* out,
-                        int* out_iters, double* out_stats);
+                        float threshold, int depth, int quick_depth, int max_n_cutting,
+                        int decompose_components, int decompose_components_per_iter, int n_concave_edges,
+                        float concave_eps, float concave_threshold, int concave_iters, int merge_hulls,
+                        struct gpu_result* out, int* out_iters, double* out_stats);
 
 /* replaces gpu_preprocess (csrc/preprocess.h:18-24). The PaMO remesh is not
  * part of this library: always returns -1 with an explanatory error. */

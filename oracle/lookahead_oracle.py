@@ -335,12 +335,99 @@ def _sort_parts(parts):
     return out
 
 
+def _ub(itm, total):
+    """Branch-and-bound bound of one item (cuda/la_expand.cu:434-457)."""
+    ps = np.float32(0)
+    for x in itm.lc:
+        ps = np.float32(ps + x)
+    return np.float32(ps + np.float32(itm.lc[-1] * np.float32(total - len(itm.lc))))
+
+
+def _prune_gap(itm, best_ub, total):
+    """Rough-hull volume gap above which an item's new parts are pruned
+    (cuda/la_expand.cu:380-399); 1e30 disables, -1 always prunes."""
+    nl = len(itm.lc)
+    if not (best_ub < np.float32(1e29) and 0 < nl < total):
+        return 1e30
+    ps = np.float32(0)
+    for x in itm.lc:
+        ps = np.float32(ps + x)
+    rv_thr = np.float32(best_ub - ps)
+    if rv_thr <= 0:
+        return -1.0
+    r = np.float32(rv_thr / np.float32(0.3))
+    return float(np.float32(np.float32(np.float32(r * r) * r) *
+                            np.float32(np.float32(np.float32(4.0) * PI_F) / np.float32(3.0))))
+
+
+def quick_levels(cur, extra, quick_depth, total, med):
+    """Quick levels (csrc/lookahead.c:656-727; cuda/la_refine.cu:52-351): one
+    child per item, its worst part cut at the box midpoint along the axis with
+    the most balanced halves, cost max(v+, v-) / (v+ + v- + 1e-10) in float32,
+    the first axis on ties. Axes spanning <= 2 med + 1e-6 are skipped; an item
+    with every axis skipped becomes an extra leaf (level costs padded with
+    zeros), one with no valid cut is dropped, and a level with no child at
+    all leaves the items as they are. The last quick level prunes with the
+    bound over its input items. Returns the items that end the search."""
+    ext = np.float32(np.float32(2.0) * med + np.float32(1e-6))
+    for q in range(quick_depth):
+        best_ub = np.float32(1e30)
+        if q == quick_depth - 1:
+            for itm in cur:
+                if len(itm.lc) > 0:
+                    best_ub = min(best_ub, _ub(itm, total))
+        nxt = []
+        for itm in cur:
+            wm, wh, wv, wt, wlo, whi = itm.parts[-1]
+            assert 0 < len(wv) <= 1000000 and 0 < len(wt) <= 1000000
+            span = [np.float32(whi[j] - wlo[j]) for j in range(3)]
+            if all(sp <= ext for sp in span):
+                lc = list(itm.lc) + [np.float32(0)] * (min(total, LA_MAX_LEVELS) - len(itm.lc))
+                extra.append(Item(itm.src, itm.cut0, itm.parts, lc))
+                continue
+            best, bc = None, np.float32(1e30)
+            for ax in range(3):
+                if span[ax] <= ext:
+                    continue
+                pl = np.zeros(4, np.float32)
+                pl[ax] = 1.0
+                pl[3] = -np.float32(np.float32(wlo[ax] + whi[ax]) * np.float32(0.5))
+                st, pv, pt, nv, nt = O.plane_cut(wv, wt, pl)
+                if st != 0 or len(pv) == 0 or len(nv) == 0:
+                    continue
+                pm, nmv = mesh_volume_abs(pv, pt), mesh_volume_abs(nv, nt)
+                fp, fn = np.float32(pm), np.float32(nmv)
+                cost = np.float32(max(fp, fn) / np.float32(np.float32(fp + fn) + np.float32(1e-10)))
+                if cost < bc:
+                    bc, best = cost, (pm, pv, pt, nmv, nv, nt)
+            if best is None:
+                continue
+            assert len(itm.parts) + 1 <= LA_MAX_PARTS
+            pm, pv, pt, nmv, nv, nt = best
+            gap = _prune_gap(itm, best_ub, total)
+            ph, prp = hull_volume(pv, gap, pm)
+            nh, prn = hull_volume(nv, gap, nmv)
+            ch = _sort_parts(list(itm.parts[:-1]) + [(pm, ph, pv, pt, pv.min(0), pv.max(0)),
+                                                     (nmv, nh, nv, nt, nv.min(0), nv.max(0))])
+            lc = list(itm.lc)
+            if len(lc) < LA_MAX_LEVELS:
+                lc.append(cost_rv(ch[-1][1], ch[-1][0]))
+            c_item = Item(itm.src, itm.cut0, ch, lc)
+            c_item.pruned = bool(prp or prn)
+            nxt.append(c_item)
+        if not nxt:
+            break
+        cur = nxt
+    return cur
+
+
 def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, threshold=0.05, depth=2,
               max_n_cutting=16, decompose_components=False, dc_per_iter=True, n_concave_edges=32,
-              concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None):
+              concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None, quick_depth=0):
     """Returns (parts, iters): parts = [(v, t, hv, ht, mesh_vol, hull_vol, hausdorff)].
     deadline (time.perf_counter() value): stop before an iteration that would
-    start after it (bounded CPU-baseline samples)."""
+    start after it (bounded CPU-baseline samples). quick_depth: quick levels
+    after the full ones (csrc/lookahead.c:656-727)."""
     import time
     tau = np.float32(threshold)
     v0 = np.asarray(verts, np.float32)
@@ -352,6 +439,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
     med = np.float32(tau * np.float32(0.25))
     if med > np.float32(0.015):
         med = np.float32(0.015)
+    total = depth + quick_depth  # csrc/lookahead.c:510
 
     def run_components():
         nonlocal parts
@@ -420,7 +508,9 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
         leaf_recs = []
         for d in range(depth):
             last = d == depth - 1
-            full = (not last) or depth == 1
+            # the last full level is only scored (branch-and-bound, no caps)
+            # unless quick levels follow (csrc/lookahead.c:589)
+            full = (not last) or depth == 1 or quick_depth > 0
             w = width if d == 0 else width2
             nxt = []
             best_ub = np.float32(1e30)
@@ -431,7 +521,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
                     ps = np.float32(0)
                     for x in itm.lc:
                         ps = np.float32(ps + x)
-                    ub = np.float32(ps + np.float32(itm.lc[-1] * np.float32(depth - len(itm.lc))))
+                    ub = np.float32(ps + np.float32(itm.lc[-1] * np.float32(total - len(itm.lc))))
                     best_ub = min(best_ub, ub)
             any_child = False
             for itm in cur:
@@ -449,7 +539,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
                     small = all(np.float32(whi[j] - wlo[j]) <= ext for j in range(3))
                     if small:
                         if k == 0:
-                            lc = list(itm.lc) + [np.float32(0)] * (min(depth, LA_MAX_LEVELS) - len(itm.lc))
+                            lc = list(itm.lc) + [np.float32(0)] * (min(total, LA_MAX_LEVELS) - len(itm.lc))
                             extra.append(Item(itm.src, itm.cut0, itm.parts, lc))
                         continue
                     if k < w:
@@ -489,7 +579,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
                             continue
                         gap = 1e30
                         nl = len(itm.lc)
-                        if best_ub < np.float32(1e29) and 0 < nl < depth:
+                        if best_ub < np.float32(1e29) and 0 < nl < total:
                             ps = np.float32(0)
                             for x in itm.lc:
                                 ps = np.float32(ps + x)
@@ -516,11 +606,10 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
                     leaves.extend(cur)
                 cur = []
                 break
-            if not any_child and d > 0:
-                leaves.extend(cur)
+            if not any_child:
+                break  # the items stay (csrc/lookahead.c:579-581)
             cur = nxt
-            if not cur:
-                break
+        cur = quick_levels(cur, extra, quick_depth, total, med)
         leaves.extend(cur)
         # evaluate + apply
         tw0 = width + n_edge

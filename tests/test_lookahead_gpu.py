@@ -152,3 +152,41 @@ def test_parity_with_serial_oracle_bench_parameters(ctx):
         for (gv, gt, ghv, ght), rp in zip(g, r):
             assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
             assert ghv.tobytes() == rp[2].tobytes() and np.array_equal(ght, rp[3]), ci
+
+
+@pytest.mark.parametrize("quick,depth", [(1, 2), (2, 2), (1, 1), (2, 1)])
+def test_parity_with_serial_oracle_quick_levels(ctx, quick, depth):
+    """Quick levels after the full ones (csrc/lookahead.c:656-727,
+    cuda/la_refine.cu:52-351): the batched GPU search over four solids makes
+    the serial oracle's decisions for each, part for part, and frees every
+    half it built; the single-mesh entry agrees with the batch."""
+    import lookahead_oracle as LO
+    cases = [(_rotated(datasets.l_shape_union()[0], 1), datasets.l_shape_union()[1]),
+             (_rotated(datasets.hollow_tube()[0], 2), datasets.hollow_tube()[1])]
+    cases += sdf.sdf_meshes(2, seed=11, res=16)
+    cases = [(_normalize(v), t) for v, t in cases]
+    kw = dict(max_iters=12, width=12, width2=3, threshold=0.05, depth=depth, quick_depth=quick, n_concave_edges=4,
+              max_n_cutting=4)
+    print(f"quick levels: quick_depth {quick} depth {depth}", flush=True)
+    before = ctx.heap_stats()
+    gpu, info = ctx.batch_lookahead_decompose(cases, return_info=True, **kw)
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)  # every quick half freed
+    for ci, (v, t) in enumerate(cases):
+        hv, ht = ctx._input_hull(v)
+        ref, iters = LO.lookahead(v, t, hv, ht, **kw)
+        assert info[ci]["iters"] == iters, (ci, info[ci]["iters"], iters)
+        assert len(gpu[ci]) == len(ref), (ci, len(gpu[ci]), len(ref))
+        key = lambda p: (len(p[1]), p[0].tobytes())
+        g = sorted(gpu[ci], key=key)
+        r = sorted(ref, key=key)
+        for (gv, gt, ghv, ght), rp in zip(g, r):
+            assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
+            assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, ci
+        if info[ci]["iters"] < kw["max_iters"]:
+            cost = _check_parts(gpu[ci], v, t, info[ci])
+            assert (cost < kw["threshold"]).all(), (ci, cost.max())
+    single = ctx.lookahead_decompose(*cases[2], **kw)
+    assert len(single) == len(gpu[2])
+    for (a, b, _, _), (c, d, _, _) in zip(single, gpu[2]):
+        assert a.tobytes() == c.tobytes() and np.array_equal(b, d)

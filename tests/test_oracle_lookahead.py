@@ -78,3 +78,68 @@ def test_oracle_search_l_shape_and_tube():
             assert oracle.edge_manifold_report(p[1]) == (0, 0)
             if iters < 20:
                 assert max(LO.cost_rv(p[5], p[4]), p[6]) < np.float32(0.05)
+
+
+def _quick_item(v, t, lc):
+    v = np.asarray(v, np.float32)
+    t = np.asarray(t, np.int32)
+    h, _ = LO.hull_volume(v)
+    return LO.Item(0, 3, [(LO.mesh_volume_abs(v, t), h, v, t, v.min(0), v.max(0))], list(lc))
+
+
+def test_quick_level_most_balanced_axis():
+    """The L-shape (box [0,2]x[0,1]x[0,1] plus box [0,1]x[1,2]x[0,1]), cut at
+    its box midpoints: x and y split the volume 1 : 2 (cost 2/3), z splits it
+    1.5 : 1.5 (cost 1/2), so the quick child is the z cut. Each half is an
+    L-shaped slab whose hull exceeds it by 0.25, which fixes the level cost."""
+    v, t = datasets.l_shape_union()
+    extra = []
+    out = LO.quick_levels([_quick_item(v, t, [np.float32(0.2)])], extra, 1, 3, np.float32(0.0125))
+    assert len(out) == 1 and extra == []
+    c = out[0]
+    assert c.cut0 == 3 and not c.pruned and len(c.parts) == 2
+    assert sorted((float(p[4][2]), float(p[5][2])) for p in c.parts) == [(0.0, 0.5), (0.5, 1.0)]
+    assert all(abs(p[0] - 1.5) < 1e-12 for p in c.parts)
+    rv = 0.3 * np.cbrt(3.0 / (4.0 * math.pi) * 0.25)
+    assert len(c.lc) == 2 and c.lc[0] == np.float32(0.2) and abs(float(c.lc[1]) - rv) < 1e-6
+
+
+def test_quick_level_tie_takes_first_axis_and_skips_thin_axes():
+    """Flattened to 0.01 in z, the z axis spans less than 2 med + 1e-6 and is
+    skipped; x and y tie at 2/3 and the first axis (x, midpoint 1) wins."""
+    v, t = datasets.l_shape_union()
+    v = np.asarray(v, np.float32).copy()
+    v[:, 2] *= np.float32(0.01)
+    out = LO.quick_levels([_quick_item(v, t, [np.float32(0.2)])], [], 1, 3, np.float32(0.0125))
+    xs = sorted((float(p[4][0]), float(p[5][0]), round(p[0], 9)) for p in out[0].parts)
+    assert xs == [(0.0, 1.0, 0.02), (1.0, 2.0, 0.01)]
+
+
+def test_quick_level_small_part_is_extra_leaf():
+    """A worst part spanning <= 2 med + 1e-6 along every axis is not cut: the
+    item becomes an extra leaf with its level costs padded with zeros to the
+    total number of levels, and with no child at all the level keeps the
+    items as they are."""
+    v, t = datasets.l_shape_union()
+    v = np.asarray(v, np.float32) * np.float32(0.005)
+    it = _quick_item(v, t, [np.float32(0.2)])
+    extra = []
+    out = LO.quick_levels([it], extra, 2, 4, np.float32(0.0125))
+    assert out == [it]
+    assert len(extra) == 1 and extra[0].lc == [np.float32(0.2), 0.0, 0.0, 0.0]
+
+
+def test_oracle_search_with_quick_levels():
+    """quick_depth 1 and 2 after two full levels: the decomposition still
+    partitions the solid into closed parts below the threshold."""
+    v, t = datasets.l_shape_union()
+    v = _norm(v)
+    hv, ht, _ = LO.hull_mesh(v)
+    for q in (1, 2):
+        parts, iters = LO.lookahead(v, t, hv, ht, max_iters=20, width=12, width2=3, n_concave_edges=4, quick_depth=q)
+        assert iters < 20 and len(parts) >= 2
+        total = abs(oracle.mesh_volume(v, t))
+        assert abs(sum(abs(oracle.mesh_volume(p[0], p[1])) for p in parts) - total) < 1e-5 * total
+        for p in parts:
+            assert oracle.edge_manifold_report(p[1]) == (0, 0)
+            assert max(LO.cost_rv(p[5], p[4]), p[6]) < np.float32(0.05)
