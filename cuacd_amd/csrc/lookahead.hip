@@ -950,7 +950,11 @@ struct Engine {
             }
         }
         // greedy matching per mesh: repeatedly the cheapest pair under the
-        // threshold (lowest pair index on ties), then drop its two parts
+        // threshold, then drop its two parts. Ties go the way the reference's
+        // one-block matching resolves them (cuda/postprocess_merge.cu:404-480):
+        // 256 threads each keep the first minimum of their stride of the pair
+        // list, and a tree reduction keeps the left operand unless the right one
+        // is strictly smaller.
         size_t b = 0;
         while (b < pairs.size()) {
             const int m = pairs[b].m;
@@ -960,15 +964,24 @@ struct Engine {
             const int np = (int)S.parts.size();
             std::vector<char> used(np, 0);
             std::vector<std::pair<int, int>> match;
+            constexpr int MT = 256;
+            float vals[MT];
+            long long idxs[MT];
             for (;;) {
-                float bc = tau;
-                size_t bi = SIZE_MAX;
-                for (size_t i = b; i < e; ++i) {
-                    const MPair& q = pairs[i];
-                    if (used[q.p1] || used[q.p2]) continue;
-                    if (q.cost < bc) { bc = q.cost; bi = i; }
+                for (int t = 0; t < MT; ++t) {
+                    vals[t] = tau;
+                    idxs[t] = -1;
+                    for (size_t i = b + t; i < e; i += MT) {
+                        const MPair& q = pairs[i];
+                        if (used[q.p1] || used[q.p2]) continue;
+                        if (q.cost < vals[t]) { vals[t] = q.cost; idxs[t] = (long long)i; }
+                    }
                 }
-                if (bi == SIZE_MAX) break;
+                for (int off = MT / 2; off > 0; off >>= 1)
+                    for (int t = 0; t < off; ++t)
+                        if (vals[t + off] < vals[t]) { vals[t] = vals[t + off]; idxs[t] = idxs[t + off]; }
+                if (idxs[0] < 0 || vals[0] >= tau) break;
+                const size_t bi = (size_t)idxs[0];
                 used[pairs[bi].p1] = used[pairs[bi].p2] = 1;
                 match.push_back({(int)bi, 0});
             }

@@ -421,9 +421,156 @@ def quick_levels(cur, extra, quick_depth, total, med):
     return cur
 
 
+MERGE_INVALID, MERGE_RV_REJECT, MERGE_BBOX_REJECT = np.float32(1e20), np.float32(1e10), np.float32(1e9)
+
+
+def shared_plane(av, at, bv, tol=np.float32(1e-3)):
+    """First face of hull A (in face order) whose outward plane has all of A's
+    hull vertices within tol on its inner side and all of B's hull vertices
+    within tol of it (cuda/merge.cuh:111-165), as (nx, ny, nz, d) in float32;
+    None if there is none."""
+    f = np.float32
+    av = np.asarray(av, np.float32)
+    bv = np.asarray(bv, np.float32)
+    for tri in np.asarray(at, np.int64):
+        a, b, c = av[tri[0]], av[tri[1]], av[tri[2]]
+        e1 = [f(b[j] - a[j]) for j in range(3)]
+        e2 = [f(c[j] - a[j]) for j in range(3)]
+        nx = f(f(e1[1] * e2[2]) - f(e1[2] * e2[1]))
+        ny = f(f(e1[2] * e2[0]) - f(e1[0] * e2[2]))
+        nz = f(f(e1[0] * e2[1]) - f(e1[1] * e2[0]))
+        ln = f(np.sqrt(f(f(f(nx * nx) + f(ny * ny)) + f(nz * nz))))
+        if ln < f(1e-20):
+            continue
+        inv = f(f(1.0) / ln)
+        nx, ny, nz = f(nx * inv), f(ny * inv), f(nz * inv)
+        d = f(f(f(nx * a[0]) + f(ny * a[1])) + f(nz * a[2]))
+        sa = ((nx * av[:, 0] + ny * av[:, 1]) + nz * av[:, 2]) - d
+        if (sa > tol).any():
+            continue
+        sb = ((nx * bv[:, 0] + ny * bv[:, 1]) + nz * bv[:, 2]) - d
+        if (np.abs(sb) > tol).any():
+            continue
+        return nx, ny, nz, d
+    return None
+
+
+def merge_match(cost, tau):
+    """Greedy matching of the reference (cuda/postprocess_merge.cu:404-480):
+    256 threads each take the first minimum of their stride of the pair costs
+    (below tau), a tree reduction keeps the left operand unless the right one
+    is strictly smaller, the winning pair is recorded and every pair sharing a
+    part with it is invalidated; until no pair is below tau. Pair index
+    idx = p1 (p1 - 1) / 2 + p2, p1 > p2. Returns [(p1, p2)] in match order."""
+    c = np.array(cost, np.float32)
+    n = len(c)
+    P = 1
+    while P * (P - 1) // 2 < n:
+        P += 1
+    tau = np.float32(tau)
+    out = []
+    while True:
+        vals = np.full(256, tau, np.float32)
+        idxs = np.full(256, -1, np.int64)
+        for tid in range(256):
+            for i in range(tid, n, 256):
+                if c[i] < vals[tid]:
+                    vals[tid], idxs[tid] = c[i], i
+        off = 128
+        while off > 0:
+            for tid in range(off):
+                if vals[tid + off] < vals[tid]:
+                    vals[tid], idxs[tid] = vals[tid + off], idxs[tid + off]
+            off >>= 1
+        bi = int(idxs[0])
+        if bi < 0 or vals[0] >= tau:
+            break
+        p1 = 1
+        while p1 * (p1 + 1) // 2 <= bi:
+            p1 += 1
+        p2 = bi - p1 * (p1 - 1) // 2
+        out.append((p1, p2))
+        c[bi] = MERGE_INVALID
+        for k in range(P):
+            for a, b in ((p1, k), (p2, k), (k, p1), (k, p2)):
+                if a > b:
+                    c[a * (a - 1) // 2 + b] = MERGE_INVALID
+    return out
+
+
+def merge_pass(parts, tau):
+    """The merge-hulls post-pass, one round (csrc/lookahead.c:988-1067;
+    cuda/postprocess_merge.cu): per pair (p1 > p2) of the same connected
+    component (or unassigned), reject when the rv cost of the summed hull
+    volumes is >= tau or the boxes are >= tau / 2 apart; else the exact hull
+    of both hulls' vertices gives the merged rv cost, and below tau the
+    Hausdorff distance between that hull and both meshes (triangles on a
+    shared hull face dropped) raises it to max(rv, H). Matched pairs become
+    one part in slot p1 (A's vertices and triangles, then B's), the other slot
+    is removed, the order of the rest is kept."""
+    f = np.float32
+    tau = f(tau)
+    P = len(parts)
+    if P < 2:
+        return parts
+    cost = []
+    hulls = {}
+    for p1 in range(1, P):
+        for p2 in range(p1):
+            A, B = parts[p1], parts[p2]
+            if len(A.v) == 0 or len(B.v) == 0 or A.hv is None or B.hv is None:
+                cost.append(MERGE_INVALID)
+                continue
+            if A.cc != 0 and B.cc != 0 and A.cc != B.cc:
+                cost.append(MERGE_INVALID)
+                continue
+            msum = A.mvol + B.mvol
+            if cost_rv(A.hvol + B.hvol, msum) >= tau:
+                cost.append(MERGE_RV_REJECT)
+                continue
+            g = [max(f(0), max(f(A.lo[j] - B.hi[j]), f(B.lo[j] - A.hi[j]))) for j in range(3)]
+            g2 = f(f(f(g[0] * g[0]) + f(g[1] * g[1])) + f(g[2] * g[2]))
+            half = f(tau * f(0.5))
+            if g2 >= f(half * half):
+                cost.append(MERGE_BBOX_REJECT)
+                continue
+            hv, ht, hvol = hull_mesh(np.concatenate([A.hv, B.hv]))
+            if len(ht) == 0:
+                cost.append(MERGE_INVALID)
+                continue
+            rv = cost_rv(hvol, msum)
+            if rv >= tau:
+                cost.append(rv)
+                continue
+            pl = shared_plane(A.hv, A.ht, B.hv)
+            cv = np.concatenate([A.v, B.v])
+            ct = np.concatenate([A.t, B.t + len(A.v)]).astype(np.int32)
+            if pl is not None:
+                nx, ny, nz, d = pl
+                s = np.abs(((nx * cv[:, 0] + ny * cv[:, 1]) + nz * cv[:, 2]) - d) <= f(1e-3)
+                ct = ct[~s[ct].all(axis=1)]
+            h = hausdorff(hv, ht, cv, ct) if len(ct) else f(0)
+            c = max(rv, f(h))
+            cost.append(c)
+            if c < tau:
+                hulls[(p1, p2)] = (hv, ht, hvol)
+    out = list(parts)
+    dead = set()
+    for p1, p2 in merge_match(cost, tau):
+        A, B = parts[p1], parts[p2]
+        hv, ht, hvol = hulls[(p1, p2)]
+        m = Part(np.concatenate([A.v, B.v]), np.concatenate([A.t, B.t + len(A.v)]).astype(np.int32),
+                 A.mvol + B.mvol, hvol, hv, ht, cc=A.cc if A.cc else B.cc)
+        out[p1] = m
+        dead.add(p2)
+    return [q for i, q in enumerate(out) if i not in dead]
+
+
+
 def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, threshold=0.05, depth=2,
               max_n_cutting=16, decompose_components=False, dc_per_iter=True, n_concave_edges=32,
-              concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None, quick_depth=0):
+              concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None, quick_depth=0,
+              merge_hulls=False):
     """Returns (parts, iters): parts = [(v, t, hv, ht, mesh_vol, hull_vol, hausdorff)].
     deadline (time.perf_counter() value): stop before an iteration that would
     start after it (bounded CPU-baseline samples). quick_depth: quick levels
@@ -662,4 +809,6 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
     if decompose_components:
         run_components()
         run_hulls()
+    if merge_hulls:
+        parts = merge_pass(parts, tau)
     return [(p.v, p.t, p.hv, p.ht, p.mvol, p.hvol, p.haus) for p in parts], iters

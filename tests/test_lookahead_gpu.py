@@ -190,3 +190,36 @@ def test_parity_with_serial_oracle_quick_levels(ctx, quick, depth):
     assert len(single) == len(gpu[2])
     for (a, b, _, _), (c, d, _, _) in zip(single, gpu[2]):
         assert a.tobytes() == c.tobytes() and np.array_equal(b, d)
+
+
+def test_parity_with_serial_oracle_merge(ctx):
+    """merge_hulls=True (csrc/lookahead.c:988-1067, cuda/postprocess_merge.cu):
+    on four solids where the pass merges 1-9 pairs, the batched GPU search
+    and the serial oracle end with the same parts, merged ones included
+    (vertex bits, triangles, hull volume), and every chunk is freed."""
+    import lookahead_oracle as LO
+    cases = [(_rotated(datasets.hollow_tube()[0], 2), datasets.hollow_tube()[1])]
+    cases += sdf.sdf_meshes(4, seed=21, res=16)[:3]
+    cases = [(_normalize(v), t) for v, t in cases]
+    kw = dict(max_iters=12, width=12, width2=3, threshold=0.05, depth=2, n_concave_edges=4, max_n_cutting=4,
+              merge_hulls=True)
+    before = ctx.heap_stats()
+    gpu, info = ctx.batch_lookahead_decompose(cases, return_info=True, **kw)
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)
+    plain = ctx.batch_lookahead_decompose(cases, **dict(kw, merge_hulls=False))
+    merged_any = False
+    for ci, (v, t) in enumerate(cases):
+        hv, ht = ctx._input_hull(v)
+        ref, iters = LO.lookahead(v, t, hv, ht, **kw)
+        assert info[ci]["iters"] == iters, (ci, info[ci]["iters"], iters)
+        assert len(gpu[ci]) == len(ref), (ci, len(gpu[ci]), len(ref))
+        merged_any |= len(gpu[ci]) < len(plain[ci])
+        key = lambda p: (len(p[1]), p[0].tobytes())
+        g = sorted([p + (info[ci]["hull_vol"][k],) for k, p in enumerate(gpu[ci])], key=key)
+        r = sorted(ref, key=key)
+        for k, ((gv, gt, ghv, ght, ghvol), rp) in enumerate(zip(g, r)):
+            assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), (ci, k)
+            assert abs(float(ghvol) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, (ci, k)
+            assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, (ci, k)
+    assert merged_any

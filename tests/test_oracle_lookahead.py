@@ -143,3 +143,67 @@ def test_oracle_search_with_quick_levels():
         for p in parts:
             assert oracle.edge_manifold_report(p[1]) == (0, 0)
             assert max(LO.cost_rv(p[5], p[4]), p[6]) < np.float32(0.05)
+
+
+def _bitrev8(x):
+    return int(f"{x:08b}"[::-1], 2)
+
+
+def test_merge_match_tie_rule():
+    """The oracle's simulation of the reference's 256-thread matching equals
+    its closed form: the cheapest pair under tau, ties to the smallest
+    bit-reversed (index mod 256), then the smallest index; pairs sharing a
+    part with a match drop out. Ties are dense here (four cost levels)."""
+    rng = np.random.default_rng(5)
+    levels = np.array([0.0, 0.01, 0.02, 1e20], np.float32)
+    for P in (5, 30, 40, 60):
+        n = P * (P - 1) // 2
+        for _ in range(3):
+            c = levels[rng.integers(0, 4, n)]
+            got = LO.merge_match(c, 0.05)
+            want, used = [], set()
+            pairs = [(p1, p2) for p1 in range(1, P) for p2 in range(p1)]
+            while True:
+                ok = [i for i, (a, b) in enumerate(pairs) if c[i] < np.float32(0.05) and a not in used and b not in used]
+                if not ok:
+                    break
+                i = min(ok, key=lambda i: (c[i], _bitrev8(i % 256), i))
+                want.append(pairs[i])
+                used.update(pairs[i])
+            assert got == want
+    # a tie the lowest-index rule would resolve the other way: index 254
+    # (23, 1) and index 256 (23, 3) cost the same; thread 0 beats thread 254
+    c = np.full(300, 1e20, np.float32)
+    c[254] = c[256] = 0.0
+    assert LO.merge_match(c, 0.05) == [(23, 3)]
+
+
+def _part(lo, hi, cc=0):
+    v, t = datasets.box_mesh(lo, hi)
+    hv, ht, hvol = LO.hull_mesh(v)
+    return LO.Part(v, t, LO.mesh_volume_abs(v, t), hvol, hv, ht, cc=cc)
+
+
+def test_merge_pass_thin_slabs():
+    """Two 1 x 1 x 0.02 slabs sharing the face x = 1 merge: their hull is the
+    2 x 1 x 0.02 box (no volume gap) and the shared face lies within 0.01 of
+    the hull's surface, under tau. The merged part is A's vertices and
+    triangles, then B's, in A's slot (the higher index)."""
+    a, b = _part((0, 0, 0), (1, 1, 0.02)), _part((1, 0, 0), (2, 1, 0.02))
+    out = LO.merge_pass([a, b], 0.05)
+    assert len(out) == 1
+    m = out[0]
+    assert np.array_equal(m.v, np.concatenate([b.v, a.v]))
+    assert np.array_equal(m.t, np.concatenate([b.t, a.t + 8]))
+    assert abs(m.mvol - 0.04) < 1e-9 and abs(m.hvol - 0.04) < 1e-9 and len(m.hv) == 8
+
+
+def test_merge_pass_rejections():
+    """No merge: slabs 0.03 apart (box gap >= tau / 2); unit cubes sharing a
+    face (the shared face reaches 0.5 inside the merged hull, so the Hausdorff
+    distance is 0.5); slabs of different connected components."""
+    f = LO.merge_pass
+    assert len(f([_part((0, 0, 0), (1, 1, 0.02)), _part((1.03, 0, 0), (2, 1, 0.02))], 0.05)) == 2
+    assert len(f([_part((0, 0, 0), (1, 1, 1)), _part((1, 0, 0), (2, 1, 1))], 0.05)) == 2
+    assert len(f([_part((0, 0, 0), (1, 1, 0.02), cc=1), _part((1, 0, 0), (2, 1, 0.02), cc=2)], 0.05)) == 2
+    assert len(f([_part((0, 0, 0), (1, 1, 0.02), cc=1), _part((1, 0, 0), (2, 1, 0.02), cc=1)], 0.05)) == 1
