@@ -207,3 +207,19 @@ def test_merge_pass_rejections():
     assert len(f([_part((0, 0, 0), (1, 1, 1)), _part((1, 0, 0), (2, 1, 1))], 0.05)) == 2
     assert len(f([_part((0, 0, 0), (1, 1, 0.02), cc=1), _part((1, 0, 0), (2, 1, 0.02), cc=2)], 0.05)) == 2
     assert len(f([_part((0, 0, 0), (1, 1, 0.02), cc=1), _part((1, 0, 0), (2, 1, 0.02), cc=1)], 0.05)) == 1
+
+
+def test_shared_plane_closed_form():
+    """A unit cube's hull and a flat square on its face x = 1: that face's
+    outward plane (1, 0, 0, 1) has the cube on its inner side and the square
+    on it. Lifted 5e-4 off the face the square is still within the 1e-3
+    tolerance; lifted 2e-3 it is not, and a square beside the cube (not on any
+    face plane) finds nothing."""
+    v, t = datasets.box_mesh()
+    hv, ht, _ = LO.hull_mesh(v)
+    sq = np.array([[1, 0.2, 0.2], [1, 0.8, 0.2], [1, 0.8, 0.8], [1, 0.2, 0.8]], np.float32)
+    pl = LO.shared_plane(hv, ht, sq)
+    assert pl is not None and np.allclose(pl, (1, 0, 0, 1), atol=1e-7)
+    assert LO.shared_plane(hv, ht, sq + np.float32([5e-4, 0, 0])) is not None
+    assert LO.shared_plane(hv, ht, sq + np.float32([2e-3, 0, 0])) is None
+    assert LO.shared_plane(hv, ht, sq + np.float32([0.5, 0, 0])) is None
