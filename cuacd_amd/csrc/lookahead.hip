@@ -1059,11 +1059,13 @@ struct Engine {
             const float inv = 1.0f / len;
             nx *= inv; ny *= inv; nz *= inv;
             const float d = nx * a[0] + ny * a[1] + nz * a[2];
+            // B on the plane first: it fails at once unless B is flat, while A
+            // (whose own hull face this is) always passes its test
             bool ok = true;
-            for (int j = 0; j < a_hnv && ok; ++j)
-                if (nx * av[3 * j] + ny * av[3 * j + 1] + nz * av[3 * j + 2] - d > tol) ok = false;
             for (int j = 0; j < b_hnv && ok; ++j)
                 if (std::fabs(nx * bv[3 * j] + ny * bv[3 * j + 1] + nz * bv[3 * j + 2] - d) > tol) ok = false;
+            for (int j = 0; j < a_hnv && ok; ++j)
+                if (nx * av[3 * j] + ny * av[3 * j + 1] + nz * av[3 * j + 2] - d > tol) ok = false;
             if (ok) {
                 out[0] = nx; out[1] = ny; out[2] = nz; out[3] = d;
                 return true;
