@@ -7,7 +7,7 @@ width/width2 30/5, depth 2, quick depth 0, 16 concave edges, 16 parts cut in
 parallel, at most 100 iterations, hull merge on. A step is one batched search
 over all 64 meshes (the meshes are independent; every launch spans all of
 them). Metric: meshes per second. After the timed plain steps, one step of
-the learned-proposal form (the shipped network keeps the top 20 level-0
+the learned-proposal form (the shipped network keeps the top 16 level-0
 candidates of each cutting part) is timed and reported under "learned".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--meshes N]
@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run the known-size device copy (HBM counter calibration) and exit")
-    ap.add_argument("--top-k", type=int, default=20,
+    ap.add_argument("--top-k", type=int, default=16,
                     help="learned-proposal step after the plain steps (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
