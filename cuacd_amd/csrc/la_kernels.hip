@@ -1209,7 +1209,10 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
 //     default 16384, the most one workgroup hulls)
 //   5 multi: multi-apex rounds in the eight-wave LDS tier (1025-2048 points;
 //     CUACD_HULL_MULTI, 0/1; default 1)
-constexpr int LA_TUNE_N = 6;
+//   6 mlp_fused: the proposal network's inference runs its point layers and
+//     pooling as one kernel per candidate instead of layer-by-layer GEMMs
+//     (CUACD_MLP_FUSED, 0/1; default 1; bit-identical scores)
+constexpr int LA_TUNE_N = 7;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1227,8 +1230,8 @@ int tune_clamp(int key, int v) {
 void tune_init() {
     if (g_tune_init) return;
     const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN", "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP", "CUACD_KDOP_MIN",
-                                  "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1};
+                                  "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI", "CUACD_MLP_FUSED"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1686,6 +1689,11 @@ int la_launch_gather(gpu_ctx_t ctx, LaGatherTask* d_tasks, int n) {
     hipLaunchKernelGGL(la_gather_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+int mlp_fused_on() {
+    tune_init();
+    return g_tune[6];
 }
 
 int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n) {

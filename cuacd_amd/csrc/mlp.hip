@@ -166,6 +166,162 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(GemmArgs g) {
         }
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference of the point layers and the pooling: a persistent grid of
+// 256-thread workgroups, each with the transposed point-layer weights in LDS,
+// strides over the candidates; per candidate wave w owns samples 64w..64w+63
+// (4 x 8 MFMA 16 x 16 x 16 tiles) and the activations stay in LDS as BF16.
+// The MFMA operand layout, the k order (16-wide steps, ascending; layer 1
+// adds the GEMM's second, all-zero step), the epilogue (bias, ReLU, BF16
+// rounding) and the pooling rule are those of mlp_gemm_kernel and
+// mlp_maxpool, so G equals the layer-by-layer path's bit for bit; only
+// G [C x 256] reaches memory.
+// ---------------------------------------------------------------------------
+constexpr int FP = HD + 8;    // activation / weight row pitch in LDS (bf16)
+constexpr int FP1 = KIN + 8;  // pitch of the transposed first layer
+constexpr unsigned FUSED_LDS = (unsigned)(NS * FP * 2 + 2 * HD * FP * 2 + HD * FP1 * 2 + NS * 4 + 2 * 4 * HD * 4);
+
+__device__ __forceinline__ s4v_t lds_frag(const bf16_t* p) { return *reinterpret_cast<const s4v_t*>(p); }
+
+__global__ __launch_bounds__(256) void mlp_encode_pool_kernel(const float* pts, const float* planes,
+                                                              const int* part_of, int C, const bf16_t* W,
+                                                              const float* Wf, float* G) {
+    extern __shared__ __align__(16) char fl[];
+    bf16_t* H = (bf16_t*)fl;                    // [NS][FP] X, then the layer outputs
+    bf16_t* W2t = H + NS * FP;                  // [HD][FP] W2 transposed: W2t[n][k]
+    bf16_t* W3t = W2t + HD * FP;                // [HD][FP]
+    bf16_t* W1t = W3t + HD * FP;                // [HD][FP1]
+    float* Sd = (float*)(W1t + HD * FP1);       // [NS] signed distances
+    float* red = Sd + NS;                       // [2][4][HD] per-wave side maxima
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i < HD * HD; i += 256) {
+        const int k = i / HD, n = i % HD;
+        W2t[n * FP + k] = W[P_W2 + i];
+        W3t[n * FP + k] = W[P_W3 + i];
+    }
+    for (int i = tid; i < KIN * HD; i += 256) {
+        const int k = i / HD, n = i % HD;
+        W1t[n * FP1 + k] = W[P_W1 + i];
+    }
+    const int r0 = 64 * wv;
+    const int fr = lane & 15, fk = 4 * (lane >> 4);
+    for (int c = blockIdx.x; c < C; c += gridDim.x) {
+        __syncthreads();
+        {
+            // sample tid of the candidate's part (the rows of mlp_pack_points)
+            const float* pp = pts + ((long long)part_of[c] * NS + tid) * 3;
+            const float* pl = planes + 4ll * c;
+            const float sd = pl[0] * pp[0] + pl[1] * pp[1] + pl[2] * pp[2] + pl[3];
+            Sd[tid] = sd;
+            bf16_t* x = H + tid * FP;
+            x[0] = f2bf(pp[0]);
+            x[1] = f2bf(pp[1]);
+            x[2] = f2bf(pp[2]);
+            x[3] = f2bf(sd);
+            for (int j = 4; j < KIN; ++j) x[j] = 0;
+        }
+        __syncthreads();
+        f4v_t acc[4][8];
+        // layer 1: K = 16
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] = f4v_t{0.f, 0.f, 0.f, 0.f};
+        {
+            s4v_t a[4], b[8];
+            const s4v_t z = s4v_t{0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = lds_frag(H + (r0 + 16 * i + fr) * FP + fk);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[j] = lds_frag(W1t + (16 * j + fr) * FP1 + fk);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(z, z, acc[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+        auto store = [&](const float* bias) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int n = 16 * j + fr;
+                    const float bn = bias[n];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        float v = 1.0f * acc[i][j][r];
+                        v += bn;
+                        v = fmaxf(v, 0.0f);
+                        H[(r0 + 16 * i + fk + r) * FP + n] = f2bf(v);
+                    }
+                }
+        };
+        auto layer = [&](const bf16_t* Wt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = f4v_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+            for (int k0 = 0; k0 < HD; k0 += 16) {
+                s4v_t a[4], b[8];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = lds_frag(H + (r0 + 16 * i + fr) * FP + k0 + fk);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) b[j] = lds_frag(Wt + (16 * j + fr) * FP + k0 + fk);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+        };
+        store(Wf + P_B1);
+        __syncthreads();
+        layer(W2t);
+        __syncthreads();
+        store(Wf + P_B2);
+        __syncthreads();
+        layer(W3t);
+        // layer 3 epilogue: BF16-rounded activations pooled per side of the plane
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int n = 16 * j + fr;
+            const float bn = Wf[P_B3 + n];
+            float bp = -1e30f, bq = -1e30f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = 1.0f * acc[i][j][r];
+                    v += bn;
+                    v = fmaxf(v, 0.0f);
+                    v = bf2f(f2bf(v));
+                    if (Sd[r0 + 16 * i + fk + r] >= 0.0f) bp = fmaxf(bp, v);
+                    else bq = fmaxf(bq, v);
+                }
+            bp = fmaxf(bp, __shfl_xor(bp, 16));
+            bp = fmaxf(bp, __shfl_xor(bp, 32));
+            bq = fmaxf(bq, __shfl_xor(bq, 16));
+            bq = fmaxf(bq, __shfl_xor(bq, 32));
+            if (lane < 16) {
+                red[(0 * 4 + wv) * HD + n] = bp;
+                red[(1 * 4 + wv) * HD + n] = bq;
+            }
+        }
+        __syncthreads();
+        {
+            const int side = tid >> 7, n = tid & (HD - 1);
+            float m = red[(side * 4 + 0) * HD + n];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) m = fmaxf(m, red[(side * 4 + q) * HD + n]);
+            G[(long long)c * 2 * HD + tid] = m > -1e30f ? m : 0.0f;
+        }
+    }
+}
+
 // Split-K reduction with the GEMM's epilogue: C = alpha * sum_z part[z] ...
 __global__ void mlp_splitk_reduce(GemmArgs g, int nz) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,8 +639,9 @@ struct Act {
 
 // activations of C candidates (C * NS encoder rows); `train` adds the
 // backward buffers
-int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int C, bool train, Act* a) {
-    const size_t R = (size_t)C * NS;
+int act_layout(gpu_ctx_t ctx, gpu_mlp* mp, int C, bool train, Act* a, bool fused = false) {
+    // the fused inference keeps the encoder rows (X, A1-A3, S) in LDS
+    const size_t R = fused ? 0 : (size_t)C * NS;
     size_t need = 0;
     auto sz = [&](size_t n, size_t e) { need += (n * e + 255) / 256 * 256; };
     sz(R * KIN, 2); sz(R * HD, 2); sz(R * HD, 2); sz(R * HD, 2);
@@ -547,6 +704,30 @@ int forward(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, const float* d_plane
     if (int rc = gemm(ctx, mp, R, HD, HD, a.A2, HD, 1, W + P_W3, HD, 1, Wf + P_B3, 1, nullptr, 0, nullptr, 0, a.A3, HD))
         return rc;
     hipLaunchKernelGGL(mlp_maxpool, dim3((unsigned)C), dim3(HD), 0, ctx->stream, a.A3, a.S, C, a.G, a.arg);
+    HIP_TRY(hipGetLastError());
+    {
+        const long long n = (long long)C * KH;
+        hipLaunchKernelGGL(mlp_head_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, a.G,
+                           d_planes, C, a.Hin);
+        HIP_TRY(hipGetLastError());
+    }
+    if (int rc = gemm(ctx, mp, C, HD, KH, a.Hin, KH, 1, W + P_H1, HD, 1, Wf + P_C1, 1, nullptr, 0, nullptr, 0, a.B1, HD))
+        return rc;
+    if (int rc = gemm(ctx, mp, C, HD, HD, a.B1, HD, 1, W + P_H2, HD, 1, Wf + P_C2, 1, nullptr, 0, nullptr, 0, a.B2, HD))
+        return rc;
+    return gemm(ctx, mp, C, NO, HD, a.B2, HD, 1, W + P_H3, NO, 1, Wf + P_C3, 0, nullptr, 0, a.Y, NO, nullptr, 0);
+}
+
+// forward through the fused point-layer kernel (inference only): G, then the head
+int forward_fused(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, const float* d_planes, const int* d_part, int C,
+                  Act& a) {
+    const bf16_t* W = mp->wb;
+    const float* Wf = mp->w;
+    HIP_TRY(hipFuncSetAttribute((const void*)mlp_encode_pool_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)FUSED_LDS));
+    const int grid = std::max(1, std::min(C, ctx->n_cu));
+    hipLaunchKernelGGL(mlp_encode_pool_kernel, dim3((unsigned)grid), dim3(256), FUSED_LDS, ctx->stream, d_pts,
+                       d_planes, d_part, C, W, Wf, a.G);
     HIP_TRY(hipGetLastError());
     {
         const long long n = (long long)C * KH;
@@ -676,14 +857,18 @@ int stage_err(gpu_ctx_t ctx, int rc) {
 namespace cuacd {
 int mlp_forward_dev(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* d_planes, const int* d_part,
                     int C, float* d_out) {
-    // bounded activations: slices of at most 8192 candidates (2M encoder rows)
+    // bounded activations: slices of at most 8192 candidates (2M encoder
+    // rows) layer by layer; the fused kernel keeps those rows in LDS
     (void)P;
-    const int slice = 8192;
+    const bool fused = mlp_fused_on() != 0;
+    const int slice = fused ? 65536 : 8192;
     Act a;
-    if (int rc = act_layout(ctx, mp, std::min(slice, C), false, &a)) return rc;
+    if (int rc = act_layout(ctx, mp, std::min(slice, C), false, &a, fused)) return rc;
     for (int c0 = 0; c0 < C; c0 += slice) {
         const int n = std::min(slice, C - c0);
-        if (int rc = forward(ctx, mp, d_pts, d_planes + 4ll * c0, d_part + c0, n, a)) return rc;
+        if (int rc = fused ? forward_fused(ctx, mp, d_pts, d_planes + 4ll * c0, d_part + c0, n, a)
+                           : forward(ctx, mp, d_pts, d_planes + 4ll * c0, d_part + c0, n, a))
+            return rc;
         HIP_TRY(hipMemcpy2DAsync(d_out + c0, 4, a.Y, 4 * NO, 4, n, hipMemcpyDeviceToDevice, ctx->stream));
     }
     return 0;

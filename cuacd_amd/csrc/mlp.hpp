@@ -23,4 +23,8 @@ int mlp_sample_dev(gpu_ctx_t ctx, const MlpSamplePart* parts, int P, float* d_ou
 int mlp_forward_dev(gpu_ctx_t ctx, gpu_mlp* mp, const float* d_pts, int P, const float* d_planes, const int* d_part,
                     int C, float* d_out);
 
+// Tuning knob 6 (gpu_set_tuning): inference through the fused point-layer
+// kernel (1) or the layer-by-layer GEMMs (0); the scores are the same.
+int mlp_fused_on();
+
 }  // namespace cuacd

@@ -232,8 +232,9 @@ int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const
  * first passes, 0/1), 3 kdop_min (k-DOP prefilter above this many points,
  * >= 4), 4 chunk_above (prefiltered sets above it are hulled in chunks of
  * half that size first, 1024..16384), 5 multi (several apexes per quickhull
- * round in the eight-wave LDS tier, 0/1). Values are clamped to their
- * ranges. */
+ * round in the eight-wave LDS tier, 0/1), 6 mlp_fused (the proposal
+ * network's inference as one fused point-layer kernel, 0/1; same scores).
+ * Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);
 

@@ -78,6 +78,42 @@ def test_forward_and_gradients_match_oracle(ctx):
         net.close()
 
 
+def test_fused_inference_equals_layer_by_layer(ctx):
+    """The fused point-layer kernel (tuning knob mlp_fused) gives the scores
+    of the layer-by-layer GEMMs bit for bit: same MFMA tiles, k order, bias,
+    ReLU, BF16 rounding and pooling. Trained weights, a batch with every part
+    cut by many planes (more candidates than workgroups, so each workgroup
+    strides over several), planes that leave one side empty, and one
+    candidate alone."""
+    import mlp_oracle as MO
+    net = PR.ProposalNet.load_default(ctx)
+    old = ctx.get_tuning()["mlp_fused"]
+    try:
+        rng = np.random.default_rng(9)
+        P, C = 7, 2000
+        pts = rng.uniform(-1, 1, size=(P, 256, 3)).astype(np.float32)
+        n = rng.normal(size=(C, 3))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        d = rng.uniform(-0.8, 0.8, size=C)
+        d[:50] = 2.0      # every sample on the positive side
+        d[50:100] = -2.0  # every sample on the negative side
+        planes = np.concatenate([n, d[:, None]], axis=1).astype(np.float32)
+        part_of = np.sort(rng.integers(0, P, C)).astype(np.int32)
+        out = {}
+        for fused in (0, 1):
+            ctx.set_tuning("mlp_fused", fused)
+            out[fused] = net.predict(pts, planes, part_of) + np.float32(0.0)
+        assert np.array_equal(out[0], out[1]), np.abs(out[0] - out[1]).max()
+        y_ref, _ = MO.forward(net.params().astype(np.float64), pts, planes[:300], part_of[:300])
+        assert np.allclose(out[1][:300], y_ref, rtol=2e-3, atol=2e-3)
+        one = net.predict(pts[:1], planes[:1], np.zeros(1, np.int32))
+        ctx.set_tuning("mlp_fused", 0)
+        assert np.array_equal(one, net.predict(pts[:1], planes[:1], np.zeros(1, np.int32)))
+    finally:
+        ctx.set_tuning("mlp_fused", old)
+        net.close()
+
+
 def test_training_lowers_the_loss(ctx):
     rng = np.random.default_rng(4)
     P, C = 24, 240
