@@ -92,6 +92,10 @@ struct IPart {
     float lo[3], hi[3];
     int cc = 0;
     int store = -1;       // index of the owning half in the iteration's store, -1 = decomp part
+    // level-0 halves: their hull mesh (device heap chunk), adopted with the half when its cut is applied
+    float* hv = nullptr;
+    int* ht = nullptr;
+    int hnv = 0, hnt = 0;
 };
 struct Item {
     int mesh = 0, src = 0, cut0 = 0;
@@ -1160,10 +1164,12 @@ struct Engine {
         // level-0 candidates per cutting part, for the training record
         std::vector<std::vector<int>> rec_k(cutting.size());
         std::vector<std::vector<float>> rec_pl(cutting.size());
-        // the iteration's store of halves built by full cuts
+        // the iteration's store of halves built by full cuts (and the hull
+        // meshes of the level-0 ones)
         struct Half {
             float* v;
             bool keep;
+            float* hv;
         };
         std::vector<Half> store;
         std::vector<Item> extra;       // leaves for parts too small to cut
@@ -1350,9 +1356,9 @@ struct Engine {
                             nn.lo[j] = t.nlo[j]; nn.hi[j] = t.nhi[j];
                         }
                         pp.store = (int)store.size();
-                        store.push_back({t.pv, false});
+                        store.push_back({t.pv, false, nullptr});
                         nn.store = (int)store.size();
-                        store.push_back({t.nvp, false});
+                        store.push_back({t.nvp, false, nullptr});
                         ch.parts[par.np - 1] = pp;
                         ch.parts[par.np] = nn;
                         ch.np = par.np + 1;
@@ -1360,10 +1366,12 @@ struct Engine {
                         const int ci = (int)nxt.size();
                         nxt.push_back(ch);
                         nxt_c.push_back(cur_c[cs[q].item]);
+                        // level-0 halves keep their hull mesh: the two of the
+                        // applied cut become parts that need it
                         LaHullTask a{};
-                        a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30;
+                        a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30; a.emit = d == 0;
                         LaHullTask b{};
-                        b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30;
+                        b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30; b.emit = d == 0;
                         ht.push_back(a);
                         ht_who.push_back({ci, 0});
                         ht.push_back(b);
@@ -1554,7 +1562,12 @@ struct Engine {
             if (full)
                 for (size_t q = 0; q < ht.size(); ++q) {
                     Item& ch = nxt[ht_who[q].first];
-                    ch.parts[ch.np - 2 + ht_who[q].second].hvol = ht[q].vol;
+                    IPart& x = ch.parts[ch.np - 2 + ht_who[q].second];
+                    x.hvol = ht[q].vol;
+                    if (ht[q].hv) {
+                        x.hv = ht[q].hv; x.ht = ht[q].ht; x.hnv = ht[q].hnv; x.hnt = ht[q].hnt;
+                        store[x.store].hv = ht[q].hv;
+                    }
                 }
             for (Item& ch : nxt) {
                 sort_item_parts(ch);
@@ -1635,6 +1648,7 @@ struct Engine {
                 store[x.store].keep = true;
                 Part q;
                 q.v = x.v; q.t = x.t; q.nv = x.nv; q.nt = x.nt; q.mvol = x.mvol; q.hvol = x.hvol; q.haus = 0.0f;
+                q.hv = x.hv; q.ht = x.ht; q.hnv = x.hnv; q.hnt = x.hnt; q.own_hull = x.hv != nullptr;
                 if (q.mvol > q.hvol) q.mvol = q.hvol;  // cuda/la_lifecycle.cu:292-296
                 q.cc = cc; q.own = true;
                 for (int j = 0; j < 3; ++j) { q.lo[j] = x.lo[j]; q.hi[j] = x.hi[j]; }
@@ -1646,7 +1660,10 @@ struct Engine {
             }
         }
         for (const Half& hf : store)
-            if (!hf.keep) to_free.push_back(hf.v);
+            if (!hf.keep) {
+                to_free.push_back(hf.v);
+                if (hf.hv) to_free.push_back(hf.hv);
+            }
         const int frc = flush_frees();
         lap(S_APPLY);
         return frc;
