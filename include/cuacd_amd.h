@@ -224,9 +224,10 @@ int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const
                    int n_meshes, const int* mesh_of, const float* planes, int n_cuts, float tau, float* out_score,
                    double* out_detail, double* out_stats);
 
-/* Launch-shape knobs of the hull tiers (no reference counterpart; the
- * autotuner sets them). They select kernels and capacities only - every
- * setting computes the same canonical hulls. Process-wide. Keys: 0 wide_min
+/* Launch-shape knobs of the hull tiers and the proposal network's inference
+ * (no reference counterpart; the autotuner sets them). They select kernels
+ * and capacities only - every setting computes the same canonical hulls and
+ * the same scores. Process-wide. Keys: 0 wide_min
  * (wave-tier sets above it run four waves per hull; 0 disables), 1 vcap8
  * (first-pass vertex capacity in eighths, 1..8), 2 half_cap (reduced-capacity
  * first passes, 0/1), 3 kdop_min (k-DOP prefilter above this many points,
