@@ -223,3 +223,30 @@ def test_parity_with_serial_oracle_merge(ctx):
             assert abs(float(ghvol) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, (ci, k)
             assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, (ci, k)
     assert merged_any
+
+
+@pytest.mark.parametrize("dc_final,dc_iter", [(True, True), (False, False), (True, False)])
+def test_parity_with_serial_oracle_component_options(ctx, dc_final, dc_iter):
+    """decompose_components (a components pass after the loop) and
+    no_decompose_components_per_iter (no components pass inside it),
+    reference csrc/lookahead.c:336-345, 909-931, 969-985: the batched GPU
+    search and the serial oracle end with the same parts on three solids."""
+    import lookahead_oracle as LO
+    cases = [(_rotated(datasets.hollow_tube()[0], 2), datasets.hollow_tube()[1])]
+    cases += sdf.sdf_meshes(2, seed=31, res=16)
+    cases = [(_normalize(v), t) for v, t in cases]
+    kw = dict(max_iters=10, width=12, width2=3, threshold=0.05, depth=2, n_concave_edges=4, max_n_cutting=4)
+    print(f"component options: decompose_components {dc_final} per-iteration {dc_iter}", flush=True)
+    gpu, info = ctx.batch_lookahead_decompose(cases, return_info=True, decompose_components=dc_final,
+                                              no_decompose_components_per_iter=not dc_iter, **kw)
+    for ci, (v, t) in enumerate(cases):
+        hv, ht = ctx._input_hull(v)
+        ref, iters = LO.lookahead(v, t, hv, ht, decompose_components=dc_final, dc_per_iter=dc_iter, **kw)
+        assert info[ci]["iters"] == iters, (ci, info[ci]["iters"], iters)
+        assert len(gpu[ci]) == len(ref), (ci, len(gpu[ci]), len(ref))
+        key = lambda p: (len(p[1]), p[0].tobytes())
+        g = sorted(gpu[ci], key=key)
+        r = sorted(ref, key=key)
+        for (gv, gt, ghv, ght), rp in zip(g, r):
+            assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
+            assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, ci
