@@ -1172,6 +1172,21 @@ struct Engine {
             float* hv;
         };
         std::vector<Half> store;
+        // an error return leaves the halves not yet adopted to the free list
+        // (the caller flushes it); the normal path releases them itself
+        struct StoreGuard {
+            std::vector<Half>& st;
+            std::vector<void*>& tf;
+            bool released = false;
+            ~StoreGuard() {
+                if (released) return;
+                for (const Half& hf : st)
+                    if (!hf.keep) {
+                        tf.push_back(hf.v);
+                        if (hf.hv) tf.push_back(hf.hv);
+                    }
+            }
+        } store_guard{store, to_free};
         std::vector<Item> extra;       // leaves for parts too small to cut
         // level-0 items per cutting part and initial cut, -1 = no item
         std::vector<std::vector<int>> l0(cutting.size());
@@ -1664,6 +1679,7 @@ struct Engine {
                 to_free.push_back(hf.v);
                 if (hf.hv) to_free.push_back(hf.hv);
             }
+        store_guard.released = true;
         const int frc = flush_frees();
         lap(S_APPLY);
         return frc;

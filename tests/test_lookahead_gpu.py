@@ -250,3 +250,20 @@ def test_parity_with_serial_oracle_component_options(ctx, dc_final, dc_iter):
         for (gv, gt, ghv, ght), rp in zip(g, r):
             assert gv.tobytes() == rp[0].tobytes() and np.array_equal(gt, rp[1]), ci
             assert abs(abs(oracle.mesh_volume(ghv, ght)) - rp[5]) <= 1e-6 * max(rp[5], 1e-12) + 1e-12, ci
+
+
+def test_quick_level_error_paths(ctx):
+    """quick_depth < 0 is rejected; depth + quick_depth beyond the item's
+    part capacity (8 parts: the reference's KERR_LA_OVERFLOW,
+    cuda/la_refine.cu:267-274) is an error, not a silent truncation; the
+    heap stays balanced either way."""
+    v, t = datasets.l_shape_union()
+    v = _normalize(_rotated(v, 1))
+    before = ctx.heap_stats()
+    with pytest.raises(Exception):
+        ctx.batch_lookahead_decompose([(v, t)], max_iters=3, width=12, width2=3, n_concave_edges=4, quick_depth=-1)
+    with pytest.raises(Exception, match="capacity"):
+        ctx.batch_lookahead_decompose([(v, t)], max_iters=3, width=12, width2=3, n_concave_edges=4, depth=2,
+                                      quick_depth=6)
+    after = ctx.heap_stats()
+    assert after[0] == before[0] and after[3] == before[3], (before, after)
