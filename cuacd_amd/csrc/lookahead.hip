@@ -1357,6 +1357,17 @@ struct Engine {
                         h[q] = t;
                     }
                     LA_TRY(cut_batch(h));
+                    // level-0 halves keep their hull mesh (the two of the applied
+                    // cut become parts that need it) while the meshes' worst-case
+                    // size (n vertices, 2n triangles, 12 B each) fits in a quarter
+                    // of the device pool; otherwise the part-hull phase builds them
+                    bool emit0 = false;
+                    if (d == 0) {
+                        double worst = 0.0;
+                        for (const LaCutTask& t : h)
+                            if (t.status == 0 && t.pv && t.nvp) worst += 36.0 * (t.pnv + t.nnv) + 512.0;
+                        emit0 = worst <= 0.25 * (double)ctx->pool_bytes;
+                    }
                     for (size_t q = 0; q < cs.size(); ++q) {
                         const LaCutTask& t = h[q];
                         if (t.status != 0 || !t.pv || !t.nvp) continue;
@@ -1381,12 +1392,10 @@ struct Engine {
                         const int ci = (int)nxt.size();
                         nxt.push_back(ch);
                         nxt_c.push_back(cur_c[cs[q].item]);
-                        // level-0 halves keep their hull mesh: the two of the
-                        // applied cut become parts that need it
                         LaHullTask a{};
-                        a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30; a.emit = d == 0;
+                        a.pts = t.pv; a.n = t.pnv; a.prune_gap = 1e30; a.emit = emit0;
                         LaHullTask b{};
-                        b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30; b.emit = d == 0;
+                        b.pts = t.nvp; b.n = t.nnv; b.prune_gap = 1e30; b.emit = emit0;
                         ht.push_back(a);
                         ht_who.push_back({ci, 0});
                         ht.push_back(b);
