@@ -4,6 +4,7 @@
 // the device, reserves the pool and initialises the heaps; pool_usage and
 // heap_stats read the device bookkeeping back. There is no module loading:
 // the kernels are linked into this library.
+#include <stdio.h>
 #include <stdlib.h>
 #include "ctx.hpp"
 #include "kerr.h"
@@ -291,11 +292,41 @@ int gpu_timer_stop(gpu_ctx_t ctx, float* ms) {
     HIP_TRY(hipEventElapsedTime(ms, ctx->ev_a, ctx->ev_b));
     return 0;
 }
+// Marketing name of the board: the runtime's device name, else the name the
+// amdgpu driver publishes for the PCI device (boards with a FRU record), else
+// the PCI device id.
+static void board_name(const hipDeviceProp_t& prop, int device, char* out, size_t len) {
+    if (prop.name[0]) {
+        snprintf(out, len, "%s", prop.name);
+        return;
+    }
+    char nm[256] = {0};
+    if (hipDeviceGetName(nm, (int)sizeof(nm), device) == hipSuccess && nm[0]) {
+        snprintf(out, len, "%s", nm);
+        return;
+    }
+    char path[128];
+    snprintf(path, sizeof(path), "/sys/bus/pci/devices/%04x:%02x:%02x.0/product_name", prop.pciDomainID,
+             prop.pciBusID, prop.pciDeviceID);
+    if (FILE* f = fopen(path, "r")) {
+        size_t k = fread(nm, 1, sizeof(nm) - 1, f);
+        fclose(f);
+        while (k > 0 && (nm[k - 1] == '\n' || nm[k - 1] == ' ')) nm[--k] = 0;
+        if (k > 0) {
+            snprintf(out, len, "%s", nm);
+            return;
+        }
+    }
+    snprintf(out, len, "AMD GPU at PCI %04x:%02x:%02x", prop.pciDomainID, prop.pciBusID, prop.pciDeviceID);
+}
+
 int gpu_device_info(gpu_ctx_t ctx, char* name, int name_len, int* n_cu, size_t* hbm_bytes) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
     if (name && name_len > 0) {
-        snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        char board[200];
+        board_name(prop, ctx->device, board, sizeof(board));
+        snprintf(name, (size_t)name_len, "%s (%s)", board, prop.gcnArchName);
     }
     if (n_cu) *n_cu = prop.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;

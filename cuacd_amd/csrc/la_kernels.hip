@@ -140,12 +140,17 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
     const int nv = hc.nv;
     const int nt = hc.nt;
     const unsigned long long vb = al16(12ull * nv);
-    if (tid == 0) sb->red[0] = (unsigned long long)heap_alloc(pool, 0, vb + 12ull * nt, err);
+    // emit 2: the mesh is an optional extra (built later when missing), so a
+    // full heap leaves it out without an error
+    if (tid == 0) sb->red[0] = (unsigned long long)heap_alloc(pool, 0, vb + 12ull * nt, t.emit == 2 ? nullptr : err);
     __syncthreads();
     float* chunk = (float*)sb->red[0];
     __syncthreads();
     if (!chunk) {
-        if (tid == 0) { t.hv = nullptr; t.ht = nullptr; t.hnv = t.hnt = 0; t.err = KERR_HULL_HEAP_OUTPUT; }
+        if (tid == 0) {
+            t.hv = nullptr; t.ht = nullptr; t.hnv = t.hnt = 0;
+            if (t.emit != 2) t.err = KERR_HULL_HEAP_OUTPUT;
+        }
         return true;
     }
     int* tris = (int*)((char*)chunk + vb);

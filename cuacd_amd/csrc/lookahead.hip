@@ -154,6 +154,14 @@ void frame_plane(const float lo[3], const float hi[3], const float pl[4], float 
     out[3] = dd / sc;
 }
 
+// smallest power of two >= x (x >= 1): the size of the device-heap block
+// that holds x bytes
+static unsigned long long pow2_ceil_ull(unsigned long long x) {
+    unsigned long long p = 256;
+    while (p < x) p <<= 1;
+    return p;
+}
+
 struct MeshState {
     std::vector<Part> parts;
     int cc_counter = 0;
@@ -369,6 +377,18 @@ struct Engine {
                 continue;
             }
             memcpy(o.data(), oh + ooff[k], sizeof(LaCompOut) * nc);
+            if ((int)S.parts.size() + nc - 1 > LA_MAX_DECOMP) {
+                // this part's pieces and those of the parts after it belong to no
+                // part yet: free them before failing
+                for (size_t k2 = k; k2 < h.size(); ++k2)
+                    for (int c = 0; c < h[k2].n_comp; ++c) {
+                        LaCompOut x;
+                        memcpy(&x, oh + ooff[k2] + sizeof(LaCompOut) * c, sizeof(LaCompOut));
+                        to_free.push_back(x.v);
+                    }
+                (void)flush_frees();
+                REQUIRE(false, "lookahead: more than %d parts", LA_MAX_DECOMP);
+            }
             Part old = S.parts[i];
             int cc_base = 0;
             const bool fresh = old.cc == 0;
@@ -384,10 +404,7 @@ struct Engine {
                 q.cc = fresh ? cc_base + c + 1 : old.cc;
                 for (int j = 0; j < 3; ++j) { q.lo[j] = o[c].lo[j]; q.hi[j] = o[c].hi[j]; }
                 if (c == 0) S.parts[i] = q;
-                else {
-                    REQUIRE((int)S.parts.size() < LA_MAX_DECOMP, "lookahead: more than %d parts", LA_MAX_DECOMP);
-                    S.parts.push_back(q);
-                }
+                else S.parts.push_back(q);
             }
         }
         return flush_frees();
@@ -1371,14 +1388,19 @@ struct Engine {
                     LA_TRY(cut_batch(h));
                     // level-0 halves keep their hull mesh (the two of the applied
                     // cut become parts that need it) while the meshes' worst-case
-                    // size (n vertices, 2n triangles, 12 B each) fits in a quarter
-                    // of the device pool; otherwise the part-hull phase builds them
-                    bool emit0 = false;
+                    // blocks (n vertices, 2n triangles, 12 B each, plus the header,
+                    // rounded up to the power-of-two block) fit in a quarter of the
+                    // pool's unused space; the emission is optional (emit 2): a
+                    // mesh that finds no room is built by the part-hull phase
+                    int emit0 = 0;
                     if (d == 0) {
                         double worst = 0.0;
                         for (const LaCutTask& t : h)
-                            if (t.status == 0 && t.pv && t.nvp) worst += 36.0 * (t.pnv + t.nnv) + 512.0;
-                        emit0 = worst <= 0.25 * (double)ctx->pool_bytes;
+                            if (t.status == 0 && t.pv && t.nvp)
+                                worst += (double)pow2_ceil_ull(36ull * t.pnv + 288) +
+                                         (double)pow2_ceil_ull(36ull * t.nnv + 288);
+                        const double room = (double)ctx->pool_bytes - (double)gpu_pool_usage(ctx);
+                        emit0 = worst <= 0.25 * room ? 2 : 0;
                     }
                     for (size_t q = 0; q < cs.size(); ++q) {
                         const LaCutTask& t = h[q];
@@ -1551,9 +1573,11 @@ struct Engine {
                 // lowest bounds; round 2: the others, except those whose bound is
                 // above the best leaf cost found for their cutting part. Such a
                 // leaf can never be the part's argmin, so the decisions are the
-                // same as scoring every leaf.
+                // same as scoring every leaf. A recording run (training targets:
+                // every level-0 candidate's best path cost) scores every leaf,
+                // since a skipped leaf would leave its first cut's cost too high.
                 std::vector<int> r1, r2;
-                if (leaf_bnb_on()) {
+                if (leaf_bnb_on() && !ctx->record) {
                     std::vector<std::vector<int>> by_c(cutting.size());
                     for (size_t i = 0; i < cur.size(); ++i) by_c[cur_c[i]].push_back((int)i);
                     std::vector<char> first(cur.size(), 0);
@@ -1676,6 +1700,9 @@ struct Engine {
             if (!hp[0] || !hp[1]) continue;
             const int m = cutting[c].first;
             MeshState& S = M[m];
+            // room for the second half first: on failure nothing has changed hands
+            // (the store guard frees both halves, the part keeps its mesh)
+            REQUIRE((int)S.parts.size() < LA_MAX_DECOMP, "lookahead: more than %d parts", LA_MAX_DECOMP);
             Part& old = S.parts[cutting[c].second];
             const int cc = old.cc;
             drop_part(old);
@@ -1689,10 +1716,7 @@ struct Engine {
                 q.cc = cc; q.own = true;
                 for (int j = 0; j < 3; ++j) { q.lo[j] = x.lo[j]; q.hi[j] = x.hi[j]; }
                 if (s2 == 0) old = q;
-                else {
-                    REQUIRE((int)S.parts.size() < LA_MAX_DECOMP, "lookahead: more than %d parts", LA_MAX_DECOMP);
-                    S.parts.push_back(q);
-                }
+                else S.parts.push_back(q);
             }
         }
         for (const Half& hf : store)

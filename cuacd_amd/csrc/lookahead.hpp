@@ -21,7 +21,8 @@ constexpr int LA_MAX_COMP = 256;  // components kept per part (more merge into t
 struct LaHullTask {
     const float* pts;    // xyz
     int n;
-    int emit;            // 1: write the hull mesh into a heap-0 chunk (hv, ht)
+    int emit;            // 1: write the hull mesh into a heap-0 chunk (hv, ht); 2: the same
+                         // when the heap has room (no error when it has not)
     double mesh_vol;     // for pruning
     double prune_gap;    // prune when max(rough_vol - mesh_vol, 0) > prune_gap; >= 1e29 disables
     QFrame frame;        // quantization frame of the full set (k-DOP path)

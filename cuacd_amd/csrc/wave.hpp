@@ -100,8 +100,9 @@ template <class T> __device__ __forceinline__ bool cas_acq_rel(T* p, T& expected
 template <class T> __device__ __forceinline__ T fetch_add_relaxed(T* p, T v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// err == nullptr: the caller handles the failure itself (nothing recorded)
 __device__ __forceinline__ void err_or(int* err, int bits) {
-    if (bits) __hip_atomic_fetch_or(err, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bits && err) __hip_atomic_fetch_or(err, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace cuacd

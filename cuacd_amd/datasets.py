@@ -169,6 +169,40 @@ def l_shape_union():
     return voxel_solid(occ, (0.0, 0.0, 0.0), 1.0)
 
 
+def u_shape():
+    """[0,3] x [0,2] x [0,1] minus the notch [1,2] x [1,2] x [0,1] (unit
+    voxels): volume 5, two concave edges at (1,1,z) and (2,1,z)."""
+    occ = np.ones((3, 2, 1), bool)
+    occ[1, 1, 0] = False
+    return voxel_solid(occ)
+
+
+def comb_solid(teeth):
+    """A comb of unit voxels: a row of 2 teeth - 1 cells along x with a tooth
+    on every other cell. Each of the teeth - 1 gaps has two concave edges, the
+    vertical lattice edges at (x, 1, z) on the gap's sides."""
+    occ = np.zeros((2 * teeth - 1, 2, 1), bool)
+    occ[:, 0, 0] = True
+    occ[0::2, 1, 0] = True
+    return voxel_solid(occ)
+
+
+def chevron_prism():
+    """The square [0,2]^2 with the V notch (0,2), (1,1), (2,2) removed,
+    extruded over z in [0,1]: volume 3, hull gap 1. Its only concave edge is
+    the vertical edge at (1,1); the plane x = 1 contains that edge but no
+    face (the notch walls are sloped), so it splits the solid into two
+    convex halves of volume 1.5."""
+    poly = [(0, 0), (2, 0), (2, 2), (1, 1), (0, 2)]  # counter-clockwise
+    v = np.array([(x, y, 0) for x, y in poly] + [(x, y, 1) for x, y in poly], np.float32)
+    cap = [(0, 1, 3), (1, 2, 3), (0, 3, 4)]
+    t = [(a, c, b) for a, b, c in cap] + [(a + 5, b + 5, c + 5) for a, b, c in cap]
+    for i in range(5):
+        a, b = i, (i + 1) % 5
+        t += [(a, b, b + 5), (a, b + 5, a + 5)]
+    return v, np.array(t, np.int32)
+
+
 def cut_candidates(verts, tris, n_edges=16, n_total=30, threshold=3.49, seed=0):
     """Candidate planes (a, b, c, d) for one closed mesh, float32 [n_total, 4]:
     up to n_edges planes through sampled concave edges - edges whose interior

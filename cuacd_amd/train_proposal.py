@@ -1,14 +1,15 @@
 """Train the learned cut proposal on the GPU and save its weights.
 
-    python -m cuacd_amd.train_proposal [--meshes 256] [--epochs 60] [--out PATH]
+    python -m cuacd_amd.train_proposal [--meshes 1024] [--epochs 70] [--source scorer] [--out PATH]
 
 Training meshes: seeded SDF solids (the benchmark's generator, seed stream
 1000+, marching tetrahedra on a 48^3 grid, 2k-50k triangles) - disjoint from
-the benchmark's seed-4 set. Triples (--source search, the default): plain
-lookahead runs on them record every cutting part of every iteration and each
-level-0 candidate's path cost, the value the search minimises. --source
-scorer instead scores the search's level-0 candidates of parts (each mesh and
-the pieces after 1-3 iterations) with the batched exact scorer. 10% of the
+the benchmark's seed-4 set. Triples (--source scorer, the default and the
+source of the shipped weights): the search's level-0 candidates of parts
+(each mesh and the pieces after 1-3 iterations) scored by the batched exact
+scorer. --source search instead records, in plain lookahead runs, every
+cutting part of every iteration and each level-0 candidate's path cost (the
+value the search minimises; a recording run scores every leaf). 10% of the
 parts are held out to report the ranking quality (is the exact best
 candidate among the predicted top 8, and the regret).
 """
@@ -28,7 +29,7 @@ def log(*a):
     print("[train]", *a, file=sys.stderr, flush=True)
 
 
-def build_dataset(ctx, n_meshes, seed=1000, iters=(0, 1, 2, 3), source="search"):
+def build_dataset(ctx, n_meshes, seed=1000, iters=(0, 1, 2, 3), source="scorer"):
     from . import proposal as PR
     from . import sdf
     t0 = time.perf_counter()
@@ -68,12 +69,12 @@ def split(data, frac=0.1, seed=0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--meshes", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--meshes", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=70)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch-parts", type=int, default=64)
     ap.add_argument("--out", default=DEFAULT_OUT)
-    ap.add_argument("--source", choices=["search", "scorer"], default="search")
+    ap.add_argument("--source", choices=["search", "scorer"], default="scorer")
     args = ap.parse_args()
     import cuacd_amd
     from . import proposal as PR

@@ -100,7 +100,26 @@ static int oem_get(OEdgeMap* m, int a, int b) {
     return s ? *s : -1;
 }
 
+/* One incremental construction with room for cap faces created in all
+ * (dead ones included); -2 when that is not enough. */
+static int orc_hull_cap(const int* q, int n, unsigned char* is_vertex, int* faces, int face_cap, int cap);
+
+/* Exact hull of integer points by incremental insertion in input order
+ * (strictly visible faces only, so points on a face are not inserted).
+ * The number of faces created over all insertions depends on the order and
+ * can exceed any fixed multiple of n on lattices with many coplanar points,
+ * so the face store grows until the construction fits. */
 int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face_cap) {
+    int64_t cap = 8 * (int64_t)n + 64;
+    for (;;) {
+        int r = orc_hull_cap(q, n, is_vertex, faces, face_cap, (int)cap);
+        if (r != -2) return r;
+        cap *= 2;
+        if (cap > (1 << 28)) return -1;
+    }
+}
+
+static int orc_hull_cap(const int* q, int n, unsigned char* is_vertex, int* faces, int face_cap, int cap) {
     memset(is_vertex, 0, (size_t)n);
     if (n < 4) return 0;
     /* initial tetrahedron: first points (input order) that span 0D..3D */
@@ -120,7 +139,6 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
     if (orient64(q + 3 * i0, q + 3 * i1, q + 3 * i2, q + 3 * i3) < 0) {
         int t = i1; i1 = i2; i2 = t;
     }
-    int cap = 8 * n + 64;
     OFace* F = (OFace*)malloc(sizeof(OFace) * (size_t)cap);
     OEdgeMap em;
     uint64_t H = 64;
@@ -201,7 +219,7 @@ int orc_hull(const int* q, int n, unsigned char* is_vertex, int* faces, int face
         }
     }
     free(F); free(em.k); free(em.v); free(hz); free(vis);
-    return failed ? -1 : out;
+    return failed ? -2 : out;
 }
 
 /* Canonical form of the exact hull (the GPU library emits the same):

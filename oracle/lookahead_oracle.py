@@ -237,20 +237,7 @@ def concave_planes(p, n_concave, eps, thr):
     cnt = L.orc_concave_edges(O._p(v), len(v), O._p(t), len(t), cpos, cneg, pm, nm, O._p(ri), O._p(rf), cap)
     cnt = min(cnt, cap)
     order = sorted(range(cnt), key=lambda e: (ri[e, 0], ri[e, 1], ri[e, 2]))
-    res = []
-    rng = 1
-    seen = 0
-    K = n_concave
-    for e in order:
-        seen += 1
-        if len(res) < K:
-            res.append(e)
-        else:
-            rng = (rng * 1664525 + 1013904223) & 0xFFFFFFFF
-            thr_r = int(np.float32(np.float32(K) / np.float32(seen)) * np.float32(4294967295.0))
-            if rng < thr_r:
-                rng = (rng * 1664525 + 1013904223) & 0xFFFFFFFF
-                res[rng % K] = e
+    res = [order[j] for j in reservoir(len(order), n_concave)]
     planes = []
     f = np.float32
     eps = f(eps)
@@ -335,29 +322,120 @@ def _sort_parts(parts):
     return out
 
 
-def _ub(itm, total):
-    """Branch-and-bound bound of one item (cuda/la_expand.cu:434-457)."""
+def level_sum(lc):
+    """Sum of an item's level costs in level order, in float32."""
     ps = np.float32(0)
-    for x in itm.lc:
+    for x in lc:
         ps = np.float32(ps + x)
-    return np.float32(ps + np.float32(itm.lc[-1] * np.float32(total - len(itm.lc))))
+    return ps
 
 
-def _prune_gap(itm, best_ub, total):
+def path_cost(lc):
+    """Path cost of a leaf: the mean of its level costs, summed in level order
+    and divided in float32 (cuda/la_refine.cu:450-457)."""
+    return np.float32(level_sum(lc) / np.float32(len(lc)))
+
+
+def item_ub(lc, total):
+    """Branch-and-bound upper bound of one item: its level costs so far plus
+    its last level cost for every level still to come, since level costs do
+    not increase along a path (cuda/la_expand.cu:434-457)."""
+    return np.float32(level_sum(lc) + np.float32(lc[-1] * np.float32(total - len(lc))))
+
+
+def best_ub(lcs, total):
+    """Smallest item_ub over items with at least one level (1e30 if none)."""
+    b = np.float32(1e30)
+    for lc in lcs:
+        if len(lc) > 0:
+            b = min(b, item_ub(lc, total))
+    return b
+
+
+def prune_gap(lc, bub, total):
     """Rough-hull volume gap above which an item's new parts are pruned
-    (cuda/la_expand.cu:380-399); 1e30 disables, -1 always prunes."""
-    nl = len(itm.lc)
-    if not (best_ub < np.float32(1e29) and 0 < nl < total):
+    (cuda/la_expand.cu:380-399): the remaining cost budget bub - sum(lc) as
+    an rv cost, turned back into the volume of a sphere of radius
+    budget / 0.3. 1e30 disables pruning (no bound, or an item with no level
+    or already at the last level), -1 always prunes (no budget left)."""
+    nl = len(lc)
+    if not (bub < np.float32(1e29) and 0 < nl < total):
         return 1e30
-    ps = np.float32(0)
-    for x in itm.lc:
-        ps = np.float32(ps + x)
-    rv_thr = np.float32(best_ub - ps)
+    rv_thr = np.float32(bub - level_sum(lc))
     if rv_thr <= 0:
         return -1.0
     r = np.float32(rv_thr / np.float32(0.3))
     return float(np.float32(np.float32(np.float32(r * r) * r) *
                             np.float32(np.float32(np.float32(4.0) * PI_F) / np.float32(3.0))))
+
+
+def select_cutting(costs, tau, max_n_cutting):
+    """The parts cut in one iteration (cuda/la_lifecycle.cu:90-144): a stable
+    ascending sort of the parts by full cost, then the parts at or above tau,
+    which form a suffix of that order; when more than max_n_cutting are above,
+    the last (costliest) max_n_cutting. Returns (order, cutting): order is the
+    sorted permutation of the input, cutting the positions in that order."""
+    order = sorted(range(len(costs)), key=lambda i: costs[i])
+    c = [costs[i] for i in order]
+    n = len(c)
+    first = next((i for i in range(n) if c[i] >= tau), n)
+    start = n - max_n_cutting if n - first > max_n_cutting else first
+    return order, [i for i in range(start, n) if c[i] >= tau][:max_n_cutting]
+
+
+def per_cut_best(tw0, paths):
+    """Smallest path cost per first cut index (1e30 where no leaf is left)."""
+    best = [np.float32(1e30)] * tw0
+    for cut0, pc in paths:
+        if 0 <= cut0 < tw0 and pc < best[cut0]:
+            best[cut0] = pc
+    return best
+
+
+def evaluate(tw0, paths, level0):
+    """Best first cut of one cutting part (cuda/la_refine.cu:418-527).
+    paths: (first cut index, path cost) of every leaf that was not pruned
+    (full leaves, scored last-level children, extra leaves); each first cut
+    keeps its smallest path cost. When no path is left (every subtree pruned
+    or cut-less), the fallback is the greedy one: each valid level-0 item's
+    own level-0 cost, level0 = {first cut index: level-0 cost}. Returns
+    (best cost, first cut index), the lowest index among equal costs; index
+    None when nothing is below 1e30."""
+    best = per_cut_best(tw0, paths)
+    if not any(b < np.float32(1e30) for b in best):
+        for k, c0 in level0.items():
+            best[k] = c0
+    bc, bi = np.float32(1e30), None
+    for k in range(tw0):
+        if best[k] < bc:
+            bc, bi = best[k], k
+    return bc, bi
+
+
+def lcg_next(s):
+    """The reservoir's linear congruential step (Numerical Recipes constants;
+    cuda/la_concave.cu lcg_next)."""
+    return (s * 1664525 + 1013904223) & 0xFFFFFFFF
+
+
+def reservoir(n_edges, k):
+    """Reservoir sample of k of n_edges concave edges in their scan order
+    (cuda/la_concave.cu:276-301): the first k fill the slots; afterwards the
+    seen-th edge draws r from the LCG (seeded with 1) and replaces slot
+    (next draw) mod k when r < (uint32)(float(k) / float(seen) * 4294967295f).
+    Returns the scan indices held by the slots, in slot order."""
+    res, rng = [], 1
+    for seen in range(1, n_edges + 1):
+        e = seen - 1
+        if len(res) < k:
+            res.append(e)
+            continue
+        rng = lcg_next(rng)
+        thr = int(np.float32(np.float32(k) / np.float32(seen)) * np.float32(4294967295.0))
+        if rng < thr:
+            rng = lcg_next(rng)
+            res[rng % k] = e
+    return res
 
 
 def quick_levels(cur, extra, quick_depth, total, med):
@@ -371,11 +449,7 @@ def quick_levels(cur, extra, quick_depth, total, med):
     bound over its input items. Returns the items that end the search."""
     ext = np.float32(np.float32(2.0) * med + np.float32(1e-6))
     for q in range(quick_depth):
-        best_ub = np.float32(1e30)
-        if q == quick_depth - 1:
-            for itm in cur:
-                if len(itm.lc) > 0:
-                    best_ub = min(best_ub, _ub(itm, total))
+        bub = best_ub([itm.lc for itm in cur], total) if q == quick_depth - 1 else np.float32(1e30)
         nxt = []
         for itm in cur:
             wm, wh, wv, wt, wlo, whi = itm.parts[-1]
@@ -404,7 +478,7 @@ def quick_levels(cur, extra, quick_depth, total, med):
                 continue
             assert len(itm.parts) + 1 <= LA_MAX_PARTS
             pm, pv, pt, nmv, nv, nt = best
-            gap = _prune_gap(itm, best_ub, total)
+            gap = prune_gap(itm.lc, bub, total)
             ph, prp = hull_volume(pv, gap, pm)
             nh, prn = hull_volume(nv, gap, nmv)
             ch = _sort_parts(list(itm.parts[:-1]) + [(pm, ph, pv, pt, pv.min(0), pv.max(0)),
@@ -570,11 +644,14 @@ def merge_pass(parts, tau):
 def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, threshold=0.05, depth=2,
               max_n_cutting=16, decompose_components=False, dc_per_iter=True, n_concave_edges=32,
               concave_eps=0.005, concave_threshold=3.49, concave_iters=10, deadline=None, quick_depth=0,
-              merge_hulls=False):
+              merge_hulls=False, trace=None):
     """Returns (parts, iters): parts = [(v, t, hv, ht, mesh_vol, hull_vol, hausdorff)].
     deadline (time.perf_counter() value): stop before an iteration that would
     start after it (bounded CPU-baseline samples). quick_depth: quick levels
-    after the full ones (csrc/lookahead.c:656-727)."""
+    after the full ones (csrc/lookahead.c:656-727). trace: a list that gets
+    one record per cutting part and iteration (its position after the sort,
+    the best path cost, the chosen first cut's index and plane, and every
+    first cut's best path cost before the greedy fallback)."""
     import time
     tau = np.float32(threshold)
     v0 = np.asarray(verts, np.float32)
@@ -631,11 +708,8 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
         for p in parts:
             if p.hv is not None and len(p.ht) and p.haus <= 0 and cost_rv(p.hvol, p.mvol) < tau:
                 p.haus = hausdorff(p.hv, p.ht, p.v, p.t)
-        parts = sorted(parts, key=lambda q: q.cost())  # stable
-        np_ = len(parts)
-        first = next((i for i in range(np_) if parts[i].cost() >= tau), np_)
-        start = np_ - max_n_cutting if np_ - first > max_n_cutting else first
-        cutting = [i for i in range(start, np_) if parts[i].cost() >= tau][:max_n_cutting]
+        order, cutting = select_cutting([q.cost() for q in parts], tau, max_n_cutting)
+        parts = [parts[i] for i in order]
         if not cutting:
             break
         iters = it + 1
@@ -646,6 +720,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
         n_edge = max(len(e) for e in eplanes) if eplanes else 0
         extra = []
         l0 = [dict() for _ in cutting]
+        l0_planes = [dict() for _ in cutting]
         # items: parts as (mvol, hvol, v, t, lo, hi) tuples, worst last
         cur = []
         for c, i in enumerate(cutting):
@@ -660,16 +735,7 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
             full = (not last) or depth == 1 or quick_depth > 0
             w = width if d == 0 else width2
             nxt = []
-            best_ub = np.float32(1e30)
-            if not full:
-                for itm in cur:
-                    if len(itm.lc) <= 0:
-                        continue
-                    ps = np.float32(0)
-                    for x in itm.lc:
-                        ps = np.float32(ps + x)
-                    ub = np.float32(ps + np.float32(itm.lc[-1] * np.float32(total - len(itm.lc))))
-                    best_ub = min(best_ub, ub)
+            bub = np.float32(1e30) if full else best_ub([itm.lc for itm in cur], total)
             any_child = False
             for itm in cur:
                 tw = w + (n_edge if d == 0 else 0)
@@ -720,33 +786,18 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
                         any_child = True
                         if d == 0:
                             l0[itm.src][k] = c_item
+                            l0_planes[itm.src][k] = np.array(pl, np.float32)
                     else:
                         pvol, nvol, npc, nnc, ppts, npts = score_cut(wv, wt, pl)
                         if npc == 0 or nnc == 0:
                             continue
-                        gap = 1e30
-                        nl = len(itm.lc)
-                        if best_ub < np.float32(1e29) and 0 < nl < total:
-                            ps = np.float32(0)
-                            for x in itm.lc:
-                                ps = np.float32(ps + x)
-                            rv_thr = np.float32(best_ub - ps)
-                            if rv_thr <= 0:
-                                gap = -1.0
-                            else:
-                                r = np.float32(rv_thr / np.float32(0.3))
-                                gap = float(np.float32(np.float32(np.float32(r * r) * r) *
-                                                       np.float32(np.float32(np.float32(4.0) * PI_F) / np.float32(3.0))))
+                        gap = prune_gap(itm.lc, bub, total)
                         ph, prp = hull_volume(ppts, gap, pvol)
                         nh, prn = hull_volume(npts, gap, nvol)
                         worst = max(cost_rv(ph, pvol), cost_rv(nh, nvol))
                         for x in itm.parts[:-1]:
                             worst = max(worst, cost_rv(x[1], x[0]))
-                        s = np.float32(0)
-                        for x in itm.lc:
-                            s = np.float32(s + x)
-                        s = np.float32(s + worst)
-                        leaf_recs.append((itm.src, cut0, np.float32(s / np.float32(len(itm.lc) + 1)), prp or prn))
+                        leaf_recs.append((itm.src, cut0, path_cost(list(itm.lc) + [worst]), prp or prn))
                         any_child = True
             if not full:
                 if not any_child:
@@ -760,39 +811,23 @@ def lookahead(verts, tris, hull_v, hull_t, max_iters=100, width=60, width2=5, th
         leaves.extend(cur)
         # evaluate + apply
         tw0 = width + n_edge
-        best = [[np.float32(1e30)] * tw0 for _ in cutting]
-
-        def consider(src, cut0, path, pruned):
-            if pruned:
-                return
-            if 0 <= cut0 < tw0 and path < best[src][cut0]:
-                best[src][cut0] = path
-
+        paths = [[] for _ in cutting]
         for itm in leaves:
-            if itm.pruned or len(itm.lc) == 0:
-                continue
-            s = np.float32(0)
-            for x in itm.lc:
-                s = np.float32(s + x)
-            consider(itm.src, itm.cut0, np.float32(s / np.float32(len(itm.lc))), False)
+            if not itm.pruned and len(itm.lc) > 0:
+                paths[itm.src].append((itm.cut0, path_cost(itm.lc)))
         for src, cut0, path, pruned in leaf_recs:
-            consider(src, cut0, path, pruned)
+            if not pruned:
+                paths[src].append((cut0, path))
         for itm in extra:
-            s = np.float32(0)
-            for x in itm.lc:
-                s = np.float32(s + x)
-            consider(itm.src, itm.cut0, np.float32(s / np.float32(len(itm.lc))), False)
+            paths[itm.src].append((itm.cut0, path_cost(itm.lc)))
         new_parts = []
         for c, i in enumerate(cutting):
-            if not any(b < np.float32(1e30) for b in best[c]):
-                for k, itm in l0[c].items():
-                    if len(itm.lc) >= 1:
-                        best[c][k] = itm.lc[0]
-            bc, bi = np.float32(1e30), 0
-            for k in range(tw0):
-                if best[c][k] < bc:
-                    bc, bi = best[c][k], k
-            if bi not in l0[c]:
+            l0c = {k: itm.lc[0] for k, itm in l0[c].items() if len(itm.lc) >= 1}
+            bc, bi = evaluate(tw0, paths[c], l0c)
+            if trace is not None:
+                trace.append({"iter": it, "part": i, "best_cost": bc, "best_cut": bi,
+                              "plane": l0_planes[c].get(bi), "per_cut": per_cut_best(tw0, paths[c])})
+            if bi is None or bi not in l0[c]:
                 continue
             pos, neg = l0[c][bi].store
             cc = parts[i].cc

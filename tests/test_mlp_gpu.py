@@ -229,3 +229,30 @@ def test_search_record_and_error_paths(ctx):
         net.close()
     # the failed call left no proposal behind: the plain search still runs
     ctx.batch_lookahead_decompose(meshes[:1], **dict(SEARCH, max_iters=1))
+
+
+def test_recorded_targets_are_every_first_cuts_best_path(ctx):
+    """The training record holds, per cutting part, every level-0
+    candidate's best path cost - the oracle's per-first-cut minimum over all
+    leaves. The search's two-round leaf bound may skip leaves that cannot
+    change the argmin; a recording run scores them all, so no candidate is
+    recorded too high or as failed."""
+    import lookahead_oracle as LO
+    from cuacd_amd import sdf
+    kw = dict(SEARCH, max_iters=3, merge_hulls=False)
+    for v, t in sdf.sdf_meshes(2, seed=5, res=20):
+        pts, pl, po, tg = PR.record_dataset(ctx, [(v, t)], **kw)
+        hv, ht = ctx._input_hull(v)
+        trace = []
+        LO.lookahead(v, t, hv, ht, trace=trace, **kw)
+        starts = np.searchsorted(po, np.arange(len(pts) + 1))
+        recorded = [np.sort(tg[a:b][tg[a:b] < PR.FAIL_SCORE]) for a, b in zip(starts[:-1], starts[1:])]
+        want = [np.sort(np.array([c for c in tr["per_cut"] if c < 1e29], np.float32)) for tr in trace]
+        # compare the parts with at least one finite path (a part with fewer
+        # than two candidates is not recorded; one whose cuts all failed has
+        # nothing to compare)
+        recorded = [r for r in recorded if len(r)]
+        want = [w for w in want if len(w)]
+        assert len(recorded) == len(want)
+        for r, w in zip(recorded, want):
+            assert len(r) == len(w) and np.allclose(r, w, rtol=0, atol=1e-7), (r[:5], w[:5])
