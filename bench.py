@@ -1,27 +1,37 @@
-"""Benchmark of the lookahead decomposition on one MI355X; prints ONE JSON line.
+"""Benchmark on one MI355X; prints ONE JSON line.
 
-Workload: 64 watertight SDF meshes (smooth union of 3-8 random capsules/boxes
-minus 0-3 spheres, marching tetrahedra on a 48^3 grid, 2k-50k triangles,
-extent 2, seed 4), decomposed with lookahead_decompose at tau = 0.05,
-width/width2 30/5, depth 2, quick depth 0, 16 concave edges, 16 parts cut in
-parallel, at most 100 iterations, hull merge on. A step is one batched search
-over all 64 meshes (the meshes are independent; every launch spans all of
-them). Metric: meshes per second. After the timed plain steps, one step of
-the learned-proposal form (the shipped network keeps the top 16 level-0
-candidates of each cutting part) is timed and reported under "learned".
+The line's metric is the plain lookahead decomposition (meshes per second):
+64 watertight SDF meshes (smooth union of 3-8 random capsules/boxes minus 0-3
+spheres, marching tetrahedra on a 48^3 grid, 2k-50k triangles, extent 2,
+seed 4), lookahead_decompose at tau 0.05, width/width2 30/5, depth 2, quick
+depth 0, 16 concave edges, 16 parts cut in parallel, at most 100 iterations,
+hull merge on. A step is one batched search over all 64 meshes. --steps and
+--warmup apply to these steps.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--meshes N]
+The same line carries the other workloads as objects, each with its own
+roofline and CPU baseline:
+  learned    the learned proposal on the same 64 meshes at tau 0.05 and 0.03
+             (the shipped network keeps the top_k level-0 candidates of each
+             cutting part), against the plain search at the same tau;
+  hull_batch batch exact hulls + FP64 volumes of 1024 clouds x 256 points
+             uniform in the unit ball (seed 0), inputs resident in HBM;
+  hausdorff  directed Hausdorff distances both ways between 512 noisy level-3
+             icospheres (1280 triangles, radial scale 1 + 0.15 N(0,1), seed 0)
+             and their exact hulls, 4096 stratified area-uniform samples per
+             mesh, as a replayed HIP-graph plan over resident meshes;
+  score      the batched exact cut scorer: 256 SDF meshes (64^3 grid,
+             500-60k triangles, seed 3), 30 candidate planes each, tau 0.05.
 
-`--workload score` measures the batched exact cut scorer instead (candidate
-cuts scored per second): 256 SDF meshes (marching tetrahedra on a 64^3 grid,
-500-60k triangles, seed 3), 30 candidate planes each (up to 16 through
-sampled concave edges, the rest axis cuts), tau 0.05; a step scores all 7680
-cuts (cut, hulls of both halves, Hausdorff where the volume cost is under
-tau). `--autotune` tunes the hull launch shapes on the workload first
-(untimed).
+CPU baselines run first, before the GPU is touched: the oracle on all host
+cores (worker processes or OpenMP threads, at most 16) and on one core, each
+on a bounded sample of its workload. They are baselines, not targets.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--workload all|lookahead|learned|hull|hausdorff|score]
 """
 import argparse
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -32,9 +42,234 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_cutting=16,
               n_concave_edges=16, merge_hulls=True)
+LA_LIMITER = ("latency: each hull is a chain of dependent insertion steps (LDS reads and writes, block barriers); "
+              "HBM traffic is ~1x the algorithmic bytes, so the HBM fraction only says how far from a bandwidth "
+              "bound this work sits")
 
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """Cores the CPU baselines may use: OMP_NUM_THREADS when set (the GPU box
+    sets the one-GPU share), else this process's CPU affinity; at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
+
+
+def _oracle():
+    p = os.path.join(ROOT, "oracle")
+    if p not in sys.path:
+        sys.path.insert(0, p)
+    import oracle
+    import lookahead_oracle
+    return oracle, lookahead_oracle
+
+
+def _worker_init():
+    O, _ = _oracle()
+    O.set_threads(1)
+
+
+# ---------------------------------------------------------------------------
+# workloads (host)
+
+def lookahead_workload(n=64):
+    from cuacd_amd import sdf
+    return sdf.sdf_meshes(n, seed=4, res=48, min_tris=2000, max_tris=50000)
+
+
+def score_workload(n=256):
+    from cuacd_amd import datasets, sdf
+    meshes = sdf.sdf_meshes(n, seed=3, res=64, min_tris=500, max_tris=60000)
+    planes = [datasets.cut_candidates(v, t, n_edges=16, n_total=30, seed=m) for m, (v, t) in enumerate(meshes)]
+    return meshes, planes
+
+
+def hull_workload():
+    from cuacd_amd import datasets
+    return datasets.ball_clouds(1024, 256, seed=0)
+
+
+def hausdorff_workload():
+    from cuacd_amd import datasets
+    return datasets.noisy_icospheres(512, 3, 0.15, seed=0)
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines (the oracle; run before the GPU is initialised)
+
+def _la_task(args):
+    m, v, t, deadline = args
+    O, LO = _oracle()
+    if time.perf_counter() >= deadline:
+        return m, 0, 0.0, False
+    hv, ht, _ = LO.hull_mesh(v)  # the canonical exact hull, as the GPU builds it
+    kw = dict(PARAMS)
+    kw.pop("merge_hulls")
+    t0 = time.perf_counter()
+    _, done = LO.lookahead(v, t, hv, ht, deadline=deadline, **kw)
+    dt = time.perf_counter() - t0
+    return m, done, dt, time.perf_counter() < deadline
+
+
+def cpu_lookahead(meshes, cores, budget_s):
+    """The serial lookahead oracle (Python control over the C oracle's cut,
+    hull, scoring and Hausdorff pieces) on the meshes from the smallest up,
+    each until it converges, for about budget_s: on one core in this process,
+    and on `cores` worker processes (one mesh each). A mesh cut short counts
+    by the fraction of its search iterations done (the GPU run gives each
+    mesh's iteration count). Without the merge post-pass."""
+    O, _ = _oracle()
+    O.set_threads(1)
+    order = [int(m) for m in np.argsort([len(t) for _, t in meshes], kind="stable")]
+    single = []
+    t_end = time.perf_counter() + budget_s
+    for m in order:
+        r = _la_task((m, meshes[m][0], meshes[m][1], t_end))
+        single.append(r)
+        if not r[3]:
+            break
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores, initializer=_worker_init) as pool:
+        pool.map(int, range(cores))  # workers up before the clock starts
+        t0 = time.perf_counter()
+        deadline = t0 + budget_s
+        multi = pool.map(_la_task, [(m, meshes[m][0], meshes[m][1], deadline) for m in order], chunksize=1)
+        wall = time.perf_counter() - t0
+    return {"single": single, "multi": [r for r in multi if r[1] > 0], "wall": wall, "cores": cores}
+
+
+def finish_cpu_lookahead(raw, meshes, gpu_iters):
+    sizes = [len(t) for _, t in meshes]
+
+    def worth(rs):
+        return sum(min(max(done, 1) / max(int(gpu_iters[m]), 1), 1.0) for m, done, _, _ in rs if done > 0)
+
+    def which(rs):
+        ms = [m for m, done, _, _ in rs if done > 0]
+        return f"{len(ms)} meshes of {min(sizes[m] for m in ms)}-{max(sizes[m] for m in ms)} tris" if ms else "none"
+
+    s_time = sum(dt for _, _, dt, _ in raw["single"])
+    ws, wm = worth(raw["single"]), worth(raw["multi"])
+    return {"value": wm / raw["wall"], "unit": "meshes/s", "cores": raw["cores"], "kind": "port",
+            "sample": f"the set's smallest meshes, each to convergence, on {raw['cores']} worker processes for "
+                      f"{raw['wall']:.1f} s: {which(raw['multi'])}, {wm:.2f} meshes' worth of search iterations "
+                      f"(without the merge pass)",
+            "single_core": {"value": ws / max(s_time, 1e-9), "cores": 1,
+                            "sample": f"{which(raw['single'])}, {ws:.2f} meshes' worth in {s_time:.1f} s"}}
+
+
+def cpu_hull(clouds, cores, budget_s):
+    """The oracle's exact hull + FP64 volume (C, OpenMP over clouds) on the
+    full 1024-cloud batch: passes on one thread, then on `cores` threads."""
+    O, _ = _oracle()
+    packed = np.ascontiguousarray(np.concatenate(clouds, 0), np.float32)
+    off = np.zeros(len(clouds) + 1, np.int32)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    out = {}
+    for thr in (1, cores):
+        O.set_threads(thr)
+        O.hull_batch(packed, off, check_general=False)
+        reps, t0 = 0, time.perf_counter()
+        while reps == 0 or time.perf_counter() - t0 < budget_s / 2:
+            O.hull_batch(packed, off, check_general=False)
+            reps += 1
+        dt = time.perf_counter() - t0
+        out[thr] = (reps * len(clouds) / dt, f"{reps} passes over the {len(clouds)}-cloud batch in {dt:.2f} s")
+    O.set_threads(cores)
+    return {"value": out[cores][0], "unit": "clouds/s", "cores": cores, "kind": "port",
+            "sample": out[cores][1] + f" on {cores} threads",
+            "single_core": {"value": out[1][0], "cores": 1, "sample": out[1][1]}}
+
+
+def _surface_samples(v, t, n, rng):
+    a, b, c = v[t[:, 0]].astype(np.float64), v[t[:, 1]].astype(np.float64), v[t[:, 2]].astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    tri = rng.choice(len(t), size=n, p=area / area.sum())
+    r1, r2 = np.sqrt(rng.uniform(size=(n, 1))), rng.uniform(size=(n, 1))
+    p = (1 - r1) * a[tri] + r1 * (1 - r2) * b[tri] + r1 * r2 * c[tri]
+    return p.astype(np.float32)
+
+
+def cpu_hausdorff(meshes, cores, budget_s):
+    """FP64 brute force (the oracle: exact point-triangle distances to every
+    triangle, C, OpenMP over pairs) of both directed distances between each
+    icosphere and its exact hull, 4096 area-uniform samples per surface (the
+    workload's sample count; numpy's sampler - the distance work does not
+    depend on where the samples fall): batches of meshes on one thread, then
+    on `cores` threads, for about budget_s / 2 each; only the distance calls
+    are timed."""
+    O, LO = _oracle()
+    rng = np.random.default_rng(0)
+    hulls = [LO.hull_mesh(v)[:2] for v, _ in meshes]
+    res = {}
+    for thr in (1, cores):
+        O.set_threads(thr)
+        done, spent, i = 0, 0.0, 0
+        while spent < budget_s / 2 and done < len(meshes):
+            idx = [(i + k) % len(meshes) for k in range(thr)]
+            i += thr
+            sm = np.stack([_surface_samples(*meshes[k], 4096, rng) for k in idx])
+            sh = np.stack([_surface_samples(*hulls[k], 4096, rng) for k in idx])
+            t0 = time.perf_counter()
+            O.directed_batch(sm, [hulls[k] for k in idx])
+            O.directed_batch(sh, [meshes[k] for k in idx])
+            spent += time.perf_counter() - t0
+            done += thr
+        res[thr] = (done, spent)
+    O.set_threads(cores)
+    return {"value": res[cores][0] / res[cores][1], "unit": "meshes/s", "cores": cores, "kind": "port",
+            "sample": f"{res[cores][0]} meshes (both directions) in {res[cores][1]:.2f} s on {cores} threads",
+            "single_core": {"value": res[1][0] / res[1][1], "cores": 1,
+                            "sample": f"{res[1][0]} meshes (both directions) in {res[1][1]:.2f} s on one thread"}}
+
+
+def _score_task(args):
+    m, v, t, planes, tau, deadline = args
+    _, LO = _oracle()
+    done, t0 = 0, time.perf_counter()
+    for pl in planes:
+        if time.perf_counter() >= deadline:
+            break
+        LO.cut_cost(v, t, pl, tau)
+        done += 1
+    return m, done, time.perf_counter() - t0
+
+
+def cpu_score(meshes, planes, tau, cores, budget_s):
+    """The serial oracle scorer (C cut, exact hull, reference-sampler
+    Hausdorff) on cuts of the set's meshes from the smallest up: one core in
+    this process, then `cores` worker processes (one mesh each)."""
+    O, _ = _oracle()
+    O.set_threads(1)
+    order = [int(m) for m in np.argsort([len(t) for _, t in meshes], kind="stable")]
+    m0 = order[0]
+    _, d1, s1 = _score_task((m0, meshes[m0][0], meshes[m0][1], planes[m0], tau, time.perf_counter() + budget_s / 2))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores, initializer=_worker_init) as pool:
+        pool.map(int, range(cores))
+        t0 = time.perf_counter()
+        deadline = t0 + budget_s / 2
+        res = pool.map(_score_task, [(m, meshes[m][0], meshes[m][1], planes[m], tau, deadline)
+                                     for m in order[:4 * cores]], chunksize=1)
+        wall = time.perf_counter() - t0
+    cuts = sum(d for _, d, _ in res)
+    sizes = [len(meshes[m][1]) for m, d, _ in res if d > 0]
+    return {"value": cuts / wall, "unit": "cuts/s", "cores": cores, "kind": "port",
+            "sample": f"{cuts} cuts of {len(sizes)} of the smallest meshes ({min(sizes)}-{max(sizes)} tris) on "
+                      f"{cores} worker processes in {wall:.1f} s",
+            "single_core": {"value": d1 / s1, "cores": 1,
+                            "sample": f"{d1} cuts of the smallest mesh ({len(meshes[m0][1])} tris) in {s1:.1f} s"}}
+
+
+# ---------------------------------------------------------------------------
+# GPU legs
 
 def kernel_name(cls, wide_min, half_cap):
     """Kernel behind a hull class reported by the library."""
@@ -59,175 +294,6 @@ def kernel_name(cls, wide_min, half_cap):
     return f"la_hull_wave<{ppl}>"
 
 
-def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def workload(n):
-    from cuacd_amd import sdf
-    return sdf.sdf_meshes(n, seed=4, res=48, min_tris=2000, max_tris=50000)
-
-
-def cpu_baseline(ctx, meshes, gpu_iters, budget_s=20.0):
-    """The serial lookahead oracle (Python control over the C oracle's cut,
-    hull, scoring and Hausdorff pieces, one core) on a bounded sample: the
-    meshes of the set from the smallest up, each run until it converges,
-    until about budget_s of oracle time is spent; a mesh the budget cuts
-    short counts by the fraction of its search iterations done. The input
-    hulls (an input of the search) are computed outside the timed calls."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import lookahead_oracle as LO
-    sizes = [len(t) for _, t in meshes]
-    kw = dict(PARAMS)
-    kw.pop("merge_hulls")
-    spent, worth, used = 0.0, 0.0, []
-    for m in np.argsort(sizes, kind="stable"):
-        m = int(m)
-        if spent >= budget_s:
-            break
-        v, t = meshes[m]
-        hv, ht = ctx._input_hull(v)
-        t0 = time.perf_counter()
-        _, done = LO.lookahead(v, t, hv, ht, deadline=t0 + (budget_s - spent), **kw)
-        spent += time.perf_counter() - t0
-        frac = min(max(done, 1) / max(int(gpu_iters[m]), 1), 1.0)
-        worth += frac
-        used.append(m)
-        if frac < 1.0:
-            break
-    lo, hi = min(sizes[m] for m in used), max(sizes[m] for m in used)
-    which = (f"the smallest mesh of the set ({lo} tris)" if len(used) == 1 else
-             f"the {len(used)} smallest meshes of the set ({lo}-{hi} tris)")
-    return {"value": worth / spent, "unit": "meshes/s", "cores": 1, "kind": "port",
-            "sample": f"{which}: {worth:.2f} meshes' worth of search iterations in {spent:.1f} s "
-                      f"without the merge pass"}
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--meshes", type=int, default=64)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["lookahead", "score"], default="lookahead")
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
-                    help="only run the known-size device copy (HBM counter calibration) and exit")
-    ap.add_argument("--top-k", type=int, default=16,
-                    help="learned-proposal step after the plain steps (0 skips it)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: one GPU only (asked for {args.gpus}); the meshes of the batch are independent and "
-              "shard across GPUs without communication (DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    import cuacd_amd
-
-    if args.copy_probe:
-        ctx = cuacd_amd.Context(device=0)
-        nb = args.copy_probe << 20
-        ms = cuacd_amd._gpu.copy_probe(nb, 3)
-        print(json.dumps({"copy_probe_bytes": nb, "launches": 3, "ms_per_copy": ms,
-                          "gbs": 2 * nb / (ms * 1e-3) / 1e9}), flush=True)
-        ctx.close()
-        return
-    if args.workload == "score":
-        return main_score(args)
-    t0 = time.perf_counter()
-    meshes = workload(args.meshes)
-    tris = np.array([len(t) for _, t in meshes])
-    log(f"workload: {len(meshes)} meshes, {tris.min()}-{tris.max()} tris (mean {tris.mean():.0f}), "
-        f"generated in {time.perf_counter() - t0:.1f} s")
-
-    ctx = cuacd_amd.Context(device=0)
-    name, n_cu, hbm = ctx.device_info()
-    log(f"device {name}, {n_cu} CUs, {hbm / 2**30:.0f} GiB")
-    # first touch of every kernel (not a step): one small mesh
-    ctx.batch_lookahead_decompose(meshes[:1], **dict(PARAMS, max_iters=2))
-
-    for _ in range(args.warmup):
-        ctx.batch_lookahead_decompose(meshes, **PARAMS)
-    times = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, **PARAMS)
-        times.append(time.perf_counter() - t0)
-    stats = ctx.last_lookahead_stats
-    ms_per_step = 1e3 * float(np.mean(times))
-    n_parts = np.array([len(p) for p in parts])
-    iters = np.array([i["iters"] for i in info])
-    max_conc = 0.0
-    for i in info:
-        rv = 0.3 * np.cbrt(3 / (4 * np.pi) * np.maximum(i["hull_vol"] - i["mesh_vol"], 0))
-        c = np.maximum(rv, i["hausdorff"])
-        max_conc = max(max_conc, float(c.max()) if len(c) else 0.0)
-    log(f"step {ms_per_step / 1e3:.2f} s; parts {n_parts.sum()} (per mesh {n_parts.min()}-{n_parts.max()}), "
-        f"iterations {iters.min()}-{iters.max()}, meshes at max_iters {(iters >= PARAMS['max_iters']).sum()}")
-    gpu_ms = sum(s["ms"] for s in stats.values())
-    for k, s in stats.items():
-        log(f"  {k:14s} {s['ms']:9.1f} ms  {s['launches']:5d} launches  {s['alg_bytes'] / 1e9:8.3f} GB "
-            f"({s['alg_bytes'] / max(s['ms'], 1e-9) / 1e6:7.1f} GB/s)")
-    log(f"  GPU time in phases {gpu_ms / 1e3:.2f} s of {ms_per_step / 1e3:.2f} s")
-    sec = ctx.last_lookahead_sections
-    log("  host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()) +
-        f" (sum {sum(sec.values()) / 1e3:.2f} s)")
-    # single-kernel timings: HIP events around each launch of that kernel alone
-    kern = {}
-    for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel"),
-                     ("hausdorff", "hd_task_kernel")):
-        kern[name] = stats[ph]
-    wide_min = ctx.get_tuning("hull_wide_min")
-    half_cap = ctx.get_tuning("hull_half_cap") != 0
-    for cls, k in ctx.last_lookahead_kernels.items():
-        kern[kernel_name(cls, wide_min, half_cap)] = k
-    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
-        log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
-            f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
-    dom = max(kern, key=lambda k: kern[k]["ms"])
-    d = kern[dom]
-    achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-
-    learned = None
-    if args.top_k > 0:
-        learned = learned_run(ctx, meshes, ms_per_step, int(n_parts.sum()), args.top_k)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(ctx, meshes, iters)
-        log(f"cpu baseline {cpu['value']:.4f} meshes/s ({cpu['sample']})")
-
-    value = len(meshes) / (ms_per_step * 1e-3)
-    line = {
-        "metric": "lookahead meshes/sec at tau=0.05 (plain)",
-        "value": value,
-        "unit": "meshes/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid",
-        "data": "synthetic",
-        "config": {"workload": "plain lookahead_decompose: 64 SDF meshes (marching tetrahedra 48^3, 2k-50k tris, "
-                               "extent 2, seed 4), tau 0.05, width/width2 30/5, depth 2, quick 0, max_iters 100, "
-                               "merge_hulls on, n_concave_edges 16, max_n_cutting 16",
-                   "meshes": len(meshes), "tris_total": int(tris.sum()), "parts_total": int(n_parts.sum()),
-                   "max_part_concavity": max_conc, "iterations_max": int(iters.max())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],
-                     "avg_launch_ms": d["ms"] / max(d["launches"], 1),
-                     "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
-        "cpu_baseline": cpu,
-        "learned": learned,
-    }
-    print(json.dumps(line), flush=True)
-    ctx.close()
-
-
 def max_concavity(info):
     m = 0.0
     for i in info:
@@ -237,136 +303,421 @@ def max_concavity(info):
     return m
 
 
-def learned_run(ctx, meshes, plain_ms, plain_parts, top_k):
-    """One step of the learned-proposal form (the shipped network ranks each
-    cutting part's level-0 candidates; only the top_k are evaluated exactly)."""
-    from cuacd_amd import proposal as PR
-    net = PR.ProposalNet.load_default(ctx)
-    try:
+def _roofline(kern, bound="hbm"):
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    launches = max(d["launches"], 1)
+    avg_ms = d["ms"] / launches
+    per_launch = d["alg_bytes"] / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": bound, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "kernel_ms_total": d["ms"],
+            "launches": d["launches"], "avg_launch_ms": avg_ms, "alg_bytes_per_launch": per_launch}
+
+
+def gpu_lookahead(ctx, meshes, steps, warmup, tau=0.05, verbose=True):
+    kw = dict(PARAMS, threshold=tau)
+    for _ in range(warmup):
+        ctx.batch_lookahead_decompose(meshes, **kw)
+    times = []
+    for _ in range(max(steps, 1)):
         t0 = time.perf_counter()
-        parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, proposal=net, top_k=top_k, **PARAMS)
-        ms = 1e3 * (time.perf_counter() - t0)
-    finally:
-        net.close()
-    pr = ctx.last_lookahead_proposal
-    n_parts = sum(len(p) for p in parts)
-    iters = max(i["iters"] for i in info)
-    out = {"metric": "lookahead meshes/sec at tau=0.05 (learned proposal)", "value": len(meshes) / (ms * 1e-3),
-           "unit": "meshes/s", "ms_per_step": ms, "top_k": top_k, "speedup_vs_plain": plain_ms / ms,
-           "parts_total": n_parts, "parts_vs_plain": n_parts / plain_parts - 1.0,
-           "max_part_concavity": max_concavity(info), "iterations_max": int(iters),
-           "candidates_ranked": pr["ranked"], "candidates_kept": pr["kept"], "fallbacks": pr["fallbacks"],
-           "proposal_ms": ctx.last_lookahead_stats["proposal"]["ms"]}
-    log(f"learned top{top_k}: {ms / 1e3:.2f} s ({out['value']:.1f} meshes/s, x{out['speedup_vs_plain']:.2f}), "
-        f"parts {n_parts} ({100 * out['parts_vs_plain']:+.1f}%), max concavity {out['max_part_concavity']:.4f}, "
-        f"kept {pr['kept']} of {pr['ranked']} candidates, network {out['proposal_ms']:.1f} ms")
-    st = ctx.last_lookahead_stats
-    log("  learned phases: " + ", ".join(f"{k} {v['ms']:.0f} ms" for k, v in st.items()) +
-        f" (GPU {sum(v['ms'] for v in st.values()) / 1e3:.2f} s)")
-    sec = ctx.last_lookahead_sections
-    log("  learned host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()))
+        parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, **kw)
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * float(np.mean(times))
+    n_parts = np.array([len(p) for p in parts])
+    iters = np.array([i["iters"] for i in info])
+    out = {"ms": ms, "parts": int(n_parts.sum()), "iters": iters, "max_conc": max_concavity(info),
+           "stats": ctx.last_lookahead_stats, "sections": ctx.last_lookahead_sections,
+           "kernels": ctx.last_lookahead_kernels}
+    if verbose:
+        log(f"plain tau {tau}: step {ms / 1e3:.2f} s; parts {n_parts.sum()} (per mesh {n_parts.min()}-"
+            f"{n_parts.max()}), iterations {iters.min()}-{iters.max()}, meshes at max_iters "
+            f"{(iters >= PARAMS['max_iters']).sum()}, max concavity {out['max_conc']:.4f}")
     return out
 
 
-def score_workload(n):
-    from cuacd_amd import datasets, sdf
-    meshes = sdf.sdf_meshes(n, seed=3, res=64, min_tris=500, max_tris=60000)
-    planes = [datasets.cut_candidates(v, t, n_edges=16, n_total=30, seed=m) for m, (v, t) in enumerate(meshes)]
-    return meshes, planes
+def lookahead_kernels(ctx, r):
+    stats = r["stats"]
+    gpu_ms = sum(s["ms"] for s in stats.values())
+    for k, s in stats.items():
+        log(f"  {k:14s} {s['ms']:9.1f} ms  {s['launches']:5d} launches  {s['alg_bytes'] / 1e9:8.3f} GB "
+            f"({s['alg_bytes'] / max(s['ms'], 1e-9) / 1e6:7.1f} GB/s)")
+    log(f"  GPU time in phases {gpu_ms / 1e3:.2f} s of {r['ms'] / 1e3:.2f} s")
+    sec = r["sections"]
+    log("  host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()) +
+        f" (sum {sum(sec.values()) / 1e3:.2f} s)")
+    kern = {}
+    for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel")):
+        kern[name] = stats[ph]
+    wide_min = ctx.get_tuning("hull_wide_min")
+    half_cap = ctx.get_tuning("hull_half_cap") != 0
+    for cls, k in r["kernels"].items():
+        kern[kernel_name(cls, wide_min, half_cap)] = k
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
+        log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
+            f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
+    return kern
 
 
-def cpu_score_baseline(meshes, planes, tau, budget_s=20.0):
-    """The serial oracle scorer (one core) on the cuts of the smallest mesh,
-    as many as fit in the time budget."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import lookahead_oracle as LO
-    m = int(np.argmin([len(t) for _, t in meshes]))
-    v, t = meshes[m]
-    done = 0
-    t0 = time.perf_counter()
-    for pl in planes[m]:
-        LO.cut_cost(v, t, pl, tau)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "cuts/s", "cores": 1, "kind": "port",
-            "sample": f"{done} cuts of the smallest mesh ({len(t)} tris) by the serial oracle scorer "
-                      f"(C cut, exact hull, reference-sampler Hausdorff) in {dt:.1f} s"}
+def gpu_learned(ctx, meshes, plain, top_k, reps):
+    """The learned proposal at each tau against the plain search at the same tau."""
+    from cuacd_amd import proposal as PR
+    net = PR.ProposalNet.load_default(ctx)
+    out = {}
+    try:
+        for tau, pl in plain.items():
+            kw = dict(PARAMS, threshold=tau)
+            ctx.batch_lookahead_decompose(meshes, proposal=net, top_k=top_k, **kw)  # warm
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                parts, info = ctx.batch_lookahead_decompose(meshes, return_info=True, proposal=net, top_k=top_k,
+                                                            **kw)
+                times.append(time.perf_counter() - t0)
+            ms = 1e3 * float(np.mean(times))
+            pr = ctx.last_lookahead_proposal
+            n_parts = sum(len(p) for p in parts)
+            iters = np.array([i["iters"] for i in info])
+            o = {"metric": f"lookahead meshes/sec at tau={tau} (learned proposal)",
+                 "value": len(meshes) / (ms * 1e-3), "unit": "meshes/s", "ms_per_step": ms,
+                 "ms_per_step_runs": [1e3 * x for x in times], "top_k": top_k,
+                 "plain_ms_per_step": pl["ms"], "speedup_vs_plain": pl["ms"] / ms, "parts_total": n_parts,
+                 "plain_parts_total": pl["parts"], "parts_vs_plain": n_parts / pl["parts"] - 1.0,
+                 "max_part_concavity": max_concavity(info), "plain_max_part_concavity": pl["max_conc"],
+                 "every_part_within_tau": bool(max_concavity(info) <= tau + 1e-6),
+                 "meshes_at_max_iters": int((iters >= PARAMS["max_iters"]).sum()),
+                 "candidates_ranked": pr["ranked"], "candidates_kept": pr["kept"], "fallbacks": pr["fallbacks"],
+                 "proposal_ms": ctx.last_lookahead_stats["proposal"]["ms"]}
+            out[f"tau_{tau}"] = o
+            log(f"learned top{top_k} tau {tau}: {ms / 1e3:.3f} s ({o['value']:.1f} meshes/s, "
+                f"x{o['speedup_vs_plain']:.2f} vs plain {pl['ms'] / 1e3:.2f} s), parts {n_parts} "
+                f"({100 * o['parts_vs_plain']:+.1f}%), max concavity {o['max_part_concavity']:.4f}, kept "
+                f"{pr['kept']} of {pr['ranked']} candidates, network {o['proposal_ms']:.1f} ms; runs "
+                + ", ".join(f"{x:.3f}" for x in times))
+            st = ctx.last_lookahead_stats
+            log("  learned phases: " + ", ".join(f"{k} {v['ms']:.0f} ms" for k, v in st.items()) +
+                f" (GPU {sum(v['ms'] for v in st.values()) / 1e3:.2f} s)")
+            sec = ctx.last_lookahead_sections
+            log("  learned host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()))
+    finally:
+        net.close()
+    return out
 
 
-def main_score(args):
+def gpu_hull(ctx, clouds, launches):
+    """Exact hulls + FP64 volumes of the resident batch: gpu_hull_dandc_dev,
+    one hull_wave_kernel<4> launch per step; HIP events on the context's
+    stream around `launches` back-to-back launches."""
+    from cuacd_amd import _gpu
+    n, npts = len(clouds), len(clouds[0])
+    packed = np.ascontiguousarray(np.concatenate(clouds, 0), np.float32)
+    off = np.arange(n + 1, dtype=np.int32) * npts
+    mhv, mht = npts, 2 * npts - 4
+    bufs = []
+
+    def dalloc(nbytes):
+        bufs.append(_gpu.dev_alloc(int(nbytes)))
+        return bufs[-1]
+
+    d_pts, d_off = dalloc(packed.nbytes), dalloc(off.nbytes)
+    d_v, d_t = dalloc(n * mhv * 12), dalloc(n * mht * 12)
+    d_nv, d_nt, d_e, d_vol = dalloc(n * 4), dalloc(n * 4), dalloc(n * 4), dalloc(n * 8)
+    try:
+        _gpu.memcpy_h2d(d_pts, packed.ctypes.data, packed.nbytes)
+        _gpu.memcpy_h2d(d_off, off.ctypes.data, off.nbytes)
+
+        def step():
+            _gpu.hull_dandc_dev(d_pts, d_off, n, npts, mhv, mht, d_v, d_t, d_nv, d_nt, d_e, d_vol)
+
+        for _ in range(5):
+            step()
+        _gpu.sync()
+        t0 = time.perf_counter()
+        _gpu.timer_start()
+        for _ in range(launches):
+            step()
+        gpu_ms = _gpu.timer_stop()
+        _gpu.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / launches
+        nv, nt, err = (np.empty(n, np.int32) for _ in range(3))
+        vol = np.empty(n, np.float64)
+        for a, d in ((nv, d_nv), (nt, d_nt), (err, d_e), (vol, d_vol)):
+            _gpu.memcpy_d2h(a.ctypes.data, d, a.nbytes)
+    finally:
+        for b in bufs:
+            _gpu.dev_free(b)
+    if (err != 0).any():
+        raise RuntimeError(f"hull batch reported errors {np.unique(err)}")
+    kms = gpu_ms / launches
+    # algorithmic bytes per launch: the clouds and offsets read, each hull's
+    # vertices and triangles, counts, error word and FP64 volume written
+    alg = packed.nbytes + off.nbytes + int(nv.sum()) * 12 + int(nt.sum()) * 12 + n * (12 + 8)
+    achieved = alg / (kms * 1e-3) / 1e9
+    log(f"hull batch: {kms:.4f} ms per launch ({n / (kms * 1e-3):.0f} clouds/s), wall {wall_ms:.4f} ms, "
+        f"mean hull nv {nv.mean():.1f} nt {nt.mean():.1f}, mean volume {vol.mean():.4f}")
+    return {"metric": "batch_hull_volume clouds/sec (exact hull + FP64 volume)", "value": n / (wall_ms * 1e-3),
+            "unit": "clouds/s", "ms_per_step": wall_ms, "launches": launches,
+            "config": {"workload": "batch_hull_volume: 1024 clouds x 256 fp32 points uniform in the unit ball "
+                                   "(rejection-sampled from [-1,1]^3, seed 0), resident in HBM",
+                       "hull_nv_mean": float(nv.mean()), "hull_nt_mean": float(nt.mean()),
+                       "volume_mean": float(vol.mean())},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "hull_wave_kernel<4>",
+                         "avg_launch_ms": kms, "alg_bytes_per_launch": alg, "limiter": LA_LIMITER}}
+
+
+def gpu_hausdorff(ctx, meshes, replays, n_samples=4096):
+    """Both directed distances between each icosphere and its exact hull on
+    resident meshes: a Hausdorff plan (preparation + query launches captured
+    as a HIP graph) replayed `replays` times; HIP events on the context's
+    stream around the replays. The hulls are built first, outside the timed
+    region."""
+    from cuacd_amd import _gpu
+    hulls = [(hv, ht) for hv, ht, _ in ctx.batch_hull_dandc_mesh([v for v, _ in meshes])]
+    n = len(meshes)
+
+    def pack(ms):
+        vs = [np.ascontiguousarray(v, np.float32) for v, _ in ms]
+        ts = [np.ascontiguousarray(t, np.int32) for _, t in ms]
+        vo = np.zeros(n + 1, np.int32)
+        to = np.zeros(n + 1, np.int32)
+        vo[1:] = np.cumsum([len(v) for v in vs])
+        to[1:] = np.cumsum([len(t) for t in ts])
+        return np.concatenate(vs), vo, np.concatenate(ts), to
+
+    av, avo, at, ato = pack(meshes)
+    bv, bvo, bt, bto = pack(hulls)
+    bufs = []
+
+    def up(a):
+        bufs.append(_gpu.dev_alloc(max(int(a.nbytes), 16)))
+        _gpu.memcpy_h2d(bufs[-1], a.ctypes.data, a.nbytes)
+        return bufs[-1]
+
+    plan = 0
+    try:
+        d_av, d_at, d_bv, d_bt = up(av), up(at), up(bv), up(bt)
+        d_out = up(np.zeros(2 * n, np.float32))
+        plan = _gpu.hausdorff_plan_create(d_av, d_at, d_bv, d_bt, avo.ctypes.data, ato.ctypes.data,
+                                          bvo.ctypes.data, bto.ctypes.data, n, 1, n_samples, d_out, d_out + 4 * n,
+                                          0, 0)
+        for _ in range(3):
+            _gpu.hausdorff_plan_run(plan)
+        _gpu.sync()
+        t0 = time.perf_counter()
+        _gpu.timer_start()
+        for _ in range(replays):
+            _gpu.hausdorff_plan_run(plan)
+        gpu_ms = _gpu.timer_stop()
+        _gpu.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / replays
+        d2 = np.empty(2 * n, np.float32)
+        _gpu.memcpy_d2h(d2.ctypes.data, d_out, d2.nbytes)
+    finally:
+        if plan:
+            _gpu.hausdorff_plan_destroy(plan)
+        for b in bufs:
+            _gpu.dev_free(b)
+    d = np.sqrt(d2)
+    rms = gpu_ms / replays
+    # compulsory bytes: both meshes of each pair read once, two distances written
+    alg = av.nbytes + at.nbytes + bv.nbytes + bt.nbytes + 8 * n
+    achieved = alg / (rms * 1e-3) / 1e9
+    # brute-force-equivalent work: every sample against every triangle of the
+    # other surface, ~70 flops per point-triangle distance
+    flops = float(n_samples) * (len(at) + len(bt)) * 70.0
+    tfl = flops / (rms * 1e-3) / 1e12
+    log(f"hausdorff: {rms:.3f} ms per replay ({n / (rms * 1e-3):.0f} meshes/s), wall {wall_ms:.3f} ms; "
+        f"H(mesh->hull) mean {d[:n].mean():.4f}, H(hull->mesh) mean {d[n:].mean():.4f}; brute-force-equivalent "
+        f"{tfl:.1f} TFLOP/s")
+    return {"metric": "concavity metric meshes/sec (directed Hausdorff both ways, 4096 samples)",
+            "value": n / (wall_ms * 1e-3), "unit": "meshes/s", "ms_per_step": wall_ms, "replays": replays,
+            "config": {"workload": "512 level-3 icospheres (642 v / 1280 t, radial scale 1 + 0.15 N(0,1), seed 0) "
+                                   "against their exact hulls, 4096 stratified area-uniform samples per surface, "
+                                   "resident in HBM, one HIP-graph plan replay per step",
+                       "hull_nv_mean": float(np.mean([len(h[0]) for h in hulls])),
+                       "h_mesh_to_hull_mean": float(d[:n].mean()), "h_hull_to_mesh_mean": float(d[n:].mean())},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "hd_prep_kernel + hd_query_kernel (one graph replay)", "avg_launch_ms": rms,
+                         "alg_bytes_per_launch": alg,
+                         "valu_equivalent": {"gflop_per_launch": flops / 1e9, "achieved_tflops": tfl,
+                                             "peak_tflops": FP32_PEAK_TFLOPS, "frac": tfl / FP32_PEAK_TFLOPS,
+                                             "note": "brute-force-equivalent work (every sample against every "
+                                                     "triangle, 70 flops each); the BVH does less"}}}
+
+
+def gpu_score(ctx, meshes, planes, steps, tau=0.05):
     import cuacd_amd
-    from cuacd_amd import autotune
-    tau = 0.05
-    n = 256 if args.meshes == 64 else args.meshes
-    t0 = time.perf_counter()
-    meshes, planes = score_workload(n)
-    tris = np.array([len(t) for _, t in meshes])
     mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, p in enumerate(planes)])
     pl = np.concatenate(planes)
-    log(f"score workload: {len(meshes)} meshes, {tris.min()}-{tris.max()} tris (mean {tris.mean():.0f}), "
-        f"{len(pl)} cuts, generated in {time.perf_counter() - t0:.1f} s")
-    ctx = cuacd_amd.Context(device=0)
-    name, n_cu, hbm = ctx.device_info()
-    log(f"device {name}, {n_cu} CUs, {hbm / 2**30:.0f} GiB")
     ctx.batch_score_cuts(meshes[:1], mesh_of[:30], pl[:30], threshold=tau)  # first touch
-    if args.autotune:
-        best, trials = autotune.tune(ctx, lambda: ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau), log=log)
-        log(f"autotune: {best} ({len(trials)} trials)")
-    for _ in range(args.warmup):
-        ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau)
+    ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau)  # warm
     times = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t0 = time.perf_counter()
         score, detail, st = ctx.batch_score_cuts(meshes, mesh_of, pl, threshold=tau, return_detail=True)
         times.append(time.perf_counter() - t0)
-    ms_per_step = 1e3 * float(np.mean(times))
+    ms = 1e3 * float(np.mean(times))
     scored = int(detail[:, 9].sum())
     n_h = int((detail[:, 7:9] >= 0).sum())
-    log(f"step {ms_per_step:.1f} ms; {scored} of {len(pl)} cuts scored, {n_h} halves with Hausdorff; "
-        f"stages: cuts {st[0]:.1f} ms, hulls {st[1]:.1f} ms, hausdorff {st[2]:.1f} ms")
+    log(f"score: {ms:.1f} ms per step ({len(pl) / (ms * 1e-3):.0f} cuts/s); {scored} of {len(pl)} cuts scored, "
+        f"{n_h} halves with Hausdorff; stages: cuts {st[0]:.1f} ms, hulls {st[1]:.1f} ms, hausdorff {st[2]:.1f} ms")
     kern = {"la_cut_kernel": {"ms": st[0], "launches": 1, "alg_bytes": st[3]}}
     wide_min = ctx.get_tuning("hull_wide_min")
     half_cap = ctx.get_tuning("hull_half_cap") != 0
     for i, cls in enumerate(cuacd_amd.LOOKAHEAD_KERNELS):
-        ms, la, by = st[6 + 3 * i: 9 + 3 * i]
-        kern[kernel_name(cls, wide_min, half_cap)] = {"ms": ms, "launches": int(la), "alg_bytes": by}
-    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
+        k_ms, la, by = st[6 + 3 * i: 9 + 3 * i]
+        kern[kernel_name(cls, wide_min, half_cap)] = {"ms": k_ms, "launches": int(la), "alg_bytes": by}
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:5]:
         log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
             f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
-    dom = max(kern, key=lambda k: kern[k]["ms"])
-    d = kern[dom]
-    achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    cpu = None
+    tris = np.array([len(t) for _, t in meshes])
+    roof = _roofline(kern)
+    roof["limiter"] = LA_LIMITER
+    return {"metric": "candidate cuts scored/sec (plain, batched exact scorer)", "value": len(pl) / (ms * 1e-3),
+            "unit": "cuts/s", "ms_per_step": ms, "steps": steps,
+            "config": {"workload": f"batched cut scoring: {len(meshes)} SDF meshes (marching tetrahedra 64^3, "
+                                   f"500-60k tris, extent 2, seed 3), 30 candidate planes each (16 concave-edge "
+                                   f"planes + axis cuts), tau 0.05", "cuts": int(len(pl)),
+                       "tris_total": int(tris.sum()), "stage_ms": {"cuts": st[0], "hulls": st[1], "hausdorff": st[2]}},
+            "roofline": roof}
+
+
+# ---------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--meshes", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["all", "lookahead", "learned", "hull", "hausdorff", "score"],
+                    default="all")
+    ap.add_argument("--autotune", action="store_true", help="tune the hull launch shapes on the score workload")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run the known-size device copy (HBM counter calibration) and exit")
+    ap.add_argument("--top-k", type=int, default=16)
+    ap.add_argument("--learned-reps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: one GPU only (asked for {args.gpus}); the meshes of the batch are independent and "
+              "shard across GPUs without communication", file=sys.stderr)
+        sys.exit(2)
+    w = args.workload
+    want = {k: w in ("all", k) for k in ("lookahead", "learned", "hull", "hausdorff", "score")}
+    want["lookahead"] = want["lookahead"] or want["learned"]
+    cores = host_cores()
+
+    # ---- workloads and CPU baselines, before the GPU is touched
+    t0 = time.perf_counter()
+    meshes = lookahead_workload(args.meshes) if want["lookahead"] else None
+    clouds = hull_workload() if want["hull"] else None
+    spheres = hausdorff_workload() if want["hausdorff"] else None
+    smeshes, splanes = score_workload() if want["score"] else (None, None)
+    log(f"workloads generated in {time.perf_counter() - t0:.1f} s; CPU baselines on {cores} cores")
+    if meshes is not None:
+        tris = np.array([len(t) for _, t in meshes])
+        log(f"lookahead workload: {len(meshes)} meshes, {tris.min()}-{tris.max()} tris (mean {tris.mean():.0f})")
+    cpu = {}
     if not args.no_cpu:
-        cpu = cpu_score_baseline(meshes, planes, tau)
-        log(f"cpu baseline {cpu['value']:.3f} cuts/s ({cpu['sample']})")
-    line = {
-        "metric": "candidate cuts scored/sec (plain, batched exact scorer)",
-        "value": len(pl) / (ms_per_step * 1e-3),
-        "unit": "cuts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid",
-        "data": "synthetic",
-        "config": {"workload": f"batched cut scoring: {len(meshes)} SDF meshes (marching tetrahedra 64^3, "
-                               f"500-60k tris, extent 2, seed 3), 30 candidate planes each (16 concave-edge "
-                               f"planes + axis cuts), tau 0.05", "cuts": int(len(pl)),
-                   "tris_total": int(tris.sum()), "autotuned": bool(args.autotune),
-                   "tuning": ctx.get_tuning()},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": dom, "kernel_ms_total": d["ms"], "launches": d["launches"],
-                     "avg_launch_ms": d["ms"] / max(d["launches"], 1),
-                     "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
-        "cpu_baseline": cpu,
-    }
+        t0 = time.perf_counter()
+        if want["score"]:
+            cpu["score"] = cpu_score(smeshes, splanes, 0.05, cores, budget_s=12.0)
+        if want["lookahead"] and w in ("all", "lookahead"):
+            cpu["lookahead_raw"] = cpu_lookahead(meshes, cores, budget_s=12.0)
+        if want["hull"]:
+            cpu["hull"] = cpu_hull(clouds, cores, budget_s=6.0)
+        if want["hausdorff"]:
+            cpu["hausdorff"] = cpu_hausdorff(spheres, cores, budget_s=10.0)
+        log(f"CPU baselines done in {time.perf_counter() - t0:.1f} s")
+
+    import cuacd_amd
+    ctx = cuacd_amd.Context(device=0)
+    name, n_cu, hbm = ctx.device_info()
+    log(f"device {name}, {n_cu} CUs, {hbm / 2**30:.0f} GiB")
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        ms = cuacd_amd._gpu.copy_probe(nb, 3)
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 3, "ms_per_copy": ms,
+                          "gbs": 2 * nb / (ms * 1e-3) / 1e9}), flush=True)
+        ctx.close()
+        return
+    if args.autotune and want["score"]:
+        from cuacd_amd import autotune
+        mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, p in enumerate(splanes)])
+        pl = np.concatenate(splanes)
+        best, trials = autotune.tune(ctx, lambda: ctx.batch_score_cuts(smeshes, mesh_of, pl, threshold=0.05),
+                                     log=log)
+        log(f"autotune: {best} ({len(trials)} trials)")
+
+    objs = {}
+    line = None
+    if want["lookahead"]:
+        ctx.batch_lookahead_decompose(meshes[:1], **dict(PARAMS, max_iters=2))  # first touch
+        plain = gpu_lookahead(ctx, meshes, args.steps, args.warmup)
+        kern = lookahead_kernels(ctx, plain)
+        roof = _roofline(kern)
+        roof["limiter"] = LA_LIMITER
+        cb = None
+        if "lookahead_raw" in cpu:
+            cb = finish_cpu_lookahead(cpu["lookahead_raw"], meshes, plain["iters"])
+            log(f"cpu baseline lookahead {cb['value']:.4f} meshes/s on {cb['cores']} cores "
+                f"({cb['single_core']['value']:.4f} on one)")
+        tris = np.array([len(t) for _, t in meshes])
+        line = {
+            "metric": "lookahead meshes/sec at tau=0.05 (plain)",
+            "value": len(meshes) / (plain["ms"] * 1e-3),
+            "unit": "meshes/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": plain["ms"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid",
+            "data": "synthetic",
+            "config": {"workload": "plain lookahead_decompose: 64 SDF meshes (marching tetrahedra 48^3, 2k-50k "
+                                   "tris, extent 2, seed 4), tau 0.05, width/width2 30/5, depth 2, quick 0, "
+                                   "max_iters 100, merge_hulls on, n_concave_edges 16, max_n_cutting 16",
+                       "meshes": len(meshes), "tris_total": int(tris.sum()), "parts_total": plain["parts"],
+                       "max_part_concavity": plain["max_conc"], "iterations_max": int(plain["iters"].max())},
+            "roofline": roof,
+            "cpu_baseline": cb,
+        }
+        if want["learned"] and args.top_k > 0:
+            plains = {0.05: plain, 0.03: gpu_lookahead(ctx, meshes, 1, 0, tau=0.03)}
+            objs["learned"] = gpu_learned(ctx, meshes, plains, args.top_k, args.learned_reps)
+    if want["hull"]:
+        o = gpu_hull(ctx, clouds, 50)
+        o["cpu_baseline"] = cpu.get("hull")
+        objs["hull_batch"] = o
+    if want["hausdorff"]:
+        o = gpu_hausdorff(ctx, spheres, 20)
+        o["cpu_baseline"] = cpu.get("hausdorff")
+        objs["hausdorff"] = o
+    if want["score"]:
+        o = gpu_score(ctx, smeshes, splanes, steps=3)
+        o["cpu_baseline"] = cpu.get("score")
+        objs["score"] = o
+    for k, o in objs.items():
+        c = o.get("cpu_baseline") if isinstance(o, dict) else None
+        if c:
+            log(f"cpu baseline {k}: {c['value']:.4g} {c['unit']} on {c['cores']} cores, "
+                f"{c['single_core']['value']:.4g} on one")
+    if line is None:
+        # a single non-lookahead workload: its object is the line
+        k, o = next(iter(objs.items()))
+        line = {"metric": o["metric"], "value": o["value"], "unit": o["unit"], "n_gpus": 1,
+                "steps": o.get("steps", o.get("launches", o.get("replays"))), "warmup": args.warmup,
+                "ms_per_step": o["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "fp32 geometry / fp64 volumes / int32 exact hull grid", "data": "synthetic",
+                "config": o["config"], "roofline": o["roofline"], "cpu_baseline": o.get("cpu_baseline")}
+    else:
+        line.update(objs)
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -1146,11 +1146,100 @@ struct Engine {
         return 0;
     }
 
+    // ---- observability (reference csrc/lookahead.c:284-436, 745-892) --------
+    // CUACD_LEAK_BISECT: after each phase, the live block counts (allocations
+    // minus frees) of both heaps and their change since the last snapshot.
+    static bool bisect_on() {
+        static int v = -1;
+        if (v < 0) {
+            const char* e = getenv("CUACD_LEAK_BISECT");
+            v = (e && e[0] && e[0] != '0') ? 1 : 0;
+        }
+        return v == 1;
+    }
+    long long bisect_prev[2] = {0, 0};
+    int bisect_snap(int iter, const char* label) {
+        if (!bisect_on()) return 0;
+        unsigned long long st[6];
+        LA_TRY(gpu_heap_stats(ctx, st));
+        const long long h = (long long)(st[1] - st[2]), sc = (long long)(st[4] - st[5]);
+        fprintf(stderr,
+                "[bisect iter=%d] %-24s heap_diff=%lld (%+lld) [ha=%llu hf=%llu]  scratch_diff=%lld (%+lld) "
+                "[sa=%llu sf=%llu]\n",
+                iter, label, h, h - bisect_prev[0], st[1], st[2], sc, sc - bisect_prev[1], st[4], st[5]);
+        bisect_prev[0] = h;
+        bisect_prev[1] = sc;
+        return 0;
+    }
+
+    // verbose: per-part table of every active mesh after the iteration's sort
+    // ('*' = at or above the threshold); CUACD_DUMP_PARTS_DIR=<dir>: every such
+    // part written to <dir>/iter%03d_part%03d.bin (mesh%03d_ prefixed when the
+    // batch has several meshes) as int32 {nv, nt}, float32 {mesh_vol,
+    // hull_vol}, float32 verts [nv*3], int32 tris [nt*3].
+    int diag_parts(const std::vector<int>& act, int iter) {
+        const char* dir = getenv("CUACD_DUMP_PARTS_DIR");
+        if (dir && !dir[0]) dir = nullptr;
+        if (!P.verbose && !dir) return 0;
+        for (int m : act) {
+            const MeshState& S = M[m];
+            if (P.verbose) {
+                if (M.size() > 1) fprintf(stderr, "[la] mesh %d iter %d\n", m, iter);
+                fprintf(stderr, "[la]   part   nv   nt  mesh_vol  hull_vol    rv_cost  hausdorff  full_cost\n");
+                for (size_t i = 0; i < S.parts.size(); ++i) {
+                    const Part& q = S.parts[i];
+                    const float rc = cost_rv(q.hvol, q.mvol), fc = cost_full(q);
+                    fprintf(stderr, "[la]   %3zu  %4d %4d  %9.6f %9.6f  %9.6f %10.6f %10.6f%s\n", i, q.nv, q.nt,
+                            (float)q.mvol, (float)q.hvol, rc, q.haus, fc, fc >= P.threshold ? " *" : "");
+                }
+            }
+        }
+        if (!dir) return 0;
+        std::vector<std::pair<const void*, size_t>> src;
+        std::vector<std::pair<int, int>> who;
+        for (int m : act)
+            for (size_t i = 0; i < M[m].parts.size(); ++i) {
+                const Part& q = M[m].parts[i];
+                if (q.nv <= 0 || q.nt <= 0 || !q.v || !q.t) continue;
+                src.push_back({q.v, 12ull * q.nv});
+                src.push_back({q.t, 12ull * q.nt});
+                who.push_back({m, (int)i});
+            }
+        std::vector<size_t> offs;
+        int rc = 0;
+        const char* hb = gather_to_host(src, offs, &rc);
+        if (rc) return rc;
+        for (size_t k = 0; k < who.size(); ++k) {
+            const Part& q = M[who[k].first].parts[who[k].second];
+            char path[1200];
+            if (M.size() > 1)
+                snprintf(path, sizeof(path), "%s/mesh%03d_iter%03d_part%03d.bin", dir, who[k].first, iter,
+                         who[k].second);
+            else
+                snprintf(path, sizeof(path), "%s/iter%03d_part%03d.bin", dir, iter, who[k].second);
+            FILE* f = fopen(path, "wb");
+            if (!f) continue;
+            const int hdr[2] = {q.nv, q.nt};
+            const float vols[2] = {(float)q.mvol, (float)q.hvol};
+            fwrite(hdr, sizeof(int), 2, f);
+            fwrite(vols, sizeof(float), 2, f);
+            fwrite(hb + offs[2 * k], 1, 12ull * q.nv, f);
+            fwrite(hb + offs[2 * k + 1], 1, 12ull * q.nt, f);
+            fclose(f);
+        }
+        return 0;
+    }
+
     // ---- one search iteration over the meshes in `act`; returns meshes still active
     int iterate(std::vector<int>& act, int iter) {
+        const auto iter_t0 = std::chrono::steady_clock::now();
+        LA_TRY(bisect_snap(iter, "iter_start"));
         if (P.dc_per_iter) LA_TRY(components(act));
+        LA_TRY(bisect_snap(iter, "components"));
         LA_TRY(hull_parts(act));
+        LA_TRY(bisect_snap(iter, "part_hulls"));
         LA_TRY(hausdorff_parts(act));
+        LA_TRY(bisect_snap(iter, "hausdorff_parts"));
         // sort by full cost, pick the cutting parts (cuda/la_lifecycle.cu:90-144)
         std::vector<std::pair<int, int>> cutting;   // (mesh, part index)
         std::vector<int> cut_begin(M.size(), 0), cut_count(M.size(), 0);
@@ -1174,6 +1263,19 @@ struct Engine {
             if (cut_count[m] == 0) S.done = true;
             else next.push_back(m);
         }
+        LA_TRY(diag_parts(act, iter));
+        if (P.verbose)
+            for (int m : act) {
+                if (M.size() > 1) fprintf(stderr, "[la] mesh %d", m);
+                fprintf(stderr, "[la] iter %d: n_cutting=%d", iter, cut_count[m]);
+                if (P.verbose >= 2) {
+                    fprintf(stderr, " cutting_indices=[");
+                    for (int c = 0; c < cut_count[m]; ++c)
+                        fprintf(stderr, "%s%d", c ? " " : "", cutting[cut_begin[m] + c].second);
+                    fprintf(stderr, "]");
+                }
+                fprintf(stderr, "\n");
+            }
         act = next;
         lap(S_PRE);
         if (cutting.empty()) return 0;
@@ -1189,6 +1291,7 @@ struct Engine {
                 n_edge[cutting[c].first] = std::max(n_edge[cutting[c].first], (int)eplanes[c].size() / 4);
         }
         lap(S_CONC);
+        LA_TRY(bisect_snap(iter, "concave_edges"));
         const int total_levels = P.depth + P.quick_depth;
         // level-0 candidates per cutting part, for the training record
         std::vector<std::vector<int>> rec_k(cutting.size());
@@ -1684,9 +1787,16 @@ struct Engine {
             int bi = 0;
             for (size_t k = 0; k < best[c].size(); ++k)
                 if (best[c][k] < bc) { bc = best[c][k]; bi = (int)k; }
-            if (P.verbose)
+            if (P.verbose) {
                 fprintf(stderr, "[la] mesh %d iter %d part %d: best cut %d cost %.6f\n", cutting[c].first, iter,
                         cutting[c].second, bi, bc);
+                if (P.verbose >= 2) {
+                    fprintf(stderr, "[la]   best path per first cut:");
+                    for (size_t k = 0; k < best[c].size(); ++k)
+                        if (best[c][k] < 1e30f) fprintf(stderr, " %zu:%.6f", k, best[c][k]);
+                    fprintf(stderr, "\n");
+                }
+            }
             if (bi >= (int)l0[c].size() || l0[c][bi] < 0) continue;
             const Item& w = level0_items[l0[c][bi]];
             // adopt the unsorted halves of the level-0 cut: positive half replaces the part
@@ -1727,6 +1837,12 @@ struct Engine {
         store_guard.released = true;
         const int frc = flush_frees();
         lap(S_APPLY);
+        if (frc == 0) LA_TRY(bisect_snap(iter, "apply_cuts"));
+        if (P.verbose) {
+            const auto now = std::chrono::steady_clock::now();
+            fprintf(stderr, "[la] iter %d: %.1f ms\n", iter,
+                    std::chrono::duration<double, std::milli>(now - iter_t0).count());
+        }
         return frc;
     }
 };
@@ -1824,6 +1940,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         act.push_back(m);
     }
     E.lap(S_SETUP);
+    rc = E.bisect_snap(-1, "post_init");
     for (int iter = 0; iter < P.max_iters && !act.empty() && rc == 0; ++iter) rc = E.iterate(act, iter);
     // meshes that ran out of iterations: last apply still needs components + hulls
     if (rc == 0 && !act.empty()) {

@@ -23,6 +23,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <omp.h>
 #include <string.h>
 
 /* ---- quantization (cuda/hull_dandc.cuh:1351-1411) ------------------------ */
@@ -497,3 +498,8 @@ int orc_hull_batch_serial(const float* pts, const int* offsets, int n_clouds, un
     }
     return bad;
 }
+
+/* OpenMP threads of the batch functions (CPU baselines on one core and on
+ * all of them). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }

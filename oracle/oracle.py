@@ -43,8 +43,19 @@ def lib():
         L.orc_directed_hausdorff.argtypes = [P, C.c_int, P, P, C.c_int]
         L.orc_directed_hausdorff.restype = C.c_double
         L.orc_directed_batch.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """OpenMP threads of the batch functions (hull_batch, directed_batch, ...)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def _p(a):

