@@ -13,6 +13,7 @@
 //     (kept in ascending input order, so lowest-index duplicate resolution is
 //     preserved) replaces the cloud, until the block kernel can finish it.
 #include <stdlib.h>
+#include <algorithm>
 #include <vector>
 #include "ctx.hpp"
 #include "hull.hpp"
@@ -303,6 +304,55 @@ int launch_block(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
     return 0;
 }
 
+// ---- the allocation-free entry point --------------------------------------
+// gpu_hull_dandc_dev_noalloc's block tier: the work of hull_block_kernel for
+// items item0 + b (workgroup b), each in workspace b of the caller's region.
+template <int PPL>
+__global__ __launch_bounds__(64 * BLOCK_NW) void hull_block_pass_kernel(HullItemArgs a, int mode, int item0) {
+    __shared__ HullBlockLDS sb;
+    const int item = item0 + (int)blockIdx.x;
+    if (item >= a.n_items) return;
+    const HullWS<AS_GLOBAL> w = hull_ws_attach(
+        hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)blockIdx.x * a.gws_stride, a.VC, a.NP), &sb);
+    hull_item<PPL, AS_GLOBAL, BLOCK_NW>(a, mode, item, w, &sb);
+}
+
+// Block tier over a fixed workspace region of ws_cap bytes: passes of as
+// many items as fit (same kernel choice and workspace layout as launch_block).
+int launch_block_passes(gpu_ctx_t ctx, HullItemArgs a, int maxn, char* ws, unsigned long long ws_cap) {
+    if (a.n_items <= 0) return 0;
+    using PassFn = void (*)(HullItemArgs, int, int);
+    PassFn k;
+    constexpr int T = 64 * BLOCK_NW;
+    if (maxn <= 4 * T) k = hull_block_pass_kernel<4>;
+    else if (maxn <= 8 * T) k = hull_block_pass_kernel<8>;
+    else if (maxn <= 16 * T) k = hull_block_pass_kernel<16>;
+    else k = hull_block_pass_kernel<32>;
+    const int np = (maxn + 63) / 64 * 64;
+    a.NP = np;
+    a.VC = np < 32704 ? np : 32704;
+    a.gws_stride = (hull_ws_bytes<AS_GLOBAL>(a.VC, a.NP) + 255ull) & ~255ull;
+    const unsigned long long per = ws_cap / a.gws_stride;
+    REQUIRE(per >= 1, "a %d-point cloud needs %llu workspace bytes, above the %llu-byte scratch", maxn,
+            a.gws_stride, ws_cap);
+    a.gws = ws;
+    for (long long i0 = 0; i0 < a.n_items; i0 += (long long)per) {
+        const long long cnt = std::min<long long>((long long)per, a.n_items - i0);
+        hipLaunchKernelGGL(k, dim3((unsigned)cnt), dim3(T), 0, ctx->stream, a, 0, (int)i0);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// Its workspace: a file-scope device array, one copy per GPU (each device that
+// loads the library holds its own until the process ends). On the host the
+// array's name is not a device address: host code uses the address looked up
+// for the device (scratch_ptr), never the name.
+constexpr unsigned long long HULL_SCRATCH_BYTES = 64ull << 20;
+__device__ __align__(256) char scratch[HULL_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+char* scratch_ptr[SCRATCH_MAX_DEVICES];  // per device, set at the entry's first call there
+
 }  // namespace
 
 // Device-pointer entry: all buffers on the device; stream-ordered. Clouds
@@ -431,6 +481,45 @@ static int hull_batch_device(gpu_ctx_t ctx, const float* d_pts, const int* d_off
 }
 
 extern "C" {
+
+int gpu_hull_dandc_dev_noalloc(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, int n_hulls, int max_pts,
+                               int max_hv, int max_ht, float* d_verts, int* d_tris, int* d_nv, int* d_nt,
+                               int* d_err, double* d_vol) {
+    REQUIRE(ctx->device >= 0 && ctx->device < SCRATCH_MAX_DEVICES, "gpu_hull_dandc_dev_noalloc: device %d",
+            ctx->device);
+    HIP_TRY(hipSetDevice(ctx->device));
+    REQUIRE(n_hulls >= 0 && max_pts >= 0 && max_hv >= 0 && max_ht >= 0,
+            "gpu_hull_dandc_dev_noalloc: invalid arguments");
+    REQUIRE(max_pts <= BLOCK_MAX, "gpu_hull_dandc_dev_noalloc: clouds above %d points are not supported",
+            BLOCK_MAX);
+    if (n_hulls == 0) return 0;
+    // first call on this device (now current): its copy of the array
+    if (!scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        REQUIRE(p != nullptr, "gpu_hull_dandc_dev_noalloc: no device address for the scratch array");
+        scratch_ptr[ctx->device] = (char*)p;
+    }
+    HullItemArgs a{};
+    a.pts = d_pts;
+    a.offsets = d_offsets;
+    a.n_items = n_hulls;
+    a.max_hv = max_hv;
+    a.max_ht = max_ht;
+    a.out_verts = d_verts;
+    a.out_tris = d_tris;
+    a.out_nv = d_nv;
+    a.out_nt = d_nt;
+    a.out_err = d_err;
+    a.out_vol = d_vol;
+    a.min_n = 0;
+    a.max_n = WAVE_MAX;
+    int rc = launch_wave(ctx, a, max_pts < WAVE_MAX ? max_pts : WAVE_MAX, 0);
+    if (rc || max_pts <= WAVE_MAX) return rc;
+    a.min_n = WAVE_MAX + 1;
+    a.max_n = BLOCK_MAX;
+    return launch_block_passes(ctx, a, max_pts, scratch_ptr[ctx->device], HULL_SCRATCH_BYTES);
+}
 
 int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, int n_hulls, int max_pts,
                        int max_hv, int max_ht, float* d_verts, int* d_tris, int* d_nv, int* d_nt, int* d_err,

@@ -183,6 +183,20 @@ int gpu_hull_dandc_dev(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, 
                        int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* d_verts,
                        int* d_tris, int* d_nv, int* d_nt, int* d_errors, double* d_volumes);
 
+/* gpu_hull_dandc_dev without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory - code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. Same
+ * arguments and the same outputs, bit for bit; clouds of at most 16384
+ * points (max_pts_per_hull above that is rejected with an error). The
+ * workspaces of clouds above 1024 points come from a fixed 64 MiB device
+ * array per GPU (held until the process ends), in passes of as many clouds
+ * as fit; it calls no hipMalloc, hipFree or allocating library routine.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other contexts on the same device) race. */
+int gpu_hull_dandc_dev_noalloc(gpu_ctx_t ctx, const float* d_pts, const int* d_offsets, int n_hulls,
+                               int max_pts_per_hull, int max_hull_verts, int max_hull_tris, float* d_verts,
+                               int* d_tris, int* d_nv, int* d_nt, int* d_errors, double* d_volumes);
+
 /* ---- batched directed Hausdorff (no reference counterpart) -----------------
  * For pairs (A_i, B_i): out_ab[i] = max over samples p of A_i of the distance
  * from p to B_i's surface, out_ba[i] the other way round. mode 0 samples like
