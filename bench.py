@@ -11,8 +11,10 @@ hull merge on. A step is one batched search over all 64 meshes. --steps and
 The same line carries the other workloads as objects, each with its own
 roofline and CPU baseline:
   learned    the learned proposal on the same 64 meshes at tau 0.05 and 0.03
-             (the shipped network keeps the top_k level-0 candidates of each
-             cutting part), against the plain search at the same tau;
+             (the shipped network keeps the top 16 and top 24 level-0
+             candidates of each cutting part: the fewest that keep the part
+             count within 10 % of the plain search's), against the plain
+             search at the same tau;
   hull_batch batch exact hulls + FP64 volumes of 1024 clouds x 256 points
              uniform in the unit ball (seed 0), inputs resident in HBM;
   hausdorff  directed Hausdorff distances both ways between 512 noisy level-3
@@ -20,7 +22,9 @@ roofline and CPU baseline:
              and their exact hulls, 4096 stratified area-uniform samples per
              mesh, as a replayed HIP-graph plan over resident meshes;
   score      the batched exact cut scorer: 256 SDF meshes (64^3 grid,
-             500-60k triangles, seed 3), 30 candidate planes each, tau 0.05.
+             500-60k triangles, seed 3), 30 candidate planes each, tau 0.05;
+             eager (host-sized stages), and as a score plan replayed as one
+             captured HIP graph (object "graph").
 
 CPU baselines run first, before the GPU is touched: the oracle on all host
 cores (worker processes or OpenMP threads, at most 16) and on one core, each
@@ -45,6 +49,10 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_cutting=16,
               n_concave_edges=16, merge_hulls=True)
+# VALU busy fraction of the hull tiers, from their SQ counters
+# (profiles/lookahead_64sdf_hull_tier_counters.md): counters cannot be read
+# inside the timed run, so the line carries the profiled figure beside HBM
+HULL_VALU_BUSY = {"la_hull_ldsv<4,8>": 0.018, "la_hull_ldsv<4,4>": 0.021, "la_hull_block<8>": 0.007}
 LA_LIMITER = ("latency: each hull is a chain of dependent insertion steps (LDS reads and writes, block barriers); "
               "HBM traffic is ~1x the algorithmic bytes, so the HBM fraction only says how far from a bandwidth "
               "bound this work sits")
@@ -271,19 +279,21 @@ def cpu_score(meshes, planes, tau, cores, budget_s):
 # ---------------------------------------------------------------------------
 # GPU legs
 
-def kernel_name(cls, wide_min, half_cap):
+def kernel_name(cls, wide_min, half_cap, ldsb8=0):
     """Kernel behind a hull class reported by the library."""
     if cls == "kdop_prefilter":
         return "la_kdop_kernel"
     if cls.endswith("_retry"):
         base = cls[:-6]
         n = int(base[6:])
-        return "la_hull_block<4>" if base.startswith("hull_b") else f"la_hull_lds4<{n // 256}>"
+        return f"la_hull_block<{n // 512}>" if base.startswith("hull_b") else f"la_hull_lds4<{n // 256}>"
     if cls.startswith("hull_b"):
         n = int(cls[6:])
         ppl = n // 512
         if n == 2048 and half_cap:
             return "la_hull_ldsv<4,8>"
+        if n == 4096 and half_cap and ldsb8:
+            return "la_hull_ldsv<8,8>"
         return f"la_hull_block<{ppl}>"
     n = int(cls[6:])
     ppl = n // 64
@@ -337,6 +347,17 @@ def gpu_lookahead(ctx, meshes, steps, warmup, tau=0.05, verbose=True):
     return out
 
 
+def _add_kernel(kern, name, k):
+    """Several timer classes can run the same kernel (a size class and the
+    retries of a smaller tier): their times, launches and bytes add up."""
+    if name in kern:
+        o = kern[name]
+        kern[name] = {"ms": o["ms"] + k["ms"], "launches": o["launches"] + k["launches"],
+                      "alg_bytes": o["alg_bytes"] + k["alg_bytes"]}
+    else:
+        kern[name] = dict(k)
+
+
 def lookahead_kernels(ctx, r):
     stats = r["stats"]
     gpu_ms = sum(s["ms"] for s in stats.values())
@@ -352,21 +373,26 @@ def lookahead_kernels(ctx, r):
         kern[name] = stats[ph]
     wide_min = ctx.get_tuning("hull_wide_min")
     half_cap = ctx.get_tuning("hull_half_cap") != 0
+    ldsb8 = ctx.get_tuning("hull_ldsb8_vcap")
     for cls, k in r["kernels"].items():
-        kern[kernel_name(cls, wide_min, half_cap)] = k
-    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:6]:
+        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8), k)
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"]):
+        if k["launches"] == 0:
+            continue
         log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
             f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
     return kern
 
 
-def gpu_learned(ctx, meshes, plain, top_k, reps):
-    """The learned proposal at each tau against the plain search at the same tau."""
+def gpu_learned(ctx, meshes, plain, top_ks, reps):
+    """The learned proposal at each tau against the plain search at the same
+    tau; top_ks maps tau to the number of level-0 candidates kept per part."""
     from cuacd_amd import proposal as PR
     net = PR.ProposalNet.load_default(ctx)
     out = {}
     try:
         for tau, pl in plain.items():
+            top_k = top_ks[tau]
             kw = dict(PARAMS, threshold=tau)
             ctx.batch_lookahead_decompose(meshes, proposal=net, top_k=top_k, **kw)  # warm
             times = []
@@ -548,6 +574,33 @@ def gpu_hausdorff(ctx, meshes, replays, n_samples=4096):
                                                      "triangle, 70 flops each); the BVH does less"}}}
 
 
+def gpu_score_graph(ctx, meshes, mesh_of, pl, tau, steps, want_score, want_detail):
+    """The same batch as a score plan: every stage sized on the device and the
+    whole level replayed as one captured HIP graph; a step is one replay plus
+    the readback and the host's cost formula, checked against the eager run."""
+    try:
+        plan = ctx.score_plan(meshes, mesh_of, pl, threshold=tau)
+    except RuntimeError as e:
+        log(f"score graph: not planned ({e})")
+        return {"planned": False, "reason": str(e)}
+    try:
+        plan.run()  # warm
+        times = []
+        same = True
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            sc, dt = plan.run(return_detail=True)
+            times.append(time.perf_counter() - t0)
+            same = same and sc.tobytes() == want_score.tobytes() and dt.tobytes() == want_detail.tobytes()
+    finally:
+        plan.close()
+    ms = 1e3 * float(np.mean(times))
+    log(f"score graph: {ms:.1f} ms per replay ({len(pl) / (ms * 1e-3):.0f} cuts/s), identical to eager: {same}")
+    return {"planned": True, "value": len(pl) / (ms * 1e-3), "unit": "cuts/s", "ms_per_step": ms,
+            "ms_per_step_runs": [1e3 * x for x in times], "identical_to_eager": bool(same),
+            "host_syncs_per_step": 1}
+
+
 def gpu_score(ctx, meshes, planes, steps, tau=0.05):
     import cuacd_amd
     mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, p in enumerate(planes)])
@@ -567,17 +620,21 @@ def gpu_score(ctx, meshes, planes, steps, tau=0.05):
     kern = {"la_cut_kernel": {"ms": st[0], "launches": 1, "alg_bytes": st[3]}}
     wide_min = ctx.get_tuning("hull_wide_min")
     half_cap = ctx.get_tuning("hull_half_cap") != 0
+    ldsb8 = ctx.get_tuning("hull_ldsb8_vcap")
     for i, cls in enumerate(cuacd_amd.LOOKAHEAD_KERNELS):
         k_ms, la, by = st[6 + 3 * i: 9 + 3 * i]
-        kern[kernel_name(cls, wide_min, half_cap)] = {"ms": k_ms, "launches": int(la), "alg_bytes": by}
-    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"])[:5]:
+        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8), {"ms": k_ms, "launches": int(la), "alg_bytes": by})
+    for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"]):
+        if k["launches"] == 0:
+            continue
         log(f"  kernel {name:22s} {k['ms']:9.1f} ms  {k['launches']:5d} launches  "
             f"{k['alg_bytes'] / max(k['ms'], 1e-9) / 1e6:7.1f} GB/s algorithmic")
     tris = np.array([len(t) for _, t in meshes])
     roof = _roofline(kern)
     roof["limiter"] = LA_LIMITER
+    graph = gpu_score_graph(ctx, meshes, mesh_of, pl, tau, steps, score, detail)
     return {"metric": "candidate cuts scored/sec (plain, batched exact scorer)", "value": len(pl) / (ms * 1e-3),
-            "unit": "cuts/s", "ms_per_step": ms, "steps": steps,
+            "unit": "cuts/s", "ms_per_step": ms, "steps": steps, "graph": graph,
             "config": {"workload": f"batched cut scoring: {len(meshes)} SDF meshes (marching tetrahedra 64^3, "
                                    f"500-60k tris, extent 2, seed 3), 30 candidate planes each (16 concave-edge "
                                    f"planes + axis cuts), tau 0.05", "cuts": int(len(pl)),
@@ -599,7 +656,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="tune the hull launch shapes on the score workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run the known-size device copy (HBM counter calibration) and exit")
-    ap.add_argument("--top-k", type=int, default=16)
+    ap.add_argument("--top-k", default="16,24",
+                    help="level-0 candidates kept per part by the learned proposal at tau 0.05 and 0.03 "
+                         "(one value for both, or two; 0 skips the learned step)")
     ap.add_argument("--learned-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -661,6 +720,9 @@ def main():
         kern = lookahead_kernels(ctx, plain)
         roof = _roofline(kern)
         roof["limiter"] = LA_LIMITER
+        if roof["kernel"] in HULL_VALU_BUSY:
+            roof["valu_busy"] = {"frac": HULL_VALU_BUSY[roof["kernel"]], "measured": "rocprofv3 SQ counters",
+                                 "profile": "profiles/lookahead_64sdf_hull_tier_counters.md"}
         cb = None
         if "lookahead_raw" in cpu:
             cb = finish_cpu_lookahead(cpu["lookahead_raw"], meshes, plain["iters"])
@@ -688,9 +750,11 @@ def main():
             "roofline": roof,
             "cpu_baseline": cb,
         }
-        if want["learned"] and args.top_k > 0:
+        ks = [int(x) for x in str(args.top_k).split(",")]
+        top_ks = {0.05: ks[0], 0.03: ks[-1]}
+        if want["learned"] and min(ks) > 0:
             plains = {0.05: plain, 0.03: gpu_lookahead(ctx, meshes, 1, 0, tau=0.03)}
-            objs["learned"] = gpu_learned(ctx, meshes, plains, args.top_k, args.learned_reps)
+            objs["learned"] = gpu_learned(ctx, meshes, plains, top_ks, args.learned_reps)
     if want["hull"]:
         o = gpu_hull(ctx, clouds, 50)
         o["cpu_baseline"] = cpu.get("hull")

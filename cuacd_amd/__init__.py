@@ -38,11 +38,12 @@ LOOKAHEAD_SECTIONS = ("setup", "refresh_select", "concave_planes", "level0", "de
 # hull kernel classes timed per launch (Context.last_lookahead_kernels)
 LOOKAHEAD_KERNELS = ("hull_w64", "hull_w128", "hull_w256", "hull_w512", "hull_w1024", "hull_b2048", "hull_b4096",
                      "hull_b8192", "hull_b16384", "kdop_prefilter", "hull_w512_retry", "hull_w1024_retry",
-                     "hull_b2048_retry")
+                     "hull_b2048_retry", "hull_b4096_retry")
 
 # launch-shape knobs of the hull tiers (gpu_set_tuning keys)
 TUNING_KEYS = {"hull_wide_min": 0, "hull_vcap_eighths": 1, "hull_half_cap": 2, "kdop_min_points": 3,
-               "hull_chunk_above": 4, "hull_multi": 5, "mlp_fused": 6}
+               "hull_chunk_above": 4, "hull_multi": 5, "mlp_fused": 6, "hull_ldsv8_vcap": 7, "hull_ldsb8_vcap": 8,
+               "cut_lds_verts": 9, "hull_lpt": 10}
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 
@@ -366,6 +367,12 @@ class Context:
                                  vp.ctypes.data, vn.ctypes.data, sz.ctypes.data, st.ctypes.data)
         return vp, vn, sz, st
 
+    def score_plan(self, meshes, mesh_of, planes, threshold=0.05):
+        """The batch of batch_score_cuts as a replayable plan (ScorePlan): the
+        meshes and planes stay on the device and one run() is a captured HIP
+        graph of every stage, sized on the device, with the same scores."""
+        return ScorePlan(self, meshes, mesh_of, planes, threshold)
+
     def batch_score_cuts(self, meshes, mesh_of, planes, threshold=0.05, return_detail=False):
         """Exact score of many candidate cuts in one batch.
 
@@ -530,3 +537,54 @@ def _part_tuple(r):
     hv = np.frombuffer(hvb, dtype=np.float32).reshape(hnv, 3).copy()
     ht = np.frombuffer(htb, dtype=np.int32).reshape(hnt, 3).copy()
     return (v, t, hv, ht) + tuple(r[8:])
+
+
+def _pack_meshes(meshes):
+    vs = [_f32(v).reshape(-1, 3) for v, _ in meshes]
+    ts = [_i32(t).reshape(-1, 3) for _, t in meshes]
+    voff = np.zeros(len(meshes) + 1, np.int32)
+    toff = np.zeros(len(meshes) + 1, np.int32)
+    voff[1:] = np.cumsum([len(x) for x in vs])
+    toff[1:] = np.cumsum([len(x) for x in ts])
+    allv = np.ascontiguousarray(np.concatenate(vs)) if vs else np.zeros((0, 3), np.float32)
+    allt = np.ascontiguousarray(np.concatenate(ts)) if ts else np.zeros((0, 3), np.int32)
+    return allv, voff, allt, toff
+
+
+class ScorePlan:
+    """A fixed batch of cut scores replayed as one HIP graph
+    (gpu_score_plan_create / _run / _destroy). run() returns the float32
+    scores, and with return_detail the [n, 10] detail of batch_score_cuts."""
+
+    def __init__(self, ctx, meshes, mesh_of, planes, threshold=0.05):
+        mesh_of = _i32(mesh_of).reshape(-1)
+        pl = _f32(planes).reshape(-1, 4)
+        self.n = len(mesh_of)
+        if len(pl) != self.n:
+            raise ValueError("mesh_of and planes must have the same length")
+        if self.n == 0:
+            raise ValueError("a score plan needs at least one cut")
+        if mesh_of.min() < 0 or mesh_of.max() >= len(meshes):
+            raise ValueError("mesh_of index out of range")
+        allv, voff, allt, toff = _pack_meshes(meshes)
+        self._h = _gpu.score_plan_create(allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
+                                         len(meshes), mesh_of.ctypes.data, pl.ctypes.data, self.n, float(threshold))
+
+    def run(self, return_detail=False):
+        if not self._h:
+            raise RuntimeError("score plan is closed")
+        score = np.zeros(self.n, np.float32)
+        detail = np.zeros((self.n, 10), np.float64)
+        _gpu.score_plan_run(self._h, score.ctypes.data, detail.ctypes.data)
+        return (score, detail) if return_detail else score
+
+    def close(self):
+        if self._h:
+            _gpu.score_plan_destroy(self._h)
+            self._h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -23,6 +23,9 @@ DEFAULT_SPACE = {
     "hull_half_cap": [0, 1],
     "kdop_min_points": [128, 256, 512],
     "hull_multi": [0, 1],
+    "hull_ldsv8_vcap": [448, 1024],
+    "hull_ldsb8_vcap": [0, 1024],
+    "hull_lpt": [0, 1],
 }
 
 CACHE_PATH = os.path.join(os.path.expanduser("~"), ".cache", "cuacd_amd", "tuning.json")

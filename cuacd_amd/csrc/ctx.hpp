@@ -18,6 +18,7 @@ enum KtId {
     KT_HULL_B4, KT_HULL_B8, KT_HULL_B16, KT_HULL_B32,              // multi-wave hulls, workspace in HBM
     KT_KDOP,
     KT_HULL_R8, KT_HULL_R16, KT_HULL_RB4,  // full-capacity retries of the half-capacity first passes
+    KT_HULL_RB8,                           // HBM retries of the 2049-4096-point LDS first pass
     KT_COUNT
 };
 struct KTimers {

@@ -9,6 +9,7 @@
 // cuda/test_hausdorff.cu:8-39) and the batch API with plans (no reference
 // counterpart) used for concavity scoring of many mesh/hull pairs.
 #include <stdlib.h>
+#include <algorithm>
 #include <vector>
 #include "ctx.hpp"
 #include "hausdorff.hpp"
@@ -98,9 +99,11 @@ __device__ __forceinline__ V3 bary(V3 a, V3 b, V3 c, uint32_t ha, uint32_t hb) {
 // Preparation, one workgroup per task: the target's BVH, the source's
 // sample layout (mode 0: per-triangle counts and their exclusive prefix;
 // mode 1: the cumulative area), the sample count; clears the result.
-__global__ __launch_bounds__(HD_T) void hd_prep_kernel(HdTask* tasks, int n_tasks, char* __restrict__ ws_base) {
+__global__ __launch_bounds__(HD_T) void hd_prep_kernel(HdTask* tasks, int n_tasks, const int* dcount,
+                                                       char* __restrict__ ws_base) {
     __shared__ HdLDS s;
-    if ((int)blockIdx.x >= n_tasks) return;
+    // tasks[0, n_tasks), or the first min(*dcount, n_tasks) when the count is on the device
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n_tasks) : n_tasks)) return;
     HdTask& tk = tasks[blockIdx.x];
     const int tid = threadIdx.x, wv = tid >> 6;
     if (tk.snt <= 0 || tk.tnt <= 0) {
@@ -163,11 +166,21 @@ __global__ __launch_bounds__(HD_T) void hd_prep_kernel(HdTask* tasks, int n_task
 // Queries: `chunks` workgroups per task, each answering a contiguous share
 // of the task's samples; the maximum is merged with an atomic max on the
 // result's bits (non-negative floats order as unsigned integers).
-__global__ __launch_bounds__(HD_T) void hd_query_kernel(const HdTask* __restrict__ tasks, int n_tasks, int chunks,
-                                                        char* __restrict__ ws_base) {
+__global__ __launch_bounds__(HD_T) void hd_query_kernel(const HdTask* __restrict__ tasks, int n_tasks,
+                                                        const int* dcount, int chunks, char* __restrict__ ws_base) {
     __shared__ int qoff[HD_MAX_LEVELS], qcnt[HD_MAX_LEVELS];
     __shared__ float qred[HD_T / 64];
-    const int ti = (int)blockIdx.x / chunks, ch = (int)blockIdx.x % chunks;
+    // XCD-aware order: the dispatcher hands consecutive workgroups to the 8
+    // XCDs in turn, so slot L (task L / chunks, chunk L % chunks) is renumbered
+    // to give each XCD a contiguous range of slots - the chunks of a task (and
+    // neighbouring tasks) share one XCD's L2 instead of fetching the task's
+    // BVH and triangles into several. A bijection of [0, nb), nb the slots of
+    // the tasks present (all of the grid unless the count is on the device).
+    const unsigned nb = dcount ? (unsigned)min(*dcount, n_tasks) * (unsigned)chunks : gridDim.x, b = blockIdx.x;
+    if (b >= nb) return;
+    const unsigned q8 = nb / 8u, r8 = nb % 8u, x8 = b % 8u, j8 = b / 8u;
+    const unsigned slot = (x8 < r8 ? x8 * (q8 + 1u) : r8 * (q8 + 1u) + (x8 - r8) * q8) + j8;
+    const int ti = (int)(slot / (unsigned)chunks), ch = (int)(slot % (unsigned)chunks);
     if (ti >= n_tasks) return;
     const HdTask tk = tasks[ti];
     const int tid = threadIdx.x, wv = tid >> 6;
@@ -251,13 +264,15 @@ __global__ __launch_bounds__(HD_T) void hd_query_kernel(const HdTask* __restrict
 
 int launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(hd_prep_kernel, dim3(n), dim3(HD_T), 0, ctx->stream, (HdTask*)d_tasks, n, d_ws);
+    hipLaunchKernelGGL(hd_prep_kernel, dim3(n), dim3(HD_T), 0, ctx->stream, (HdTask*)d_tasks, n, (const int*)nullptr,
+                       d_ws);
     HIP_TRY(hipGetLastError());
     // about 4096 query workgroups per launch, at least one per task
     int chunks = (4096 + n - 1) / n;
     if (chunks > 64) chunks = 64;
     if (chunks < 1) chunks = 1;
-    hipLaunchKernelGGL(hd_query_kernel, dim3((unsigned)(n * chunks)), dim3(HD_T), 0, ctx->stream, d_tasks, n, chunks,
+    hipLaunchKernelGGL(hd_query_kernel, dim3((unsigned)(n * chunks)), dim3(HD_T), 0, ctx->stream, d_tasks, n,
+                       (const int*)nullptr, chunks,
                        d_ws);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -268,6 +283,19 @@ int launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
 // Internal entry for other kernels' hosts (lookahead): run a device task table.
 int hd_launch_tasks(gpu_ctx_t ctx, const HdTask* d_tasks, int n, char* d_ws) {
     return launch_tasks(ctx, d_tasks, n, d_ws);
+}
+
+// The same with the task count on the device: at most cap tasks, *dcount of
+// them present; `chunks` query workgroups per task.
+int hd_launch_tasks_dev(gpu_ctx_t ctx, const HdTask* d_tasks, int cap, const int* dcount, int chunks, char* d_ws) {
+    if (cap <= 0) return 0;
+    chunks = std::max(1, std::min(64, chunks));
+    hipLaunchKernelGGL(hd_prep_kernel, dim3(cap), dim3(HD_T), 0, ctx->stream, (HdTask*)d_tasks, cap, dcount, d_ws);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(hd_query_kernel, dim3((unsigned)(cap * chunks)), dim3(HD_T), 0, ctx->stream, d_tasks, cap,
+                       dcount, chunks, d_ws);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 extern "C" {

@@ -167,10 +167,12 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
 }
 
 template <int PPL>
-__global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int* ids, int n, DPool* pool, int* err) {
+__global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int* ids, int n, const int* dcount,
+                                                   DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
-    if ((int)blockIdx.x >= n) return;
+    // ids[0, n), or the first min(*dcount, n) when the count is on the device
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
     LaHullTask& t = tasks[ids[blockIdx.x]];
     const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
     la_hull_item<PPL, AS_LDS, 1>(t, w, &sb, pool, err);
@@ -202,11 +204,12 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
 // footprint, so several hulls share a CU); a set whose hull needs more
 // vertices is queued on `retry` for the full-capacity kernel.
 template <int PPL, int NW>
-__global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n, int vcap,
-                                                       int* retry, int multi, DPool* pool, int* err) {
+__global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const int* ids, int n,
+                                                       const int* dcount, int vcap, int* retry, int multi,
+                                                       DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
-    if ((int)blockIdx.x >= n) return;
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
     const int id = ids[blockIdx.x];
     LaHullTask& t = tasks[id];
     constexpr int CAP = 64 * NW * PPL;
@@ -243,15 +246,15 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
 constexpr int KD_T = 256;
 constexpr int KD_NW = KD_T / 64;
 
-__global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const int* ids, int n, int2* res,
-                                                       DPool* pool, int* err) {
+__global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const int* ids, int n,
+                                                       const int* dcount, int2* res, DPool* pool, int* err) {
     __shared__ HullBlockLDS sb;
     __shared__ __align__(16) char lds[20480];
     __shared__ float ep[2 * KDOP_AXES_N][3];
     __shared__ double4 dpl[2 * 128 + 8];
     __shared__ unsigned long long kred[KD_NW][16];
     __shared__ int s_cnt[KD_NW + 2];
-    if ((int)blockIdx.x >= n) return;
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
     LaHullTask& t = tasks[ids[blockIdx.x]];
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const float* P = t.pts;
@@ -508,15 +511,29 @@ __device__ inline void la_block_bbox(const float* v, int nv, float* lo, float* h
     (void)s;
 }
 
-__global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, DPool* pool, int* err) {
+__global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, DPool* pool, int* err, int lds_nv) {
     __shared__ PcLDS s;
     __shared__ float scr[PC_T / 64][6];
-    if ((int)blockIdx.x >= n) return;
-    LaCutTask& T = tasks[blockIdx.x];
-    PcResult r = plane_cut_block(T.v, T.t, T.nv, T.nt, T.plane[0], T.plane[1], T.plane[2], T.plane[3], pool, err, &s);
+    extern __shared__ __align__(16) unsigned char pc_dyn[];
+    // the candidates of one part are consecutive tasks: groups of 16 per XCD
+    const int slot = (int)xcd_group_slot(blockIdx.x, gridDim.x, 16u);
+    if (slot >= n) return;
+    LaCutTask& T = tasks[slot];
+    PcResult r = T.nv <= lds_nv ? plane_cut_block_lds(T.v, T.t, T.nv, T.nt, T.plane[0], T.plane[1], T.plane[2],
+                                                      T.plane[3], pool, err, &s, pc_dyn)
+                                : plane_cut_block(T.v, T.t, T.nv, T.nt, T.plane[0], T.plane[1], T.plane[2],
+                                                  T.plane[3], pool, err, &s);
     float plo[3], phi[3], nlo[3], nhi[3];
     const bool ok = r.status == 0 && r.pos.v && r.neg.v;
-    if (ok) {
+    if (ok && r.has_box) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            plo[j] = s.boxr[j];
+            phi[j] = s.boxr[3 + j];
+            nlo[j] = s.boxr[6 + j];
+            nhi[j] = s.boxr[9 + j];
+        }
+    } else if (ok) {
         la_block_bbox(r.pos.v, r.pos.nv, plo, phi, &s, scr);
         la_block_bbox(r.neg.v, r.neg.nv, nlo, nhi, &s, scr);
     }
@@ -546,32 +563,40 @@ __global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, D
 // both sides (in index order); the crossing points - one per undirected edge
 // with strictly opposite signs, in order of first occurrence over (triangle,
 // edge) - come from that short list alone.
-__global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int n, DPool* pool, int* err) {
+__global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int n, DPool* pool, int* err,
+                                                        int lds_nv) {
     __shared__ PcLDS s;
+    extern __shared__ __align__(16) unsigned char pc_dyn[];
     __shared__ double dsum[2][PC_T / 64];
     __shared__ int wcnt[PC_T / 64];
-    if ((int)blockIdx.x >= n) return;
-    LaScoreTask& T = tasks[blockIdx.x];
+    // the candidates of one item's part are consecutive tasks: groups of 8 per XCD
+    const int slot = (int)xcd_group_slot(blockIdx.x, gridDim.x, 8u);
+    if (slot >= n) return;
+    LaScoreTask& T = tasks[slot];
     const int tid = threadIdx.x, wv = tid >> 6;
     const float* V = T.v;
     const int* Tr = T.t;
     const int nv = T.nv, nt = T.nt;
     const float pa = T.plane[0], pb = T.plane[1], pc = T.plane[2], pd = T.plane[3];
-    // A: signs (int8), side flags (two byte arrays), crossing-triangle list
-    const unsigned long long szA = 3ull * al16(nv) + 4ull * nt + 64;
-    char* A = (char*)pc_alloc(pool, szA, &s, 1, err, 1);
+    // One flag byte per vertex - bits 0-1 sign code (1 positive, 2 negative, 0
+    // on the plane), bit 2 in the positive point set, bit 3 in the negative
+    // one - in LDS when the mesh fits the launch's dynamic LDS, else in heap
+    // scratch; then the crossing-triangle list (heap).
+    const bool flags_in_lds = nv <= lds_nv;
+    const unsigned long long fb = flags_in_lds ? 0ull : al16(nv);
+    char* A = (char*)pc_alloc(pool, fb + 4ull * nt + 64, &s, 1, err, 1);
     if (!A) return;
-    int8_t* sg = (int8_t*)A;
-    uint8_t* m1 = (uint8_t*)(A + al16(nv));
-    uint8_t* m2 = m1 + al16(nv);
-    int* xl = (int*)(m2 + al16(nv));
+    uint8_t* fl = flags_in_lds ? pc_dyn : (uint8_t*)A;
+    int* xl = (int*)(A + fb);
     for (int i = tid; i < nv; i += PC_T) {
         const float d = pc_side(pa, pb, pc, pd, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
-        sg[i] = d > PC_EPS ? 1 : (d < -PC_EPS ? -1 : 0);
-        m1[i] = 0;
-        m2[i] = 0;
+        fl[i] = d > PC_EPS ? 1 : (d < -PC_EPS ? 2 : 0);
     }
     __syncthreads();
+    auto sgn = [&](int i) -> int {
+        const int c = fl[i] & 3;
+        return c == 1 ? 1 : (c == 2 ? -1 : 0);
+    };
     // divergence origin: the point of the plane nearest the coordinate origin
     const double n2 = (double)pa * pa + (double)pb * pb + (double)pc * pc;
     const double ox = -(double)pd * pa / n2, oy = -(double)pd * pb / n2, oz = -(double)pd * pc / n2;
@@ -593,8 +618,8 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         o[2] = az + t * (bz - az);
     };
     auto mark = [&](int i, int side) {
-        if (side & 1) m1[i] = 1;
-        if (side & 2) m2[i] = 1;
+        const unsigned bits = (unsigned)side << 2;
+        if ((fl[i] & bits) != bits) atomicOr((unsigned*)(fl + (i & ~3)), bits << (8 * (i & 3)));
     };
     double vp = 0.0, vn = 0.0;
     int cntp = 0, cntn = 0;
@@ -604,7 +629,7 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         bool crossing = false;
         if (k < nt) {
             const int i0 = Tr[3 * k], i1 = Tr[3 * k + 1], i2 = Tr[3 * k + 2];
-            const int s0 = sg[i0], s1 = sg[i1], s2 = sg[i2];
+            const int s0 = sgn(i0), s1 = sgn(i1), s2 = sgn(i2);
             const float* p0 = V + 3 * i0;
             const float* p1 = V + 3 * i1;
             const float* p2 = V + 3 * i2;
@@ -696,7 +721,10 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
     }
     __syncthreads();
     // instances of the strictly crossing edges over the listed triangles
-    auto strict = [&](int a, int b) { return (sg[a] > 0 && sg[b] < 0) || (sg[a] < 0 && sg[b] > 0); };
+    auto strict = [&](int a, int b) {
+        const int sa = sgn(a), sb = sgn(b);
+        return (sa > 0 && sb < 0) || (sa < 0 && sb > 0);
+    };
     auto ninst = [&](int x) {
         const int k = xl[x];
         const int a = Tr[3 * k], b = Tr[3 * k + 1], c = Tr[3 * k + 2];
@@ -745,7 +773,10 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
     const int n_new = pc_block_scan(nx, xnew, &s, owned_new);
     // point sets: marked original vertices in index order, then the new points
     int c1 = 0, c2 = 0;
-    for (int i = tid; i < nv; i += PC_T) { c1 += m1[i]; c2 += m2[i]; }
+    for (int i = tid; i < nv; i += PC_T) {
+        c1 += (fl[i] >> 2) & 1;
+        c2 += (fl[i] >> 3) & 1;
+    }
     c1 = wave_sum_i(c1);
     c2 = wave_sum_i(c2);
     if (lane_id() == 0) { s.cnt[wv] = c1; s.cnt[4 + wv] = c2; }
@@ -763,7 +794,7 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
             int bp = 0, bn = 0;
             for (int b0 = 0; b0 < nv; b0 += PC_T) {
                 const int i = b0 + tid;
-                const int f1 = i < nv ? m1[i] : 0, f2 = i < nv ? m2[i] : 0;
+                const int f1 = i < nv ? (fl[i] >> 2) & 1 : 0, f2 = i < nv ? (fl[i] >> 3) & 1 : 0;
                 const uint64_t mp = ballot(f1), mn_ = ballot(f2);
                 if (lane_id() == 0) { s.cnt[wv] = __popcll(mp); s.cnt[4 + wv] = __popcll(mn_); }
                 __syncthreads();
@@ -1196,7 +1227,7 @@ int ensure_kdop_axes(gpu_ctx_t ctx) {
     return 0;
 }
 
-using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
+using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 
 // Launch-shape knobs of the hull tiers (they choose kernels and capacities,
 // never results: every tier computes the same canonical hull). Process-wide;
@@ -1217,7 +1248,28 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, DPool*, int*);
 //   6 mlp_fused: the proposal network's inference runs its point layers and
 //     pooling as one kernel per candidate instead of layer-by-layer GEMMs
 //     (CUACD_MLP_FUSED, 0/1; default 1; bit-identical scores)
-constexpr int LA_TUNE_N = 7;
+//   7 ldsv8_vcap: first-pass vertex capacity of the eight-wave LDS tier
+//     (1025-2048 points; CUACD_HULL_LDSV8_VCAP, 64..1024, default 1024).
+//     At 448 or less with multi-apex rounds the workspace fits two
+//     workgroups per CU; hulls with more corners are redone in the HBM tier
+//     (the same canonical hull). On SDF surface parts a quarter of the sets
+//     have more than 448 corners, and their redo costs more than the second
+//     workgroup gains, hence the default
+//   8 ldsb8_vcap: vertex capacity of an LDS first pass for the 2049-4096-point
+//     sets (CUACD_HULL_LDSB8_VCAP, 0 = none: straight to the HBM tier; else
+//     64..1024; default 1024); hulls with more corners are redone in the HBM
+//     tier
+//   9 cut_lds_verts: plane cuts of meshes with up to this many vertices keep
+//     their per-vertex state in LDS and write no per-triangle scratch
+//     (CUACD_CUT_LDS_VERTS, 0 = every cut on the heap-scratch path, else
+//     256..110592; default 98304); bit-identical halves either way
+//   10 lpt: each hull tier's sets are launched largest first (CUACD_HULL_LPT,
+//     0/1; default 0): the longest insertion chains start on the first
+//     workgroups and the short ones fill in behind them. On the 64-mesh
+//     search step it measured slower (2.40 s against 2.31 s): the tiers
+//     already overlap on side streams, and task order keeps a mesh's sets
+//     together
+constexpr int LA_TUNE_N = 11;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1228,15 +1280,20 @@ int tune_clamp(int key, int v) {
         case 2: return v != 0;
         case 3: return std::max(4, v);
         case 4: return std::min(HULL_NP_MAX, std::max(1024, v));
+        case 7: return std::min(1024, std::max(64, v / 32 * 32));
+        case 8: return v <= 0 ? 0 : std::min(1024, std::max(64, v / 32 * 32));
+        case 9: return v <= 0 ? 0 : std::min(PC_LDS_VERTS_MAX, std::max(256, v));
         default: return v != 0;
     }
 }
 
 void tune_init() {
     if (g_tune_init) return;
-    const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN", "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP", "CUACD_KDOP_MIN",
-                                  "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI", "CUACD_MLP_FUSED"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1};
+    const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN",   "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP",
+                                  "CUACD_KDOP_MIN",        "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI",
+                                  "CUACD_MLP_FUSED",       "CUACD_HULL_LDSV8_VCAP",  "CUACD_HULL_LDSB8_VCAP",
+                                  "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1248,6 +1305,8 @@ int hull_wide_min() { tune_init(); return g_tune[0]; }
 int hull_vcap_eighths() { tune_init(); return g_tune[1]; }
 int hull_half_cap() { tune_init(); return g_tune[2]; }
 int hull_multi() { tune_init(); return g_tune[5]; }
+int hull_ldsv8_vcap() { tune_init(); return g_tune[7]; }
+int hull_ldsb8_vcap() { tune_init(); return g_tune[8]; }
 
 // One hull call: its size tiers are independent, so they run on separate
 // streams - the latency-bound large-set tiers (a few hulls of thousands of
@@ -1262,6 +1321,10 @@ struct HullCall {
     std::vector<void*> tmp;
     bool used[3] = {false, false, false};
     bool joined = false;
+    // a caller's arena instead of stream-ordered allocations (a captured
+    // launch sequence must not allocate): bump allocation, 256-byte aligned
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     explicit HullCall(gpu_ctx_t c) : ctx(c) {}
     ~HullCall() {
         if (!joined) (void)join();
@@ -1280,6 +1343,13 @@ struct HullCall {
         return 0;
     }
     template <class T> T* alloc(size_t n) {
+        if (arena) {
+            const size_t b = (n * sizeof(T) + 255) / 256 * 256;
+            if (arena_used + b > arena_cap) return nullptr;
+            T* p = (T*)(arena + arena_used);
+            arena_used += b;
+            return p;
+        }
         T* p = dev_tmp<T>(ctx, n);
         if (p) tmp.push_back(p);
         return p;
@@ -1303,8 +1373,24 @@ int retry_grid(gpu_ctx_t ctx, size_t cap) {
     return (int)std::min(cap, (size_t)std::max(1, ctx->n_cu));
 }
 
+// Zero n ints (retry counters) with a kernel: such clears may sit inside a
+// captured graph, where memset nodes are avoided.
+__global__ void la_zero_ints_kernel(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int zero_ints(gpu_ctx_t ctx, int* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(la_zero_ints_kernel, dim3((unsigned)std::max(1, std::min(64, (n + 255) / 256))), dim3(256), 0,
+                       s, p, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// One wave tier (sets of up to 64 * ppl points). d_ids holds nids task ids,
+// or - with dcnt given - up to nids of them, *dcnt on the device (the grid is
+// sized for nids and the kernels read the count).
 int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int nids, int ppl,
-                    double bytes) {
+                    double bytes, const int* dcnt = nullptr) {
     gpu_ctx_t ctx = hc.ctx;
     if (nids <= 0) return 0;
     const int kid = ppl == 1 ? KT_HULL_W1 : ppl == 2 ? KT_HULL_W2 : ppl == 4 ? KT_HULL_W4 : ppl == 8 ? KT_HULL_W8
@@ -1313,7 +1399,7 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
     if (wm > 0 && ppl >= 8 && 64 * ppl > wm && hull_half_cap()) {
         // first pass with a reduced vertex capacity (several hulls per CU),
         // then the sets whose hulls did not fit, at full capacity
-        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, int, DPool*, int*);
+        using CapFn = void (*)(LaHullTask*, const int*, int, const int*, int, int*, int, DPool*, int*);
         using FullFn = void (*)(LaHullTask*, const int*, int, const int*, int, DPool*, int*);
         CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>;
         FullFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
@@ -1330,12 +1416,12 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
         int* d_retry = hc.alloc<int>((size_t)nids + 1);
         REQUIRE(d_retry, "lookahead: id list allocation failed");
-        HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
+        if (int rc = zero_ints(ctx, d_retry, 1, ctx->stream)) return rc;
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
         hipLaunchKernelGGL(kc, dim3((unsigned)(size_t)nids), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
-                           (int)(size_t)nids, vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
+                           (int)(size_t)nids, dcnt, vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         hipEvent_t t1 = kt_begin(ctx, s);
@@ -1357,7 +1443,7 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
         hipLaunchKernelGGL(k, dim3((unsigned)(size_t)nids), dim3(64 * LA_LDS4_NW), lds, s, d_tasks, d_ids,
-                           (int)(size_t)nids, (const int*)nullptr, multi, ctx->d_pool, ctx->d_err);
+                           (int)(size_t)nids, dcnt, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         return 0;
@@ -1370,7 +1456,7 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
     hipStream_t s;
     if (int rc = hc.fork(sk, &s)) return rc;
     hipEvent_t t0 = kt_begin(ctx, s);
-    hipLaunchKernelGGL(k, dim3((unsigned)(size_t)nids), dim3(64), lds, s, d_tasks, d_ids, (int)(size_t)nids,
+    hipLaunchKernelGGL(k, dim3((unsigned)(size_t)nids), dim3(64), lds, s, d_tasks, d_ids, (int)(size_t)nids, dcnt,
                        ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
     kt_end(ctx, kid, t0, bytes, s);
@@ -1380,22 +1466,29 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
 using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, DPool*,
                          int*);
 
+// One block tier (sets above 1024 points, at most maxn). With dcnt given,
+// d_ids holds up to n ids, *dcnt on the device, and maxn is the tier's bound.
 int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int n, int maxn,
-                     double bytes) {
+                     double bytes, const int* dcnt = nullptr) {
     gpu_ctx_t ctx = hc.ctx;
     if (n <= 0) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
-    if (maxn <= 4 * T && hull_half_cap()) {
-        // first pass with the workspace in LDS (1024 hull vertices), then the
-        // sets whose hulls did not fit, with the workspace in HBM
-        using CapFn = void (*)(LaHullTask*, const int*, int, int, int*, int, DPool*, int*);
-        CapFn kc = la_hull_ldsv<4, LA_BLOCK_NW>;
+    const bool lds4 = maxn <= 4 * T && hull_half_cap();
+    const bool lds8 = !lds4 && maxn <= 8 * T && hull_half_cap() && hull_ldsb8_vcap() > 0;
+    if (lds4 || lds8) {
+        // first pass with the workspace in LDS (a bounded number of hull
+        // vertices), then the sets whose hulls did not fit, with the workspace
+        // in HBM
+        using CapFn = void (*)(LaHullTask*, const int*, int, const int*, int, int*, int, DPool*, int*);
+        CapFn kc = lds4 ? la_hull_ldsv<4, LA_BLOCK_NW> : la_hull_ldsv<8, LA_BLOCK_NW>;
+        BlockFn kr = lds4 ? la_hull_block<4> : la_hull_block<8>;
         const int multi = hull_multi();
-        const unsigned ldc = hull_ws_bytes<AS_LDS>(1024, 4 * T, multi != 0);
+        const int vc = lds4 ? hull_ldsv8_vcap() : hull_ldsb8_vcap();
+        const unsigned ldc = hull_ws_bytes<AS_LDS>(vc, (lds4 ? 4 : 8) * T, multi != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         int* d_retry = hc.alloc<int>((size_t)n + 1);
         REQUIRE(d_retry, "lookahead: id list allocation failed");
-        HIP_TRY(hipMemsetAsync(d_retry, 0, sizeof(int), ctx->stream));
+        if (int rc = zero_ints(ctx, d_retry, 1, ctx->stream)) return rc;
         const int np = (maxn + 63) / 64 * 64;
         const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(np, np) + 255ull) & ~255ull;
         const int g = retry_grid(ctx, (size_t)n);
@@ -1404,15 +1497,15 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kc, dim3((unsigned)n), dim3(T), ldc, s, d_tasks, d_ids, n, 1024, d_retry, multi, ctx->d_pool,
-                           ctx->d_err);
+        hipLaunchKernelGGL(kc, dim3((unsigned)n), dim3(T), ldc, s, d_tasks, d_ids, n, dcnt, vc, d_retry, multi,
+                           ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, KT_HULL_B4, t0, bytes, s);
+        kt_end(ctx, lds4 ? KT_HULL_B4 : KT_HULL_B8, t0, bytes, s);
         hipEvent_t t1 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(la_hull_block<4>, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_retry + 1, n,
-                           (const int*)d_retry, gws, stride, np, np, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kr, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_retry + 1, n, (const int*)d_retry, gws,
+                           stride, np, np, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
-        kt_end(ctx, KT_HULL_RB4, t1, 0.0, s);
+        kt_end(ctx, lds4 ? KT_HULL_RB4 : KT_HULL_RB8, t1, 0.0, s);
         return 0;
     }
     BlockFn k;
@@ -1424,6 +1517,20 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
     const int np = (maxn + 63) / 64 * 64;
     const int vc = np < 32704 ? np : 32704;
     const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull;
+    if (dcnt) {
+        // device count: one workspace per CU, the grid strides over the list
+        const int g = retry_grid(ctx, (size_t)n);
+        char* gws = hc.alloc<char>(stride * (size_t)g + 256);
+        REQUIRE(gws, "lookahead: hull workspace allocation failed");
+        hipStream_t s;
+        if (int rc = hc.fork(sk, &s)) return rc;
+        hipEvent_t t0 = kt_begin(ctx, s);
+        hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_ids, n, dcnt, gws, stride, vc, np,
+                           ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+        kt_end(ctx, kid, t0, bytes, s);
+        return 0;
+    }
     // bounded workspace: launch in slices of at most 2048 tasks
     const int slice = 2048;
     const int g = std::min(slice, n);
@@ -1449,7 +1556,16 @@ int ppl_class(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 5
 // and 513-1024 LDS tiers on one stream each, the small-set tiers last.
 // `pre`: the sets of the call that skip the k-DOP prefilter, launched while
 // it runs on the context's stream.
-int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn, bool pre) {
+int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pair<int, int>>& idn_in, bool pre) {
+    // largest sets first (a stable order: ties keep the task order)
+    tune_init();
+    std::vector<std::pair<int, int>> idn_sorted;
+    if (g_tune[10]) {
+        idn_sorted = idn_in;
+        std::stable_sort(idn_sorted.begin(), idn_sorted.end(),
+                         [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.second > b.second; });
+    }
+    const std::vector<std::pair<int, int>>& idn = g_tune[10] ? idn_sorted : idn_in;
     std::vector<int> cls[5];
     double cbytes[5] = {0, 0, 0, 0, 0};
     std::vector<int> b4, b8, b16, b32;
@@ -1625,7 +1741,7 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
         for (int i : big) kbytes += 12.0 * h_tasks[i].n;  // points read (the filtered copy is extra)
         hipEvent_t t0 = kt_begin(ctx);
         hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
-                           (int)big.size(), d_res, ctx->d_pool, ctx->d_err);
+                           (int)big.size(), (const int*)nullptr, d_res, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, KT_KDOP, t0, kbytes);
         std::vector<int2> res(big.size());
@@ -1645,16 +1761,126 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
     return hc.join();
 }
 
-int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n) {
+// ---- device-sized hull call -------------------------------------------------
+// Tier index of a set of m points (after the prefilter): 0-4 the wave tiers
+// of 64, 128, 256, 512, 1024 points, 5-8 the block tiers up to 2048, 4096,
+// 8192 and 16384 points - the classes of launch_by_size.
+__device__ __forceinline__ int la_tier_of(int m) {
+    if (m <= LA_WAVE_MAX) return m <= 64 ? 0 : m <= 128 ? 1 : m <= 256 ? 2 : m <= 512 ? 3 : 4;
+    return m <= 2048 ? 5 : m <= 4096 ? 6 : m <= 8192 ? 7 : 8;
+}
+
+// The sets above kmin points go to the prefilter's list (any order: every
+// set is hulled on its own).
+__global__ void la_hdev_prelist_kernel(const LaHullTask* tasks, int n, int kmin, int* counters, int* big) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (tasks[i].n > kmin) big[atomicAdd(&counters[LA_HDEV_BIG], 1)] = i;
+}
+
+// Every set with points into its tier's list: the small ones by their size,
+// the prefiltered ones by their filtered size (pruned sets and failed filters
+// drop out as in la_launch_hull); a set above `limit` is counted as oversized.
+__global__ void la_hdev_bin_kernel(const LaHullTask* tasks, int n, int kmin, int limit, const int* big,
+                                   const int2* res, int* counters, int* lists) {
+    const int stride = gridDim.x * blockDim.x, g = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = g; i < n; i += stride) {
+        const int m = tasks[i].n;
+        if (m <= 0 || m > kmin) continue;
+        const int t = la_tier_of(m);
+        lists[(size_t)t * n + atomicAdd(&counters[1 + t], 1)] = i;
+    }
+    const int nb = counters[LA_HDEV_BIG];
+    for (int j = g; j < nb; j += stride) {
+        const int2 r = res[j];
+        if (r.y || r.x < 0) continue;
+        if (r.x > limit) {
+            atomicAdd(&counters[LA_HDEV_OVER], 1);
+            continue;
+        }
+        const int t = la_tier_of(r.x);
+        lists[(size_t)t * n + atomicAdd(&counters[1 + t], 1)] = big[j];
+    }
+}
+
+size_t la_hull_dev_work_bytes(int n) {
+    const size_t nn = (size_t)std::max(n, 1);
+    return 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256 +
+           (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256;
+}
+
+size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n) {
+    // retry lists of the four tiers with a first pass, and one global
+    // workspace per CU for each block tier at its largest set size
+    const size_t nn = (size_t)std::max(n, 1), g = (size_t)retry_grid(ctx, nn);
+    size_t b = 4 * ((4 * (nn + 1) + 255) / 256 * 256);
+    for (int np = 2048; np <= HULL_NP_MAX; np *= 2) {
+        const int vc = np < 32704 ? np : 32704;
+        b += ((hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull) * g + 512;
+    }
+    return b;
+}
+
+int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(la_cut_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err);
+    if (int rc = ensure_kdop_axes(ctx)) return rc;
+    tune_init();
+    const size_t nn = (size_t)n;
+    int* counters = (int*)work;
+    int* big = (int*)(work + 256);
+    int2* res = (int2*)(work + 256 + (4 * nn + 255) / 256 * 256);
+    int* lists = (int*)((char*)res + (8 * nn + 255) / 256 * 256);
+    const int kmin = std::min(kdop_min_points(), LA_WAVE_MAX);
+    const unsigned gb = (unsigned)std::max(1, std::min(1024, (n + 255) / 256));
+    if (int rc = zero_ints(ctx, counters, LA_HDEV_NCOUNT, ctx->stream)) return rc;
+    hipLaunchKernelGGL(la_hdev_prelist_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks, n,
+                       kmin, counters, big);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)n), dim3(KD_T), 0, ctx->stream, d_tasks, (const int*)big, n,
+                       (const int*)(counters + LA_HDEV_BIG), res, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(la_hdev_bin_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks, n, kmin,
+                       g_tune[4], (const int*)big, (const int2*)res, counters, lists);
+    HIP_TRY(hipGetLastError());
+    HullCall hc(ctx);
+    hc.arena = arena;
+    hc.arena_cap = arena_bytes;
+    auto L = [&](int t) { return (const int*)(lists + (size_t)t * nn); };
+    auto C = [&](int t) { return (const int*)(counters + 1 + t); };
+    // the launch_by_size order and streams: block tiers (largest first) on
+    // side streams 0 and 1, the 1024-point wave tier on stream 2, the small
+    // wave tiers on the context's stream
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(8), n, HULL_NP_MAX, 0.0, C(8))) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(7), n, 8192, 0.0, C(7))) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(6), n, 4096, 0.0, C(6))) return rc;
+    if (int rc = launch_block_ids(hc, 1, d_tasks, L(5), n, 2048, 0.0, C(5))) return rc;
+    const int ppls[5] = {1, 2, 4, 8, 16};
+    if (int rc = launch_wave_ids(hc, 2, d_tasks, L(4), n, ppls[4], 0.0, C(4))) return rc;
+    for (int c = 3; c >= 0; --c)
+        if (int rc = launch_wave_ids(hc, -1, d_tasks, L(c), n, ppls[c], 0.0, C(c))) return rc;
+    return hc.join();
+}
+
+int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n, int max_nv) {
+    if (n <= 0) return 0;
+    int lds_nv;
+    size_t lds;
+    if (int rc = pc_lds_launch(ctx, (const void*)la_cut_kernel, max_nv, &lds_nv, &lds)) return rc;
+    hipLaunchKernelGGL(la_cut_kernel, dim3(n), dim3(PC_T), lds, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err,
+                       lds_nv);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n) {
+int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n, int max_nv) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(la_score_kernel, dim3(n), dim3(PC_T), 0, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err);
+    // one flag byte per vertex: cut_lds_verts() bounds it like the cut's state
+    const int lds_nv = std::min(std::max(max_nv, 0), cut_lds_verts());
+    const size_t lds = lds_nv > 0 ? ((size_t)lds_nv + 15) / 16 * 16 : 0;
+    if (lds > 0)
+        HIP_TRY(hipFuncSetAttribute((const void*)la_score_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL(la_score_kernel, dim3(n), dim3(PC_T), lds, ctx->stream, d_tasks, n, ctx->d_pool, ctx->d_err,
+                       lds_nv);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1699,6 +1925,21 @@ int la_launch_gather(gpu_ctx_t ctx, LaGatherTask* d_tasks, int n) {
 int mlp_fused_on() {
     tune_init();
     return g_tune[6];
+}
+
+int cut_lds_verts() {
+    tune_init();
+    return g_tune[9];
+}
+
+int pc_lds_launch(gpu_ctx_t ctx, const void* kernel, int max_nv, int* lds_nv, size_t* lds_bytes) {
+    (void)ctx;
+    const int cap = cut_lds_verts();
+    *lds_nv = std::min(std::max(max_nv, 0), cap);
+    *lds_bytes = *lds_nv > 0 ? (size_t)pc_lds_bytes(*lds_nv) : 0;
+    if (*lds_bytes > 0)
+        HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds_bytes));
+    return 0;
 }
 
 int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n) {

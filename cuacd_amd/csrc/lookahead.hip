@@ -588,7 +588,9 @@ struct Engine {
         REQUIRE(dt.alloc(ctx, h.size()) == 0, "lookahead: task allocation failed");
         HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaCutTask) * h.size(), hipMemcpyHostToDevice, ctx->stream));
         LA_TRY(tbeg());
-        LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size()));
+        int max_nv = 0;
+        for (const LaCutTask& c : h) max_nv = std::max(max_nv, c.nv);
+        LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size(), max_nv));
         LA_TRY(tend(PH_CUT, 0.0));
         HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1597,7 +1599,9 @@ struct Engine {
                     HIP_TRY(hipMemcpyAsync(dt.p, h.data(), sizeof(LaScoreTask) * h.size(), hipMemcpyHostToDevice,
                                            ctx->stream));
                     LA_TRY(tbeg());
-                    LA_TRY(la_launch_score(ctx, dt.p, (int)h.size()));
+                    int max_nv = 0;
+                    for (const LaScoreTask& t : h) max_nv = std::max(max_nv, t.nv);
+                    LA_TRY(la_launch_score(ctx, dt.p, (int)h.size(), max_nv));
                     LA_TRY(tend(PH_SCORE, 0.0));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaScoreTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));

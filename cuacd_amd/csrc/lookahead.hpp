@@ -139,8 +139,15 @@ struct LaSeg {
 };
 int la_launch_segcopy(gpu_ctx_t ctx, const LaSeg* d_segs, int n);
 int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks, int n);
-int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n);
-int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n);
+// LDS-resident plane cuts (plane_cut_block_lds) for meshes of up to
+// cut_lds_verts() vertices; pc_lds_launch sizes a launch whose largest mesh
+// has max_nv vertices (tasks above *lds_nv take the heap-scratch path) and
+// raises the kernel's dynamic-LDS limit to *lds_bytes.
+int cut_lds_verts();
+int pc_lds_launch(gpu_ctx_t ctx, const void* kernel, int max_nv, int* lds_nv, size_t* lds_bytes);
+// max_nv: the most vertices of any task's mesh (sizes the LDS-resident cut)
+int la_launch_cut(gpu_ctx_t ctx, LaCutTask* d_tasks, int n, int max_nv);
+int la_launch_score(gpu_ctx_t ctx, LaScoreTask* d_tasks, int n, int max_nv);
 int la_launch_components(gpu_ctx_t ctx, LaCompTask* d_tasks, int n);
 int la_launch_edges(gpu_ctx_t ctx, LaEdgeTask* d_tasks, int n, float cos_pos, float cos_neg, int pos_mode,
                     int neg_mode);
@@ -148,6 +155,25 @@ int la_launch_bbox(gpu_ctx_t ctx, LaBoxTask* d_tasks, int n);
 // heap_free every non-null pointer of the list (device array)
 int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n);
 
+// Device-sized hull call: the hulls of tasks[0, n) (tasks with n <= 0 are
+// skipped) with the k-DOP prefilter, the size classification and every tier's
+// task list on the device - no host readback, so the launches can be
+// captured in a HIP graph. `work` (la_hull_dev_work_bytes) holds the counters
+// and lists, `arena` (la_hull_dev_arena_bytes) the retry lists and global
+// workspaces. A set still above chunk_above points after the prefilter is not
+// hulled: it sets work counter LA_HDEV_OVER, and the caller must take the
+// host-driven path (la_launch_hull) instead.
+constexpr int LA_HDEV_TIERS = 9;
+constexpr int LA_HDEV_BIG = 0;                       // counters: [0] sets to prefilter,
+constexpr int LA_HDEV_OVER = 1 + LA_HDEV_TIERS;      // [1, 10) per tier, [10] oversized sets
+constexpr int LA_HDEV_NCOUNT = 16;
+size_t la_hull_dev_work_bytes(int n);
+size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n);
+int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes);
+
 }  // namespace cuacd
 
 int hd_launch_tasks(gpu_ctx_t ctx, const cuacd::HdTask* d_tasks, int n, char* d_ws);
+// the same with the task count on the device (at most cap tasks, *dcount present)
+int hd_launch_tasks_dev(gpu_ctx_t ctx, const cuacd::HdTask* d_tasks, int cap, const int* dcount, int chunks,
+                        char* d_ws);

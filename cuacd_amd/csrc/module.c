@@ -204,6 +204,33 @@ static PyObject* m_score_cuts(PyObject* self, PyObject* args) {
     Py_RETURN_NONE;
 }
 
+static PyObject* m_score_plan_create(PyObject* self, PyObject* args) {
+    unsigned long long v, vo, t, to, mo, pl;
+    int nm, nc;
+    float tau;
+    if (!PyArg_ParseTuple(args, "KKKKiKKif", &v, &vo, &t, &to, &nm, &mo, &pl, &nc, &tau)) return NULL;
+    NEED_CTX();
+    score_plan_t plan = NULL;
+    int rc = gpu_score_plan_create(g_ctx, P(const float*, v), P(const int*, vo), P(const int*, t), P(const int*, to),
+                                   nm, P(const int*, mo), P(const float*, pl), nc, tau, &plan);
+    if (rc) return raise_error(rc);
+    return PyLong_FromUnsignedLongLong((unsigned long long)(uintptr_t)plan);
+}
+static PyObject* m_score_plan_run(PyObject* self, PyObject* args) {
+    unsigned long long p, osc, odt;
+    if (!PyArg_ParseTuple(args, "KKK", &p, &osc, &odt)) return NULL;
+    NEED_CTX();
+    int rc = gpu_score_plan_run(g_ctx, P(score_plan_t, p), P(float*, osc), P(double*, odt));
+    if (rc) return raise_error(rc);
+    Py_RETURN_NONE;
+}
+static PyObject* m_score_plan_destroy(PyObject* self, PyObject* args) {
+    unsigned long long p;
+    if (!PyArg_ParseTuple(args, "K", &p)) return NULL;
+    gpu_score_plan_destroy(g_ctx, P(score_plan_t, p));
+    Py_RETURN_NONE;
+}
+
 static PyObject* m_set_tuning(PyObject* self, PyObject* args) {
     int key, value;
     if (!PyArg_ParseTuple(args, "ii", &key, &value)) return NULL;
@@ -717,6 +744,9 @@ static PyMethodDef methods[] = {
     {"set_tuning", m_set_tuning, METH_VARARGS, "Set a hull launch-shape knob (key, value)."},
     {"get_tuning", m_get_tuning, METH_VARARGS, "Read a hull launch-shape knob."},
     {"score_cuts", m_score_cuts, METH_VARARGS, "Batched exact cut scores: cut, hulls, max(0.3 rv, H) per half."},
+    {"score_plan_create", m_score_plan_create, METH_VARARGS, "Plan a batch of cut scores as one captured graph."},
+    {"score_plan_run", m_score_plan_run, METH_VARARGS, "Replay a score plan; scores and detail out."},
+    {"score_plan_destroy", m_score_plan_destroy, METH_VARARGS, "Free a score plan."},
     {"kdop_hull", m_kdop_hull, METH_VARARGS, "k-DOP prefiltered exact hull."},
     {"test_hausdorff", m_test_hausdorff, METH_VARARGS, "Bidirectional sampled Hausdorff distance."},
     {"test_postprocess_dc", m_test_postprocess_dc, METH_VARARGS, "Split a mesh into connected components."},

@@ -5,8 +5,10 @@
 // gpu_plane_cut_batch (no reference counterpart): one workgroup per (mesh,
 // plane) pair, returning per-cut volumes and sizes; the lookahead search and
 // the candidate scorer use the device function directly.
+#include <algorithm>
 #include <vector>
 #include "ctx.hpp"
+#include "lookahead.hpp"
 #include "plane_cut.hpp"
 
 using namespace cuacd;
@@ -26,9 +28,11 @@ namespace {
 
 __global__ __launch_bounds__(PC_T) void pc_single_kernel(const float* V, const int* T, int nv, int nt, float pa,
                                                          float pb, float pc, float pd, DPool* pool, int* err,
-                                                         PcCutOut* out) {
+                                                         PcCutOut* out, int lds_nv) {
     __shared__ PcLDS s;
-    PcResult r = plane_cut_block(V, T, nv, nt, pa, pb, pc, pd, pool, err, &s);
+    extern __shared__ __align__(16) unsigned char pc_dyn[];
+    PcResult r = nv <= lds_nv ? plane_cut_block_lds(V, T, nv, nt, pa, pb, pc, pd, pool, err, &s, pc_dyn)
+                              : plane_cut_block(V, T, nv, nt, pa, pb, pc, pd, pool, err, &s);
     if (threadIdx.x == 0) {
         out->pv = r.pos.v; out->pt = r.pos.t; out->pnv = r.pos.nv; out->pnt = r.pos.nt; out->pvol = r.pos.vol;
         out->nv = r.neg.v; out->nt = r.neg.t; out->nnv = r.neg.nv; out->nnt = r.neg.nt; out->nvol = r.neg.vol;
@@ -55,10 +59,12 @@ struct PcBatchArgs {
     DPool* pool;
     int* err;
     PcCutOut* out;
+    int lds_nv;  // meshes with up to this many vertices take the LDS-resident cut
 };
 
 __global__ __launch_bounds__(PC_T) void pc_batch_kernel(PcBatchArgs a) {
     __shared__ PcLDS s;
+    extern __shared__ __align__(16) unsigned char pc_dyn[];
     const int i = blockIdx.x;
     if (i >= a.n) return;
     const int m = a.mesh_of[i];
@@ -66,7 +72,8 @@ __global__ __launch_bounds__(PC_T) void pc_batch_kernel(PcBatchArgs a) {
     const int* T = a.tris + 3ll * a.tri_off[m];
     const int nv = a.vert_off[m + 1] - a.vert_off[m], nt = a.tri_off[m + 1] - a.tri_off[m];
     const float4 p = a.planes[i];
-    PcResult r = plane_cut_block(V, T, nv, nt, p.x, p.y, p.z, p.w, a.pool, a.err, &s);
+    PcResult r = nv <= a.lds_nv ? plane_cut_block_lds(V, T, nv, nt, p.x, p.y, p.z, p.w, a.pool, a.err, &s, pc_dyn)
+                                : plane_cut_block(V, T, nv, nt, p.x, p.y, p.z, p.w, a.pool, a.err, &s);
     if (threadIdx.x == 0) {
         PcCutOut& o = a.out[i];
         o.pv = r.pos.v; o.pt = r.pos.t; o.pnv = r.pos.nv; o.pnt = r.pos.nt; o.pvol = r.pos.vol;
@@ -92,8 +99,11 @@ int gpu_test_plane_cut(gpu_ctx_t ctx, const float* verts, int nv, const int* tri
     PcCutOut* d_out = (PcCutOut*)(d + ((bv + 255) & ~255ull) + ((bt + 255) & ~255ull));
     HIP_TRY(hipMemcpyAsync(d_v, verts, bv, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(d_t, tris, bt, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(pc_single_kernel, dim3(1), dim3(PC_T), 0, ctx->stream, d_v, d_t, nv, nt, pa, pb, pc, pd,
-                       ctx->d_pool, ctx->d_err, d_out);
+    int lds_nv;
+    size_t lds;
+    if (int rc = pc_lds_launch(ctx, (const void*)pc_single_kernel, nv, &lds_nv, &lds)) return rc;
+    hipLaunchKernelGGL(pc_single_kernel, dim3(1), dim3(PC_T), lds, ctx->stream, d_v, d_t, nv, nt, pa, pb, pc, pd,
+                       ctx->d_pool, ctx->d_err, d_out, lds_nv);
     HIP_TRY(hipGetLastError());
     PcCutOut h;
     HIP_TRY(hipMemcpyAsync(&h, d_out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -151,8 +161,17 @@ int gpu_plane_cut_batch(gpu_ctx_t ctx, const float* verts, const int* vert_off, 
     HIP_TRY(hipMemcpyAsync(d_to, tri_off, 4ull * (n_meshes + 1), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(d_mo, mesh_of, 4ull * n_cuts, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(d_pl, planes, 16ull * n_cuts, hipMemcpyHostToDevice, ctx->stream));
-    PcBatchArgs a{d_v, d_vo, d_t, d_to, d_mo, d_pl, n_cuts, ctx->d_pool, ctx->d_err, d_out};
-    hipLaunchKernelGGL(pc_batch_kernel, dim3(n_cuts), dim3(PC_T), 0, ctx->stream, a);
+    int max_nv = 0;
+    for (int i = 0; i < n_cuts; ++i) {
+        REQUIRE(mesh_of[i] >= 0 && mesh_of[i] < n_meshes, "plane_cut_batch: cut %d names mesh %d of %d", i,
+                mesh_of[i], n_meshes);
+        max_nv = std::max(max_nv, vert_off[mesh_of[i] + 1] - vert_off[mesh_of[i]]);
+    }
+    int lds_nv;
+    size_t lds;
+    if (int rc = pc_lds_launch(ctx, (const void*)pc_batch_kernel, max_nv, &lds_nv, &lds)) return rc;
+    PcBatchArgs a{d_v, d_vo, d_t, d_to, d_mo, d_pl, n_cuts, ctx->d_pool, ctx->d_err, d_out, lds_nv};
+    hipLaunchKernelGGL(pc_batch_kernel, dim3(n_cuts), dim3(PC_T), lds, ctx->stream, a);
     HIP_TRY(hipGetLastError());
     std::vector<PcCutOut> h(n_cuts);
     HIP_TRY(hipMemcpyAsync(h.data(), d_out, sizeof(PcCutOut) * n_cuts, hipMemcpyDeviceToHost, ctx->stream));

@@ -57,9 +57,43 @@ int sc_check(gpu_ctx_t ctx, const char* where) {
     return 0;
 }
 
+// Sizes an eager run saw (for a plan of the same batch).
+struct ScoreSizes {
+    int hd_tasks = 0;                  // Hausdorff tasks (two per measured half)
+    unsigned long long hd_ws = 0;      // their workspace bytes
+};
+
+// Per cut: the score from both halves' costs; the detail row when asked
+// (shared by the eager run and the plan). mv/hv/crv/hd per half 2i + side,
+// has_hull[q] whether the half had a hull task.
+void sc_finish(int n_cuts, const std::vector<LaCutTask>& ct, const std::vector<char>& has_hull,
+               const std::vector<double>& mv, const std::vector<double>& hv, const std::vector<float>& crv,
+               const std::vector<float>& hd, float* out_score, double* out_detail) {
+    for (int i = 0; i < n_cuts; ++i) {
+        const size_t p = 2 * (size_t)i, n = p + 1;
+        const bool ok = has_hull[p] && has_hull[n];
+        float sc = 1e30f;
+        if (ok) {
+            const float cp = std::max(crv[p], hd[p] > 0.0f ? hd[p] : 0.0f);
+            const float cn = std::max(crv[n], hd[n] > 0.0f ? hd[n] : 0.0f);
+            sc = std::max(cp, cn);
+        }
+        out_score[i] = sc;
+        if (out_detail) {
+            double* o = out_detail + 10 * (size_t)i;
+            o[0] = ct[i].status;
+            o[1] = mv[p]; o[2] = mv[n];
+            o[3] = hv[p]; o[4] = hv[n];
+            o[5] = crv[p]; o[6] = crv[n];
+            o[7] = hd[p]; o[8] = hd[n];
+            o[9] = ok ? 1.0 : 0.0;
+        }
+    }
+}
+
 int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris, const int* tri_off,
               int n_meshes, const int* mesh_of, const float* planes, int n_cuts, float tau, float* out_score,
-              double* out_detail, double* out_stats) {
+              double* out_detail, double* out_stats, ScoreSizes* sizes = nullptr) {
     REQUIRE(n_meshes >= 0 && n_cuts >= 0, "score_cuts: negative sizes");
     if (n_cuts == 0) return 0;
     REQUIRE(n_meshes > 0, "score_cuts: cuts without meshes");
@@ -120,7 +154,9 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
     REQUIRE(dct.alloc(ctx, ct.size()) == 0, "score_cuts: task allocation failed");
     HIP_TRY(hipMemcpyAsync(dct.p, ct.data(), sizeof(LaCutTask) * ct.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipEventRecord(e0, ctx->stream));
-    LA_SC_TRY(la_launch_cut(ctx, dct.p, n_cuts));
+    int max_nv = 0;
+    for (const LaCutTask& c : ct) max_nv = std::max(max_nv, c.nv);
+    LA_SC_TRY(la_launch_cut(ctx, dct.p, n_cuts, max_nv));
     LA_SC_TRY(phase_end(0));
     HIP_TRY(hipMemcpyAsync(ct.data(), dct.p, sizeof(LaCutTask) * ct.size(), hipMemcpyDeviceToHost, ctx->stream));
     LA_SC_TRY(sc_check(ctx, "cuts"));
@@ -242,32 +278,19 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
             if (rc == 0)
                 for (size_t k = 0; k < who.size(); ++k) hd[who[k]] = std::sqrt(std::max(r[2 * k], r[2 * k + 1]));
         }
+        if (sizes) {
+            sizes->hd_tasks = (int)h.size();
+            sizes->hd_ws = ws;
+        }
     }
     const int frc = free_chunks();
     if (rc == 0) rc = frc;
     if (rc) return rc;
 
     // ---- scores --------------------------------------------------------------
-    for (int i = 0; i < n_cuts; ++i) {
-        const size_t p = 2 * (size_t)i, n = p + 1;
-        const bool ok = hof[p] && hof[n];
-        float sc = 1e30f;
-        if (ok) {
-            const float cp = std::max(crv[p], hd[p] > 0.0f ? hd[p] : 0.0f);
-            const float cn = std::max(crv[n], hd[n] > 0.0f ? hd[n] : 0.0f);
-            sc = std::max(cp, cn);
-        }
-        out_score[i] = sc;
-        if (out_detail) {
-            double* o = out_detail + 10 * (size_t)i;
-            o[0] = ct[i].status;
-            o[1] = mv[p]; o[2] = mv[n];
-            o[3] = hv[p]; o[4] = hv[n];
-            o[5] = crv[p]; o[6] = crv[n];
-            o[7] = hd[p]; o[8] = hd[n];
-            o[9] = ok ? 1.0 : 0.0;
-        }
-    }
+    std::vector<char> has_hull(hof.size());
+    for (size_t q = 0; q < hof.size(); ++q) has_hull[q] = hof[q] != nullptr;
+    sc_finish(n_cuts, ct, has_hull, mv, hv, crv, hd, out_score, out_detail);
     if (out_stats) {
         double cut_bytes = 0, hull_bytes = 0;
         for (const LaCutTask& c : ct) cut_bytes += 12.0 * (c.nv + c.nt + c.pnv + c.pnt + c.nnv + c.nnt) + 16.0;
@@ -294,4 +317,316 @@ extern "C" int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert
     (void)read_and_clear_err(ctx);
     return score_run(ctx, verts, vert_off, tris, tri_off, n_meshes, mesh_of, planes, n_cuts, tau, out_score,
                      out_detail, out_stats);
+}
+
+// ---------------------------------------------------------------------------
+// Score plan: the same scoring of a fixed batch, with every stage sized on the
+// device (cut -> hull tasks -> k-DOP prefilter -> tier lists -> tiered hulls ->
+// Hausdorff task table and workspace -> Hausdorff -> frees), captured once as
+// a HIP graph and replayed. A replay has no host synchronisation inside; the
+// host reads the per-half results back once and forms the scores exactly as
+// gpu_score_cuts does (same FP64 volume costs, same decision which halves'
+// Hausdorff distances count).
+// ---------------------------------------------------------------------------
+namespace {
+
+// The hull task of each half: the cut's vertices of that side when the cut
+// produced both halves, else an empty task (skipped by the hull call).
+__global__ void sc_hull_tasks_kernel(const LaCutTask* ct, int n_cuts, LaHullTask* ht) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n_cuts; q += gridDim.x * blockDim.x) {
+        const LaCutTask& c = ct[q >> 1];
+        const bool ok = c.status == 0 && c.pv && c.nvp;
+        LaHullTask h{};
+        h.pts = (q & 1) ? c.nvp : c.pv;
+        h.n = ok ? ((q & 1) ? c.nnv : c.pnv) : 0;
+        h.emit = 1;
+        h.prune_gap = 1e30;
+        ht[q] = h;
+    }
+}
+
+// Hausdorff tasks (half -> hull and hull -> half) of every half with a hull
+// whose volume gap is under g_max - a bound a little above the gap at which
+// 0.3 rv reaches tau, so the host's exact rule picks from a superset.
+// Workspace offsets come from an atomic bump of ws_top; a task table or
+// workspace overflow is flagged in flags (the run is then redone eagerly).
+__global__ void sc_hd_tasks_kernel(const LaCutTask* ct, const LaHullTask* ht, int n_cuts, double g_max,
+                                   HdTask* hd, float* d2, int* half_of, int* counts, unsigned long long* ws_top,
+                                   unsigned long long ws_cap, int hd_cap) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n_cuts; q += gridDim.x * blockDim.x) {
+        const LaHullTask& x = ht[q];
+        if (x.n <= 0 || !x.hv) continue;
+        const LaCutTask& c = ct[q >> 1];
+        const bool neg = q & 1;
+        double mv = neg ? c.nvol : c.pvol;
+        if (mv > x.vol) mv = x.vol;
+        const double g = x.vol - mv > 0.0 ? x.vol - mv : 0.0;
+        if (!(g < g_max)) continue;
+        const float* pv = neg ? c.nvp : c.pv;
+        const int* pt = neg ? c.ntp : c.pt;
+        const int pnv = neg ? c.nnv : c.pnv, pnt = neg ? c.nnt : c.pnt;
+        const unsigned long long w0 = (hd_ws_bytes(pnt, x.hnt) + 255ull) & ~255ull;
+        const unsigned long long w1 = (hd_ws_bytes(x.hnt, pnt) + 255ull) & ~255ull;
+        const int k = atomicAdd(&counts[0], 2);
+        if (k + 2 > hd_cap) {
+            atomicOr(&counts[1], 1);
+            continue;
+        }
+        const unsigned long long off = atomicAdd(ws_top, w0 + w1);
+        HdTask a{x.hv, x.ht, pv, pt, x.hnv, x.hnt, pnv, pnt, off, 0, 0, d2 + k, nullptr, nullptr};
+        HdTask b{pv, pt, x.hv, x.ht, pnv, pnt, x.hnv, x.hnt, off + w0, 0, 0, d2 + k + 1, nullptr, nullptr};
+        if (off + w0 + w1 > ws_cap) {
+            atomicOr(&counts[1], 2);
+            a.snt = a.tnt = b.snt = b.tnt = 0;  // empty tasks: their results read 0
+        }
+        hd[k] = a;
+        hd[k + 1] = b;
+        half_of[k >> 1] = q;
+    }
+}
+
+// Tasks [count, cap) of the Hausdorff table left from an earlier replay would
+// be skipped by the count; the first count are rewritten each replay.
+__global__ void sc_zero_kernel(int* counts, unsigned long long* ws_top) {
+    if (threadIdx.x < 4) counts[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *ws_top = 0;
+}
+
+// Frees the halves and the hull meshes of a replay.
+__global__ void sc_free_kernel(const LaCutTask* ct, const LaHullTask* ht, int n_cuts, DPool* pool, int* err) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n_cuts; q += gridDim.x * blockDim.x) {
+        const LaCutTask& c = ct[q >> 1];
+        heap_free(pool, (q & 1) ? c.nvp : c.pv, err);
+        if (ht[q].n > 0) heap_free(pool, ht[q].hv, err);
+    }
+}
+
+}  // namespace
+
+struct score_plan {
+    gpu_ctx_t ctx = nullptr;
+    int n_cuts = 0, max_nv = 0, hd_cap = 0, chunks = 1;
+    float tau = 0.0f;
+    double g_max = 0.0;
+    float* dv = nullptr;
+    int* dt = nullptr;
+    LaCutTask* dct = nullptr;
+    LaHullTask* dht = nullptr;
+    char* hwork = nullptr;
+    char* harena = nullptr;
+    size_t harena_bytes = 0;
+    HdTask* dhd = nullptr;
+    float* dd2 = nullptr;
+    int* dhalf = nullptr;
+    int* dcounts = nullptr;  // [0] Hausdorff tasks, [1] overflow flags; ws_top after them
+    char* dws = nullptr;
+    unsigned long long ws_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<LaCutTask> ct;
+    std::vector<LaHullTask> ht;
+};
+
+namespace {
+
+void plan_free(score_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    void* bufs[] = {p->dv, p->dt, p->dct, p->dht, p->hwork, p->harena, p->dhd, p->dd2, p->dhalf, p->dcounts, p->dws};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+// The launches of one replay, on the context's stream (the hull tiers fork
+// to the side streams and join back).
+int plan_launch(score_plan* p) {
+    gpu_ctx_t ctx = p->ctx;
+    const int nc = p->n_cuts, nh = 2 * nc;
+    LA_SC_TRY(la_launch_cut(ctx, p->dct, nc, p->max_nv));
+    const unsigned g = (unsigned)std::max(1, std::min(1024, (nh + 255) / 256));
+    hipLaunchKernelGGL(sc_hull_tasks_kernel, dim3(g), dim3(256), 0, ctx->stream, (const LaCutTask*)p->dct, nc, p->dht);
+    HIP_TRY(hipGetLastError());
+    LA_SC_TRY(la_launch_hull_dev(ctx, p->dht, nh, p->hwork, p->harena, p->harena_bytes));
+    unsigned long long* ws_top = (unsigned long long*)(p->dcounts + 4);
+    hipLaunchKernelGGL(sc_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, p->dcounts, ws_top);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sc_hd_tasks_kernel, dim3(g), dim3(256), 0, ctx->stream, (const LaCutTask*)p->dct,
+                       (const LaHullTask*)p->dht, nc, p->g_max, p->dhd, p->dd2, p->dhalf, p->dcounts, ws_top,
+                       p->ws_cap, p->hd_cap);
+    HIP_TRY(hipGetLastError());
+    LA_SC_TRY(hd_launch_tasks_dev(ctx, p->dhd, p->hd_cap, p->dcounts, p->chunks, p->dws));
+    hipLaunchKernelGGL(sc_free_kernel, dim3(g), dim3(256), 0, ctx->stream, (const LaCutTask*)p->dct,
+                       (const LaHullTask*)p->dht, nc, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Reads a finished replay back and forms the scores. Returns 1 (and leaves
+// the outputs alone) when a device table overflowed.
+int plan_finish(score_plan* p, float* out_score, double* out_detail) {
+    gpu_ctx_t ctx = p->ctx;
+    const int nc = p->n_cuts, nh = 2 * nc;
+    int counts[4];
+    HIP_TRY(hipMemcpyAsync(p->ct.data(), p->dct, sizeof(LaCutTask) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(p->ht.data(), p->dht, sizeof(LaHullTask) * nh, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(counts, p->dcounts, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
+    int over = 0;
+    HIP_TRY(hipMemcpyAsync(&over, p->hwork + 4 * LA_HDEV_OVER, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int e = read_and_clear_err(ctx);
+    if (e) {
+        char buf[256];
+        kerr_decode(e, buf, sizeof(buf));
+        set_error(ctx, "score_plan: %s", buf);
+        return e;
+    }
+    if (counts[1] || over) return 1;
+    const int nhd = std::min(counts[0], p->hd_cap);
+    std::vector<float> d2(nhd);
+    std::vector<int> half(nhd / 2);
+    if (nhd > 0) {
+        HIP_TRY(hipMemcpyAsync(d2.data(), p->dd2, 4ull * nhd, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(half.data(), p->dhalf, 4ull * (nhd / 2), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<double> mv(nh, 0.0), hv(nh, 0.0);
+    std::vector<float> crv(nh, 0.0f), hd(nh, -1.0f);
+    std::vector<char> has_hull(nh, 0);
+    for (int q = 0; q < nh; ++q) {
+        const LaCutTask& c = p->ct[q / 2];
+        mv[q] = (q & 1) ? c.nvol : c.pvol;
+        if (p->ht[q].n <= 0) continue;
+        has_hull[q] = 1;
+        hv[q] = p->ht[q].vol;
+        if (mv[q] > hv[q]) mv[q] = hv[q];  // cuda/la_lifecycle.cu:292-296
+        crv[q] = sc_cost_rv(hv[q], mv[q]);
+    }
+    // the Hausdorff distance counts for the halves the eager rule picks
+    for (int k = 0; k < nhd / 2; ++k) {
+        const int q = half[k];
+        if (q >= 0 && q < nh && has_hull[q] && crv[q] < p->tau)
+            hd[q] = std::sqrt(std::max(d2[2 * k], d2[2 * k + 1]));
+    }
+    sc_finish(nc, p->ct, has_hull, mv, hv, crv, hd, out_score, out_detail);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris,
+                                     const int* tri_off, int n_meshes, const int* mesh_of, const float* planes,
+                                     int n_cuts, float tau, score_plan_t* out_plan) {
+    if (!ctx || !out_plan) return -1;
+    *out_plan = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    (void)read_and_clear_err(ctx);
+    REQUIRE(n_cuts > 0, "score_plan: no cuts");
+    // one eager run: validates the batch, sizes the Hausdorff stage
+    std::vector<float> s0(n_cuts);
+    ScoreSizes sz;
+    if (int rc = score_run(ctx, verts, vert_off, tris, tri_off, n_meshes, mesh_of, planes, n_cuts, tau, s0.data(),
+                           nullptr, nullptr, &sz))
+        return rc;
+    score_plan* p = new score_plan();
+    p->ctx = ctx;
+    p->n_cuts = n_cuts;
+    p->tau = tau;
+    // the gap at which 0.3 rv = tau, widened by 1e-6 (rounding of cbrt and of
+    // the cast to float cannot move a cost by that much)
+    const double r = (double)tau / 0.3;
+    p->g_max = r * r * r * (4.0 * M_PI / 3.0) * (1.0 + 1e-6);
+    const size_t tv = (size_t)vert_off[n_meshes], tt = (size_t)tri_off[n_meshes];
+    const int nh = 2 * n_cuts;
+    p->hd_cap = 2 * nh;
+    p->chunks = std::max(1, std::min(64, (4096 + std::max(sz.hd_tasks, 1) - 1) / std::max(sz.hd_tasks, 1)));
+    p->ws_cap = sz.hd_ws + sz.hd_ws / 4 + (1ull << 20);
+    p->harena_bytes = la_hull_dev_arena_bytes(ctx, nh);
+    bool ok = hipMalloc(&p->dv, 12 * tv + 16) == hipSuccess && hipMalloc(&p->dt, 12 * tt + 16) == hipSuccess &&
+              hipMalloc(&p->dct, sizeof(LaCutTask) * n_cuts) == hipSuccess &&
+              hipMalloc(&p->dht, sizeof(LaHullTask) * nh) == hipSuccess &&
+              hipMalloc(&p->hwork, la_hull_dev_work_bytes(nh)) == hipSuccess &&
+              hipMalloc(&p->harena, p->harena_bytes) == hipSuccess &&
+              hipMalloc(&p->dhd, sizeof(HdTask) * p->hd_cap) == hipSuccess &&
+              hipMalloc(&p->dd2, 4ull * p->hd_cap) == hipSuccess &&
+              hipMalloc(&p->dhalf, 4ull * (p->hd_cap / 2) + 16) == hipSuccess &&
+              hipMalloc(&p->dcounts, 64) == hipSuccess && hipMalloc(&p->dws, p->ws_cap + 256) == hipSuccess;
+    if (!ok) {
+        plan_free(p);
+        set_error(ctx, "score_plan: device allocation failed");
+        return -1;
+    }
+    p->ct.resize(n_cuts);
+    p->ht.resize(nh);
+    for (int i = 0; i < n_cuts; ++i) {
+        const int m = mesh_of[i];
+        LaCutTask& c = p->ct[i];
+        memset(&c, 0, sizeof(c));
+        c.v = p->dv + 3 * (size_t)vert_off[m];
+        c.t = p->dt + 3 * (size_t)tri_off[m];
+        c.nv = vert_off[m + 1] - vert_off[m];
+        c.nt = tri_off[m + 1] - tri_off[m];
+        for (int j = 0; j < 4; ++j) c.plane[j] = planes[4 * i + j];
+        p->max_nv = std::max(p->max_nv, c.nv);
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(p->dv, verts, 12 * tv, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(p->dt, tris, 12 * tt, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(p->dct, p->ct.data(), sizeof(LaCutTask) * n_cuts, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = -1;
+    // one uncaptured pass of the device-sized launches (kernel attributes,
+    // constant tables), checked against the eager scores, then the capture
+    KTimers* kt = ctx->kt;
+    ctx->kt = nullptr;
+    if (rc == 0) rc = plan_launch(p);
+    std::vector<float> s1(n_cuts);
+    if (rc == 0) rc = plan_finish(p, s1.data(), nullptr);
+    if (rc == 0 && memcmp(s0.data(), s1.data(), 4ull * n_cuts) != 0) {
+        set_error(ctx, "score_plan: the device-sized pass disagrees with the eager scores");
+        rc = -1;
+    } else if (rc == 1) {
+        set_error(ctx, "score_plan: a set above the one-workgroup hull size or a full table (use gpu_score_cuts)");
+        rc = -1;
+    }
+    if (rc == 0) {
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = -1;
+        } else {
+            const int lrc = plan_launch(p);
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &p->graph);
+            if (lrc != 0 || e != hipSuccess) {
+                rc = lrc ? lrc : -1;
+                set_error(ctx, "score_plan: graph capture failed (%s)", hipGetErrorString(e));
+            } else if (hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
+                rc = -1;
+                set_error(ctx, "score_plan: graph instantiation failed");
+            }
+        }
+    }
+    ctx->kt = kt;
+    if (rc) {
+        plan_free(p);
+        return rc;
+    }
+    *out_plan = p;
+    return 0;
+}
+
+extern "C" int gpu_score_plan_run(gpu_ctx_t ctx, score_plan_t plan, float* out_score, double* out_detail) {
+    if (!ctx || !plan) return -1;
+    HIP_TRY(hipSetDevice(ctx->device));
+    REQUIRE(plan->ctx == ctx, "score_plan: plan of another context");
+    HIP_TRY(hipGraphLaunch(plan->exec, ctx->stream));
+    const int rc = plan_finish(plan, out_score, out_detail);
+    REQUIRE(rc != 1, "score_plan: a device table overflowed on replay");
+    return rc;
+}
+
+extern "C" void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan) {
+    if (!plan) return;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    plan_free(plan);
 }

@@ -100,6 +100,21 @@ template <class T> __device__ __forceinline__ bool cas_acq_rel(T* p, T& expected
 template <class T> __device__ __forceinline__ T fetch_add_relaxed(T* p, T v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// XCD-aware workgroup order. The dispatcher hands consecutive workgroups of a
+// launch to the 8 XCDs in turn; this renumbers workgroup b of nb so that each
+// XCD runs groups of G consecutive task slots (groups still go round the XCDs,
+// keeping the load spread). Tasks that read the same data (the candidate
+// planes of one part) then share one XCD's L2 instead of fetching the data
+// into several. A bijection of [0, nb); the last nb mod 8G workgroups keep
+// their own index.
+__device__ __forceinline__ unsigned xcd_group_slot(unsigned b, unsigned nb, unsigned G) {
+    const unsigned span = 8u * G;
+    const unsigned full = nb / span * span;
+    if (b >= full) return b;
+    const unsigned x = b % 8u, j = b / 8u;
+    return ((j / G) * 8u + x) * G + (j % G);
+}
+
 // err == nullptr: the caller handles the failure itself (nothing recorded)
 __device__ __forceinline__ void err_or(int* err, int bits) {
     if (bits && err) __hip_atomic_fetch_or(err, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -137,18 +137,18 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * convex hulls (hull_vert_off/hull_tri_off), decomposed independently with the
  * same parameters in one search whose launches span all meshes. out[m] gets
  * mesh m's parts (free each with gpu_result_free); out_iters[m] (optional) the
- * iterations it ran; out_stats (optional, 77 doubles): [0, 27) per phase -
+ * iterations it ran; out_stats (optional, 80 doubles): [0, 27) per phase -
  * components, part hulls, Hausdorff, concave edges, level-0 cuts, leaf
  * scoring, item hulls, merge, learned proposal - the GPU milliseconds,
  * launches and algorithmic bytes; [27, 35) host wall milliseconds by section
  * - set-up, per-iteration refresh and selection, concave planes, level 0,
  * deeper levels, evaluate and apply, post-loop passes and merge, result
- * readback; [35, 74) per hull kernel class - one-workgroup hulls of <= 64,
+ * readback; [35, 77) per hull kernel class - one-workgroup hulls of <= 64,
  * 128, 256, 512, 1024 points, multi-wave hulls of <= 2048, 4096, 8192, 16384
- * points, k-DOP prefilter, full-capacity retries of the 512-, 1024- and
- * 2048-point first passes - the milliseconds between HIP events around each
- * launch, launches and algorithmic bytes (points read, volumes written);
- * [74, 77) learned proposal: level-0 candidates kept, candidates ranked,
+ * points, k-DOP prefilter, full-capacity retries of the 512-, 1024-, 2048-
+ * and 4096-point first passes - the milliseconds between HIP events around
+ * each launch, launches and algorithmic bytes (points read, volumes written);
+ * [77, 80) learned proposal: level-0 candidates kept, candidates ranked,
  * fallbacks (gpu_lookahead_set_proposal). */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off,
@@ -230,13 +230,32 @@ int gpu_plane_cut_batch(gpu_ctx_t ctx, const float* verts, const int* vert_off, 
  * larger cost of the two halves, 1e30 when the cut fails or leaves a side
  * empty. out_detail (optional, 10 doubles per cut): status, mesh volume
  * pos/neg (clamped to the hull volume), hull volume pos/neg, rv cost pos/neg,
- * H pos/neg (-1 when not measured), 1 if scored. out_stats (optional, 45
+ * H pos/neg (-1 when not measured), 1 if scored. out_stats (optional, 48
  * doubles): GPU ms of the cut, hull and Hausdorff stages, algorithmic bytes of
- * the cut and hull stages, hull tasks, then per hull kernel class (the 13
+ * the cut and hull stages, hull tasks, then per hull kernel class (the 14
  * classes of gpu_lookahead_batch's statistics) ms, launches, bytes. */
 int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris, const int* tri_off,
                    int n_meshes, const int* mesh_of, const float* planes, int n_cuts, float tau, float* out_score,
                    double* out_detail, double* out_stats);
+
+/* Score plan (no reference counterpart; the scorer of gpu_score_cuts as one
+ * device-sized level): the meshes and planes of a fixed batch are uploaded
+ * once, and every stage is sized on the device - the cut, the hull tasks of
+ * the halves, the k-DOP prefilter, each hull tier's task list and count, the
+ * Hausdorff task table and workspace offsets, the frees - so one replay is a
+ * captured HIP graph with no host synchronisation inside. create runs the
+ * batch once eagerly (gpu_score_cuts) to size the Hausdorff workspace, once
+ * through the device-sized launches (its scores must equal the eager ones),
+ * then captures. run replays the graph, reads the per-half results back once
+ * and writes the same out_score / out_detail as gpu_score_cuts. A batch with a
+ * half above one workgroup's hull size after the prefilter is refused at
+ * create (gpu_score_cuts handles it). */
+typedef struct score_plan* score_plan_t;
+int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int* tris,
+                          const int* tri_off, int n_meshes, const int* mesh_of, const float* planes, int n_cuts,
+                          float tau, score_plan_t* plan);
+int gpu_score_plan_run(gpu_ctx_t ctx, score_plan_t plan, float* out_score, double* out_detail);
+void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
 
 /* Launch-shape knobs of the hull tiers and the proposal network's inference
  * (no reference counterpart; the autotuner sets them). They select kernels
@@ -248,7 +267,15 @@ int gpu_score_cuts(gpu_ctx_t ctx, const float* verts, const int* vert_off, const
  * >= 4), 4 chunk_above (prefiltered sets above it are hulled in chunks of
  * half that size first, 1024..16384), 5 multi (several apexes per quickhull
  * round in the eight-wave LDS tier, 0/1), 6 mlp_fused (the proposal
- * network's inference as one fused point-layer kernel, 0/1; same scores).
+ * network's inference as one fused point-layer kernel, 0/1; same scores),
+ * 7 ldsv8_vcap (first-pass vertex capacity of the eight-wave LDS tier,
+ * 64..1024 in steps of 32; hulls with more corners are redone in HBM),
+ * 8 ldsb8_vcap (vertex capacity of an LDS first pass for sets of 2049-4096
+ * points, 0 = none, else 64..1024; default 1024), 9 cut_lds_verts (plane
+ * cuts of meshes with up to this many vertices keep their per-vertex state
+ * in LDS and write no per-triangle scratch; 0 = none, else 256..110592;
+ * default 98304; the same halves either way), 10 hull_lpt (each hull tier's
+ * sets launched largest first, 0/1; default 0).
  * Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);

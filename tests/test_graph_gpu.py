@@ -91,3 +91,85 @@ def test_plan_graph_replays_match_eager(ctx):
         for p in bufs:
             lib.gpu_dev_free(g, p)
         lib.gpu_destroy(g)
+
+
+def test_score_plan_replays_match_eager_scoring(ctx):
+    """The scorer as one device-sized level, called through the C-ABI:
+    gpu_score_plan_create sizes every stage on the device (cut, hull tasks,
+    k-DOP prefilter, tier lists, Hausdorff tasks and workspace, frees) and
+    captures the launches as a HIP graph; every gpu_score_plan_run replays it.
+    Each replay - with eager scoring of another batch in between - gives the
+    scores and detail of gpu_score_cuts bit for bit, and the device heap is
+    back where it started once the plan is gone."""
+    from cuacd_amd import sdf
+    lib = C.CDLL(os.path.join(ROOT, "cuacd_amd", "libcuacd_hip.so"))
+    lib.gpu_init.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_size_t]
+    lib.gpu_destroy.argtypes = [C.c_void_p]
+    lib.gpu_last_error.argtypes = [C.c_void_p]
+    lib.gpu_last_error.restype = C.c_char_p
+    lib.gpu_heap_stats.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gpu_score_cuts.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                                                      C.c_float] + [C.c_void_p] * 3
+    lib.gpu_score_plan_create.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_void_p,
+                                                                             C.c_int, C.c_float,
+                                                                             C.POINTER(C.c_void_p)]
+    lib.gpu_score_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.gpu_score_plan_destroy.argtypes = [C.c_void_p, C.c_void_p]
+
+    meshes = sdf.sdf_meshes(6, seed=21, res=32) + datasets.noisy_icospheres(4, 3, 0.15, seed=3)
+    rng = np.random.default_rng(9)
+    planes, mesh_of = [], []
+    for m, (v, t) in enumerate(meshes):
+        for _ in range(8):
+            nrm = rng.standard_normal(3)
+            nrm /= np.linalg.norm(nrm)
+            c = v.mean(0) + rng.uniform(-0.3, 0.3, 3) * (v.max(0) - v.min(0))
+            planes.append([*nrm, -float(nrm @ c)])
+            mesh_of.append(m)
+    planes = np.array(planes, np.float32)
+    mesh_of = np.array(mesh_of, np.int32)
+    n = len(mesh_of)
+    allv, voff, allt, toff = _pack(meshes)
+    tau = 0.05
+
+    def eager(g, sel):
+        mo = np.ascontiguousarray(mesh_of[sel])
+        pl = np.ascontiguousarray(planes[sel])
+        sc = np.zeros(len(mo), np.float32)
+        dt = np.zeros((len(mo), 10), np.float64)
+        rc = lib.gpu_score_cuts(g, allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
+                                len(meshes), mo.ctypes.data, pl.ctypes.data, len(mo), tau, sc.ctypes.data,
+                                dt.ctypes.data, None)
+        assert rc == 0, lib.gpu_last_error(g)
+        return sc, dt
+
+    g = C.c_void_p()
+    assert lib.gpu_init(C.byref(g), 0, C.c_size_t(1 << 30)) == 0
+    try:
+        h0 = np.zeros(6, np.uint64)
+        assert lib.gpu_heap_stats(g, h0.ctypes.data) == 0
+        want_s, want_d = eager(g, slice(None))
+        assert np.isfinite(want_s).sum() > n // 2 and (want_d[:, 7:9] > 0).any()  # some halves measure H
+        plan = C.c_void_p()
+        rc = lib.gpu_score_plan_create(g, allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
+                                       len(meshes), mesh_of.ctypes.data, planes.ctypes.data, n, tau,
+                                       C.byref(plan))
+        assert rc == 0, lib.gpu_last_error(g)
+        try:
+            for rep in range(3):
+                sc = np.full(n, -7.0, np.float32)
+                dt = np.full((n, 10), -7.0, np.float64)
+                rc = lib.gpu_score_plan_run(g, plan, sc.ctypes.data, dt.ctypes.data)
+                assert rc == 0, lib.gpu_last_error(g)
+                assert sc.tobytes() == want_s.tobytes(), rep
+                assert dt.tobytes() == want_d.tobytes(), rep
+                # eager scoring of another batch between the replays
+                s2, _ = eager(g, slice(0, n, 3))
+                assert s2.tobytes() == want_s[0:n:3].tobytes()
+        finally:
+            lib.gpu_score_plan_destroy(g, plan)
+        h1 = np.zeros(6, np.uint64)
+        assert lib.gpu_heap_stats(g, h1.ctypes.data) == 0
+        assert h1[0] == h0[0] and h1[3] == h0[3], (h0, h1)
+    finally:
+        lib.gpu_destroy(g)

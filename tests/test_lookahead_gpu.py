@@ -307,3 +307,25 @@ def test_parity_on_closed_form_solids(ctx, name, kw):
         # the convex split: two halves of volume 1.5 in one iteration
         assert iters == 1 and len(g) == 2
         assert np.allclose(sorted(info[0]["mesh_vol"]), [1.5, 1.5], atol=1e-6)
+
+
+def test_cut_paths_give_the_same_search(ctx):
+    """The level-0 cuts on the LDS-resident path and on the heap-scratch path
+    (knob cut_lds_verts) drive the search to the same parts and hulls, bit
+    for bit."""
+    from cuacd_amd import sdf
+    meshes = sdf.sdf_meshes(3, seed=12, res=24)
+    kw = dict(max_iters=4, width=30, width2=5, threshold=0.05, depth=2, max_n_cutting=16, n_concave_edges=16)
+    old = ctx.get_tuning("cut_lds_verts")
+    out = []
+    try:
+        for cap in (98304, 0):
+            ctx.set_tuning("cut_lds_verts", cap)
+            out.append(ctx.batch_lookahead_decompose(meshes, **kw))
+    finally:
+        ctx.set_tuning("cut_lds_verts", old)
+    for pa, pb in zip(*out):
+        assert len(pa) == len(pb)
+        for x, y in zip(pa, pb):
+            for u, w in zip(x, y):
+                assert np.asarray(u).tobytes() == np.asarray(w).tobytes()

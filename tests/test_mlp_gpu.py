@@ -256,3 +256,37 @@ def test_recorded_targets_are_every_first_cuts_best_path(ctx):
         assert len(recorded) == len(want)
         for r, w in zip(recorded, want):
             assert len(r) == len(w) and np.allclose(r, w, rtol=0, atol=1e-7), (r[:5], w[:5])
+
+
+BENCH_SEARCH = dict(max_iters=100, width=30, width2=5, depth=2, max_n_cutting=16, n_concave_edges=16,
+                    merge_hulls=True)
+
+
+@pytest.mark.parametrize("tau,top_k", [(0.05, 16), (0.03, 24)])
+def test_learned_search_on_the_lookahead_benchmark_meshes(ctx, tau, top_k):
+    """The learned proposal on the 64-mesh lookahead workload (SDF blobs,
+    seed 4, 48^3 grid, 2k-50k triangles) at the benchmark's operating points
+    (top 16 candidates per part at tau 0.05, top 24 at 0.03): every part of
+    every converged mesh meets tau, no mesh is left at max_iters that the
+    plain search finished, the part count stays within 10 % of the plain
+    search's, and the network really pruned (kept <= a third of the ranked
+    candidates)."""
+    from cuacd_amd import sdf
+    meshes = sdf.sdf_meshes(64, seed=4, res=48, min_tris=2000, max_tris=50000)
+    kw = dict(BENCH_SEARCH, threshold=tau)
+    net = PR.ProposalNet.load_default(ctx)
+    try:
+        plain, pinfo = ctx.batch_lookahead_decompose(meshes, return_info=True, **kw)
+        learned, linfo = ctx.batch_lookahead_decompose(meshes, return_info=True, proposal=net, top_k=top_k, **kw)
+        pr = ctx.last_lookahead_proposal
+    finally:
+        net.close()
+    assert pr["ranked"] > 0 and 0 < pr["kept"] <= pr["ranked"] / 3, pr
+    for pi, li in zip(pinfo, linfo):
+        if pi["iters"] < kw["max_iters"]:
+            assert li["iters"] < kw["max_iters"]
+        if li["iters"] < kw["max_iters"]:
+            assert _cost(li).max() <= tau + 1e-6
+    n_plain = sum(len(p) for p in plain)
+    n_learned = sum(len(p) for p in learned)
+    assert n_learned <= 1.10 * n_plain, (n_learned, n_plain)

@@ -20,6 +20,16 @@ pytestmark = pytest.mark.gpu
 REL = 1e-5
 
 
+@pytest.fixture(params=["lds", "heap", "mixed"])
+def cut_path(request, ctx):
+    """The LDS-resident cut (every mesh here fits), the heap-scratch cut, and
+    both in one launch (meshes above 256 vertices take the heap path)."""
+    old = ctx.get_tuning("cut_lds_verts")
+    ctx.set_tuning("cut_lds_verts", {"lds": 98304, "heap": 0, "mixed": 256}[request.param])
+    yield request.param
+    ctx.set_tuning("cut_lds_verts", old)
+
+
 def _signed_volume(v, t):
     v = v.astype(np.float64)
     a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
@@ -136,10 +146,11 @@ def test_two_disjoint_cubes(ctx):
     assert abs(_signed_volume(pv, pt) - 0.125) < 1e-6 and abs(_signed_volume(nv, nt) - 0.125) < 1e-6
 
 
-def test_random_planes_noisy_spheres_batch(ctx):
-    """128 random cuts of 32 star-shaped noisy icospheres in one launch: every
-    half's FP64 volume equals the oracle's; the same cuts one at a time agree."""
-    meshes = datasets.noisy_icospheres(32, 3, 0.15, seed=5)
+def test_random_planes_noisy_spheres_batch(ctx, cut_path):
+    """128 random cuts of 32 star-shaped noisy icospheres (subdivision levels
+    2 and 3: 162 and 642 vertices) in one launch: every half's FP64 volume
+    equals the oracle's; the same cuts one at a time agree."""
+    meshes = datasets.noisy_icospheres(16, 3, 0.15, seed=5) + datasets.noisy_icospheres(16, 2, 0.15, seed=6)
     rng = np.random.default_rng(7)
     n = 128
     mesh_of = rng.integers(0, len(meshes), n).astype(np.int32)
@@ -192,9 +203,10 @@ def test_heap_balanced_after_cuts(ctx):
     assert after[0] == before[0] and after[3] == before[3], (before, after)
 
 
-def test_bit_exact_against_serial_oracle(ctx):
+def test_bit_exact_against_serial_oracle(ctx, cut_path):
     """The GPU cut's output (vertex bits, triangle indices, including the cap
-    triangulation) equals the serial oracle's on every case."""
+    triangulation) equals the serial oracle's on every case, on both cut
+    paths."""
     import test_oracle_cut as toc
     cases = toc._cut_cases()
     meshes = datasets.noisy_icospheres(4, 3, 0.15, seed=21)
@@ -209,3 +221,45 @@ def test_bit_exact_against_serial_oracle(ctx):
         pv, pt, nv, nt = ctx.plane_cut(v, t, plane)
         assert pv.tobytes() == opv.tobytes() and np.array_equal(pt, opt), name
         assert nv.tobytes() == onv.tobytes() and np.array_equal(nt, ont), name
+
+
+def test_cut_paths_agree_on_sdf_meshes(ctx):
+    """Marching-tetrahedra meshes (thousands of triangles, on-plane vertices
+    from axis-aligned planes through grid values) cut by axis planes and
+    random planes: the LDS-resident and heap-scratch paths give the same
+    halves, bit for bit, and the same FP64 volumes."""
+    from cuacd_amd import sdf
+    meshes = sdf.sdf_meshes(6, seed=8, res=32)
+    rng = np.random.default_rng(4)
+    planes, mesh_of = [], []
+    for m, (v, t) in enumerate(meshes):
+        lo, hi = v.min(0), v.max(0)
+        for ax in range(3):
+            for f in (0.3, 0.5):
+                p = np.zeros(4, np.float32)
+                p[ax] = 1
+                p[3] = -(lo[ax] + f * (hi[ax] - lo[ax]))
+                planes.append(p)
+                mesh_of.append(m)
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        planes.append(np.float32([*n, -float(n @ v.mean(0))]))
+        mesh_of.append(m)
+    planes = np.array(planes, np.float32)
+    mesh_of = np.array(mesh_of, np.int32)
+    old = ctx.get_tuning("cut_lds_verts")
+    try:
+        got = {}
+        for name, cap in (("lds", 98304), ("heap", 0)):
+            ctx.set_tuning("cut_lds_verts", cap)
+            batch = ctx.batch_plane_cut(meshes, mesh_of, planes)
+            single = [ctx.plane_cut(*meshes[mesh_of[i]], tuple(planes[i])) for i in range(0, len(planes), 3)]
+            got[name] = (batch, single)
+        (b0, s0), (b1, s1) = got["lds"], got["heap"]
+        for x, y in zip(b0, b1):
+            assert np.array_equal(x, y)
+        for a, b in zip(s0, s1):
+            for x, y in zip(a, b):
+                assert x.tobytes() == y.tobytes()
+    finally:
+        ctx.set_tuning("cut_lds_verts", old)
