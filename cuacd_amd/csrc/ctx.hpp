@@ -59,6 +59,7 @@ struct gpu_ctx {
     struct gpu_mlp* proposal;
     int proposal_topk;
     void* record;  // level-0 training record of the search (lookahead.hip), null when off
+    long long chunk_subtasks;  // chunk hulls of oversized sets, counted since gpu_init
 };
 
 namespace cuacd {

@@ -252,6 +252,8 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
     __shared__ __align__(16) char lds[20480];
     __shared__ float ep[2 * KDOP_AXES_N][3];
     __shared__ double4 dpl[2 * 128 + 8];
+    __shared__ float4 fpl[2 * 128 + 8];  // the same planes in FP32 ...
+    __shared__ float fbd[2 * 128 + 8];   // ... and a bound on that form's error
     __shared__ unsigned long long kred[KD_NW][16];
     __shared__ int s_cnt[KD_NW + 2];
     if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
@@ -383,7 +385,15 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             const double e1x = b[0] - a[0], e1y = b[1] - a[1], e1z = b[2] - a[2];
             const double e2x = cc[0] - a[0], e2y = cc[1] - a[1], e2z = cc[2] - a[2];
             const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-            dpl[c++] = make_double4(nx, ny, nz, nx * a[0] + ny * a[1] + nz * a[2]);
+            const double pw = nx * a[0] + ny * a[1] + nz * a[2];
+            dpl[c] = make_double4(nx, ny, nz, pw);
+            // FP32 form: the four conversions and three roundings of the
+            // fused chain each err by at most 2^-24 of T = (|nx| + |ny| +
+            // |nz|) 8192 + |w| (grid coordinates are below 8192 in
+            // magnitude); 2^-20 T is four times that sum
+            fpl[c] = make_float4((float)nx, (float)ny, (float)nz, (float)pw);
+            fbd[c] = (float)((fabs(nx) + fabs(ny) + fabs(nz)) * 8192.0 + fabs(pw)) * 0x1p-20f;
+            ++c;
         }
         s_cnt[KD_NW] = c;
     }
@@ -409,10 +419,20 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             const double qx = (double)qcoord(full, 0, sel3(v, full.ax[0]));
             const double qy = (double)qcoord(full, 1, sel3(v, full.ax[1]));
             const double qz = (double)qcoord(full, 2, sel3(v, full.ax[2]));
-            // every product and sum is an integer below 2^53: exact in FP64
+            // every product and sum is an integer below 2^53: exact in FP64.
+            // The FP32 form decides when it is farther from 0 than its error
+            // bound; only the rest take the exact test, so the decision is
+            // the exact one.
+            const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
             for (int f = 0; f < npl; ++f) {
-                const double4 pl = dpl[f];
-                if (pl.x * qx + pl.y * qy + pl.z * qz - pl.w >= 0.0) { keep = true; break; }
+                const float4 pf = fpl[f];
+                const float sf = fmaf(pf.x, fx, fmaf(pf.y, fy, fmaf(pf.z, fz, -pf.w)));
+                const float bf = fbd[f];
+                if (sf > bf) { keep = true; break; }
+                if (sf >= -bf) {
+                    const double4 pl = dpl[f];
+                    if (pl.x * qx + pl.y * qy + pl.z * qz - pl.w >= 0.0) { keep = true; break; }
+                }
             }
         }
         const uint64_t m = ballot(keep);
@@ -580,14 +600,15 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
     const float pa = T.plane[0], pb = T.plane[1], pc = T.plane[2], pd = T.plane[3];
     // One flag byte per vertex - bits 0-1 sign code (1 positive, 2 negative, 0
     // on the plane), bit 2 in the positive point set, bit 3 in the negative
-    // one - in LDS when the mesh fits the launch's dynamic LDS, else in heap
-    // scratch; then the crossing-triangle list (heap).
+    // one - in LDS when the mesh fits the launch's dynamic LDS, else in a heap
+    // chunk of its own.
     const bool flags_in_lds = nv <= lds_nv;
-    const unsigned long long fb = flags_in_lds ? 0ull : al16(nv);
-    char* A = (char*)pc_alloc(pool, fb + 4ull * nt + 64, &s, 1, err, 1);
-    if (!A) return;
-    uint8_t* fl = flags_in_lds ? pc_dyn : (uint8_t*)A;
-    int* xl = (int*)(A + fb);
+    char* F = nullptr;
+    if (!flags_in_lds) {
+        F = (char*)pc_alloc(pool, al16(nv), &s, 1, err, 1);
+        if (!F) return;
+    }
+    uint8_t* fl = flags_in_lds ? pc_dyn : (uint8_t*)F;
     for (int i = tid; i < nv; i += PC_T) {
         const float d = pc_side(pa, pb, pc, pd, V[3 * i], V[3 * i + 1], V[3 * i + 2]);
         fl[i] = d > PC_EPS ? 1 : (d < -PC_EPS ? 2 : 0);
@@ -597,6 +618,29 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         const int c = fl[i] & 3;
         return c == 1 ? 1 : (c == 2 ? -1 : 0);
     };
+    // triangles with vertices strictly on both sides, counted first so that one
+    // scratch chunk holds their list and the crossing-edge hash (every heap
+    // operation synchronises the XCD's L2 with memory)
+    int nx0 = 0;
+    for (int k = tid; k < nt; k += PC_T) {
+        const int s0 = sgn(Tr[3 * k]), s1 = sgn(Tr[3 * k + 1]), s2 = sgn(Tr[3 * k + 2]);
+        nx0 += ((s0 > 0) | (s1 > 0) | (s2 > 0)) & ((s0 < 0) | (s1 < 0) | (s2 < 0));
+    }
+    nx0 = wave_sum_i(nx0);
+    if (lane_id() == 0) wcnt[wv] = nx0;
+    __syncthreads();
+    nx0 = 0;
+    for (int q = 0; q < PC_T / 64; ++q) nx0 += wcnt[q];
+    __syncthreads();
+    uint32_t H = 16;
+    while ((int)H < 4 * nx0 + 2) H <<= 1;
+    const unsigned long long szX = al16(4ull * nx0);
+    char* A = (char*)pc_alloc(pool, szX + 12ull * H + 8ull * nx0 + 64, &s, 2, err, 1);
+    if (!A) {
+        if (tid == 0 && F) heap_free(pool, F, err);
+        return;
+    }
+    int* xl = (int*)A;
     // divergence origin: the point of the plane nearest the coordinate origin
     const double n2 = (double)pa * pa + (double)pb * pb + (double)pc * pc;
     const double ox = -(double)pd * pa / n2, oy = -(double)pd * pb / n2, oz = -(double)pd * pc / n2;
@@ -730,13 +774,7 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         const int a = Tr[3 * k], b = Tr[3 * k + 1], c = Tr[3 * k + 2];
         return (int)strict(a, b) + (int)strict(b, c) + (int)strict(c, a);
     };
-    uint32_t H = 16;
-    while ((int)H < 4 * nx + 2) H <<= 1;
-    char* B = (char*)pc_alloc(pool, 12ull * H + 8ull * nx + 64, &s, 2, err, 1);
-    if (!B) {
-        if (tid == 0) heap_free(pool, A, err);
-        return;
-    }
+    char* B = A + szX;  // nx == nx0
     unsigned long long* hk = (unsigned long long*)B;
     int* hmin = (int*)(B + 8ull * H);
     int* xoff = hmin + H;
@@ -787,9 +825,11 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
     float* pp = nullptr;
     float* npp = nullptr;
     if (totp > 0 && totn > 0) {
-        pp = (float*)pc_alloc(pool, 12ull * (nposo + n_new) + 16, &s, 3, err, 1);
-        npp = (float*)pc_alloc(pool, 12ull * (nnego + n_new) + 16, &s, 0, err, 1);
-        if (pp && npp) {
+        // both point sets in one chunk (freed through the positive set's pointer)
+        const unsigned long long pos_bytes = al16(12ull * (nposo + n_new));
+        pp = (float*)pc_alloc(pool, pos_bytes + 12ull * (nnego + n_new) + 16, &s, 3, err, 1);
+        npp = pp ? (float*)((char*)pp + pos_bytes) : nullptr;
+        if (pp) {
             // originals, stable by index: per-chunk ballot compaction
             int bp = 0, bn = 0;
             for (int b0 = 0; b0 < nv; b0 += PC_T) {
@@ -844,8 +884,8 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         T.npts = npp;
         T.np_pts = pp ? nposo + n_new : 0;
         T.nn_pts = npp ? nnego + n_new : 0;
-        heap_free(pool, B, err);
         heap_free(pool, A, err);
+        if (F) heap_free(pool, F, err);
     }
 }
 
@@ -1269,7 +1309,12 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //     search step it measured slower (2.40 s against 2.31 s): the tiers
 //     already overlap on side streams, and task order keeps a mesh's sets
 //     together
-constexpr int LA_TUNE_N = 11;
+//   11 b4_split: percent of the 1025-2048-point sets (the smallest) hulled by
+//     the HBM-workspace kernel on the context's stream beside the LDS tier's
+//     stream (CUACD_HULL_B4_SPLIT, 0..90; default 0): that kernel needs 5 KB
+//     of LDS, so its workgroups fit on the CUs the LDS tier fills, whose
+//     VALUs are mostly idle
+constexpr int LA_TUNE_N = 12;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1283,6 +1328,7 @@ int tune_clamp(int key, int v) {
         case 7: return std::min(1024, std::max(64, v / 32 * 32));
         case 8: return v <= 0 ? 0 : std::min(1024, std::max(64, v / 32 * 32));
         case 9: return v <= 0 ? 0 : std::min(PC_LDS_VERTS_MAX, std::max(256, v));
+        case 11: return std::min(90, std::max(0, v));
         default: return v != 0;
     }
 }
@@ -1292,8 +1338,8 @@ void tune_init() {
     const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN",   "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP",
                                   "CUACD_KDOP_MIN",        "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI",
                                   "CUACD_MLP_FUSED",       "CUACD_HULL_LDSV8_VCAP",  "CUACD_HULL_LDSB8_VCAP",
-                                  "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0};
+                                  "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT",         "CUACD_HULL_B4_SPLIT"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1469,12 +1515,12 @@ using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsign
 // One block tier (sets above 1024 points, at most maxn). With dcnt given,
 // d_ids holds up to n ids, *dcnt on the device, and maxn is the tier's bound.
 int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int n, int maxn,
-                     double bytes, const int* dcnt = nullptr) {
+                     double bytes, const int* dcnt = nullptr, bool hbm_only = false) {
     gpu_ctx_t ctx = hc.ctx;
     if (n <= 0) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
-    const bool lds4 = maxn <= 4 * T && hull_half_cap();
-    const bool lds8 = !lds4 && maxn <= 8 * T && hull_half_cap() && hull_ldsb8_vcap() > 0;
+    const bool lds4 = !hbm_only && maxn <= 4 * T && hull_half_cap();
+    const bool lds8 = !hbm_only && !lds4 && maxn <= 8 * T && hull_half_cap() && hull_ldsb8_vcap() > 0;
     if (lds4 || lds8) {
         // first pass with the workspace in LDS (a bounded number of hull
         // vertices), then the sets whose hulls did not fit, with the workspace
@@ -1568,9 +1614,10 @@ int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pai
     const std::vector<std::pair<int, int>>& idn = g_tune[10] ? idn_sorted : idn_in;
     std::vector<int> cls[5];
     double cbytes[5] = {0, 0, 0, 0, 0};
-    std::vector<int> b4, b8, b16, b32;
-    int m4 = 0, m8 = 0, m16 = 0, m32 = 0;
-    double y4 = 0, y8 = 0, y16 = 0, y32 = 0;
+    std::vector<int> b4, b8, b16, b32, b4h;
+    std::vector<std::pair<int, int>> b4n;
+    int m4 = 0, m8 = 0, m16 = 0, m32 = 0, m4h = 0;
+    double y4 = 0, y8 = 0, y16 = 0, y32 = 0, y4h = 0;
     for (auto& q : idn) {
         const int n = q.second;
         const double y = 12.0 * n + 8.0;  // points read, volume written
@@ -1579,22 +1626,38 @@ int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pai
             const int c = p == 1 ? 0 : p == 2 ? 1 : p == 4 ? 2 : p == 8 ? 3 : 4;
             cls[c].push_back(q.first);
             cbytes[c] += y;
-        } else if (n <= 2048) { b4.push_back(q.first); m4 = std::max(m4, n); y4 += y; }
+        } else if (n <= 2048) { b4n.push_back(q); }
         else if (n <= 4096) { b8.push_back(q.first); m8 = std::max(m8, n); y8 += y; }
         else if (n <= 8192) { b16.push_back(q.first); m16 = std::max(m16, n); y16 += y; }
         else { b32.push_back(q.first); m32 = std::max(m32, n); y32 += y; }
+    }
+    // the 1025-2048-point tier, less the smallest b4_split percent of its sets
+    // (ties by task order), which go to the HBM-workspace kernel
+    {
+        const size_t nh = b4n.size() * (size_t)g_tune[11] / 100;
+        std::vector<int> pos(b4n.size());
+        for (size_t j = 0; j < pos.size(); ++j) pos[j] = (int)j;
+        std::stable_sort(pos.begin(), pos.end(), [&](int a, int b) { return b4n[a].second < b4n[b].second; });
+        std::vector<char> to_hbm(b4n.size(), 0);
+        for (size_t j = 0; j < nh; ++j) to_hbm[pos[j]] = 1;
+        for (size_t j = 0; j < b4n.size(); ++j) {
+            const int id = b4n[j].first, n = b4n[j].second;
+            const double y = 12.0 * n + 8.0;
+            if (to_hbm[j]) { b4h.push_back(id); m4h = std::max(m4h, n); y4h += y; }
+            else { b4.push_back(id); m4 = std::max(m4, n); y4 += y; }
+        }
     }
     // every tier's id list in one upload, before any launch of this call on
     // the context's stream (a pageable copy queued behind a running tier
     // would hold the host until that tier finishes)
     gpu_ctx_t ctx = hc.ctx;
-    const std::vector<int>* lists[9] = {&b32, &b16, &b8, &b4, &cls[4], &cls[3], &cls[2], &cls[1], &cls[0]};
-    size_t off[9], total = 0;
-    for (int k = 0; k < 9; ++k) { off[k] = total; total += lists[k]->size(); }
+    const std::vector<int>* lists[10] = {&b32, &b16, &b8, &b4, &cls[4], &cls[3], &cls[2], &cls[1], &cls[0], &b4h};
+    size_t off[10], total = 0;
+    for (int k = 0; k < 10; ++k) { off[k] = total; total += lists[k]->size(); }
     if (total == 0) return 0;
     std::vector<int> all;
     all.reserve(total);
-    for (int k = 0; k < 9; ++k) all.insert(all.end(), lists[k]->begin(), lists[k]->end());
+    for (int k = 0; k < 10; ++k) all.insert(all.end(), lists[k]->begin(), lists[k]->end());
     int* d_all = hc.alloc<int>(total);
     REQUIRE(d_all, "lookahead: id list allocation failed");
     HIP_TRY(hipMemcpyAsync(d_all, all.data(), 4 * total, hipMemcpyHostToDevice, ctx->stream));
@@ -1604,6 +1667,7 @@ int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pai
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(1), N(1), m16, y16)) return rc;
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(2), N(2), m8, y8)) return rc;
     if (int rc = launch_block_ids(hc, 1, d_tasks, L(3), N(3), m4, y4)) return rc;
+    if (int rc = launch_block_ids(hc, -1, d_tasks, L(9), N(9), m4h, y4h, nullptr, true)) return rc;
     const int ppls[5] = {1, 2, 4, 8, 16};
     if (int rc = launch_wave_ids(hc, 2, d_tasks, L(4), N(4), ppls[4], cbytes[4])) return rc;
     for (int c = 3; c >= 0; --c)
@@ -1655,6 +1719,7 @@ static int hull_oversized(gpu_ctx_t ctx, LaHullTask* d_tasks, std::vector<std::p
                 sub_of.push_back((int)i);
             }
         }
+        ctx->chunk_subtasks += (long long)sub.size();
         int* d_ids = dev_tmp<int>(ctx, over.size());
         int* d_off = dev_tmp<int>(ctx, over.size() + 1);
         LaHullTask* d_sub = dev_tmp<LaHullTask>(ctx, sub.size());
@@ -1770,6 +1835,11 @@ __device__ __forceinline__ int la_tier_of(int m) {
     return m <= 2048 ? 5 : m <= 4096 ? 6 : m <= 8192 ? 7 : 8;
 }
 
+// Without a chunk table every oversized set is left unhulled.
+__global__ void la_hdev_mark_over_kernel(int* counters) {
+    if (threadIdx.x == 0) counters[LA_HDEV_FAIL] += counters[LA_HDEV_OVER];
+}
+
 // The sets above kmin points go to the prefilter's list (any order: every
 // set is hulled on its own).
 __global__ void la_hdev_prelist_kernel(const LaHullTask* tasks, int n, int kmin, int* counters, int* big) {
@@ -1781,7 +1851,7 @@ __global__ void la_hdev_prelist_kernel(const LaHullTask* tasks, int n, int kmin,
 // the prefiltered ones by their filtered size (pruned sets and failed filters
 // drop out as in la_launch_hull); a set above `limit` is counted as oversized.
 __global__ void la_hdev_bin_kernel(const LaHullTask* tasks, int n, int kmin, int limit, const int* big,
-                                   const int2* res, int* counters, int* lists) {
+                                   const int2* res, int* counters, int* lists, int* over) {
     const int stride = gridDim.x * blockDim.x, g = blockIdx.x * blockDim.x + threadIdx.x;
     for (int i = g; i < n; i += stride) {
         const int m = tasks[i].n;
@@ -1794,7 +1864,7 @@ __global__ void la_hdev_bin_kernel(const LaHullTask* tasks, int n, int kmin, int
         const int2 r = res[j];
         if (r.y || r.x < 0) continue;
         if (r.x > limit) {
-            atomicAdd(&counters[LA_HDEV_OVER], 1);
+            over[atomicAdd(&counters[LA_HDEV_OVER], 1)] = big[j];
             continue;
         }
         const int t = la_tier_of(r.x);
@@ -1802,10 +1872,99 @@ __global__ void la_hdev_bin_kernel(const LaHullTask* tasks, int n, int kmin, int
     }
 }
 
+// Oversized sets: chunk tasks of chunk points each (contiguous in the table).
+__global__ void la_hdev_chunk_kernel(const LaHullTask* tasks, const int* over, int* counters, int chunk,
+                                     LaHullTask* sub, int cap, int2* range) {
+    const int no = counters[LA_HDEV_OVER];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < no; j += gridDim.x * blockDim.x) {
+        const LaHullTask& p = tasks[over[j]];
+        const float* P = p.filt ? p.filt : p.pts;
+        const int m = p.filt ? p.nf : p.n;
+        const int k = (m + chunk - 1) / chunk;
+        const int f = atomicAdd(&counters[LA_HDEV_SUBN], k);
+        if (f + k > cap) {
+            atomicAdd(&counters[LA_HDEV_FAIL], 1);
+            range[j] = make_int2(-1, 0);
+            continue;
+        }
+        for (int q = 0; q < k; ++q) {
+            LaHullTask c{};
+            c.pts = P + 3ll * q * chunk;
+            c.n = min(chunk, m - q * chunk);
+            c.emit = 1;
+            c.prune_gap = 1e30;
+            c.frame = p.frame;
+            c.use_frame = 1;
+            sub[f + q] = c;
+        }
+        range[j] = make_int2(f, k);
+    }
+}
+
+// Every chunk task empty (n = 0) before a replay fills the table.
+__global__ void la_hdev_clear_kernel(LaHullTask* sub, int cap) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) sub[i].n = 0;
+}
+
+// One workgroup per oversized set: the union of its chunks' corners (a chunk
+// without a hull passes its points through) replaces the set's filtered
+// points, as la_union_kernel does; the set then goes to its tier's list, or
+// counts as unhulled when it is still above the limit.
+__global__ __launch_bounds__(256) void la_hdev_union_kernel(LaHullTask* tasks, const int* over, int* counters,
+                                                            const int2* range, const LaHullTask* sub, int limit,
+                                                            int n, int* lists, DPool* pool, int* err) {
+    __shared__ unsigned long long sp;
+    __shared__ int sfail;
+    const int no = counters[LA_HDEV_OVER];
+    for (int j = blockIdx.x; j < no; j += gridDim.x) {
+        __syncthreads();
+        const int2 rg = range[j];
+        if (rg.x < 0) continue;
+        LaHullTask& t = tasks[over[j]];
+        int total = 0, bad = 0;
+        for (int q = rg.x; q < rg.x + rg.y; ++q) {
+            total += sub[q].hv ? sub[q].hnv : sub[q].n;
+            bad |= !sub[q].hv && sub[q].err != 0;
+        }
+        if (threadIdx.x == 0) {
+            sfail = bad || total > limit;
+            sp = sfail ? 0ull : (unsigned long long)heap_alloc(pool, 1, 12ull * total + 16, err);
+        }
+        __syncthreads();
+        float* dst = (float*)sp;
+        if (dst) {
+            int o = 0;
+            for (int q = rg.x; q < rg.x + rg.y; ++q) {
+                const float* v = sub[q].hv ? sub[q].hv : sub[q].pts;
+                const int m = sub[q].hv ? sub[q].hnv : sub[q].n;
+                for (int i = threadIdx.x; i < 3 * m; i += 256) dst[3 * o + i] = v[i];
+                o += m;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = rg.x; q < rg.x + rg.y; ++q)
+                if (sub[q].hv) heap_free(pool, sub[q].hv, err);
+            if (sfail) {
+                atomicAdd(&counters[LA_HDEV_FAIL], 1);
+            } else if (!dst) {
+                t.err = KERR_KDOP_SCRATCH_OOM;
+                err_or(err, KERR_KDOP_SCRATCH_OOM);
+            } else {
+                if (t.filt) heap_free(pool, t.filt, err);
+                t.filt = dst;
+                t.nf = total;
+                const int tr = la_tier_of(total);
+                lists[(size_t)tr * n + atomicAdd(&counters[1 + tr], 1)] = over[j];
+            }
+        }
+    }
+}
+
 size_t la_hull_dev_work_bytes(int n) {
     const size_t nn = (size_t)std::max(n, 1);
     return 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256 +
-           (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256;
+           (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256;
 }
 
 size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n) {
@@ -1820,7 +1979,8 @@ size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n) {
     return b;
 }
 
-int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes) {
+int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes,
+                       const LaHullDevChunks* chunks, bool prefilter) {
     if (n <= 0) return 0;
     if (int rc = ensure_kdop_axes(ctx)) return rc;
     tune_init();
@@ -1829,18 +1989,44 @@ int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, ch
     int* big = (int*)(work + 256);
     int2* res = (int2*)(work + 256 + (4 * nn + 255) / 256 * 256);
     int* lists = (int*)((char*)res + (8 * nn + 255) / 256 * 256);
-    const int kmin = std::min(kdop_min_points(), LA_WAVE_MAX);
+    int* over = (int*)((char*)lists + (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256);
+    int2* range = (int2*)((char*)over + (4 * nn + 255) / 256 * 256);
+    // without the prefilter every set is binned by its own size (the chunk
+    // hulls of oversized sets, as hull_oversized hulls them)
+    const int kmin = prefilter ? std::min(kdop_min_points(), LA_WAVE_MAX) : 0x7fffffff;
     const unsigned gb = (unsigned)std::max(1, std::min(1024, (n + 255) / 256));
     if (int rc = zero_ints(ctx, counters, LA_HDEV_NCOUNT, ctx->stream)) return rc;
-    hipLaunchKernelGGL(la_hdev_prelist_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks, n,
-                       kmin, counters, big);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)n), dim3(KD_T), 0, ctx->stream, d_tasks, (const int*)big, n,
-                       (const int*)(counters + LA_HDEV_BIG), res, ctx->d_pool, ctx->d_err);
-    HIP_TRY(hipGetLastError());
+    if (prefilter) {
+        hipLaunchKernelGGL(la_hdev_prelist_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks,
+                           n, kmin, counters, big);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)n), dim3(KD_T), 0, ctx->stream, d_tasks, (const int*)big,
+                           n, (const int*)(counters + LA_HDEV_BIG), res, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(la_hdev_bin_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks, n, kmin,
-                       g_tune[4], (const int*)big, (const int2*)res, counters, lists);
+                       g_tune[4], (const int*)big, (const int2*)res, counters, lists, over);
     HIP_TRY(hipGetLastError());
+    if (chunks && chunks->cap > 0) {
+        // oversized sets: chunk hulls (a nested device-sized call), then unions
+        const int cap = chunks->cap;
+        const unsigned gc = (unsigned)std::max(1, std::min(1024, (cap + 255) / 256));
+        hipLaunchKernelGGL(la_hdev_clear_kernel, dim3(gc), dim3(256), 0, ctx->stream, chunks->sub, cap);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(la_hdev_chunk_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks,
+                           (const int*)over, counters, g_tune[4] / 2, chunks->sub, cap, range);
+        HIP_TRY(hipGetLastError());
+        if (int rc = la_launch_hull_dev(ctx, chunks->sub, cap, chunks->work, chunks->arena, chunks->arena_bytes,
+                                        nullptr, false))
+            return rc;
+        hipLaunchKernelGGL(la_hdev_union_kernel, dim3((unsigned)std::max(1, std::min(n, 1024))), dim3(256), 0,
+                           ctx->stream, d_tasks, (const int*)over, counters, (const int2*)range,
+                           (const LaHullTask*)chunks->sub, g_tune[4], n, lists, ctx->d_pool, ctx->d_err);
+        HIP_TRY(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(la_hdev_mark_over_kernel, dim3(1), dim3(64), 0, ctx->stream, counters);
+        HIP_TRY(hipGetLastError());
+    }
     HullCall hc(ctx);
     hc.arena = arena;
     hc.arena_cap = arena_bytes;

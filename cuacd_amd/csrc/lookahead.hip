@@ -1619,8 +1619,7 @@ struct Engine {
                     for (size_t u = 0; u < qs.size(); ++u) {
                         const LaScoreTask& t = h[u];
                         const Cand& cd = cands[qs[u]];
-                        if (t.ppts) pts_free.push_back(t.ppts);
-                        if (t.npts) pts_free.push_back(t.npts);
+                        if (t.ppts) pts_free.push_back(t.ppts);  // the negative set shares its chunk
                         if (t.npt == 0 || t.nnt == 0 || !t.ppts || !t.npts) continue;
                         const Item& par = cur[cd.item];
                         // pruning threshold (cuda/la_expand.cu:380-399)

@@ -63,8 +63,8 @@ struct LaScoreTask {
     // results
     double pvol, nvol;   // half-space volumes (divergence origin on the plane)
     int npt, nnt;        // triangles (whole or split pieces) on each side; 0 = empty side
-    float* ppts; int np_pts;
-    float* npts; int nn_pts;
+    float* ppts; int np_pts;  // point sets of the two sides: one heap chunk,
+    float* npts; int nn_pts;  // freed through ppts (npts points into it)
 };
 
 // Connected components of one part (union-find over triangle vertices).
@@ -165,11 +165,30 @@ int la_launch_free(gpu_ctx_t ctx, void** d_ptrs, int n);
 // host-driven path (la_launch_hull) instead.
 constexpr int LA_HDEV_TIERS = 9;
 constexpr int LA_HDEV_BIG = 0;                       // counters: [0] sets to prefilter,
-constexpr int LA_HDEV_OVER = 1 + LA_HDEV_TIERS;      // [1, 10) per tier, [10] oversized sets
+constexpr int LA_HDEV_OVER = 1 + LA_HDEV_TIERS;      // [1, 10) per tier, [10] oversized sets,
+constexpr int LA_HDEV_SUBN = LA_HDEV_OVER + 1;       // [11] chunk tasks used,
+constexpr int LA_HDEV_FAIL = LA_HDEV_SUBN + 1;       // [12] sets left unhulled (see below)
 constexpr int LA_HDEV_NCOUNT = 16;
+// Oversized sets (above chunk_above points after the prefilter) are hulled in
+// chunks of half that size first, as la_launch_hull does, when the caller gives
+// a chunk table: the chunk hulls run as a nested device-sized call, then each
+// set becomes the union of its chunks' corners. A set whose union is still
+// too large (la_launch_hull would chunk it again) or whose chunks do not fit
+// the table is left unhulled and counted in counter LA_HDEV_FAIL; the caller
+// must then take the host-driven path.
+struct LaHullDevChunks {
+    LaHullTask* sub = nullptr;  // chunk task table
+    int cap = 0;                // its capacity
+    char* work = nullptr;       // la_hull_dev_work_bytes(cap)
+    char* arena = nullptr;      // la_hull_dev_arena_bytes(ctx, cap)
+    size_t arena_bytes = 0;
+};
 size_t la_hull_dev_work_bytes(int n);
 size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n);
-int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes);
+// prefilter = false skips the k-DOP pass (the chunk hulls of oversized sets,
+// which la_launch_hull also hulls unfiltered, keeping their set's frame).
+int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, char* arena, size_t arena_bytes,
+                       const LaHullDevChunks* chunks = nullptr, bool prefilter = true);
 
 }  // namespace cuacd
 

@@ -405,7 +405,7 @@ __global__ void sc_free_kernel(const LaCutTask* ct, const LaHullTask* ht, int n_
 
 struct score_plan {
     gpu_ctx_t ctx = nullptr;
-    int n_cuts = 0, max_nv = 0, hd_cap = 0, chunks = 1;
+    int n_cuts = 0, max_nv = 0, hd_cap = 0, hd_chunks = 1;
     float tau = 0.0f;
     double g_max = 0.0;
     float* dv = nullptr;
@@ -415,6 +415,7 @@ struct score_plan {
     char* hwork = nullptr;
     char* harena = nullptr;
     size_t harena_bytes = 0;
+    LaHullDevChunks chunks;  // chunk table for the oversized halves (cap 0: none seen)
     HdTask* dhd = nullptr;
     float* dd2 = nullptr;
     int* dhalf = nullptr;
@@ -433,7 +434,8 @@ void plan_free(score_plan* p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void* bufs[] = {p->dv, p->dt, p->dct, p->dht, p->hwork, p->harena, p->dhd, p->dd2, p->dhalf, p->dcounts, p->dws};
+    void* bufs[] = {p->dv,  p->dt,  p->dct,    p->dht,     p->hwork, p->harena,         p->dhd,
+                    p->dd2, p->dhalf, p->dcounts, p->dws, p->chunks.sub, p->chunks.work, p->chunks.arena};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -448,7 +450,8 @@ int plan_launch(score_plan* p) {
     const unsigned g = (unsigned)std::max(1, std::min(1024, (nh + 255) / 256));
     hipLaunchKernelGGL(sc_hull_tasks_kernel, dim3(g), dim3(256), 0, ctx->stream, (const LaCutTask*)p->dct, nc, p->dht);
     HIP_TRY(hipGetLastError());
-    LA_SC_TRY(la_launch_hull_dev(ctx, p->dht, nh, p->hwork, p->harena, p->harena_bytes));
+    LA_SC_TRY(la_launch_hull_dev(ctx, p->dht, nh, p->hwork, p->harena, p->harena_bytes,
+                                 p->chunks.cap > 0 ? &p->chunks : nullptr));
     unsigned long long* ws_top = (unsigned long long*)(p->dcounts + 4);
     hipLaunchKernelGGL(sc_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, p->dcounts, ws_top);
     HIP_TRY(hipGetLastError());
@@ -456,7 +459,7 @@ int plan_launch(score_plan* p) {
                        (const LaHullTask*)p->dht, nc, p->g_max, p->dhd, p->dd2, p->dhalf, p->dcounts, ws_top,
                        p->ws_cap, p->hd_cap);
     HIP_TRY(hipGetLastError());
-    LA_SC_TRY(hd_launch_tasks_dev(ctx, p->dhd, p->hd_cap, p->dcounts, p->chunks, p->dws));
+    LA_SC_TRY(hd_launch_tasks_dev(ctx, p->dhd, p->hd_cap, p->dcounts, p->hd_chunks, p->dws));
     hipLaunchKernelGGL(sc_free_kernel, dim3(g), dim3(256), 0, ctx->stream, (const LaCutTask*)p->dct,
                        (const LaHullTask*)p->dht, nc, ctx->d_pool, ctx->d_err);
     HIP_TRY(hipGetLastError());
@@ -472,8 +475,8 @@ int plan_finish(score_plan* p, float* out_score, double* out_detail) {
     HIP_TRY(hipMemcpyAsync(p->ct.data(), p->dct, sizeof(LaCutTask) * nc, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(p->ht.data(), p->dht, sizeof(LaHullTask) * nh, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(counts, p->dcounts, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
-    int over = 0;
-    HIP_TRY(hipMemcpyAsync(&over, p->hwork + 4 * LA_HDEV_OVER, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int over = 0;  // halves left unhulled by the device-sized call
+    HIP_TRY(hipMemcpyAsync(&over, p->hwork + 4 * LA_HDEV_FAIL, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const int e = read_and_clear_err(ctx);
     if (e) {
@@ -526,9 +529,11 @@ extern "C" int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const in
     // one eager run: validates the batch, sizes the Hausdorff stage
     std::vector<float> s0(n_cuts);
     ScoreSizes sz;
+    const long long chunks0 = ctx->chunk_subtasks;
     if (int rc = score_run(ctx, verts, vert_off, tris, tri_off, n_meshes, mesh_of, planes, n_cuts, tau, s0.data(),
                            nullptr, nullptr, &sz))
         return rc;
+    const long long n_chunks = ctx->chunk_subtasks - chunks0;
     score_plan* p = new score_plan();
     p->ctx = ctx;
     p->n_cuts = n_cuts;
@@ -540,7 +545,7 @@ extern "C" int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const in
     const size_t tv = (size_t)vert_off[n_meshes], tt = (size_t)tri_off[n_meshes];
     const int nh = 2 * n_cuts;
     p->hd_cap = 2 * nh;
-    p->chunks = std::max(1, std::min(64, (4096 + std::max(sz.hd_tasks, 1) - 1) / std::max(sz.hd_tasks, 1)));
+    p->hd_chunks = std::max(1, std::min(64, (4096 + std::max(sz.hd_tasks, 1) - 1) / std::max(sz.hd_tasks, 1)));
     p->ws_cap = sz.hd_ws + sz.hd_ws / 4 + (1ull << 20);
     p->harena_bytes = la_hull_dev_arena_bytes(ctx, nh);
     bool ok = hipMalloc(&p->dv, 12 * tv + 16) == hipSuccess && hipMalloc(&p->dt, 12 * tt + 16) == hipSuccess &&
@@ -552,6 +557,15 @@ extern "C" int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const in
               hipMalloc(&p->dd2, 4ull * p->hd_cap) == hipSuccess &&
               hipMalloc(&p->dhalf, 4ull * (p->hd_cap / 2) + 16) == hipSuccess &&
               hipMalloc(&p->dcounts, 64) == hipSuccess && hipMalloc(&p->dws, p->ws_cap + 256) == hipSuccess;
+    if (ok && n_chunks > 0) {
+        // the eager run hulled n_chunks chunks of oversized halves
+        LaHullDevChunks& c = p->chunks;
+        c.cap = (int)std::min<long long>(n_chunks + n_chunks / 4 + 16, 1 << 30);
+        c.arena_bytes = la_hull_dev_arena_bytes(ctx, c.cap);
+        ok = hipMalloc(&c.sub, sizeof(LaHullTask) * c.cap) == hipSuccess &&
+             hipMalloc(&c.work, la_hull_dev_work_bytes(c.cap)) == hipSuccess &&
+             hipMalloc(&c.arena, c.arena_bytes) == hipSuccess;
+    }
     if (!ok) {
         plan_free(p);
         set_error(ctx, "score_plan: device allocation failed");

@@ -275,7 +275,9 @@ void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
  * cuts of meshes with up to this many vertices keep their per-vertex state
  * in LDS and write no per-triangle scratch; 0 = none, else 256..110592;
  * default 98304; the same halves either way), 10 hull_lpt (each hull tier's
- * sets launched largest first, 0/1; default 0).
+ * sets launched largest first, 0/1; default 0), 11 hull_b4_split (percent
+ * of the 1025-2048-point sets, the smallest, hulled by the HBM-workspace
+ * kernel beside the LDS tier, 0..90; default 0).
  * Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);

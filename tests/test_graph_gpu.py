@@ -115,6 +115,8 @@ def test_score_plan_replays_match_eager_scoring(ctx):
                                                                              C.POINTER(C.c_void_p)]
     lib.gpu_score_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.gpu_score_plan_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gpu_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.gpu_get_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
 
     meshes = sdf.sdf_meshes(6, seed=21, res=32) + datasets.noisy_icospheres(4, 3, 0.15, seed=3)
     rng = np.random.default_rng(9)
@@ -145,31 +147,44 @@ def test_score_plan_replays_match_eager_scoring(ctx):
 
     g = C.c_void_p()
     assert lib.gpu_init(C.byref(g), 0, C.c_size_t(1 << 30)) == 0
+    chunk_above = C.c_int()
+    assert lib.gpu_get_tuning(g, 4, C.byref(chunk_above)) == 0
     try:
-        h0 = np.zeros(6, np.uint64)
-        assert lib.gpu_heap_stats(g, h0.ctypes.data) == 0
-        want_s, want_d = eager(g, slice(None))
-        assert np.isfinite(want_s).sum() > n // 2 and (want_d[:, 7:9] > 0).any()  # some halves measure H
-        plan = C.c_void_p()
-        rc = lib.gpu_score_plan_create(g, allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
-                                       len(meshes), mesh_of.ctypes.data, planes.ctypes.data, n, tau,
-                                       C.byref(plan))
-        assert rc == 0, lib.gpu_last_error(g)
-        try:
-            for rep in range(3):
-                sc = np.full(n, -7.0, np.float32)
-                dt = np.full((n, 10), -7.0, np.float64)
-                rc = lib.gpu_score_plan_run(g, plan, sc.ctypes.data, dt.ctypes.data)
-                assert rc == 0, lib.gpu_last_error(g)
-                assert sc.tobytes() == want_s.tobytes(), rep
-                assert dt.tobytes() == want_d.tobytes(), rep
-                # eager scoring of another batch between the replays
-                s2, _ = eager(g, slice(0, n, 3))
-                assert s2.tobytes() == want_s[0:n:3].tobytes()
-        finally:
-            lib.gpu_score_plan_destroy(g, plan)
-        h1 = np.zeros(6, np.uint64)
-        assert lib.gpu_heap_stats(g, h1.ctypes.data) == 0
-        assert h1[0] == h0[0] and h1[3] == h0[3], (h0, h1)
+        _check_plan(lib, g, eager, allv, voff, allt, toff, len(meshes), mesh_of, planes, n, tau)
+        # the oversized-set path: with chunk_above at 1024 the halves above it
+        # are hulled in chunks (on the device in the plan, by the host's loop
+        # in the eager run) and must still give the same scores
+        assert lib.gpu_set_tuning(g, 4, 1024) == 0
+        _check_plan(lib, g, eager, allv, voff, allt, toff, len(meshes), mesh_of, planes, n, tau)
     finally:
+        lib.gpu_set_tuning(g, 4, chunk_above.value)
         lib.gpu_destroy(g)
+
+
+def _check_plan(lib, g, eager, allv, voff, allt, toff, n_meshes, mesh_of, planes, n, tau):
+    """A plan of the whole batch: three replays equal to the eager scores and
+    detail, eager work between them, and the heap balanced afterwards."""
+    h0 = np.zeros(6, np.uint64)
+    assert lib.gpu_heap_stats(g, h0.ctypes.data) == 0
+    want_s, want_d = eager(g, slice(None))
+    assert np.isfinite(want_s).sum() > n // 2 and (want_d[:, 7:9] > 0).any()  # some halves measure H
+    plan = C.c_void_p()
+    rc = lib.gpu_score_plan_create(g, allv.ctypes.data, voff.ctypes.data, allt.ctypes.data, toff.ctypes.data,
+                                   n_meshes, mesh_of.ctypes.data, planes.ctypes.data, n, tau, C.byref(plan))
+    assert rc == 0, lib.gpu_last_error(g)
+    try:
+        for rep in range(3):
+            sc = np.full(n, -7.0, np.float32)
+            dt = np.full((n, 10), -7.0, np.float64)
+            rc = lib.gpu_score_plan_run(g, plan, sc.ctypes.data, dt.ctypes.data)
+            assert rc == 0, lib.gpu_last_error(g)
+            assert sc.tobytes() == want_s.tobytes(), rep
+            assert dt.tobytes() == want_d.tobytes(), rep
+            # eager scoring of another batch between the replays
+            s2, _ = eager(g, slice(0, n, 3))
+            assert s2.tobytes() == want_s[0:n:3].tobytes()
+    finally:
+        lib.gpu_score_plan_destroy(g, plan)
+    h1 = np.zeros(6, np.uint64)
+    assert lib.gpu_heap_stats(g, h1.ctypes.data) == 0
+    assert h1[0] == h0[0] and h1[3] == h0[3], (h0, h1)
