@@ -141,6 +141,8 @@ void gpu_destroy(gpu_ctx_t ctx) {
         if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
     }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->ev_tier[i]) (void)hipEventDestroy(ctx->ev_tier[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     free(ctx);
 }
@@ -177,6 +179,7 @@ int gpu_init(gpu_ctx_t* out, int device, size_t pool_bytes) {
         HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming));
     }
     HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) HIP_TRY(hipEventCreateWithFlags(&ctx->ev_tier[i], hipEventDisableTiming));
     HIP_TRY(hipEventCreate(&ctx->ev_a));
     HIP_TRY(hipEventCreate(&ctx->ev_b));
     size_t free_b = 0, total_b = 0;

@@ -60,6 +60,7 @@ struct gpu_ctx {
     int proposal_topk;
     void* record;  // level-0 training record of the search (lookahead.hip), null when off
     long long chunk_subtasks;  // chunk hulls of oversized sets, counted since gpu_init
+    hipEvent_t ev_tier[2];     // first passes of the eight-wave LDS hull tiers (launch order)
 };
 
 namespace cuacd {

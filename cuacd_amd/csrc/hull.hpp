@@ -112,12 +112,15 @@ template <int AS> struct HullWS {
     asptr<AS_LDS, uint16_t> mlist;  // [MA_W][MA_CAP][5]: visible faces, ha, hb, hface, hnew (ranks >= 1)
     asptr<AS_LDS, uint8_t> mhedge;  // [MA_W][MA_CAP]
     asptr<AS_LDS, int> mcand;       // [MA_W][MA_CI] candidate table, then [MA_W] conflict rows, misc
+    asptr<AS_LDS, f4v> mcpl;        // [MA_W][MA_CPL] planes of each accepted cone's faces, horizon order
     int VC, FC, NP;
 };
 
 // Multi-apex rounds: at most MA_W candidates, MA_CAP visible faces and
 // horizon edges for each candidate but the first (which uses the main arrays).
-constexpr int MA_W = 8, MA_CAP = 128, MA_CI = 12;
+// The planes of a cone of at most MA_CPL faces are also kept in cone order, so
+// that the points of the faces it replaced scan them without an id lookup.
+constexpr int MA_W = 8, MA_CAP = 128, MA_CI = 12, MA_CPL = 32;
 
 // FC = 2*VC + 8 faces bound the live faces of a hull with VC vertices plus the
 // transient pair created by one insertion (nhz <= nvis + 2). Face and vertex
@@ -164,6 +167,7 @@ __host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int
         w.mlist = (asptr<AS_LDS, uint16_t>)take(MA_W * MA_CAP * 5 * 2u);
         w.mhedge = (asptr<AS_LDS, uint8_t>)take(MA_W * MA_CAP);
         w.mcand = (asptr<AS_LDS, int>)take((MA_W * MA_CI + MA_W + 16) * 4u);
+        w.mcpl = (asptr<AS_LDS, f4v>)take(MA_W * MA_CPL * 16u);
     }
     (void)nw;  // bm / wpre / ctr live in HullBlockLDS (hull_ws_attach)
     w.VC = vc;
@@ -255,7 +259,7 @@ __device__ __forceinline__ bool face_above(const HullWS<AS> w, int f, int qx, in
 }
 
 template <int AS>
-__device__ __forceinline__ void face_set_plane(const HullWS<AS> w, int f, s4v a, s4v b, s4v c) {
+__device__ __forceinline__ f4v face_set_plane(const HullWS<AS> w, int f, s4v a, s4v b, s4v c) {
     const int bx = b.x - a.x, by = b.y - a.y, bz = b.z - a.z;
     const int cx = c.x - a.x, cy = c.y - a.y, cz = c.z - a.z;
     // exact in int32: |components| < 2^29
@@ -266,7 +270,9 @@ __device__ __forceinline__ void face_set_plane(const HullWS<AS> w, int f, s4v a,
     const float il = l2 > 0.0f ? rsqrtf(l2) : 0.0f;
     const float ux = nx * il, uy = ny * il, uz = nz * il;
     const float ww = -fmaf(ux, (float)a.x, fmaf(uy, (float)a.y, uz * (float)a.z));
-    w.fpl[f] = mk_f4(ux, uy, uz, ww);
+    const f4v pl = mk_f4(ux, uy, uz, ww);
+    w.fpl[f] = pl;
+    return pl;
 }
 
 template <class P> __device__ __forceinline__ int atom_exch(P p, int v) {
@@ -515,6 +521,10 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
         // be non-extreme points (their distances are quantised); the exact
         // incremental construction still ends at the unique hull.
         static_assert(AS == AS_LDS && NW > 1 && NW <= MA_W, "multi-apex rounds need an LDS workspace and several waves");
+        // a conflict key is the face's best point: its quantised distance
+        // above the high bits, its (inverted) local index in the low ones
+        static_assert(PPL * T <= 4096, "multi-apex keys hold point indices below 4096");
+        constexpr uint32_t IMASK = PPL * T <= 2048 ? 0x7FFu : 0xFFFu;
         const int wv = tid >> 6, lane = tid & 63;
         asptr<AS_LDS, int> cand = w.mcand;
         asptr<AS_LDS, int> crow = w.mcand + MA_W * MA_CI;
@@ -537,7 +547,7 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
 #pragma unroll
             for (int k = 0; k < PPL; ++k)
                 if (conf[k] >= 0) {
-                    const uint32_t key = (fkey(dist[k]) & ~0x7FFu) | (0x7FFu - (uint32_t)(k * T + tid));
+                    const uint32_t key = (fkey(dist[k]) & ~IMASK) | (IMASK - (uint32_t)(k * T + tid));
                     __hip_atomic_fetch_max(&w.mfmax[conf[k]], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             __syncthreads();
@@ -553,125 +563,130 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
             if (lane == 0) sb->red[wv] = bk;
             __syncthreads();
             uint64_t rk[NW];
-#pragma unroll
-            for (int q = 0; q < NW; ++q) rk[q] = sb->red[q];
-            // rank order: descending keys (insertion sort, identical in every thread)
-#pragma unroll
-            for (int a = 1; a < NW; ++a)
-#pragma unroll
-                for (int b = a; b > 0; --b)
-                    if (rk[b] > rk[b - 1]) { const uint64_t t = rk[b]; rk[b] = rk[b - 1]; rk[b - 1] = t; }
-            if (rk[0] == 0) break;
             int ncand = 0;
 #pragma unroll
-            for (int q = 0; q < NW; ++q) ncand += rk[q] != 0;
-            // the owners of the candidate points publish them
+            for (int q = 0; q < NW; ++q) {
+                rk[q] = sb->red[q];
+                ncand += rk[q] != 0;
+            }
+            if (ncand == 0) break;
+            // the owners of the candidate points publish them at their rank,
+            // in descending key order (keys differ: they hold distinct face ids)
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 if (!rk[q]) continue;
-                const int ip = (int)(0x7FFu - ((uint32_t)(rk[q] >> 32) & 0x7FFu));
+                const int ip = (int)(IMASK - ((uint32_t)(rk[q] >> 32) & IMASK));
                 if (tid == ip % T) {
+                    int rank = 0;
+#pragma unroll
+                    for (int o = 0; o < NW; ++o) rank += rk[o] > rk[q];
                     const int slot = ip / T;
-                    cand[q * MA_CI + 0] = (int)(uint32_t)(rk[q] >> 32);
-                    cand[q * MA_CI + 1] = (int)(uint32_t)rk[q];
-                    cand[q * MA_CI + 2] = ip;
-                    cand[q * MA_CI + 3] = pick(px, slot);
-                    cand[q * MA_CI + 4] = pick(py, slot);
-                    cand[q * MA_CI + 5] = pick(pz, slot);
+                    cand[rank * MA_CI + 0] = (int)(uint32_t)(rk[q] >> 32);
+                    cand[rank * MA_CI + 1] = (int)(uint32_t)rk[q];
+                    cand[rank * MA_CI + 2] = ip;
+                    cand[rank * MA_CI + 3] = pick(px, slot);
+                    cand[rank * MA_CI + 4] = pick(py, slot);
+                    cand[rank * MA_CI + 5] = pick(pz, slot);
                 }
             }
             __syncthreads();
-            // D: wave r finds candidate r's visible region and horizon
+            // D: wave r finds candidate r's visible region and, on the way, its
+            // horizon: every visible face tests its three neighbours at once
+            // (visit marks, planes and back links loaded together); the first
+            // visitor of a visible neighbour queues it, and every neighbour
+            // not visible gives a horizon edge (its test is deterministic, so
+            // every lane that reaches it agrees)
             if (wv < ncand) {
                 const int r = wv;
                 const int f0 = cand[r * MA_CI + 1];
                 const int qx = cand[r * MA_CI + 3], qy = cand[r * MA_CI + 4], qz = cand[r * MA_CI + 5];
+                const float fqx = (float)qx, fqy = (float)qy, fqz = (float)qz;
                 asptr<AS_LDS, uint16_t> V = r == 0 ? (asptr<AS_LDS, uint16_t>)w.vis : lvis(r);
                 const int vcap = r == 0 ? FC : MA_CAP;
+                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
+                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
+                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
+                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
+                const int hcap = r == 0 ? VC : MA_CAP;
                 const uint32_t vb = 1u << r, sbit = 1u << (8 + r);
                 if (lane == 0) {
                     __hip_atomic_fetch_or(&w.mfm[f0], vb | sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     V[0] = (uint16_t)f0;
                 }
                 wsync();
-                int head = 0, nv = 1;
+                int head = 0, nv = 1, nh = 0;
                 bool over = false;
                 while (head < nv) {
                     const int cnt = min(nv - head, 64);
-                    int g3[3] = {0, 0, 0};
-                    bool ap[3] = {false, false, false};
-                    if (lane < cnt) {
-                        const int f = V[head + lane];
-                        const u4v nb = w.fn[f];
-                        g3[0] = nb.x; g3[1] = nb.y; g3[2] = nb.z;
+                    const bool act = lane < cnt;
+                    int f = 0;
+                    u4v fvl = {0, 0, 0, 0}, nb = {0, 0, 0, 0};
+                    if (act) {
+                        f = V[head + lane];
+                        fvl = w.fv[f];
+                        nb = w.fn[f];
+                    }
+                    const int gg[3] = {nb.x, nb.y, nb.z};
+                    const int vv[3] = {fvl.x, fvl.y, fvl.z};
+                    uint32_t old[3] = {vb, vb, vb};
+                    bool vis[3] = {false, false, false};
+                    int back[3] = {0, 0, 0};
+                    if (act) {
+                        f4v pl[3];
+                        u4v gn[3];
 #pragma unroll
                         for (int e = 0; e < 3; ++e) {
-                            const uint32_t old = __hip_atomic_fetch_or(&w.mfm[g3[e]], vb, __ATOMIC_RELAXED,
-                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                            if (!(old & vb)) {
-                                float t;
-                                if (face_above(w, g3[e], qx, qy, qz, &t)) {
-                                    __hip_atomic_fetch_or(&w.mfm[g3[e]], sbit, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                                    ap[e] = true;
-                                }
+                            old[e] = __hip_atomic_fetch_or(&w.mfm[gg[e]], vb, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                            pl[e] = w.fpl[gg[e]];
+                            gn[e] = w.fn[gg[e]];
+                        }
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            // face_above on the loaded plane
+                            const float t = fmaf(pl[e].x, fqx, fmaf(pl[e].y, fqy, fmaf(pl[e].z, fqz, pl[e].w)));
+                            bool above = t > HULL_FILTER_EPS;
+                            if (!above && t >= -HULL_FILTER_EPS) {
+                                const u4v gv = w.fv[gg[e]];
+                                above = orient_exact(w.vq[gv.x], w.vq[gv.y], w.vq[gv.z], qx, qy, qz) > 0.0;
                             }
+                            vis[e] = above;
+                            back[e] = (gn[e].x == f) ? 0 : ((gn[e].y == f) ? 1 : 2);
                         }
                     }
                     head += cnt;
                     int add = 0;
 #pragma unroll
                     for (int e = 0; e < 3; ++e) {
-                        const uint64_t m = ballot(ap[e]);
+                        const bool ap = act && vis[e] && !(old[e] & vb);
+                        if (ap)
+                            __hip_atomic_fetch_or(&w.mfm[gg[e]], sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint64_t m = ballot(ap);
                         const int pos = nv + add + mask_rank(m);
-                        if (ap[e]) {
-                            if (pos < vcap) V[pos] = (uint16_t)g3[e];
+                        if (ap) {
+                            if (pos < vcap) V[pos] = (uint16_t)gg[e];
                             else over = true;
                         }
                         add += __popcll(m);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const bool h = act && !vis[e];
+                        const uint64_t m = ballot(h);
+                        const int pos = nh + mask_rank(m);
+                        if (h && pos < hcap) {
+                            HA[pos] = (uint16_t)vv[e];
+                            HB[pos] = (uint16_t)vv[e == 2 ? 0 : e + 1];
+                            HF[pos] = (uint16_t)gg[e];
+                            HE[pos] = (uint8_t)back[e];
+                        }
+                        nh += __popcll(m);
                     }
                     nv += add;
                     wsync();
                     if (nv > vcap) { over = true; nv = vcap; break; }
                 }
                 over = ballot(over) != 0;
-                // horizon edges: visible face -> neighbour not visible from this candidate
-                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
-                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
-                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
-                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
-                const int hcap = r == 0 ? VC : MA_CAP;
-                int nh = 0;
-                if (!over) {
-                    for (int j0 = 0; j0 < nv; j0 += 64) {
-                        const int j = j0 + lane;
-                        int f = 0;
-                        u4v fvl = {0, 0, 0, 0}, nb = {0, 0, 0, 0};
-                        if (j < nv) {
-                            f = V[j];
-                            fvl = w.fv[f];
-                            nb = w.fn[f];
-                        }
-                        const int vv[3] = {fvl.x, fvl.y, fvl.z};
-                        const int gg[3] = {nb.x, nb.y, nb.z};
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            const bool h = j < nv && !(w.mfm[gg[e]] & sbit);
-                            const uint64_t m = ballot(h);
-                            const int pos = nh + mask_rank(m);
-                            if (h) {
-                                if (pos < hcap) {
-                                    const u4v gn = w.fn[gg[e]];
-                                    HA[pos] = (uint16_t)vv[e];
-                                    HB[pos] = (uint16_t)vv[e == 2 ? 0 : e + 1];
-                                    HF[pos] = (uint16_t)gg[e];
-                                    HE[pos] = (uint8_t)((gn.x == f) ? 0 : ((gn.y == f) ? 1 : 2));
-                                }
-                            }
-                            nh += __popcll(m);
-                        }
-                    }
-                }
                 if (lane == 0) {
                     cand[r * MA_CI + 6] = nv;
                     cand[r * MA_CI + 7] = nh;
@@ -692,49 +707,74 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                 }
             }
             __syncthreads();
-            // F: one thread accepts greedily by rank and assigns vertex and face ids
-            if (tid == 0) {
-                uint32_t acc = 0;
-                int nacc = 0, nf = nfree, ft = ftop, st = HULL_OK;
-                for (int r = 0; r < ncand; ++r) {
-                    const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
-                    const bool ok = cand[r * MA_CI + 8] != 0;
-                    if (!ok) {
+            // F: wave 0 accepts greedily by rank and assigns vertex and face
+            // ids: lane r loads candidate r's sizes and conflict row, the rank
+            // walk runs on scalar broadcasts, then each accepted lane writes
+            // its cone's extra face ids (from the free list, then the top)
+            if (wv == 0) {
+                const int nc = __builtin_amdgcn_readfirstlane(ncand);
+                int c_nv = 0, c_nh = 0, c_ok = 0, c_row = 0;
+                if (lane < nc) {
+                    c_nv = cand[lane * MA_CI + 6];
+                    c_nh = cand[lane * MA_CI + 7];
+                    c_ok = cand[lane * MA_CI + 8];
+                    c_row = crow[lane];
+                }
+                uint32_t acc = 0, blocked = 0;  // blocked: visited the visible faces of an accepted one
+                int nacc = 0, st = HULL_OK;
+                int nf = __builtin_amdgcn_readfirstlane(nfree), ft = __builtin_amdgcn_readfirstlane(ftop);
+                const int nv0 = __builtin_amdgcn_readfirstlane(nvert);
+                int my_vid = 0, my_free = 0, my_ff = 0, my_top = 0;
+                for (int r = 0; r < nc; ++r) {
+                    const int nv = read_lane(c_nv, r), nh = read_lane(c_nh, r);
+                    const uint32_t row = (uint32_t)read_lane(c_row, r);
+                    if (!read_lane(c_ok, r)) {
                         if (r == 0) { st = HULL_OVERFLOW; break; }
                         continue;
                     }
-                    bool clash = ((uint32_t)crow[r] & acc) != 0;
-                    for (int a = 0; a < r && !clash; ++a)
-                        if (((acc >> a) & 1u) && ((uint32_t)crow[a] & (1u << r))) clash = true;
-                    if (clash) continue;
+                    if ((row & acc) || ((blocked >> r) & 1u)) continue;
                     const int extra = nh > nv ? nh - nv : 0;
                     const int from_free = min(extra, nf);
                     const int from_top = extra - from_free;
-                    if (nvert + nacc >= VC || ft + from_top > FC) {
+                    if (nv0 + nacc >= VC || ft + from_top > FC) {
                         if (r == 0) { st = HULL_OVERFLOW; break; }
                         continue;
                     }
-                    asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
-                    for (int k = 0; k < from_free; ++k) HN[nv + k] = w.ffree[nf - 1 - k];
+                    if (lane == r) {
+                        my_vid = nv0 + nacc;
+                        my_free = nf;
+                        my_ff = from_free;
+                        my_top = ft;
+                    }
                     nf -= from_free;
-                    for (int k = 0; k < from_top; ++k) HN[nv + from_free + k] = (uint16_t)(ft + k);
                     ft += from_top;
-                    cand[r * MA_CI + 9] = nvert + nacc;
                     ++nacc;
                     acc |= 1u << r;
+                    blocked |= row;
                 }
                 // the surplus visible faces of the accepted cones go back to the free list
-                for (int r = 0; r < ncand; ++r)
+                int my_push = 0;
+                for (int r = 0; r < nc; ++r)
                     if ((acc >> r) & 1u) {
-                        const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
-                        cand[r * MA_CI + 10] = nf;
+                        const int nv = read_lane(c_nv, r), nh = read_lane(c_nh, r);
+                        if (lane == r) my_push = nf;
                         nf += nv > nh ? nv - nh : 0;
                     }
-                misc[0] = (int)acc;
-                misc[1] = nacc;
-                misc[2] = ft;
-                misc[3] = nf;
-                misc[4] = st;
+                if (lane < nc && ((acc >> lane) & 1u)) {
+                    asptr<AS_LDS, uint16_t> HN = lane == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(lane);
+                    const int extra = c_nh > c_nv ? c_nh - c_nv : 0;
+                    for (int k = 0; k < extra; ++k)
+                        HN[c_nv + k] = k < my_ff ? w.ffree[my_free - 1 - k] : (uint16_t)(my_top + k - my_ff);
+                    cand[lane * MA_CI + 9] = my_vid;
+                    cand[lane * MA_CI + 10] = my_push;
+                }
+                if (lane == 0) {
+                    misc[0] = (int)acc;
+                    misc[1] = nacc;
+                    misc[2] = ft;
+                    misc[3] = nf;
+                    misc[4] = st;
+                }
             }
             __syncthreads();
             const uint32_t acc = (uint32_t)misc[0];
@@ -783,7 +823,8 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                     const int id = HN[j];
                     w.fv[id] = mk_u4(a, c, vid, (uint16_t)it);
                     w.fn[id] = mk_u4(HF[j], HN[nx], HN[pv], 1);
-                    face_set_plane(w, id, w.vq[a], w.vq[c], sp);
+                    const f4v pl = face_set_plane(w, id, w.vq[a], w.vq[c], sp);
+                    if (nh <= MA_CPL) w.mcpl[r * MA_CPL + j] = pl;
                 }
                 wsync();
                 for (int j = lane; j < nh; j += 64) {
@@ -808,26 +849,50 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                     conf[k] = (conf[k] & ~m) | (-2 & m);
                 }
             }
+            // the lane's points whose face was replaced this round, as a bit
+            // mask: the wave then loops as often as its busiest lane needs,
+            // not once per point slot that any lane needs
+            uint32_t need = 0;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
                 const int c = conf[k];
-                if (c >= 0 && w.fv[c].w == (uint16_t)it) {
-                    const int r = w.mfcone[c];
-                    const int nh = cand[r * MA_CI + 7];
-                    asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
-                    int best = -1;
-                    float bd = 0.0f;
-                    for (int j = 0; j < nh; ++j) {
-                        const int id = HN[j];
-                        float t;
-                        if (face_above(w, id, px[k], py[k], pz[k], &t) && (best < 0 || t > bd)) {
-                            best = id;
-                            bd = t;
-                        }
+                if (c >= 0 && w.fv[c].w == (uint16_t)it) need |= 1u << k;
+            }
+            while (need) {
+                const int k = __ffs(need) - 1;
+                need &= need - 1;
+                const int c = pick(conf, k);
+                const int qx = pick(px, k), qy = pick(py, k), qz = pick(pz, k);
+                const int r = w.mfcone[c];
+                const int nh = cand[r * MA_CI + 7];
+                asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                // the cone's planes in cone order when it kept them (the same
+                // test as face_above, on the same plane values), else by id
+                const bool cp = nh <= MA_CPL;
+                const asptr<AS_LDS, f4v> CP = w.mcpl + r * MA_CPL;
+                const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
+                int bj = -1;
+                float bd = 0.0f;
+                for (int j = 0; j < nh; ++j) {
+                    const f4v pl = cp ? CP[j] : w.fpl[HN[j]];
+                    const float t = fmaf(pl.x, fx, fmaf(pl.y, fy, fmaf(pl.z, fz, pl.w)));
+                    bool above = t > HULL_FILTER_EPS;
+                    if (!above && t >= -HULL_FILTER_EPS) {
+                        const u4v v = w.fv[HN[j]];
+                        above = orient_exact(w.vq[v.x], w.vq[v.y], w.vq[v.z], qx, qy, qz) > 0.0;
                     }
-                    conf[k] = best;
-                    dist[k] = bd;
+                    if (above && (bj < 0 || t > bd)) {
+                        bj = j;
+                        bd = t;
+                    }
                 }
+                const int best = bj >= 0 ? (int)HN[bj] : -1;
+#pragma unroll
+                for (int kk = 0; kk < PPL; ++kk)
+                    if (kk == k) {
+                        conf[kk] = best;
+                        dist[kk] = bd;
+                    }
             }
         }
         res.nvert = nvert;

@@ -93,10 +93,11 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
     }
     HullResult r;
-    if constexpr (AS == AS_LDS && NW > 1) {
-        // multi-apex rounds (the workspace was carved with their scratch); the
-        // point index field of their keys holds 11 bits
-        if (multi && n <= 2048) r = block_quickhull<PPL, AS, NW, true>(w, sb, n, px, py, pz);
+    if constexpr (AS == AS_LDS && NW >= 8 && PPL * T <= 4096) {
+        // multi-apex rounds (the workspace was carved with their scratch; the
+        // four-wave tiers keep single-apex steps and do not compile them); the
+        // point index field of their keys holds 12 bits
+        if (multi && n <= PPL * T) r = block_quickhull<PPL, AS, NW, true>(w, sb, n, px, py, pz);
         else r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
     } else {
         r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
@@ -196,6 +197,25 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
         __syncthreads();
         LaHullTask& t = tasks[ids[j]];
         la_hull_item<PPL, AS_LDS, LA_LDS4_NW>(t, w, &sb, pool, err, nullptr, -1, multi != 0);
+        if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
+    }
+}
+
+// Eight waves per hull, full capacity in LDS, multi-apex rounds when `multi`:
+// the retries of the eight-wave first passes of the wave tiers (the sets
+// with the most corners). Strides over its list like la_hull_lds4.
+template <int PPL>
+__global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_lds8(LaHullTask* tasks, const int* ids, int n,
+                                                                const int* dcount, int multi, DPool* pool, int* err) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ HullBlockLDS sb;
+    const int nn = dcount ? min(*dcount, n) : n;
+    constexpr int CAP = 64 * LA_BLOCK_NW * PPL;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb);
+    for (int j = blockIdx.x; j < nn; j += gridDim.x) {
+        __syncthreads();
+        LaHullTask& t = tasks[ids[j]];
+        la_hull_item<PPL, AS_LDS, LA_BLOCK_NW>(t, w, &sb, pool, err, nullptr, -1, multi != 0);
         if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
     }
 }
@@ -1283,7 +1303,7 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //   4 chunk_above: prefiltered sets above this many points are hulled in
 //     chunks of half that size first (CUACD_HULL_CHUNK_ABOVE, 1024..16384;
 //     default 16384, the most one workgroup hulls)
-//   5 multi: multi-apex rounds in the eight-wave LDS tier (1025-2048 points;
+//   5 multi: multi-apex rounds in the eight-wave LDS tiers (1025-4096 points;
 //     CUACD_HULL_MULTI, 0/1; default 1)
 //   6 mlp_fused: the proposal network's inference runs its point layers and
 //     pooling as one kernel per candidate instead of layer-by-layer GEMMs
@@ -1313,8 +1333,24 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //     the HBM-workspace kernel on the context's stream beside the LDS tier's
 //     stream (CUACD_HULL_B4_SPLIT, 0..90; default 0): that kernel needs 5 KB
 //     of LDS, so its workgroups fit on the CUs the LDS tier fills, whose
-//     VALUs are mostly idle
-constexpr int LA_TUNE_N = 12;
+//     VALUs are mostly idle. On the 64-mesh search step 25 and 40 measured
+//     slower (2.43 s and 2.50 s against 2.27 s)
+//   12 order: the four-wave and wave tiers of a hull call wait for the first
+//     passes of the eight-wave LDS tiers (CUACD_HULL_ORDER, 0/1; default 1).
+//     An eight-wave LDS workgroup needs a whole CU's LDS, so it only starts on
+//     a CU that no smaller workgroup holds; launched beside the smaller tiers
+//     it waits for such CUs, and its launch stretches far beyond its work (on
+//     the 64-mesh search step the 1025-2048-point tier was in flight 583 ms
+//     for 178 ms of work per 256 workgroup slots; ordered, 322 ms)
+//   13 w8: first passes of the wave tiers of 513-1024 points (bit 0) and
+//     257-512 points (bit 1) on eight waves with multi-apex rounds instead of
+//     four waves with single-apex steps; bits 2 (513-1024 points) and 3
+//     (257-512 points): the same for their full-capacity retries
+//     (CUACD_HULL_W8, 0..15; default 5). On the 64-mesh search step a
+//     513-1024-point hull takes 0.38 ms on eight waves against 0.69 ms on
+//     four (two workgroups per CU against three), and the step went from
+//     2.05 s to 1.96 s; the 257-512-point tier packs better on four waves
+constexpr int LA_TUNE_N = 14;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1329,6 +1365,7 @@ int tune_clamp(int key, int v) {
         case 8: return v <= 0 ? 0 : std::min(1024, std::max(64, v / 32 * 32));
         case 9: return v <= 0 ? 0 : std::min(PC_LDS_VERTS_MAX, std::max(256, v));
         case 11: return std::min(90, std::max(0, v));
+        case 13: return std::min(15, std::max(0, v));
         default: return v != 0;
     }
 }
@@ -1338,8 +1375,9 @@ void tune_init() {
     const char* env[LA_TUNE_N] = {"CUACD_HULL_WIDE_MIN",   "CUACD_HULL_VCAP8", "CUACD_HULL_HALFCAP",
                                   "CUACD_KDOP_MIN",        "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI",
                                   "CUACD_MLP_FUSED",       "CUACD_HULL_LDSV8_VCAP",  "CUACD_HULL_LDSB8_VCAP",
-                                  "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT",         "CUACD_HULL_B4_SPLIT"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0};
+                                  "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT",         "CUACD_HULL_B4_SPLIT",
+                                  "CUACD_HULL_ORDER",      "CUACD_HULL_W8"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0, 1, 5};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1447,17 +1485,29 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
         // then the sets whose hulls did not fit, at full capacity
         using CapFn = void (*)(LaHullTask*, const int*, int, const int*, int, int*, int, DPool*, int*);
         using FullFn = void (*)(LaHullTask*, const int*, int, const int*, int, DPool*, int*);
-        CapFn kc = ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>;
-        FullFn kf = ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>;
+        // the w8 knob (bit 0: 513-1024 points, bit 1: 257-512 points) runs
+        // the first pass on eight waves with multi-apex rounds instead of
+        // four waves with single-apex steps
+        const bool w8 = (g_tune[13] >> (ppl == 16 ? 0 : 1)) & 1;
+        const int nw = w8 ? LA_BLOCK_NW : LA_LDS4_NW;
+        CapFn kc = w8 ? (ppl == 8 ? la_hull_ldsv<1, LA_BLOCK_NW> : la_hull_ldsv<2, LA_BLOCK_NW>)
+                      : (ppl == 8 ? la_hull_ldsv<2, LA_LDS4_NW> : la_hull_ldsv<4, LA_LDS4_NW>);
+        // bits 2 (513-1024 points) and 3 (257-512 points): the full-capacity
+        // retries on eight waves with multi-apex rounds
+        const bool w8f = (g_tune[13] >> (ppl == 16 ? 2 : 3)) & 1;
+        const int nwf = w8f ? LA_BLOCK_NW : LA_LDS4_NW;
+        FullFn kf = w8f ? (ppl == 8 ? la_hull_lds8<1> : la_hull_lds8<2>)
+                        : (ppl == 8 ? la_hull_lds4<2> : la_hull_lds4<4>);
         // the canonical hulls of these sets have few corners (mostly under a
         // quarter of the points), so 3/8 of the points fits nearly all
         const int vcap = (64 * ppl * hull_vcap_eighths()) / 8;
-        // single-apex steps: on these sets (513-1024 points) the larger LDS
-        // footprint of the multi-apex rounds costs more occupancy than the
-        // rounds save
-        const int multi = 0;
+        // four waves take single-apex steps: on these sets (257-1024 points)
+        // the larger LDS footprint of multi-apex rounds costs more occupancy
+        // than four candidates per round save
+        const int multi = w8 ? hull_multi() : 0;
+        const int multi_full = w8f ? hull_multi() : 0;
         const unsigned ldc = hull_ws_bytes<AS_LDS>(vcap, 64 * ppl, multi != 0);
-        const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl, multi != 0);
+        const unsigned ldf = hull_ws_bytes<AS_LDS>(64 * ppl, 64 * ppl, multi_full != 0);
         HIP_TRY(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldc));
         HIP_TRY(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldf));
         int* d_retry = hc.alloc<int>((size_t)nids + 1);
@@ -1466,13 +1516,13 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kc, dim3((unsigned)(size_t)nids), dim3(64 * LA_LDS4_NW), ldc, s, d_tasks, d_ids,
+        hipLaunchKernelGGL(kc, dim3((unsigned)(size_t)nids), dim3(64 * nw), ldc, s, d_tasks, d_ids,
                            (int)(size_t)nids, dcnt, vcap, d_retry, multi, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
         hipEvent_t t1 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, (size_t)nids)), dim3(64 * LA_LDS4_NW), ldf, s, d_tasks,
-                           d_retry + 1, (int)(size_t)nids, d_retry, multi, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kf, dim3((unsigned)retry_grid(ctx, (size_t)nids)), dim3(64 * nwf), ldf, s, d_tasks,
+                           d_retry + 1, (int)(size_t)nids, d_retry, multi_full, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, ppl == 8 ? KT_HULL_R8 : KT_HULL_R16, t1, 0.0, s);
         return 0;
@@ -1512,10 +1562,19 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
 using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, DPool*,
                          int*);
 
+// The context's stream (and so every side stream forked from it later)
+// waits for the recorded first passes of the eight-wave LDS tiers.
+int wait_tiers(gpu_ctx_t ctx, hipEvent_t e8, hipEvent_t e4) {
+    if (e8) HIP_TRY(hipStreamWaitEvent(ctx->stream, e8, 0));
+    if (e4) HIP_TRY(hipStreamWaitEvent(ctx->stream, e4, 0));
+    return 0;
+}
+
 // One block tier (sets above 1024 points, at most maxn). With dcnt given,
 // d_ids holds up to n ids, *dcnt on the device, and maxn is the tier's bound.
 int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids, int n, int maxn,
-                     double bytes, const int* dcnt = nullptr, bool hbm_only = false) {
+                     double bytes, const int* dcnt = nullptr, bool hbm_only = false,
+                     hipEvent_t after_first = nullptr) {
     gpu_ctx_t ctx = hc.ctx;
     if (n <= 0) return 0;
     constexpr int T = 64 * LA_BLOCK_NW;
@@ -1547,6 +1606,7 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
                            ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, lds4 ? KT_HULL_B4 : KT_HULL_B8, t0, bytes, s);
+        if (after_first) HIP_TRY(hipEventRecord(after_first, s));
         hipEvent_t t1 = kt_begin(ctx, s);
         hipLaunchKernelGGL(kr, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_retry + 1, n, (const int*)d_retry, gws,
                            stride, np, np, ctx->d_pool, ctx->d_err);
@@ -1663,11 +1723,16 @@ int launch_by_size(HullCall& hc, LaHullTask* d_tasks, const std::vector<std::pai
     HIP_TRY(hipMemcpyAsync(d_all, all.data(), 4 * total, hipMemcpyHostToDevice, ctx->stream));
     auto L = [&](int k) { return d_all + off[k]; };
     auto N = [&](int k) { return (int)lists[k]->size(); };
+    // with the order knob, the eight-wave LDS first passes are recorded and
+    // the smaller tiers wait for them
+    hipEvent_t e8 = g_tune[12] && N(2) > 0 ? ctx->ev_tier[0] : nullptr;
+    hipEvent_t e4 = g_tune[12] && N(3) > 0 ? ctx->ev_tier[1] : nullptr;
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(0), N(0), m32, y32)) return rc;
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(1), N(1), m16, y16)) return rc;
-    if (int rc = launch_block_ids(hc, 0, d_tasks, L(2), N(2), m8, y8)) return rc;
-    if (int rc = launch_block_ids(hc, 1, d_tasks, L(3), N(3), m4, y4)) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(2), N(2), m8, y8, nullptr, false, e8)) return rc;
+    if (int rc = launch_block_ids(hc, 1, d_tasks, L(3), N(3), m4, y4, nullptr, false, e4)) return rc;
     if (int rc = launch_block_ids(hc, -1, d_tasks, L(9), N(9), m4h, y4h, nullptr, true)) return rc;
+    if (int rc = wait_tiers(ctx, e8, e4)) return rc;
     const int ppls[5] = {1, 2, 4, 8, 16};
     if (int rc = launch_wave_ids(hc, 2, d_tasks, L(4), N(4), ppls[4], cbytes[4])) return rc;
     for (int c = 3; c >= 0; --c)
@@ -2035,10 +2100,13 @@ int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, ch
     // the launch_by_size order and streams: block tiers (largest first) on
     // side streams 0 and 1, the 1024-point wave tier on stream 2, the small
     // wave tiers on the context's stream
+    hipEvent_t e8 = g_tune[12] ? ctx->ev_tier[0] : nullptr;
+    hipEvent_t e4 = g_tune[12] ? ctx->ev_tier[1] : nullptr;
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(8), n, HULL_NP_MAX, 0.0, C(8))) return rc;
     if (int rc = launch_block_ids(hc, 0, d_tasks, L(7), n, 8192, 0.0, C(7))) return rc;
-    if (int rc = launch_block_ids(hc, 0, d_tasks, L(6), n, 4096, 0.0, C(6))) return rc;
-    if (int rc = launch_block_ids(hc, 1, d_tasks, L(5), n, 2048, 0.0, C(5))) return rc;
+    if (int rc = launch_block_ids(hc, 0, d_tasks, L(6), n, 4096, 0.0, C(6), false, e8)) return rc;
+    if (int rc = launch_block_ids(hc, 1, d_tasks, L(5), n, 2048, 0.0, C(5), false, e4)) return rc;
+    if (int rc = wait_tiers(ctx, e8, e4)) return rc;
     const int ppls[5] = {1, 2, 4, 8, 16};
     if (int rc = launch_wave_ids(hc, 2, d_tasks, L(4), n, ppls[4], 0.0, C(4))) return rc;
     for (int c = 3; c >= 0; --c)

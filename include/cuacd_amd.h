@@ -266,7 +266,7 @@ void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
  * first passes, 0/1), 3 kdop_min (k-DOP prefilter above this many points,
  * >= 4), 4 chunk_above (prefiltered sets above it are hulled in chunks of
  * half that size first, 1024..16384), 5 multi (several apexes per quickhull
- * round in the eight-wave LDS tier, 0/1), 6 mlp_fused (the proposal
+ * round in the eight-wave LDS tiers, 0/1), 6 mlp_fused (the proposal
  * network's inference as one fused point-layer kernel, 0/1; same scores),
  * 7 ldsv8_vcap (first-pass vertex capacity of the eight-wave LDS tier,
  * 64..1024 in steps of 32; hulls with more corners are redone in HBM),
@@ -277,7 +277,12 @@ void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
  * default 98304; the same halves either way), 10 hull_lpt (each hull tier's
  * sets launched largest first, 0/1; default 0), 11 hull_b4_split (percent
  * of the 1025-2048-point sets, the smallest, hulled by the HBM-workspace
- * kernel beside the LDS tier, 0..90; default 0).
+ * kernel beside the LDS tier, 0..90; default 0), 12 hull_order (the
+ * four-wave and wave tiers of a hull call wait for the first passes of the
+ * eight-wave LDS tiers, which need a whole CU each, 0/1), 13 hull_w8 (first
+ * passes of the 513-1024-point (bit 0) and 257-512-point (bit 1) wave tiers
+ * on eight waves with multi-apex rounds; bits 2 and 3: the same for the
+ * full-capacity retries of these tiers; 0..15).
  * Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);
