@@ -49,10 +49,15 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 PARAMS = dict(max_iters=100, width=30, width2=5, threshold=0.05, depth=2, max_n_cutting=16,
               n_concave_edges=16, merge_hulls=True)
-# VALU busy fraction of the hull tiers, from their SQ counters
-# (profiles/lookahead_64sdf_hull_tier_counters.md): counters cannot be read
-# inside the timed run, so the line carries the profiled figure beside HBM
-HULL_VALU_BUSY = {"la_hull_ldsv<4,8>": 0.018, "la_hull_ldsv<4,4>": 0.021, "la_hull_block<8>": 0.007}
+# VALU busy fraction and HBM bytes per launch (FETCH_SIZE x 2.00 +
+# WRITE_SIZE x 1.00, calibrated on a copy of known size) of the hull tiers,
+# from their counters (profiles/lookahead_64sdf_hull_tier_counters.md):
+# counters cannot be read inside the timed run, so the line carries the
+# profiled figures beside the live HBM fraction
+HULL_VALU_BUSY = {"la_hull_ldsv<2,8>": 0.042, "la_hull_ldsv<4,8>": 0.024, "la_hull_ldsv<2,4>": 0.034,
+                  "la_hull_ldsv<8,8>": 0.024, "la_hull_block<8>": 0.004}
+HULL_TRAFFIC = {"la_hull_ldsv<2,8>": 74.90e6, "la_hull_ldsv<4,8>": 49.78e6, "la_hull_ldsv<2,4>": 59.41e6,
+                "la_hull_ldsv<8,8>": 78.93e6}
 LA_LIMITER = ("latency: each hull is a chain of dependent insertion steps (LDS reads and writes, block barriers); "
               "HBM traffic is ~1x the algorithmic bytes, so the HBM fraction only says how far from a bandwidth "
               "bound this work sits")
@@ -279,14 +284,20 @@ def cpu_score(meshes, planes, tau, cores, budget_s):
 # ---------------------------------------------------------------------------
 # GPU legs
 
-def kernel_name(cls, wide_min, half_cap, ldsb8=0):
-    """Kernel behind a hull class reported by the library."""
+def kernel_name(cls, wide_min, half_cap, ldsb8=0, w8=0):
+    """Kernel behind a hull class reported by the library (w8: the hull_w8
+    knob, eight-wave multi-apex kernels for the 513-1024-point (bit 0) and
+    257-512-point (bit 1) wave tiers, bits 2 and 3 for their retries)."""
     if cls == "kdop_prefilter":
         return "la_kdop_kernel"
     if cls.endswith("_retry"):
         base = cls[:-6]
         n = int(base[6:])
-        return f"la_hull_block<{n // 512}>" if base.startswith("hull_b") else f"la_hull_lds4<{n // 256}>"
+        if base.startswith("hull_b"):
+            return f"la_hull_block<{n // 512}>"
+        if (w8 >> (2 if n == 1024 else 3)) & 1:
+            return f"la_hull_lds8<{n // 512}>"
+        return f"la_hull_lds4<{n // 256}>"
     if cls.startswith("hull_b"):
         n = int(cls[6:])
         ppl = n // 512
@@ -299,6 +310,8 @@ def kernel_name(cls, wide_min, half_cap, ldsb8=0):
     ppl = n // 64
     if wide_min > 0 and ppl >= 4 and n > wide_min:
         if ppl >= 8 and half_cap:
+            if (w8 >> (0 if ppl == 16 else 1)) & 1:
+                return f"la_hull_ldsv<{ppl // 8},8>"
             return f"la_hull_ldsv<{ppl // 4},4>"
         return f"la_hull_lds4<{ppl // 4}>"
     return f"la_hull_wave<{ppl}>"
@@ -374,8 +387,9 @@ def lookahead_kernels(ctx, r):
     wide_min = ctx.get_tuning("hull_wide_min")
     half_cap = ctx.get_tuning("hull_half_cap") != 0
     ldsb8 = ctx.get_tuning("hull_ldsb8_vcap")
+    w8 = ctx.get_tuning("hull_w8")
     for cls, k in r["kernels"].items():
-        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8), k)
+        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8, w8), k)
     for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"]):
         if k["launches"] == 0:
             continue
@@ -621,9 +635,10 @@ def gpu_score(ctx, meshes, planes, steps, tau=0.05):
     wide_min = ctx.get_tuning("hull_wide_min")
     half_cap = ctx.get_tuning("hull_half_cap") != 0
     ldsb8 = ctx.get_tuning("hull_ldsb8_vcap")
+    w8 = ctx.get_tuning("hull_w8")
     for i, cls in enumerate(cuacd_amd.LOOKAHEAD_KERNELS):
         k_ms, la, by = st[6 + 3 * i: 9 + 3 * i]
-        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8), {"ms": k_ms, "launches": int(la), "alg_bytes": by})
+        _add_kernel(kern, kernel_name(cls, wide_min, half_cap, ldsb8, w8), {"ms": k_ms, "launches": int(la), "alg_bytes": by})
     for name, k in sorted(kern.items(), key=lambda x: -x[1]["ms"]):
         if k["launches"] == 0:
             continue
@@ -723,6 +738,10 @@ def main():
         if roof["kernel"] in HULL_VALU_BUSY:
             roof["valu_busy"] = {"frac": HULL_VALU_BUSY[roof["kernel"]], "measured": "rocprofv3 SQ counters",
                                  "profile": "profiles/lookahead_64sdf_hull_tier_counters.md"}
+        if roof["kernel"] in HULL_TRAFFIC:
+            roof["traffic"] = HULL_TRAFFIC[roof["kernel"]]
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, "
+                                      "profiles/lookahead_64sdf_hull_tier_counters.md")
         cb = None
         if "lookahead_raw" in cpu:
             cb = finish_cpu_lookahead(cpu["lookahead_raw"], meshes, plain["iters"])
