@@ -44,7 +44,7 @@ LOOKAHEAD_KERNELS = ("hull_w64", "hull_w128", "hull_w256", "hull_w512", "hull_w1
 TUNING_KEYS = {"hull_wide_min": 0, "hull_vcap_eighths": 1, "hull_half_cap": 2, "kdop_min_points": 3,
                "hull_chunk_above": 4, "hull_multi": 5, "mlp_fused": 6, "hull_ldsv8_vcap": 7, "hull_ldsb8_vcap": 8,
                "cut_lds_verts": 9, "hull_lpt": 10, "hull_b4_split": 11,
-               "hull_order": 12, "hull_w8": 13}
+               "hull_order": 12, "hull_w8": 13, "hull_multi_hbm": 14}
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcuacd_hip.so")
 

@@ -106,7 +106,8 @@ template <int AS> struct HullWS {
     asptr<AS_LDS, int> wpre;    // [NP/32] prefix popcounts of bm
     asptr<AS_LDS, int> ctr;     // [8] counters: 0 nvis, 1 nhz
     // multi-apex scratch (LDS workspaces with several waves; null otherwise)
-    asptr<AS_LDS, uint32_t> mfm;    // [FC] per face: bit r visited by candidate r, bit 8+r visible from it
+    asptr<AS_LDS, uint32_t> mfm;    // [FC/2] per face, 16 bits (face f in word f/2, bits 16 (f&1) up):
+                                    // bit r visited by candidate r, bit 8+r visible from it
     asptr<AS_LDS, uint32_t> mfmax;  // [FC] per face: best conflict key (quantised distance, point index)
     asptr<AS_LDS, uint8_t> mfcone;  // [FC] candidate whose cone replaced the face this round
     asptr<AS_LDS, uint16_t> mlist;  // [MA_W][MA_CAP][5]: visible faces, ha, hb, hface, hnew (ranks >= 1)
@@ -116,11 +117,47 @@ template <int AS> struct HullWS {
     int VC, FC, NP;
 };
 
+// Candidate lists of the multi-apex rounds: candidate 0 uses the workspace's
+// own arrays, the others LDS scratch; with a workspace in global memory both
+// are addressed through generic pointers.
+template <int AS, class T> struct MaList { using type = T*; };
+template <class T> struct MaList<AS_LDS, T> { using type = asptr<AS_LDS, T>; };
+
 // Multi-apex rounds: at most MA_W candidates, MA_CAP visible faces and
 // horizon edges for each candidate but the first (which uses the main arrays).
 // The planes of a cone of at most MA_CPL faces are also kept in cone order, so
 // that the points of the faces it replaced scan them without an id lookup.
 constexpr int MA_W = 8, MA_CAP = 128, MA_CI = 12, MA_CPL = 32;
+
+// LDS scratch of the multi-apex rounds for fc faces, carved from p (16-byte
+// aligned sub-arrays; p == nullptr only sizes it). An LDS workspace carries
+// it after its own arrays; a workspace in global memory takes it from the
+// kernel's dynamic LDS (hull_ma_attach).
+template <int AS>
+__host__ __device__ __forceinline__ unsigned hull_ma_layout(char* p, unsigned fc, HullWS<AS>* w) {
+    unsigned off = 0;
+    auto take = [&](unsigned bytes) {
+        char* r = p ? p + off : nullptr;
+        off += (bytes + 15u) / 16u * 16u;
+        return r;
+    };
+    w->mfm = (asptr<AS_LDS, uint32_t>)take((fc + 1u) / 2u * 4u);
+    w->mfmax = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
+    w->mfcone = (asptr<AS_LDS, uint8_t>)take(fc);
+    w->mlist = (asptr<AS_LDS, uint16_t>)take(MA_W * MA_CAP * 5 * 2u);
+    w->mhedge = (asptr<AS_LDS, uint8_t>)take(MA_W * MA_CAP);
+    w->mcand = (asptr<AS_LDS, int>)take((MA_W * MA_CI + MA_W + 16) * 4u);
+    w->mcpl = (asptr<AS_LDS, f4v>)take(MA_W * MA_CPL * 16u);
+    return off;
+}
+__host__ __device__ inline unsigned hull_ma_bytes(int vc) {
+    HullWS<AS_LDS> w{};
+    return hull_ma_layout<AS_LDS>(nullptr, 2u * (unsigned)vc + 8u, &w);
+}
+template <int AS> __device__ __forceinline__ HullWS<AS> hull_ma_attach(HullWS<AS> w, char* lds) {
+    hull_ma_layout<AS>(lds, (unsigned)w.FC, &w);
+    return w;
+}
 
 // FC = 2*VC + 8 faces bound the live faces of a hull with VC vertices plus the
 // transient pair created by one insertion (nhz <= nvis + 2). Face and vertex
@@ -160,15 +197,7 @@ __host__ __device__ __forceinline__ unsigned hull_ws_layout(char* p, int vc, int
     w.vout = (asptr<AS, uint16_t>)take(v * 2u);
     w.vref = (asptr<AS, uint8_t>)take(v);
     w.cv = (asptr<AS, int>)take(v * 8u);
-    if (multi && AS == AS_LDS) {
-        w.mfm = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
-        w.mfmax = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
-        w.mfcone = (asptr<AS_LDS, uint8_t>)take(fc);
-        w.mlist = (asptr<AS_LDS, uint16_t>)take(MA_W * MA_CAP * 5 * 2u);
-        w.mhedge = (asptr<AS_LDS, uint8_t>)take(MA_W * MA_CAP);
-        w.mcand = (asptr<AS_LDS, int>)take((MA_W * MA_CI + MA_W + 16) * 4u);
-        w.mcpl = (asptr<AS_LDS, f4v>)take(MA_W * MA_CPL * 16u);
-    }
+    if (multi && AS == AS_LDS) off += hull_ma_layout(p ? p + off : nullptr, fc, &w);
     (void)nw;  // bm / wpre / ctr live in HullBlockLDS (hull_ws_attach)
     w.VC = vc;
     w.FC = (int)fc;
@@ -520,11 +549,16 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
         // other, which would then be in its own visible region. Candidates may
         // be non-extreme points (their distances are quantised); the exact
         // incremental construction still ends at the unique hull.
-        static_assert(AS == AS_LDS && NW > 1 && NW <= MA_W, "multi-apex rounds need an LDS workspace and several waves");
+        static_assert(NW > 1 && NW <= MA_W, "multi-apex rounds need several waves");
         // a conflict key is the face's best point: its quantised distance
         // above the high bits, its (inverted) local index in the low ones
-        static_assert(PPL * T <= 4096, "multi-apex keys hold point indices below 4096");
-        constexpr uint32_t IMASK = PPL * T <= 2048 ? 0x7FFu : 0xFFFu;
+        static_assert(PPL * T <= 8192, "multi-apex keys hold point indices below 8192");
+        constexpr uint32_t IMASK = PPL * T <= 2048 ? 0x7FFu : (PPL * T <= 4096 ? 0xFFFu : 0x1FFFu);
+        using L16 = typename MaList<AS, uint16_t>::type;
+        using L8 = typename MaList<AS, uint8_t>::type;
+        // the 16 mark bits of face f in its word
+        auto mw = [&](int f) { return w.mfm + (f >> 1); };
+        auto msh = [](int f) { return (uint32_t)(f & 1) << 4; };
         const int wv = tid >> 6, lane = tid & 63;
         asptr<AS_LDS, int> cand = w.mcand;
         asptr<AS_LDS, int> crow = w.mcand + MA_W * MA_CI;
@@ -540,7 +574,8 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
         };
         for (int it = 1;; ++it) {
             // A: clear the per-face marks and best keys
-            for (int f = tid; f < ftop; f += T) { w.mfm[f] = 0u; w.mfmax[f] = 0u; }
+            for (int f = tid; f < ftop; f += T) w.mfmax[f] = 0u;
+            for (int q = tid; q < (ftop + 1) >> 1; q += T) w.mfm[q] = 0u;
             if (tid < MA_W) crow[tid] = 0;
             __syncthreads();
             // B: best conflict key per face (quantised distance, lowest index)
@@ -601,16 +636,16 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                 const int f0 = cand[r * MA_CI + 1];
                 const int qx = cand[r * MA_CI + 3], qy = cand[r * MA_CI + 4], qz = cand[r * MA_CI + 5];
                 const float fqx = (float)qx, fqy = (float)qy, fqz = (float)qz;
-                asptr<AS_LDS, uint16_t> V = r == 0 ? (asptr<AS_LDS, uint16_t>)w.vis : lvis(r);
+                L16 V = r == 0 ? (L16)w.vis : (L16)lvis(r);
                 const int vcap = r == 0 ? FC : MA_CAP;
-                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
-                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
-                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
-                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
+                L16 HA = r == 0 ? (L16)w.ha : (L16)lha(r);
+                L16 HB = r == 0 ? (L16)w.hb : (L16)lhb(r);
+                L16 HF = r == 0 ? (L16)w.hface : (L16)lhf(r);
+                L8 HE = r == 0 ? (L8)w.hedge : (L8)(w.mhedge + r * MA_CAP);
                 const int hcap = r == 0 ? VC : MA_CAP;
                 const uint32_t vb = 1u << r, sbit = 1u << (8 + r);
                 if (lane == 0) {
-                    __hip_atomic_fetch_or(&w.mfm[f0], vb | sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_or(mw(f0), (vb | sbit) << msh(f0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     V[0] = (uint16_t)f0;
                 }
                 wsync();
@@ -636,8 +671,8 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                         u4v gn[3];
 #pragma unroll
                         for (int e = 0; e < 3; ++e) {
-                            old[e] = __hip_atomic_fetch_or(&w.mfm[gg[e]], vb, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                            old[e] = __hip_atomic_fetch_or(mw(gg[e]), vb << msh(gg[e]), __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP) >> msh(gg[e]);
                             pl[e] = w.fpl[gg[e]];
                             gn[e] = w.fn[gg[e]];
                         }
@@ -660,7 +695,8 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                     for (int e = 0; e < 3; ++e) {
                         const bool ap = act && vis[e] && !(old[e] & vb);
                         if (ap)
-                            __hip_atomic_fetch_or(&w.mfm[gg[e]], sbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_or(mw(gg[e]), sbit << msh(gg[e]), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
                         const uint64_t m = ballot(ap);
                         const int pos = nv + add + mask_rank(m);
                         if (ap) {
@@ -696,7 +732,7 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
             __syncthreads();
             // E: conflicts - candidate r's visible faces against the others' visited faces
             for (int f = tid; f < ftop; f += T) {
-                const uint32_t m = w.mfm[f];
+                const uint32_t m = (w.mfm[f >> 1] >> msh(f)) & 0xFFFFu;
                 uint32_t sv = (m >> 8) & 0xFFu;
                 const uint32_t vis_by = m & 0xFFu;
                 while (sv) {
@@ -761,7 +797,7 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                         nf += nv > nh ? nv - nh : 0;
                     }
                 if (lane < nc && ((acc >> lane) & 1u)) {
-                    asptr<AS_LDS, uint16_t> HN = lane == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(lane);
+                    L16 HN = lane == 0 ? (L16)w.hnew : (L16)lhn(lane);
                     const int extra = c_nh > c_nv ? c_nh - c_nv : 0;
                     for (int k = 0; k < extra; ++k)
                         HN[c_nv + k] = k < my_ff ? w.ffree[my_free - 1 - k] : (uint16_t)(my_top + k - my_ff);
@@ -785,12 +821,12 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                 const int nv = cand[r * MA_CI + 6], nh = cand[r * MA_CI + 7];
                 const int vid = cand[r * MA_CI + 9], push = cand[r * MA_CI + 10];
                 const int qx = cand[r * MA_CI + 3], qy = cand[r * MA_CI + 4], qz = cand[r * MA_CI + 5];
-                asptr<AS_LDS, uint16_t> V = r == 0 ? (asptr<AS_LDS, uint16_t>)w.vis : lvis(r);
-                asptr<AS_LDS, uint16_t> HA = r == 0 ? (asptr<AS_LDS, uint16_t>)w.ha : lha(r);
-                asptr<AS_LDS, uint16_t> HB = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hb : lhb(r);
-                asptr<AS_LDS, uint16_t> HF = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hface : lhf(r);
-                asptr<AS_LDS, uint8_t> HE = r == 0 ? (asptr<AS_LDS, uint8_t>)w.hedge : w.mhedge + r * MA_CAP;
-                asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                L16 V = r == 0 ? (L16)w.vis : (L16)lvis(r);
+                L16 HA = r == 0 ? (L16)w.ha : (L16)lha(r);
+                L16 HB = r == 0 ? (L16)w.hb : (L16)lhb(r);
+                L16 HF = r == 0 ? (L16)w.hface : (L16)lhf(r);
+                L8 HE = r == 0 ? (L8)w.hedge : (L8)(w.mhedge + r * MA_CAP);
+                L16 HN = r == 0 ? (L16)w.hnew : (L16)lhn(r);
                 if (lane == 0) {
                     w.vq[vid] = mk_s4(qx, qy, qz, 0);
                     w.vpid[vid] = cand[r * MA_CI + 2];
@@ -865,7 +901,7 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                 const int qx = pick(px, k), qy = pick(py, k), qz = pick(pz, k);
                 const int r = w.mfcone[c];
                 const int nh = cand[r * MA_CI + 7];
-                asptr<AS_LDS, uint16_t> HN = r == 0 ? (asptr<AS_LDS, uint16_t>)w.hnew : lhn(r);
+                L16 HN = r == 0 ? (L16)w.hnew : (L16)lhn(r);
                 // the cone's planes in cone order when it kept them (the same
                 // test as face_above, on the same plane values), else by id
                 const bool cp = nh <= MA_CPL;

@@ -93,10 +93,10 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
     }
     HullResult r;
-    if constexpr (AS == AS_LDS && NW >= 8 && PPL * T <= 4096) {
-        // multi-apex rounds (the workspace was carved with their scratch; the
-        // four-wave tiers keep single-apex steps and do not compile them); the
-        // point index field of their keys holds 12 bits
+    if constexpr (NW >= 8 && PPL * T <= 8192) {
+        // multi-apex rounds (the workspace was carved or attached with their
+        // scratch; the four-wave tiers keep single-apex steps and do not
+        // compile them); the point index field of their keys holds 13 bits
         if (multi && n <= PPL * T) r = block_quickhull<PPL, AS, NW, true>(w, sb, n, px, py, pz);
         else r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
     } else {
@@ -240,19 +240,24 @@ __global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const
 
 // Workspace slot blockIdx.x of gws; ids[0, n), or the first min(*dcount, n)
 // when dcount is given, strided over the grid.
+// With `multi`, multi-apex rounds: their per-face marks and keys in the
+// dynamic LDS (hull_ma_bytes(VC)), the faces and vertices in the global
+// workspace.
 template <int PPL>
 __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* tasks, const int* ids, int n,
                                                                   const int* dcount, char* gws,
                                                                   unsigned long long stride, int VC, int NP,
-                                                                  DPool* pool, int* err) {
+                                                                  int multi, DPool* pool, int* err) {
     __shared__ HullBlockLDS sb;
+    extern __shared__ __align__(16) char lds_ma[];
     const int nn = dcount ? min(*dcount, n) : n;
-    const HullWS<AS_GLOBAL> w =
+    HullWS<AS_GLOBAL> w =
         hull_ws_attach(hull_ws_carve<AS_GLOBAL>(gws + (unsigned long long)blockIdx.x * stride, VC, NP), &sb);
+    if (multi) w = hull_ma_attach(w, lds_ma);
     for (int j = blockIdx.x; j < nn; j += gridDim.x) {
         __syncthreads();
         LaHullTask& t = tasks[ids[j]];
-        la_hull_item<PPL, AS_GLOBAL, LA_BLOCK_NW>(t, w, &sb, pool, err);
+        la_hull_item<PPL, AS_GLOBAL, LA_BLOCK_NW>(t, w, &sb, pool, err, nullptr, -1, multi != 0);
         if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
     }
 }
@@ -1350,7 +1355,14 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //     513-1024-point hull takes 0.38 ms on eight waves against 0.69 ms on
 //     four (two workgroups per CU against three), and the step went from
 //     2.05 s to 1.96 s; the 257-512-point tier packs better on four waves
-constexpr int LA_TUNE_N = 14;
+//   14 multi_hbm: multi-apex rounds in the HBM-workspace tiers of up to 8192
+//     points (their per-face marks and keys in LDS; CUACD_HULL_MULTI_HBM, 0/1;
+//     default 1). These tiers hold the hulls with the most corners, so the
+//     rounds pay most there: on the 256-mesh scorer workload the 4097-8192-
+//     point tier went from 260 ms to 67 ms per step (14.8k -> 27.0k cuts/s),
+//     and the 64-mesh search step from 1.98 s to 1.83 s, although the
+//     kernels spill (224 and 608 bytes per lane at 4096 and 8192 points)
+constexpr int LA_TUNE_N = 15;
 int g_tune[LA_TUNE_N];
 bool g_tune_init = false;
 
@@ -1376,8 +1388,8 @@ void tune_init() {
                                   "CUACD_KDOP_MIN",        "CUACD_HULL_CHUNK_ABOVE", "CUACD_HULL_MULTI",
                                   "CUACD_MLP_FUSED",       "CUACD_HULL_LDSV8_VCAP",  "CUACD_HULL_LDSB8_VCAP",
                                   "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT",         "CUACD_HULL_B4_SPLIT",
-                                  "CUACD_HULL_ORDER",      "CUACD_HULL_W8"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0, 1, 5};
+                                  "CUACD_HULL_ORDER",      "CUACD_HULL_W8",          "CUACD_HULL_MULTI_HBM"};
+    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0, 1, 5, 1};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);
@@ -1389,6 +1401,7 @@ int hull_wide_min() { tune_init(); return g_tune[0]; }
 int hull_vcap_eighths() { tune_init(); return g_tune[1]; }
 int hull_half_cap() { tune_init(); return g_tune[2]; }
 int hull_multi() { tune_init(); return g_tune[5]; }
+int hull_multi_hbm() { tune_init(); return g_tune[14]; }
 int hull_ldsv8_vcap() { tune_init(); return g_tune[7]; }
 int hull_ldsb8_vcap() { tune_init(); return g_tune[8]; }
 
@@ -1559,8 +1572,18 @@ int launch_wave_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids,
     return 0;
 }
 
-using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, DPool*,
-                         int*);
+using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsigned long long, int, int, int,
+                         DPool*, int*);
+
+// Multi-apex rounds for an HBM-workspace kernel of vc vertices (knob
+// multi_hbm; their LDS scratch fits up to 8192 vertices): the flag, and the
+// dynamic LDS it needs, set on the kernel.
+int block_multi(gpu_ctx_t ctx, BlockFn k, int vc, int* multi, unsigned* lds) {
+    *multi = (hull_multi_hbm() && vc <= 8192) ? 1 : 0;
+    *lds = *multi ? hull_ma_bytes(vc) : 0u;
+    if (*lds > 0) HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds));
+    return 0;
+}
 
 // The context's stream (and so every side stream forked from it later)
 // waits for the recorded first passes of the eight-wave LDS tiers.
@@ -1607,9 +1630,12 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
         HIP_TRY(hipGetLastError());
         kt_end(ctx, lds4 ? KT_HULL_B4 : KT_HULL_B8, t0, bytes, s);
         if (after_first) HIP_TRY(hipEventRecord(after_first, s));
+        int mr;
+        unsigned lr;
+        if (int rc = block_multi(ctx, kr, np, &mr, &lr)) return rc;
         hipEvent_t t1 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(kr, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_retry + 1, n, (const int*)d_retry, gws,
-                           stride, np, np, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(kr, dim3((unsigned)g), dim3(T), lr, s, d_tasks, d_retry + 1, n, (const int*)d_retry, gws,
+                           stride, np, np, mr, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, lds4 ? KT_HULL_RB4 : KT_HULL_RB8, t1, 0.0, s);
         return 0;
@@ -1623,6 +1649,9 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
     const int np = (maxn + 63) / 64 * 64;
     const int vc = np < 32704 ? np : 32704;
     const unsigned long long stride = (hull_ws_bytes<AS_GLOBAL>(vc, np) + 255ull) & ~255ull;
+    int mk;
+    unsigned lk;
+    if (int rc = block_multi(ctx, k, vc, &mk, &lk)) return rc;
     if (dcnt) {
         // device count: one workspace per CU, the grid strides over the list
         const int g = retry_grid(ctx, (size_t)n);
@@ -1631,7 +1660,7 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
         hipStream_t s;
         if (int rc = hc.fork(sk, &s)) return rc;
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(T), 0, s, d_tasks, d_ids, n, dcnt, gws, stride, vc, np,
+        hipLaunchKernelGGL(k, dim3((unsigned)g), dim3(T), lk, s, d_tasks, d_ids, n, dcnt, gws, stride, vc, np, mk,
                            ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes, s);
@@ -1647,8 +1676,8 @@ int launch_block_ids(HullCall& hc, int sk, LaHullTask* d_tasks, const int* d_ids
     for (int b = 0; b < n; b += slice) {
         const int cnt = std::min(slice, n - b);
         hipEvent_t t0 = kt_begin(ctx, s);
-        hipLaunchKernelGGL(k, dim3(cnt), dim3(T), 0, s, d_tasks, d_ids + b, cnt, (const int*)nullptr, gws, stride,
-                           vc, np, ctx->d_pool, ctx->d_err);
+        hipLaunchKernelGGL(k, dim3(cnt), dim3(T), lk, s, d_tasks, d_ids + b, cnt, (const int*)nullptr, gws, stride,
+                           vc, np, mk, ctx->d_pool, ctx->d_err);
         HIP_TRY(hipGetLastError());
         kt_end(ctx, kid, t0, bytes * (double)cnt / (double)n, s);
     }
