@@ -289,7 +289,7 @@ def kernel_name(cls, wide_min, half_cap, ldsb8=0, w8=0):
     knob, eight-wave multi-apex kernels for the 513-1024-point (bit 0) and
     257-512-point (bit 1) wave tiers, bits 2 and 3 for their retries)."""
     if cls == "kdop_prefilter":
-        return "la_kdop_kernel"
+        return "la_kdop (extremes + rough hull + filter)"
     if cls.endswith("_retry"):
         base = cls[:-6]
         n = int(base[6:])

@@ -267,26 +267,35 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
 // keep the points on or outside some rough-hull face (exactly, on the full
 // set's grid), in input order. The filtered set keeps the full set's
 // quantization frame and the input order, so its canonical hull equals the
-// full set's.
+// full set's. Three launches over the same list, one record per set between
+// them: the extremes (four waves per set, passes over the points), the rough
+// hull of the 80 extreme points (one wave per set: its insertion steps need
+// no block barrier, and many sets share a CU), the filter (four waves).
 constexpr int KD_T = 256;
 constexpr int KD_NW = KD_T / 64;
+constexpr int KD_NE = 2 * KDOP_AXES_N;   // extreme points
+constexpr int KD_MAXPL = 2 * 128 + 8;    // rough faces kept as filter planes
 
-__global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const int* ids, int n,
-                                                       const int* dcount, int2* res, DPool* pool, int* err) {
-    __shared__ HullBlockLDS sb;
-    __shared__ __align__(16) char lds[20480];
-    __shared__ float ep[2 * KDOP_AXES_N][3];
-    __shared__ double4 dpl[2 * 128 + 8];
-    __shared__ float4 fpl[2 * 128 + 8];  // the same planes in FP32 ...
-    __shared__ float fbd[2 * 128 + 8];   // ... and a bound on that form's error
+struct KdopRec {
+    QFrame full;                      // the full set's frame
+    int status;                       // 0 filter, 1 no filter (hull the full set), 2 pruned
+    int npl;                          // filter planes
+    float ep[KD_NE][3];               // extreme points (axis a: max at 2a, min at 2a + 1)
+    unsigned char face[KD_MAXPL][3];  // filter planes as extreme-point triples
+};
+
+// Launch 1: bbox -> frame, then the extreme points along the 40 axes, 8 axes
+// per pass: max key (value, lowest index), min key (value, lowest index).
+__global__ __launch_bounds__(KD_T) void la_kdop_ext_kernel(LaHullTask* tasks, const int* ids, int n,
+                                                           const int* dcount, KdopRec* rec) {
+    __shared__ float bmn[KD_NW][3], bmx[KD_NW][3];
     __shared__ unsigned long long kred[KD_NW][16];
-    __shared__ int s_cnt[KD_NW + 2];
     if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
-    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const LaHullTask& t = tasks[ids[blockIdx.x]];
+    KdopRec& R = rec[blockIdx.x];
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
     const float* P = t.pts;
     const int np = t.n;
-    // full-set bbox -> frame
     float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
     for (int i = tid; i < np; i += KD_T)
         for (int j = 0; j < 3; ++j) {
@@ -299,13 +308,11 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         mx[j] = wave_max_f(mx[j]);
     }
     if (lane == 0)
-        for (int j = 0; j < 3; ++j) { sb.bmn[wv][j] = mn[j]; sb.bmx[wv][j] = mx[j]; }
+        for (int j = 0; j < 3; ++j) { bmn[wv][j] = mn[j]; bmx[wv][j] = mx[j]; }
     __syncthreads();
     for (int j = 0; j < 3; ++j)
-        for (int q = 0; q < KD_NW; ++q) { mn[j] = fminf(mn[j], sb.bmn[q][j]); mx[j] = fmaxf(mx[j], sb.bmx[q][j]); }
-    __syncthreads();
-    const QFrame full = qframe_make(mn, mx);
-    // extremes, 8 axes per pass: max key (value, lowest index), min key (value, lowest index)
+        for (int q = 0; q < KD_NW; ++q) { mn[j] = fminf(mn[j], bmn[q][j]); mx[j] = fmaxf(mx[j], bmx[q][j]); }
+    if (tid == 0) R.full = qframe_make(mn, mx);
     for (int a0 = 0; a0 < KDOP_AXES_N; a0 += 8) {
         unsigned long long kmax[8], kmin[8];
 #pragma unroll
@@ -339,37 +346,66 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             const int q = tid & 7, isMin = tid >> 3;
             const int idx = isMin ? (int)(uint32_t)k : (int)(0xFFFFFFFFu - (uint32_t)k);
             const int slot = 2 * (a0 + q) + isMin;
-            ep[slot][0] = P[3ll * idx];
-            ep[slot][1] = P[3ll * idx + 1];
-            ep[slot][2] = P[3ll * idx + 2];
+            R.ep[slot][0] = P[3ll * idx];
+            R.ep[slot][1] = P[3ll * idx + 1];
+            R.ep[slot][2] = P[3ll * idx + 2];
         }
         __syncthreads();
     }
-    // rough hull of the 80 extreme points, all waves cooperating (PPL 1 x 256 lanes)
-    constexpr int NE = 2 * KDOP_AXES_N;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 128, 256), &sb);
+}
+
+// Launch 2, one wave per set: the rough hull of the extreme points (exact, on
+// a grid of their own box), its FP64 volume for the bound pruning, and the
+// live rough faces as the filter's planes. A set without a usable rough hull
+// is hulled whole; a pruned one keeps the rough volume as its bound.
+constexpr int KD_VC = 128;      // hull vertex capacity (the 80 extreme points)
+constexpr unsigned KD_WS = 16384;  // LDS of its workspace (la_launch_kdop checks the size)
+__global__ __launch_bounds__(64) void la_kdop_rough_kernel(LaHullTask* tasks, const int* ids, int n,
+                                                           const int* dcount, KdopRec* rec, int2* res) {
+    __shared__ HullBlockLDS sb;
+    __shared__ __align__(16) char lds[KD_WS];
+    __shared__ float ep[KD_NE][3];
+    __shared__ int s_npl;
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    KdopRec& R = rec[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int np = t.n;
+    for (int i = tid; i < 3 * KD_NE; i += 64) (&ep[0][0])[i] = (&R.ep[0][0])[i];
+    __syncthreads();
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, KD_VC, KD_VC), &sb);
     float emn[3] = {1e30f, 1e30f, 1e30f}, emx[3] = {-1e30f, -1e30f, -1e30f};
-    for (int i = 0; i < NE; ++i)
+    for (int i = 0; i < KD_NE; ++i)
         for (int j = 0; j < 3; ++j) { emn[j] = fminf(emn[j], ep[i][j]); emx[j] = fmaxf(emx[j], ep[i][j]); }
     const QFrame rf = qframe_make(emn, emx);
-    int px[1] = {0}, py[1] = {0}, pz[1] = {0};
-    if (tid < NE) {
-        const float v[3] = {ep[tid][0], ep[tid][1], ep[tid][2]};
-        px[0] = qcoord(rf, 0, sel3(v, rf.ax[0]));
-        py[0] = qcoord(rf, 1, sel3(v, rf.ax[1]));
-        pz[0] = qcoord(rf, 2, sel3(v, rf.ax[2]));
+    // extreme point i = k * 64 + tid in slot k
+    int px[2] = {0, 0}, py[2] = {0, 0}, pz[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = k * 64 + tid;
+        if (i < KD_NE) {
+            const float v[3] = {ep[i][0], ep[i][1], ep[i][2]};
+            px[k] = qcoord(rf, 0, sel3(v, rf.ax[0]));
+            py[k] = qcoord(rf, 1, sel3(v, rf.ax[1]));
+            pz[k] = qcoord(rf, 2, sel3(v, rf.ax[2]));
+        }
     }
-    HullResult r = block_quickhull<1, AS_LDS, KD_NW>(w, &sb, NE, px, py, pz);
-    if (r.status != HULL_OK) {
-        // degenerate extremes (flat set): no prefilter, hull the full set
-        if (tid == 0) { res[blockIdx.x] = make_int2(np, 0); t.use_frame = 1; t.frame = full; t.filt = nullptr; t.nf = 0; }
-        return;
+    const QFrame full = R.full;
+    HullResult r = block_quickhull<2, AS_LDS, 1>(w, &sb, KD_NE, px, py, pz);
+    HullCanon rc;
+    rc.nt = 0;
+    if (r.status == HULL_OK) {
+        auto epos = [&](int p, float* o3) { o3[0] = ep[p][0]; o3[1] = ep[p][1]; o3[2] = ep[p][2]; };
+        rc = hull_canon<AS_LDS, 1>(w, &sb, r, KD_NE, epos);
     }
-    auto epos = [&](int p, float* o3) { o3[0] = ep[p][0]; o3[1] = ep[p][1]; o3[2] = ep[p][2]; };
-    const HullCanon rc = hull_canon<AS_LDS, KD_NW>(w, &sb, r, NE, epos);
-    if (rc.nt <= 0) {
-        // no usable rough hull: no prefilter, hull the full set
-        if (tid == 0) { res[blockIdx.x] = make_int2(np, 0); t.use_frame = 1; t.frame = full; t.filt = nullptr; t.nf = 0; }
+    if (r.status != HULL_OK || rc.nt <= 0) {
+        // degenerate extremes (flat set) or no usable rough hull: no
+        // prefilter, hull the full set
+        if (tid == 0) {
+            R.status = 1;
+            res[blockIdx.x] = make_int2(np, 0);
+            t.use_frame = 1; t.frame = full; t.filt = nullptr; t.nf = 0;
+        }
         return;
     }
     const double rough = rc.vol;
@@ -378,6 +414,7 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         const double gap = fmax(rough - t.mesh_vol, 0.0);
         if (gap > t.prune_gap) {
             if (tid == 0) {
+                R.status = 2;
                 t.vol = rough;
                 t.pruned = 1;
                 t.nf = 0;
@@ -387,48 +424,73 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
             return;
         }
     }
-    // exact face planes of the rough hull on the full set's grid: a point is
-    // dropped only when it lies strictly inside every face, i.e. strictly
-    // inside a solid whose vertices are points of the set, so it cannot be an
-    // extreme point; quantized duplicates share the decision. (Both grids are
-    // right-handed, so the faces keep their outward orientation.)
+    // the live faces (hull_canon marks them), in face-id order
+    if (tid == 0) {
+        int c = 0;
+        for (int f = 0; f < r.ftop && c < KD_MAXPL; ++f) {
+            if (!(w.fvis[f] & 16)) continue;
+            const u4v fv = w.fv[f];
+            R.face[c][0] = (unsigned char)w.vpid[fv.x];
+            R.face[c][1] = (unsigned char)w.vpid[fv.y];
+            R.face[c][2] = (unsigned char)w.vpid[fv.z];
+            ++c;
+        }
+        s_npl = c;
+        R.npl = c;
+        R.status = 0;
+    }
+}
+
+// Launch 3: the exact face planes of the rough hull on the full set's grid; a
+// point is dropped only when it lies strictly inside every face, i.e.
+// strictly inside a solid whose vertices are points of the set, so it cannot
+// be an extreme point; quantized duplicates share the decision. (Both grids
+// are right-handed, so the faces keep their outward orientation.) The kept
+// points, in input order, go to a scratch copy.
+__global__ __launch_bounds__(KD_T) void la_kdop_filter_kernel(LaHullTask* tasks, const int* ids, int n,
+                                                              const int* dcount, const KdopRec* rec, int2* res,
+                                                              DPool* pool, int* err) {
+    __shared__ double4 dpl[KD_MAXPL];
+    __shared__ float4 fpl[KD_MAXPL];  // the same planes in FP32 ...
+    __shared__ float fbd[KD_MAXPL];   // ... and a bound on that form's error
+    __shared__ int s_cnt[KD_NW];
+    __shared__ unsigned long long s_filt;
+    if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
+    const KdopRec& R = rec[blockIdx.x];
+    if (R.status != 0) return;
+    LaHullTask& t = tasks[ids[blockIdx.x]];
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const float* P = t.pts;
+    const int np = t.n;
+    const QFrame full = R.full;
+    const int npl = R.npl;
     auto fq = [&](int p, double* o) {
-        const float v[3] = {ep[p][0], ep[p][1], ep[p][2]};
+        const float v[3] = {R.ep[p][0], R.ep[p][1], R.ep[p][2]};
         o[0] = (double)qcoord(full, 0, sel3(v, full.ax[0]));
         o[1] = (double)qcoord(full, 1, sel3(v, full.ax[1]));
         o[2] = (double)qcoord(full, 2, sel3(v, full.ax[2]));
     };
-    if (tid == 0) {
-        int c = 0;
-        for (int f = 0; f < r.ftop && c < 2 * 128 + 8; ++f) {
-            if (!(w.fvis[f] & 16)) continue;  // live face (hull_canon reused fn)
-            const u4v fv = w.fv[f];
-            double a[3], b[3], cc[3];
-            fq(w.vpid[fv.x], a);
-            fq(w.vpid[fv.y], b);
-            fq(w.vpid[fv.z], cc);
-            const double e1x = b[0] - a[0], e1y = b[1] - a[1], e1z = b[2] - a[2];
-            const double e2x = cc[0] - a[0], e2y = cc[1] - a[1], e2z = cc[2] - a[2];
-            const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-            const double pw = nx * a[0] + ny * a[1] + nz * a[2];
-            dpl[c] = make_double4(nx, ny, nz, pw);
-            // FP32 form: the four conversions and three roundings of the
-            // fused chain each err by at most 2^-24 of T = (|nx| + |ny| +
-            // |nz|) 8192 + |w| (grid coordinates are below 8192 in
-            // magnitude); 2^-20 T is four times that sum
-            fpl[c] = make_float4((float)nx, (float)ny, (float)nz, (float)pw);
-            fbd[c] = (float)((fabs(nx) + fabs(ny) + fabs(nz)) * 8192.0 + fabs(pw)) * 0x1p-20f;
-            ++c;
-        }
-        s_cnt[KD_NW] = c;
+    for (int c = tid; c < npl; c += KD_T) {
+        double a[3], b[3], cc[3];
+        fq(R.face[c][0], a);
+        fq(R.face[c][1], b);
+        fq(R.face[c][2], cc);
+        const double e1x = b[0] - a[0], e1y = b[1] - a[1], e1z = b[2] - a[2];
+        const double e2x = cc[0] - a[0], e2y = cc[1] - a[1], e2z = cc[2] - a[2];
+        const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+        const double pw = nx * a[0] + ny * a[1] + nz * a[2];
+        dpl[c] = make_double4(nx, ny, nz, pw);
+        // FP32 form: the four conversions and three roundings of the fused
+        // chain each err by at most 2^-24 of T = (|nx| + |ny| + |nz|) 8192 +
+        // |w| (grid coordinates are below 8192 in magnitude); 2^-20 T is four
+        // times that sum
+        fpl[c] = make_float4((float)nx, (float)ny, (float)nz, (float)pw);
+        fbd[c] = (float)((fabs(nx) + fabs(ny) + fabs(nz)) * 8192.0 + fabs(pw)) * 0x1p-20f;
     }
-    __syncthreads();
-    const int npl = s_cnt[KD_NW];
     // filtered buffer: the points on or outside some rough face, in input order
-    if (tid == 0) sb.red[0] = (unsigned long long)heap_alloc(pool, 1, 12ull * (np + NE), err);
+    if (tid == 0) s_filt = (unsigned long long)heap_alloc(pool, 1, 12ull * (np + KD_NE), err);
     __syncthreads();
-    float* filt = (float*)sb.red[0];
-    __syncthreads();
+    float* filt = (float*)s_filt;
     if (!filt) {
         if (tid == 0) { t.err = KERR_KDOP_SCRATCH_OOM; res[blockIdx.x] = make_int2(-1, 0); }
         return;
@@ -479,6 +541,22 @@ __global__ __launch_bounds__(KD_T) void la_kdop_kernel(LaHullTask* tasks, const 
         t.use_frame = 1;
         res[blockIdx.x] = make_int2(base, 0);
     }
+}
+
+// The three launches of the prefilter over ids[0, n) (or the first *dcount).
+int la_launch_kdop(gpu_ctx_t ctx, LaHullTask* d_tasks, const int* d_ids, int n, const int* dcount, int2* d_res,
+                   KdopRec* d_rec, hipStream_t s) {
+    if (n <= 0) return 0;
+    REQUIRE(hull_ws_bytes<AS_LDS>(KD_VC, KD_VC) <= KD_WS, "k-DOP rough hull workspace exceeds its LDS");
+    hipLaunchKernelGGL(la_kdop_ext_kernel, dim3((unsigned)n), dim3(KD_T), 0, s, d_tasks, d_ids, n, dcount, d_rec);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(la_kdop_rough_kernel, dim3((unsigned)n), dim3(64), 0, s, d_tasks, d_ids, n, dcount, d_rec,
+                       d_res);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(la_kdop_filter_kernel, dim3((unsigned)n), dim3(KD_T), 0, s, d_tasks, d_ids, n, dcount,
+                       (const KdopRec*)d_rec, d_res, ctx->d_pool, ctx->d_err);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 // Sets above one workgroup's reach (HULL_NP_MAX points after the prefilter)
@@ -1894,14 +1972,14 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
         REQUIRE(big.size() < (1u << 31), "too many hull tasks");
         int* d_ids = hc.alloc<int>(big.size());
         int2* d_res = hc.alloc<int2>(big.size());
-        REQUIRE(d_ids && d_res, "lookahead: k-DOP buffers allocation failed");
+        KdopRec* d_rec = hc.alloc<KdopRec>(big.size());
+        REQUIRE(d_ids && d_res && d_rec, "lookahead: k-DOP buffers allocation failed");
         HIP_TRY(hipMemcpyAsync(d_ids, big.data(), 4 * big.size(), hipMemcpyHostToDevice, ctx->stream));
         double kbytes = 0;
         for (int i : big) kbytes += 12.0 * h_tasks[i].n;  // points read (the filtered copy is extra)
         hipEvent_t t0 = kt_begin(ctx);
-        hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)big.size()), dim3(KD_T), 0, ctx->stream, d_tasks, d_ids,
-                           (int)big.size(), (const int*)nullptr, d_res, ctx->d_pool, ctx->d_err);
-        HIP_TRY(hipGetLastError());
+        if (int rc = la_launch_kdop(ctx, d_tasks, d_ids, (int)big.size(), nullptr, d_res, d_rec, ctx->stream))
+            return rc;
         kt_end(ctx, KT_KDOP, t0, kbytes);
         std::vector<int2> res(big.size());
         HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -2058,7 +2136,8 @@ __global__ __launch_bounds__(256) void la_hdev_union_kernel(LaHullTask* tasks, c
 size_t la_hull_dev_work_bytes(int n) {
     const size_t nn = (size_t)std::max(n, 1);
     return 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256 +
-           (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256;
+           (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256 + (4 * nn + 255) / 256 * 256 + (8 * nn + 255) / 256 * 256 +
+           (sizeof(KdopRec) * nn + 255) / 256 * 256;
 }
 
 size_t la_hull_dev_arena_bytes(gpu_ctx_t ctx, int n) {
@@ -2085,6 +2164,7 @@ int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, ch
     int* lists = (int*)((char*)res + (8 * nn + 255) / 256 * 256);
     int* over = (int*)((char*)lists + (4 * nn * LA_HDEV_TIERS + 255) / 256 * 256);
     int2* range = (int2*)((char*)over + (4 * nn + 255) / 256 * 256);
+    KdopRec* krec = (KdopRec*)((char*)range + (8 * nn + 255) / 256 * 256);
     // without the prefilter every set is binned by its own size (the chunk
     // hulls of oversized sets, as hull_oversized hulls them)
     const int kmin = prefilter ? std::min(kdop_min_points(), LA_WAVE_MAX) : 0x7fffffff;
@@ -2094,9 +2174,9 @@ int la_launch_hull_dev(gpu_ctx_t ctx, LaHullTask* d_tasks, int n, char* work, ch
         hipLaunchKernelGGL(la_hdev_prelist_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks,
                            n, kmin, counters, big);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(la_kdop_kernel, dim3((unsigned)n), dim3(KD_T), 0, ctx->stream, d_tasks, (const int*)big,
-                           n, (const int*)(counters + LA_HDEV_BIG), res, ctx->d_pool, ctx->d_err);
-        HIP_TRY(hipGetLastError());
+        if (int rc = la_launch_kdop(ctx, d_tasks, (const int*)big, n, (const int*)(counters + LA_HDEV_BIG), res, krec,
+                                    ctx->stream))
+            return rc;
     }
     hipLaunchKernelGGL(la_hdev_bin_kernel, dim3(gb), dim3(256), 0, ctx->stream, (const LaHullTask*)d_tasks, n, kmin,
                        g_tune[4], (const int*)big, (const int2*)res, counters, lists, over);
