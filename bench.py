@@ -352,6 +352,7 @@ def gpu_lookahead(ctx, meshes, steps, warmup, tau=0.05, verbose=True):
     iters = np.array([i["iters"] for i in info])
     out = {"ms": ms, "parts": int(n_parts.sum()), "iters": iters, "max_conc": max_concavity(info),
            "stats": ctx.last_lookahead_stats, "sections": ctx.last_lookahead_sections,
+           "sections_gpu": ctx.last_lookahead_sections_gpu, "sections_syncs": ctx.last_lookahead_sections_syncs,
            "kernels": ctx.last_lookahead_kernels}
     if verbose:
         log(f"plain tau {tau}: step {ms / 1e3:.2f} s; parts {n_parts.sum()} (per mesh {n_parts.min()}-"
@@ -381,6 +382,10 @@ def lookahead_kernels(ctx, r):
     sec = r["sections"]
     log("  host sections: " + ", ".join(f"{k} {v / 1e3:.2f} s" for k, v in sec.items()) +
         f" (sum {sum(sec.values()) / 1e3:.2f} s)")
+    sg, sy = r["sections_gpu"], r["sections_syncs"]
+    log("  outside GPU phases, by section: " +
+        ", ".join(f"{k} {(sec[k] - sg[k]) / 1e3:.3f} s / {sy[k]} waits" for k in sec) +
+        f" (sum {(sum(sec.values()) - sum(sg.values())) / 1e3:.2f} s, {sum(sy.values())} host waits)")
     kern = {}
     for ph, name in (("leaf_scoring", "la_score_kernel"), ("level0_cuts", "la_cut_kernel")):
         kern[name] = stats[ph]
@@ -768,6 +773,10 @@ def main():
                        "max_part_concavity": plain["max_conc"], "iterations_max": int(plain["iters"].max())},
             "roofline": roof,
             "cpu_baseline": cb,
+            # host wall time of the step by section of the search, the GPU-phase
+            # time inside each section and the host waits on the stream
+            "host_sections": {k: {"wall_ms": plain["sections"][k], "gpu_phase_ms": plain["sections_gpu"][k],
+                                  "host_waits": plain["sections_syncs"][k]} for k in plain["sections"]},
         }
         ks = [int(x) for x in str(args.top_k).split(",")]
         top_ks = {0.05: ks[0], 0.03: ks[-1]}

@@ -498,7 +498,7 @@ class Context:
             raise ValueError("width + 4*n_concave_edges must be < 512")
         iters = np.zeros(n, np.int32)
         NP_, NS_, NK_ = len(LOOKAHEAD_PHASES), len(LOOKAHEAD_SECTIONS), len(LOOKAHEAD_KERNELS)
-        stats = np.zeros(3 * NP_ + NS_ + 3 * NK_ + 3, np.float64)
+        stats = np.zeros(3 * NP_ + NS_ + 3 * NK_ + 3 + 2 * NS_, np.float64)
         _gpu.lookahead_set_proposal(proposal.h if proposal is not None else 0, int(top_k))
         try:
             raw = _gpu.lookahead_batch(n, v.ctypes.data, vo.ctypes.data, t.ctypes.data, to.ctypes.data,
@@ -518,6 +518,9 @@ class Context:
         kt = stats[k0:k0 + 3 * NK_].reshape(NK_, 3)
         pr = stats[k0 + 3 * NK_:]
         self.last_lookahead_proposal = {"kept": int(pr[0]), "ranked": int(pr[1]), "fallbacks": int(pr[2])}
+        # per section: the GPU-phase time inside it (ms) and the host waits on the stream
+        self.last_lookahead_sections_gpu = {name: float(pr[3 + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
+        self.last_lookahead_sections_syncs = {name: int(pr[3 + NS_ + i]) for i, name in enumerate(LOOKAHEAD_SECTIONS)}
         self.last_lookahead_kernels = {name: {"ms": float(kt[i, 0]), "launches": int(kt[i, 1]),
                                               "alg_bytes": float(kt[i, 2])}
                                        for i, name in enumerate(LOOKAHEAD_KERNELS)}

@@ -61,7 +61,14 @@ struct gpu_ctx {
     void* record;  // level-0 training record of the search (lookahead.hip), null when off
     long long chunk_subtasks;  // chunk hulls of oversized sets, counted since gpu_init
     hipEvent_t ev_tier[2];     // first passes of the eight-wave LDS hull tiers (launch order)
+    long long host_syncs;      // host waits on the context's streams, counted since gpu_init
 };
+
+// A host wait for a stream of the context, counted in host_syncs.
+inline hipError_t ctx_stream_sync(gpu_ctx_t ctx, hipStream_t s) {
+    ++ctx->host_syncs;
+    return hipStreamSynchronize(s);
+}
 
 namespace cuacd {
 

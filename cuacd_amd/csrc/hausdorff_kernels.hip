@@ -331,7 +331,7 @@ int gpu_test_hausdorff(gpu_ctx_t ctx, const float* hv, int hnv, const int* ht, i
     if (rc) return rc;
     float h[2];
     HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     *out = sqrtf(fmaxf(h[0], h[1]));
     return 0;
 }
@@ -375,7 +375,7 @@ static int plan_create(gpu_ctx_t ctx, const float* d_av, const int* d_at, const 
         return -1;
     }
     HIP_TRY(hipMemcpyAsync(p->d_tasks, t.data(), sizeof(HdTask) * t.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     if (capture && p->n_tasks > 0) {
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const int rc = launch_tasks(ctx, p->d_tasks, p->n_tasks, p->d_ws);
@@ -416,7 +416,7 @@ int gpu_hausdorff_plan_run(gpu_ctx_t ctx, hd_plan_t plan) {
 
 void gpu_hausdorff_plan_destroy(gpu_ctx_t ctx, hd_plan_t plan) {
     if (!plan) return;
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx) (void)ctx_stream_sync(ctx, ctx->stream);
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
     (void)hipFree(plan->d_tasks);
@@ -462,7 +462,7 @@ int gpu_hausdorff_batch(gpu_ctx_t ctx, const float* av, const int* avoff, const 
             HIP_TRY(hipMemcpyAsync(out_samples_a, d_sa, smp, hipMemcpyDeviceToHost, ctx->stream));
         if (out_samples_b && d_sb)
             HIP_TRY(hipMemcpyAsync(out_samples_b, d_sb, smp, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         for (int i = 0; i < n_pairs; ++i) {
             out_ab[i] = sqrtf(out_ab[i]);
             out_ba[i] = sqrtf(out_ba[i]);

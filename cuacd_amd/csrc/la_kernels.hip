@@ -1872,7 +1872,7 @@ static int hull_oversized(gpu_ctx_t ctx, LaHullTask* d_tasks, std::vector<std::p
         for (size_t i = 0; i < over.size(); ++i)
             HIP_TRY(hipMemcpyAsync(&par[i], d_tasks + over[i].first, sizeof(LaHullTask), hipMemcpyDeviceToHost,
                                    ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         std::vector<LaHullTask> sub;
         std::vector<int> sub_of;
         for (size_t i = 0; i < over.size(); ++i) {
@@ -1908,7 +1908,7 @@ static int hull_oversized(gpu_ctx_t ctx, LaHullTask* d_tasks, std::vector<std::p
         }
         HIP_TRY(hipMemcpyAsync(sub.data(), d_sub, sizeof(LaHullTask) * sub.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         std::vector<LaUnionSeg> segs;
         std::vector<int> ids, off(1, 0), tot(over.size(), 0);
         for (size_t i = 0, k = 0; i < over.size(); ++i) {
@@ -1983,7 +1983,7 @@ int la_launch_hull(gpu_ctx_t ctx, LaHullTask* d_tasks, const LaHullTask* h_tasks
         kt_end(ctx, KT_KDOP, t0, kbytes);
         std::vector<int2> res(big.size());
         HIP_TRY(hipMemcpyAsync(res.data(), d_res, sizeof(int2) * big.size(), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         std::vector<std::pair<int, int>> fin, over;
         for (size_t i = 0; i < big.size(); ++i) {
             if (res[i].y) continue;          // pruned: volume is the rough bound

@@ -209,6 +209,8 @@ struct Engine {
     std::vector<void*> to_free;  // device heap chunks to free at the next flush
     double stats[PH_COUNT][3] = {};
     double sect[S_COUNT] = {};
+    double sect_gpu[S_COUNT] = {}, sect_sync[S_COUNT] = {}, gpu_pending = 0.0;
+    long long sync_mark = 0;
     std::chrono::steady_clock::time_point t_mark = std::chrono::steady_clock::now();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int hull_phase = PH_IHULL;
@@ -259,7 +261,7 @@ struct Engine {
                                ctx->stream) != hipSuccess ||
                 la_launch_segcopy(ctx, dsegs.p, (int)segs.size()) != 0 ||
                 hipMemcpyAsync(pinned, stage.p, total, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-                hipStreamSynchronize(ctx->stream) != hipSuccess) {
+                ctx_stream_sync(ctx, ctx->stream) != hipSuccess) {
                 set_error(ctx, "lookahead: batched readback failed");
                 *rc = -1;
                 return nullptr;
@@ -268,10 +270,15 @@ struct Engine {
         return pinned;
     }
 
+    // a section's wall time, the GPU-phase time inside it and its host waits
     void lap(int s) {
         const auto t = std::chrono::steady_clock::now();
         sect[s] += std::chrono::duration<double, std::milli>(t - t_mark).count();
         t_mark = t;
+        sect_gpu[s] += gpu_pending;
+        gpu_pending = 0.0;
+        sect_sync[s] += (double)(ctx->host_syncs - sync_mark);
+        sync_mark = ctx->host_syncs;
     }
     int tbeg() {
         if (!ev0) {
@@ -283,6 +290,7 @@ struct Engine {
     }
     int tend(int ph, double bytes, int launches = 1) {
         HIP_TRY(hipEventRecord(ev1, ctx->stream));
+        ++ctx->host_syncs;
         HIP_TRY(hipEventSynchronize(ev1));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
@@ -290,6 +298,7 @@ struct Engine {
         stats[ph][0] += ms;
         stats[ph][1] += launches;
         stats[ph][2] += bytes;
+        gpu_pending += ms;
         return 0;
     }
     ~Engine() {
@@ -300,7 +309,7 @@ struct Engine {
     }
 
     int check_err(const char* where) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         int e = read_and_clear_err(ctx);
         if (e) {
             char buf[256];
@@ -318,7 +327,7 @@ struct Engine {
         HIP_TRY(hipMemcpyAsync(d.p, to_free.data(), sizeof(void*) * to_free.size(), hipMemcpyHostToDevice,
                                ctx->stream));
         LA_TRY(la_launch_free(ctx, d.p, (int)to_free.size()));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         to_free.clear();
         return 0;
     }
@@ -593,7 +602,7 @@ struct Engine {
         LA_TRY(la_launch_cut(ctx, dt.p, (int)h.size(), max_nv));
         LA_TRY(tend(PH_CUT, 0.0));
         HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaCutTask) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         // mesh read, both halves written, volumes
         for (auto& x : h) stats[PH_CUT][2] += 12.0 * (x.nv + x.nt + x.pnv + x.pnt + x.nnv + x.nnt) + 16.0;
         return check_err("cuts");
@@ -1131,7 +1140,7 @@ struct Engine {
         R->pts.resize(base + sel.size() * MLP_SAMPLES * 3);
         HIP_TRY(hipMemcpyAsync(R->pts.data() + base, d.p, sizeof(float) * sel.size() * MLP_SAMPLES * 3,
                                hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         for (size_t i = 0; i < sel.size(); ++i) {
             const int c = sel[i];
             const Part& p = M[cutting[c].first].parts[cutting[c].second];
@@ -1605,7 +1614,7 @@ struct Engine {
                     LA_TRY(tend(PH_SCORE, 0.0));
                     HIP_TRY(hipMemcpyAsync(h.data(), dt.p, sizeof(LaScoreTask) * h.size(), hipMemcpyDeviceToHost,
                                            ctx->stream));
-                    HIP_TRY(hipStreamSynchronize(ctx->stream));
+                    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
                     {
                         // mesh read, both point sets written, volumes and counts
                         double by = 0;
@@ -1873,6 +1882,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     }
     Engine E;
     E.ctx = ctx;
+    E.sync_mark = ctx->host_syncs;
     // per-kernel timers for the run; detached on every exit path
     struct KtGuard {
         gpu_ctx_t c;
@@ -2018,7 +2028,7 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
     HIP_TRY(hipFreeAsync(d_t, ctx->stream));
     HIP_TRY(hipFreeAsync(d_hv, ctx->stream));
     HIP_TRY(hipFreeAsync(d_ht, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     E.lap(S_OUT);
     if (stats_out) {
         for (int ph = 0; ph < PH_COUNT; ++ph)
@@ -2031,6 +2041,10 @@ static int lookahead_run(gpu_ctx_t ctx, int n_meshes, const float* verts, const 
         pr[0] = (double)E.proposal_kept;
         pr[1] = (double)E.proposal_total;
         pr[2] = (double)E.proposal_fallbacks;
+        for (int sc = 0; sc < S_COUNT; ++sc) {
+            pr[3 + sc] = E.sect_gpu[sc];
+            pr[3 + S_COUNT + sc] = E.sect_sync[sc];
+        }
     }
     if (rc) {
         for (int m = 0; m < n_meshes; ++m) gpu_result_free(&out[m]);
@@ -2093,6 +2107,7 @@ extern "C" int gpu_lookahead_candidates(gpu_ctx_t ctx, int n_meshes, const float
     if (n_meshes == 0) return 0;
     Engine E;
     E.ctx = ctx;
+    E.sync_mark = ctx->host_syncs;
     E.P = Params{};
     E.P.width = width;
     E.P.threshold = threshold;
@@ -2155,7 +2170,7 @@ extern "C" int gpu_lookahead_candidates(gpu_ctx_t ctx, int n_meshes, const float
     }
     (void)hipFreeAsync(d_v, ctx->stream);
     (void)hipFreeAsync(d_t, ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)ctx_stream_sync(ctx, ctx->stream);
     return rc;
 }
 

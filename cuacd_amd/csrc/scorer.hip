@@ -46,7 +46,7 @@ float sc_cost_rv(double hv, double mv) {
 }
 
 int sc_check(gpu_ctx_t ctx, const char* where) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     const int e = read_and_clear_err(ctx);
     if (e) {
         char buf[256];
@@ -173,7 +173,7 @@ int score_run(gpu_ctx_t ctx, const float* verts, const int* vert_off, const int*
         HIP_TRY(hipMemcpyAsync(d.p, chunks.data(), sizeof(void*) * chunks.size(), hipMemcpyHostToDevice,
                                ctx->stream));
         LA_SC_TRY(la_launch_free(ctx, d.p, (int)chunks.size()));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
         chunks.clear();
         return 0;
     };
@@ -477,7 +477,7 @@ int plan_finish(score_plan* p, float* out_score, double* out_detail) {
     HIP_TRY(hipMemcpyAsync(counts, p->dcounts, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
     int over = 0;  // halves left unhulled by the device-sized call
     HIP_TRY(hipMemcpyAsync(&over, p->hwork + 4 * LA_HDEV_FAIL, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     const int e = read_and_clear_err(ctx);
     if (e) {
         char buf[256];
@@ -492,7 +492,7 @@ int plan_finish(score_plan* p, float* out_score, double* out_detail) {
     if (nhd > 0) {
         HIP_TRY(hipMemcpyAsync(d2.data(), p->dd2, 4ull * nhd, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(half.data(), p->dhalf, 4ull * (nhd / 2), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx_stream_sync(ctx, ctx->stream));
     }
     std::vector<double> mv(nh, 0.0), hv(nh, 0.0);
     std::vector<float> crv(nh, 0.0f), hd(nh, -1.0f);
@@ -589,7 +589,7 @@ extern "C" int gpu_score_plan_create(gpu_ctx_t ctx, const float* verts, const in
         hipMemcpyAsync(p->dt, tris, 12 * tt, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
         hipMemcpyAsync(p->dct, p->ct.data(), sizeof(LaCutTask) * n_cuts, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        ctx_stream_sync(ctx, ctx->stream) != hipSuccess)
         rc = -1;
     // one uncaptured pass of the device-sized launches (kernel attributes,
     // constant tables), checked against the eager scores, then the capture
@@ -641,6 +641,6 @@ extern "C" int gpu_score_plan_run(gpu_ctx_t ctx, score_plan_t plan, float* out_s
 
 extern "C" void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan) {
     if (!plan) return;
-    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx) (void)ctx_stream_sync(ctx, ctx->stream);
     plan_free(plan);
 }

@@ -137,7 +137,7 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * convex hulls (hull_vert_off/hull_tri_off), decomposed independently with the
  * same parameters in one search whose launches span all meshes. out[m] gets
  * mesh m's parts (free each with gpu_result_free); out_iters[m] (optional) the
- * iterations it ran; out_stats (optional, 80 doubles): [0, 27) per phase -
+ * iterations it ran; out_stats (optional, 96 doubles): [0, 27) per phase -
  * components, part hulls, Hausdorff, concave edges, level-0 cuts, leaf
  * scoring, item hulls, merge, learned proposal - the GPU milliseconds,
  * launches and algorithmic bytes; [27, 35) host wall milliseconds by section
@@ -149,7 +149,9 @@ int lookahead_decompose(gpu_ctx_t ctx, const float* verts, int nv, const int* tr
  * and 4096-point first passes - the milliseconds between HIP events around
  * each launch, launches and algorithmic bytes (points read, volumes written);
  * [77, 80) learned proposal: level-0 candidates kept, candidates ranked,
- * fallbacks (gpu_lookahead_set_proposal). */
+ * fallbacks (gpu_lookahead_set_proposal); [80, 88) per host section, the GPU
+ * milliseconds of the phases inside it; [88, 96) per host section, the host
+ * waits on the context's streams. */
 int gpu_lookahead_batch(gpu_ctx_t ctx, int n_meshes, const float* verts, const int* vert_off, const int* tris,
                         const int* tri_off, const float* hull_verts, const int* hull_vert_off,
                         const int* hull_tris, const int* hull_tri_off, int max_iters, int width, int width2,
