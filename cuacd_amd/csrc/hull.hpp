@@ -108,7 +108,9 @@ template <int AS> struct HullWS {
     // multi-apex scratch (LDS workspaces with several waves; null otherwise)
     asptr<AS_LDS, uint32_t> mfm;    // [FC/2] per face, 16 bits (face f in word f/2, bits 16 (f&1) up):
                                     // bit r visited by candidate r, bit 8+r visible from it
-    asptr<AS_LDS, uint32_t> mfmax;  // [FC] per face: best conflict key (quantised distance, point index)
+    asptr<AS_LDS, uint32_t> mfmax;  // [FC] per face: best conflict key (quantised distance, point index);
+                                    // only for up to 8192 vertices (hull_ma_layout)
+    asptr<AS_LDS, unsigned long long> msk;  // [MA_W] per wave share: best (conflict key << 32 | face)
     asptr<AS_LDS, uint8_t> mfcone;  // [FC] candidate whose cone replaced the face this round
     asptr<AS_LDS, uint16_t> mlist;  // [MA_W][MA_CAP][5]: visible faces, ha, hb, hface, hnew (ranks >= 1)
     asptr<AS_LDS, uint8_t> mhedge;  // [MA_W][MA_CAP]
@@ -142,7 +144,10 @@ __host__ __device__ __forceinline__ unsigned hull_ma_layout(char* p, unsigned fc
         return r;
     };
     w->mfm = (asptr<AS_LDS, uint32_t>)take((fc + 1u) / 2u * 4u);
-    w->mfmax = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
+    w->msk = (asptr<AS_LDS, unsigned long long>)take(MA_W * 8u);
+    // per-face keys while they fit (2 + 4 + 1 bytes per face); above 8192
+    // vertices (3 bytes per face) the rounds reduce keys per wave share
+    if (fc <= 2u * 8192u + 8u) w->mfmax = (asptr<AS_LDS, uint32_t>)take(fc * 4u);
     w->mfcone = (asptr<AS_LDS, uint8_t>)take(fc);
     w->mlist = (asptr<AS_LDS, uint16_t>)take(MA_W * MA_CAP * 5 * 2u);
     w->mhedge = (asptr<AS_LDS, uint8_t>)take(MA_W * MA_CAP);
@@ -552,8 +557,9 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
         static_assert(NW > 1 && NW <= MA_W, "multi-apex rounds need several waves");
         // a conflict key is the face's best point: its quantised distance
         // above the high bits, its (inverted) local index in the low ones
-        static_assert(PPL * T <= 8192, "multi-apex keys hold point indices below 8192");
-        constexpr uint32_t IMASK = PPL * T <= 2048 ? 0x7FFu : (PPL * T <= 4096 ? 0xFFFu : 0x1FFFu);
+        static_assert(PPL * T <= 16384, "multi-apex keys hold point indices below 16384");
+        constexpr uint32_t IMASK =
+            PPL * T <= 2048 ? 0x7FFu : (PPL * T <= 4096 ? 0xFFFu : (PPL * T <= 8192 ? 0x1FFFu : 0x3FFFu));
         using L16 = typename MaList<AS, uint16_t>::type;
         using L8 = typename MaList<AS, uint8_t>::type;
         // the 16 mark bits of face f in its word
@@ -572,36 +578,70 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             __builtin_amdgcn_wave_barrier();
         };
+        if (tid < MA_W) w.msk[tid] = 0ull;
+        __syncthreads();
         for (int it = 1;; ++it) {
-            // A: clear the per-face marks and best keys
-            for (int f = tid; f < ftop; f += T) w.mfmax[f] = 0u;
-            for (int q = tid; q < (ftop + 1) >> 1; q += T) w.mfm[q] = 0u;
-            if (tid < MA_W) crow[tid] = 0;
-            __syncthreads();
-            // B: best conflict key per face (quantised distance, lowest index)
+            if constexpr (PPL * T <= 8192) {
+                // A: clear the per-face marks and best keys
+                for (int f = tid; f < ftop; f += T) w.mfmax[f] = 0u;
+                for (int q = tid; q < (ftop + 1) >> 1; q += T) w.mfm[q] = 0u;
+                if (tid < MA_W) crow[tid] = 0;
+                __syncthreads();
+                // B: best conflict key per face (quantised distance, lowest index)
 #pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                if (conf[k] >= 0) {
-                    const uint32_t key = (fkey(dist[k]) & ~IMASK) | (IMASK - (uint32_t)(k * T + tid));
-                    __hip_atomic_fetch_max(&w.mfmax[conf[k]], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (int k = 0; k < PPL; ++k)
+                    if (conf[k] >= 0) {
+                        const uint32_t key = (fkey(dist[k]) & ~IMASK) | (IMASK - (uint32_t)(k * T + tid));
+                        __hip_atomic_fetch_max(&w.mfmax[conf[k]], key, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                __syncthreads();
+                // C: candidates - the best face of each wave's faces (face
+                // ids interleaved over the waves, so every wave has faces)
+                uint64_t bk = 0;
+                for (int f = wv + NW * lane; f < ftop; f += NW * 64) {
+                    const uint32_t k32 = w.mfmax[f];
+                    const uint64_t v = ((uint64_t)k32 << 32) | (uint32_t)f;
+                    if (k32 && v > bk) bk = v;
                 }
-            __syncthreads();
-            // C: candidates - the best face of each wave's faces (face ids
-            // interleaved over the waves, so every wave has faces), ranked
-            uint64_t bk = 0;
-            for (int f = wv + NW * lane; f < ftop; f += NW * 64) {
-                const uint32_t k32 = w.mfmax[f];
-                const uint64_t v = ((uint64_t)k32 << 32) | (uint32_t)f;
-                if (k32 && v > bk) bk = v;
+                bk = wave_max_u64(bk);
+                if (lane == 0) w.msk[wv] = bk;
+                __syncthreads();
+            } else {
+                // A: clear the per-face marks. B (same interval: it reads no
+                // shared state): the same candidates without per-face keys -
+                // the best conflict point over the faces of each share (face
+                // f is in share f % NW), as (key << 32 | face): per lane and
+                // share, then over the wave, then over the waves
+                for (int q = tid; q < (ftop + 1) >> 1; q += T) w.mfm[q] = 0u;
+                if (tid < MA_W) crow[tid] = 0;
+                uint64_t sk[NW];
+#pragma unroll
+                for (int q = 0; q < NW; ++q) sk[q] = 0;
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    if (conf[k] >= 0) {
+                        const uint32_t key = (fkey(dist[k]) & ~IMASK) | (IMASK - (uint32_t)(k * T + tid));
+                        const uint64_t v = ((uint64_t)key << 32) | (uint32_t)conf[k];
+                        const int sh = conf[k] % NW;
+#pragma unroll
+                        for (int q = 0; q < NW; ++q)
+                            if (q == sh && v > sk[q]) sk[q] = v;
+                    }
+#pragma unroll
+                for (int q = 0; q < NW; ++q) {
+                    sk[q] = wave_max_u64(sk[q]);
+                    if (lane == 0 && sk[q])
+                        __hip_atomic_fetch_max(&w.msk[q], (unsigned long long)sk[q], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                __syncthreads();
             }
-            bk = wave_max_u64(bk);
-            if (lane == 0) sb->red[wv] = bk;
-            __syncthreads();
             uint64_t rk[NW];
             int ncand = 0;
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
-                rk[q] = sb->red[q];
+                rk[q] = w.msk[q];
                 ncand += rk[q] != 0;
             }
             if (ncand == 0) break;
@@ -625,6 +665,9 @@ __device__ __forceinline__ HullResult block_quickhull(const HullWS<AS> w, HullBl
                 }
             }
             __syncthreads();
+            // the share maxima were read before the barrier above: clear them
+            // for the next round (several barriers before its first update)
+            if (tid < MA_W) w.msk[tid] = 0ull;
             // D: wave r finds candidate r's visible region and, on the way, its
             // horizon: every visible face tests its three neighbours at once
             // (visit marks, planes and back links loaded together); the first

@@ -93,10 +93,10 @@ __device__ __forceinline__ bool la_hull_item(LaHullTask& t, const HullWS<AS> w, 
         }
     }
     HullResult r;
-    if constexpr (NW >= 8 && PPL * T <= 8192) {
+    if constexpr (NW >= 8 && PPL * T <= 16384) {
         // multi-apex rounds (the workspace was carved or attached with their
         // scratch; the four-wave tiers keep single-apex steps and do not
-        // compile them); the point index field of their keys holds 13 bits
+        // compile them); the point index field of their keys holds 14 bits
         if (multi && n <= PPL * T) r = block_quickhull<PPL, AS, NW, true>(w, sb, n, px, py, pz);
         else r = block_quickhull<PPL, AS, NW>(w, sb, n, px, py, pz);
     } else {
@@ -1433,8 +1433,8 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //     513-1024-point hull takes 0.38 ms on eight waves against 0.69 ms on
 //     four (two workgroups per CU against three), and the step went from
 //     2.05 s to 1.96 s; the 257-512-point tier packs better on four waves
-//   14 multi_hbm: multi-apex rounds in the HBM-workspace tiers of up to 8192
-//     points (their per-face marks and keys in LDS; CUACD_HULL_MULTI_HBM, 0/1;
+//   14 multi_hbm: multi-apex rounds in the HBM-workspace tiers (their
+//     per-face marks in LDS; CUACD_HULL_MULTI_HBM, 0/1;
 //     default 1). These tiers hold the hulls with the most corners, so the
 //     rounds pay most there: on the 256-mesh scorer workload the 4097-8192-
 //     point tier went from 260 ms to 67 ms per step (14.8k -> 27.0k cuts/s),
@@ -1654,10 +1654,10 @@ using BlockFn = void (*)(LaHullTask*, const int*, int, const int*, char*, unsign
                          DPool*, int*);
 
 // Multi-apex rounds for an HBM-workspace kernel of vc vertices (knob
-// multi_hbm; their LDS scratch fits up to 8192 vertices): the flag, and the
-// dynamic LDS it needs, set on the kernel.
+// multi_hbm; their LDS scratch, 3 bytes per face and 15.5 KB, fits up to
+// 16384 vertices): the flag, and the dynamic LDS it needs, set on the kernel.
 int block_multi(gpu_ctx_t ctx, BlockFn k, int vc, int* multi, unsigned* lds) {
-    *multi = (hull_multi_hbm() && vc <= 8192) ? 1 : 0;
+    *multi = (hull_multi_hbm() && vc <= 16384) ? 1 : 0;
     *lds = *multi ? hull_ma_bytes(vc) : 0u;
     if (*lds > 0) HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds));
     return 0;

@@ -285,7 +285,7 @@ void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
  * passes of the 513-1024-point (bit 0) and 257-512-point (bit 1) wave tiers
  * on eight waves with multi-apex rounds; bits 2 and 3: the same for the
  * full-capacity retries of these tiers; 0..15), 14 hull_multi_hbm
- * (multi-apex rounds in the HBM-workspace tiers of up to 8192 points, 0/1).
+ * (multi-apex rounds in the HBM-workspace tiers, 0/1).
  * Values are clamped to their ranges. */
 int gpu_set_tuning(gpu_ctx_t ctx, int key, int value);
 int gpu_get_tuning(gpu_ctx_t ctx, int key, int* value);
