@@ -26,6 +26,9 @@ DEFAULT_SPACE = {
     "hull_ldsv8_vcap": [448, 1024],
     "hull_ldsb8_vcap": [0, 1024],
     "hull_lpt": [0, 1],
+    "hull_order": [0, 1],
+    "hull_w8": [0, 1, 5],
+    "hull_multi_hbm": [0, 1],
 }
 
 CACHE_PATH = os.path.join(os.path.expanduser("~"), ".cache", "cuacd_amd", "tuning.json")

@@ -124,3 +124,34 @@ def test_autotuner_keeps_scores(ctx):
     finally:
         for k, v in start.items():
             ctx.set_tuning(k, v)
+
+
+def test_launch_knobs_keep_scores_on_large_halves(ctx):
+    """Halves of 64^3 SDF meshes reach every hull tier (257-16384 points after
+    the prefilter): the launch order, the eight-wave wave tiers, multi-apex
+    rounds in the LDS and the HBM-workspace tiers pick kernels only - every
+    score and every half's hull volume must be the same bits."""
+    meshes = sdf.sdf_meshes(4, seed=5, res=64, min_tris=12000, max_tris=60000)
+    planes = [datasets.cut_candidates(v, t, n_edges=6, n_total=12, seed=m) for m, (v, t) in enumerate(meshes)]
+    mesh_of = np.concatenate([np.full(len(p), m, np.int32) for m, p in enumerate(planes)])
+    planes = np.concatenate(planes)
+    start = ctx.get_tuning()
+    import cuacd_amd
+    ref_s, ref_d, st = ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=0.05, return_detail=True)
+    launches = {k: st[6 + 3 * i + 1] for i, k in enumerate(cuacd_amd.LOOKAHEAD_KERNELS)}
+    # the multi-wave tiers the knobs act on were used
+    assert launches["hull_w1024"] > 0 and launches["hull_b2048"] > 0 and launches["hull_b4096"] > 0, launches
+    variants = [{"hull_order": 0}, {"hull_w8": 0}, {"hull_w8": 15}, {"hull_multi_hbm": 0}, {"hull_multi": 0},
+                {"hull_multi": 0, "hull_multi_hbm": 0, "hull_w8": 0, "hull_order": 0}]
+    try:
+        for var in variants:
+            for k, v in var.items():
+                ctx.set_tuning(k, v)
+            s, d, _ = ctx.batch_score_cuts(meshes, mesh_of, planes, threshold=0.05, return_detail=True)
+            assert np.array_equal(s, ref_s), var
+            assert np.array_equal(d, ref_d), var
+            for k, v in start.items():
+                ctx.set_tuning(k, v)
+    finally:
+        for k, v in start.items():
+            ctx.set_tuning(k, v)
