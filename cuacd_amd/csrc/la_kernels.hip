@@ -634,7 +634,12 @@ __device__ inline void la_block_bbox(const float* v, int nv, float* lo, float* h
     (void)s;
 }
 
-__global__ __launch_bounds__(PC_T) void la_cut_kernel(LaCutTask* tasks, int n, DPool* pool, int* err, int lds_nv) {
+// at least four workgroups per CU (the caps' ear clipping is a serial chain
+// on one wave, so resident workgroups, not registers, set the throughput:
+// 128 VGPRs with some spills measured 149 ms per 64-mesh step against 168 ms
+// at three workgroups and 215 ms at two)
+__global__ __launch_bounds__(PC_T, 4) void la_cut_kernel(LaCutTask* tasks, int n, DPool* pool, int* err,
+                                                         int lds_nv) {
     __shared__ PcLDS s;
     __shared__ float scr[PC_T / 64][6];
     extern __shared__ __align__(16) unsigned char pc_dyn[];

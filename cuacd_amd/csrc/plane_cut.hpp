@@ -219,6 +219,20 @@ __device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv,
     mnu = wave_min_f(mnu); mxu = wave_max_f(mxu); mnv = wave_min_f(mnv); mxv = wave_max_f(mxv);
     const float ext = fmaxf(mxu - mnu, mxv - mnv);
     const float sliver = -1e-6f * ext * ext;
+    // up to 64 R vertices: lane l keeps vertices l + 64 k in registers with a
+    // removed bit each (a clipped vertex is exactly one whose predecessor no
+    // longer links to it), so the ear test reads no LDS
+    constexpr int R = 8;
+    const bool regs = n <= 64 * R;
+    float ru[R], rv[R];
+    uint32_t rem = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = lane + 64 * k;
+        ru[k] = 0.0f;
+        rv[k] = 0.0f;
+        if (regs && i < n) { ru[k] = uv[i].u; rv[k] = uv[i].v; }
+    }
     __threadfence_block();
     auto conv = [&](int p, int c, int q) { return cross2(uv[p], uv[c], uv[q]); };
     int nt = 0, remaining = n, cur = 0, lap = 0;
@@ -247,13 +261,23 @@ __device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv,
         } else {
             const P2 A = uv[p], B = uv[cur], C = uv[q];
             bool bad = false;
-            for (int i = lane; i < n; i += 64) {
-                if (i == p || i == cur || i == q) continue;
-                if (nxt[prv[i]] != i) continue;  // removed
-                const P2 X = uv[i];
-                if (fabsf(X.u - A.u) + fabsf(X.v - A.v) > 1e-6f && fabsf(X.u - B.u) + fabsf(X.v - B.v) > 1e-6f &&
-                    fabsf(X.u - C.u) + fabsf(X.v - C.v) > 1e-6f && pc_pt_in_tri(X, A, B, C))
-                    bad = true;
+            auto inside = [&](const P2& X) {
+                return fabsf(X.u - A.u) + fabsf(X.v - A.v) > 1e-6f && fabsf(X.u - B.u) + fabsf(X.v - B.v) > 1e-6f &&
+                       fabsf(X.u - C.u) + fabsf(X.v - C.v) > 1e-6f && pc_pt_in_tri(X, A, B, C);
+            };
+            if (regs) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int i = lane + 64 * k;
+                    if (i >= n || ((rem >> k) & 1u) || i == p || i == cur || i == q) continue;
+                    if (inside(P2{ru[k], rv[k]})) bad = true;
+                }
+            } else {
+                for (int i = lane; i < n; i += 64) {
+                    if (i == p || i == cur || i == q) continue;
+                    if (nxt[prv[i]] != i) continue;  // removed
+                    if (inside(uv[i])) bad = true;
+                }
             }
             ear = !__any(bad);
         }
@@ -265,6 +289,7 @@ __device__ inline int pc_earclip(const int* poly, int n, const P2* uv, int* prv,
                 nxt[p] = q;
                 prv[q] = p;
             }
+            if (lane == (cur & 63)) rem |= 1u << (cur >> 6);
             __threadfence_block();
             ++nt;
             --remaining;
