@@ -1387,7 +1387,8 @@ using WaveFn = void (*)(LaHullTask*, const int*, int, const int*, DPool*, int*);
 //     their point capacity (CUACD_HULL_VCAP8, 1..8; default 3)
 //   2 half_cap: reduced-capacity first passes on/off (CUACD_HULL_HALFCAP; default 1)
 //   3 kdop_min: sets above this many points are k-DOP prefiltered
-//     (CUACD_KDOP_MIN, >= 4; default 256)
+//     (CUACD_KDOP_MIN, >= 4; default 512: re-tuned on the 64-mesh search
+//     step with the current tiers, 1.71-1.73 s against 1.76-1.80 s at 256)
 //   4 chunk_above: prefiltered sets above this many points are hulled in
 //     chunks of half that size first (CUACD_HULL_CHUNK_ABOVE, 1024..16384;
 //     default 16384, the most one workgroup hulls)
@@ -1472,7 +1473,7 @@ void tune_init() {
                                   "CUACD_MLP_FUSED",       "CUACD_HULL_LDSV8_VCAP",  "CUACD_HULL_LDSB8_VCAP",
                                   "CUACD_CUT_LDS_VERTS",   "CUACD_HULL_LPT",         "CUACD_HULL_B4_SPLIT",
                                   "CUACD_HULL_ORDER",      "CUACD_HULL_W8",          "CUACD_HULL_MULTI_HBM"};
-    const int def[LA_TUNE_N] = {256, 3, 1, 256, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0, 1, 5, 1};
+    const int def[LA_TUNE_N] = {256, 3, 1, 512, HULL_NP_MAX, 1, 1, 1024, 1024, 98304, 0, 0, 1, 5, 1};
     for (int k = 0; k < LA_TUNE_N; ++k) {
         const char* e = getenv(env[k]);
         g_tune[k] = tune_clamp(k, e ? atoi(e) : def[k]);

@@ -266,7 +266,7 @@ void gpu_score_plan_destroy(gpu_ctx_t ctx, score_plan_t plan);
  * (wave-tier sets above it run four waves per hull; 0 disables), 1 vcap8
  * (first-pass vertex capacity in eighths, 1..8), 2 half_cap (reduced-capacity
  * first passes, 0/1), 3 kdop_min (k-DOP prefilter above this many points,
- * >= 4), 4 chunk_above (prefiltered sets above it are hulled in chunks of
+ * >= 4; default 512), 4 chunk_above (prefiltered sets above it are hulled in chunks of
  * half that size first, 1024..16384), 5 multi (several apexes per quickhull
  * round in the eight-wave LDS tiers, 0/1), 6 mlp_fused (the proposal
  * network's inference as one fused point-layer kernel, 0/1; same scores),

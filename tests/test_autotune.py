@@ -8,7 +8,7 @@ import cuacd_amd
 
 class FakeCtx:
     def __init__(self):
-        self.t = {"hull_wide_min": 256, "hull_vcap_eighths": 3, "hull_half_cap": 1, "kdop_min_points": 256,
+        self.t = {"hull_wide_min": 256, "hull_vcap_eighths": 3, "hull_half_cap": 1, "kdop_min_points": 512,
                   "hull_chunk_above": 16384, "hull_multi": 1, "hull_ldsv8_vcap": 1024,
                   "hull_ldsb8_vcap": 1024, "cut_lds_verts": 98304, "hull_lpt": 0,
                   "hull_b4_split": 0, "hull_order": 1, "hull_w8": 5, "hull_multi_hbm": 1}
