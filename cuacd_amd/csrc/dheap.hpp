@@ -99,12 +99,19 @@ __device__ inline char* dh_pop_from(DPool* pool, DHeap* hp, int c, int s, unsign
 }
 
 // Push the chain first..last (linked through next, offsets / 256) onto stripe s.
+// The CAS releases (the link and the blocks' last contents are visible to the
+// next owner's acquiring pop); it acquires nothing, since the pusher only
+// stores the old head into the link. On CDNA an agent-scope release writes
+// back the XCD's L2 and an acquire invalidates it, so a push pays the first
+// and not the second.
 __device__ inline void dh_push_chain(DHeap* hp, int c, int s, DBlockHdr* last, uint32_t first_off) {
     unsigned long long old = dh_head(hp, c, s);
     for (;;) {
         __hip_atomic_store(&last->next, (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned long long neu = (((old >> 32) + 1ull) << 32) | first_off;
-        if (cas_acq_rel(&hp->heads[c][s].v, old, neu)) break;
+        if (__hip_atomic_compare_exchange_strong(&hp->heads[c][s].v, &old, neu, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            break;
     }
 }
 
@@ -166,6 +173,58 @@ __device__ inline void heap_free(DPool* pool, void* p, int* err) {
     dh_push_chain(hp, c, s, hd, off);
     fetch_add_relaxed(&hp->cnt[s].outstanding, (unsigned long long)0 - bytes);
     fetch_add_relaxed(&hp->cnt[s].frees, 1ull);
+}
+
+// Free N blocks (nullptr entries are skipped) from one thread with a single
+// release: every link is stored against the head read just before, then one
+// agent-scope release fence, then one relaxed CAS per block. The fence
+// synchronizes with the acquiring pop that reads a head value written by any
+// of these CASes (or by a later CAS on the same head), so the links and the
+// blocks' contents are visible to the next owner as with a releasing CAS. A
+// CAS that finds the head moved (another push or pop, or an earlier block of
+// this call with the same class) pushes that block again with dh_push_chain.
+// An agent-scope release is a write-back of the XCD's whole L2 on CDNA (and
+// the wave waits for it): one per batch instead of one per block.
+template <int N>
+__device__ inline void heap_free_n(DPool* pool, void* const (&ps)[N], int* err) {
+    DBlockHdr* hd[N];
+    unsigned long long old[N];
+    const int s = dh_stripe();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        hd[i] = nullptr;
+        old[i] = 0;
+        if (!ps[i]) continue;
+        DBlockHdr* h = (DBlockHdr*)((char*)ps[i] - DH_HDR);
+        const unsigned long long meta = h->meta;
+        const int c = (int)(meta >> 56);
+        const int hid = (int)((meta >> 48) & 0xFF);
+        if (h->magic != DH_MAGIC_LIVE || c >= DH_NCLASS || hid > 1) {
+            err_or(err, KERR_HEAP_CORRUPT);
+            continue;
+        }
+        DHeap* hp = hid ? &pool->scratch : &pool->heap;
+        h->magic = DH_MAGIC_FREE;
+        old[i] = dh_head(hp, c, s);
+        __hip_atomic_store(&h->next, (uint32_t)old[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hd[i] = h;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (!hd[i]) continue;
+        const unsigned long long meta = hd[i]->meta;
+        const int c = (int)(meta >> 56);
+        DHeap* hp = ((meta >> 48) & 0xFF) ? &pool->scratch : &pool->heap;
+        const uint32_t off = (uint32_t)((unsigned long long)((char*)hd[i] - pool->base) >> 8);
+        const unsigned long long neu = (((old[i] >> 32) + 1ull) << 32) | off;
+        unsigned long long exp = old[i];
+        if (!__hip_atomic_compare_exchange_strong(&hp->heads[c][s].v, &exp, neu, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))
+            dh_push_chain(hp, c, s, hd[i], off);
+        fetch_add_relaxed(&hp->cnt[s].outstanding, (unsigned long long)0 - (meta & 0xFFFFFFFFFFFFull));
+        fetch_add_relaxed(&hp->cnt[s].frees, 1ull);
+    }
 }
 
 // Free up to blockDim.x blocks, one per thread (p == nullptr: nothing), with

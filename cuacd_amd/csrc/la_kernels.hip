@@ -992,8 +992,8 @@ __global__ __launch_bounds__(PC_T) void la_score_kernel(LaScoreTask* tasks, int 
         T.npts = npp;
         T.np_pts = pp ? nposo + n_new : 0;
         T.nn_pts = npp ? nnego + n_new : 0;
-        heap_free(pool, A, err);
-        if (F) heap_free(pool, F, err);
+        void* const f[2] = {A, F};
+        heap_free_n(pool, f, err);
     }
 }
 

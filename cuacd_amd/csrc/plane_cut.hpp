@@ -1080,8 +1080,8 @@ __device__ inline PcResult plane_cut_block(const float* __restrict__ V, const in
         else R.neg = m;
     }
     if (tid == 0) {
-        if (Cbuf) heap_free(pool, Cbuf, err);
-        heap_free(pool, B, err);
+        void* const f[2] = {Cbuf, B};
+        heap_free_n(pool, f, err);
     }
     __syncthreads();
     if (!R.pos.v || !R.neg.v) R.status = 2;
@@ -1691,9 +1691,8 @@ __device__ inline PcResult plane_cut_block_lds(const float* __restrict__ V, cons
     if (chp) R.pos = PcMesh{chp, otp, onv_p, ont_p, pc_volume(chp, otp, ont_p, s)};
     if (chn) R.neg = PcMesh{chn, otn, onv_n, ont_n, pc_volume(chn, otn, ont_n, s)};
     if (tid == 0) {
-        if (Cbuf) heap_free(pool, Cbuf, err);
-        heap_free(pool, B, err);
-        heap_free(pool, sl, err);
+        void* const f[3] = {Cbuf, B, sl};
+        heap_free_n(pool, f, err);
     }
     __syncthreads();
     if (!R.pos.v || !R.neg.v) R.status = 2;
