@@ -238,7 +238,7 @@ __device__ __forceinline__ float sel3(const float* v, int i) { return i == 0 ? v
 constexpr int HULL_NP_MAX = 16384;  // largest point set one workgroup hulls
 
 // Small block-wide scratch that always lives in LDS (reductions, broadcasts,
-// counters and the output-order bitmap).
+// counters). The output-order bitmap is HullBits, sized per kernel.
 struct HullBlockLDS {
     unsigned long long red[16];
     double dred[16];
@@ -246,13 +246,22 @@ struct HullBlockLDS {
     int ctr[8];
     float bmn[16][3];  // per-wave AABB partials
     float bmx[16][3];
-    uint32_t bm[HULL_NP_MAX / 32];
-    int wpre[HULL_NP_MAX / 32];
 };
 
-template <int AS> __device__ __forceinline__ HullWS<AS> hull_ws_attach(HullWS<AS> w, HullBlockLDS* sb) {
-    w.bm = (asptr<AS_LDS, uint32_t>)sb->bm;
-    w.wpre = (asptr<AS_LDS, int>)sb->wpre;
+// Output-order bitmap (one bit per input point) and its per-word prefix
+// counts, in LDS, for point sets of up to NPMAX points: a one-wave tier of
+// 256 points needs 64 bytes where the 16384-point bound takes 4 KB, and LDS
+// is what limits the small tiers' workgroups per CU.
+template <int NPMAX> struct HullBits {
+    static_assert(NPMAX > 0 && NPMAX <= HULL_NP_MAX, "HullBits capacity");
+    uint32_t bm[(NPMAX + 31) / 32];
+    int wpre[(NPMAX + 31) / 32];
+};
+
+template <int AS, int NPMAX>
+__device__ __forceinline__ HullWS<AS> hull_ws_attach(HullWS<AS> w, HullBlockLDS* sb, HullBits<NPMAX>* hb) {
+    w.bm = (asptr<AS_LDS, uint32_t>)hb->bm;
+    w.wpre = (asptr<AS_LDS, int>)hb->wpre;
     w.ctr = (asptr<AS_LDS, int>)sb->ctr;
     return w;
 }

@@ -213,20 +213,22 @@ template <int PPL>
 __global__ __launch_bounds__(64) void hull_wave_kernel(HullItemArgs a, int mode) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<64 * PPL> hb;
     const int item = blockIdx.x;
     if (item >= a.n_items) return;
     // compile-time capacities: every sub-array address folds into ds offsets
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb, &hb);
     hull_item<PPL, AS_LDS, 1>(a, mode, item, w, &sb);
 }
 
 template <int PPL>
 __global__ __launch_bounds__(64 * BLOCK_NW) void hull_block_kernel(HullItemArgs a, int mode) {
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<HULL_NP_MAX> hb;
     const int item = blockIdx.x;
     if (item >= a.n_items) return;
-    const HullWS<AS_GLOBAL> w =
-        hull_ws_attach(hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)item * a.gws_stride, a.VC, a.NP), &sb);
+    const HullWS<AS_GLOBAL> w = hull_ws_attach(
+        hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)item * a.gws_stride, a.VC, a.NP), &sb, &hb);
     hull_item<PPL, AS_GLOBAL, BLOCK_NW>(a, mode, item, w, &sb);
 }
 
@@ -310,10 +312,11 @@ int launch_block(gpu_ctx_t ctx, HullItemArgs a, int maxn, int mode) {
 template <int PPL>
 __global__ __launch_bounds__(64 * BLOCK_NW) void hull_block_pass_kernel(HullItemArgs a, int mode, int item0) {
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<HULL_NP_MAX> hb;
     const int item = item0 + (int)blockIdx.x;
     if (item >= a.n_items) return;
     const HullWS<AS_GLOBAL> w = hull_ws_attach(
-        hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)blockIdx.x * a.gws_stride, a.VC, a.NP), &sb);
+        hull_ws_carve<AS_GLOBAL>(a.gws + (unsigned long long)blockIdx.x * a.gws_stride, a.VC, a.NP), &sb, &hb);
     hull_item<PPL, AS_GLOBAL, BLOCK_NW>(a, mode, item, w, &sb);
 }
 

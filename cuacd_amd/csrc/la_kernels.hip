@@ -172,10 +172,11 @@ __global__ __launch_bounds__(64) void la_hull_wave(LaHullTask* tasks, const int*
                                                    DPool* pool, int* err) {
     extern __shared__ __align__(16) char lds[];
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<64 * PPL> hb;
     // ids[0, n), or the first min(*dcount, n) when the count is on the device
     if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
     LaHullTask& t = tasks[ids[blockIdx.x]];
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, 64 * PPL, 64 * PPL), &sb, &hb);
     la_hull_item<PPL, AS_LDS, 1>(t, w, &sb, pool, err);
     if (threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
@@ -192,7 +193,8 @@ __global__ __launch_bounds__(64 * LA_LDS4_NW) void la_hull_lds4(LaHullTask* task
     // (a retry list filled by the previous launch); the grid strides over them
     const int nn = dcount ? min(*dcount, n) : n;
     constexpr int CAP = 64 * LA_LDS4_NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb);
+    __shared__ HullBits<CAP> hb;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb, &hb);
     for (int j = blockIdx.x; j < nn; j += gridDim.x) {
         __syncthreads();
         LaHullTask& t = tasks[ids[j]];
@@ -211,7 +213,8 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_lds8(LaHullTask* tas
     __shared__ HullBlockLDS sb;
     const int nn = dcount ? min(*dcount, n) : n;
     constexpr int CAP = 64 * LA_BLOCK_NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb);
+    __shared__ HullBits<CAP> hb;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, CAP, CAP, multi != 0), &sb, &hb);
     for (int j = blockIdx.x; j < nn; j += gridDim.x) {
         __syncthreads();
         LaHullTask& t = tasks[ids[j]];
@@ -233,7 +236,8 @@ __global__ __launch_bounds__(64 * NW) void la_hull_ldsv(LaHullTask* tasks, const
     const int id = ids[blockIdx.x];
     LaHullTask& t = tasks[id];
     constexpr int CAP = 64 * NW * PPL;
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, vcap, CAP, multi != 0), &sb);
+    __shared__ HullBits<CAP> hb;
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, vcap, CAP, multi != 0), &sb, &hb);
     const bool done = la_hull_item<PPL, AS_LDS, NW>(t, w, &sb, pool, err, retry, id, multi != 0);
     if (done && threadIdx.x == 0 && t.filt) { heap_free(pool, t.filt, err); t.filt = nullptr; }
 }
@@ -249,10 +253,11 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
                                                                   unsigned long long stride, int VC, int NP,
                                                                   int multi, DPool* pool, int* err) {
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<HULL_NP_MAX> hb;
     extern __shared__ __align__(16) char lds_ma[];
     const int nn = dcount ? min(*dcount, n) : n;
     HullWS<AS_GLOBAL> w =
-        hull_ws_attach(hull_ws_carve<AS_GLOBAL>(gws + (unsigned long long)blockIdx.x * stride, VC, NP), &sb);
+        hull_ws_attach(hull_ws_carve<AS_GLOBAL>(gws + (unsigned long long)blockIdx.x * stride, VC, NP), &sb, &hb);
     if (multi) w = hull_ma_attach(w, lds_ma);
     for (int j = blockIdx.x; j < nn; j += gridDim.x) {
         __syncthreads();
@@ -358,11 +363,16 @@ __global__ __launch_bounds__(KD_T) void la_kdop_ext_kernel(LaHullTask* tasks, co
 // a grid of their own box), its FP64 volume for the bound pruning, and the
 // live rough faces as the filter's planes. A set without a usable rough hull
 // is hulled whole; a pruned one keeps the rough volume as its bound.
-constexpr int KD_VC = 128;      // hull vertex capacity (the 80 extreme points)
-constexpr unsigned KD_WS = 16384;  // LDS of its workspace (la_launch_kdop checks the size)
+// Vertex capacity: the 80 extreme points (a hull of them has at most 80
+// corners, and the face arrays hold 2 * 80 + 8, the live faces plus one
+// insertion's transient pair). A smaller workspace fits more of these
+// one-wave workgroups on a CU (14 at 11.5 KB of LDS each).
+constexpr int KD_VC = KD_NE;
+constexpr unsigned KD_WS = 9776;  // LDS of its workspace (la_launch_kdop checks the size)
 __global__ __launch_bounds__(64) void la_kdop_rough_kernel(LaHullTask* tasks, const int* ids, int n,
                                                            const int* dcount, KdopRec* rec, int2* res) {
     __shared__ HullBlockLDS sb;
+    __shared__ HullBits<KD_NE> hb;
     __shared__ __align__(16) char lds[KD_WS];
     __shared__ float ep[KD_NE][3];
     __shared__ int s_npl;
@@ -373,7 +383,7 @@ __global__ __launch_bounds__(64) void la_kdop_rough_kernel(LaHullTask* tasks, co
     const int np = t.n;
     for (int i = tid; i < 3 * KD_NE; i += 64) (&ep[0][0])[i] = (&R.ep[0][0])[i];
     __syncthreads();
-    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, KD_VC, KD_VC), &sb);
+    const HullWS<AS_LDS> w = hull_ws_attach(hull_ws_carve<AS_LDS>(lds, KD_VC, KD_VC), &sb, &hb);
     float emn[3] = {1e30f, 1e30f, 1e30f}, emx[3] = {-1e30f, -1e30f, -1e30f};
     for (int i = 0; i < KD_NE; ++i)
         for (int j = 0; j < 3; ++j) { emn[j] = fminf(emn[j], ep[i][j]); emx[j] = fmaxf(emx[j], ep[i][j]); }
