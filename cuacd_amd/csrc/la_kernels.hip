@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64 * LA_BLOCK_NW) void la_hull_block(LaHullTask* ta
 // set's grid), in input order. The filtered set keeps the full set's
 // quantization frame and the input order, so its canonical hull equals the
 // full set's. Three launches over the same list, one record per set between
-// them: the extremes (four waves per set, passes over the points), the rough
+// them: the extremes (one wave per set, passes over the points), the rough
 // hull of the 80 extreme points (one wave per set: its insertion steps need
 // no block barrier, and many sets share a CU), the filter (four waves).
 constexpr int KD_T = 256;
@@ -289,41 +289,33 @@ struct KdopRec {
     unsigned char face[KD_MAXPL][3];  // filter planes as extreme-point triples
 };
 
-// Launch 1: bbox -> frame, then the extreme points along the 40 axes, 8 axes
-// per pass: max key (value, lowest index), min key (value, lowest index).
-__global__ __launch_bounds__(KD_T) void la_kdop_ext_kernel(LaHullTask* tasks, const int* ids, int n,
-                                                           const int* dcount, KdopRec* rec) {
-    __shared__ float bmn[KD_NW][3], bmx[KD_NW][3];
-    __shared__ unsigned long long kred[KD_NW][16];
+// Launch 1, one wave per set: the extreme points along the 40 axes, 8 axes
+// per pass over the points - max key (value, lowest index), min key (value,
+// lowest index) - and the bounding box (the full set's frame) in the first
+// pass. One wave per set: with four, the 16 key reductions of a pass cost as
+// much as the pass itself for the typical set of about 1000 points, and the
+// passes needed block barriers. min / max and the keys are independent of
+// the visiting order, so the record is the same for any lane count.
+__global__ __launch_bounds__(64) void la_kdop_ext_kernel(LaHullTask* tasks, const int* ids, int n,
+                                                         const int* dcount, KdopRec* rec) {
     if ((int)blockIdx.x >= (dcount ? min(*dcount, n) : n)) return;
     const LaHullTask& t = tasks[ids[blockIdx.x]];
     KdopRec& R = rec[blockIdx.x];
-    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const int lane = threadIdx.x;
     const float* P = t.pts;
     const int np = t.n;
     float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
-    for (int i = tid; i < np; i += KD_T)
-        for (int j = 0; j < 3; ++j) {
-            const float x = P[3ll * i + j];
-            mn[j] = fminf(mn[j], x);
-            mx[j] = fmaxf(mx[j], x);
-        }
-    for (int j = 0; j < 3; ++j) {
-        mn[j] = wave_min_f(mn[j]);
-        mx[j] = wave_max_f(mx[j]);
-    }
-    if (lane == 0)
-        for (int j = 0; j < 3; ++j) { bmn[wv][j] = mn[j]; bmx[wv][j] = mx[j]; }
-    __syncthreads();
-    for (int j = 0; j < 3; ++j)
-        for (int q = 0; q < KD_NW; ++q) { mn[j] = fminf(mn[j], bmn[q][j]); mx[j] = fmaxf(mx[j], bmx[q][j]); }
-    if (tid == 0) R.full = qframe_make(mn, mx);
     for (int a0 = 0; a0 < KDOP_AXES_N; a0 += 8) {
         unsigned long long kmax[8], kmin[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) { kmax[q] = 0ull; kmin[q] = ~0ull; }
-        for (int i = tid; i < np; i += KD_T) {
+        for (int i = lane; i < np; i += 64) {
             const float x = P[3ll * i], y = P[3ll * i + 1], z = P[3ll * i + 2];
+            if (a0 == 0) {
+                mn[0] = fminf(mn[0], x); mx[0] = fmaxf(mx[0], x);
+                mn[1] = fminf(mn[1], y); mx[1] = fmaxf(mx[1], y);
+                mn[2] = fminf(mn[2], z); mx[2] = fmaxf(mx[2], z);
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const float d = c_kdop_axes[a0 + q][0] * x + c_kdop_axes[a0 + q][1] * y + c_kdop_axes[a0 + q][2] * z;
@@ -334,29 +326,29 @@ __global__ __launch_bounds__(KD_T) void la_kdop_ext_kernel(LaHullTask* tasks, co
                 kmin[q] = kn < kmin[q] ? kn : kmin[q];
             }
         }
+        // lane q < 8 takes the max key of axis a0 + q, lane 8 + q its min key
+        unsigned long long mine = 0ull;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            kmax[q] = wave_max_u64(kmax[q]);
-            kmin[q] = wave_min_u64(kmin[q]);
+            const unsigned long long hi = wave_max_u64(kmax[q]);
+            const unsigned long long lo = wave_min_u64(kmin[q]);
+            if (lane == q) mine = hi;
+            if (lane == 8 + q) mine = lo;
         }
-        if (lane == 0)
-            for (int q = 0; q < 8; ++q) { kred[wv][q] = kmax[q]; kred[wv][8 + q] = kmin[q]; }
-        __syncthreads();
-        if (tid < 16) {
-            unsigned long long k = kred[0][tid];
-            for (int w2 = 1; w2 < KD_NW; ++w2) {
-                const unsigned long long o = kred[w2][tid];
-                k = tid < 8 ? (o > k ? o : k) : (o < k ? o : k);
-            }
-            const int q = tid & 7, isMin = tid >> 3;
-            const int idx = isMin ? (int)(uint32_t)k : (int)(0xFFFFFFFFu - (uint32_t)k);
+        if (lane < 16) {
+            const int q = lane & 7, isMin = lane >> 3;
+            const int idx = isMin ? (int)(uint32_t)mine : (int)(0xFFFFFFFFu - (uint32_t)mine);
             const int slot = 2 * (a0 + q) + isMin;
             R.ep[slot][0] = P[3ll * idx];
             R.ep[slot][1] = P[3ll * idx + 1];
             R.ep[slot][2] = P[3ll * idx + 2];
         }
-        __syncthreads();
     }
+    for (int j = 0; j < 3; ++j) {
+        mn[j] = wave_min_f(mn[j]);
+        mx[j] = wave_max_f(mx[j]);
+    }
+    if (lane == 0) R.full = qframe_make(mn, mx);
 }
 
 // Launch 2, one wave per set: the rough hull of the extreme points (exact, on
@@ -558,7 +550,7 @@ int la_launch_kdop(gpu_ctx_t ctx, LaHullTask* d_tasks, const int* d_ids, int n, 
                    KdopRec* d_rec, hipStream_t s) {
     if (n <= 0) return 0;
     REQUIRE(hull_ws_bytes<AS_LDS>(KD_VC, KD_VC) <= KD_WS, "k-DOP rough hull workspace exceeds its LDS");
-    hipLaunchKernelGGL(la_kdop_ext_kernel, dim3((unsigned)n), dim3(KD_T), 0, s, d_tasks, d_ids, n, dcount, d_rec);
+    hipLaunchKernelGGL(la_kdop_ext_kernel, dim3((unsigned)n), dim3(64), 0, s, d_tasks, d_ids, n, dcount, d_rec);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(la_kdop_rough_kernel, dim3((unsigned)n), dim3(64), 0, s, d_tasks, d_ids, n, dcount, d_rec,
                        d_res);
