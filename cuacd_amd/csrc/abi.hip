@@ -144,6 +144,7 @@ void gpu_destroy(gpu_ctx_t ctx) {
     for (int i = 0; i < 2; ++i)
         if (ctx->ev_tier[i]) (void)hipEventDestroy(ctx->ev_tier[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->la_pinned) (void)hipHostFree(ctx->la_pinned);
     free(ctx);
 }
 

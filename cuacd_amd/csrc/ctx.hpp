@@ -62,6 +62,10 @@ struct gpu_ctx {
     long long chunk_subtasks;  // chunk hulls of oversized sets, counted since gpu_init
     hipEvent_t ev_tier[2];     // first passes of the eight-wave LDS hull tiers (launch order)
     long long host_syncs;      // host waits on the context's streams, counted since gpu_init
+    // grow-only pinned staging of the search's batched readbacks; kept across
+    // calls (pinning tens of MB costs milliseconds), freed with the context
+    char* la_pinned;
+    size_t la_pinned_cap;
 };
 
 // A host wait for a stream of the context, counted in host_syncs.

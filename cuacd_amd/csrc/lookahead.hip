@@ -215,13 +215,11 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int hull_phase = PH_IHULL;
     long long proposal_kept = 0, proposal_total = 0, proposal_fallbacks = 0;
-    char* pinned = nullptr;  // grow-only pinned staging for batched readbacks
-    size_t pinned_cap = 0;
-
     // Device -> host for many small device ranges at once: one segmented
     // device copy into a packed staging buffer, one transfer into pinned
-    // memory. Returns the packed host copy (range k at offs[k], 16-byte
-    // aligned), valid until the next call.
+    // memory (the context's grow-only pinned staging, kept across calls).
+    // Returns the packed host copy (range k at offs[k], 16-byte aligned),
+    // valid until the next call.
     const char* gather_to_host(const std::vector<std::pair<const void*, size_t>>& src, std::vector<size_t>& offs,
                                int* rc) {
         *rc = 0;
@@ -231,18 +229,20 @@ struct Engine {
             offs[k] = total;
             total += (src[k].second + 15) / 16 * 16;
         }
+        char*& pinned = ctx->la_pinned;
         if (total == 0) return pinned;
-        if (total > pinned_cap) {
+        if (total > ctx->la_pinned_cap) {
             if (pinned) (void)hipHostFree(pinned);
             pinned = nullptr;
-            pinned_cap = 0;
+            ctx->la_pinned_cap = 0;
             const size_t cap = std::max(total + total / 2, (size_t)(1 << 20));
             if (hipHostMalloc((void**)&pinned, cap, hipHostMallocDefault) != hipSuccess) {
+                pinned = nullptr;
                 set_error(ctx, "lookahead: pinned staging allocation failed");
                 *rc = -1;
                 return nullptr;
             }
-            pinned_cap = cap;
+            ctx->la_pinned_cap = cap;
         }
         DevBuf<char> stage;
         DevBuf<LaSeg> dsegs;
@@ -305,7 +305,6 @@ struct Engine {
         print_hull_stats();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (pinned) (void)hipHostFree(pinned);
     }
 
     int check_err(const char* where) {
